@@ -1,0 +1,273 @@
+/*
+ * mtmd.h - C ABI of the MI355X molecular-dynamics engine (libmtmd.so).
+ *
+ * This is the drop-in boundary for MolTwister's `moldyn` hot path. Every entry
+ * point names the reference interface it replaces (paths are relative to the
+ * MolTwister source tree). Plain C types only: pointers, sizes, PODs.
+ *
+ * Units are the reference's internal ones (Cmd/MolTwisterCmdMolDyn/Integrators/
+ * Constants.h:23-40): length in Angstrom, mass in g/mol, energy in kJ/mol,
+ * charge in e, time in 100 fs, momentum in (g/mol)*Angstrom/(100 fs).
+ * Configuration fields keep the reference's user units (fs, K, atm).
+ *
+ * Ownership: the caller owns every host buffer; mtmd_create deep-copies the
+ * system, and getters write only into caller-provided buffers. A handle is
+ * single-threaded (one HIP stream); different handles may be driven from
+ * different host threads.
+ *
+ * Error convention: every int-returning call returns MTMD_OK (0) or a negative
+ * MTMD_E* code; mtmd_last_error(h) gives the message (mtmd_last_error(NULL)
+ * gives the message of the last failed mtmd_create on this thread).
+ */
+#ifndef MTMD_H
+#define MTMD_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MTMD_ABI_VERSION 1
+
+/* Number of points in every force/energy profile and the maximum number of
+ * non-bonded force fields per atom-type pair
+ * (Cmd/MolTwisterCmdMolDyn/ForceFields/MDFFMatrices.h:26-27). */
+#define MTMD_NUM_POINTS 1000
+#define MTMD_MAX_FF_PER_PAIR 5
+
+/* status codes */
+enum {
+    MTMD_OK = 0,
+    MTMD_EINVAL = -1,        /* invalid argument or unsupported option       */
+    MTMD_EHIP = -2,          /* HIP runtime error                            */
+    MTMD_ENEIGH = -3,        /* neighbour list overflow after regrowth       */
+    MTMD_ENAN = -4,          /* non-finite value detected                    */
+    MTMD_EBOX = -5,          /* box too small for the cell/neighbour scheme  */
+    MTMD_ENOMEM = -6,        /* device or host allocation failed             */
+    MTMD_ENODEV = -7         /* no HIP device                                */
+};
+
+/* ensembles: same values as SMolDynConfigStruct::Ensemble
+ * (Cmd/MolTwisterCmdMolDyn/Config/MolDynConfigStruct.h:26) */
+enum { MTMD_NVT = 0, MTMD_NPT = 1, MTMD_NVE = 2 };
+
+/* engines (no counterpart in the reference; the reference has one algorithm):
+ *  EXACT - the reference algorithm: cell and full neighbour lists rebuilt at rc
+ *          before every force call, tabulated FP32 profiles with FP64 geometry
+ *          and accumulation, the reference summation order per atom.
+ *  FP64  - same arithmetic as EXACT per pair, but neighbour lists at rc + skin
+ *          rebuilt on displacement, atoms spatially sorted at each rebuild.
+ *  MIXED - as FP64 but pair geometry and table interpolation in FP32,
+ *          per-atom accumulation, integration and reductions in FP64. */
+enum { MTMD_ENGINE_EXACT = 0, MTMD_ENGINE_FP64 = 1, MTMD_ENGINE_MIXED = 2 };
+
+/* Coulomb treatment.
+ *  ALLPAIRS - minimum-image sum over all atoms in index order, no cutoff
+ *             (Cmd/MolTwisterCmdMolDyn/ForceFields/FunctorCalcForce.cpp:258-278).
+ *  CUTOFF   - the same pair term restricted to pairs inside the non-bonded cutoff.
+ *  AUTO     - ALLPAIRS for the EXACT engine, CUTOFF otherwise. Systems whose
+ *             charges are all zero skip the Coulomb pass (every term is +-0). */
+enum { MTMD_COUL_AUTO = 0, MTMD_COUL_ALLPAIRS = 1, MTMD_COUL_CUTOFF = 2 };
+
+/* force-field kinds */
+enum { MTMD_NB_LJ = 1, MTMD_NB_BUCK = 2, MTMD_NB_LJ1208 = 3 };
+enum { MTMD_BOND_HARM = 1, MTMD_BOND_MORSE = 2 };
+enum { MTMD_ANGLE_HARM = 1 };
+enum { MTMD_DIH_HARM = 1 };
+
+/* Run configuration: the fields of SMolDynConfigStruct
+ * (Cmd/MolTwisterCmdMolDyn/Config/MolDynConfigStruct.h:24-65) that the engine
+ * reads, with the defaults of CMolDynConfig::resetToDefaults
+ * (Cmd/MolTwisterCmdMolDyn/Config/MolDynConfig.cpp:289-328), plus engine
+ * options at the end. */
+typedef struct mtmd_config {
+    int32_t ensemble;               /* MTMD_NVT | MTMD_NPT | MTMD_NVE          */
+    double  temperature;            /* K                                       */
+    double  temperature_relax;      /* fs, Nose-Hoover tau                     */
+    int32_t temperature_nh_len;     /* chain length M                          */
+    int32_t temperature_respa;      /* RESPA steps n                           */
+    double  pressure;               /* atm                                     */
+    double  pressure_relax;         /* fs                                      */
+    int32_t pressure_nh_len;
+    int32_t pressure_respa;
+    double  cutoff_radius;          /* Angstrom                                */
+    double  neigh_shell;            /* Angstrom, sizes the cells               */
+    double  cutoff_force;           /* per-component force cap, <= 0 disables  */
+    int32_t n_steps;
+    double  timestep;               /* fs                                      */
+    int32_t out_stride;
+    double  scale12, scale13, scale14, scale1N;
+    double  opt_learning_rate;
+    int32_t opt_max_steps;
+    double  opt_accuracy;
+    int32_t verbose;
+    char    info_file[256];         /* thermo table written by mtmd_run        */
+    /* engine options (no counterpart in the reference) */
+    int32_t engine;                 /* MTMD_ENGINE_*                           */
+    int32_t coulomb;                /* MTMD_COUL_*                             */
+    double  skin;                   /* Angstrom, list radius = cutoff + skin   */
+    int32_t block_size;             /* force-kernel workgroup size, 0 = auto   */
+    int32_t reserved[7];
+} mtmd_config;
+
+/* One non-bonded force field between two atom types. Parameters:
+ *  LJ:     p[0]=epsilon, p[1]=sigma, p[2]=alpha (default 1)
+ *          (MDFF/Non-Bonded/MolTwisterMDFFNonBonded_LJ.cpp:131-162)
+ *  BUCK:   p[0]=A, p[1]=rho, p[2]=C (MolTwisterMDFFNonBonded_Buck.cpp:116-143)
+ *  LJ1208: p[0]=epsilon, p[1]=sigma, p[2]=alpha (default 1)
+ *          (MolTwisterMDFFNonBonded_LJ1208.cpp:131-170) */
+typedef struct mtmd_nonbonded_ff {
+    int32_t type_a, type_b, kind, pad;
+    double  p[4];
+} mtmd_nonbonded_ff;
+
+/* One bonded force-field type. Parameters:
+ *  BOND_HARM:  p[0]=k, p[1]=r0          U = k (r - r0)^2
+ *  BOND_MORSE: p[0]=D, p[1]=alpha, p[2]=r0
+ *  ANGLE_HARM: p[0]=k, p[1]=theta0[deg] U = k (theta - theta0)^2
+ *  DIH_HARM:   p[0]=K, p[1]=D, p[2]=N   U = K (1 + D cos(N phi)) */
+typedef struct mtmd_bonded_ff {
+    int32_t kind, pad;
+    double  p[4];
+} mtmd_bonded_ff;
+
+/* The system: the inputs CMDFFMatrices::prepareFFMatrices and CSimulationBox
+ * read from CMolTwisterState (MDFFMatrices.cpp:181-536,
+ * SimulationBox.cpp:27-133). Atom types are indices in first-appearance order
+ * of the atom names (MolTwisterState.cpp:459-484); the box is the periodic cell
+ * centred on the origin with widths box[0..2] (C3DView::calcPBC with a user
+ * PBC). Topology lists are the global MD bond/angle/dihedral lists in the order
+ * CMolTwisterStateTools::getAllMD*InSystem returns them; molecule membership is
+ * derived from the bonds (MolecularSystemTools.cpp:238-280). */
+typedef struct mtmd_system {
+    int32_t n_atoms;
+    int32_t n_types;
+    const double*  pos;        /* [3*n_atoms] x,y,z                               */
+    const double*  mom;        /* [3*n_atoms] or NULL: random momenta scaled to T */
+    const double*  mass;       /* [n_atoms]                                       */
+    const double*  charge;     /* [n_atoms]                                       */
+    const int32_t* type;       /* [n_atoms]                                       */
+    const uint8_t* mobile;     /* [n_atoms] or NULL (all mobile)                  */
+    double box[3];
+    int32_t n_nonbonded;   const mtmd_nonbonded_ff* nonbonded;
+    int32_t n_bond_ff;     const mtmd_bonded_ff*    bond_ff;
+    int32_t n_angle_ff;    const mtmd_bonded_ff*    angle_ff;
+    int32_t n_dihedral_ff; const mtmd_bonded_ff*    dihedral_ff;
+    int32_t n_bonds;       const int32_t* bonds;      /* [3*n]: a, b, ff          */
+    int32_t n_angles;      const int32_t* angles;     /* [4*n]: a, centre, c, ff  */
+    int32_t n_dihedrals;   const int32_t* dihedrals;  /* [5*n]: a, b, c, d, ff    */
+    uint64_t seed;             /* for random momenta when mom == NULL            */
+} mtmd_system;
+
+/* Thermodynamic output (SimulationBox.cpp:168-231, MDLoop.cpp:391-404).
+ * T and P use the reference definitions: T from dim*N degrees of freedom,
+ * P from the per-atom sum of Fpi.r with wrapped coordinates. */
+typedef struct mtmd_thermo {
+    double T;        /* K                   */
+    double P;        /* atm                 */
+    double V;        /* Angstrom^3          */
+    double U;        /* kJ/mol              */
+    double K;        /* kJ/mol              */
+    double Etot;     /* kJ/mol              */
+    double sum_p2m;  /* sum p.p/m (reduced) */
+    double sum_fr;   /* sum F.r   (reduced) */
+} mtmd_thermo;
+
+typedef struct mtmd_handle mtmd_handle;
+
+/* ---- library ---- */
+int         mtmd_abi_version(void);
+const char* mtmd_last_error(const mtmd_handle* h);
+const char* mtmd_status_string(int status);
+
+/* CMolDynConfig::resetToDefaults (MolDynConfig.cpp:289-328) */
+void mtmd_config_defaults(mtmd_config* cfg);
+/* CMolDynConfig::set (MolDynConfig.cpp:95-287): keys and values as the
+ * `moldyn cfg set <key> <value>` command; returns MTMD_OK or MTMD_EINVAL with
+ * the reference's "Error: ..." text in err (if err != NULL). */
+int mtmd_config_set(mtmd_config* cfg, const char* key, const char* value,
+                    char* err, size_t err_len);
+
+/* Profiles: CMDFF*::calc1DForceProfile / calc1DPotentialProfile
+ * (MolTwisterMDFFNonBonded.cpp:126-172, MolTwisterMDFFBond.cpp:133-179,
+ * MolTwisterMDFFAngle.cpp:210-257, MolTwisterMDFFDih.cpp:272-320).
+ * which: 0 non-bonded, 1 bond, 2 angle, 3 dihedral. Writes n points of
+ * x, -dU/dx and U as float. This is also what `moldyn ff *profile` prints. */
+int mtmd_profile(int which, int kind, const double* params, float x0, float x1,
+                 int n, float* x, float* f, float* e);
+
+/* ---- engine ----
+ * Replaces CSimulationBox::CSimulationBox + CVelVerlet::CVelVerlet +
+ * CMDFFMatrices::CMDFFMatrices (SimulationBox.cpp:27-79, VelVerlet.cpp:31-48,
+ * MDFFMatrices.cpp:117-133). */
+int  mtmd_create(const mtmd_system* sys, const mtmd_config* cfg, mtmd_handle** out);
+void mtmd_destroy(mtmd_handle* h);
+
+/* particles_[k].r_/p_ (SimulationBox.h:77); layouts [3*n] */
+int mtmd_set_state(mtmd_handle* h, const double* pos, const double* mom);
+int mtmd_get_state(mtmd_handle* h, double* pos, double* mom);
+/* NH_T_.eta_/p_eta_ (NHChain.h:59-61); arrays of temperature_nh_len */
+int mtmd_set_thermostat(mtmd_handle* h, const double* eta, const double* p_eta);
+int mtmd_get_thermostat(mtmd_handle* h, double* eta, double* p_eta);
+
+/* CMDFFMatrices::genNeighList, cell half (MDFFMatrices.cpp:151-159,
+ * FunctorGenCellList.cpp:49-169) on the current positions. Any output may be
+ * NULL. ixyz [3*n]: cell of each atom; cell_first/cell_count [ncells]
+ * (first = -1 for empty cells); cell_list [n]. ncells_xyz[3] receives the grid. */
+int mtmd_build_cells(mtmd_handle* h, int32_t* ncells_xyz, int32_t* ixyz,
+                     int32_t* cell_first, int32_t* cell_count, int32_t* cell_list);
+
+/* CMDFFMatrices::genNeighList, neighbour half (MDFFMatrices.cpp:161-179,
+ * FunctorGenNeighList.cpp:72-191): full lists at the cutoff radius in the
+ * reference order. count [n]; list [n*max_per_atom] row-major (may be NULL);
+ * *required receives the largest count. Returns MTMD_ENEIGH when a row does
+ * not fit max_per_atom (count and required are still filled). */
+int mtmd_build_neighbors(mtmd_handle* h, int32_t max_per_atom, int32_t* count,
+                         int32_t* list, int32_t* required);
+
+/* CVelVerlet::calcParticleForces (VelVerlet.cpp:152-158,213-222): forces on
+ * the current positions, uncapped. F and Fpi [3*n], U [n]; any may be NULL. */
+int mtmd_compute_forces(mtmd_handle* h, double* F, double* Fpi, float* U);
+
+/* CMDLoop::runSimulation inner loop (MDLoop.cpp:124-138): nsteps of
+ * NH_P, NH_T, velocity Verlet (with force cap), NH_T, NH_P, PBC wrap.
+ * The first call computes the initial forces (MDLoop.cpp:121). */
+int mtmd_step(mtmd_handle* h, int32_t nsteps);
+
+/* calcTemp/calcPress/calcV/calcUtot/calcCurrentKineticEnergy
+ * (SimulationBox.cpp:168-231) on the current state and last forces. */
+int mtmd_get_thermo(mtmd_handle* h, mtmd_thermo* out);
+
+/* Per-atom forces and energies of the last force evaluation (capped, as the
+ * integrator uses them). */
+int mtmd_get_forces(mtmd_handle* h, double* F, float* U);
+
+/* CMDLoop::runOptimization (MDLoop.cpp:143-173) with
+ * CSimulationBox::gradientDescentStep (SimulationBox.cpp:197-208): at most
+ * max_steps gradient-descent steps r += lr*F; stops when |dU| < accuracy and
+ * the step index exceeds out_stride. *steps_done receives the count. */
+int mtmd_optimize(mtmd_handle* h, int32_t max_steps, double learning_rate,
+                  double accuracy, int32_t* steps_done);
+
+/* Diagnostics: neighbour-list rebuilds since create, current list capacity. */
+int mtmd_get_stats(mtmd_handle* h, int64_t* rebuilds, int64_t* steps,
+                   int32_t* max_neighbors);
+
+/* Blocks until all queued device work of the handle is done. */
+int mtmd_synchronize(mtmd_handle* h);
+
+/* CMDSimulator::run (Simulator/MDSimulator.cpp:35-47): creates the engine,
+ * computes initial forces and runs cfg->n_steps, writing the reference's
+ * thermo table every out_stride steps to cfg->info_file (and to `log` when
+ * not NULL). */
+int mtmd_run(const mtmd_config* cfg, const mtmd_system* sys, void* log_file);
+
+/* CMDSimulator::optimize (Simulator/MDSimulator.cpp:49-66). */
+int mtmd_run_optimize(const mtmd_config* cfg, const mtmd_system* sys, void* log_file);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MTMD_H */

@@ -1,0 +1,1266 @@
+/*
+ * mtmd_oracle.c - CPU restatement of MolTwister's `moldyn` hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY (see mtmd_oracle.h): the checker for libmtmd.so,
+ * never part of the product. Compiled with -O2 -ffp-contract=off on x86-64
+ * (SSE2 arithmetic, no FMA), like the reference CPU build.
+ *
+ * Reference paths are relative to the MolTwister source tree. Abbreviations:
+ *   FCF = Cmd/MolTwisterCmdMolDyn/ForceFields/FunctorCalcForce.cpp
+ *   FGC = Cmd/MolTwisterCmdMolDyn/ForceFields/FunctorGenCellList.cpp
+ *   FGN = Cmd/MolTwisterCmdMolDyn/ForceFields/FunctorGenNeighList.cpp
+ *   MFM = Cmd/MolTwisterCmdMolDyn/ForceFields/MDFFMatrices.cpp
+ *   VV  = Cmd/MolTwisterCmdMolDyn/Integrators/VelVerlet.cpp
+ *   NHC = Cmd/MolTwisterCmdMolDyn/Integrators/NHChain.cpp
+ *   SB  = Cmd/MolTwisterCmdMolDyn/SimulationBox/SimulationBox.cpp
+ *   MDL = Cmd/MolTwisterCmdMolDyn/MDLoop/MDLoop.cpp
+ *   V3  = Utilities/3DVector.cpp
+ */
+#include <math.h>
+#include <float.h>
+#include <limits.h>
+#include <stdlib.h>
+#include <string.h>
+#include "mtmd_oracle.h"
+
+#define NPTS MTMD_NUM_POINTS
+#define MAXFF MTMD_MAX_FF_PER_PAIR
+#define CONV_T 120.272          /* Constants.h:27 */
+#define CONV_t 1.0E2            /* Constants.h:30 */
+#define CONV_PRESS 16388.2461   /* Constants.h:32 */
+#define COULOMB_K 1389.32f      /* Constants.h:40 */
+#define TO_RAD 0.017453292519943295 /* MolTwisterMDFFAngle_Harm.h:36 */
+
+/* ------------------------------------------------------------------ */
+/* 3-vectors: C3DVector arithmetic (V3:34-42,243-321)                  */
+/* ------------------------------------------------------------------ */
+typedef struct { double x, y, z; } v3;
+static inline v3 V(double x, double y, double z) { v3 r; r.x = x; r.y = y; r.z = z; return r; }
+static inline v3 vsub(v3 a, v3 b) { return V(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline v3 vadd(v3 a, v3 b) { return V(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline v3 vmul(v3 a, double s) { return V(a.x * s, a.y * s, a.z * s); }
+static inline double vdot(v3 a, v3 b) { return (a.x * b.x) + (a.y * b.y) + (a.z * b.z); }
+static inline double vnorm(v3 a) { return sqrt(a.x * a.x + a.y * a.y + a.z * a.z); }
+static inline v3 vcross(v3 a, v3 b)
+{
+    return V(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+
+/* x86-64 cvttss2si / cvttsd2si semantics for int(float) / int(double):
+ * truncation, and INT_MIN for NaN or out of range. */
+static inline int f2i(float v)
+{
+    if (!(v >= -2147483648.0f && v < 2147483648.0f)) return INT_MIN;
+    return (int)v;
+}
+static inline int d2i(double v)
+{
+    if (!(v > -2147483649.0 && v < 2147483648.0)) return INT_MIN;
+    return (int)v;
+}
+
+/* C3DVector::moveToSameSideOfPBCAsThis (V3:133-165): shifts v by one box
+ * width toward `self` on each axis where |self - v| > w/2. */
+static inline v3 move_same_side(v3 self, v3 v, const double w[3])
+{
+    double dX = self.x - v.x, dY = self.y - v.y, dZ = self.z - v.z;
+    dX = (dX < 0.0) ? -dX : dX;
+    dY = (dY < 0.0) ? -dY : dY;
+    dZ = (dZ < 0.0) ? -dZ : dZ;
+    if (dX > (w[0] / 2.0)) { if (self.x > v.x) v.x += w[0]; else v.x -= w[0]; }
+    if (dY > (w[1] / 2.0)) { if (self.y > v.y) v.y += w[1]; else v.y -= w[1]; }
+    if (dZ > (w[2] / 2.0)) { if (self.z > v.z) v.z += w[2]; else v.z -= w[2]; }
+    return v;
+}
+
+/* C3DVector::distToAcrossPBC2 (V3:68-90) */
+static inline double dist2_pbc(v3 self, v3 v, const double w[3])
+{
+    double dX = self.x - v.x, dY = self.y - v.y, dZ = self.z - v.z;
+    dX = (dX < 0.0) ? -dX : dX;
+    dY = (dY < 0.0) ? -dY : dY;
+    dZ = (dZ < 0.0) ? -dZ : dZ;
+    if (dX > (w[0] / 2.0)) dX = w[0] - dX;
+    if (dY > (w[1] / 2.0)) dY = w[1] - dY;
+    if (dZ > (w[2] / 2.0)) dZ = w[2] - dZ;
+    return dX * dX + dY * dY + dZ * dZ;
+}
+
+/* ------------------------------------------------------------------ */
+/* Analytic force fields used to fill the tables                        */
+/* ------------------------------------------------------------------ */
+
+/* r12*RInv with the R == 0 guard: calcNonBondForceCoeffs12
+ * (MolTwisterMDFFNonBonded.cpp:174-181), calcBondForceCoeffs12
+ * (MolTwisterMDFFBond.cpp:181-188), FCF:283-301 */
+static inline v3 coeffs12(v3 r1, v3 r2)
+{
+    v3 r12 = vsub(r2, r1);
+    double R = vnorm(r12);
+    double RInv = (R == 0.0) ? 1.0 / 1E-10 : 1.0 / R;
+    return vmul(r12, RInv);
+}
+
+/* CMDFFNonBonded_*::calcForces -> f2 and ::calcPotential
+ * LJ  MolTwisterMDFFNonBonded_LJ.cpp:131-162
+ * Buck MolTwisterMDFFNonBonded_Buck.cpp:116-143
+ * LJ1208 MolTwisterMDFFNonBonded_LJ1208.cpp:131-170 */
+static v3 nb_force(int kind, const double* p, v3 r1, v3 r2)
+{
+    v3 r12 = vsub(r2, r1);
+    double R = vnorm(r12);
+    double RInv = (R == 0.0) ? 1.0 / 1E-10 : 1.0 / R;
+    double RInv2 = RInv * RInv;
+    double c = 0.0;
+    if (kind == MTMD_NB_LJ) {
+        double eps = p[0], sig = p[1], alpha = p[2];
+        double RInv6 = RInv2 * RInv2 * RInv2;
+        double RInv12 = RInv6 * RInv6;
+        double s2 = sig * sig, s6 = s2 * s2 * s2, s12 = s6 * s6;
+        c = 4.0 * eps * RInv2 * (12.0 * s12 * RInv12 - 6.0 * alpha * s6 * RInv6);
+    } else if (kind == MTMD_NB_BUCK) {
+        double A = p[0], rho = p[1], C = p[2];
+        double RInv4 = RInv2 * RInv2;
+        double RInv6 = RInv4 * RInv2;
+        double rhoInv = (rho == 0.0) ? 1E-10 : 1.0 / rho;
+        c = RInv2 * (A * R * rhoInv * exp(-R * rhoInv) - 6.0 * C * RInv6);
+    } else if (kind == MTMD_NB_LJ1208) {
+        double eps = p[0], sig = p[1], alpha = p[2];
+        double RInv4 = RInv2 * RInv2;
+        double RInv6 = RInv4 * RInv2;
+        double RInv8 = RInv4 * RInv4;
+        double RInv12 = RInv6 * RInv6;
+        double s2 = sig * sig, s4 = s2 * s2, s6 = s4 * s2, s8 = s4 * s4, s12 = s6 * s6;
+        c = 4.0 * eps * RInv2 * (12.0 * s12 * RInv12 - 8.0 * alpha * s8 * RInv8);
+    }
+    return vmul(r12, c);
+}
+
+static double nb_pot(int kind, const double* p, v3 r1, v3 r2)
+{
+    v3 r12 = vsub(r2, r1);
+    double R = vnorm(r12);
+    double RInv = (R == 0.0) ? 1.0 / 1E-10 : 1.0 / R;
+    double RInv2 = RInv * RInv;
+    if (kind == MTMD_NB_LJ) {
+        double eps = p[0], sig = p[1], alpha = p[2];
+        double RInv6 = RInv2 * RInv2 * RInv2;
+        double RInv12 = RInv6 * RInv6;
+        double s2 = sig * sig, s6 = s2 * s2 * s2, s12 = s6 * s6;
+        return 4.0 * eps * (s12 * RInv12 - alpha * s6 * RInv6);
+    } else if (kind == MTMD_NB_BUCK) {
+        double A = p[0], rho = p[1], C = p[2];
+        double RInv4 = RInv2 * RInv2;
+        double RInv6 = RInv4 * RInv2;
+        double rhoInv = (rho == 0.0) ? 1E-10 : 1.0 / rho;
+        return A * exp(-R * rhoInv) - C * RInv6;
+    } else if (kind == MTMD_NB_LJ1208) {
+        double eps = p[0], sig = p[1], alpha = p[2];
+        double RInv4 = RInv2 * RInv2;
+        double RInv6 = RInv4 * RInv2;
+        double RInv8 = RInv4 * RInv4;
+        double RInv12 = RInv6 * RInv6;
+        double s2 = sig * sig, s4 = s2 * s2, s6 = s4 * s2, s8 = s4 * s4, s12 = s6 * s6;
+        return 4.0 * eps * (s12 * RInv12 - alpha * s8 * RInv8);
+    }
+    return 0.0;
+}
+
+/* CMDFFBond_Harm (MolTwisterMDFFBond_Harm.cpp:90-110),
+ * CMDFFBond_Morse (MolTwisterMDFFBond_Morse.cpp:95-117) */
+static v3 bond_force(int kind, const double* p, v3 r1, v3 r2)
+{
+    v3 r12 = vsub(r2, r1);
+    double R = vnorm(r12);
+    double RInv = (R == 0.0) ? 1.0 / 1E-10 : 1.0 / R;
+    double c = 0.0;
+    if (kind == MTMD_BOND_HARM) {
+        double k = p[0], r0 = p[1];
+        double deltaR = R - r0;
+        c = -2.0 * k * deltaR * RInv;
+    } else if (kind == MTMD_BOND_MORSE) {
+        double D = p[0], alpha = p[1], r0 = p[2];
+        double deltaR = R - r0;
+        double Exp = exp(-alpha * deltaR);
+        c = -2.0 * D * alpha * RInv * Exp * (1.0 - Exp);
+    }
+    return vmul(r12, c);
+}
+
+static double bond_pot(int kind, const double* p, v3 r1, v3 r2)
+{
+    v3 r12 = vsub(r2, r1);
+    double R = vnorm(r12);
+    if (kind == MTMD_BOND_HARM) {
+        double deltaR = R - p[1];
+        return p[0] * deltaR * deltaR;
+    } else if (kind == MTMD_BOND_MORSE) {
+        double deltaR = R - p[2];
+        double P = (1.0 - exp(-p[1] * deltaR));
+        return p[0] * P * P;
+    }
+    return 0.0;
+}
+
+/* calcAngularForceCoeffs13: FCF:303-324, MolTwisterMDFFAngle.cpp:62-83 */
+static v3 ang_coeffs13(v3 r1, v3 r2, v3 r3)
+{
+    v3 r12 = vsub(r1, r2);
+    double R = vnorm(r12);
+    double RInv = (R == 0.0) ? 1.0 / 1E-10 : 1.0 / R;
+    v3 K = vmul(r12, RInv);
+    v3 r32 = vsub(r3, r2);
+    double A = K.x * r32.x + K.y * r32.y + K.z * r32.z;
+    double B = r32.x * r32.x + r32.y * r32.y + r32.z * r32.z;
+    double sqrtB = sqrt(B);
+    double sqrtB3 = (B == 0.0) ? sqrt(1E-30) : sqrtB * sqrtB * sqrtB;
+    double dW_dx = (K.x * B - r32.x * A) / sqrtB3;
+    double dW_dy = (K.y * B - r32.y * A) / sqrtB3;
+    double dW_dz = (K.z * B - r32.z * A) / sqrtB3;
+    double W = (sqrtB == 0.0) ? A / 10E-3 : A / sqrtB;
+    double dDiv = sqrt(1.0 - W * W);
+    double dTheta_dW = (dDiv == 0.0) ? -1.0 / 1E-10 : -1.0 / dDiv;
+    return V(dTheta_dW * dW_dx, dTheta_dW * dW_dy, dTheta_dW * dW_dz);
+}
+
+/* calcAngularForceCoeffs2: FCF:326-346 */
+static v3 ang_coeffs2(v3 r1, v3 r2, v3 r3)
+{
+    v3 r12 = vsub(r1, r2);
+    v3 r32 = vsub(r3, r2);
+    double A = r12.x * r12.x + r12.y * r12.y + r12.z * r12.z;
+    double B = r32.x * r32.x + r32.y * r32.y + r32.z * r32.z;
+    double C = r12.x * r32.x + r12.y * r32.y + r12.z * r32.z;
+    double AB = A * B;
+    double sqrtAB = sqrt(AB);
+    double sqrtAB3 = (AB == 0.0) ? sqrt(1E-30) : sqrtAB * sqrtAB * sqrtAB;
+    double dW_dx = ((2.0 * r2.x - r1.x - r3.x) * AB + (r32.x * A + r12.x * B) * C) / sqrtAB3;
+    double dW_dy = ((2.0 * r2.y - r1.y - r3.y) * AB + (r32.y * A + r12.y * B) * C) / sqrtAB3;
+    double dW_dz = ((2.0 * r2.z - r1.z - r3.z) * AB + (r32.z * A + r12.z * B) * C) / sqrtAB3;
+    double W = (sqrtAB == 0.0) ? A / 10E-3 : C / sqrtAB;
+    double dDiv = sqrt(1.0 - W * W);
+    double dTheta_dW = (dDiv == 0.0) ? -1.0 / 1E-10 : -1.0 / dDiv;
+    return V(dTheta_dW * dW_dx, dTheta_dW * dW_dy, dTheta_dW * dW_dz);
+}
+
+/* calcDihedralForceCoeffs14 without the final sign flip:
+ * MolTwisterMDFFDih.cpp:64-87; FCF:348-377 is the same followed by *= -1 */
+static v3 dih_coeffs14(v3 r1, v3 r2, v3 r3, v3 r4)
+{
+    v3 r21 = vsub(r1, r2);
+    v3 r23 = vsub(r3, r2);
+    v3 r32 = vmul(r23, -1.0);
+    v3 r34 = vsub(r4, r3);
+    v3 n1 = vcross(r23, r21);
+    double n1Norm = vnorm(n1);
+    double n1NormInv = (n1Norm == 0.0) ? 1.0 / 1E-10 : 1.0 / n1Norm;
+    v3 K = vmul(n1, n1NormInv);
+    double a = r32.y * r34.z - r32.z * r34.y;
+    double b = r32.z * r34.x - r32.x * r34.z;
+    double c = r32.x * r34.y - r32.y * r34.x;
+    double A = K.x * a + K.y * b + K.z * c;
+    double B = a * a + b * b + c * c;
+    double sqrtBInv = (B == 0.0) ? 1.0 / 1E-20 : 1.0 / sqrt(B);
+    double sqrtBInv3 = sqrtBInv * sqrtBInv * sqrtBInv;
+    double W = A * sqrtBInv;
+    double dW_dx = ((K.y * r32.z - K.z * r32.y) * B - (r32.z * (r32.z * r34.x - r32.x * r34.z) - r32.y * (r32.x * r34.y - r32.y * r34.x)) * A) * sqrtBInv3;
+    double dW_dy = ((K.z * r32.x - K.x * r32.z) * B - (r32.x * (r32.x * r34.y - r32.y * r34.x) - r32.z * (r32.y * r34.z - r32.z * r34.y)) * A) * sqrtBInv3;
+    double dW_dz = ((K.x * r32.y - K.y * r32.x) * B - (r32.y * (r32.y * r34.z - r32.z * r34.y) - r32.x * (r32.z * r34.x - r32.x * r34.z)) * A) * sqrtBInv3;
+    double div = sqrt(1.0 - W * W);
+    double dTheta_dW = (div == 0.0) ? 1.0 / 1E-10 : 1.0 / div;
+    return V(dTheta_dW * dW_dx, dTheta_dW * dW_dy, dTheta_dW * dW_dz);
+}
+
+/* calcDihedralForceCoeffs23 without the final sign flip:
+ * MolTwisterMDFFDih.cpp:89-134; FCF:379-424 flips the sign */
+static v3 dih_coeffs23(v3 r1, v3 r2, v3 r3, v3 r4)
+{
+    v3 r21 = vsub(r1, r2);
+    v3 r23 = vsub(r3, r2);
+    v3 r32 = vmul(r23, -1.0);
+    v3 r34 = vsub(r4, r3);
+    double K1 = r32.z * r34.x - r32.x * r34.z;
+    double K2 = r34.z - r32.z;
+    double K3 = r23.z * r21.x - r23.x * r21.z;
+    double K4 = r32.x * r34.y - r32.y * r34.x;
+    double K5 = r32.y - r34.y;
+    double K6 = r23.x * r21.y - r23.y * r21.x;
+    double L1 = r32.y * r34.z - r32.z * r34.y;
+    double L2 = -K2;
+    double L3 = r23.y * r21.z - r23.z * r21.y;
+    double L4 = K4;
+    double L5 = r34.x - r32.x;
+    double L6 = K6;
+    double M1 = L1;
+    double M2 = -K5;
+    double M3 = L3;
+    double M4 = K1;
+    double M5 = -L5;
+    double M6 = K3;
+    double A = L3 * L3 + K3 * K3 + K6 * K6;
+    double B = L1 * L1 + K1 * K1 + K4 * K4;
+    double C = L3 * L1 + K3 * K1 + K6 * K4;
+    double AB = A * B;
+    double sqrtABInv = (AB == 0.0) ? 1.0 / 1E-20 : 1.0 / sqrt(AB);
+    double sqrtABInv3 = sqrtABInv * sqrtABInv * sqrtABInv;
+    double W = C * sqrtABInv;
+    double dC_dx = K2 * K3 + r21.y * K4 + K5 * K6 - r21.z * K1;
+    double dC_dy = L2 * L3 - r21.x * L4 + L5 * L6 + r21.z * L1;
+    double dC_dz = M2 * M3 + r21.x * M4 + M5 * M6 - r21.y * M1;
+    double dW_dx = (dC_dx * AB - C * ((r21.y * K6 - r21.z * K3) * B + (K1 * K2 + K4 * K5) * A)) * sqrtABInv3;
+    double dW_dy = (dC_dy * AB - C * ((r21.z * L3 - r21.x * L6) * B + (L1 * L2 + L4 * L5) * A)) * sqrtABInv3;
+    double dW_dz = (dC_dz * AB - C * ((r21.x * M6 - r21.y * M3) * B + (M1 * M2 + M4 * M5) * A)) * sqrtABInv3;
+    double div = sqrt(1.0 - W * W);
+    double dTheta_dW = (div == 0.0) ? 1.0 / 1E-10 : 1.0 / div;
+    return V(dTheta_dW * dW_dx, dTheta_dW * dW_dy, dTheta_dW * dW_dz);
+}
+
+/* CMDFFAngle_Harm::calcForces -> f3 and ::calcPotential
+ * (MolTwisterMDFFAngle_Harm.cpp:94-130) */
+static double angle_theta(v3 r1, v3 r2, v3 r3)
+{
+    v3 r21 = vsub(r1, r2);
+    v3 r23 = vsub(r3, r2);
+    double R21 = vnorm(r21), R23 = vnorm(r23);
+    double R21Inv = (R21 == 0.0) ? 1.0 / 1E-10 : 1.0 / R21;
+    double R23Inv = (R23 == 0.0) ? 1.0 / 1E-10 : 1.0 / R23;
+    double cosTheta = vdot(r21, r23) * (R21Inv * R23Inv);
+    return acos(cosTheta);
+}
+
+/* CMDFFDih_Harm::calcForces / ::calcPotential (MolTwisterMDFFDih_Harm.cpp:96-132) */
+static double dihedral_phi(v3 r1, v3 r2, v3 r3, v3 r4)
+{
+    v3 r21 = vsub(r1, r2);
+    v3 r23 = vsub(r3, r2);
+    v3 r32 = vmul(r23, -1.0);
+    v3 r34 = vsub(r4, r3);
+    v3 n1 = vcross(r23, r21);
+    v3 n2 = vcross(r32, r34);
+    double n1n2 = vnorm(n1) * vnorm(n2);
+    double cosPhi = (n1n2 == 0.0) ? vdot(n1, n2) / 1E-20 : vdot(n1, n2) / n1n2;
+    return acos(cosPhi);
+}
+
+/* ------------------------------------------------------------------ */
+/* Profiles: calc1DForceProfile / calc1DPotentialProfile               */
+/* ------------------------------------------------------------------ */
+int orc_profile(int which, int kind, const double* params, float x0, float x1,
+                int n, float* xo, float* fo, float* eo)
+{
+    double p[4] = {0, 0, 0, 0};
+    int i;
+    if (n <= 0) return -1;
+    for (i = 0; i < 4; i++) p[i] = params[i];
+    float delta = (x1 - x0) / (float)(n - 1);
+    for (i = 0; i < n; i++) {
+        float x = x0 + (float)i * delta;
+        double fval = 0.0, e = 0.0;
+        if (which == 0 || which == 1) {
+            /* MolTwisterMDFFNonBonded.cpp:126-172, MolTwisterMDFFBond.cpp:133-179 */
+            v3 r1 = V(0.0, 0.0, 0.0), r2 = V((double)x, 0.0, 0.0);
+            v3 f2 = (which == 0) ? nb_force(kind, p, r1, r2) : bond_force(kind, p, r1, r2);
+            v3 c = coeffs12(r1, r2);
+            fval = (c.x != 0.0) ? f2.x / c.x : 0.0;
+            e = (which == 0) ? nb_pot(kind, p, r1, r2) : bond_pot(kind, p, r1, r2);
+        } else if (which == 2) {
+            /* MolTwisterMDFFAngle.cpp:210-257 with CMDFFAngle_Harm */
+            v3 r1 = V(-1.0, 0.0, 0.0), r2 = V(0.0, 0.0, 0.0);
+            v3 r3 = V(-cos((double)x), sin((double)x), 0.0);
+            double Theta = angle_theta(r1, r2, r3);
+            double deltaTheta = Theta - p[1] * TO_RAD;
+            if (kind == MTMD_ANGLE_HARM) {
+                double dU_dTheta = -2.0 * p[0] * deltaTheta;
+                v3 f3 = vmul(ang_coeffs13(r1, r2, r3), dU_dTheta);
+                v3 c = ang_coeffs13(r1, r2, r3);
+                fval = (c.x != 0.0) ? f3.x / c.x : 0.0;
+                e = p[0] * deltaTheta * deltaTheta;
+            }
+        } else if (which == 3) {
+            /* MolTwisterMDFFDih.cpp:272-320 with CMDFFDih_Harm */
+            v3 r1 = V(0.0, 1.0, 0.0), r2 = V(0.0, 0.0, 0.0), r3 = V(0.0, 0.0, 1.0);
+            v3 r4 = V(sin((double)x), cos((double)x), 1.0);
+            if (kind == MTMD_DIH_HARM) {
+                double K = p[0];
+                int D = (int)p[1], N = (int)p[2];
+                double phi = dihedral_phi(r1, r2, r3, r4);
+                double dU_dPhi = -K * (double)(D * N) * sin((double)N * phi);
+                v3 f4 = vmul(dih_coeffs14(r1, r2, r3, r4), dU_dPhi);
+                v3 c = dih_coeffs14(r1, r2, r3, r4);
+                fval = (c.x != 0.0) ? f4.x / c.x : 0.0;
+                e = K * (1.0 + (double)D * cos((double)N * phi));
+            }
+        } else {
+            return -1;
+        }
+        if (xo) xo[i] = x;
+        if (fo) fo[i] = (float)fval;
+        if (eo) eo[i] = (float)e;
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Engine state                                                         */
+/* ------------------------------------------------------------------ */
+typedef struct { float x, f, e; } pt;   /* CMDFFMatrices::CPoint */
+
+struct orc_md {
+    int n, ntypes;
+    v3 *r, *p;
+    double* m;
+    float* q;
+    int* type;
+    int* mol;
+    unsigned char* mobile;
+    double L[3];
+    int ensemble;
+    /* tables */
+    pt* nb;            /* [((row*T + col)*MAXFF + ff)*NPTS + pt] */
+    int* nbcount;      /* [row*T + col] */
+    pt *bondtab, *angtab, *dihtab;
+    /* per-atom bonded lists (MFM:384-520) */
+    int *bfirst, *bnum, *bj, *btype;
+    int *afirst, *anum, *ai, *aj, *atype; unsigned char* acenter;
+    int *dfirst, *dnum, *di, *dj, *dl, *dtype; unsigned char* dcenter;
+    /* cells / neighbours */
+    float rc, shell;
+    int nc[3];
+    int *cix, *cfirst, *ccount, *clist;
+    int maxn;
+    int *ncount, *nlist;
+    /* functor parameters */
+    float scale12, scale13, scale14, scale1N;
+    int coulomb;        /* MTMD_COUL_ALLPAIRS or MTMD_COUL_CUTOFF */
+    int all_q_zero;
+    /* integrator */
+    double dt, Fcut;
+    int have_forces;
+    v3* F;
+    float* U;
+    /* NH thermostat (NHChain.h:55-66) */
+    int M, nresp;
+    double T, tau;
+    double *eta, *peta, *Q;
+    double w[3];
+};
+
+int orc_natoms(const orc_md* o) { return o->n; }
+
+/* CMt::exp / CMt::sinhXoverX (Cmd/MolTwisterCmdMolDyn/Integrators/Math.cpp:27-54) */
+double orc_cmt_exp(double x)
+{
+    if (x > 1.0) return 2.71828 * x;
+    return exp(x);
+}
+
+double orc_cmt_sinhxoverx(double x)
+{
+    static const double a[6] = { 1.0, 1.0 / 6.0, 1.0 / 120.0, 1.0 / 5040.0, 1.0 / 362880.0, 1.0 / 39916800.0 };
+    double sum = 0.0, xn = 1.0;
+    int n;
+    for (n = 0; n < 6; n++) { sum += a[n] * xn; xn *= (x * x); }
+    return sum;
+}
+
+static uint64_t splitmix64(uint64_t* s)
+{
+    uint64_t z = (*s += 0x9E3779B97F4A7C15ULL);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+/* uniform double in [0, 1) with 53 random bits */
+static double urand(uint64_t* s) { return (double)(splitmix64(s) >> 11) * (1.0 / 9007199254740992.0); }
+
+/* CSimulationBox::calcTemp (SB:168-180): sum p.p/m / (dim N), reduced units */
+static double calc_temp_reduced(const orc_md* o)
+{
+    double T = 0.0;
+    int k;
+    for (k = 0; k < o->n; k++) T += vdot(o->p[k], o->p[k]) / o->m[k];
+    return T / (3 * (double)o->n);
+}
+
+/* mol indices: connected components over the bond graph (MolecularSystemTools.cpp:238-280);
+ * only equality of labels matters to the force functor (FCF:147-156) */
+static int uf_find(int* par, int a) { while (par[a] != a) { par[a] = par[par[a]]; a = par[a]; } return a; }
+
+orc_md* orc_create(const mtmd_system* s, const mtmd_config* cfg)
+{
+    int n = s->n_atoms, T = s->n_types, k, j, i, c, r;
+    orc_md* o = (orc_md*)calloc(1, sizeof(orc_md));
+    o->n = n; o->ntypes = T;
+    o->r = (v3*)calloc(n, sizeof(v3));
+    o->p = (v3*)calloc(n, sizeof(v3));
+    o->m = (double*)calloc(n, sizeof(double));
+    o->q = (float*)calloc(n, sizeof(float));
+    o->type = (int*)calloc(n, sizeof(int));
+    o->mol = (int*)calloc(n, sizeof(int));
+    o->mobile = (unsigned char*)calloc(n, 1);
+    o->F = (v3*)calloc(n, sizeof(v3));
+    o->U = (float*)calloc(n, sizeof(float));
+    o->L[0] = s->box[0]; o->L[1] = s->box[1]; o->L[2] = s->box[2];
+    o->all_q_zero = 1;
+    for (k = 0; k < n; k++) {
+        o->r[k] = V(s->pos[3 * k], s->pos[3 * k + 1], s->pos[3 * k + 2]);
+        o->m[k] = s->mass[k];
+        o->q[k] = (float)s->charge[k];
+        if (o->q[k] != 0.0f) o->all_q_zero = 0;
+        o->type[k] = s->type[k];
+        o->mobile[k] = s->mobile ? s->mobile[k] : 1;
+    }
+
+    o->rc = (float)cfg->cutoff_radius;
+    o->shell = (float)cfg->neigh_shell;
+    o->scale12 = (float)cfg->scale12; o->scale13 = (float)cfg->scale13;
+    o->scale14 = (float)cfg->scale14; o->scale1N = (float)cfg->scale1N;
+    o->coulomb = (cfg->coulomb == MTMD_COUL_CUTOFF) ? MTMD_COUL_CUTOFF : MTMD_COUL_ALLPAIRS;
+    o->ensemble = cfg->ensemble;
+    o->dt = cfg->timestep / CONV_t;
+    o->Fcut = cfg->cutoff_force;
+
+    /* non-bonded table matrix (MFM:249-296) */
+    o->nb = (pt*)calloc((size_t)T * T * MAXFF * NPTS, sizeof(pt));
+    o->nbcount = (int*)calloc((size_t)T * T, sizeof(int));
+    for (c = 0; c < T; c++)
+        for (r = 0; r < T; r++) {
+            int cnt = 0;
+            for (i = 0; i < s->n_nonbonded; i++) {
+                const mtmd_nonbonded_ff* e = &s->nonbonded[i];
+                int match = (e->type_a == c && e->type_b == r) || (e->type_b == c && e->type_a == r);
+                if (!match || cnt >= MAXFF) continue;
+                float xs[NPTS], fs[NPTS], es[NPTS];
+                orc_profile(0, e->kind, e->p, 0.01f, o->rc, NPTS, xs, fs, es);
+                for (j = 0; j < NPTS; j++) {
+                    pt* d = &o->nb[(((size_t)r * T + c) * MAXFF + cnt) * NPTS + j];
+                    d->x = xs[j]; d->f = fs[j]; d->e = es[j];
+                }
+                cnt++;
+            }
+            o->nbcount[r * T + c] = cnt;
+        }
+    /* bonded tables (MFM:298-382) */
+    o->bondtab = (pt*)calloc((size_t)(s->n_bond_ff + 1) * NPTS, sizeof(pt));
+    o->angtab = (pt*)calloc((size_t)(s->n_angle_ff + 1) * NPTS, sizeof(pt));
+    o->dihtab = (pt*)calloc((size_t)(s->n_dihedral_ff + 1) * NPTS, sizeof(pt));
+    for (i = 0; i < s->n_bond_ff + s->n_angle_ff + s->n_dihedral_ff; i++) {
+        float xs[NPTS], fs[NPTS], es[NPTS];
+        pt* dst;
+        if (i < s->n_bond_ff) {
+            orc_profile(1, s->bond_ff[i].kind, s->bond_ff[i].p, 0.01f, o->rc, NPTS, xs, fs, es);
+            dst = o->bondtab + (size_t)i * NPTS;
+        } else if (i < s->n_bond_ff + s->n_angle_ff) {
+            int t = i - s->n_bond_ff;
+            orc_profile(2, s->angle_ff[t].kind, s->angle_ff[t].p, 0.0f, (float)(2.0 * M_PI), NPTS, xs, fs, es);
+            dst = o->angtab + (size_t)t * NPTS;
+        } else {
+            int t = i - s->n_bond_ff - s->n_angle_ff;
+            orc_profile(3, s->dihedral_ff[t].kind, s->dihedral_ff[t].p, 0.0f, (float)(2.0 * M_PI), NPTS, xs, fs, es);
+            dst = o->dihtab + (size_t)t * NPTS;
+        }
+        for (j = 0; j < NPTS; j++) { dst[j].x = xs[j]; dst[j].f = fs[j]; dst[j].e = es[j]; }
+    }
+
+    /* molecule indices from all bonds */
+    {
+        int* par = (int*)malloc(sizeof(int) * (n > 0 ? n : 1));
+        for (k = 0; k < n; k++) par[k] = k;
+        for (j = 0; j < s->n_bonds; j++) {
+            int a = s->bonds[3 * j], b = s->bonds[3 * j + 1];
+            if (a < 0 || b < 0 || a >= n || b >= n) continue;
+            a = uf_find(par, a); b = uf_find(par, b);
+            if (a != b) par[a < b ? b : a] = a < b ? a : b;
+        }
+        for (k = 0; k < n; k++) o->mol[k] = uf_find(par, k);
+        free(par);
+    }
+
+    /* per-atom bond lists (MFM:384-416) */
+    o->bfirst = (int*)calloc(n, sizeof(int)); o->bnum = (int*)calloc(n, sizeof(int));
+    o->bj = (int*)calloc(2 * s->n_bonds + 1, sizeof(int)); o->btype = (int*)calloc(2 * s->n_bonds + 1, sizeof(int));
+    {
+        int g = 0;
+        for (k = 0; k < n; k++) {
+            int cnt = 0;
+            o->bfirst[k] = g;
+            for (j = 0; j < s->n_bonds; j++) {
+                int a1 = s->bonds[3 * j], a2 = s->bonds[3 * j + 1], t = s->bonds[3 * j + 2];
+                int to = -1;
+                if (a1 == k) to = a2;
+                if (a2 == k) to = a1;
+                if (to != -1 && t != -1) { o->bj[g] = to; o->btype[g] = t; g++; cnt++; }
+            }
+            o->bnum[k] = cnt;
+        }
+    }
+    /* per-atom angle lists (MFM:418-462) */
+    o->afirst = (int*)calloc(n, sizeof(int)); o->anum = (int*)calloc(n, sizeof(int));
+    o->ai = (int*)calloc(3 * s->n_angles + 1, sizeof(int)); o->aj = (int*)calloc(3 * s->n_angles + 1, sizeof(int));
+    o->atype = (int*)calloc(3 * s->n_angles + 1, sizeof(int)); o->acenter = (unsigned char*)calloc(3 * s->n_angles + 1, 1);
+    {
+        int g = 0;
+        for (k = 0; k < n; k++) {
+            int cnt = 0;
+            o->afirst[k] = g;
+            for (j = 0; j < s->n_angles; j++) {
+                const int* a = s->angles + 4 * j;
+                int I = -1, J = -1, center = 0;
+                if (a[0] == k) { I = a[1]; J = a[2]; }
+                if (a[1] == k) { center = 1; I = a[0]; J = a[2]; }
+                if (a[2] == k) { I = a[1]; J = a[0]; }
+                if (I != -1 && J != -1 && a[3] != -1) {
+                    o->ai[g] = I; o->aj[g] = J; o->atype[g] = a[3]; o->acenter[g] = (unsigned char)center;
+                    g++; cnt++;
+                }
+            }
+            o->anum[k] = cnt;
+        }
+    }
+    /* per-atom dihedral lists (MFM:464-520) */
+    o->dfirst = (int*)calloc(n, sizeof(int)); o->dnum = (int*)calloc(n, sizeof(int));
+    o->di = (int*)calloc(4 * s->n_dihedrals + 1, sizeof(int)); o->dj = (int*)calloc(4 * s->n_dihedrals + 1, sizeof(int));
+    o->dl = (int*)calloc(4 * s->n_dihedrals + 1, sizeof(int)); o->dtype = (int*)calloc(4 * s->n_dihedrals + 1, sizeof(int));
+    o->dcenter = (unsigned char*)calloc(4 * s->n_dihedrals + 1, 1);
+    {
+        int g = 0;
+        for (k = 0; k < n; k++) {
+            int cnt = 0;
+            o->dfirst[k] = g;
+            for (j = 0; j < s->n_dihedrals; j++) {
+                const int* d = s->dihedrals + 5 * j;
+                int I = -1, J = -1, Lx = -1, center = 0;
+                if (d[0] == k) { I = d[1]; J = d[2]; Lx = d[3]; }
+                if (d[1] == k) { center = 1; I = d[0]; J = d[2]; Lx = d[3]; }
+                if (d[2] == k) { center = 1; I = d[3]; J = d[1]; Lx = d[0]; }
+                if (d[3] == k) { I = d[2]; J = d[1]; Lx = d[0]; }
+                if (I != -1 && J != -1 && Lx != -1 && d[4] != -1) {
+                    o->di[g] = I; o->dj[g] = J; o->dl[g] = Lx; o->dtype[g] = d[4];
+                    o->dcenter[g] = (unsigned char)center;
+                    g++; cnt++;
+                }
+            }
+            o->dnum[k] = cnt;
+        }
+    }
+
+    /* cell grid from the initial box (MFM:43-58) */
+    {
+        float R = o->rc + o->shell;
+        for (i = 0; i < 3; i++) {
+            float wf = (float)o->L[i];
+            o->nc[i] = (int)floor((double)(wf / R));
+        }
+    }
+    o->cix = (int*)calloc(3 * (size_t)n + 3, sizeof(int));
+    {
+        size_t ncell = (size_t)(o->nc[0] > 0 ? o->nc[0] : 0) * (o->nc[1] > 0 ? o->nc[1] : 0) * (o->nc[2] > 0 ? o->nc[2] : 0);
+        o->cfirst = (int*)calloc(ncell + 1, sizeof(int));
+        o->ccount = (int*)calloc(ncell + 1, sizeof(int));
+    }
+    o->clist = (int*)calloc((size_t)n + 1, sizeof(int));
+    o->maxn = 0; o->ncount = (int*)calloc((size_t)n + 1, sizeof(int)); o->nlist = NULL;
+
+    /* NH thermostat (SB:53-79, NHC:32-52,115-124) */
+    o->M = cfg->temperature_nh_len;
+    o->nresp = cfg->temperature_respa;
+    o->T = cfg->temperature / CONV_T;
+    o->tau = cfg->temperature_relax / CONV_t;
+    {
+        const double w0 = 1.0 / (1.0 - pow(2.0, 1.0 / 3.0));
+        o->w[0] = w0; o->w[1] = 1.0 - 2.0 * w0; o->w[2] = w0;
+    }
+    o->eta = (double*)calloc(o->M + 1, sizeof(double));
+    o->peta = (double*)calloc(o->M + 1, sizeof(double));
+    o->Q = (double*)calloc(o->M + 1, sizeof(double));
+    if (o->M > 0) {
+        o->Q[0] = (double)(3 * n) * o->T * o->tau * o->tau;
+        for (j = 1; j < o->M; j++) o->Q[j] = o->T * o->tau * o->tau;
+    }
+
+    /* momenta: given, or uniform in [-1,1] per component, immobile zeroed,
+     * rescaled to T (SB:98-133) */
+    if (s->mom) {
+        for (k = 0; k < n; k++) o->p[k] = V(s->mom[3 * k], s->mom[3 * k + 1], s->mom[3 * k + 2]);
+    } else {
+        uint64_t st = s->seed;
+        double Tnow, ps;
+        for (k = 0; k < n; k++) {
+            double px = 2.0 * urand(&st) - 1.0, py = 2.0 * urand(&st) - 1.0, pz = 2.0 * urand(&st) - 1.0;
+            if (px == 0.0) px = 1.0;
+            if (py == 0.0) py = 1.0;
+            if (pz == 0.0) pz = 1.0;
+            o->p[k] = V(px, py, pz);
+        }
+        for (k = 0; k < n; k++) if (!o->mobile[k]) o->p[k] = V(0.0, 0.0, 0.0);
+        Tnow = calc_temp_reduced(o);
+        ps = sqrt(o->T / Tnow);
+        for (k = 0; k < n; k++) { o->p[k].x *= ps; o->p[k].y *= ps; o->p[k].z *= ps; }
+    }
+    return o;
+}
+
+void orc_destroy(orc_md* o)
+{
+    if (!o) return;
+    free(o->r); free(o->p); free(o->m); free(o->q); free(o->type); free(o->mol); free(o->mobile);
+    free(o->F); free(o->U); free(o->nb); free(o->nbcount); free(o->bondtab); free(o->angtab); free(o->dihtab);
+    free(o->bfirst); free(o->bnum); free(o->bj); free(o->btype);
+    free(o->afirst); free(o->anum); free(o->ai); free(o->aj); free(o->atype); free(o->acenter);
+    free(o->dfirst); free(o->dnum); free(o->di); free(o->dj); free(o->dl); free(o->dtype); free(o->dcenter);
+    free(o->cix); free(o->cfirst); free(o->ccount); free(o->clist); free(o->ncount); free(o->nlist);
+    free(o->eta); free(o->peta); free(o->Q);
+    free(o);
+}
+
+void orc_set_state(orc_md* o, const double* pos, const double* mom)
+{
+    int k;
+    for (k = 0; k < o->n; k++) {
+        if (pos) o->r[k] = V(pos[3 * k], pos[3 * k + 1], pos[3 * k + 2]);
+        if (mom) o->p[k] = V(mom[3 * k], mom[3 * k + 1], mom[3 * k + 2]);
+    }
+    if (pos) o->have_forces = 0;
+}
+
+void orc_get_state(const orc_md* o, double* pos, double* mom)
+{
+    int k;
+    for (k = 0; k < o->n; k++) {
+        if (pos) { pos[3 * k] = o->r[k].x; pos[3 * k + 1] = o->r[k].y; pos[3 * k + 2] = o->r[k].z; }
+        if (mom) { mom[3 * k] = o->p[k].x; mom[3 * k + 1] = o->p[k].y; mom[3 * k + 2] = o->p[k].z; }
+    }
+}
+
+void orc_set_thermostat(orc_md* o, const double* eta, const double* p_eta)
+{
+    int j;
+    for (j = 0; j < o->M; j++) { if (eta) o->eta[j] = eta[j]; if (p_eta) o->peta[j] = p_eta[j]; }
+}
+
+void orc_get_thermostat(const orc_md* o, double* eta, double* p_eta)
+{
+    int j;
+    for (j = 0; j < o->M; j++) { if (eta) eta[j] = o->eta[j]; if (p_eta) p_eta[j] = o->peta[j]; }
+}
+
+void orc_get_mol(const orc_md* o, int32_t* mol) { memcpy(mol, o->mol, sizeof(int) * o->n); }
+
+/* ------------------------------------------------------------------ */
+/* Cell list: CCellList::updatePBC (MFM:67-76), CFunctorGenCellList    */
+/* (FGC:49-73), kernelAssembleList (FGC:75-142)                         */
+/* ------------------------------------------------------------------ */
+int orc_build_cells(orc_md* o, int32_t* ncells_xyz, int32_t* ixyz,
+                    int32_t* cell_first, int32_t* cell_count, int32_t* cell_list)
+{
+    int k, i, n = o->n;
+    int Nx = o->nc[0], Ny = o->nc[1], Nz = o->nc[2];
+    int total = Nx * Ny * Nz;
+    float wf[3], lf[3];
+    int err = 0, next = 0;
+    if (Nx < 1 || Ny < 1 || Nz < 1) return -1;   /* the reference indexes out of bounds here */
+    for (i = 0; i < 3; i++) { wf[i] = (float)o->L[i]; lf[i] = -wf[i] / 2.0f; }
+    for (i = 0; i < total; i++) { o->cfirst[i] = -1; o->ccount[i] = 0; }   /* resetCellList, MFM:78-85 */
+    for (k = 0; k < n; k++) {
+        v3 r = o->r[k];
+        int ix = d2i(floor((r.x - (double)lf[0]) * (double)Nx / (double)wf[0]));
+        int iy = d2i(floor((r.y - (double)lf[1]) * (double)Ny / (double)wf[1]));
+        int iz = d2i(floor((r.z - (double)lf[2]) * (double)Nz / (double)wf[2]));
+        ix = (ix < 0) ? 0 : ((ix >= Nx) ? Nx - 1 : ix);
+        iy = (iy < 0) ? 0 : ((iy >= Ny) ? Ny - 1 : iy);
+        iz = (iz < 0) ? 0 : ((iz >= Nz) ? Nz - 1 : iz);
+        o->cix[3 * k] = ix; o->cix[3 * k + 1] = iy; o->cix[3 * k + 2] = iz;
+    }
+    for (k = 0; k < n; k++) {
+        int c = o->cix[3 * k] + Nx * (o->cix[3 * k + 1] + o->cix[3 * k + 2] * Ny);
+        o->ccount[c]++;
+    }
+    for (i = 0; i < total; i++)
+        if (o->ccount[i] != 0) { o->cfirst[i] = next; next += o->ccount[i]; o->ccount[i] = 0; }
+    for (k = 0; k < n; k++) {
+        int c = o->cix[3 * k] + Nx * (o->cix[3 * k + 1] + o->cix[3 * k + 2] * Ny);
+        if (c < total) {
+            if (o->cfirst[c] != -1) {
+                int e = o->cfirst[c] + o->ccount[c];
+                if (e < n) { o->clist[e] = k; o->ccount[c]++; } else err = 2;
+            }
+        } else err = 1;
+    }
+    if (ncells_xyz) { ncells_xyz[0] = Nx; ncells_xyz[1] = Ny; ncells_xyz[2] = Nz; }
+    if (ixyz) memcpy(ixyz, o->cix, sizeof(int) * 3 * (size_t)n);
+    if (cell_first) memcpy(cell_first, o->cfirst, sizeof(int) * (size_t)total);
+    if (cell_count) memcpy(cell_count, o->ccount, sizeof(int) * (size_t)total);
+    if (cell_list) memcpy(cell_list, o->clist, sizeof(int) * (size_t)n);
+    return err;
+}
+
+/* ------------------------------------------------------------------ */
+/* Neighbour list: CFunctorGenNeighList::operator() (FGN:72-151).       */
+/* Rows are kept complete (the reference's grow-and-retry loop,         */
+/* MFM:161-179, changes only the capacity, not the content).            */
+/* ------------------------------------------------------------------ */
+static int neigh_row(const orc_md* o, int k, int* out, int cap)
+{
+    int Nx = o->nc[0], Ny = o->nc[1], Nz = o->nc[2];
+    float wf[3], lf[3];
+    double w[3];
+    float rc2 = o->rc;
+    int i, cnt = 0, ix, iy, iz;
+    rc2 *= rc2;
+    for (i = 0; i < 3; i++) {
+        wf[i] = (float)o->L[i]; lf[i] = -wf[i] / 2.0f;
+        w[i] = (double)(lf[i] + wf[i]) - (double)lf[i];
+    }
+    for (ix = o->cix[3 * k] - 1; ix <= o->cix[3 * k] + 1; ix++) {
+        int Ix = (ix >= 0) ? ((ix < Nx) ? ix : 0) : Nx - 1;
+        for (iy = o->cix[3 * k + 1] - 1; iy <= o->cix[3 * k + 1] + 1; iy++) {
+            int Iy = (iy >= 0) ? ((iy < Ny) ? iy : 0) : Ny - 1;
+            for (iz = o->cix[3 * k + 2] - 1; iz <= o->cix[3 * k + 2] + 1; iz++) {
+                int Iz = (iz >= 0) ? ((iz < Nz) ? iz : 0) : Nz - 1;
+                int c = Ix + Nx * (Iy + Iz * Ny);
+                int num = o->ccount[c], first = o->cfirst[c], e;
+                for (e = 0; e < num; e++) {
+                    int a = o->clist[first + e];
+                    double d2 = dist2_pbc(o->r[k], o->r[a], w);
+                    if (d2 < (double)rc2) {
+                        if (a == k) continue;
+                        if (cnt < cap) out[cnt] = a;
+                        cnt++;
+                    }
+                }
+            }
+        }
+    }
+    return cnt;
+}
+
+int orc_build_neighbors(orc_md* o, int32_t max_per_atom, int32_t* count, int32_t* list)
+{
+    int k, mx = 0;
+    int* tmp;
+    for (k = 0; k < o->n; k++) {
+        int c = neigh_row(o, k, NULL, 0);
+        o->ncount[k] = c;
+        if (c > mx) mx = c;
+    }
+    free(o->nlist);
+    o->maxn = mx > 0 ? mx : 1;
+    o->nlist = (int*)calloc((size_t)o->n * o->maxn, sizeof(int));
+    for (k = 0; k < o->n; k++) neigh_row(o, k, o->nlist + (size_t)k * o->maxn, o->maxn);
+    if (count) memcpy(count, o->ncount, sizeof(int) * (size_t)o->n);
+    if (list) {
+        tmp = (int*)list;
+        for (k = 0; k < o->n; k++) {
+            int c = o->ncount[k] < max_per_atom ? o->ncount[k] : max_per_atom;
+            memcpy(tmp + (size_t)k * max_per_atom, o->nlist + (size_t)k * o->maxn, sizeof(int) * (size_t)c);
+        }
+    }
+    return mx;
+}
+
+/* ------------------------------------------------------------------ */
+/* Force functor (FCF:95-281, 426-709)                                  */
+/* ------------------------------------------------------------------ */
+
+/* calcIndexOfLowerPointOfInterpolationLine (FCF:426-430) */
+static inline int lower_index(float x, const pt* first, const pt* last)
+{
+    float delta = (last->x - first->x) / (float)(NPTS - 1);
+    return f2i((x - first->x) / delta);
+}
+
+/* interpolateLinear (FCF:457-465) */
+static inline float interp(float r, float x1, float y1, float x2, float y2)
+{
+    float denom = x1 - x2;
+    float a, b;
+    if (denom == 0.0f) denom = 3.0f * FLT_MIN;
+    a = (y1 - y2) / denom;
+    b = y1 - a * x1;
+    return a * r + b;
+}
+
+/* calcAngle (FCF:432-440) */
+static float calc_angle(v3 rk, v3 ri, v3 rj)
+{
+    v3 rik = vsub(rk, ri), rij = vsub(rj, ri);
+    float dot = (float)vdot(rik, rij);
+    float RR = (float)vnorm(rik) * (float)vnorm(rij);
+    float cs = (RR != 0.0f) ? dot / RR : 1.0f;
+    return (float)acos((double)cs);
+}
+
+/* calcDihedral (FCF:442-455) */
+static float calc_dihedral(v3 rk, v3 ri, v3 rj, v3 rl)
+{
+    v3 rik = vsub(rk, ri), rij = vsub(rj, ri), rjl = vsub(rl, rj);
+    v3 n1 = vcross(rik, rij), n2 = vcross(rij, rjl);
+    float dot = (float)vdot(n1, n2);
+    float nn = (float)vnorm(n1) * (float)vnorm(n2);
+    float cs = (nn != 0.0f) ? dot / nn : 1.0f;
+    if (cs < -1.0f) cs = -1.0f;
+    if (cs > 1.0f) cs = 1.0f;
+    return (float)acos((double)cs);
+}
+
+/* table force on r_k along r_k - r_i, optionally energy: the shared body of
+ * calcForceNonBondedOn_r_k per FF entry (FCF:481-504) and calcForceBondOn_r_k
+ * (FCF:547-567) */
+static inline void table_eval(const pt* tab, float x, int want_e, float* fsum, float* esum)
+{
+    int idx = lower_index(x, &tab[0], &tab[NPTS - 1]);
+    if (idx >= 0 && idx < NPTS - 1) {
+        const pt* p1 = &tab[idx];
+        const pt* p2 = &tab[idx + 1];
+        *fsum += interp(x, p1->x, p1->f, p2->x, p2->f);
+        if (want_e) *esum += interp(x, p1->x, p1->e, p2->x, p2->e);
+    }
+}
+
+/* scaleForcesAndPotentials (FCF:95-157) */
+static void scale_pair(const orc_md* o, v3* f, float* u, int i, int molk, int k)
+{
+    int j, hasB = 0, hasA = 0, hasD = 0;
+    for (j = 0; j < o->bnum[k]; j++)
+        if (i == o->bj[o->bfirst[k] + j]) {
+            *f = vmul(*f, (double)o->scale12); *u *= o->scale12; hasB = 1; break;
+        }
+    if (!hasB)
+        for (j = 0; j < o->anum[k]; j++) {
+            int g = o->afirst[k] + j;
+            if (!o->acenter[g] && i == o->aj[g]) {
+                *f = vmul(*f, (double)o->scale13); *u *= o->scale13; hasA = 1; break;
+            }
+        }
+    if (!hasB && !hasA)
+        for (j = 0; j < o->dnum[k]; j++) {
+            int g = o->dfirst[k] + j;
+            if (!o->dcenter[g] && i == o->dl[g]) {
+                *f = vmul(*f, (double)o->scale14); *u *= o->scale14; hasD = 1; break;
+            }
+        }
+    if (!hasB && !hasA && !hasD) {
+        int moli = o->mol[i];
+        if (moli >= 0 && molk >= 0 && moli == molk) { *f = vmul(*f, (double)o->scale1N); *u *= o->scale1N; }
+    }
+}
+
+/* calcForceCoulombOn_r_k (FCF:514-535) */
+static inline v3 coulomb_pair(const orc_md* o, v3 rk, v3 ri, int k, int i, float* U)
+{
+    v3 r = vsub(rk, ri);
+    float qi = o->q[i], qk = o->q[k];
+    float R = (float)vnorm(r);
+    double R3 = (double)(R * R * R);
+    double K = (double)(COULOMB_K * qk * qi);
+    if (i > k && R != 0.0f) *U += (float)K / R;
+    if (R3 != 0.0) return vmul(r, K / R3);
+    return V(0.0, 0.0, 0.0);
+}
+
+static void force_atom(const orc_md* o, int k, const double wbox[3], v3* Fout, float* Uout)
+{
+    v3 F = V(0.0, 0.0, 0.0), f;
+    float U = 0.0f;
+    int j, i, molk;
+    v3 rk;
+    *Fout = F; *Uout = 0.0f;
+    if (!o->mobile[k]) return;
+    rk = o->r[k];
+    molk = o->mol[k];
+
+    /* bonds (FCF:172-184, 537-575) */
+    for (j = 0; j < o->bnum[k]; j++) {
+        int g = o->bfirst[k] + j, ib = o->bj[g];
+        const pt* tab = o->bondtab + (size_t)o->btype[g] * NPTS;
+        v3 ri = move_same_side(rk, o->r[ib], wbox);
+        float r = (float)vnorm(vsub(rk, ri));
+        float mdU = 0.0f, e = 0.0f;
+        table_eval(tab, r, ib > k, &mdU, &e);
+        if (ib > k) U += e;
+        f = vmul(coeffs12(ri, rk), (double)mdU);
+        F = vadd(F, f);
+    }
+    /* angles (FCF:186-207, 577-642) */
+    for (j = 0; j < o->anum[k]; j++) {
+        int g = o->afirst[k] + j;
+        const pt* tab = o->angtab + (size_t)o->atype[g] * NPTS;
+        v3 ri = move_same_side(rk, o->r[o->ai[g]], wbox);
+        v3 rj = move_same_side(rk, o->r[o->aj[g]], wbox);
+        float mdU = 0.0f, e = 0.0f;
+        if (o->acenter[g]) {
+            /* calcForceAngularOn_r_i(r_i, r_k, r_j): angle at k */
+            float theta = calc_angle(ri, rk, rj);
+            table_eval(tab, theta, 1, &mdU, &e);
+            U += e;
+            f = vmul(ang_coeffs2(rj, rk, ri), (double)mdU);
+        } else {
+            float theta = calc_angle(rk, ri, rj);
+            table_eval(tab, theta, 0, &mdU, &e);
+            f = vmul(ang_coeffs13(rj, ri, rk), (double)mdU);
+        }
+        F = vadd(F, f);
+    }
+    /* dihedrals (FCF:209-232, 644-709) */
+    for (j = 0; j < o->dnum[k]; j++) {
+        int g = o->dfirst[k] + j;
+        const pt* tab = o->dihtab + (size_t)o->dtype[g] * NPTS;
+        v3 ri = move_same_side(rk, o->r[o->di[g]], wbox);
+        v3 rj = move_same_side(rk, o->r[o->dj[g]], wbox);
+        v3 rl = move_same_side(rk, o->r[o->dl[g]], wbox);
+        float mdU = 0.0f, e = 0.0f;
+        if (o->dcenter[g]) {
+            /* calcForceDihedralOn_r_i(r_i, r_k, r_j, r_l) */
+            float theta = calc_dihedral(ri, rk, rj, rl);
+            table_eval(tab, theta, 1, &mdU, &e);
+            U += 0.5f * e;
+            f = vmul(vmul(dih_coeffs23(rl, rj, rk, ri), -1.0), (double)mdU);
+        } else {
+            float theta = calc_dihedral(rk, ri, rj, rl);
+            table_eval(tab, theta, 0, &mdU, &e);
+            f = vmul(vmul(dih_coeffs14(rl, rj, ri, rk), -1.0), (double)mdU);
+        }
+        F = vadd(F, f);
+    }
+    /* neighbours (FCF:234-256, 467-512) */
+    for (j = 0; j < o->ncount[k]; j++) {
+        int ia = o->nlist[(size_t)k * o->maxn + j];
+        float u = 0.0f, r, mdU = 0.0f;
+        v3 ri = move_same_side(rk, o->r[ia], wbox);
+        r = (float)vnorm(vsub(rk, ri));
+        if (r < 1E-3f) {
+            f = V(0.0, 0.0, 0.0);
+        } else {
+            int T = o->ntypes, row = o->type[k], col = o->type[ia], ff;
+            int cnt = o->nbcount[row * T + col];
+            for (ff = 0; ff < cnt; ff++) {
+                const pt* tab = o->nb + (((size_t)row * T + col) * MAXFF + ff) * NPTS;
+                table_eval(tab, r, ia > k, &mdU, &u);
+            }
+            f = vmul(coeffs12(ri, rk), (double)mdU);
+        }
+        scale_pair(o, &f, &u, ia, molk, k);
+        F = vadd(F, f);
+        U += u;
+    }
+    /* Coulomb (FCF:258-278) */
+    if (!o->all_q_zero) {
+        if (o->coulomb == MTMD_COUL_ALLPAIRS) {
+            for (i = 0; i < o->n; i++) {
+                float u = 0.0f;
+                v3 ri = move_same_side(rk, o->r[i], wbox);
+                f = coulomb_pair(o, rk, ri, k, i, &u);
+                scale_pair(o, &f, &u, i, molk, k);
+                F = vadd(F, f);
+                U += u;
+            }
+        } else {
+            for (j = 0; j < o->ncount[k]; j++) {
+                int ia = o->nlist[(size_t)k * o->maxn + j];
+                float u = 0.0f;
+                v3 ri = move_same_side(rk, o->r[ia], wbox);
+                f = coulomb_pair(o, rk, ri, k, ia, &u);
+                scale_pair(o, &f, &u, ia, molk, k);
+                F = vadd(F, f);
+                U += u;
+            }
+        }
+    }
+    *Fout = F;
+    *Uout = U;
+}
+
+static void compute_forces_internal(orc_md* o)
+{
+    int k;
+    double wbox[3];
+    orc_build_cells(o, NULL, NULL, NULL, NULL, NULL);
+    orc_build_neighbors(o, 0, NULL, NULL);
+    for (k = 0; k < 3; k++) {
+        /* CFunctorCalcForce pbc_ (FCF:41): [-Lf/2, Lf/2] in double */
+        double Lf = (double)(float)o->L[k];
+        double hi = Lf / 2.0, lo = -Lf / 2.0;
+        wbox[k] = hi - lo;
+    }
+    for (k = 0; k < o->n; k++) force_atom(o, k, wbox, &o->F[k], &o->U[k]);
+    o->have_forces = 1;
+}
+
+void orc_compute_forces(orc_md* o, double* F, float* U)
+{
+    compute_forces_internal(o);
+    orc_get_forces(o, F, U);
+}
+
+void orc_get_forces(const orc_md* o, double* F, float* U)
+{
+    int k;
+    for (k = 0; k < o->n; k++) {
+        if (F) { F[3 * k] = o->F[k].x; F[3 * k + 1] = o->F[k].y; F[3 * k + 2] = o->F[k].z; }
+        if (U) U[k] = o->U[k];
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Nose-Hoover chain (NHC:54-113) with CFctT (MDL:29-60)                */
+/* ------------------------------------------------------------------ */
+static double fctT_G(const orc_md* o, int j, double beta)
+{
+    const double g = (double)(3 * o->n);
+    if (j == 0) {
+        double s = 0.0;
+        int k;
+        for (k = 0; k < o->n; k++) s += vdot(o->p[k], o->p[k]) / o->m[k];
+        return s - (g / beta);
+    }
+    return (o->peta[j - 1] * o->peta[j - 1] / o->Q[j - 1]) - (1.0 / beta);
+}
+
+static void fctT_scale(orc_md* o, double coeff)
+{
+    int k;
+    for (k = 0; k < o->n; k++) o->p[k] = vmul(o->p[k], coeff);
+}
+
+void orc_nh_propagate(orc_md* o)
+{
+    double beta = 1.0 / o->T, coeff;
+    int M = o->M, mu, i, j;
+    if (M > 0 && o->peta[0] > 15.0) o->peta[0] = 10.0;
+    if (M <= 0) return;
+    for (mu = 0; mu < 3; mu++) {
+        double Dt = o->w[mu] * o->dt / (double)o->nresp;
+        for (i = 0; i < o->nresp; i++) {
+            o->peta[M - 1] += (Dt / 4.0) * fctT_G(o, M - 1, beta);
+            for (j = 0; j < M - 1; j++) {
+                coeff = orc_cmt_exp(-(Dt / 8.0) * (o->peta[j + 1] / o->Q[j + 1]));
+                o->peta[j] *= coeff;
+                o->peta[j] += (Dt / 4.0) * fctT_G(o, j, beta);
+                coeff = orc_cmt_exp(-(Dt / 8.0) * (o->peta[j + 1] / o->Q[j + 1]));
+                o->peta[j] *= coeff;
+            }
+            for (j = 0; j < M; j++) o->eta[j] += (Dt / 2.0) * (o->peta[j] / o->Q[j]);
+            coeff = orc_cmt_exp(-((Dt / 2.0) * (o->peta[0] / o->Q[0])));
+            fctT_scale(o, coeff);
+            for (j = M - 2; j >= 0; j--) {
+                coeff = orc_cmt_exp(-(Dt / 8.0) * (o->peta[j + 1] / o->Q[j + 1]));
+                o->peta[j] *= coeff;
+                o->peta[j] += (Dt / 4.0) * fctT_G(o, j, beta);
+                coeff = orc_cmt_exp(-(Dt / 8.0) * (o->peta[j + 1] / o->Q[j + 1]));
+                o->peta[j] *= coeff;
+            }
+            o->peta[M - 1] += (Dt / 4.0) * fctT_G(o, M - 1, beta);
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Integrator (VV:55-122), wrap (SB:135-166), MD loop (MDL:124-138)     */
+/* ------------------------------------------------------------------ */
+static void cut_forces(orc_md* o)
+{
+    int k;
+    double Fc = o->Fcut;
+    for (k = 0; k < o->n; k++) {
+        v3* F = &o->F[k];
+        if (fabs(F->x) > Fc) F->x = ((F->x >= 0.0) ? 1.0 : -1.0) * Fc;
+        if (fabs(F->y) > Fc) F->y = ((F->y >= 0.0) ? 1.0 : -1.0) * Fc;
+        if (fabs(F->z) > Fc) F->z = ((F->z >= 0.0) ? 1.0 : -1.0) * Fc;
+    }
+}
+
+static inline void pbc_add(double* pos, double L, double Lm) { *pos -= (double)d2i((*pos + L) / Lm) * Lm; }
+
+static void pbc_wrap(orc_md* o)
+{
+    int k;
+    double LX = o->L[0], LY = o->L[1], LZ = o->L[2];
+    double X2 = LX / 2.0, Y2 = LY / 2.0, Z2 = LZ / 2.0;
+    for (k = 0; k < o->n; k++) {
+        v3* r = &o->r[k];
+        if (r->x >= X2) pbc_add(&r->x, X2, LX);
+        if (r->x < -X2) pbc_add(&r->x, -X2, LX);
+        if (r->y >= Y2) pbc_add(&r->y, Y2, LY);
+        if (r->y < -Y2) pbc_add(&r->y, -Y2, LY);
+        if (r->z >= Z2) pbc_add(&r->z, Z2, LZ);
+        if (r->z < -Z2) pbc_add(&r->z, -Z2, LZ);
+    }
+}
+
+static void vv_step(orc_md* o)
+{
+    double dt_2 = o->dt / 2.0;
+    int k;
+    for (k = 0; k < o->n; k++) {
+        double mk = o->m[k];
+        o->p[k] = vadd(o->p[k], vmul(o->F[k], dt_2));
+        o->r[k] = vadd(o->r[k], vmul(o->p[k], o->dt / mk));
+    }
+    compute_forces_internal(o);
+    if (o->Fcut > 0) cut_forces(o);
+    for (k = 0; k < o->n; k++) o->p[k] = vadd(o->p[k], vmul(o->F[k], dt_2));
+}
+
+void orc_step(orc_md* o, int nsteps)
+{
+    int t;
+    if (!o->have_forces) compute_forces_internal(o);   /* calcInitialForces, MDL:189-192 */
+    for (t = 0; t < nsteps; t++) {
+        if (o->ensemble == MTMD_NVT) orc_nh_propagate(o);
+        vv_step(o);
+        if (o->ensemble == MTMD_NVT) orc_nh_propagate(o);
+        pbc_wrap(o);
+    }
+}
+
+/* calcTemp, calcPress, calcV, calcUtot, calcCurrentKineticEnergy (SB:168-231) */
+void orc_thermo(orc_md* o, double* out)
+{
+    double sp = 0.0, spf = 0.0, Kt = 0.0, Ut = 0.0, V, Tred, P;
+    int k;
+    if (!o->have_forces) compute_forces_internal(o);
+    for (k = 0; k < o->n; k++) sp += vdot(o->p[k], o->p[k]) / o->m[k];
+    Tred = sp / (3 * (double)o->n);
+    V = o->L[0] * o->L[1] * o->L[2];
+    for (k = 0; k < o->n; k++) {
+        double p2 = vdot(o->p[k], o->p[k]) / o->m[k];
+        double f = vdot(o->F[k], o->r[k]);
+        spf += (p2 + f);
+    }
+    P = (1.0 / (3.0 * V)) * spf;
+    for (k = 0; k < o->n; k++) Kt += vdot(o->p[k], o->p[k]) / o->m[k];
+    Kt = 0.5 * Kt;
+    for (k = 0; k < o->n; k++) Ut += (double)o->U[k];
+    out[0] = Tred * CONV_T;
+    out[1] = P * CONV_PRESS;
+    out[2] = V;
+    out[3] = Ut;
+    out[4] = Kt;
+    out[5] = Ut + Kt;
+    out[6] = sp;
+    {
+        double sf = 0.0;
+        for (k = 0; k < o->n; k++) sf += vdot(o->F[k], o->r[k]);
+        out[7] = sf;
+    }
+}
+
+/* CMDLoop::runOptimization (MDL:143-173), gradientDescentStep (SB:197-208) */
+int orc_optimize(orc_md* o, int max_steps, double lr, double accuracy, int out_stride)
+{
+    double Un = 0.0, Unp1, Udiff;
+    int i, k;
+    compute_forces_internal(o);
+    for (k = 0; k < o->n; k++) Un += (double)o->U[k];
+    for (i = 0; i < max_steps; i++) {
+        compute_forces_internal(o);
+        for (k = 0; k < o->n; k++) o->r[k] = vadd(o->r[k], vmul(o->F[k], lr));
+        Unp1 = 0.0;
+        for (k = 0; k < o->n; k++) Unp1 += (double)o->U[k];
+        Udiff = fabs(Unp1 - Un);
+        Un = Unp1;
+        if ((Udiff < accuracy) && (i > out_stride)) { i++; break; }
+    }
+    o->have_forces = 0;
+    return i;
+}

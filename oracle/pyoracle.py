@@ -1,0 +1,284 @@
+"""ctypes wrappers for the checkers (TEST INFRASTRUCTURE ONLY).
+
+  Oracle    - oracle/liboracle.so, the C restatement (mtmd_oracle.c)
+  Reference - oracle/_ref/libmtref.so, MolTwister's own CPU sources driven by
+              ref_driver.cpp (present only where /root/reference was built)
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use this
+module; the product never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from moltwister_amd import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_LIB = os.path.join(HERE, "liboracle.so")
+REF_LIB = os.path.join(HERE, "_ref", "libmtref.so")
+REF_TREE = "/root/reference"
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+_fp = C.POINTER(C.c_float)
+_vp = C.c_void_p
+
+
+def build(with_ref: bool | None = None) -> None:
+    """make the oracle (and the reference driver where the reference exists)."""
+    subprocess.run(["make", "-s", "-C", HERE, "oracle"], check=True)
+    if with_ref is None:
+        with_ref = os.path.isdir(REF_TREE)
+    if with_ref:
+        subprocess.run(["make", "-s", "-j8", "-C", HERE, "ref", f"REF={REF_TREE}"], check=True)
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_LIB)
+
+
+def _f64(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _i32(a):
+    return a.ctypes.data_as(_ip)
+
+
+def _f32(a):
+    return a.ctypes.data_as(_fp)
+
+
+_olib = None
+_rlib = None
+
+
+def _oracle_lib():
+    global _olib
+    if _olib is None:
+        if not os.path.exists(ORACLE_LIB):
+            build(with_ref=False)
+        L = C.CDLL(ORACLE_LIB)
+        L.orc_profile.argtypes = [C.c_int, C.c_int, _dp, C.c_float, C.c_float, C.c_int, _fp, _fp, _fp]
+        L.orc_create.restype = _vp
+        L.orc_create.argtypes = [C.POINTER(abi.MtmdSystem), C.POINTER(abi.MtmdConfig)]
+        L.orc_destroy.argtypes = [_vp]
+        L.orc_set_state.argtypes = [_vp, _dp, _dp]
+        L.orc_get_state.argtypes = [_vp, _dp, _dp]
+        L.orc_set_thermostat.argtypes = [_vp, _dp, _dp]
+        L.orc_get_thermostat.argtypes = [_vp, _dp, _dp]
+        L.orc_get_mol.argtypes = [_vp, _ip]
+        L.orc_build_cells.argtypes = [_vp, _ip, _ip, _ip, _ip, _ip]
+        L.orc_build_neighbors.argtypes = [_vp, C.c_int32, _ip, _ip]
+        L.orc_compute_forces.argtypes = [_vp, _dp, _fp]
+        L.orc_nh_propagate.argtypes = [_vp]
+        L.orc_step.argtypes = [_vp, C.c_int]
+        L.orc_thermo.argtypes = [_vp, _dp]
+        L.orc_get_forces.argtypes = [_vp, _dp, _fp]
+        L.orc_optimize.argtypes = [_vp, C.c_int, C.c_double, C.c_double, C.c_int]
+        L.orc_cmt_exp.restype = C.c_double
+        L.orc_cmt_exp.argtypes = [C.c_double]
+        L.orc_cmt_sinhxoverx.restype = C.c_double
+        L.orc_cmt_sinhxoverx.argtypes = [C.c_double]
+        _olib = L
+    return _olib
+
+
+def _ref_lib():
+    global _rlib
+    if _rlib is None:
+        if not os.path.exists(REF_LIB):
+            raise FileNotFoundError(REF_LIB)
+        L = C.CDLL(REF_LIB)
+        L.ref_profile.argtypes = [C.c_int, C.c_int, _dp, C.c_float, C.c_float, C.c_int, _fp, _fp, _fp]
+        L.ref_create.restype = _vp
+        L.ref_create.argtypes = [C.POINTER(abi.MtmdSystem), C.POINTER(abi.MtmdConfig)]
+        L.ref_destroy.argtypes = [_vp]
+        L.ref_set_state.argtypes = [_vp, _dp, _dp]
+        L.ref_get_mol.argtypes = [_vp, _ip]
+        L.ref_build_cells.argtypes = [_vp, _ip, _ip, _ip, _ip, _ip]
+        L.ref_build_neighbors.argtypes = [_vp, C.c_int, _ip, _ip, _ip]
+        L.ref_compute_forces.argtypes = [_vp, _dp, _dp, _fp]
+        L.ref_set_thermostat.argtypes = [_vp, _dp, _dp]
+        L.ref_get_thermostat.argtypes = [_vp, _dp, _dp]
+        L.ref_nh_propagate.argtypes = [_vp, _dp]
+        L.ref_cmt_exp.restype = C.c_double
+        L.ref_cmt_exp.argtypes = [C.c_double]
+        L.ref_cmt_sinhxoverx.restype = C.c_double
+        L.ref_cmt_sinhxoverx.argtypes = [C.c_double]
+        _rlib = L
+    return _rlib
+
+
+def profile(which: int, kind: int, params, x0: float, x1: float, n: int = 1000, ref: bool = False):
+    p = np.zeros(4)
+    p[:len(params)] = params
+    x, f, e = (np.zeros(n, np.float32) for _ in range(3))
+    lib = _ref_lib() if ref else _oracle_lib()
+    fn = lib.ref_profile if ref else lib.orc_profile
+    rc = fn(which, kind, _f64(p), x0, x1, n, _f32(x), _f32(f), _f32(e))
+    assert rc == 0
+    return x, f, e
+
+
+class _Base:
+    def __init__(self, system, cfg):
+        self.n = system.n_atoms
+        self.system = system
+        self.cfg = cfg
+        self._csys = system.to_c()
+
+
+class Oracle(_Base):
+    """The C restatement driven through ctypes."""
+
+    def __init__(self, system, cfg):
+        super().__init__(system, cfg)
+        self.lib = _oracle_lib()
+        self.h = self.lib.orc_create(C.byref(self._csys), C.byref(cfg))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.orc_destroy(self.h)
+            self.h = None
+
+    def set_state(self, pos=None, mom=None):
+        pos = None if pos is None else np.ascontiguousarray(pos, np.float64)
+        mom = None if mom is None else np.ascontiguousarray(mom, np.float64)
+        self.lib.orc_set_state(self.h, _f64(pos) if pos is not None else None, _f64(mom) if mom is not None else None)
+
+    def get_state(self):
+        pos, mom = np.zeros((self.n, 3)), np.zeros((self.n, 3))
+        self.lib.orc_get_state(self.h, _f64(pos), _f64(mom))
+        return pos, mom
+
+    def set_thermostat(self, eta, p_eta):
+        eta, p_eta = np.ascontiguousarray(eta, np.float64), np.ascontiguousarray(p_eta, np.float64)
+        self.lib.orc_set_thermostat(self.h, _f64(eta), _f64(p_eta))
+
+    def get_thermostat(self):
+        M = self.cfg.temperature_nh_len
+        eta, p_eta = np.zeros(M), np.zeros(M)
+        self.lib.orc_get_thermostat(self.h, _f64(eta), _f64(p_eta))
+        return eta, p_eta
+
+    def mol(self):
+        m = np.zeros(self.n, np.int32)
+        self.lib.orc_get_mol(self.h, _i32(m))
+        return m
+
+    def build_cells(self):
+        nc = np.zeros(3, np.int32)
+        self.lib.orc_build_cells(self.h, _i32(nc), None, None, None, None)
+        tot = int(np.prod(nc))
+        ixyz = np.zeros((self.n, 3), np.int32)
+        first, count, lst = np.zeros(tot, np.int32), np.zeros(tot, np.int32), np.zeros(self.n, np.int32)
+        err = self.lib.orc_build_cells(self.h, _i32(nc), _i32(ixyz), _i32(first), _i32(count), _i32(lst))
+        return dict(err=err, nc=nc, ixyz=ixyz, first=first, count=count, list=lst)
+
+    def build_neighbors(self):
+        self.build_cells()
+        mx = self.lib.orc_build_neighbors(self.h, 0, None, None)
+        cnt = np.zeros(self.n, np.int32)
+        lst = np.zeros((self.n, max(mx, 1)), np.int32)
+        self.lib.orc_build_neighbors(self.h, max(mx, 1), _i32(cnt), _i32(lst))
+        return cnt, lst
+
+    def compute_forces(self):
+        F, U = np.zeros((self.n, 3)), np.zeros(self.n, np.float32)
+        self.lib.orc_compute_forces(self.h, _f64(F), _f32(U))
+        return F, U
+
+    def get_forces(self):
+        F, U = np.zeros((self.n, 3)), np.zeros(self.n, np.float32)
+        self.lib.orc_get_forces(self.h, _f64(F), _f32(U))
+        return F, U
+
+    def nh_propagate(self):
+        self.lib.orc_nh_propagate(self.h)
+
+    def step(self, n=1):
+        self.lib.orc_step(self.h, n)
+
+    def thermo(self):
+        out = np.zeros(8)
+        self.lib.orc_thermo(self.h, _f64(out))
+        return dict(zip(["T", "P", "V", "U", "K", "Etot", "sum_p2m", "sum_fr"], out))
+
+    def optimize(self, max_steps, lr, accuracy, out_stride):
+        return self.lib.orc_optimize(self.h, max_steps, lr, accuracy, out_stride)
+
+
+class Reference(_Base):
+    """MolTwister's own functors and NH chain (oracle/_ref/libmtref.so)."""
+
+    def __init__(self, system, cfg):
+        super().__init__(system, cfg)
+        self.lib = _ref_lib()
+        self.h = self.lib.ref_create(C.byref(self._csys), C.byref(cfg))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.ref_destroy(self.h)
+            self.h = None
+
+    def set_state(self, pos=None, mom=None):
+        pos = None if pos is None else np.ascontiguousarray(pos, np.float64)
+        mom = None if mom is None else np.ascontiguousarray(mom, np.float64)
+        self.lib.ref_set_state(self.h, _f64(pos) if pos is not None else None, _f64(mom) if mom is not None else None)
+
+    def set_thermostat(self, eta, p_eta):
+        eta, p_eta = np.ascontiguousarray(eta, np.float64), np.ascontiguousarray(p_eta, np.float64)
+        self.lib.ref_set_thermostat(self.h, _f64(eta), _f64(p_eta))
+
+    def get_thermostat(self):
+        M = self.cfg.temperature_nh_len
+        eta, p_eta = np.zeros(M), np.zeros(M)
+        self.lib.ref_get_thermostat(self.h, _f64(eta), _f64(p_eta))
+        return eta, p_eta
+
+    def mol(self):
+        m = np.zeros(self.n, np.int32)
+        self.lib.ref_get_mol(self.h, _i32(m))
+        return m
+
+    def build_cells(self):
+        nc = np.zeros(3, np.int32)
+        self.lib.ref_build_cells(self.h, _i32(nc), None, None, None, None)
+        tot = int(np.prod(nc))
+        ixyz = np.zeros((self.n, 3), np.int32)
+        first, count, lst = np.zeros(tot, np.int32), np.zeros(tot, np.int32), np.zeros(self.n, np.int32)
+        err = self.lib.ref_build_cells(self.h, _i32(nc), _i32(ixyz), _i32(first), _i32(count), _i32(lst))
+        return dict(err=err, nc=nc, ixyz=ixyz, first=first, count=count, list=lst)
+
+    def build_neighbors(self):
+        self.build_cells()
+        cap = C.c_int32(0)
+        mx = self.lib.ref_build_neighbors(self.h, 0, None, None, C.byref(cap))
+        cnt = np.zeros(self.n, np.int32)
+        lst = np.zeros((self.n, max(mx, 1)), np.int32)
+        self.lib.ref_build_neighbors(self.h, max(mx, 1), _i32(cnt), _i32(lst), C.byref(cap))
+        return cnt, lst
+
+    def compute_forces(self):
+        F, Fpi, U = np.zeros((self.n, 3)), np.zeros((self.n, 3)), np.zeros(self.n, np.float32)
+        rc = self.lib.ref_compute_forces(self.h, _f64(F), _f64(Fpi), _f32(U))
+        assert rc == 0
+        return F, U
+
+    def nh_propagate(self):
+        mom = np.zeros((self.n, 3))
+        self.lib.ref_nh_propagate(self.h, _f64(mom))
+        return mom
+
+
+def cmt_exp(x, ref=False):
+    return _ref_lib().ref_cmt_exp(x) if ref else _oracle_lib().orc_cmt_exp(x)
+
+
+def cmt_sinhxoverx(x, ref=False):
+    return _ref_lib().ref_cmt_sinhxoverx(x) if ref else _oracle_lib().orc_cmt_sinhxoverx(x)

@@ -1,0 +1,366 @@
+// mtmd_device.h - device arithmetic that must match the reference CPU build.
+//
+// Each helper keeps the operation order and precision of the reference
+// function named beside it (paths relative to the MolTwister tree;
+// FCF = Cmd/MolTwisterCmdMolDyn/ForceFields/FunctorCalcForce.cpp,
+// V3 = Utilities/3DVector.cpp). Contraction into FMA is disabled inside every
+// helper, so the helpers stay exact even when inlined into kernels of a
+// translation unit built with contraction enabled.
+#pragma once
+#include "mtmd_internal.h"
+#include <float.h>
+#include <limits.h>
+
+#define MT_EXACT_FP _Pragma("clang fp contract(off)")
+
+struct mt_d3 { double x, y, z; };
+
+__device__ __forceinline__ mt_d3 mt_sub(mt_d3 a, mt_d3 b) { MT_EXACT_FP return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ mt_d3 mt_add(mt_d3 a, mt_d3 b) { MT_EXACT_FP return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ mt_d3 mt_mul(mt_d3 a, double s) { MT_EXACT_FP return {a.x * s, a.y * s, a.z * s}; }
+// C3DVector::operator* (V3:298-301) and norm2/norm (V3:34-42)
+__device__ __forceinline__ double mt_dot(mt_d3 a, mt_d3 b)
+{
+    MT_EXACT_FP
+    return (a.x * b.x) + (a.y * b.y) + (a.z * b.z);
+}
+__device__ __forceinline__ double mt_norm(mt_d3 a)
+{
+    MT_EXACT_FP
+    return sqrt(a.x * a.x + a.y * a.y + a.z * a.z);
+}
+// C3DVector::cross (V3:243-252)
+__device__ __forceinline__ mt_d3 mt_cross(mt_d3 a, mt_d3 b)
+{
+    MT_EXACT_FP
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+
+// int(float) / int(double) as x86-64 cvtt* instructions give them: truncation,
+// INT_MIN for NaN and out-of-range values (the reference is a CPU build)
+__device__ __forceinline__ int mt_f2i(float v)
+{
+    if (!(v >= -2147483648.0f && v < 2147483648.0f)) return INT_MIN;
+    return (int)v;
+}
+__device__ __forceinline__ int mt_d2i(double v)
+{
+    if (!(v > -2147483649.0 && v < 2147483648.0)) return INT_MIN;
+    return (int)v;
+}
+
+// C3DVector::moveToSameSideOfPBCAsThis (V3:133-165)
+__device__ __forceinline__ mt_d3 mt_move_same_side(mt_d3 self, mt_d3 v, const double* w)
+{
+    MT_EXACT_FP
+    double dX = self.x - v.x, dY = self.y - v.y, dZ = self.z - v.z;
+    dX = (dX < 0.0) ? -dX : dX;
+    dY = (dY < 0.0) ? -dY : dY;
+    dZ = (dZ < 0.0) ? -dZ : dZ;
+    if (dX > (w[0] / 2.0)) { if (self.x > v.x) v.x += w[0]; else v.x -= w[0]; }
+    if (dY > (w[1] / 2.0)) { if (self.y > v.y) v.y += w[1]; else v.y -= w[1]; }
+    if (dZ > (w[2] / 2.0)) { if (self.z > v.z) v.z += w[2]; else v.z -= w[2]; }
+    return v;
+}
+
+// C3DVector::distToAcrossPBC2 (V3:68-90)
+__device__ __forceinline__ double mt_dist2_pbc(mt_d3 self, mt_d3 v, const double* w)
+{
+    MT_EXACT_FP
+    double dX = self.x - v.x, dY = self.y - v.y, dZ = self.z - v.z;
+    dX = (dX < 0.0) ? -dX : dX;
+    dY = (dY < 0.0) ? -dY : dY;
+    dZ = (dZ < 0.0) ? -dZ : dZ;
+    if (dX > (w[0] / 2.0)) dX = w[0] - dX;
+    if (dY > (w[1] / 2.0)) dY = w[1] - dY;
+    if (dZ > (w[2] / 2.0)) dZ = w[2] - dZ;
+    return dX * dX + dY * dY + dZ * dZ;
+}
+
+// calcNonBondForceCoeffs12 / calcBondForceCoeffs12 (FCF:283-301)
+__device__ __forceinline__ mt_d3 mt_coeffs12(mt_d3 r1, mt_d3 r2)
+{
+    MT_EXACT_FP
+    mt_d3 r12 = mt_sub(r2, r1);
+    double R = mt_norm(r12);
+    double RInv = (R == 0.0) ? 1.0 / 1E-10 : 1.0 / R;
+    return mt_mul(r12, RInv);
+}
+
+// calcAngularForceCoeffs13 (FCF:303-324)
+__device__ inline mt_d3 mt_ang_coeffs13(mt_d3 r1, mt_d3 r2, mt_d3 r3)
+{
+    MT_EXACT_FP
+    mt_d3 r12 = mt_sub(r1, r2);
+    double R = mt_norm(r12);
+    double RInv = (R == 0.0) ? 1.0 / 1E-10 : 1.0 / R;
+    mt_d3 K = mt_mul(r12, RInv);
+    mt_d3 r32 = mt_sub(r3, r2);
+    double A = K.x * r32.x + K.y * r32.y + K.z * r32.z;
+    double B = r32.x * r32.x + r32.y * r32.y + r32.z * r32.z;
+    double sqrtB = sqrt(B);
+    double sqrtB3 = (B == 0.0) ? sqrt(1E-30) : sqrtB * sqrtB * sqrtB;
+    double dW_dx = (K.x * B - r32.x * A) / sqrtB3;
+    double dW_dy = (K.y * B - r32.y * A) / sqrtB3;
+    double dW_dz = (K.z * B - r32.z * A) / sqrtB3;
+    double W = (sqrtB == 0.0) ? A / 10E-3 : A / sqrtB;
+    double dDiv = sqrt(1.0 - W * W);
+    double dTheta_dW = (dDiv == 0.0) ? -1.0 / 1E-10 : -1.0 / dDiv;
+    return {dTheta_dW * dW_dx, dTheta_dW * dW_dy, dTheta_dW * dW_dz};
+}
+
+// calcAngularForceCoeffs2 (FCF:326-346)
+__device__ inline mt_d3 mt_ang_coeffs2(mt_d3 r1, mt_d3 r2, mt_d3 r3)
+{
+    MT_EXACT_FP
+    mt_d3 r12 = mt_sub(r1, r2);
+    mt_d3 r32 = mt_sub(r3, r2);
+    double A = r12.x * r12.x + r12.y * r12.y + r12.z * r12.z;
+    double B = r32.x * r32.x + r32.y * r32.y + r32.z * r32.z;
+    double C = r12.x * r32.x + r12.y * r32.y + r12.z * r32.z;
+    double AB = A * B;
+    double sqrtAB = sqrt(AB);
+    double sqrtAB3 = (AB == 0.0) ? sqrt(1E-30) : sqrtAB * sqrtAB * sqrtAB;
+    double dW_dx = ((2.0 * r2.x - r1.x - r3.x) * AB + (r32.x * A + r12.x * B) * C) / sqrtAB3;
+    double dW_dy = ((2.0 * r2.y - r1.y - r3.y) * AB + (r32.y * A + r12.y * B) * C) / sqrtAB3;
+    double dW_dz = ((2.0 * r2.z - r1.z - r3.z) * AB + (r32.z * A + r12.z * B) * C) / sqrtAB3;
+    double W = (sqrtAB == 0.0) ? A / 10E-3 : C / sqrtAB;
+    double dDiv = sqrt(1.0 - W * W);
+    double dTheta_dW = (dDiv == 0.0) ? -1.0 / 1E-10 : -1.0 / dDiv;
+    return {dTheta_dW * dW_dx, dTheta_dW * dW_dy, dTheta_dW * dW_dz};
+}
+
+// calcDihedralForceCoeffs14 before its final sign flip (FCF:348-377)
+__device__ inline mt_d3 mt_dih_coeffs14(mt_d3 r1, mt_d3 r2, mt_d3 r3, mt_d3 r4)
+{
+    MT_EXACT_FP
+    mt_d3 r21 = mt_sub(r1, r2);
+    mt_d3 r23 = mt_sub(r3, r2);
+    mt_d3 r32 = mt_mul(r23, -1.0);
+    mt_d3 r34 = mt_sub(r4, r3);
+    mt_d3 n1 = mt_cross(r23, r21);
+    double n1Norm = mt_norm(n1);
+    double n1NormInv = (n1Norm == 0.0) ? 1.0 / 1E-10 : 1.0 / n1Norm;
+    mt_d3 K = mt_mul(n1, n1NormInv);
+    double a = r32.y * r34.z - r32.z * r34.y;
+    double b = r32.z * r34.x - r32.x * r34.z;
+    double c = r32.x * r34.y - r32.y * r34.x;
+    double A = K.x * a + K.y * b + K.z * c;
+    double B = a * a + b * b + c * c;
+    double sqrtBInv = (B == 0.0) ? 1.0 / 1E-20 : 1.0 / sqrt(B);
+    double sqrtBInv3 = sqrtBInv * sqrtBInv * sqrtBInv;
+    double W = A * sqrtBInv;
+    double dW_dx = ((K.y * r32.z - K.z * r32.y) * B - (r32.z * (r32.z * r34.x - r32.x * r34.z) - r32.y * (r32.x * r34.y - r32.y * r34.x)) * A) * sqrtBInv3;
+    double dW_dy = ((K.z * r32.x - K.x * r32.z) * B - (r32.x * (r32.x * r34.y - r32.y * r34.x) - r32.z * (r32.y * r34.z - r32.z * r34.y)) * A) * sqrtBInv3;
+    double dW_dz = ((K.x * r32.y - K.y * r32.x) * B - (r32.y * (r32.y * r34.z - r32.z * r34.y) - r32.x * (r32.z * r34.x - r32.x * r34.z)) * A) * sqrtBInv3;
+    double div = sqrt(1.0 - W * W);
+    double dTheta_dW = (div == 0.0) ? 1.0 / 1E-10 : 1.0 / div;
+    return {dTheta_dW * dW_dx, dTheta_dW * dW_dy, dTheta_dW * dW_dz};
+}
+
+// calcDihedralForceCoeffs23 before its final sign flip (FCF:379-424)
+__device__ inline mt_d3 mt_dih_coeffs23(mt_d3 r1, mt_d3 r2, mt_d3 r3, mt_d3 r4)
+{
+    MT_EXACT_FP
+    mt_d3 r21 = mt_sub(r1, r2);
+    mt_d3 r23 = mt_sub(r3, r2);
+    mt_d3 r32 = mt_mul(r23, -1.0);
+    mt_d3 r34 = mt_sub(r4, r3);
+    double K1 = r32.z * r34.x - r32.x * r34.z;
+    double K2 = r34.z - r32.z;
+    double K3 = r23.z * r21.x - r23.x * r21.z;
+    double K4 = r32.x * r34.y - r32.y * r34.x;
+    double K5 = r32.y - r34.y;
+    double K6 = r23.x * r21.y - r23.y * r21.x;
+    double L1 = r32.y * r34.z - r32.z * r34.y;
+    double L2 = -K2;
+    double L3 = r23.y * r21.z - r23.z * r21.y;
+    double L4 = K4;
+    double L5 = r34.x - r32.x;
+    double L6 = K6;
+    double M1 = L1, M2 = -K5, M3 = L3, M4 = K1, M5 = -L5, M6 = K3;
+    double A = L3 * L3 + K3 * K3 + K6 * K6;
+    double B = L1 * L1 + K1 * K1 + K4 * K4;
+    double C = L3 * L1 + K3 * K1 + K6 * K4;
+    double AB = A * B;
+    double sqrtABInv = (AB == 0.0) ? 1.0 / 1E-20 : 1.0 / sqrt(AB);
+    double sqrtABInv3 = sqrtABInv * sqrtABInv * sqrtABInv;
+    double W = C * sqrtABInv;
+    double dC_dx = K2 * K3 + r21.y * K4 + K5 * K6 - r21.z * K1;
+    double dC_dy = L2 * L3 - r21.x * L4 + L5 * L6 + r21.z * L1;
+    double dC_dz = M2 * M3 + r21.x * M4 + M5 * M6 - r21.y * M1;
+    double dW_dx = (dC_dx * AB - C * ((r21.y * K6 - r21.z * K3) * B + (K1 * K2 + K4 * K5) * A)) * sqrtABInv3;
+    double dW_dy = (dC_dy * AB - C * ((r21.z * L3 - r21.x * L6) * B + (L1 * L2 + L4 * L5) * A)) * sqrtABInv3;
+    double dW_dz = (dC_dz * AB - C * ((r21.x * M6 - r21.y * M3) * B + (M1 * M2 + M4 * M5) * A)) * sqrtABInv3;
+    double div = sqrt(1.0 - W * W);
+    double dTheta_dW = (div == 0.0) ? 1.0 / 1E-10 : 1.0 / div;
+    return {dTheta_dW * dW_dx, dTheta_dW * dW_dy, dTheta_dW * dW_dz};
+}
+
+// calcAngle (FCF:432-440)
+__device__ __forceinline__ float mt_calc_angle(mt_d3 rk, mt_d3 ri, mt_d3 rj)
+{
+    MT_EXACT_FP
+    mt_d3 rik = mt_sub(rk, ri), rij = mt_sub(rj, ri);
+    float dot = (float)mt_dot(rik, rij);
+    float RR = (float)mt_norm(rik) * (float)mt_norm(rij);
+    float cs = (RR != 0.0f) ? dot / RR : 1.0f;
+    return (float)acos((double)cs);
+}
+
+// calcDihedral (FCF:442-455)
+__device__ __forceinline__ float mt_calc_dihedral(mt_d3 rk, mt_d3 ri, mt_d3 rj, mt_d3 rl)
+{
+    MT_EXACT_FP
+    mt_d3 rik = mt_sub(rk, ri), rij = mt_sub(rj, ri), rjl = mt_sub(rl, rj);
+    mt_d3 n1 = mt_cross(rik, rij), n2 = mt_cross(rij, rjl);
+    float dot = (float)mt_dot(n1, n2);
+    float nn = (float)mt_norm(n1) * (float)mt_norm(n2);
+    float cs = (nn != 0.0f) ? dot / nn : 1.0f;
+    if (cs < -1.0f) cs = -1.0f;
+    if (cs > 1.0f) cs = 1.0f;
+    return (float)acos((double)cs);
+}
+
+// interpolateLinear (FCF:457-465)
+__device__ __forceinline__ float mt_interp(float r, float x1, float y1, float x2, float y2)
+{
+    MT_EXACT_FP
+    float denom = x1 - x2;
+    if (denom == 0.0f) denom = 3.0f * FLT_MIN;
+    float a = (y1 - y2) / denom;
+    float b = y1 - a * x1;
+    return a * r + b;
+}
+
+// calcIndexOfLowerPointOfInterpolationLine (FCF:426-430) + interpolation of
+// one profile (FCF:481-504, 547-567)
+__device__ __forceinline__ void mt_table_eval(const float4* tab, float x, bool want_e, float& fsum, float& esum)
+{
+    MT_EXACT_FP
+    float x0 = tab[0].x, xl = tab[MT_NPTS - 1].x;
+    float delta = (xl - x0) / (float)(MT_NPTS - 1);
+    int idx = mt_f2i((x - x0) / delta);
+    if (idx >= 0 && idx < MT_NPTS - 1) {
+        float4 p1 = tab[idx], p2 = tab[idx + 1];
+        fsum += mt_interp(x, p1.x, p1.y, p2.x, p2.y);
+        if (want_e) esum += mt_interp(x, p1.x, p1.z, p2.x, p2.z);
+    }
+}
+
+// calcForceCoulombOn_r_k (FCF:514-535)
+__device__ __forceinline__ mt_d3 mt_coulomb_pair(mt_d3 rk, mt_d3 ri, float qk, float qi, int k, int i, float& U)
+{
+    MT_EXACT_FP
+    mt_d3 r = mt_sub(rk, ri);
+    float R = (float)mt_norm(r);
+    double R3 = (double)(R * R * R);
+    double K = (double)(MT_COULOMB_K * qk * qi);
+    if (i > k && R != 0.0f) U += (float)K / R;
+    if (R3 != 0.0) return mt_mul(r, K / R3);
+    return {0.0, 0.0, 0.0};
+}
+
+// scaleForcesAndPotentials (FCF:95-157) over atom k's bonded lists
+__device__ inline void mt_scale_pair(const MtDev& d, const MtParams& p, mt_d3& f, float& u, int i, int molk, int k)
+{
+    MT_EXACT_FP
+    const MtBonded& b = d.bl;
+    bool hasB = false, hasA = false, hasD = false;
+    if (b.has_bonds) {
+        int f0 = b.b_first[k], nn = b.b_num[k];
+        for (int j = 0; j < nn; j++)
+            if (i == b.b_j[f0 + j]) { f = mt_mul(f, (double)p.scale12); u *= p.scale12; hasB = true; break; }
+    }
+    if (!hasB && b.has_angles) {
+        int f0 = b.a_first[k], nn = b.a_num[k];
+        for (int j = 0; j < nn; j++)
+            if (!b.a_center[f0 + j] && i == b.a_j[f0 + j]) {
+                f = mt_mul(f, (double)p.scale13); u *= p.scale13; hasA = true; break;
+            }
+    }
+    if (!hasB && !hasA && b.has_dihedrals) {
+        int f0 = b.d_first[k], nn = b.d_num[k];
+        for (int j = 0; j < nn; j++)
+            if (!b.d_center[f0 + j] && i == b.d_l[f0 + j]) {
+                f = mt_mul(f, (double)p.scale14); u *= p.scale14; hasD = true; break;
+            }
+    }
+    if (!hasB && !hasA && !hasD) {
+        int moli = d.mol[i];
+        if (moli >= 0 && molk >= 0 && moli == molk) { f = mt_mul(f, (double)p.scale1N); u *= p.scale1N; }
+    }
+}
+
+// bonds, angles and dihedrals acting on atom k (FCF:172-232, 537-709).
+// `orig` is k's original index (its bonded rows); `slot_of` maps an original
+// index to the array slot holding that atom (identity when atoms are unsorted).
+__device__ inline void mt_exact_bonded(const MtDev& d, const MtParams& p, int k, mt_d3& F, float& U,
+                                       int orig = -1, const int* slot_of = nullptr)
+{
+    MT_EXACT_FP
+    const MtBonded& b = d.bl;
+    if (orig < 0) orig = k;
+    double4 rk4 = d.pos[k];
+    mt_d3 rk = {rk4.x, rk4.y, rk4.z};
+#define MT_POS(o) ([&]() { double4 q = d.pos[slot_of ? slot_of[(o)] : (o)]; return mt_d3{q.x, q.y, q.z}; }())
+    if (b.has_bonds) {
+        int f0 = b.b_first[orig], nn = b.b_num[orig];
+        for (int j = 0; j < nn; j++) {
+            int ib = b.b_j[f0 + j];
+            const float4* tab = d.btab + (size_t)b.b_type[f0 + j] * MT_NPTS;
+            mt_d3 ri = mt_move_same_side(rk, MT_POS(ib), p.wbox);
+            float r = (float)mt_norm(mt_sub(rk, ri));
+            float mdU = 0.0f, e = 0.0f;
+            mt_table_eval(tab, r, ib > orig, mdU, e);
+            if (ib > orig) U += e;
+            F = mt_add(F, mt_mul(mt_coeffs12(ri, rk), (double)mdU));
+        }
+    }
+    if (b.has_angles) {
+        int f0 = b.a_first[orig], nn = b.a_num[orig];
+        for (int j = 0; j < nn; j++) {
+            int g = f0 + j;
+            const float4* tab = d.atab + (size_t)b.a_type[g] * MT_NPTS;
+            mt_d3 ri = mt_move_same_side(rk, MT_POS(b.a_i[g]), p.wbox);
+            mt_d3 rj = mt_move_same_side(rk, MT_POS(b.a_j[g]), p.wbox);
+            float mdU = 0.0f, e = 0.0f;
+            mt_d3 f;
+            if (b.a_center[g]) {
+                float theta = mt_calc_angle(ri, rk, rj);
+                mt_table_eval(tab, theta, true, mdU, e);
+                U += e;
+                f = mt_mul(mt_ang_coeffs2(rj, rk, ri), (double)mdU);
+            } else {
+                float theta = mt_calc_angle(rk, ri, rj);
+                mt_table_eval(tab, theta, false, mdU, e);
+                f = mt_mul(mt_ang_coeffs13(rj, ri, rk), (double)mdU);
+            }
+            F = mt_add(F, f);
+        }
+    }
+    if (b.has_dihedrals) {
+        int f0 = b.d_first[orig], nn = b.d_num[orig];
+        for (int j = 0; j < nn; j++) {
+            int g = f0 + j;
+            const float4* tab = d.dtab + (size_t)b.d_type[g] * MT_NPTS;
+            mt_d3 ri = mt_move_same_side(rk, MT_POS(b.d_i[g]), p.wbox);
+            mt_d3 rj = mt_move_same_side(rk, MT_POS(b.d_j[g]), p.wbox);
+            mt_d3 rl = mt_move_same_side(rk, MT_POS(b.d_l[g]), p.wbox);
+            float mdU = 0.0f, e = 0.0f;
+            mt_d3 f;
+            if (b.d_center[g]) {
+                float theta = mt_calc_dihedral(ri, rk, rj, rl);
+                mt_table_eval(tab, theta, true, mdU, e);
+                U += 0.5f * e;
+                f = mt_mul(mt_mul(mt_dih_coeffs23(rl, rj, rk, ri), -1.0), (double)mdU);
+            } else {
+                float theta = mt_calc_dihedral(rk, ri, rj, rl);
+                mt_table_eval(tab, theta, false, mdU, e);
+                f = mt_mul(mt_mul(mt_dih_coeffs14(rl, rj, ri, rk), -1.0), (double)mdU);
+            }
+            F = mt_add(F, f);
+        }
+    }
+#undef MT_POS
+}

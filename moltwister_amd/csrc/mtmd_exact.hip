@@ -1,0 +1,524 @@
+// mtmd_exact.hip - reference-exact kernels for gfx950 and the shared
+// integrator/thermostat/reduction kernels.
+//
+// Built with -ffp-contract=off: every expression below keeps the precision and
+// operation order of the reference CPU build (x86-64 SSE, no FMA), so cell
+// lists, neighbour lists and per-atom forces come out bit-identical.
+// Reference paths are relative to the MolTwister tree:
+//   FCF = Cmd/MolTwisterCmdMolDyn/ForceFields/FunctorCalcForce.cpp
+//   FGC = Cmd/MolTwisterCmdMolDyn/ForceFields/FunctorGenCellList.cpp
+//   FGN = Cmd/MolTwisterCmdMolDyn/ForceFields/FunctorGenNeighList.cpp
+//   V3  = Utilities/3DVector.cpp
+#include "mtmd_internal.h"
+#include "mtmd_device.h"
+#include <float.h>
+
+#define TPB 256
+
+// ---------------------------------------------------------------------------
+// small utilities
+// ---------------------------------------------------------------------------
+__global__ void k_fill_i32(int* p, int v, int n)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+static inline int nblk(int n, int b) { return (n + b - 1) / b; }
+
+int mt_reduce_blocks(int n) { int b = nblk(n, TPB); return b < 1 ? 1 : (b > 1024 ? 1024 : b); }
+
+// ---------------------------------------------------------------------------
+// cell list: CFunctorGenCellList::operator() (FGC:49-73) + kernelAssembleList
+// (FGC:75-142) as count / scan / scatter / per-cell stable order
+// ---------------------------------------------------------------------------
+__global__ void k_exact_cell_keys(const double4* __restrict__ pos, int n, int Nx, int Ny, int Nz,
+                                  float wx, float wy, float wz, float lx, float ly, float lz,
+                                  int* __restrict__ key, int* __restrict__ cxyz, int* __restrict__ count)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    double4 r = pos[k];
+    int ix = mt_d2i(floor((r.x - (double)lx) * (double)Nx / (double)wx));
+    int iy = mt_d2i(floor((r.y - (double)ly) * (double)Ny / (double)wy));
+    int iz = mt_d2i(floor((r.z - (double)lz) * (double)Nz / (double)wz));
+    ix = (ix < 0) ? 0 : ((ix >= Nx) ? Nx - 1 : ix);
+    iy = (iy < 0) ? 0 : ((iy >= Ny) ? Ny - 1 : iy);
+    iz = (iz < 0) ? 0 : ((iz >= Nz) ? Nz - 1 : iz);
+    int c = ix + Nx * (iy + iz * Ny);
+    key[k] = c;
+    if (cxyz) { cxyz[3 * k] = ix; cxyz[3 * k + 1] = iy; cxyz[3 * k + 2] = iz; }
+    atomicAdd(&count[c], 1);
+}
+
+// single-workgroup exclusive scan over the cells: first = -1 for empty cells
+// (the reference assigns starts only to non-empty cells, FGC:93-103)
+__global__ void __launch_bounds__(1024) k_scan_cells(const int* __restrict__ count, int* __restrict__ first,
+                                                     int* __restrict__ fill, int ncell)
+{
+    __shared__ int sums[1024];
+    int t = threadIdx.x;
+    int chunk = (ncell + 1023) / 1024;
+    int b = t * chunk, e = min(b + chunk, ncell);
+    int s = 0;
+    for (int c = b; c < e; c++) s += count[c];
+    sums[t] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        int v = (t >= off) ? sums[t - off] : 0;
+        __syncthreads();
+        sums[t] += v;
+        __syncthreads();
+    }
+    int run = sums[t] - s;
+    for (int c = b; c < e; c++) {
+        int cnt = count[c];
+        first[c] = cnt ? run : -1;
+        fill[c] = 0;
+        run += cnt;
+    }
+}
+
+__global__ void k_cell_scatter(const int* __restrict__ key, int n, const int* __restrict__ first,
+                               int* __restrict__ fill, int* __restrict__ list)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    int c = key[k];
+    int slot = atomicAdd(&fill[c], 1);
+    list[first[c] + slot] = k;
+}
+
+// ascending atom index inside each cell (the reference fills cells in atom order)
+__global__ void k_cell_sort(const int* __restrict__ first, const int* __restrict__ count,
+                            int* __restrict__ list, int ncell)
+{
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    int cnt = count[c];
+    if (cnt < 2) return;
+    int* a = list + first[c];
+    for (int i = 1; i < cnt; i++) {
+        int v = a[i], j = i - 1;
+        while (j >= 0 && a[j] > v) { a[j + 1] = a[j]; j--; }
+        a[j + 1] = v;
+    }
+}
+
+void mt_exact_cells(const MtDev& d, const MtParams& p, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fill_i32, dim3(nblk(d.ncells, TPB)), dim3(TPB), 0, s, d.cell_count, 0, d.ncells);
+    hipLaunchKernelGGL(k_exact_cell_keys, dim3(nblk(d.n, TPB)), dim3(TPB), 0, s, d.pos, d.n, d.nc[0], d.nc[1],
+                       d.nc[2], p.Lf[0], p.Lf[1], p.Lf[2], p.lowf[0], p.lowf[1], p.lowf[2], d.cell_key,
+                       d.cell_xyz, d.cell_count);
+    hipLaunchKernelGGL(k_scan_cells, dim3(1), dim3(1024), 0, s, d.cell_count, d.cell_first, d.cell_fill, d.ncells);
+    hipLaunchKernelGGL(k_cell_scatter, dim3(nblk(d.n, TPB)), dim3(TPB), 0, s, d.cell_key, d.n, d.cell_first,
+                       d.cell_fill, d.cell_list);
+    hipLaunchKernelGGL(k_cell_sort, dim3(nblk(d.ncells, TPB)), dim3(TPB), 0, s, d.cell_first, d.cell_count,
+                       d.cell_list, d.ncells);
+}
+
+// ---------------------------------------------------------------------------
+// neighbour list: CFunctorGenNeighList::operator() (FGN:72-151), same cell
+// traversal (ix outer, iz inner, wrapped with the reference's rule), same
+// predicate d2 < double(float rc^2), self skipped. Rows are column-major.
+// ---------------------------------------------------------------------------
+__global__ void k_exact_neigh(const double4* __restrict__ pos, int n, const int* __restrict__ cxyz,
+                              const int* __restrict__ cfirst, const int* __restrict__ ccount,
+                              const int* __restrict__ clist, int Nx, int Ny, int Nz,
+                              double wx, double wy, double wz, float rc2, int ncap,
+                              int* __restrict__ ncount, uint32_t* __restrict__ nl, int* __restrict__ nreq)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const double w[3] = {wx, wy, wz};
+    double4 rk4 = pos[k];
+    mt_d3 rk = {rk4.x, rk4.y, rk4.z};
+    int cx = cxyz[3 * k], cy = cxyz[3 * k + 1], cz = cxyz[3 * k + 2];
+    int cnt = 0;
+    for (int ix = cx - 1; ix <= cx + 1; ix++) {
+        int Ix = (ix >= 0) ? ((ix < Nx) ? ix : 0) : Nx - 1;
+        for (int iy = cy - 1; iy <= cy + 1; iy++) {
+            int Iy = (iy >= 0) ? ((iy < Ny) ? iy : 0) : Ny - 1;
+            for (int iz = cz - 1; iz <= cz + 1; iz++) {
+                int Iz = (iz >= 0) ? ((iz < Nz) ? iz : 0) : Nz - 1;
+                int c = Ix + Nx * (Iy + Iz * Ny);
+                int num = ccount[c], f0 = cfirst[c];
+                for (int e = 0; e < num; e++) {
+                    int a = clist[f0 + e];
+                    double4 ra4 = pos[a];
+                    mt_d3 ra = {ra4.x, ra4.y, ra4.z};
+                    double d2 = mt_dist2_pbc(rk, ra, w);
+                    if (d2 < (double)rc2) {
+                        if (a == k) continue;
+                        if (cnt < ncap) nl[(size_t)cnt * n + k] = (uint32_t)a;
+                        cnt++;
+                    }
+                }
+            }
+        }
+    }
+    ncount[k] = cnt;
+    if (cnt > ncap) atomicMax(nreq, cnt);
+}
+
+void mt_exact_neighbors(const MtDev& d, const MtParams& p, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_exact_neigh, dim3(nblk(d.n, TPB)), dim3(TPB), 0, s, d.pos, d.n, d.cell_xyz, d.cell_first,
+                       d.cell_count, d.cell_list, d.nc[0], d.nc[1], d.nc[2], p.wnl[0], p.wnl[1], p.wnl[2], p.rc2f,
+                       d.ncap, d.ncount, d.nlist, d.nreq);
+}
+
+// ---------------------------------------------------------------------------
+// force functor (FCF:159-281): bonded, neighbours, then (separate kernel)
+// all-pairs Coulomb continuing the same per-atom accumulation in index order
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(TPB) k_exact_force(MtDev d, MtParams p)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= d.n) return;
+    mt_d3 F = {0.0, 0.0, 0.0};
+    float U = 0.0f;
+    if (!d.mobile[k]) { d.frc[k] = make_double4(0.0, 0.0, 0.0, 0.0); d.epot[k] = 0.0f; return; }
+    mt_exact_bonded(d, p, k, F, U);
+    int molk = d.mol[k];
+    double4 rk4 = d.pos[k];
+    mt_d3 rk = {rk4.x, rk4.y, rk4.z};
+    int cnt = d.ncount[k];
+    int T = d.T, row = d.type[k];
+    for (int j = 0; j < cnt; j++) {
+        int ia = (int)d.nlist[(size_t)j * d.n + k];
+        double4 ri4 = d.pos[ia];
+        mt_d3 ri = mt_move_same_side(rk, {ri4.x, ri4.y, ri4.z}, p.wbox);
+        mt_d3 f;
+        float u = 0.0f;
+        float r = (float)mt_norm(mt_sub(rk, ri));
+        if (r < 1E-3f) {
+            f = {0.0, 0.0, 0.0};
+        } else {
+            int col = d.type[ia];
+            int nff = d.nbcnt[row * T + col];
+            float mdU = 0.0f;
+            for (int ff = 0; ff < nff; ff++) {
+                const float4* tab = d.nbtab + ((size_t)(row * T + col) * MT_MAXFF + ff) * MT_NPTS;
+                mt_table_eval(tab, r, ia > k, mdU, u);
+            }
+            f = mt_mul(mt_coeffs12(ri, rk), (double)mdU);
+        }
+        mt_scale_pair(d, p, f, u, ia, molk, k);
+        F = mt_add(F, f);
+        U += u;
+    }
+    if (p.coulomb_on && !p.coulomb_allpairs) {
+        for (int j = 0; j < cnt; j++) {
+            int ia = (int)d.nlist[(size_t)j * d.n + k];
+            double4 ri4 = d.pos[ia];
+            mt_d3 ri = mt_move_same_side(rk, {ri4.x, ri4.y, ri4.z}, p.wbox);
+            float u = 0.0f;
+            mt_d3 f = mt_coulomb_pair(rk, ri, (float)rk4.w, (float)ri4.w, k, ia, u);
+            mt_scale_pair(d, p, f, u, ia, molk, k);
+            F = mt_add(F, f);
+            U += u;
+        }
+    }
+    d.frc[k] = make_double4(F.x, F.y, F.z, 0.0);
+    d.epot[k] = U;
+}
+
+// all-pairs minimum-image Coulomb (FCF:258-278, 514-535): each workgroup owns
+// TPB atoms k and walks all atoms i in index order through LDS tiles
+__global__ void __launch_bounds__(TPB) k_exact_coulomb_allpairs(MtDev d, MtParams p)
+{
+    __shared__ double sx[TPB], sy[TPB], sz[TPB];
+    __shared__ float sq[TPB];
+    __shared__ int smol[TPB];
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    bool active = (k < d.n) && d.mobile[k];
+    mt_d3 F = {0.0, 0.0, 0.0}, rk = {0.0, 0.0, 0.0};
+    float U = 0.0f, qk = 0.0f;
+    int molk = -1, xk = 0;
+    if (active) {
+        double4 f4 = d.frc[k];
+        F = {f4.x, f4.y, f4.z};
+        U = d.epot[k];
+        double4 r4 = d.pos[k];
+        rk = {r4.x, r4.y, r4.z};
+        qk = (float)r4.w;
+        molk = d.mol[k];
+        xk = d.xmol[k];
+    }
+    for (int base = 0; base < d.n; base += TPB) {
+        int i = base + threadIdx.x;
+        __syncthreads();
+        if (i < d.n) {
+            double4 r4 = d.pos[i];
+            sx[threadIdx.x] = r4.x; sy[threadIdx.x] = r4.y; sz[threadIdx.x] = r4.z;
+            sq[threadIdx.x] = (float)r4.w;
+            smol[threadIdx.x] = d.mol[i];
+        }
+        __syncthreads();
+        if (!active || qk == 0.0f) continue;   // every term is +-0: the sum is unchanged
+        int lim = min(TPB, d.n - base);
+        for (int t = 0; t < lim; t++) {
+            float qi = sq[t];
+            int ia = base + t;
+            if (qi == 0.0f) continue;
+            mt_d3 ri = mt_move_same_side(rk, {sx[t], sy[t], sz[t]}, p.wbox);
+            float u = 0.0f;
+            mt_d3 f = mt_coulomb_pair(rk, ri, qk, qi, k, ia, u);
+            if (xk || (smol[t] == molk && molk >= 0)) mt_scale_pair(d, p, f, u, ia, molk, k);
+            F = mt_add(F, f);
+            U += u;
+        }
+    }
+    if (active) {
+        d.frc[k] = make_double4(F.x, F.y, F.z, 0.0);
+        d.epot[k] = U;
+    }
+}
+
+void mt_exact_forces(const MtDev& d, const MtParams& p, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_exact_force, dim3(nblk(d.n, TPB)), dim3(TPB), 0, s, d, p);
+    if (p.coulomb_on && p.coulomb_allpairs)
+        hipLaunchKernelGGL(k_exact_coulomb_allpairs, dim3(nblk(d.n, TPB)), dim3(TPB), 0, s, d, p);
+}
+
+// ---------------------------------------------------------------------------
+// integrator: CVelVerlet::propagator NVE/NVT branch (VelVerlet.cpp:74-93),
+// cutForces (:96-122), pbcWrap (SimulationBox.cpp:135-166)
+// ---------------------------------------------------------------------------
+
+__device__ inline void block_sum_store(double v, double* out)
+{
+    __shared__ double sh[TPB];
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = TPB / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = sh[0];
+}
+
+// p *= S (thermostat), p += F dt/2, r += p dt/m; optional max |r - r_ref|^2
+__global__ void __launch_bounds__(TPB) k_kick_drift(double4* __restrict__ pos, double4* __restrict__ mom,
+                                                    const double4* __restrict__ frc, int n, const double* nh,
+                                                    int scale_slot, double dt, const double4* __restrict__ posref,
+                                                    unsigned int* flags)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    float d2 = 0.0f;
+    if (k < n) {
+        double S = (scale_slot >= 0) ? nh[scale_slot] : 1.0;
+        double4 p = mom[k], r = pos[k], F = frc[k];
+        double dt_2 = dt / 2.0;
+        if (scale_slot >= 0) { p.x *= S; p.y *= S; p.z *= S; }
+        p.x = p.x + F.x * dt_2; p.y = p.y + F.y * dt_2; p.z = p.z + F.z * dt_2;
+        double c = dt / p.w;
+        r.x = r.x + p.x * c; r.y = r.y + p.y * c; r.z = r.z + p.z * c;
+        mom[k] = p;
+        pos[k] = r;
+        if (posref) {
+            double4 q = posref[k];
+            double dx = r.x - q.x, dy = r.y - q.y, dz = r.z - q.z;
+            d2 = (float)(dx * dx + dy * dy + dz * dz);
+        }
+    }
+    if (posref) {
+        // wave-level max, one atomic per wavefront
+        for (int off = 32; off > 0; off >>= 1) d2 = fmaxf(d2, __shfl_xor(d2, off));
+        if ((threadIdx.x & 63) == 0 && d2 > 0.0f) atomicMax(&flags[0], __float_as_uint(d2));
+    }
+}
+
+void mt_kick_drift(const MtDev& d, const MtParams& p, int scale_slot, int track_disp, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_kick_drift, dim3(nblk(d.n, TPB)), dim3(TPB), 0, s, d.pos, d.mom, d.frc, d.n, d.nh,
+                       scale_slot, p.dt, track_disp ? d.posref : (const double4*)nullptr, d.flags);
+}
+
+// cap F per component (cutForces), p += F dt/2, optional block partials of p.p/m
+__global__ void __launch_bounds__(TPB) k_cap_kick(double4* __restrict__ mom, double4* __restrict__ frc, int n,
+                                                  double Fcut, double dt, double* red, int want_ke)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    double ke = 0.0;
+    if (k < n) {
+        double4 F = frc[k], pm = mom[k];
+        if (Fcut > 0) {
+            if (fabs(F.x) > Fcut) F.x = ((F.x >= 0.0) ? 1.0 : -1.0) * Fcut;
+            if (fabs(F.y) > Fcut) F.y = ((F.y >= 0.0) ? 1.0 : -1.0) * Fcut;
+            if (fabs(F.z) > Fcut) F.z = ((F.z >= 0.0) ? 1.0 : -1.0) * Fcut;
+            frc[k] = F;
+        }
+        double dt_2 = dt / 2.0;
+        pm.x = pm.x + F.x * dt_2; pm.y = pm.y + F.y * dt_2; pm.z = pm.z + F.z * dt_2;
+        mom[k] = pm;
+        ke = ((pm.x * pm.x) + (pm.y * pm.y) + (pm.z * pm.z)) / pm.w;
+    }
+    if (want_ke) block_sum_store(ke, &red[blockIdx.x]);
+}
+
+void mt_cap_kick(const MtDev& d, const MtParams& p, int want_ke, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_cap_kick, dim3(nblk(d.n, TPB)), dim3(TPB), 0, s, d.mom, d.frc, d.n, p.Fcut, p.dt, d.red,
+                       want_ke);
+}
+
+// CMt::exp (Cmd/MolTwisterCmdMolDyn/Integrators/Math.cpp:29-38)
+__device__ inline double mt_cmt_exp(double x) { return (x > 1.0) ? 2.71828 * x : exp(x); }
+
+// Nose-Hoover chain propagator (NHChain.cpp:54-113) with CFctT (MDLoop.cpp:29-60).
+// Every momentum scaling of the chain multiplies all momenta by the same
+// factor, so sum p.p/m after a scaling by c is c^2 times the sum before: the
+// chain runs on the scalar KE2 and the accumulated factor is applied once by
+// the next streaming kernel. nh layout: eta[M], p_eta[M], Q[M], scale.
+__global__ void k_nh_chain(double* nh, const double* red, int nblocks, int M, int nresp, double T, double dt,
+                           double g, double w0, double w1, double w2)
+{
+    __shared__ double sh[64];
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nblocks; b += 64) s += red[b];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    double KE2 = 0.0;
+    for (int t = 0; t < 64; t++) KE2 += sh[t];
+    double* eta = nh;
+    double* peta = nh + M;
+    const double* Q = nh + 2 * M;
+    double S = 1.0;
+    double beta = 1.0 / T;
+    const double w[3] = {w0, w1, w2};
+    if (M > 0 && peta[0] > 15.0) peta[0] = 10.0;
+    if (M > 0) {
+        for (int mu = 0; mu < 3; mu++) {
+            double Dt = w[mu] * dt / (double)nresp;
+            for (int i = 0; i < nresp; i++) {
+#define MT_G(j) ((j) == 0 ? (KE2 - (g / beta)) : ((peta[(j) - 1] * peta[(j) - 1] / Q[(j) - 1]) - (1.0 / beta)))
+                peta[M - 1] += (Dt / 4.0) * MT_G(M - 1);
+                for (int j = 0; j < M - 1; j++) {
+                    double c = mt_cmt_exp(-(Dt / 8.0) * (peta[j + 1] / Q[j + 1]));
+                    peta[j] *= c;
+                    peta[j] += (Dt / 4.0) * MT_G(j);
+                    c = mt_cmt_exp(-(Dt / 8.0) * (peta[j + 1] / Q[j + 1]));
+                    peta[j] *= c;
+                }
+                for (int j = 0; j < M; j++) eta[j] += (Dt / 2.0) * (peta[j] / Q[j]);
+                double c0 = mt_cmt_exp(-((Dt / 2.0) * (peta[0] / Q[0])));
+                S *= c0;
+                KE2 *= c0 * c0;
+                for (int j = M - 2; j >= 0; j--) {
+                    double c = mt_cmt_exp(-(Dt / 8.0) * (peta[j + 1] / Q[j + 1]));
+                    peta[j] *= c;
+                    peta[j] += (Dt / 4.0) * MT_G(j);
+                    c = mt_cmt_exp(-(Dt / 8.0) * (peta[j + 1] / Q[j + 1]));
+                    peta[j] *= c;
+                }
+                peta[M - 1] += (Dt / 4.0) * MT_G(M - 1);
+#undef MT_G
+            }
+        }
+    }
+    nh[3 * M] = S;
+}
+
+void mt_nh_chain(const MtDev& d, int M, int nresp, double T, double dt, double g, const double* w, int nblocks,
+                 hipStream_t s)
+{
+    hipLaunchKernelGGL(k_nh_chain, dim3(1), dim3(64), 0, s, d.nh, d.red, nblocks, M, nresp, T, dt, g, w[0], w[1],
+                       w[2]);
+}
+
+// p *= S; pbcWrap (SimulationBox.cpp:135-166); optional partials of p.p/m
+__device__ inline void mt_pbc_add(double& pos, double L, double Lm) { pos -= (double)mt_d2i((pos + L) / Lm) * Lm; }
+
+__global__ void __launch_bounds__(TPB) k_scale_wrap(double4* __restrict__ pos, double4* __restrict__ mom, int n,
+                                                    const double* nh, int scale_slot, double LX, double LY,
+                                                    double LZ, double* red, int want_ke)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    double ke = 0.0;
+    if (k < n) {
+        double4 r = pos[k];
+        double X2 = LX / 2.0, Y2 = LY / 2.0, Z2 = LZ / 2.0;
+        if (r.x >= X2) mt_pbc_add(r.x, X2, LX);
+        if (r.x < -X2) mt_pbc_add(r.x, -X2, LX);
+        if (r.y >= Y2) mt_pbc_add(r.y, Y2, LY);
+        if (r.y < -Y2) mt_pbc_add(r.y, -Y2, LY);
+        if (r.z >= Z2) mt_pbc_add(r.z, Z2, LZ);
+        if (r.z < -Z2) mt_pbc_add(r.z, -Z2, LZ);
+        pos[k] = r;
+        double4 pm = mom[k];
+        if (scale_slot >= 0) {
+            double S = nh[scale_slot];
+            pm.x *= S; pm.y *= S; pm.z *= S;
+            mom[k] = pm;
+        }
+        ke = ((pm.x * pm.x) + (pm.y * pm.y) + (pm.z * pm.z)) / pm.w;
+    }
+    if (want_ke) block_sum_store(ke, &red[blockIdx.x]);
+}
+
+void mt_scale_wrap(const MtDev& d, const MtParams& p, int scale_slot, int want_ke, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_scale_wrap, dim3(nblk(d.n, TPB)), dim3(TPB), 0, s, d.pos, d.mom, d.n, d.nh, scale_slot,
+                       p.L[0], p.L[1], p.L[2], d.red, want_ke);
+}
+
+__global__ void __launch_bounds__(TPB) k_reduce_ke(const double4* __restrict__ mom, int n, double* red)
+{
+    double ke = 0.0;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        double4 pm = mom[k];
+        ke += ((pm.x * pm.x) + (pm.y * pm.y) + (pm.z * pm.z)) / pm.w;
+    }
+    block_sum_store(ke, &red[blockIdx.x]);
+}
+
+void mt_reduce_ke(const MtDev& d, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_reduce_ke, dim3(nblk(d.n, TPB)), dim3(TPB), 0, s, d.mom, d.n, d.red);
+}
+
+// partials of sum p.p/m, sum F.r, sum U (SimulationBox.cpp:168-231) in red[3*b..3*b+2]
+__global__ void __launch_bounds__(TPB) k_reduce_thermo(const double4* __restrict__ pos,
+                                                       const double4* __restrict__ mom,
+                                                       const double4* __restrict__ frc,
+                                                       const float* __restrict__ epot, int n, double* red)
+{
+    double a = 0.0, b = 0.0, c = 0.0;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        double4 pm = mom[k], r = pos[k], F = frc[k];
+        a += ((pm.x * pm.x) + (pm.y * pm.y) + (pm.z * pm.z)) / pm.w;
+        b += (F.x * r.x) + (F.y * r.y) + (F.z * r.z);
+        c += (double)epot[k];
+    }
+    block_sum_store(a, &red[3 * blockIdx.x]);
+    __syncthreads();
+    block_sum_store(b, &red[3 * blockIdx.x + 1]);
+    __syncthreads();
+    block_sum_store(c, &red[3 * blockIdx.x + 2]);
+}
+
+void mt_reduce_thermo(const MtDev& d, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_reduce_thermo, dim3(mt_reduce_blocks(d.n)), dim3(TPB), 0, s, d.pos, d.mom, d.frc, d.epot,
+                       d.n, d.red);
+}
+
+// CSimulationBox::gradientDescentStep (SimulationBox.cpp:197-208): r += F*lr
+__global__ void k_gradient_step(double4* __restrict__ pos, const double4* __restrict__ frc, int n, double lr)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    double4 r = pos[k], F = frc[k];
+    r.x = r.x + F.x * lr; r.y = r.y + F.y * lr; r.z = r.z + F.z * lr;
+    pos[k] = r;
+}
+
+void mt_gradient_step(const MtDev& d, double lr, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_gradient_step, dim3(nblk(d.n, TPB)), dim3(TPB), 0, s, d.pos, d.frc, d.n, lr);
+}

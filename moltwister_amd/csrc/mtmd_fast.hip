@@ -1,0 +1,515 @@
+// mtmd_fast.hip - FP64 and MIXED engines for gfx950: spatially sorted atoms,
+// neighbour lists at cutoff + skin rebuilt on displacement, pair forces
+// gathered per atom over column-major lists (no atomics, deterministic).
+//
+// Per step the conditional rebuild kernels read flags[0] (the largest squared
+// displacement since the last rebuild, written by k_fast_kick_drift) and
+// return at once when it is below (skin/2)^2, so a step never waits on the host.
+//
+// MIXED keeps an FP32 copy of every position relative to the centre of the
+// atom's home cell (the cell it sat in at the last rebuild). Each neighbour
+// word carries the home-cell offset code of the pair, so the pair vector is
+//   d = xl_k - xl_i - offset * cell_size
+// in FP32 from two small numbers: no minimum-image branch and no loss of
+// precision to the box size. Pair forces and table interpolation run in FP32;
+// the per-atom force sum, integration and reductions stay in FP64.
+#include "mtmd_internal.h"
+#include "mtmd_device.h"
+
+#define TPB 256
+
+static inline int nblk(int n, int b) { return (n + b - 1) / b; }
+
+__device__ __forceinline__ bool mt_skip_rebuild(int cond, const unsigned int* flags, float thr)
+{
+    return cond && !(__uint_as_float(flags[0]) > thr);
+}
+
+__global__ void k_fast_fill(int* p, int v, int n, int cond, const unsigned int* flags, float thr)
+{
+    if (mt_skip_rebuild(cond, flags, thr)) return;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_fast_cell_keys(const double4* __restrict__ pos, int n, int Nx, int Ny, int Nz, double LX,
+                                 double LY, double LZ, int* __restrict__ key, int* __restrict__ count, int cond,
+                                 const unsigned int* flags, float thr)
+{
+    if (mt_skip_rebuild(cond, flags, thr)) return;
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    double4 r = pos[k];
+    int ix = mt_d2i(floor((r.x + LX / 2.0) * (double)Nx / LX));
+    int iy = mt_d2i(floor((r.y + LY / 2.0) * (double)Ny / LY));
+    int iz = mt_d2i(floor((r.z + LZ / 2.0) * (double)Nz / LZ));
+    ix = ix < 0 ? 0 : (ix >= Nx ? Nx - 1 : ix);
+    iy = iy < 0 ? 0 : (iy >= Ny ? Ny - 1 : iy);
+    iz = iz < 0 ? 0 : (iz >= Nz ? Nz - 1 : iz);
+    int c = ix + Nx * (iy + Ny * iz);
+    key[k] = c;
+    atomicAdd(&count[c], 1);
+}
+
+__global__ void __launch_bounds__(1024) k_fast_scan(const int* __restrict__ count, int* __restrict__ first,
+                                                    int* __restrict__ fill, int ncell, int cond,
+                                                    const unsigned int* flags, float thr)
+{
+    if (mt_skip_rebuild(cond, flags, thr)) return;
+    __shared__ int sums[1024];
+    int t = threadIdx.x;
+    int chunk = (ncell + 1023) / 1024;
+    int b = t * chunk, e = min(b + chunk, ncell);
+    int s = 0;
+    for (int c = b; c < e; c++) s += count[c];
+    sums[t] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        int v = (t >= off) ? sums[t - off] : 0;
+        __syncthreads();
+        sums[t] += v;
+        __syncthreads();
+    }
+    int run = sums[t] - s;
+    for (int c = b; c < e; c++) {
+        first[c] = run;
+        fill[c] = 0;
+        run += count[c];
+    }
+}
+
+__global__ void k_fast_scatter(const int* __restrict__ key, int n, const int* __restrict__ first,
+                               int* __restrict__ fill, int* __restrict__ list, int cond, const unsigned int* flags,
+                               float thr)
+{
+    if (mt_skip_rebuild(cond, flags, thr)) return;
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    int c = key[k];
+    list[first[c] + atomicAdd(&fill[c], 1)] = k;
+}
+
+// ascending slot order inside each cell, so lists are reproducible run to run
+__global__ void k_fast_cell_sort(const int* __restrict__ first, const int* __restrict__ count,
+                                 int* __restrict__ list, int ncell, int cond, const unsigned int* flags, float thr)
+{
+    if (mt_skip_rebuild(cond, flags, thr)) return;
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    int cnt = count[c];
+    int* a = list + first[c];
+    for (int i = 1; i < cnt; i++) {
+        int v = a[i], j = i - 1;
+        while (j >= 0 && a[j] > v) { a[j + 1] = a[j]; j--; }
+        a[j + 1] = v;
+    }
+}
+
+void mt_fast_cells(const MtDev& d, const MtParams& p, int cond, hipStream_t s)
+{
+    const float thr = p.half_skin2;
+    hipLaunchKernelGGL(k_fast_fill, dim3(nblk(d.ncells, TPB)), dim3(TPB), 0, s, d.cell_count, 0, d.ncells, cond,
+                       d.flags, thr);
+    hipLaunchKernelGGL(k_fast_cell_keys, dim3(nblk(d.n, TPB)), dim3(TPB), 0, s, d.pos, d.n, d.nc[0], d.nc[1],
+                       d.nc[2], p.L[0], p.L[1], p.L[2], d.cell_key, d.cell_count, cond, d.flags, thr);
+    hipLaunchKernelGGL(k_fast_scan, dim3(1), dim3(1024), 0, s, d.cell_count, d.cell_first, d.cell_fill, d.ncells,
+                       cond, d.flags, thr);
+    hipLaunchKernelGGL(k_fast_scatter, dim3(nblk(d.n, TPB)), dim3(TPB), 0, s, d.cell_key, d.n, d.cell_first,
+                       d.cell_fill, d.cell_list, cond, d.flags, thr);
+    hipLaunchKernelGGL(k_fast_cell_sort, dim3(nblk(d.ncells, TPB)), dim3(TPB), 0, s, d.cell_first, d.cell_count,
+                       d.cell_list, d.ncells, cond, d.flags, thr);
+}
+
+// gather every per-atom array into cell order: new slot s <- old slot list[s]
+__global__ void k_fast_sort_apply(MtDev d)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= d.n) return;
+    int o = d.cell_list[s];
+    d.pos2[s] = d.pos[o];
+    d.mom2[s] = d.mom[o];
+    d.frc2[s] = d.frc[o];
+    d.epot2[s] = d.epot[o];
+    d.type2[s] = d.type[o];
+    d.mol2[s] = d.mol[o];
+    d.mobile2[s] = d.mobile[o];
+    d.qf2[s] = d.qf[o];
+    int orig = d.perm[o];
+    d.perm2[s] = orig;
+    d.inv[orig] = s;
+}
+
+void mt_fast_sort_apply(const MtDev& d, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fast_sort_apply, dim3(nblk(d.n, TPB)), dim3(TPB), 0, s, d);
+}
+
+// scatter back to original order: slot perm[s] <- s
+__global__ void k_fast_unsort(MtDev d)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= d.n) return;
+    int o = d.perm[s];
+    d.pos2[o] = d.pos[s];
+    d.mom2[o] = d.mom[s];
+    d.frc2[o] = d.frc[s];
+    d.epot2[o] = d.epot[s];
+    d.type2[o] = d.type[s];
+    d.mol2[o] = d.mol[s];
+    d.mobile2[o] = d.mobile[s];
+    d.qf2[o] = d.qf[s];
+}
+
+void mt_fast_unsort(const MtDev& d, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fast_unsort, dim3(nblk(d.n, TPB)), dim3(TPB), 0, s, d);
+}
+
+__global__ void k_fast_identity(int* perm, int* inv, int n)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n) { perm[s] = s; inv[s] = s; }
+}
+
+void mt_fast_identity_perm(const MtDev& d, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fast_identity, dim3(nblk(d.n, TPB)), dim3(TPB), 0, s, d.perm, d.inv, d.n);
+}
+
+// home cell, reference position and local coordinates at a rebuild
+__global__ void k_fast_home(MtDev d, MtParams p, int cond)
+{
+    if (mt_skip_rebuild(cond, d.flags, p.half_skin2)) return;
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s == 0) d.flags[4] += 1;
+    if (s >= d.n) return;
+    int c = d.cell_key[s];
+    double4 r = d.pos[s];
+    d.home[s] = c;
+    d.posref[s] = r;
+    int ix = c % d.nc[0], iy = (c / d.nc[0]) % d.nc[1], iz = c / (d.nc[0] * d.nc[1]);
+    double cx = -p.L[0] / 2.0 + ((double)ix + 0.5) * (p.L[0] / d.nc[0]);
+    double cy = -p.L[1] / 2.0 + ((double)iy + 0.5) * (p.L[1] / d.nc[1]);
+    double cz = -p.L[2] / 2.0 + ((double)iz + 0.5) * (p.L[2] / d.nc[2]);
+    double dx = r.x - cx, dy = r.y - cy, dz = r.z - cz;
+    dx -= p.L[0] * rint(dx / p.L[0]);
+    dy -= p.L[1] * rint(dy / p.L[1]);
+    dz -= p.L[2] * rint(dz / p.L[2]);
+    d.posf[s] = make_float4((float)dx, (float)dy, (float)dz, __int_as_float(d.type[s]));
+}
+
+void mt_fast_home(const MtDev& d, const MtParams& p, int cond, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fast_home, dim3(nblk(d.n, TPB)), dim3(TPB), 0, s, d, p, cond);
+}
+
+// is atom `ia` (slot) a 1-2/1-3/1-4 partner of atom k (slot)? k's rows are in
+// original indexing (FunctorCalcForce.cpp:95-145 lists)
+__device__ inline bool mt_is_partner(const MtDev& d, int k, int ia)
+{
+    const MtBonded& b = d.bl;
+    int ok = d.perm[k], oi = d.perm[ia];
+    if (b.has_bonds)
+        for (int j = 0; j < b.b_num[ok]; j++) if (b.b_j[b.b_first[ok] + j] == oi) return true;
+    if (b.has_angles)
+        for (int j = 0; j < b.a_num[ok]; j++) {
+            int g = b.a_first[ok] + j;
+            if (!b.a_center[g] && b.a_j[g] == oi) return true;
+        }
+    if (b.has_dihedrals)
+        for (int j = 0; j < b.d_num[ok]; j++) {
+            int g = b.d_first[ok] + j;
+            if (!b.d_center[g] && b.d_l[g] == oi) return true;
+        }
+    return false;
+}
+
+// neighbour list at rlist over the 27 home cells around k's home cell;
+// same-molecule pairs and cross-molecule bonded partners are left to the
+// special-pair pass
+__global__ void __launch_bounds__(TPB) k_fast_neigh(MtDev d, MtParams p, int cond)
+{
+    if (mt_skip_rebuild(cond, d.flags, p.half_skin2)) return;
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= d.n) return;
+    const int Nx = d.nc[0], Ny = d.nc[1], Nz = d.nc[2];
+    int c = d.home[k];
+    int hx = c % Nx, hy = (c / Nx) % Ny, hz = c / (Nx * Ny);
+    float4 pk = d.posf[k];
+    int molk = d.mol[k];
+    bool xk = d.xmol[d.perm[k]] != 0;
+    int cnt = 0;
+    const int ncap = d.ncap, n = d.n;
+    for (int oz = -1; oz <= 1; oz++) {
+        int cz = hz + oz; cz = cz < 0 ? cz + Nz : (cz >= Nz ? cz - Nz : cz);
+        for (int oy = -1; oy <= 1; oy++) {
+            int cy = hy + oy; cy = cy < 0 ? cy + Ny : (cy >= Ny ? cy - Ny : cy);
+            for (int ox = -1; ox <= 1; ox++) {
+                int cx = hx + ox; cx = cx < 0 ? cx + Nx : (cx >= Nx ? cx - Nx : cx);
+                int cc = cx + Nx * (cy + Ny * cz);
+                const uint32_t code = (uint32_t)((ox + 1) + 3 * (oy + 1) + 9 * (oz + 1)) << MT_NW_IDX_BITS;
+                const float sx = (float)ox * p.cs[0], sy = (float)oy * p.cs[1], sz = (float)oz * p.cs[2];
+                int f0 = d.cell_first[cc], num = d.cell_count[cc];
+                for (int e = 0; e < num; e++) {
+                    int a = d.cell_list[f0 + e];
+                    float4 pa = d.posf[a];
+                    float dx = pk.x - pa.x - sx, dy = pk.y - pa.y - sy, dz = pk.z - pa.z - sz;
+                    float r2 = dx * dx + dy * dy + dz * dz;
+                    if (r2 < p.rlist2 && a != k) {
+                        if (molk >= 0 && d.mol[a] == molk) continue;
+                        if (xk && mt_is_partner(d, k, a)) continue;
+                        if (cnt < ncap) d.nlist[(size_t)cnt * n + k] = (uint32_t)a | code;
+                        cnt++;
+                    }
+                }
+            }
+        }
+    }
+    d.ncount[k] = cnt;
+    if (cnt > ncap) { d.flags[2] = 1; atomicMax(d.nreq, cnt); }
+}
+
+void mt_fast_neighbors(const MtDev& d, const MtParams& p, int cond, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fast_neigh, dim3(nblk(d.n, TPB)), dim3(TPB), 0, s, d, p, cond);
+}
+
+// ---------------------------------------------------------------------------
+// pair forces
+// ---------------------------------------------------------------------------
+
+// MIXED: FP32 pair math on local coordinates, FP64 per-atom accumulation
+template <bool WANT_E, bool COUL>
+__global__ void __launch_bounds__(512) k_fast_force_mixed(MtDev d, MtParams p)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= d.n) return;
+    const float4 pk = d.posf[k];
+    const int tk = __float_as_int(pk.w);
+    const int T = d.T;
+    const float4* __restrict__ coef = d.nbcoef + (size_t)tk * T * MT_NPTS;
+    const float csx = p.cs[0], csy = p.cs[1], csz = p.cs[2];
+    const float rc2 = p.rc2f, x0 = p.x0, invd = p.inv_delta;
+    const float qk = COUL ? d.qf[k] : 0.0f;
+    double Fx = 0.0, Fy = 0.0, Fz = 0.0;
+    float U = 0.0f;
+    const int cnt = min(d.ncount[k], d.ncap);
+    const uint32_t* __restrict__ nl = d.nlist + k;
+    const size_t stride = (size_t)d.n;
+    for (int j = 0; j < cnt; j++) {
+        const uint32_t w = nl[(size_t)j * stride];
+        const int a = (int)(w & MT_NW_IDX_MASK);
+        const int code = (int)(w >> MT_NW_IDX_BITS);
+        const int ox = code % 3 - 1, oy = (code / 3) % 3 - 1, oz = code / 9 - 1;
+        const float4 pa = d.posf[a];
+        const float dx = pk.x - pa.x - (float)ox * csx;
+        const float dy = pk.y - pa.y - (float)oy * csy;
+        const float dz = pk.z - pa.z - (float)oz * csz;
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        if (r2 < rc2 && r2 >= 1.0e-6f) {
+            const float r = __fsqrt_rn(r2);
+            const float rinv = 1.0f / r;
+            float fr = 0.0f;
+            const int idx = (int)((r - x0) * invd);
+            if (idx >= 0 && idx < MT_NPTS - 1) {
+                const int ta = __float_as_int(pa.w);
+                const float4 c = coef[(size_t)ta * MT_NPTS + idx];
+                fr = (c.x * r + c.y) * rinv;
+                if (WANT_E) U += 0.5f * (c.z * r + c.w);
+            }
+            if (COUL) {
+                const float K = MT_COULOMB_K * qk * d.qf[a];
+                fr += K * rinv * rinv * rinv;
+                if (WANT_E) U += 0.5f * K * rinv;
+            }
+            Fx += (double)(dx * fr);
+            Fy += (double)(dy * fr);
+            Fz += (double)(dz * fr);
+        }
+    }
+    if (!d.mobile[k]) { Fx = Fy = Fz = 0.0; U = 0.0f; }
+    d.frc[k] = make_double4(Fx, Fy, Fz, 0.0);
+    if (WANT_E) d.epot[k] = U;
+}
+
+// FP64: the reference's per-pair arithmetic (FunctorCalcForce.cpp:467-535)
+// on the skin list, restricted to pairs inside rc by the reference predicate
+template <bool WANT_E, bool COUL>
+__global__ void __launch_bounds__(256) k_fast_force_fp64(MtDev d, MtParams p)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= d.n) return;
+    const double4 rk4 = d.pos[k];
+    const mt_d3 rk = {rk4.x, rk4.y, rk4.z};
+    const int row = d.type[k], T = d.T;
+    const float qk = (float)rk4.w;
+    mt_d3 F = {0.0, 0.0, 0.0};
+    float U = 0.0f;
+    const int cnt = min(d.ncount[k], d.ncap);
+    for (int j = 0; j < cnt; j++) {
+        const int a = (int)(d.nlist[(size_t)j * d.n + k] & MT_NW_IDX_MASK);
+        const double4 ra4 = d.pos[a];
+        const mt_d3 ra = {ra4.x, ra4.y, ra4.z};
+        if (!(mt_dist2_pbc(rk, ra, p.wnl) < (double)p.rc2f)) continue;
+        const mt_d3 ri = mt_move_same_side(rk, ra, p.wbox);
+        float u = 0.0f;
+        const float r = (float)mt_norm(mt_sub(rk, ri));
+        if (!(r < 1E-3f)) {
+            const int col = d.type[a];
+            const int nff = d.nbcnt[row * T + col];
+            float mdU = 0.0f;
+            for (int ff = 0; ff < nff; ff++) {
+                const float4* tab = d.nbtab + ((size_t)(row * T + col) * MT_MAXFF + ff) * MT_NPTS;
+                mt_table_eval(tab, r, WANT_E, mdU, u);
+            }
+            F = mt_add(F, mt_mul(mt_coeffs12(ri, rk), (double)mdU));
+        }
+        if (COUL) {
+            float uc = 0.0f;
+            // i > k selects the energy half; pass (a, -1) so the energy is always formed
+            mt_d3 fc = mt_coulomb_pair(rk, ri, qk, (float)ra4.w, -1, 0, uc);
+            F = mt_add(F, fc);
+            u += uc;
+        }
+        if (WANT_E) U += 0.5f * u;
+    }
+    if (!d.mobile[k]) { F = {0.0, 0.0, 0.0}; U = 0.0f; }
+    d.frc[k] = make_double4(F.x, F.y, F.z, 0.0);
+    if (WANT_E) d.epot[k] = U;
+}
+
+// bonded terms and special (same-molecule / scaled) pairs, added on top of the
+// pair forces with the exact FP64 helpers; one thread per atom with work
+__global__ void __launch_bounds__(TPB) k_fast_bonded_special(MtDev d, MtParams p, int want_e, int coul)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= d.n) return;
+    if (!d.mobile[k]) return;
+    const int o = d.perm[k];
+    const MtBonded& b = d.bl;
+    const bool has_b = (b.has_bonds && b.b_num[o]) || (b.has_angles && b.a_num[o]) ||
+                       (b.has_dihedrals && b.d_num[o]);
+    const bool has_s = d.sp.n_entries > 0 && d.sp.num[o] > 0;
+    if (!has_b && !has_s) return;
+    mt_d3 F = {0.0, 0.0, 0.0};
+    float U = 0.0f;
+    if (has_b) mt_exact_bonded(d, p, k, F, U, o, d.inv);
+    if (has_s) {
+        const double4 rk4 = d.pos[k];
+        const mt_d3 rk = {rk4.x, rk4.y, rk4.z};
+        const int row = d.type[k], T = d.T;
+        for (int j = 0; j < d.sp.num[o]; j++) {
+            const int po = d.sp.partner[d.sp.first[o] + j];
+            const float sc = d.sp.scale[d.sp.first[o] + j];
+            const int a = d.inv[po];
+            const double4 ra4 = d.pos[a];
+            const mt_d3 ra = {ra4.x, ra4.y, ra4.z};
+            if (!(mt_dist2_pbc(rk, ra, p.wnl) < (double)p.rc2f)) continue;
+            const mt_d3 ri = mt_move_same_side(rk, ra, p.wbox);
+            float u = 0.0f;
+            mt_d3 f = {0.0, 0.0, 0.0};
+            const float r = (float)mt_norm(mt_sub(rk, ri));
+            if (!(r < 1E-3f)) {
+                const int col = d.type[a];
+                float mdU = 0.0f;
+                for (int ff = 0; ff < d.nbcnt[row * T + col]; ff++)
+                    mt_table_eval(d.nbtab + ((size_t)(row * T + col) * MT_MAXFF + ff) * MT_NPTS, r, true, mdU, u);
+                f = mt_mul(mt_coeffs12(ri, rk), (double)mdU);
+            }
+            if (coul) {
+                float uc = 0.0f;
+                f = mt_add(f, mt_coulomb_pair(rk, ri, (float)rk4.w, (float)ra4.w, -1, 0, uc));
+                u += uc;
+            }
+            F = mt_add(F, mt_mul(f, (double)sc));
+            U += 0.5f * (u * sc);
+        }
+    }
+    double4 f0 = d.frc[k];
+    d.frc[k] = make_double4(f0.x + F.x, f0.y + F.y, f0.z + F.z, 0.0);
+    if (want_e) d.epot[k] += U;
+}
+
+void mt_fast_forces(const MtDev& d, const MtParams& p, int mixed, int want_e, int block, hipStream_t s)
+{
+    const bool coul = p.coulomb_on != 0;
+    if (mixed) {
+        int bs = (block == 64 || block == 128 || block == 256 || block == 512) ? block : 256;
+        dim3 g(nblk(d.n, bs)), b(bs);
+        if (want_e) {
+            if (coul) hipLaunchKernelGGL((k_fast_force_mixed<true, true>), g, b, 0, s, d, p);
+            else hipLaunchKernelGGL((k_fast_force_mixed<true, false>), g, b, 0, s, d, p);
+        } else {
+            if (coul) hipLaunchKernelGGL((k_fast_force_mixed<false, true>), g, b, 0, s, d, p);
+            else hipLaunchKernelGGL((k_fast_force_mixed<false, false>), g, b, 0, s, d, p);
+        }
+    } else {
+        int bs = (block == 64 || block == 128 || block == 256) ? block : 256;
+        dim3 g(nblk(d.n, bs)), b(bs);
+        if (want_e) {
+            if (coul) hipLaunchKernelGGL((k_fast_force_fp64<true, true>), g, b, 0, s, d, p);
+            else hipLaunchKernelGGL((k_fast_force_fp64<true, false>), g, b, 0, s, d, p);
+        } else {
+            if (coul) hipLaunchKernelGGL((k_fast_force_fp64<false, true>), g, b, 0, s, d, p);
+            else hipLaunchKernelGGL((k_fast_force_fp64<false, false>), g, b, 0, s, d, p);
+        }
+    }
+    if (d.bl.has_bonds || d.bl.has_angles || d.bl.has_dihedrals || d.sp.n_entries > 0)
+        hipLaunchKernelGGL(k_fast_bonded_special, dim3(nblk(d.n, TPB)), dim3(TPB), 0, s, d, p, want_e,
+                           coul ? 1 : 0);
+}
+
+// ---------------------------------------------------------------------------
+// integrator front half for the sorted engines: thermostat scale, kick, drift,
+// minimum-image displacement since the last rebuild, MIXED local coordinates
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(TPB) k_fast_kick_drift(MtDev d, MtParams p, int scale_slot, int mixed)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    float d2 = 0.0f;
+    if (k < d.n) {
+        double4 pm = d.mom[k], r = d.pos[k], F = d.frc[k];
+        const double dt_2 = p.dt / 2.0;
+        if (scale_slot >= 0) {
+            const double S = d.nh[scale_slot];
+            pm.x *= S; pm.y *= S; pm.z *= S;
+        }
+        pm.x += F.x * dt_2; pm.y += F.y * dt_2; pm.z += F.z * dt_2;
+        const double c = p.dt / pm.w;
+        r.x += pm.x * c; r.y += pm.y * c; r.z += pm.z * c;
+        d.mom[k] = pm;
+        d.pos[k] = r;
+        const double4 q = d.posref[k];
+        double dx = r.x - q.x, dy = r.y - q.y, dz = r.z - q.z;
+        dx -= p.L[0] * rint(dx / p.L[0]);
+        dy -= p.L[1] * rint(dy / p.L[1]);
+        dz -= p.L[2] * rint(dz / p.L[2]);
+        d2 = (float)(dx * dx + dy * dy + dz * dz);
+        if (mixed) {
+            const int c0 = d.home[k];
+            const int ix = c0 % d.nc[0], iy = (c0 / d.nc[0]) % d.nc[1], iz = c0 / (d.nc[0] * d.nc[1]);
+            double lx = r.x - (-p.L[0] / 2.0 + ((double)ix + 0.5) * (p.L[0] / d.nc[0]));
+            double ly = r.y - (-p.L[1] / 2.0 + ((double)iy + 0.5) * (p.L[1] / d.nc[1]));
+            double lz = r.z - (-p.L[2] / 2.0 + ((double)iz + 0.5) * (p.L[2] / d.nc[2]));
+            lx -= p.L[0] * rint(lx / p.L[0]);
+            ly -= p.L[1] * rint(ly / p.L[1]);
+            lz -= p.L[2] * rint(lz / p.L[2]);
+            const float4 old = d.posf[k];
+            d.posf[k] = make_float4((float)lx, (float)ly, (float)lz, old.w);
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) d2 = fmaxf(d2, __shfl_xor(d2, off));
+    if ((threadIdx.x & 63) == 0 && d2 > 0.0f) atomicMax(&d.flags[0], __float_as_uint(d2));
+}
+
+void mt_fast_kick_drift(const MtDev& d, const MtParams& p, int scale_slot, int mixed, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fast_kick_drift, dim3(nblk(d.n, TPB)), dim3(TPB), 0, s, d, p, scale_slot, mixed);
+}
+
+__global__ void k_fast_clear(unsigned int* flags) { flags[0] = 0u; }
+
+void mt_fast_clear_flags(const MtDev& d, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fast_clear, dim3(1), dim3(1), 0, s, d.flags);
+}

@@ -1,0 +1,1245 @@
+// mtmd_host.cpp - C ABI of libmtmd.so (include/mtmd.h): configuration, system
+// set-up, device memory, and the step driver for the three engines.
+//
+// Replaces, on the host side, CMolDynConfig (Config/MolDynConfig.cpp),
+// CMDFFMatrices::prepareFFMatrices (ForceFields/MDFFMatrices.cpp:181-536),
+// CSimulationBox (SimulationBox/SimulationBox.cpp), CVelVerlet
+// (Integrators/VelVerlet.cpp), CMDLoop::runSimulation/runOptimization
+// (MDLoop/MDLoop.cpp:107-173) and CMDSimulator (Simulator/MDSimulator.cpp).
+// All paths are relative to Cmd/MolTwisterCmdMolDyn/ of the MolTwister tree
+// unless stated otherwise. The state stays resident in HBM; only thermo
+// scalars and explicitly requested arrays cross PCIe.
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/mtmd.h"
+#include "mtmd_internal.h"
+
+namespace {
+
+thread_local std::string g_create_error;
+
+struct DevBuf {
+    std::vector<void*> ptrs;
+    template <class T> T* alloc(size_t count)
+    {
+        void* p = nullptr;
+        if (count == 0) count = 1;
+        if (hipMalloc(&p, count * sizeof(T)) != hipSuccess) return nullptr;
+        ptrs.push_back(p);
+        return static_cast<T*>(p);
+    }
+    void release()
+    {
+        for (void* p : ptrs) (void)hipFree(p);
+        ptrs.clear();
+    }
+};
+
+uint64_t splitmix64(uint64_t& s)
+{
+    uint64_t z = (s += 0x9E3779B97F4A7C15ULL);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+double urand(uint64_t& s) { return double(splitmix64(s) >> 11) * (1.0 / 9007199254740992.0); }
+
+} // namespace
+
+struct mtmd_handle {
+    int n = 0, T = 0;
+    mtmd_config cfg{};
+    int engine = MTMD_ENGINE_EXACT;
+    MtDev d{};
+    MtParams p{};
+    hipStream_t stream = nullptr;
+    DevBuf mem;
+    std::string err;
+
+    // thermostat (NHChain.h:55-66)
+    int M = 0, nresp = 0;
+    double Tred = 0, tau = 0, g = 0, w[3] = {0, 0, 0};
+    int nh_scale_slot = -1;
+
+    bool have_forces = false;     // forces of the current positions exist
+    bool ke_valid = false;        // d.red holds per-block sums of p.p/m of the current momenta
+    int64_t rebuilds = 0, steps = 0;
+
+    // FP64/MIXED engines
+    bool fast = false;
+    bool need_sort = true;
+    int sort_every = 100;
+    int steps_since_sort = 0;
+    double half_skin2 = 0;
+    std::vector<int> perm_host;   // valid after download_perm()
+    int* h_status = nullptr;      // pinned status words
+
+    int fail(int code, const char* fmt, ...)
+    {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof buf, fmt, ap);
+        va_end(ap);
+        err = buf;
+        return code;
+    }
+};
+
+#define HIPCHK(h, expr)                                                                          \
+    do {                                                                                         \
+        hipError_t e_ = (expr);                                                                  \
+        if (e_ != hipSuccess) return (h)->fail(MTMD_EHIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// library
+// ---------------------------------------------------------------------------
+extern "C" int mtmd_abi_version(void) { return MTMD_ABI_VERSION; }
+
+extern "C" const char* mtmd_last_error(const mtmd_handle* h)
+{
+    return h ? h->err.c_str() : g_create_error.c_str();
+}
+
+extern "C" const char* mtmd_status_string(int s)
+{
+    switch (s) {
+    case MTMD_OK: return "ok";
+    case MTMD_EINVAL: return "invalid argument";
+    case MTMD_EHIP: return "HIP runtime error";
+    case MTMD_ENEIGH: return "neighbour list overflow";
+    case MTMD_ENAN: return "non-finite value";
+    case MTMD_EBOX: return "box too small";
+    case MTMD_ENOMEM: return "out of memory";
+    case MTMD_ENODEV: return "no HIP device";
+    }
+    return "unknown status";
+}
+
+// CMolDynConfig::resetToDefaults (Config/MolDynConfig.cpp:289-328)
+extern "C" void mtmd_config_defaults(mtmd_config* c)
+{
+    if (!c) return;
+    std::memset(c, 0, sizeof *c);
+    c->timestep = 1.0;
+    c->out_stride = 100;
+    c->ensemble = MTMD_NVT;
+    c->temperature = 298.0;
+    c->temperature_relax = 20.0 * c->timestep;
+    c->temperature_nh_len = 4;
+    c->temperature_respa = 4;
+    c->pressure = 1.0;
+    c->pressure_relax = 5000.0 * c->timestep;
+    c->pressure_nh_len = 4;
+    c->pressure_respa = 4;
+    c->cutoff_radius = 10.0;
+    c->neigh_shell = 2.0;
+    c->cutoff_force = 1000.0;
+    c->n_steps = 50000;
+    std::snprintf(c->info_file, sizeof c->info_file, "%s", "out.txt");
+    c->scale12 = 0.0;
+    c->scale13 = 0.0;
+    c->scale14 = 0.5;
+    c->scale1N = 0.0;
+    c->opt_learning_rate = 1e-6;
+    c->opt_max_steps = 5000;
+    c->opt_accuracy = 0.0;
+    c->engine = MTMD_ENGINE_EXACT;
+    c->coulomb = MTMD_COUL_AUTO;
+    c->skin = 0.3;
+}
+
+// CMolDynConfig::set (Config/MolDynConfig.cpp:95-287); output-file keys the
+// engine does not consume are accepted and ignored, as strings the reference
+// only stores.
+extern "C" int mtmd_config_set(mtmd_config* c, const char* key, const char* value, char* err, size_t err_len)
+{
+    auto fail = [&](const char* msg) {
+        if (err && err_len) std::snprintf(err, err_len, "%s", msg);
+        return MTMD_EINVAL;
+    };
+    if (!c || !key || !value) return fail("Error: unknown parameter!");
+    const std::string k = key, v = value;
+    struct IntKey { const char* name; int32_t* field; };
+    struct DblKey { const char* name; double* field; };
+    const IntKey ik[] = {{"timesteps", &c->n_steps}, {"outstride", &c->out_stride},
+                         {"temperaturenhlen", &c->temperature_nh_len}, {"temperaturerespa", &c->temperature_respa},
+                         {"pressurenhlen", &c->pressure_nh_len}, {"pressurerespa", &c->pressure_respa},
+                         {"optmaxsteps", &c->opt_max_steps}, {"blocksize", &c->block_size}};
+    const DblKey dk[] = {{"timestep", &c->timestep}, {"temperature", &c->temperature},
+                         {"temperaturerelax", &c->temperature_relax}, {"pressure", &c->pressure},
+                         {"pressurerelax", &c->pressure_relax}, {"cutoffradius", &c->cutoff_radius},
+                         {"neighshell", &c->neigh_shell}, {"cutoffforce", &c->cutoff_force},
+                         {"scale12", &c->scale12}, {"scale13", &c->scale13}, {"scale14", &c->scale14},
+                         {"scale1N", &c->scale1N}, {"optlearningrate", &c->opt_learning_rate},
+                         {"optaccuracy", &c->opt_accuracy}, {"skin", &c->skin}};
+    for (const auto& e : ik) if (k == e.name) { *e.field = std::atoi(value); return MTMD_OK; }
+    for (const auto& e : dk) if (k == e.name) { *e.field = std::atof(value); return MTMD_OK; }
+    if (k == "ensemble") {
+        if (v == "nvt") c->ensemble = MTMD_NVT;
+        else if (v == "npt") c->ensemble = MTMD_NPT;
+        else if (v == "nve") c->ensemble = MTMD_NVE;
+        else return fail("Error: unknown ensemble!");
+        return MTMD_OK;
+    }
+    if (k == "infofile") { std::snprintf(c->info_file, sizeof c->info_file, "%s", value); return MTMD_OK; }
+    if (k == "xyzfile" || k == "dcdfile" || k == "xtcfile" || k == "pdistrfile" || k == "vdistrfile" ||
+        k == "xtcprecision" || k == "maxpdistr" || k == "maxvdistr")
+        return MTMD_OK;
+    if (k == "includexyzfile" || k == "includedcdfile" || k == "includextcfile" || k == "includepdistrfile" ||
+        k == "includevdistrfile" || k == "verboseoutput") {
+        if (v != "yes" && v != "no") {
+            char msg[160];
+            std::snprintf(msg, sizeof msg, "Error: '%s' parameter should be either 'yes' or 'no'!", key);
+            return fail(msg);
+        }
+        if (k == "verboseoutput") c->verbose = (v == "yes");
+        return MTMD_OK;
+    }
+    if (k == "engine") {
+        if (v == "exact") c->engine = MTMD_ENGINE_EXACT;
+        else if (v == "fp64") c->engine = MTMD_ENGINE_FP64;
+        else if (v == "mixed") c->engine = MTMD_ENGINE_MIXED;
+        else return fail("Error: unknown engine!");
+        return MTMD_OK;
+    }
+    if (k == "coulomb") {
+        if (v == "auto") c->coulomb = MTMD_COUL_AUTO;
+        else if (v == "allpairs") c->coulomb = MTMD_COUL_ALLPAIRS;
+        else if (v == "cutoff") c->coulomb = MTMD_COUL_CUTOFF;
+        else return fail("Error: unknown coulomb mode!");
+        return MTMD_OK;
+    }
+    return fail("Error: unknown parameter!");
+}
+
+// ---------------------------------------------------------------------------
+// set-up
+// ---------------------------------------------------------------------------
+namespace {
+
+template <class T> int upload(mtmd_handle* h, T*& dst, const std::vector<T>& src)
+{
+    dst = h->mem.alloc<T>(src.size());
+    if (!dst) return h->fail(MTMD_ENOMEM, "device allocation of %zu bytes failed", src.size() * sizeof(T));
+    if (!src.empty()) HIPCHK(h, hipMemcpy(dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
+    return MTMD_OK;
+}
+
+std::vector<float4> make_table(int which, int kind, const double* p, float x0, float x1)
+{
+    std::vector<float> x(MT_NPTS), f(MT_NPTS), e(MT_NPTS);
+    std::vector<float4> t(MT_NPTS);
+    mtmd_profile(which, kind, p, x0, x1, MT_NPTS, x.data(), f.data(), e.data());
+    for (int i = 0; i < MT_NPTS; i++) t[i] = make_float4(x[i], f[i], e[i], 0.0f);
+    return t;
+}
+
+// segment coefficients a, b of interpolateLinear (FunctorCalcForce.cpp:457-465)
+// for force and energy, for the MIXED engine's single-multiply evaluation
+void segment_coeffs(const float4* t, int i, float& af, float& bf, float& ae, float& be)
+{
+    float denom = t[i].x - t[i + 1].x;
+    if (denom == 0.0f) denom = 3.0f * 1.17549435e-38f;
+    af = (t[i].y - t[i + 1].y) / denom;
+    bf = t[i].y - af * t[i].x;
+    ae = (t[i].z - t[i + 1].z) / denom;
+    be = t[i].z - ae * t[i].x;
+}
+
+} // namespace
+
+static int setup(mtmd_handle* h, const mtmd_system* s)
+{
+    const int n = s->n_atoms, T = s->n_types;
+    mtmd_config& c = h->cfg;
+    h->n = n; h->T = T;
+    h->engine = c.engine;
+    h->fast = (c.engine == MTMD_ENGINE_FP64 || c.engine == MTMD_ENGINE_MIXED);
+    if (n <= 0 || T <= 0 || !s->pos || !s->mass || !s->charge || !s->type)
+        return h->fail(MTMD_EINVAL, "system needs atoms, types, positions, masses and charges");
+    if (n >= (1 << MT_NW_IDX_BITS)) return h->fail(MTMD_EINVAL, "at most %d atoms", (1 << MT_NW_IDX_BITS) - 1);
+    if (c.ensemble == MTMD_NPT) return h->fail(MTMD_EINVAL, "ensemble npt is not supported by this engine");
+    if (c.ensemble != MTMD_NVT && c.ensemble != MTMD_NVE) return h->fail(MTMD_EINVAL, "unknown ensemble");
+    if (c.cutoff_radius <= 0) return h->fail(MTMD_EINVAL, "cutoffradius must be positive");
+    if (c.timestep <= 0) return h->fail(MTMD_EINVAL, "timestep must be positive");
+    if (c.ensemble == MTMD_NVT && (c.temperature_nh_len < 1 || c.temperature_respa < 1 || c.temperature <= 0 ||
+                                   c.temperature_relax <= 0))
+        return h->fail(MTMD_EINVAL, "NVT needs temperature, temperaturerelax, temperaturenhlen and "
+                                    "temperaturerespa > 0");
+    for (int k = 0; k < n; k++)
+        if (s->type[k] < 0 || s->type[k] >= T || !(s->mass[k] > 0) || !std::isfinite(s->pos[3 * k]) ||
+            !std::isfinite(s->pos[3 * k + 1]) || !std::isfinite(s->pos[3 * k + 2]))
+            return h->fail(MTMD_EINVAL, "atom %d: type out of range, non-positive mass or non-finite position", k);
+    for (int i = 0; i < 3; i++)
+        if (!(s->box[i] > 0)) return h->fail(MTMD_EINVAL, "box widths must be positive");
+
+    MtParams& p = h->p;
+    p.rc = (float)c.cutoff_radius;
+    p.rc2f = p.rc;
+    p.rc2f *= p.rc2f;
+    for (int i = 0; i < 3; i++) {
+        p.L[i] = s->box[i];
+        p.Lf[i] = (float)s->box[i];
+        p.lowf[i] = -p.Lf[i] / 2.0f;
+        // CFunctorCalcForce::pbc_ (FunctorCalcForce.cpp:41)
+        const double Lf = (double)p.Lf[i];
+        p.wbox[i] = (Lf / 2.0) - (-Lf / 2.0);
+        // CFunctorGenNeighList::pbc_ (FunctorGenNeighList.cpp:63-69)
+        p.wnl[i] = (double)(p.lowf[i] + p.Lf[i]) - (double)p.lowf[i];
+    }
+    p.scale12 = (float)c.scale12; p.scale13 = (float)c.scale13;
+    p.scale14 = (float)c.scale14; p.scale1N = (float)c.scale1N;
+    p.dt = c.timestep / MT_CONV_t;
+    p.Fcut = c.cutoff_force;
+    bool any_q = false;
+    for (int k = 0; k < n; k++) if ((float)s->charge[k] != 0.0f) any_q = true;
+    int cm = c.coulomb;
+    if (cm == MTMD_COUL_AUTO) cm = h->fast ? MTMD_COUL_CUTOFF : MTMD_COUL_ALLPAIRS;
+    if (h->fast && cm == MTMD_COUL_ALLPAIRS)
+        return h->fail(MTMD_EINVAL, "coulomb allpairs is available in the exact engine only");
+    p.coulomb_allpairs = (cm == MTMD_COUL_ALLPAIRS);
+    p.coulomb_on = any_q ? 1 : 0;
+
+    // --- tables (MDFFMatrices.cpp:249-382) ---
+    std::vector<float4> nbtab((size_t)T * T * MT_MAXFF * MT_NPTS, make_float4(0, 0, 0, 0));
+    std::vector<int> nbcnt((size_t)T * T, 0);
+    for (int col = 0; col < T; col++)
+        for (int row = 0; row < T; row++) {
+            int cnt = 0;
+            for (int i = 0; i < s->n_nonbonded; i++) {
+                const auto& e = s->nonbonded[i];
+                const bool match = (e.type_a == col && e.type_b == row) || (e.type_b == col && e.type_a == row);
+                if (!match) continue;
+                if (cnt >= MT_MAXFF)
+                    return h->fail(MTMD_EINVAL, "more than %d non-bonded force fields for types %d-%d", MT_MAXFF,
+                                   row, col);
+                if (e.kind != MTMD_NB_LJ && e.kind != MTMD_NB_BUCK && e.kind != MTMD_NB_LJ1208)
+                    return h->fail(MTMD_EINVAL, "non-bonded entry %d: unknown kind %d", i, e.kind);
+                auto t = make_table(0, e.kind, e.p, 0.01f, p.rc);
+                std::copy(t.begin(), t.end(), nbtab.begin() + ((size_t)(row * T + col) * MT_MAXFF + cnt) * MT_NPTS);
+                cnt++;
+            }
+            nbcnt[row * T + col] = cnt;
+        }
+    auto bonded_tables = [&](int which, const mtmd_bonded_ff* ff, int cnt, float x0, float x1,
+                             std::vector<float4>& out) -> int {
+        out.assign((size_t)std::max(cnt, 1) * MT_NPTS, make_float4(0, 0, 0, 0));
+        for (int i = 0; i < cnt; i++) {
+            // a two-point profile with no outputs only validates the kind
+            if (mtmd_profile(which, ff[i].kind, ff[i].p, x0, x1, 2, nullptr, nullptr, nullptr) != MTMD_OK)
+                return h->fail(MTMD_EINVAL, "bonded force field %d (table %d): unknown kind %d", i, which, ff[i].kind);
+            auto t = make_table(which, ff[i].kind, ff[i].p, x0, x1);
+            std::copy(t.begin(), t.end(), out.begin() + (size_t)i * MT_NPTS);
+        }
+        return MTMD_OK;
+    };
+    std::vector<float4> btab, atab, dtab;
+    const float two_pi = float(2.0 * M_PI);
+    int rc_;
+    if ((rc_ = bonded_tables(1, s->bond_ff, s->n_bond_ff, 0.01f, p.rc, btab))) return rc_;
+    if ((rc_ = bonded_tables(2, s->angle_ff, s->n_angle_ff, 0.0f, two_pi, atab))) return rc_;
+    if ((rc_ = bonded_tables(3, s->dihedral_ff, s->n_dihedral_ff, 0.0f, two_pi, dtab))) return rc_;
+
+    // --- molecule indices from the bond graph (MolecularSystemTools.cpp:238-280) ---
+    std::vector<int> par(n);
+    std::iota(par.begin(), par.end(), 0);
+    auto find = [&](int a) { while (par[a] != a) { par[a] = par[par[a]]; a = par[a]; } return a; };
+    for (int j = 0; j < s->n_bonds; j++) {
+        int a = s->bonds[3 * j], b = s->bonds[3 * j + 1];
+        if (a < 0 || b < 0 || a >= n || b >= n) return h->fail(MTMD_EINVAL, "bond %d: atom index out of range", j);
+        a = find(a); b = find(b);
+        if (a != b) par[std::max(a, b)] = std::min(a, b);
+    }
+    std::vector<int> mol(n);
+    for (int k = 0; k < n; k++) mol[k] = find(k);
+
+    // --- per-atom bonded lists in the reference order (MDFFMatrices.cpp:384-520):
+    // bucket the global lists by atom, keeping global order inside each bucket ---
+    for (int j = 0; j < s->n_bonds; j++)
+        if (s->bonds[3 * j + 2] >= s->n_bond_ff) return h->fail(MTMD_EINVAL, "bond %d: force field out of range", j);
+    for (int j = 0; j < s->n_angles; j++) {
+        for (int q = 0; q < 3; q++)
+            if (s->angles[4 * j + q] < 0 || s->angles[4 * j + q] >= n)
+                return h->fail(MTMD_EINVAL, "angle %d: atom index out of range", j);
+        if (s->angles[4 * j + 3] >= s->n_angle_ff) return h->fail(MTMD_EINVAL, "angle %d: force field out of range", j);
+    }
+    for (int j = 0; j < s->n_dihedrals; j++) {
+        for (int q = 0; q < 4; q++)
+            if (s->dihedrals[5 * j + q] < 0 || s->dihedrals[5 * j + q] >= n)
+                return h->fail(MTMD_EINVAL, "dihedral %d: atom index out of range", j);
+        if (s->dihedrals[5 * j + 4] >= s->n_dihedral_ff)
+            return h->fail(MTMD_EINVAL, "dihedral %d: force field out of range", j);
+    }
+    std::vector<std::vector<int>> bonds_of(n), angles_of(n), dihs_of(n);
+    for (int j = 0; j < s->n_bonds; j++) {
+        const int a = s->bonds[3 * j], b = s->bonds[3 * j + 1];
+        bonds_of[a].push_back(j);
+        if (b != a) bonds_of[b].push_back(j);
+    }
+    for (int j = 0; j < s->n_angles; j++) {
+        const int* a = s->angles + 4 * j;
+        for (int q = 0; q < 3; q++) {
+            bool dup = false;
+            for (int r = 0; r < q; r++) dup |= (a[r] == a[q]);
+            if (!dup) angles_of[a[q]].push_back(j);
+        }
+    }
+    for (int j = 0; j < s->n_dihedrals; j++) {
+        const int* a = s->dihedrals + 5 * j;
+        for (int q = 0; q < 4; q++) {
+            bool dup = false;
+            for (int r = 0; r < q; r++) dup |= (a[r] == a[q]);
+            if (!dup) dihs_of[a[q]].push_back(j);
+        }
+    }
+    std::vector<int> b_first(n), b_num(n), b_j, b_type;
+    std::vector<int> a_first(n), a_num(n), a_i, a_j, a_type;
+    std::vector<uint8_t> a_center;
+    std::vector<int> d_first(n), d_num(n), d_i, d_j, d_l, d_type;
+    std::vector<uint8_t> d_center;
+    for (int k = 0; k < n; k++) {
+        b_first[k] = (int)b_j.size();
+        for (int j : bonds_of[k]) {
+            const int a1 = s->bonds[3 * j], a2 = s->bonds[3 * j + 1], t = s->bonds[3 * j + 2];
+            int to = -1;
+            if (a1 == k) to = a2;
+            if (a2 == k) to = a1;
+            if (to != -1 && t != -1) { b_j.push_back(to); b_type.push_back(t); }
+        }
+        b_num[k] = (int)b_j.size() - b_first[k];
+        a_first[k] = (int)a_i.size();
+        for (int j : angles_of[k]) {
+            const int* a = s->angles + 4 * j;
+            int I = -1, J = -1, centre = 0;
+            if (a[0] == k) { I = a[1]; J = a[2]; }
+            if (a[1] == k) { centre = 1; I = a[0]; J = a[2]; }
+            if (a[2] == k) { I = a[1]; J = a[0]; }
+            if (I != -1 && J != -1 && a[3] != -1) {
+                a_i.push_back(I); a_j.push_back(J); a_type.push_back(a[3]); a_center.push_back((uint8_t)centre);
+            }
+        }
+        a_num[k] = (int)a_i.size() - a_first[k];
+        d_first[k] = (int)d_i.size();
+        for (int j : dihs_of[k]) {
+            const int* a = s->dihedrals + 5 * j;
+            int I = -1, J = -1, L = -1, centre = 0;
+            if (a[0] == k) { I = a[1]; J = a[2]; L = a[3]; }
+            if (a[1] == k) { centre = 1; I = a[0]; J = a[2]; L = a[3]; }
+            if (a[2] == k) { centre = 1; I = a[3]; J = a[1]; L = a[0]; }
+            if (a[3] == k) { I = a[2]; J = a[1]; L = a[0]; }
+            if (I != -1 && J != -1 && L != -1 && a[4] != -1) {
+                d_i.push_back(I); d_j.push_back(J); d_l.push_back(L); d_type.push_back(a[4]);
+                d_center.push_back((uint8_t)centre);
+            }
+        }
+        d_num[k] = (int)d_i.size() - d_first[k];
+    }
+    // xmol: atom has a scaled partner (1-2, 1-3, 1-4) outside its own molecule
+    std::vector<uint8_t> xmol(n, 0);
+    for (int k = 0; k < n; k++) {
+        for (int j = 0; j < b_num[k]; j++) if (mol[b_j[b_first[k] + j]] != mol[k]) xmol[k] = 1;
+        for (int j = 0; j < a_num[k]; j++)
+            if (!a_center[a_first[k] + j] && mol[a_j[a_first[k] + j]] != mol[k]) xmol[k] = 1;
+        for (int j = 0; j < d_num[k]; j++)
+            if (!d_center[d_first[k] + j] && mol[d_l[d_first[k] + j]] != mol[k]) xmol[k] = 1;
+    }
+
+    // --- special pairs for the FP64/MIXED engines: every same-molecule pair and
+    // every scaled cross-molecule partner, with the reference's scale
+    // (scaleForcesAndPotentials, FunctorCalcForce.cpp:95-157); zero scales dropped ---
+    std::vector<int> sp_first(n, 0), sp_num(n, 0), sp_partner;
+    std::vector<float> sp_scale;
+    if (h->fast) {
+        std::vector<std::vector<int>> members;
+        {
+            std::vector<int> root_to_idx(n, -1);
+            for (int k = 0; k < n; k++) {
+                int r = mol[k];
+                if (root_to_idx[r] < 0) { root_to_idx[r] = (int)members.size(); members.emplace_back(); }
+                members[root_to_idx[r]].push_back(k);
+            }
+            for (auto& m : members)
+                if (m.size() > 512)
+                    return h->fail(MTMD_EINVAL, "molecule of %zu atoms: the fp64/mixed engines take molecules of at "
+                                                "most 512 atoms (use engine exact)", m.size());
+            for (int k = 0; k < n; k++) {
+                sp_first[k] = (int)sp_partner.size();
+                auto scale_of = [&](int i) -> float {
+                    for (int j = 0; j < b_num[k]; j++) if (b_j[b_first[k] + j] == i) return p.scale12;
+                    for (int j = 0; j < a_num[k]; j++)
+                        if (!a_center[a_first[k] + j] && a_j[a_first[k] + j] == i) return p.scale13;
+                    for (int j = 0; j < d_num[k]; j++)
+                        if (!d_center[d_first[k] + j] && d_l[d_first[k] + j] == i) return p.scale14;
+                    return (mol[i] == mol[k]) ? p.scale1N : 1.0f;
+                };
+                std::vector<int> cand = members[root_to_idx[mol[k]]];
+                if (xmol[k]) {
+                    for (int j = 0; j < b_num[k]; j++) cand.push_back(b_j[b_first[k] + j]);
+                    for (int j = 0; j < a_num[k]; j++) if (!a_center[a_first[k] + j]) cand.push_back(a_j[a_first[k] + j]);
+                    for (int j = 0; j < d_num[k]; j++) if (!d_center[d_first[k] + j]) cand.push_back(d_l[d_first[k] + j]);
+                }
+                std::sort(cand.begin(), cand.end());
+                cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+                for (int i : cand) {
+                    if (i == k) continue;
+                    float sc = scale_of(i);
+                    if (sc != 0.0f) { sp_partner.push_back(i); sp_scale.push_back(sc); }
+                }
+                sp_num[k] = (int)sp_partner.size() - sp_first[k];
+            }
+        }
+        // cross-molecule partners must not also appear in the neighbour list:
+        // the fast neighbour build skips same-molecule pairs and, for xmol
+        // atoms, their bonded partners
+    }
+
+    // --- cell grid ---
+    MtDev& d = h->d;
+    d.n = n; d.T = T;
+    if (!h->fast) {
+        // CCellList::init (MDFFMatrices.cpp:43-63) on the initial box
+        const float R = (float)c.cutoff_radius + (float)c.neigh_shell;
+        for (int i = 0; i < 3; i++) d.nc[i] = (int)std::floor((double)(p.Lf[i] / R));
+        if (d.nc[0] < 1 || d.nc[1] < 1 || d.nc[2] < 1)
+            return h->fail(MTMD_EBOX, "box smaller than cutoffradius + neighshell: no cell fits");
+        if (c.verbose && (d.nc[0] < 3 || d.nc[1] < 3 || d.nc[2] < 3))
+            std::fprintf(stderr, "mtmd: fewer than 3 cells on an axis; the reference visits such cells more than "
+                                 "once and so does the exact engine\n");
+    } else {
+        if (!(c.skin >= 0)) return h->fail(MTMD_EINVAL, "skin must be >= 0");
+        p.rlist = (float)(c.cutoff_radius + c.skin);
+        p.rlist2 = p.rlist * p.rlist;
+        for (int i = 0; i < 3; i++) {
+            d.nc[i] = (int)std::floor(p.L[i] / (double)p.rlist);
+            if (d.nc[i] < 3)
+                return h->fail(MTMD_EBOX, "the fp64/mixed engines need at least 3 cells of cutoff+skin per axis "
+                                          "(box %.4g, cell %.4g)", p.L[i], (double)p.rlist);
+            if (p.L[i] < 2.0 * p.rlist) return h->fail(MTMD_EBOX, "box below twice the list radius");
+            p.cs[i] = (float)(p.L[i] / d.nc[i]);
+        }
+        h->half_skin2 = (c.skin / 2.0) * (c.skin / 2.0);
+        p.half_skin2 = (float)h->half_skin2;
+        p.x0 = 0.01f;
+        {
+            // same spacing the tables use: (rc - 0.01f)/999 in float
+            const float delta = (p.rc - 0.01f) / float(MT_NPTS - 1);
+            p.inv_delta = 1.0f / delta;
+        }
+    }
+    d.ncells = d.nc[0] * d.nc[1] * d.nc[2];
+
+    // --- device memory ---
+    std::vector<double4> pos(n), mom(n);
+    std::vector<int> type(s->type, s->type + n);
+    std::vector<uint8_t> mobile(n, 1);
+    if (s->mobile) for (int k = 0; k < n; k++) mobile[k] = s->mobile[k] ? 1 : 0;
+    for (int k = 0; k < n; k++) {
+        pos[k] = make_double4(s->pos[3 * k], s->pos[3 * k + 1], s->pos[3 * k + 2], (double)(float)s->charge[k]);
+        mom[k] = make_double4(0, 0, 0, s->mass[k]);
+    }
+    if (s->mom) {
+        for (int k = 0; k < n; k++) { mom[k].x = s->mom[3 * k]; mom[k].y = s->mom[3 * k + 1]; mom[k].z = s->mom[3 * k + 2]; }
+    } else {
+        // CSimulationBox::setRandMom (SimulationBox.cpp:98-133) from a seeded generator
+        uint64_t st = s->seed;
+        for (int k = 0; k < n; k++) {
+            double px = 2.0 * urand(st) - 1.0, py = 2.0 * urand(st) - 1.0, pz = 2.0 * urand(st) - 1.0;
+            mom[k].x = (px == 0.0) ? 1.0 : px;
+            mom[k].y = (py == 0.0) ? 1.0 : py;
+            mom[k].z = (pz == 0.0) ? 1.0 : pz;
+        }
+        for (int k = 0; k < n; k++) if (!mobile[k]) { mom[k].x = mom[k].y = mom[k].z = 0.0; }
+        double Tsum = 0.0;
+        for (int k = 0; k < n; k++)
+            Tsum += ((mom[k].x * mom[k].x) + (mom[k].y * mom[k].y) + (mom[k].z * mom[k].z)) / mom[k].w;
+        const double Tnow = Tsum / (3 * (double)n);
+        const double ps = std::sqrt((c.temperature / MT_CONV_T) / Tnow);
+        for (int k = 0; k < n; k++) { mom[k].x *= ps; mom[k].y *= ps; mom[k].z *= ps; }
+    }
+    int rc2;
+#define UP(dst, src) if ((rc2 = upload(h, dst, src))) return rc2
+    UP(d.pos, pos); UP(d.mom, mom); UP(d.type, type); UP(d.mol, mol); UP(d.mobile, mobile); UP(d.xmol, xmol);
+    UP(d.nbtab, nbtab); UP(d.nbcnt, nbcnt); UP(d.btab, btab); UP(d.atab, atab); UP(d.dtab, dtab);
+    UP(d.bl.b_first, b_first); UP(d.bl.b_num, b_num); UP(d.bl.b_j, b_j); UP(d.bl.b_type, b_type);
+    UP(d.bl.a_first, a_first); UP(d.bl.a_num, a_num); UP(d.bl.a_i, a_i); UP(d.bl.a_j, a_j);
+    UP(d.bl.a_type, a_type); UP(d.bl.a_center, a_center);
+    UP(d.bl.d_first, d_first); UP(d.bl.d_num, d_num); UP(d.bl.d_i, d_i); UP(d.bl.d_j, d_j); UP(d.bl.d_l, d_l);
+    UP(d.bl.d_type, d_type); UP(d.bl.d_center, d_center);
+    d.bl.has_bonds = !b_j.empty();
+    d.bl.has_angles = !a_i.empty();
+    d.bl.has_dihedrals = !d_i.empty();
+    if (h->fast) {
+        UP(d.sp.first, sp_first); UP(d.sp.num, sp_num); UP(d.sp.partner, sp_partner); UP(d.sp.scale, sp_scale);
+        d.sp.n_entries = (int)sp_partner.size();
+        // MIXED segment coefficients, summed over each pair's force fields
+        std::vector<float4> coef((size_t)T * T * MT_NPTS, make_float4(0, 0, 0, 0));
+        for (int pr = 0; pr < T * T; pr++)
+            for (int ff = 0; ff < nbcnt[pr]; ff++) {
+                const float4* t = &nbtab[((size_t)pr * MT_MAXFF + ff) * MT_NPTS];
+                for (int i = 0; i < MT_NPTS - 1; i++) {
+                    float af, bf, ae, be;
+                    segment_coeffs(t, i, af, bf, ae, be);
+                    float4& o = coef[(size_t)pr * MT_NPTS + i];
+                    o.x += af; o.y += bf; o.z += ae; o.w += be;
+                }
+            }
+        UP(d.nbcoef, coef);
+        std::vector<int> ident(n);
+        std::iota(ident.begin(), ident.end(), 0);
+        UP(d.perm, ident); UP(d.inv, ident);
+        d.perm2 = h->mem.alloc<int>(n);
+        d.frc2 = h->mem.alloc<double4>(n); d.epot2 = h->mem.alloc<float>(n);
+        d.pos2 = h->mem.alloc<double4>(n); d.mom2 = h->mem.alloc<double4>(n);
+        d.type2 = h->mem.alloc<int>(n); d.mol2 = h->mem.alloc<int>(n); d.mobile2 = h->mem.alloc<uint8_t>(n);
+        d.posref = h->mem.alloc<double4>(n);
+        d.posf = h->mem.alloc<float4>(n);
+        d.home = h->mem.alloc<int>(n);
+        d.qf = h->mem.alloc<float>(n);
+        d.qf2 = h->mem.alloc<float>(n);
+        std::vector<float> qf(n);
+        for (int k = 0; k < n; k++) qf[k] = (float)s->charge[k];
+        HIPCHK(h, hipMemcpy(d.qf, qf.data(), n * sizeof(float), hipMemcpyHostToDevice));
+        if (!d.pos2 || !d.mom2 || !d.type2 || !d.mol2 || !d.mobile2 || !d.posref || !d.posf || !d.home || !d.qf ||
+            !d.qf2 || !d.perm2 || !d.frc2 || !d.epot2)
+            return h->fail(MTMD_ENOMEM, "device allocation failed");
+    }
+#undef UP
+    d.frc = h->mem.alloc<double4>(n);
+    d.epot = h->mem.alloc<float>(n);
+    d.cell_key = h->mem.alloc<int>(n);
+    d.cell_xyz = h->fast ? nullptr : h->mem.alloc<int>(3 * (size_t)n);
+    d.cell_count = h->mem.alloc<int>(d.ncells);
+    d.cell_first = h->mem.alloc<int>(d.ncells);
+    d.cell_fill = h->mem.alloc<int>(d.ncells);
+    d.cell_list = h->mem.alloc<int>(n);
+    d.ncount = h->mem.alloc<int>(n);
+    d.nreq = h->mem.alloc<int>(1);
+    d.red_blocks = std::max((n + 255) / 256, 3 * 1024);
+    d.red = h->mem.alloc<double>(d.red_blocks);
+    d.flags = h->mem.alloc<unsigned int>(8);
+    if (!d.frc || !d.epot || !d.cell_key || !d.cell_count || !d.cell_first || !d.cell_fill || !d.cell_list ||
+        !d.ncount || !d.nreq || !d.red || !d.flags || (!h->fast && !d.cell_xyz))
+        return h->fail(MTMD_ENOMEM, "device allocation failed");
+    HIPCHK(h, hipMemset(d.frc, 0, n * sizeof(double4)));
+    HIPCHK(h, hipMemset(d.epot, 0, n * sizeof(float)));
+    HIPCHK(h, hipMemset(d.flags, 0, 8 * sizeof(unsigned int)));
+    // neighbour capacity: the expected full count at the list radius plus margin
+    {
+        const double V = p.L[0] * p.L[1] * p.L[2];
+        const double rl = h->fast ? (double)p.rlist : c.cutoff_radius;
+        const double nexp = (double)n / V * (4.0 / 3.0) * M_PI * rl * rl * rl;
+        d.ncap = std::max(16, (int)std::ceil(nexp * 1.35) + 16);
+        if (!h->fast && (d.nc[0] < 3 || d.nc[1] < 3 || d.nc[2] < 3)) d.ncap *= 2;
+        d.ncap = std::min(d.ncap, n * 27 + 16);
+        d.nlist = h->mem.alloc<uint32_t>((size_t)d.ncap * n);
+        if (!d.nlist) return h->fail(MTMD_ENOMEM, "neighbour list allocation failed");
+    }
+
+    // --- thermostat (SimulationBox.cpp:53-79, NHChain.cpp:32-52,115-143) ---
+    h->M = (c.ensemble == MTMD_NVT) ? c.temperature_nh_len : 0;
+    h->nresp = c.temperature_respa;
+    h->Tred = c.temperature / MT_CONV_T;
+    h->tau = c.temperature_relax / MT_CONV_t;
+    h->g = (double)(3 * n);
+    {
+        const double w0 = 1.0 / (1.0 - std::pow(2.0, 1.0 / 3.0));   // NHChain.cpp:45-52
+        h->w[0] = w0; h->w[1] = 1.0 - 2.0 * w0; h->w[2] = w0;
+    }
+    const int M = std::max(h->M, 1);
+    std::vector<double> nh(3 * M + 2, 0.0);
+    if (h->M > 0) {
+        nh[2 * M] = (double)(3 * n) * h->Tred * h->tau * h->tau;
+        for (int j = 1; j < M; j++) nh[2 * M + j] = h->Tred * h->tau * h->tau;
+        // CNHChain::setRandNHPos / setRandNHMom (NHChain.cpp:126-143) from the seeded generator
+        uint64_t st = s->seed ^ 0x5DEECE66DULL;
+        for (int j = 0; j < M; j++) nh[j] = urand(st);
+        for (int j = 0; j < M; j++) {
+            double v = (2.0 * urand(st) - 1.0) * (nh[2 * M + j] / h->tau);
+            nh[M + j] = (v == 0.0) ? (nh[2 * M + j] / h->tau) : v;
+        }
+    }
+    nh[3 * M] = 1.0;
+    h->nh_scale_slot = 3 * M;
+    if ((rc2 = upload(h, d.nh, nh))) return rc2;
+    h->h_status = nullptr;
+    HIPCHK(h, hipHostMalloc((void**)&h->h_status, 16 * sizeof(int), hipHostMallocDefault));
+    return MTMD_OK;
+}
+
+extern "C" int mtmd_create(const mtmd_system* sys, const mtmd_config* cfg, mtmd_handle** out)
+{
+    g_create_error.clear();
+    if (!out) { g_create_error = "out is NULL"; return MTMD_EINVAL; }
+    *out = nullptr;
+    if (!sys || !cfg) { g_create_error = "system and config are required"; return MTMD_EINVAL; }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        g_create_error = "no HIP device";
+        return MTMD_ENODEV;
+    }
+    mtmd_handle* h = new mtmd_handle();
+    h->cfg = *cfg;
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+        g_create_error = "hipStreamCreate failed";
+        delete h;
+        return MTMD_EHIP;
+    }
+    int rc = setup(h, sys);
+    if (rc != MTMD_OK) {
+        g_create_error = h->err;
+        mtmd_destroy(h);
+        return rc;
+    }
+    *out = h;
+    return MTMD_OK;
+}
+
+extern "C" void mtmd_destroy(mtmd_handle* h)
+{
+    if (!h) return;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    h->mem.release();
+    if (h->h_status) (void)hipHostFree(h->h_status);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+}
+
+static void swap_slots(MtDev& d);
+
+// ---------------------------------------------------------------------------
+// state transfer (positions/momenta in original atom order)
+// ---------------------------------------------------------------------------
+static int download_perm(mtmd_handle* h)
+{
+    h->perm_host.resize(h->n);
+    if (!h->fast) { std::iota(h->perm_host.begin(), h->perm_host.end(), 0); return MTMD_OK; }
+    HIPCHK(h, hipMemcpyAsync(h->perm_host.data(), h->d.perm, h->n * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return MTMD_OK;
+}
+
+static int get_d4(mtmd_handle* h, const double4* src, double* out, bool xyz_only = true)
+{
+    std::vector<double4> tmp(h->n);
+    HIPCHK(h, hipMemcpyAsync(tmp.data(), src, h->n * sizeof(double4), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    int rc = download_perm(h);
+    if (rc) return rc;
+    (void)xyz_only;
+    for (int s = 0; s < h->n; s++) {
+        const int o = h->perm_host[s];
+        out[3 * o] = tmp[s].x; out[3 * o + 1] = tmp[s].y; out[3 * o + 2] = tmp[s].z;
+    }
+    return MTMD_OK;
+}
+
+// bring the FP64/MIXED engines back to the original atom order
+static int unsort(mtmd_handle* h)
+{
+    if (!h->fast) return MTMD_OK;
+    mt_fast_unsort(h->d, h->stream);
+    swap_slots(h->d);
+    mt_fast_identity_perm(h->d, h->stream);
+    HIPCHK(h, hipGetLastError());
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    h->need_sort = true;
+    h->ke_valid = false;
+    return MTMD_OK;
+}
+
+extern "C" int mtmd_set_state(mtmd_handle* h, const double* pos, const double* mom)
+{
+    if (!h) return MTMD_EINVAL;
+    int rc = unsort(h);
+    if (rc) return rc;
+    const int n = h->n;
+    std::vector<double4> tmp(n);
+    if (pos) {
+        HIPCHK(h, hipMemcpy(tmp.data(), h->d.pos, n * sizeof(double4), hipMemcpyDeviceToHost));
+        for (int k = 0; k < n; k++) { tmp[k].x = pos[3 * k]; tmp[k].y = pos[3 * k + 1]; tmp[k].z = pos[3 * k + 2]; }
+        HIPCHK(h, hipMemcpy(h->d.pos, tmp.data(), n * sizeof(double4), hipMemcpyHostToDevice));
+        h->have_forces = false;
+    }
+    if (mom) {
+        HIPCHK(h, hipMemcpy(tmp.data(), h->d.mom, n * sizeof(double4), hipMemcpyDeviceToHost));
+        for (int k = 0; k < n; k++) { tmp[k].x = mom[3 * k]; tmp[k].y = mom[3 * k + 1]; tmp[k].z = mom[3 * k + 2]; }
+        HIPCHK(h, hipMemcpy(h->d.mom, tmp.data(), n * sizeof(double4), hipMemcpyHostToDevice));
+        h->ke_valid = false;
+    }
+    return MTMD_OK;
+}
+
+extern "C" int mtmd_get_state(mtmd_handle* h, double* pos, double* mom)
+{
+    if (!h) return MTMD_EINVAL;
+    int rc;
+    if (pos && (rc = get_d4(h, h->d.pos, pos))) return rc;
+    if (mom && (rc = get_d4(h, h->d.mom, mom))) return rc;
+    return MTMD_OK;
+}
+
+extern "C" int mtmd_set_thermostat(mtmd_handle* h, const double* eta, const double* p_eta)
+{
+    if (!h) return MTMD_EINVAL;
+    if (h->M == 0) return MTMD_OK;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (eta) HIPCHK(h, hipMemcpy(h->d.nh, eta, h->M * sizeof(double), hipMemcpyHostToDevice));
+    if (p_eta) HIPCHK(h, hipMemcpy(h->d.nh + h->M, p_eta, h->M * sizeof(double), hipMemcpyHostToDevice));
+    return MTMD_OK;
+}
+
+extern "C" int mtmd_get_thermostat(mtmd_handle* h, double* eta, double* p_eta)
+{
+    if (!h) return MTMD_EINVAL;
+    if (h->M == 0) return MTMD_OK;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (eta) HIPCHK(h, hipMemcpy(eta, h->d.nh, h->M * sizeof(double), hipMemcpyDeviceToHost));
+    if (p_eta) HIPCHK(h, hipMemcpy(p_eta, h->d.nh + h->M, h->M * sizeof(double), hipMemcpyDeviceToHost));
+    return MTMD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// force evaluation
+// ---------------------------------------------------------------------------
+static int grow_neighbors(mtmd_handle* h, int need)
+{
+    MtDev& d = h->d;
+    int cap = need + need / 8 + 16;
+    void* old = d.nlist;
+    uint32_t* nl = nullptr;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (hipMalloc((void**)&nl, (size_t)cap * h->n * sizeof(uint32_t)) != hipSuccess)
+        return h->fail(MTMD_ENOMEM, "neighbour list of %d rows does not fit", cap);
+    for (auto& p : h->mem.ptrs) if (p == old) { (void)hipFree(p); p = nl; }
+    d.nlist = nl;
+    d.ncap = cap;
+    return MTMD_OK;
+}
+
+// EXACT: CMDFFMatrices::genNeighList + the force functor, every call
+static int exact_forces(mtmd_handle* h)
+{
+    MtDev& d = h->d;
+    mt_exact_cells(d, h->p, h->stream);
+    for (int attempt = 0; attempt < 4; attempt++) {
+        HIPCHK(h, hipMemsetAsync(d.nreq, 0, sizeof(int), h->stream));
+        mt_exact_neighbors(d, h->p, h->stream);
+        HIPCHK(h, hipMemcpyAsync(h->h_status, d.nreq, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        const int need = h->h_status[0];
+        if (need <= d.ncap) break;
+        int rc = grow_neighbors(h, need);
+        if (rc) return rc;
+        if (attempt == 3) return h->fail(MTMD_ENEIGH, "neighbour list keeps overflowing");
+    }
+    mt_exact_forces(d, h->p, h->stream);
+    HIPCHK(h, hipGetLastError());
+    h->rebuilds++;
+    return MTMD_OK;
+}
+
+static void swap_slots(MtDev& d)
+{
+    std::swap(d.pos, d.pos2); std::swap(d.mom, d.mom2); std::swap(d.frc, d.frc2); std::swap(d.epot, d.epot2);
+    std::swap(d.type, d.type2); std::swap(d.mol, d.mol2); std::swap(d.mobile, d.mobile2); std::swap(d.qf, d.qf2);
+    std::swap(d.perm, d.perm2);
+}
+
+// FP64/MIXED: full rebuild (optionally with spatial re-sort) or conditional
+static int fast_rebuild(mtmd_handle* h, bool force_rebuild, bool sort)
+{
+    MtDev& d = h->d;
+    const int cond = force_rebuild ? 0 : 1;
+    if (sort) {
+        // spatial re-sort: order atoms by cell (ascending original slot inside a cell)
+        mt_fast_cells(d, h->p, 0, h->stream);
+        mt_fast_sort_apply(d, h->stream);
+        swap_slots(d);
+        h->steps_since_sort = 0;
+        h->need_sort = false;
+    }
+    mt_fast_cells(d, h->p, cond, h->stream);
+    mt_fast_home(d, h->p, cond, h->stream);
+    mt_fast_neighbors(d, h->p, cond, h->stream);
+    HIPCHK(h, hipGetLastError());
+    return MTMD_OK;
+}
+
+static int fast_check_capacity(mtmd_handle* h, bool* overflowed)
+{
+    MtDev& d = h->d;
+    HIPCHK(h, hipMemcpyAsync(h->h_status, d.flags, 8 * sizeof(unsigned int), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    *overflowed = h->h_status[2] != 0;
+    h->rebuilds = (int64_t)(unsigned)h->h_status[4];
+    return MTMD_OK;
+}
+
+static int fast_forces(mtmd_handle* h, bool want_energy)
+{
+    mt_fast_forces(h->d, h->p, h->engine == MTMD_ENGINE_MIXED, want_energy ? 1 : 0, h->cfg.block_size, h->stream);
+    HIPCHK(h, hipGetLastError());
+    return MTMD_OK;
+}
+
+// a full, unconditional list build (with re-sort) and forces of the current positions
+static int fast_full_forces(mtmd_handle* h)
+{
+    MtDev& d = h->d;
+    int rc = fast_rebuild(h, true, true);
+    if (rc) return rc;
+    bool ov = false;
+    if ((rc = fast_check_capacity(h, &ov))) return rc;
+    if (ov) {
+        // count pass told us the largest row (nreq); regrow and rebuild
+        HIPCHK(h, hipMemcpy(h->h_status, d.nreq, sizeof(int), hipMemcpyDeviceToHost));
+        if ((rc = grow_neighbors(h, h->h_status[0]))) return rc;
+        HIPCHK(h, hipMemsetAsync(d.flags + 2, 0, sizeof(unsigned int), h->stream));
+        if ((rc = fast_rebuild(h, true, false))) return rc;
+        if ((rc = fast_check_capacity(h, &ov))) return rc;
+        if (ov) return h->fail(MTMD_ENEIGH, "neighbour list overflow after regrowth");
+    }
+    return fast_forces(h, true);
+}
+
+static int compute_forces_now(mtmd_handle* h)
+{
+    int rc = h->fast ? fast_full_forces(h) : exact_forces(h);
+    if (rc) return rc;
+    h->have_forces = true;
+    return MTMD_OK;
+}
+
+static int get_forces_host(mtmd_handle* h, double* F, double* Fpi, float* U)
+{
+    const int n = h->n;
+    int rc;
+    std::vector<double> tmp(3 * (size_t)n);
+    if (F || Fpi) {
+        if ((rc = get_d4(h, h->d.frc, tmp.data()))) return rc;
+        if (F) std::memcpy(F, tmp.data(), tmp.size() * sizeof(double));
+        if (Fpi) std::memcpy(Fpi, tmp.data(), tmp.size() * sizeof(double));
+    }
+    if (U) {
+        std::vector<float> u(n);
+        HIPCHK(h, hipMemcpyAsync(u.data(), h->d.epot, n * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        if ((rc = download_perm(h))) return rc;
+        for (int s = 0; s < n; s++) U[h->perm_host[s]] = u[s];
+    }
+    return MTMD_OK;
+}
+
+extern "C" int mtmd_compute_forces(mtmd_handle* h, double* F, double* Fpi, float* U)
+{
+    if (!h) return MTMD_EINVAL;
+    int rc = compute_forces_now(h);
+    if (rc) return rc;
+    return get_forces_host(h, F, Fpi, U);
+}
+
+extern "C" int mtmd_get_forces(mtmd_handle* h, double* F, float* U)
+{
+    if (!h) return MTMD_EINVAL;
+    if (!h->have_forces) {
+        int rc = compute_forces_now(h);
+        if (rc) return rc;
+    }
+    return get_forces_host(h, F, nullptr, U);
+}
+
+extern "C" int mtmd_build_cells(mtmd_handle* h, int32_t* ncxyz, int32_t* ixyz, int32_t* cfirst, int32_t* ccount,
+                                int32_t* clist)
+{
+    if (!h) return MTMD_EINVAL;
+    if (h->fast) return h->fail(MTMD_EINVAL, "reference cell lists are produced by the exact engine");
+    MtDev& d = h->d;
+    mt_exact_cells(d, h->p, h->stream);
+    HIPCHK(h, hipGetLastError());
+    if (ncxyz) { ncxyz[0] = d.nc[0]; ncxyz[1] = d.nc[1]; ncxyz[2] = d.nc[2]; }
+    if (ixyz) HIPCHK(h, hipMemcpyAsync(ixyz, d.cell_xyz, 3 * (size_t)h->n * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    if (cfirst) HIPCHK(h, hipMemcpyAsync(cfirst, d.cell_first, d.ncells * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    if (ccount) HIPCHK(h, hipMemcpyAsync(ccount, d.cell_count, d.ncells * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    if (clist) HIPCHK(h, hipMemcpyAsync(clist, d.cell_list, h->n * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return MTMD_OK;
+}
+
+extern "C" int mtmd_build_neighbors(mtmd_handle* h, int32_t max_per_atom, int32_t* count, int32_t* list,
+                                    int32_t* required)
+{
+    if (!h) return MTMD_EINVAL;
+    if (h->fast) return h->fail(MTMD_EINVAL, "reference neighbour lists are produced by the exact engine");
+    MtDev& d = h->d;
+    mt_exact_cells(d, h->p, h->stream);
+    for (int attempt = 0; attempt < 4; attempt++) {
+        HIPCHK(h, hipMemsetAsync(d.nreq, 0, sizeof(int), h->stream));
+        mt_exact_neighbors(d, h->p, h->stream);
+        HIPCHK(h, hipMemcpyAsync(h->h_status, d.nreq, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        if (h->h_status[0] <= d.ncap) break;
+        int rc = grow_neighbors(h, h->h_status[0]);
+        if (rc) return rc;
+    }
+    std::vector<int> cnt(h->n);
+    HIPCHK(h, hipMemcpy(cnt.data(), d.ncount, h->n * sizeof(int), hipMemcpyDeviceToHost));
+    int mx = 0;
+    for (int v : cnt) mx = std::max(mx, v);
+    if (required) *required = mx;
+    if (count) std::memcpy(count, cnt.data(), h->n * sizeof(int));
+    if (list && max_per_atom > 0) {
+        std::vector<uint32_t> nl((size_t)d.ncap * h->n);
+        HIPCHK(h, hipMemcpy(nl.data(), d.nlist, nl.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        for (int k = 0; k < h->n; k++) {
+            const int c = std::min(cnt[k], max_per_atom);
+            for (int j = 0; j < c; j++) list[(size_t)k * max_per_atom + j] = (int32_t)nl[(size_t)j * h->n + k];
+        }
+    }
+    h->have_forces = false;
+    return (mx > max_per_atom && list) ? h->fail(MTMD_ENEIGH, "row of %d neighbours exceeds %d", mx, max_per_atom)
+                                       : MTMD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// MD loop (MDLoop.cpp:124-138): NH_T, velocity Verlet, NH_T, wrap
+// ---------------------------------------------------------------------------
+static int md_steps(mtmd_handle* h, int nsteps, bool energy_last)
+{
+    MtDev& d = h->d;
+    const bool nvt = (h->M > 0);
+    const int nblocks = (h->n + 255) / 256;
+    int rc;
+    if (!h->have_forces) {                              // calcInitialForces (MDLoop.cpp:189-192)
+        if ((rc = compute_forces_now(h))) return rc;
+    }
+    if (h->fast && h->need_sort) {
+        // re-sort now so the step loop runs on spatially ordered atoms
+        if ((rc = fast_full_forces(h))) return rc;
+    }
+    for (int t = 0; t < nsteps; t++) {
+        const bool last = (t == nsteps - 1);
+        if (nvt) {
+            if (!h->ke_valid) mt_reduce_ke(d, h->stream);
+            mt_nh_chain(d, h->M, h->nresp, h->Tred, h->p.dt, h->g, h->w, nblocks, h->stream);
+        }
+        if (!h->fast) {
+            mt_kick_drift(d, h->p, nvt ? h->nh_scale_slot : -1, 0, h->stream);
+            if ((rc = exact_forces(h))) return rc;
+        } else {
+            const bool sort_now = (++h->steps_since_sort >= h->sort_every);
+            mt_fast_clear_flags(d, h->stream);
+            mt_fast_kick_drift(d, h->p, nvt ? h->nh_scale_slot : -1, h->engine == MTMD_ENGINE_MIXED, h->stream);
+            if ((rc = fast_rebuild(h, sort_now, sort_now))) return rc;
+            if ((rc = fast_forces(h, last && energy_last))) return rc;
+        }
+        mt_cap_kick(d, h->p, nvt ? 1 : 0, h->stream);
+        if (nvt) mt_nh_chain(d, h->M, h->nresp, h->Tred, h->p.dt, h->g, h->w, nblocks, h->stream);
+        mt_scale_wrap(d, h->p, nvt ? h->nh_scale_slot : -1, nvt ? 1 : 0, h->stream);
+        h->ke_valid = nvt;
+        h->steps++;
+    }
+    HIPCHK(h, hipGetLastError());
+    if (h->fast) {
+        bool ov = false;
+        if ((rc = fast_check_capacity(h, &ov))) return rc;
+        if (ov) {
+            HIPCHK(h, hipMemcpy(h->h_status, d.nreq, sizeof(int), hipMemcpyDeviceToHost));
+            rc = grow_neighbors(h, h->h_status[0]);
+            h->have_forces = false;
+            if (rc) return rc;
+            return h->fail(MTMD_ENEIGH, "neighbour list overflowed during these steps (capacity now %d); the "
+                                        "state is not reliable, restart from a saved state", d.ncap);
+        }
+    }
+    return MTMD_OK;
+}
+
+extern "C" int mtmd_step(mtmd_handle* h, int32_t nsteps)
+{
+    if (!h || nsteps < 0) return MTMD_EINVAL;
+    if (nsteps == 0) return MTMD_OK;
+    return md_steps(h, nsteps, true);
+}
+
+extern "C" int mtmd_get_thermo(mtmd_handle* h, mtmd_thermo* out)
+{
+    if (!h || !out) return MTMD_EINVAL;
+    if (!h->have_forces) {
+        int rc = compute_forces_now(h);
+        if (rc) return rc;
+    }
+    MtDev& d = h->d;
+    mt_reduce_thermo(d, h->stream);
+    const int nb = mt_reduce_blocks(h->n);
+    std::vector<double> part(3 * (size_t)nb);
+    HIPCHK(h, hipMemcpyAsync(part.data(), d.red, part.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    h->ke_valid = false;   // d.red now holds thermo partials
+    double sp = 0, sf = 0, su = 0;
+    for (int b = 0; b < nb; b++) { sp += part[3 * b]; sf += part[3 * b + 1]; su += part[3 * b + 2]; }
+    const double V = h->p.L[0] * h->p.L[1] * h->p.L[2];
+    out->sum_p2m = sp;
+    out->sum_fr = sf;
+    out->T = sp / (3 * (double)h->n) * MT_CONV_T;
+    out->P = (1.0 / (3.0 * V)) * (sp + sf) * MT_CONV_PRESS;
+    out->V = V;
+    out->U = su;
+    out->K = 0.5 * sp;
+    out->Etot = su + 0.5 * sp;
+    return MTMD_OK;
+}
+
+extern "C" int mtmd_optimize(mtmd_handle* h, int32_t max_steps, double lr, double accuracy, int32_t* steps_done)
+{
+    if (!h) return MTMD_EINVAL;
+    int rc;
+    auto utot = [&](double& u) -> int {
+        mt_reduce_thermo(h->d, h->stream);
+        const int nb = mt_reduce_blocks(h->n);
+        std::vector<double> part(3 * (size_t)nb);
+        HIPCHK(h, hipMemcpyAsync(part.data(), h->d.red, part.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        u = 0;
+        for (int b = 0; b < nb; b++) u += part[3 * b + 2];
+        return MTMD_OK;
+    };
+    if ((rc = compute_forces_now(h))) return rc;
+    double Un = 0.0;
+    if ((rc = utot(Un))) return rc;
+    int i = 0;
+    for (i = 0; i < max_steps; i++) {
+        if ((rc = compute_forces_now(h))) return rc;
+        mt_gradient_step(h->d, lr, h->stream);
+        double Unp1 = 0.0;
+        if ((rc = utot(Unp1))) return rc;
+        const double dU = std::fabs(Unp1 - Un);
+        Un = Unp1;
+        if (dU < accuracy && i > h->cfg.out_stride) { i++; break; }
+    }
+    h->have_forces = false;
+    h->ke_valid = false;
+    if (steps_done) *steps_done = i;
+    return MTMD_OK;
+}
+
+extern "C" int mtmd_get_stats(mtmd_handle* h, int64_t* rebuilds, int64_t* steps, int32_t* max_neighbors)
+{
+    if (!h) return MTMD_EINVAL;
+    if (h->fast) {
+        bool ov;
+        int rc = fast_check_capacity(h, &ov);
+        if (rc) return rc;
+    }
+    if (rebuilds) *rebuilds = h->rebuilds;
+    if (steps) *steps = h->steps;
+    if (max_neighbors) *max_neighbors = h->d.ncap;
+    return MTMD_OK;
+}
+
+extern "C" int mtmd_synchronize(mtmd_handle* h)
+{
+    if (!h) return MTMD_EINVAL;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return MTMD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// outer seam: CMDSimulator::run / optimize (Simulator/MDSimulator.cpp:35-66)
+// with the thermo table of CMDLoop::printHeading/updateOutput (MDLoop.cpp:194-227,370-405)
+// ---------------------------------------------------------------------------
+static void tee(FILE* a, FILE* b, const char* fmt, ...)
+{
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (a) std::fputs(buf, a);
+    if (b) std::fputs(buf, b);
+}
+
+static int run_common(const mtmd_config* cfg, const mtmd_system* sys, void* log_file, bool optimize)
+{
+    mtmd_handle* h = nullptr;
+    int rc = mtmd_create(sys, cfg, &h);
+    if (rc) return rc;
+    FILE* info = cfg->info_file[0] ? std::fopen(cfg->info_file, "w") : nullptr;
+    FILE* log = (FILE*)log_file;
+    const int stride = std::max(1, cfg->out_stride);
+    const char* ens = (cfg->ensemble == MTMD_NVT) ? "NVT" : "NVE";
+    tee(info, log, "\r\n\t----------------------------\r\n\t%s\r\n\t----------------------------\r\n",
+        optimize ? "Optimization" : "Molecular dynamics simulation");
+    if (!optimize) {
+        tee(info, log, "\t Ensemble = %s\r\n", ens);
+        if (cfg->ensemble == MTMD_NVT) tee(info, log, "\t Temperature, T = %g K\r\n", cfg->temperature);
+        if (cfg->ensemble == MTMD_NVT) tee(info, log, "\t NH T relax, tau = %g fs\r\n", cfg->temperature_relax);
+        tee(info, log, "\t Timestep, dt = %g fs\r\n", cfg->timestep);
+        tee(info, log, "\t Particles, N = %i\r\n", sys->n_atoms);
+        tee(info, log, "\t NH RESPA, n = %i\r\n", cfg->temperature_respa);
+        tee(info, log, "\t NH Chain length, M = %i\r\n", cfg->temperature_nh_len);
+        tee(info, log, "\t Init. box length, Lx = %g, Ly = %g, Lz = %g AA\r\n", sys->box[0], sys->box[1], sys->box[2]);
+        tee(info, log, "\t Dimension, d = %i\r\n", 3);
+    } else {
+        tee(info, log, "\t Learning rate, L = %g\r\n", cfg->opt_learning_rate);
+    }
+    tee(info, log, "\t----------------------------\r\n\r\n");
+    tee(info, log, "\t%-15s%-17s%-15s%-15s%-20s%-15s%-15s%-15s\r\n", optimize ? "Step" : "Timestep", "Avg.steptime[s]",
+        "Temp[K]", "Press[atm]", "Vol[AA^3]", "U[kJ/mol]", "K[kJ/mol]", "Etot[kJ/mol]");
+    auto line = [&](int t, double avg) {
+        mtmd_thermo th{};
+        int r = mtmd_get_thermo(h, &th);
+        if (r) return r;
+        // the reference passes (avg, t) to a "%-15i%-17.8f" format (MDLoop.cpp:402); the
+        // intended columns are printed here
+        tee(info, log, "\t%-15i%-17.8f%-15g%-15g%-20g%-15g%-15g%-15g\r\n", t, avg, th.T, th.P, th.V, th.U, th.K, th.Etot);
+        return 0;
+    };
+    if (!optimize) {
+        const int total = cfg->n_steps;
+        int done = 0;
+        while (done < total && !rc) {
+            // steps t = done .. done+chunk-1; output after step t when t % stride == 0
+            int next = (done % stride == 0) ? done : done + (stride - done % stride);
+            int chunk = std::min(next, total - 1) - done + 1;
+            auto t0 = std::chrono::steady_clock::now();
+            rc = md_steps(h, chunk, true);
+            if (rc) break;
+            (void)mtmd_synchronize(h);
+            double avg = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() / chunk;
+            done += chunk;
+            if ((done - 1) % stride == 0) rc = line(done - 1, avg);
+        }
+    } else {
+        int steps = 0;
+        rc = mtmd_optimize(h, cfg->opt_max_steps, cfg->opt_learning_rate, cfg->opt_accuracy, &steps);
+        if (!rc) rc = line(steps, 0.0);
+    }
+    if (rc && log) std::fprintf(log, "Error: %s\r\n", h->err.c_str());
+    if (info) std::fclose(info);
+    mtmd_destroy(h);
+    return rc;
+}
+
+extern "C" int mtmd_run(const mtmd_config* cfg, const mtmd_system* sys, void* log_file)
+{
+    if (!cfg || !sys) return MTMD_EINVAL;
+    return run_common(cfg, sys, log_file, false);
+}
+
+extern "C" int mtmd_run_optimize(const mtmd_config* cfg, const mtmd_system* sys, void* log_file)
+{
+    if (!cfg || !sys) return MTMD_EINVAL;
+    return run_common(cfg, sys, log_file, true);
+}

@@ -1,0 +1,122 @@
+// mtmd_internal.h - device data layout and kernel launchers of libmtmd.so.
+//
+// Layout in HBM (structure of arrays, one entry per atom unless noted):
+//   pos    double4  x, y, z, q            (q: the atom's float charge, widened)
+//   mom    double4  px, py, pz, m
+//   frc    double4  Fx, Fy, Fz, 0         (F == Fpi in every engine)
+//   epot   float    per-atom potential energy
+//   type, mol int32; mobile uint8
+//   posf   float4   xl, yl, zl, type-bits (MIXED: position relative to the
+//                   centre of the atom's home cell, see k_refresh_local)
+//   home   int32    home cell of the atom at the last neighbour rebuild
+// Tables (float4 per point, 16-byte aligned):
+//   nbtab  [((row*T + col)*MAXFF + ff)*NPTS + pt]  x, -dU/dx, U, 0
+//   nbcnt  [row*T + col]
+//   btab/atab/dtab [type*NPTS + pt]
+//   nbcoef [(row*T + col)*NPTS + pt]   a_f, b_f, a_e, b_e of the segment
+//          starting at pt, summed over the pair's force fields (MIXED)
+// Neighbour lists are column-major: nl[j*ncap_stride + k], so the lanes of a
+// wavefront (consecutive atoms) read consecutive words at every j.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define MT_NPTS 1000
+#define MT_MAXFF 5
+#define MT_COULOMB_K 1389.32f
+#define MT_CONV_T 120.272
+#define MT_CONV_t 1.0E2
+#define MT_CONV_PRESS 16388.2461
+
+// neighbour word of the FP64/MIXED lists: atom index in the low 27 bits, the
+// home-cell offset code (dx+1) + 3*(dy+1) + 9*(dz+1) in the top 5 bits
+#define MT_NW_IDX_BITS 27
+#define MT_NW_IDX_MASK ((1u << MT_NW_IDX_BITS) - 1u)
+
+struct MtBonded {
+    // per-atom CSR in atom (original) index order, reference order inside a row
+    int* b_first; int* b_num; int* b_j; int* b_type;
+    int* a_first; int* a_num; int* a_i; int* a_j; int* a_type; uint8_t* a_center;
+    int* d_first; int* d_num; int* d_i; int* d_j; int* d_l; int* d_type; uint8_t* d_center;
+    int has_bonds, has_angles, has_dihedrals;
+};
+
+struct MtSpecial {
+    // pairs evaluated outside the neighbour list by the FP64/MIXED engines:
+    // per atom (original index) CSR of (partner original index, scale)
+    int* first; int* num; int* partner; float* scale;
+    int n_entries;
+};
+
+struct MtDev {
+    int n, T;
+    double4* pos; double4* mom; double4* frc; float* epot;
+    int* type; int* mol; uint8_t* mobile; uint8_t* xmol;  // xmol: bonded partner outside own molecule
+    float4* nbtab; int* nbcnt; float4* btab; float4* atab; float4* dtab;
+    float4* nbcoef;
+    MtBonded bl;
+    MtSpecial sp;
+    // cells
+    int nc[3]; int ncells;
+    int* cell_key; int* cell_count; int* cell_first; int* cell_fill; int* cell_list;
+    int* cell_xyz;         // EXACT: ix, iy, iz per atom
+    // neighbours
+    int ncap; int* ncount; uint32_t* nlist; int* nreq;
+    // sorting (FP64/MIXED): perm[s] = original index of sorted slot s, inv[o] = slot
+    int* perm; int* inv;
+    int* perm2;
+    double4* pos2; double4* mom2; double4* frc2; float* epot2; int* type2; int* mol2; uint8_t* mobile2;
+    float* qf; float* qf2;  // float charges in slot order
+    double4* posref;       // positions at the last rebuild
+    float4* posf; int* home;
+    // reductions / scalars
+    double* red;           // block partials
+    int red_blocks;
+    double* nh;            // NH chain state: eta[M], p_eta[M], Q[M], scale, KE2
+    unsigned int* flags;   // [0] max displacement^2 (float bits) of this step, [2] list overflow,
+                           // [4] rebuild count
+};
+
+struct MtParams {
+    float Lf[3];           // float(L): box widths as the reference's functors see them
+    double L[3];           // double box (integrator, wrap)
+    double wbox[3];        // force min-image widths: double(Lf)/2 - (-double(Lf)/2)
+    double wnl[3];         // neighbour-search widths: double(low + Lf) - double(low)
+    float lowf[3];         // -Lf/2
+    float rc, rc2f;        // float cutoff and its float square
+    float rlist, rlist2;   // FP64/MIXED list radius
+    float scale12, scale13, scale14, scale1N;
+    int coulomb_allpairs, coulomb_on;
+    double dt, Fcut;
+    float x0, inv_delta;   // non-bonded table start and 1/spacing (MIXED)
+    float cs[3];           // cell size per axis (MIXED local coordinates)
+    float half_skin2;      // rebuild threshold on the squared displacement
+};
+
+// ---- launchers (mtmd_exact.hip) ----
+void mt_exact_cells(const MtDev& d, const MtParams& p, hipStream_t s);
+void mt_exact_neighbors(const MtDev& d, const MtParams& p, hipStream_t s);
+void mt_exact_forces(const MtDev& d, const MtParams& p, hipStream_t s);
+void mt_kick_drift(const MtDev& d, const MtParams& p, int scale_slot, int track_disp, hipStream_t s);
+void mt_cap_kick(const MtDev& d, const MtParams& p, int want_ke, hipStream_t s);
+void mt_nh_chain(const MtDev& d, int M, int nresp, double T, double dt, double g, const double* w,
+                 int nblocks, hipStream_t s);
+void mt_scale_wrap(const MtDev& d, const MtParams& p, int scale_slot, int want_ke, hipStream_t s);
+void mt_reduce_ke(const MtDev& d, hipStream_t s);
+void mt_reduce_thermo(const MtDev& d, hipStream_t s);
+void mt_gradient_step(const MtDev& d, double lr, hipStream_t s);
+
+// ---- launchers (mtmd_fast.hip) ----
+// `conditional` kernels return at once unless this step's maximum
+// displacement since the last rebuild exceeds half the skin (flags[0] vs p.half_skin2)
+void mt_fast_cells(const MtDev& d, const MtParams& p, int conditional, hipStream_t s);
+void mt_fast_sort_apply(const MtDev& d, hipStream_t s);   // writes the *2 buffers, perm2 and inv
+void mt_fast_unsort(const MtDev& d, hipStream_t s);       // writes the *2 buffers in original order
+void mt_fast_identity_perm(const MtDev& d, hipStream_t s);
+void mt_fast_home(const MtDev& d, const MtParams& p, int conditional, hipStream_t s);
+void mt_fast_neighbors(const MtDev& d, const MtParams& p, int conditional, hipStream_t s);
+void mt_fast_forces(const MtDev& d, const MtParams& p, int mixed, int want_energy, int block, hipStream_t s);
+void mt_fast_kick_drift(const MtDev& d, const MtParams& p, int scale_slot, int mixed, hipStream_t s);
+void mt_fast_clear_flags(const MtDev& d, hipStream_t s);
+
+int mt_reduce_blocks(int n);

@@ -1,0 +1,143 @@
+"""Python handle over libmtmd.so (the HIP engine), mirroring the reference's
+CSimulationBox/CVelVerlet/CMDLoop operations on one system.
+
+Every call goes through the C ABI of include/mtmd.h; a non-zero status raises
+MtmdError with the library's message. There is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+from .system import System
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+_fp = C.POINTER(C.c_float)
+
+
+class MtmdError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"mtmd status {status}: {msg}")
+        self.status = status
+
+
+def _f64(a):
+    return None if a is None else a.ctypes.data_as(_dp)
+
+
+def _i32(a):
+    return None if a is None else a.ctypes.data_as(_ip)
+
+
+def _f32(a):
+    return None if a is None else a.ctypes.data_as(_fp)
+
+
+def profile(which: int, kind: int, params, x0: float, x1: float, n: int = 1000):
+    """CMDFF*::calc1DForceProfile/calc1DPotentialProfile through mtmd_profile."""
+    lib = abi.load_library()
+    p = np.zeros(4)
+    p[:len(params)] = params
+    x, f, e = (np.zeros(n, np.float32) for _ in range(3))
+    rc = lib.mtmd_profile(which, kind, _f64(p), x0, x1, n, _f32(x), _f32(f), _f32(e))
+    if rc:
+        raise MtmdError(rc, "mtmd_profile")
+    return x, f, e
+
+
+class Engine:
+    def __init__(self, system: System, cfg: abi.MtmdConfig):
+        self.lib = abi.load_library()
+        self.system = system
+        self.cfg = cfg
+        self.n = system.n_atoms
+        self._csys = system.to_c()
+        h = C.c_void_p()
+        rc = self.lib.mtmd_create(C.byref(self._csys), C.byref(cfg), C.byref(h))
+        if rc:
+            raise MtmdError(rc, self.lib.mtmd_last_error(None).decode())
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.mtmd_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _chk(self, rc):
+        if rc:
+            raise MtmdError(rc, self.lib.mtmd_last_error(self.h).decode())
+
+    def set_state(self, pos=None, mom=None):
+        pos = None if pos is None else np.ascontiguousarray(pos, np.float64)
+        mom = None if mom is None else np.ascontiguousarray(mom, np.float64)
+        self._chk(self.lib.mtmd_set_state(self.h, _f64(pos), _f64(mom)))
+
+    def get_state(self):
+        pos, mom = np.zeros((self.n, 3)), np.zeros((self.n, 3))
+        self._chk(self.lib.mtmd_get_state(self.h, _f64(pos), _f64(mom)))
+        return pos, mom
+
+    def set_thermostat(self, eta, p_eta):
+        eta, p_eta = np.ascontiguousarray(eta, np.float64), np.ascontiguousarray(p_eta, np.float64)
+        self._chk(self.lib.mtmd_set_thermostat(self.h, _f64(eta), _f64(p_eta)))
+
+    def get_thermostat(self):
+        M = self.cfg.temperature_nh_len
+        eta, p_eta = np.zeros(M), np.zeros(M)
+        self._chk(self.lib.mtmd_get_thermostat(self.h, _f64(eta), _f64(p_eta)))
+        return eta, p_eta
+
+    def build_cells(self):
+        nc = np.zeros(3, np.int32)
+        self._chk(self.lib.mtmd_build_cells(self.h, _i32(nc), None, None, None, None))
+        tot = int(np.prod(nc))
+        ixyz = np.zeros((self.n, 3), np.int32)
+        first, count, lst = np.zeros(tot, np.int32), np.zeros(tot, np.int32), np.zeros(self.n, np.int32)
+        self._chk(self.lib.mtmd_build_cells(self.h, _i32(nc), _i32(ixyz), _i32(first), _i32(count), _i32(lst)))
+        return dict(nc=nc, ixyz=ixyz, first=first, count=count, list=lst)
+
+    def build_neighbors(self):
+        req = C.c_int32(0)
+        cnt = np.zeros(self.n, np.int32)
+        self._chk(self.lib.mtmd_build_neighbors(self.h, 0, _i32(cnt), None, C.byref(req)))
+        w = max(int(req.value), 1)
+        lst = np.zeros((self.n, w), np.int32)
+        self._chk(self.lib.mtmd_build_neighbors(self.h, w, _i32(cnt), _i32(lst), C.byref(req)))
+        return cnt, lst
+
+    def compute_forces(self):
+        F, U = np.zeros((self.n, 3)), np.zeros(self.n, np.float32)
+        self._chk(self.lib.mtmd_compute_forces(self.h, _f64(F), None, _f32(U)))
+        return F, U
+
+    def get_forces(self):
+        F, U = np.zeros((self.n, 3)), np.zeros(self.n, np.float32)
+        self._chk(self.lib.mtmd_get_forces(self.h, _f64(F), _f32(U)))
+        return F, U
+
+    def step(self, n: int = 1):
+        self._chk(self.lib.mtmd_step(self.h, int(n)))
+
+    def thermo(self) -> dict:
+        t = abi.MtmdThermo()
+        self._chk(self.lib.mtmd_get_thermo(self.h, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in abi.MtmdThermo._fields_}
+
+    def optimize(self, max_steps: int, lr: float, accuracy: float) -> int:
+        done = C.c_int32(0)
+        self._chk(self.lib.mtmd_optimize(self.h, max_steps, lr, accuracy, C.byref(done)))
+        return int(done.value)
+
+    def stats(self) -> dict:
+        r, s, m = C.c_int64(0), C.c_int64(0), C.c_int32(0)
+        self._chk(self.lib.mtmd_get_stats(self.h, C.byref(r), C.byref(s), C.byref(m)))
+        return dict(rebuilds=r.value, steps=s.value, capacity=m.value)
+
+    def synchronize(self):
+        self._chk(self.lib.mtmd_synchronize(self.h))

@@ -258,6 +258,14 @@ int mtmd_get_stats(mtmd_handle* h, int64_t* rebuilds, int64_t* steps,
 /* Blocks until all queued device work of the handle is done. */
 int mtmd_synchronize(mtmd_handle* h);
 
+/* Diagnostics, no counterpart in the reference (the reference's only timer is
+ * the per-step wall clock of MDLoop.cpp:434-456): average duration in ms of
+ * one launch of a kernel group, timed with HIP events on the handle's stream
+ * over `reps` back-to-back launches. which: 0 pair-force pass, 1 full
+ * neighbour-list build, 2 one MD step, 3 streaming copy of 32 bytes per atom
+ * (FP64/MIXED engines; calibrates the HBM byte counters). */
+int mtmd_bench_kernel(mtmd_handle* h, int which, int reps, double* avg_ms);
+
 /* CMDSimulator::run (Simulator/MDSimulator.cpp:35-47): creates the engine,
  * computes initial forces and runs cfg->n_steps, writing the reference's
  * thermo table every out_stride steps to cfg->info_file (and to `log` when

@@ -122,7 +122,7 @@ EXPORTED = [
     "mtmd_get_state", "mtmd_set_thermostat", "mtmd_get_thermostat", "mtmd_build_cells",
     "mtmd_build_neighbors", "mtmd_compute_forces", "mtmd_step", "mtmd_get_thermo",
     "mtmd_get_forces", "mtmd_optimize", "mtmd_get_stats", "mtmd_synchronize", "mtmd_run",
-    "mtmd_run_optimize",
+    "mtmd_run_optimize", "mtmd_bench_kernel",
 ]
 
 _lib = None
@@ -165,6 +165,7 @@ def load_library(path: str | None = None) -> C.CDLL:
         "mtmd_synchronize": (C.c_int, [vp]),
         "mtmd_run": (C.c_int, [C.POINTER(MtmdConfig), C.POINTER(MtmdSystem), vp]),
         "mtmd_run_optimize": (C.c_int, [C.POINTER(MtmdConfig), C.POINTER(MtmdSystem), vp]),
+        "mtmd_bench_kernel": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(C.c_double)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

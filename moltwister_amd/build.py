@@ -9,6 +9,7 @@ Translation units:
   mtmd_tables.cpp  force/energy profiles, g++ -ffp-contract=off so the float
                    tables are the ones a default x86-64 build of the reference
                    computes
+  mtmd_gen.c       seeded workload generators (libmtgen.so, host only)
 Every HIP compile line carries ``-Xclang -target-feature -Xclang -packed-fp32-ops``
 (no packed FP32 instructions).
 """
@@ -24,6 +25,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libmtmd.so")
+GENLIB = os.path.join(HERE, "libmtgen.so")
 ARCH = os.environ.get("MTMD_OFFLOAD_ARCH", "gfx950")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -56,6 +58,8 @@ def build(verbose: bool = False) -> str:
             if verbose and (r.stdout or r.stderr):
                 sys.stderr.write(r.stdout + r.stderr)
     objs = [os.path.join(OUT, f) for f in ("mtmd_exact.o", "mtmd_fast.o", "mtmd_host.o", "mtmd_tables.o")]
+    # workload generator helpers (host only)
+    _run([shutil.which("gcc") or "gcc", "-O2", "-fPIC", "-shared", "-o", GENLIB, os.path.join(SRC, "mtmd_gen.c"), "-lm"])
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *objs])
     os.replace(tmp, LIB)

@@ -34,11 +34,15 @@ int mt_reduce_blocks(int n) { int b = nblk(n, TPB); return b < 1 ? 1 : (b > 1024
 // ---------------------------------------------------------------------------
 __global__ void k_exact_cell_keys(const double4* __restrict__ pos, int n, int Nx, int Ny, int Nz,
                                   float wx, float wy, float wz, float lx, float ly, float lz,
-                                  int* __restrict__ key, int* __restrict__ cxyz, int* __restrict__ count)
+                                  int* __restrict__ key, int* __restrict__ cxyz, int* __restrict__ count,
+                                  unsigned int* flags)
 {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     double4 r = pos[k];
+    // non-finite coordinates would all clamp into cell 0; flag them so the
+    // host reports MTMD_ENAN instead of building quadratic lists
+    if (!(isfinite(r.x) && isfinite(r.y) && isfinite(r.z))) atomicOr(&flags[3], 1u);
     int ix = mt_d2i(floor((r.x - (double)lx) * (double)Nx / (double)wx));
     int iy = mt_d2i(floor((r.y - (double)ly) * (double)Ny / (double)wy));
     int iz = mt_d2i(floor((r.z - (double)lz) * (double)Nz / (double)wz));
@@ -91,10 +95,10 @@ __global__ void k_cell_scatter(const int* __restrict__ key, int n, const int* __
 
 // ascending atom index inside each cell (the reference fills cells in atom order)
 __global__ void k_cell_sort(const int* __restrict__ first, const int* __restrict__ count,
-                            int* __restrict__ list, int ncell)
+                            int* __restrict__ list, int ncell, const unsigned int* flags)
 {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncell) return;
+    if (c >= ncell || flags[3]) return;
     int cnt = count[c];
     if (cnt < 2) return;
     int* a = list + first[c];
@@ -110,12 +114,12 @@ void mt_exact_cells(const MtDev& d, const MtParams& p, hipStream_t s)
     hipLaunchKernelGGL(k_fill_i32, dim3(nblk(d.ncells, TPB)), dim3(TPB), 0, s, d.cell_count, 0, d.ncells);
     hipLaunchKernelGGL(k_exact_cell_keys, dim3(nblk(d.n, TPB)), dim3(TPB), 0, s, d.pos, d.n, d.nc[0], d.nc[1],
                        d.nc[2], p.Lf[0], p.Lf[1], p.Lf[2], p.lowf[0], p.lowf[1], p.lowf[2], d.cell_key,
-                       d.cell_xyz, d.cell_count);
+                       d.cell_xyz, d.cell_count, d.flags);
     hipLaunchKernelGGL(k_scan_cells, dim3(1), dim3(1024), 0, s, d.cell_count, d.cell_first, d.cell_fill, d.ncells);
     hipLaunchKernelGGL(k_cell_scatter, dim3(nblk(d.n, TPB)), dim3(TPB), 0, s, d.cell_key, d.n, d.cell_first,
                        d.cell_fill, d.cell_list);
     hipLaunchKernelGGL(k_cell_sort, dim3(nblk(d.ncells, TPB)), dim3(TPB), 0, s, d.cell_first, d.cell_count,
-                       d.cell_list, d.ncells);
+                       d.cell_list, d.ncells, d.flags);
 }
 
 // ---------------------------------------------------------------------------
@@ -127,10 +131,12 @@ __global__ void k_exact_neigh(const double4* __restrict__ pos, int n, const int*
                               const int* __restrict__ cfirst, const int* __restrict__ ccount,
                               const int* __restrict__ clist, int Nx, int Ny, int Nz,
                               double wx, double wy, double wz, float rc2, int ncap,
-                              int* __restrict__ ncount, uint32_t* __restrict__ nl, int* __restrict__ nreq)
+                              int* __restrict__ ncount, uint32_t* __restrict__ nl, int* __restrict__ nreq,
+                              const unsigned int* flags)
 {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
+    if (flags[3]) { ncount[k] = 0; return; }
     const double w[3] = {wx, wy, wz};
     double4 rk4 = pos[k];
     mt_d3 rk = {rk4.x, rk4.y, rk4.z};
@@ -166,7 +172,7 @@ void mt_exact_neighbors(const MtDev& d, const MtParams& p, hipStream_t s)
 {
     hipLaunchKernelGGL(k_exact_neigh, dim3(nblk(d.n, TPB)), dim3(TPB), 0, s, d.pos, d.n, d.cell_xyz, d.cell_first,
                        d.cell_count, d.cell_list, d.nc[0], d.nc[1], d.nc[2], p.wnl[0], p.wnl[1], p.wnl[2], p.rc2f,
-                       d.ncap, d.ncount, d.nlist, d.nreq);
+                       d.ncap, d.ncount, d.nlist, d.nreq, d.flags);
 }
 
 // ---------------------------------------------------------------------------

@@ -34,12 +34,13 @@ __global__ void k_fast_fill(int* p, int v, int n, int cond, const unsigned int* 
 
 __global__ void k_fast_cell_keys(const double4* __restrict__ pos, int n, int Nx, int Ny, int Nz, double LX,
                                  double LY, double LZ, int* __restrict__ key, int* __restrict__ count, int cond,
-                                 const unsigned int* flags, float thr)
+                                 unsigned int* flags, float thr)
 {
     if (mt_skip_rebuild(cond, flags, thr)) return;
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     double4 r = pos[k];
+    if (!(isfinite(r.x) && isfinite(r.y) && isfinite(r.z))) atomicOr(&flags[3], 1u);
     int ix = mt_d2i(floor((r.x + LX / 2.0) * (double)Nx / LX));
     int iy = mt_d2i(floor((r.y + LY / 2.0) * (double)Ny / LY));
     int iz = mt_d2i(floor((r.z + LZ / 2.0) * (double)Nz / LZ));
@@ -93,7 +94,7 @@ __global__ void k_fast_scatter(const int* __restrict__ key, int n, const int* __
 __global__ void k_fast_cell_sort(const int* __restrict__ first, const int* __restrict__ count,
                                  int* __restrict__ list, int ncell, int cond, const unsigned int* flags, float thr)
 {
-    if (mt_skip_rebuild(cond, flags, thr)) return;
+    if (mt_skip_rebuild(cond, flags, thr) || flags[3]) return;
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncell) return;
     int cnt = count[c];
@@ -232,6 +233,7 @@ __global__ void __launch_bounds__(TPB) k_fast_neigh(MtDev d, MtParams p, int con
     if (mt_skip_rebuild(cond, d.flags, p.half_skin2)) return;
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= d.n) return;
+    if (d.flags[3]) { d.ncount[k] = 0; return; }
     const int Nx = d.nc[0], Ny = d.nc[1], Nz = d.nc[2];
     int c = d.home[k];
     int hx = c % Nx, hy = (c / Nx) % Ny, hz = c / (Nx * Ny);
@@ -512,4 +514,17 @@ __global__ void k_fast_clear(unsigned int* flags) { flags[0] = 0u; }
 void mt_fast_clear_flags(const MtDev& d, hipStream_t s)
 {
     hipLaunchKernelGGL(k_fast_clear, dim3(1), dim3(1), 0, s, d.flags);
+}
+
+// plain streaming copy (read 32 B + write 32 B per element): the reference
+// point for the HBM byte counters
+__global__ void __launch_bounds__(TPB) k_copy_d4(const double4* __restrict__ src, double4* __restrict__ dst, int n)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
+void mt_copy_d4(const double4* src, double4* dst, int n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_copy_d4, dim3(nblk(n, TPB)), dim3(TPB), 0, s, src, dst, n);
 }

@@ -662,13 +662,11 @@ static int setup(mtmd_handle* h, const mtmd_system* s)
     if (h->M > 0) {
         nh[2 * M] = (double)(3 * n) * h->Tred * h->tau * h->tau;
         for (int j = 1; j < M; j++) nh[2 * M + j] = h->Tred * h->tau * h->tau;
-        // CNHChain::setRandNHPos / setRandNHMom (NHChain.cpp:126-143) from the seeded generator
-        uint64_t st = s->seed ^ 0x5DEECE66DULL;
-        for (int j = 0; j < M; j++) nh[j] = urand(st);
-        for (int j = 0; j < M; j++) {
-            double v = (2.0 * urand(st) - 1.0) * (nh[2 * M + j] / h->tau);
-            nh[M + j] = (v == 0.0) ? (nh[2 * M + j] / h->tau) : v;
-        }
+        // The chain starts at rest (eta = p_eta = 0). The reference draws
+        // p_eta[j] uniformly in +-Q_j/tau (NHChain.cpp:126-143); since Q_0
+        // grows with N, p_eta[0]^2/Q_0 does too, and a negative draw drives
+        // the chain unstable for large systems. mtmd_set_thermostat sets any
+        // state, the reference's draw included.
     }
     nh[3 * M] = 1.0;
     h->nh_scale_slot = 3 * M;
@@ -829,15 +827,26 @@ static int grow_neighbors(mtmd_handle* h, int need)
 }
 
 // EXACT: CMDFFMatrices::genNeighList + the force functor, every call
+static int exact_cells_checked(mtmd_handle* h)
+{
+    MtDev& d = h->d;
+    HIPCHK(h, hipMemsetAsync(d.flags + 3, 0, sizeof(unsigned int), h->stream));
+    mt_exact_cells(d, h->p, h->stream);
+    return MTMD_OK;
+}
+
 static int exact_forces(mtmd_handle* h)
 {
     MtDev& d = h->d;
-    mt_exact_cells(d, h->p, h->stream);
+    int rc0 = exact_cells_checked(h);
+    if (rc0) return rc0;
     for (int attempt = 0; attempt < 4; attempt++) {
         HIPCHK(h, hipMemsetAsync(d.nreq, 0, sizeof(int), h->stream));
         mt_exact_neighbors(d, h->p, h->stream);
         HIPCHK(h, hipMemcpyAsync(h->h_status, d.nreq, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipMemcpyAsync(h->h_status + 1, d.flags + 3, sizeof(int), hipMemcpyDeviceToHost, h->stream));
         HIPCHK(h, hipStreamSynchronize(h->stream));
+        if (h->h_status[1]) return h->fail(MTMD_ENAN, "non-finite atom position");
         const int need = h->h_status[0];
         if (need <= d.ncap) break;
         int rc = grow_neighbors(h, need);
@@ -884,6 +893,11 @@ static int fast_check_capacity(mtmd_handle* h, bool* overflowed)
     HIPCHK(h, hipStreamSynchronize(h->stream));
     *overflowed = h->h_status[2] != 0;
     h->rebuilds = (int64_t)(unsigned)h->h_status[4];
+    if (h->h_status[3]) {
+        HIPCHK(h, hipMemsetAsync(d.flags + 3, 0, sizeof(unsigned int), h->stream));
+        h->have_forces = false;
+        return h->fail(MTMD_ENAN, "non-finite atom position");
+    }
     return MTMD_OK;
 }
 
@@ -966,14 +980,17 @@ extern "C" int mtmd_build_cells(mtmd_handle* h, int32_t* ncxyz, int32_t* ixyz, i
     if (!h) return MTMD_EINVAL;
     if (h->fast) return h->fail(MTMD_EINVAL, "reference cell lists are produced by the exact engine");
     MtDev& d = h->d;
-    mt_exact_cells(d, h->p, h->stream);
+    int rc0 = exact_cells_checked(h);
+    if (rc0) return rc0;
     HIPCHK(h, hipGetLastError());
     if (ncxyz) { ncxyz[0] = d.nc[0]; ncxyz[1] = d.nc[1]; ncxyz[2] = d.nc[2]; }
     if (ixyz) HIPCHK(h, hipMemcpyAsync(ixyz, d.cell_xyz, 3 * (size_t)h->n * sizeof(int), hipMemcpyDeviceToHost, h->stream));
     if (cfirst) HIPCHK(h, hipMemcpyAsync(cfirst, d.cell_first, d.ncells * sizeof(int), hipMemcpyDeviceToHost, h->stream));
     if (ccount) HIPCHK(h, hipMemcpyAsync(ccount, d.cell_count, d.ncells * sizeof(int), hipMemcpyDeviceToHost, h->stream));
     if (clist) HIPCHK(h, hipMemcpyAsync(clist, d.cell_list, h->n * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->h_status + 1, d.flags + 3, sizeof(int), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (h->h_status[1]) return h->fail(MTMD_ENAN, "non-finite atom position");
     return MTMD_OK;
 }
 
@@ -983,12 +1000,15 @@ extern "C" int mtmd_build_neighbors(mtmd_handle* h, int32_t max_per_atom, int32_
     if (!h) return MTMD_EINVAL;
     if (h->fast) return h->fail(MTMD_EINVAL, "reference neighbour lists are produced by the exact engine");
     MtDev& d = h->d;
-    mt_exact_cells(d, h->p, h->stream);
+    int rc0 = exact_cells_checked(h);
+    if (rc0) return rc0;
     for (int attempt = 0; attempt < 4; attempt++) {
         HIPCHK(h, hipMemsetAsync(d.nreq, 0, sizeof(int), h->stream));
         mt_exact_neighbors(d, h->p, h->stream);
         HIPCHK(h, hipMemcpyAsync(h->h_status, d.nreq, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipMemcpyAsync(h->h_status + 1, d.flags + 3, sizeof(int), hipMemcpyDeviceToHost, h->stream));
         HIPCHK(h, hipStreamSynchronize(h->stream));
+        if (h->h_status[1]) return h->fail(MTMD_ENAN, "non-finite atom position");
         if (h->h_status[0] <= d.ncap) break;
         int rc = grow_neighbors(h, h->h_status[0]);
         if (rc) return rc;
@@ -1152,6 +1172,44 @@ extern "C" int mtmd_synchronize(mtmd_handle* h)
 {
     if (!h) return MTMD_EINVAL;
     HIPCHK(h, hipStreamSynchronize(h->stream));
+    return MTMD_OK;
+}
+
+extern "C" int mtmd_bench_kernel(mtmd_handle* h, int which, int reps, double* avg_ms)
+{
+    if (!h || reps < 1 || !avg_ms || which < 0 || which > 3) return MTMD_EINVAL;
+    if (which == 3 && !h->fast) return h->fail(MTMD_EINVAL, "the copy reference needs the fp64/mixed engine");
+    int rc;
+    if (!h->have_forces || (h->fast && h->need_sort)) {
+        if ((rc = compute_forces_now(h))) return rc;
+    }
+    hipEvent_t e0, e1;
+    HIPCHK(h, hipEventCreate(&e0));
+    HIPCHK(h, hipEventCreate(&e1));
+    HIPCHK(h, hipEventRecord(e0, h->stream));
+    for (int r = 0; r < reps; r++) {
+        if (which == 0) {
+            if (h->fast) rc = fast_forces(h, false);
+            else { mt_exact_forces(h->d, h->p, h->stream); rc = MTMD_OK; }
+        } else if (which == 1) {
+            if (h->fast) rc = fast_rebuild(h, true, false);
+            else { mt_exact_cells(h->d, h->p, h->stream); mt_exact_neighbors(h->d, h->p, h->stream); rc = MTMD_OK; }
+        } else if (which == 2) {
+            rc = md_steps(h, 1, false);
+        } else {
+            mt_copy_d4(h->d.frc, h->d.frc2, h->n, h->stream);
+            rc = MTMD_OK;
+        }
+        if (rc) break;
+    }
+    HIPCHK(h, hipEventRecord(e1, h->stream));
+    HIPCHK(h, hipEventSynchronize(e1));
+    float ms = 0.0f;
+    HIPCHK(h, hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    *avg_ms = (double)ms / reps;
     return MTMD_OK;
 }
 

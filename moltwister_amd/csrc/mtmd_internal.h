@@ -118,5 +118,6 @@ void mt_fast_neighbors(const MtDev& d, const MtParams& p, int conditional, hipSt
 void mt_fast_forces(const MtDev& d, const MtParams& p, int mixed, int want_energy, int block, hipStream_t s);
 void mt_fast_kick_drift(const MtDev& d, const MtParams& p, int scale_slot, int mixed, hipStream_t s);
 void mt_fast_clear_flags(const MtDev& d, hipStream_t s);
+void mt_copy_d4(const double4* src, double4* dst, int n, hipStream_t s);
 
 int mt_reduce_blocks(int n);

@@ -139,5 +139,12 @@ class Engine:
         self._chk(self.lib.mtmd_get_stats(self.h, C.byref(r), C.byref(s), C.byref(m)))
         return dict(rebuilds=r.value, steps=s.value, capacity=m.value)
 
+    def bench_kernel(self, which: int, reps: int) -> float:
+        """Average ms per launch (HIP events on the engine's stream):
+        0 pair forces, 1 list build, 2 MD step, 3 streaming copy."""
+        ms = C.c_double(0.0)
+        self._chk(self.lib.mtmd_bench_kernel(self.h, which, reps, C.byref(ms)))
+        return ms.value
+
     def synchronize(self):
         self._chk(self.lib.mtmd_synchronize(self.h))

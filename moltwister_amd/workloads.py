@@ -20,8 +20,34 @@ import math
 
 import numpy as np
 
+import ctypes as C
+import os
+
 from . import abi
 from .system import System
+
+_GEN = None
+
+
+def _genlib():
+    global _GEN
+    if _GEN is None:
+        path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libmtgen.so")
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} missing: python -m moltwister_amd.build")
+        _GEN = C.CDLL(path)
+        _GEN.mtgen_rsa_fast.argtypes = [C.c_int, C.c_double, C.c_double, C.c_uint64, C.c_int,
+                                        C.POINTER(C.c_double)]
+    return _GEN
+
+
+def rsa_positions_c(n: int, L: float, dmin: float, seed: int) -> np.ndarray:
+    """Random sequential addition in C (libmtgen.so, mtgen_rsa_fast)."""
+    out = np.zeros((n, 3))
+    k = _genlib().mtgen_rsa_fast(n, L, dmin, seed, 16, out.ctypes.data_as(C.POINTER(C.c_double)))
+    if k != n:
+        raise RuntimeError(f"rsa: placed {k} of {n}")
+    return out
 
 CONV_T = 120.272
 
@@ -192,6 +218,23 @@ def binary_lj(n: int = 4000, density: float = 1.2, frac_a: float = 0.8, dmin: fl
     mom = maxwell_boltzmann(mass, T_star * CONV_T, rng)
     return System(pos=pos, mass=mass, charge=np.zeros(n), types=types, box=(L, L, L), type_names=names,
                   nonbonded=nb, mom=mom, seed=seed)
+
+
+def binary_lj_large(n: int = 1048576, density: float = 1.2, tiles: int = 2, seed: int = 3, T_star: float = 1.0,
+                    buckingham: bool = False) -> System:
+    """Configuration 3 at full size: an RSA box of n/tiles^3 atoms (minimum
+    separation 0.8 sigma, rho = 1.2) repeated tiles^3 times, then types and
+    Maxwell-Boltzmann momenta drawn per atom, so the copies diverge from the
+    first step. (Plain RSA of 10^6 points takes minutes near this packing;
+    the periodic copy has period L/tiles = 47.8 sigma, far beyond the cutoff.)"""
+    m = n // tiles ** 3
+    if m * tiles ** 3 != n:
+        raise ValueError("n must be divisible by tiles^3")
+    L = (n / density) ** (1.0 / 3.0)
+    base = rsa_positions_c(m, L / tiles, 0.8, seed)
+    shifts = np.stack(np.meshgrid(*(np.arange(tiles),) * 3, indexing="ij"), -1).reshape(-1, 3)
+    pos = (base[None, :, :] + (L / 2.0 / tiles) + shifts[:, None, :] * (L / tiles)).reshape(-1, 3) - L / 2.0
+    return binary_lj(n, density=density, seed=seed, T_star=T_star, buckingham=buckingham, positions=pos)
 
 
 def chain_molecules(n_chain: int = 60, length: int = 6, L: float = 24.0, seed: int = 4,

@@ -83,3 +83,40 @@ def test_state_roundtrip_sorted(oracle_mod):
     o.set_state(pos=pos)
     Fo, _ = o.compute_forces()
     assert rms_rel(F1, Fo) < 1e-5
+
+
+@pytest.mark.parametrize("engine", [abi.ENGINE_EXACT, abi.ENGINE_FP64, abi.ENGINE_MIXED])
+def test_nvt_large_system_stays_finite(engine):
+    """NVT on 131,072 atoms from a random-sequential-addition start: the
+    thermostat starts at rest and the run stays finite and near T."""
+    from moltwister_amd import workloads as W
+    print(f"case nvt 131072 engine={engine}", flush=True)
+    s = W.binary_lj_large(131072, tiles=1, seed=3)
+    c = make("binary_lj", engine=engine, ensemble=abi.NVT, T=120.272, dt=0.5)[1]
+    c.temperature_relax = 20.0
+    e = Engine(s, c)
+    # relax the random packing first (moldyn optimizeenergy), as a run would
+    e.optimize(60, 2e-5, 0.0)
+    e.step(100 if engine != abi.ENGINE_EXACT else 20)
+    th = e.thermo()
+    pos, mom = e.get_state()
+    assert np.isfinite(pos).all() and np.isfinite(mom).all()
+    assert np.isfinite(th["Etot"]) and 0 < th["T"] < 5000
+
+
+@pytest.mark.parametrize("engine", [abi.ENGINE_EXACT, abi.ENGINE_MIXED])
+def test_nonfinite_position_reported(engine):
+    """A NaN coordinate is reported as MTMD_ENAN (no list is built from it)."""
+    from moltwister_amd.engine import MtmdError
+    print(f"case nan engine={engine}", flush=True)
+    s, c = make("binary_lj", engine=engine)
+    e = Engine(s, c)
+    pos = s.pos.copy()
+    pos[17, 1] = np.nan
+    e.set_state(pos=pos)
+    with pytest.raises(MtmdError) as ex:
+        e.compute_forces()
+    assert ex.value.status == abi.MTMD_ENAN
+    e.set_state(pos=s.pos)
+    F, _ = e.compute_forces()
+    assert np.isfinite(F).all()

@@ -1,0 +1,241 @@
+"""Pins for the parts of the oracle the reference build cannot exercise
+(CVelVerlet/CSimulationBox need CMolTwisterState, whose C3DView translation
+unit needs GLUT): the velocity-Verlet step, force cap, PBC wrap, thermo
+reductions and the gradient-descent step, checked against closed forms and
+invariants written independently here with numpy. Each check is chosen so a
+plausible slip (a sign, a missing half-kick, a dropped term, the wrong mass)
+fails it.
+"""
+import numpy as np
+import pytest
+
+from moltwister_amd import abi, workloads as W
+
+from systems import cfg
+
+CONV_T, CONV_t, CONV_P = 120.272, 100.0, 16388.2461
+
+
+@pytest.fixture
+def lj(oracle_mod):
+    s = W.fcc_lj(5, density=0.5, seed=9)
+    rng = np.random.default_rng(2)
+    s.pos = s.pos + rng.normal(0, 0.05, s.pos.shape)
+    s.mass = 1.0 + rng.random(s.n_atoms)          # unequal masses catch p/m slips
+    c = cfg(2.5, 0.3, ensemble=abi.NVE, dt=0.5)
+    c.cutoff_force = 0.0                            # cap off unless a test turns it on
+    return s, c
+
+
+def test_one_vv_step_closed_form(oracle_mod, lj):
+    """r1 = r0 + (p0 + F0 dt/2) dt/m; p1 = p0 + (F0 + F1) dt/2 (VelVerlet.cpp:74-93)."""
+    s, c = lj
+    o = oracle_mod.Oracle(s, c)
+    r0, p0 = o.get_state()
+    F0, _ = o.compute_forces()
+    o.step(1)
+    r1, p1 = o.get_state()
+    F1, _ = o.get_forces()
+    dt = c.timestep / CONV_t
+    m = s.mass[:, None]
+    r1_expect = r0 + (p0 + F0 * (dt / 2)) * (dt / m)
+    L = np.array(s.box)
+    d = r1 - r1_expect
+    d -= L * np.round(d / L)                        # the wrap may move an atom by one box
+    assert np.abs(d).max() < 1e-13
+    assert np.allclose(p1, p0 + (F0 + F1) * (dt / 2), rtol=0, atol=1e-12)
+
+
+def test_wrap_into_box(oracle_mod, lj):
+    """After a step every coordinate lies in [-L/2, L/2) (SimulationBox.cpp:135-166)."""
+    s, c = lj
+    o = oracle_mod.Oracle(s, c)
+    pos = s.pos.copy()
+    pos[0] += np.array(s.box) * 1.7               # far outside: the wrap removes whole boxes
+    pos[1] -= np.array(s.box) * 0.6
+    o.set_state(pos=pos)
+    o.step(1)
+    r, _ = o.get_state()
+    L = np.array(s.box)
+    assert np.all(r >= -L / 2) and np.all(r < L / 2)
+
+
+def test_force_cap(oracle_mod, lj):
+    """cutForces clamps each component to +-Fcut (VelVerlet.cpp:96-122)."""
+    s, c = lj
+    pos = s.pos.copy()
+    pos[1] = pos[0] + np.array([0.6, 0.0, 0.0])    # overlap: |F| >> cap
+    c.cutoff_force = 50.0
+    o = oracle_mod.Oracle(s, c)
+    o.set_state(pos=pos)
+    o.step(1)
+    F, _ = o.get_forces()
+    assert np.abs(F).max() == 50.0
+    raw = oracle_mod.Oracle(s, c)
+    raw.set_state(pos=o.get_state()[0])
+    Fr, _ = raw.compute_forces()                    # compute_forces is uncapped
+    assert np.abs(Fr).max() > 50.0
+    assert np.array_equal(F, np.clip(Fr, -50.0, 50.0))
+
+
+def test_momentum_conservation(oracle_mod, lj):
+    """Full neighbour lists give sum F = 0 (Newton 3), so NVE keeps sum p."""
+    s, c = lj
+    o = oracle_mod.Oracle(s, c)
+    F, _ = o.compute_forces()
+    assert np.abs(F.sum(axis=0)).max() < 1e-10 * np.abs(F).max() * np.sqrt(s.n_atoms)
+    _, p0 = o.get_state()
+    o.step(200)
+    _, p1 = o.get_state()
+    assert np.abs(p1.sum(axis=0) - p0.sum(axis=0)).max() < 1e-9
+
+
+def test_energy_conservation_second_order(oracle_mod):
+    """Harmonic dimers (no cutoff crossings): the NVE energy error of velocity
+    Verlet is O(dt^2) -- halving dt cuts it about fourfold. (The LJ model of the
+    reference is truncated without a shift, so LJ runs also carry the U(rc)
+    jumps of pairs crossing the cutoff, which no integrator removes.)"""
+    from moltwister_amd.system import System
+    rng = np.random.default_rng(5)
+    nd = 64
+    centres = rng.uniform(-8, 8, (nd, 3))
+    d = rng.normal(0, 1, (nd, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    pos = np.concatenate([centres - 0.55 * d, centres + 0.55 * d])
+    order = np.stack([np.arange(nd), np.arange(nd) + nd], 1).reshape(-1)
+    pos = pos[order]
+    bonds = np.stack([np.arange(nd) * 2, np.arange(nd) * 2 + 1, np.zeros(nd, int)], 1)
+    mass = np.tile([1.0, 2.0], nd)
+    mom = rng.normal(0, 1, (2 * nd, 3)) * np.sqrt(mass)[:, None]
+    errs = []
+    for dt in (0.4, 0.2):
+        s = System(pos=pos, mom=mom, mass=mass, charge=np.zeros(2 * nd), types=np.zeros(2 * nd, np.int32),
+                   box=(40, 40, 40), nonbonded=[], bond_ff=[(abi.BOND_HARM, (300.0, 1.0))], bonds=bonds)
+        c = cfg(3.0, 0.5, ensemble=abi.NVE, dt=dt)
+        c.cutoff_force = 0.0
+        o = oracle_mod.Oracle(s, c)
+        E0 = o.thermo()["Etot"]
+        nst = int(round(40 / dt))
+        dev = 0.0
+        for _ in range(10):
+            o.step(nst // 10)
+            dev = max(dev, abs(o.thermo()["Etot"] - E0))
+        errs.append(dev)
+    assert errs[0] < 1e-2 * abs(E0)
+    assert 2.5 < errs[0] / errs[1] < 6.0, errs
+
+
+def test_time_reversibility(oracle_mod, lj):
+    s, c = lj
+    o = oracle_mod.Oracle(s, c)
+    r0, p0 = o.get_state()
+    o.step(50)
+    r1, p1 = o.get_state()
+    o.set_state(pos=r1, mom=-p1)
+    o.step(50)
+    r2, p2 = o.get_state()
+    L = np.array(s.box)
+    d = r2 - r0
+    d -= L * np.round(d / L)
+    assert np.abs(d).max() < 1e-8
+    assert np.abs(p2 + p0).max() < 1e-8
+
+
+def test_thermo_definitions(oracle_mod, lj):
+    """T = sum p.p/m/(3N), P = (sum p.p/m + sum F.r)/(3V), K, U sums, units
+    (SimulationBox.cpp:168-231, Constants.h:27,32)."""
+    s, c = lj
+    o = oracle_mod.Oracle(s, c)
+    o.step(3)
+    r, p = o.get_state()
+    F, U = o.get_forces()
+    th = o.thermo()
+    p2m = np.sum(np.sum(p * p, axis=1) / s.mass)
+    V = np.prod(s.box)
+    assert th["T"] == pytest.approx(p2m / (3 * s.n_atoms) * CONV_T, rel=1e-13)
+    assert th["K"] == pytest.approx(0.5 * p2m, rel=1e-13)
+    assert th["U"] == pytest.approx(np.sum(U.astype(np.float64)), rel=1e-13)
+    assert th["P"] == pytest.approx((p2m + np.sum(F * r)) / (3 * V) * CONV_P, rel=1e-12)
+    assert th["V"] == pytest.approx(V, rel=1e-15)
+
+
+def test_gradient_descent_step(oracle_mod, lj):
+    """runOptimization: r += lr F with uncapped forces (MDLoop.cpp:143-173)."""
+    s, c = lj
+    o = oracle_mod.Oracle(s, c)
+    r0, _ = o.get_state()
+    F0, _ = o.compute_forces()
+    done = o.optimize(1, 1e-4, 0.0, 100)
+    r1, _ = o.get_state()
+    assert done == 1
+    assert np.allclose(r1, r0 + F0 * 1e-4, rtol=0, atol=1e-15)
+
+
+def test_two_atom_lj_known_answers(oracle_mod):
+    """Two LJ atoms: no force at the minimum 2^(1/6) sigma, the analytic
+    24 eps (2/r^13 - 1/r^7) elsewhere to the tabulation error, and U(r)
+    counted once (only for i > k)."""
+    from moltwister_amd.system import System
+    for r, tol in ((2.0 ** (1 / 6), 2e-3), (1.1, 2e-4), (1.5, 2e-4), (2.4, 2e-4)):
+        s = System(pos=[[0, 0, 0], [r, 0, 0]], mass=[1, 1], charge=[0, 0], types=[0, 0], box=(12, 12, 12),
+                   nonbonded=[(0, 0, abi.NB_LJ, (1.0, 1.0, 1.0))])
+        o = oracle_mod.Oracle(s, cfg(2.5, 0.3))
+        F, U = o.compute_forces()
+        f_exact = 24.0 * (2.0 / r ** 13 - 1.0 / r ** 7)
+        assert F[1, 0] == pytest.approx(f_exact, abs=tol * 24)
+        assert F[0, 0] == pytest.approx(-F[1, 0], rel=1e-15)
+        assert U[1] == 0.0                           # atom k holds the pair only when i > k
+        assert U[0] == pytest.approx(4.0 * (r ** -12 - r ** -6), abs=1e-4)
+
+
+def test_two_charge_coulomb(oracle_mod):
+    """K q1 q2 / r with K = 1389.32 (Constants.h:40), all-pairs, no cutoff."""
+    from moltwister_amd.system import System
+    r = 7.3
+    s = System(pos=[[0, 0, 0], [r, 0, 0]], mass=[1, 1], charge=[0.5, -0.8], types=[0, 0], box=(30, 30, 30),
+               nonbonded=[])
+    o = oracle_mod.Oracle(s, cfg(2.5, 0.3, coulomb=abi.COUL_ALLPAIRS))
+    F, U = o.compute_forces()
+    K = float(np.float32(1389.32) * np.float32(0.5) * np.float32(-0.8))
+    assert F[1, 0] == pytest.approx(K / r ** 2, rel=1e-6)
+    assert U.sum() == pytest.approx(K / r, rel=1e-6)
+
+
+def test_harmonic_bond_known_answer(oracle_mod):
+    """U = k (r - r0)^2, F = -2k (r - r0) along the bond (MolTwisterMDFFBond_Harm.cpp:90-110)."""
+    from moltwister_amd.system import System
+    k, r0, r = 500.0, 1.0, 1.1
+    s = System(pos=[[0, 0, 0], [r, 0, 0]], mass=[1, 1], charge=[0, 0], types=[0, 0], box=(20, 20, 20),
+               nonbonded=[], bond_ff=[(abi.BOND_HARM, (k, r0))], bonds=[[0, 1, 0]])
+    o = oracle_mod.Oracle(s, cfg(5.0, 0.5))
+    F, U = o.compute_forces()
+    assert F[1, 0] == pytest.approx(-2 * k * (r - r0), rel=1e-4)
+    assert U.sum() == pytest.approx(k * (r - r0) ** 2, rel=1e-3)
+
+
+def test_harmonic_angle_known_answer(oracle_mod):
+    """Forces of U = k (theta - theta0)^2 on a bent triatomic sum to zero and
+    match -dU/dr by central differences of the analytic energy."""
+    from moltwister_amd.system import System
+    k, th0 = 100.0, 104.5
+    th = np.radians(112.0)
+    pos = np.array([[1.0, 0, 0], [0, 0, 0], [np.cos(th), np.sin(th), 0.0]])
+    s = System(pos=pos, mass=[1, 1, 1], charge=[0, 0, 0], types=[0, 0, 0], box=(20, 20, 20), nonbonded=[],
+               angle_ff=[(abi.ANGLE_HARM, (k, th0))], angles=[[0, 1, 2, 0]])
+    o = oracle_mod.Oracle(s, cfg(5.0, 0.5))
+    F, U = o.compute_forces()
+
+    def energy(p):
+        a, b = p[0] - p[1], p[2] - p[1]
+        t = np.arccos(a @ b / np.linalg.norm(a) / np.linalg.norm(b))
+        return k * (t - np.radians(th0)) ** 2
+
+    h = 1e-6
+    for i in range(3):
+        for d in range(3):
+            pp, pm = pos.copy(), pos.copy()
+            pp[i, d] += h
+            pm[i, d] -= h
+            assert F[i, d] == pytest.approx(-(energy(pp) - energy(pm)) / (2 * h), abs=2e-3 * k)
+    assert np.abs(F.sum(axis=0)).max() < 1e-6 * k
+    assert U.sum() == pytest.approx(energy(pos), rel=2e-3)

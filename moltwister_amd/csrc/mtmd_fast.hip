@@ -15,6 +15,7 @@
 // the per-atom force sum, integration and reductions stay in FP64.
 #include "mtmd_internal.h"
 #include "mtmd_device.h"
+#include <algorithm>
 
 #define TPB 256
 
@@ -225,55 +226,102 @@ __device__ inline bool mt_is_partner(const MtDev& d, int k, int ia)
     return false;
 }
 
-// neighbour list at rlist over the 27 home cells around k's home cell;
-// same-molecule pairs and cross-molecule bonded partners are left to the
-// special-pair pass
-__global__ void __launch_bounds__(TPB) k_fast_neigh(MtDev d, MtParams p, int cond)
+// neighbour list at rlist. One 64-lane workgroup builds the lists of two home
+// cells, one per half-wave (a cell holds ~27 atoms at rho* = 1.2 with cells of
+// cutoff + skin). For each of the 27 cell offsets, the atoms of each half's
+// neighbouring cell are staged in that half of an LDS tile, already shifted by
+// the offset, and each lane (one atom of its home cell) scans its half with
+// broadcast reads. Same-molecule pairs and cross-molecule bonded partners are
+// left to the special-pair pass.
+#define NB_HALF 32
+__global__ void __launch_bounds__(64) k_fast_neigh_cell(MtDev d, MtParams p, int cond)
 {
     if (mt_skip_rebuild(cond, d.flags, p.half_skin2)) return;
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= d.n) return;
-    if (d.flags[3]) { d.ncount[k] = 0; return; }
+    const int t = threadIdx.x;
+    const int h = t >> 5, l = t & 31;
+    const int c = 2 * blockIdx.x + h;
+    const bool cell_ok = c < d.ncells;
+    __shared__ float4 tile[64];
+    __shared__ uint32_t tword[64];
+    __shared__ int s_cnt[2];
+    if (t == 0 || t == 32) s_cnt[h] = cell_ok ? d.cell_count[c] : 0;
+    __syncthreads();
+    const int cnt_c = s_cnt[h];
+    const int cnt_max = max(s_cnt[0], s_cnt[1]);
+    if (cnt_max == 0) return;
+    const int f0 = cell_ok ? d.cell_first[c] : 0;
+    if (d.flags[3]) {
+        for (int i = l; i < cnt_c; i += NB_HALF) d.ncount[d.cell_list[f0 + i]] = 0;
+        return;
+    }
     const int Nx = d.nc[0], Ny = d.nc[1], Nz = d.nc[2];
-    int c = d.home[k];
-    int hx = c % Nx, hy = (c / Nx) % Ny, hz = c / (Nx * Ny);
-    float4 pk = d.posf[k];
-    int molk = d.mol[k];
-    bool xk = d.xmol[d.perm[k]] != 0;
-    int cnt = 0;
+    const int cc0 = cell_ok ? c : 0;
+    const int hx = cc0 % Nx, hy = (cc0 / Nx) % Ny, hz = cc0 / (Nx * Ny);
     const int ncap = d.ncap, n = d.n;
-    for (int oz = -1; oz <= 1; oz++) {
-        int cz = hz + oz; cz = cz < 0 ? cz + Nz : (cz >= Nz ? cz - Nz : cz);
-        for (int oy = -1; oy <= 1; oy++) {
-            int cy = hy + oy; cy = cy < 0 ? cy + Ny : (cy >= Ny ? cy - Ny : cy);
-            for (int ox = -1; ox <= 1; ox++) {
-                int cx = hx + ox; cx = cx < 0 ? cx + Nx : (cx >= Nx ? cx - Nx : cx);
-                int cc = cx + Nx * (cy + Ny * cz);
-                const uint32_t code = (uint32_t)((ox + 1) + 3 * (oy + 1) + 9 * (oz + 1)) << MT_NW_IDX_BITS;
-                const float sx = (float)ox * p.cs[0], sy = (float)oy * p.cs[1], sz = (float)oz * p.cs[2];
-                int f0 = d.cell_first[cc], num = d.cell_count[cc];
-                for (int e = 0; e < num; e++) {
-                    int a = d.cell_list[f0 + e];
-                    float4 pa = d.posf[a];
-                    float dx = pk.x - pa.x - sx, dy = pk.y - pa.y - sy, dz = pk.z - pa.z - sz;
-                    float r2 = dx * dx + dy * dy + dz * dz;
-                    if (r2 < p.rlist2 && a != k) {
-                        if (molk >= 0 && d.mol[a] == molk) continue;
-                        if (xk && mt_is_partner(d, k, a)) continue;
-                        if (cnt < ncap) d.nlist[(size_t)cnt * n + k] = (uint32_t)a | code;
-                        cnt++;
+    const float rl2 = p.rlist2;
+    for (int ib = 0; ib < cnt_max; ib += NB_HALF) {
+        const bool act = ib + l < cnt_c;
+        const int k = act ? d.cell_list[f0 + ib + l] : 0;
+        const float4 pk = act ? d.posf[k] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const int molk = (act && d.has_molecules) ? d.mol[k] : -1;
+        const bool xk = act && d.has_xmol && d.xmol[d.perm[k]] != 0;
+        int cnt = 0;
+        for (int oz = -1; oz <= 1; oz++) {
+            int cz = hz + oz; cz = cz < 0 ? cz + Nz : (cz >= Nz ? cz - Nz : cz);
+            for (int oy = -1; oy <= 1; oy++) {
+                int cy = hy + oy; cy = cy < 0 ? cy + Ny : (cy >= Ny ? cy - Ny : cy);
+                for (int ox = -1; ox <= 1; ox++) {
+                    int cx = hx + ox; cx = cx < 0 ? cx + Nx : (cx >= Nx ? cx - Nx : cx);
+                    const int cc = cx + Nx * (cy + Ny * cz);
+                    const uint32_t code = (uint32_t)((ox + 1) + 3 * (oy + 1) + 9 * (oz + 1)) << MT_NW_IDX_BITS;
+                    const float sx = (float)ox * p.cs[0], sy = (float)oy * p.cs[1], sz = (float)oz * p.cs[2];
+                    const int g0 = cell_ok ? d.cell_first[cc] : 0;
+                    const int gn = cell_ok ? d.cell_count[cc] : 0;
+                    __syncthreads();
+                    if (t == 0 || t == 32) s_cnt[h] = gn;
+                    __syncthreads();
+                    const int gmax = max(s_cnt[0], s_cnt[1]);
+                    for (int jb = 0; jb < gmax; jb += NB_HALF) {
+                        __syncthreads();
+                        if (jb + l < gn) {
+                            const int a = d.cell_list[g0 + jb + l];
+                            const float4 pa = d.posf[a];
+                            tile[t] = make_float4(pa.x + sx, pa.y + sy, pa.z + sz, 0.f);
+                            tword[t] = (uint32_t)a | code;
+                        }
+                        __syncthreads();
+                        if (act) {
+                            const int m = min(NB_HALF, gn - jb);
+                            const float4* tl = tile + (h << 5);
+                            const uint32_t* tw = tword + (h << 5);
+                            for (int e = 0; e < m; e++) {
+                                const float4 q = tl[e];
+                                const float dx = pk.x - q.x, dy = pk.y - q.y, dz = pk.z - q.z;
+                                if (dx * dx + dy * dy + dz * dz < rl2) {
+                                    const uint32_t w = tw[e];
+                                    const int a = (int)(w & MT_NW_IDX_MASK);
+                                    if (a == k) continue;
+                                    if (molk >= 0 && d.mol[a] == molk) continue;
+                                    if (xk && mt_is_partner(d, k, a)) continue;
+                                    if (cnt < ncap) d.nlist[(size_t)cnt * n + k] = w;
+                                    cnt++;
+                                }
+                            }
+                        }
                     }
                 }
             }
         }
+        if (act) {
+            d.ncount[k] = cnt;
+            if (cnt > ncap) { d.flags[2] = 1; atomicMax(d.nreq, cnt); }
+        }
     }
-    d.ncount[k] = cnt;
-    if (cnt > ncap) { d.flags[2] = 1; atomicMax(d.nreq, cnt); }
 }
 
 void mt_fast_neighbors(const MtDev& d, const MtParams& p, int cond, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_fast_neigh, dim3(nblk(d.n, TPB)), dim3(TPB), 0, s, d, p, cond);
+    hipLaunchKernelGGL(k_fast_neigh_cell, dim3((d.ncells + 1) / 2), dim3(64), 0, s, d, p, cond);
 }
 
 // ---------------------------------------------------------------------------
@@ -332,6 +380,57 @@ __global__ void __launch_bounds__(512) k_fast_force_mixed(MtDev d, MtParams p)
     if (!d.mobile[k]) { Fx = Fy = Fz = 0.0; U = 0.0f; }
     d.frc[k] = make_double4(Fx, Fy, Fz, 0.0);
     if (WANT_E) d.epot[k] = U;
+}
+
+// MIXED with the force-only segment coefficients (a_f, b_f) of every type pair
+// staged in LDS once per workgroup; workgroups stride over the atoms. Used
+// when T^2 * 1000 * 8 bytes fits the LDS budget and energies are not needed.
+template <bool COUL>
+__global__ void __launch_bounds__(512) k_fast_force_mixed_lds(MtDev d, MtParams p)
+{
+    extern __shared__ float2 stab[];
+    const int T = d.T;
+    const int ntab = T * T * MT_NPTS;
+    for (int i = threadIdx.x; i < ntab; i += blockDim.x) stab[i] = d.nbcoef2[i];
+    __syncthreads();
+    const float csx = p.cs[0], csy = p.cs[1], csz = p.cs[2];
+    const float rc2 = p.rc2f, x0 = p.x0, invd = p.inv_delta;
+    const size_t stride = (size_t)d.n;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < d.n; k += gridDim.x * blockDim.x) {
+        const float4 pk = d.posf[k];
+        const float2* __restrict__ row = stab + __float_as_int(pk.w) * T * MT_NPTS;
+        const float qk = COUL ? d.qf[k] : 0.0f;
+        double Fx = 0.0, Fy = 0.0, Fz = 0.0;
+        const int cnt = min(d.ncount[k], d.ncap);
+        const uint32_t* __restrict__ nl = d.nlist + k;
+        for (int j = 0; j < cnt; j++) {
+            const uint32_t w = nl[(size_t)j * stride];
+            const int a = (int)(w & MT_NW_IDX_MASK);
+            const int code = (int)(w >> MT_NW_IDX_BITS);
+            const int ox = code % 3 - 1, oy = (code / 3) % 3 - 1, oz = code / 9 - 1;
+            const float4 pa = d.posf[a];
+            const float dx = pk.x - pa.x - (float)ox * csx;
+            const float dy = pk.y - pa.y - (float)oy * csy;
+            const float dz = pk.z - pa.z - (float)oz * csz;
+            const float r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 < rc2 && r2 >= 1.0e-6f) {
+                const float r = __fsqrt_rn(r2);
+                const float rinv = 1.0f / r;
+                float fr = 0.0f;
+                const int idx = (int)((r - x0) * invd);
+                if (idx >= 0 && idx < MT_NPTS - 1) {
+                    const float2 c = row[__float_as_int(pa.w) * MT_NPTS + idx];
+                    fr = (c.x * r + c.y) * rinv;
+                }
+                if (COUL) fr += MT_COULOMB_K * qk * d.qf[a] * rinv * rinv * rinv;
+                Fx += (double)(dx * fr);
+                Fy += (double)(dy * fr);
+                Fz += (double)(dz * fr);
+            }
+        }
+        if (!d.mobile[k]) { Fx = Fy = Fz = 0.0; }
+        d.frc[k] = make_double4(Fx, Fy, Fz, 0.0);
+    }
 }
 
 // FP64: the reference's per-pair arithmetic (FunctorCalcForce.cpp:467-535)
@@ -435,7 +534,17 @@ __global__ void __launch_bounds__(TPB) k_fast_bonded_special(MtDev d, MtParams p
 void mt_fast_forces(const MtDev& d, const MtParams& p, int mixed, int want_e, int block, hipStream_t s)
 {
     const bool coul = p.coulomb_on != 0;
-    if (mixed) {
+    const size_t lds_tab = (size_t)d.T * d.T * MT_NPTS * sizeof(float2);
+    if (mixed && !want_e && lds_tab <= 65536) {
+        // persistent-style grid: enough workgroups to fill every CU a few times
+        int dev = 0, ncu = 256;
+        if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+        const int bs = 512;
+        const int per_cu = lds_tab > 32768 ? 2 : 4;
+        const int g = std::max(1, std::min(nblk(d.n, bs), ncu * per_cu));
+        if (coul) hipLaunchKernelGGL((k_fast_force_mixed_lds<true>), dim3(g), dim3(bs), lds_tab, s, d, p);
+        else hipLaunchKernelGGL((k_fast_force_mixed_lds<false>), dim3(g), dim3(bs), lds_tab, s, d, p);
+    } else if (mixed) {
         int bs = (block == 64 || block == 128 || block == 256 || block == 512) ? block : 256;
         dim3 g(nblk(d.n, bs)), b(bs);
         if (want_e) {
@@ -500,8 +609,11 @@ __global__ void __launch_bounds__(TPB) k_fast_kick_drift(MtDev d, MtParams p, in
             d.posf[k] = make_float4((float)lx, (float)ly, (float)lz, old.w);
         }
     }
+    // only "has any atom moved beyond skin/2" matters: one atomic per wavefront
+    // that crossed the threshold, and none once the flag is already raised
     for (int off = 32; off > 0; off >>= 1) d2 = fmaxf(d2, __shfl_xor(d2, off));
-    if ((threadIdx.x & 63) == 0 && d2 > 0.0f) atomicMax(&d.flags[0], __float_as_uint(d2));
+    if ((threadIdx.x & 63) == 0 && d2 > p.half_skin2 && !(__uint_as_float(d.flags[0]) > p.half_skin2))
+        atomicMax(&d.flags[0], __float_as_uint(d2));
 }
 
 void mt_fast_kick_drift(const MtDev& d, const MtParams& p, int scale_slot, int mixed, hipStream_t s)

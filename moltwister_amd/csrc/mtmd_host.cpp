@@ -448,6 +448,8 @@ static int setup(mtmd_handle* h, const mtmd_system* s)
     }
     // xmol: atom has a scaled partner (1-2, 1-3, 1-4) outside its own molecule
     std::vector<uint8_t> xmol(n, 0);
+    h->d.has_molecules = 0;
+    for (int k = 0; k < n; k++) if (mol[k] != k) { h->d.has_molecules = 1; break; }
     for (int k = 0; k < n; k++) {
         for (int j = 0; j < b_num[k]; j++) if (mol[b_j[b_first[k] + j]] != mol[k]) xmol[k] = 1;
         for (int j = 0; j < a_num[k]; j++)
@@ -455,6 +457,9 @@ static int setup(mtmd_handle* h, const mtmd_system* s)
         for (int j = 0; j < d_num[k]; j++)
             if (!d_center[d_first[k] + j] && mol[d_l[d_first[k] + j]] != mol[k]) xmol[k] = 1;
     }
+
+    h->d.has_xmol = 0;
+    for (int k = 0; k < n; k++) if (xmol[k]) { h->d.has_xmol = 1; break; }
 
     // --- special pairs for the FP64/MIXED engines: every same-molecule pair and
     // every scaled cross-molecule partner, with the reference's scale
@@ -596,6 +601,9 @@ static int setup(mtmd_handle* h, const mtmd_system* s)
                 }
             }
         UP(d.nbcoef, coef);
+        std::vector<float2> coef2(coef.size());
+        for (size_t i = 0; i < coef.size(); i++) coef2[i] = make_float2(coef[i].x, coef[i].y);
+        UP(d.nbcoef2, coef2);
         std::vector<int> ident(n);
         std::iota(ident.begin(), ident.end(), 0);
         UP(d.perm, ident); UP(d.inv, ident);

@@ -52,8 +52,11 @@ struct MtDev {
     int n, T;
     double4* pos; double4* mom; double4* frc; float* epot;
     int* type; int* mol; uint8_t* mobile; uint8_t* xmol;  // xmol: bonded partner outside own molecule
+    int has_molecules;     // some molecule has more than one atom
+    int has_xmol;          // some atom has a bonded partner outside its molecule
     float4* nbtab; int* nbcnt; float4* btab; float4* atab; float4* dtab;
     float4* nbcoef;
+    float2* nbcoef2;       // force-only a_f, b_f per segment (MIXED, staged in LDS)
     MtBonded bl;
     MtSpecial sp;
     // cells

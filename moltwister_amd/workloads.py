@@ -108,9 +108,14 @@ WATER = dict(kb=1059.16 * 4.184 / 2.0, r0=1.012, ktheta=75.90 * 4.184 / 2.0, the
 
 
 def water_box(n_mol: int = 10000, density_g_cm3: float = 1.0, T_kelvin: float = 298.0, seed: int = 2,
-              jitter: float = 0.3) -> System:
-    """Configuration 2 (and smaller boxes): waters at random orientations on a
-    jittered cubic grid, atoms ordered O, H, H per molecule."""
+              jitter: float = 0.3, min_oo: float = 2.6, min_oh: float = 1.8, min_hh: float = 1.6,
+              tries: int = 200) -> System:
+    """Configuration 2 (and smaller boxes): waters on a jittered cubic grid at
+    random orientations, atoms ordered O, H, H per molecule. Orientations are
+    redrawn until every intermolecular O-O, O-H and H-H distance to the
+    molecules already placed clears min_oo/min_oh/min_hh: the model has no LJ
+    on H, so an H placed against a neighbouring O would collapse onto it
+    under the bare Coulomb attraction."""
     w = WATER
     m_mol = w["mO"] + 2 * w["mH"]
     n_per_A3 = density_g_cm3 * 0.6022140857 / m_mol       # molecules per A^3
@@ -125,11 +130,40 @@ def water_box(n_mol: int = 10000, density_g_cm3: float = 1.0, T_kelvin: float = 
     tmpl = np.array([[0.0, 0.0, 0.0],
                      [w["r0"], 0.0, 0.0],
                      [w["r0"] * math.cos(th), w["r0"] * math.sin(th), 0.0]])
-    R = _random_rotations(rng, n_mol)
-    pos = (centres[:, None, :] + np.einsum("mij,aj->mai", R, tmpl)).reshape(-1, 3)
+    tmpl -= tmpl.mean(axis=0)
+    # placed molecules by grid site, to find neighbours
+    site_of = {tuple(sc): m for m, sc in enumerate(sites)}
+    pos = np.zeros((n_mol, 3, 3))
+    placed = np.zeros(n_mol, bool)
+    is_h = np.array([False, True, True])
+    thr = np.where(is_h[:, None] & is_h[None, :], min_hh,
+                   np.where(~is_h[:, None] & ~is_h[None, :], min_oo, min_oh))
+    offs = np.stack(np.meshgrid(*(np.arange(-1, 2),) * 3, indexing="ij"), -1).reshape(-1, 3)
+    offs = offs[np.any(offs != 0, axis=1)]
+    for m in range(n_mol):
+        nb = [site_of.get(tuple((sites[m] + o) % g)) for o in offs]
+        nb = [j for j in nb if j is not None and placed[j]]
+        other = pos[nb] if nb else np.zeros((0, 3, 3))
+        best, best_score = None, -np.inf
+        for _ in range(tries):
+            R = _random_rotations(rng, 1)[0]
+            cand = centres[m] + tmpl @ R.T
+            if len(nb) == 0:
+                best = cand
+                break
+            d = cand[None, :, None, :] - other[:, None, :, :]
+            d -= L * np.round(d / L)
+            dist = np.sqrt(np.sum(d * d, axis=-1))            # (nb, 3 own, 3 other)
+            score = np.min(dist - thr[None, :, :])
+            if score > best_score:
+                best, best_score = cand, score
+            if score >= 0.0:
+                break
+        pos[m] = best
+        placed[m] = True
+    pos = pos.reshape(-1, 3)
     pos = (pos + L / 2.0) % L - L / 2.0
     n = 3 * n_mol
-    names = ["O", "H", "H"] * n_mol
     types = np.tile(np.array([0, 1, 1], np.int32), n_mol)
     mass = np.tile(np.array([w["mO"], w["mH"], w["mH"]]), n_mol)
     charge = np.tile(np.array([w["qO"], w["qH"], w["qH"]]), n_mol)

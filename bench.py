@@ -170,13 +170,24 @@ def main() -> int:
     if not args.skip_water:
         from moltwister_amd.engine import MtmdError
         sysW = W.water_box(10000, seed=2)
-        for eng_id, name in ((abi.ENGINE_MIXED, "mixed"), (abi.ENGINE_FP64, "fp64")):
+        # relax the generated packing first (moldyn optimizeenergy), as a run would
+        cwr = water_config(abi.ENGINE_MIXED)
+        cwr.coulomb = abi.COUL_ALLPAIRS
+        relax = Engine(sysW, cwr)
+        relax.optimize(300, 1e-4, 0.0)
+        sysW.pos, _ = relax.get_state()
+        relax.close()
+        for eng_id, coul, name in ((abi.ENGINE_MIXED, abi.COUL_ALLPAIRS, "mixed_allpairs"),
+                                   (abi.ENGINE_FP64, abi.COUL_ALLPAIRS, "fp64_allpairs"),
+                                   (abi.ENGINE_MIXED, abi.COUL_CUTOFF, "mixed_cutoff10"),
+                                   (abi.ENGINE_FP64, abi.COUL_CUTOFF, "fp64_cutoff10")):
             cwt = water_config(eng_id)
+            cwt.coulomb = coul
             cwt.block_size = args.block
             try:
                 ew = Engine(sysW, cwt)
                 ew.step(20)
-                kw = 200 if name == "mixed" else 100
+                kw = 200 if eng_id == abi.ENGINE_MIXED else 100
                 dtw = timed_steps(ew, kw)
                 water[name] = sysW.n_atoms * kw / dtw
                 water[name + "_T"] = ew.thermo()["T"]

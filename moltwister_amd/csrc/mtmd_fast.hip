@@ -505,19 +505,21 @@ __global__ void __launch_bounds__(TPB) k_fast_bonded_special(MtDev d, MtParams p
             const int a = d.inv[po];
             const double4 ra4 = d.pos[a];
             const mt_d3 ra = {ra4.x, ra4.y, ra4.z};
-            if (!(mt_dist2_pbc(rk, ra, p.wnl) < (double)p.rc2f)) continue;
+            const bool inside = mt_dist2_pbc(rk, ra, p.wnl) < (double)p.rc2f;
+            const bool coul_here = coul && (p.coulomb_allpairs || inside);
+            if (!inside && !coul_here) continue;
             const mt_d3 ri = mt_move_same_side(rk, ra, p.wbox);
             float u = 0.0f;
             mt_d3 f = {0.0, 0.0, 0.0};
             const float r = (float)mt_norm(mt_sub(rk, ri));
-            if (!(r < 1E-3f)) {
+            if (inside && !(r < 1E-3f)) {
                 const int col = d.type[a];
                 float mdU = 0.0f;
                 for (int ff = 0; ff < d.nbcnt[row * T + col]; ff++)
                     mt_table_eval(d.nbtab + ((size_t)(row * T + col) * MT_MAXFF + ff) * MT_NPTS, r, true, mdU, u);
                 f = mt_mul(mt_coeffs12(ri, rk), (double)mdU);
             }
-            if (coul) {
+            if (coul_here) {
                 float uc = 0.0f;
                 f = mt_add(f, mt_coulomb_pair(rk, ri, (float)rk4.w, (float)ra4.w, -1, 0, uc));
                 u += uc;
@@ -531,9 +533,145 @@ __global__ void __launch_bounds__(TPB) k_fast_bonded_special(MtDev d, MtParams p
     if (want_e) d.epot[k] += U;
 }
 
+// ---------------------------------------------------------------------------
+// all-pairs minimum-image Coulomb for the sorted engines (the reference's
+// Coulomb, FunctorCalcForce.cpp:258-278, 514-535). Each workgroup owns 256
+// atoms k and one of `nsplit` slices of the atoms i, staged through LDS;
+// slice sums land in part[split][k] and k_ap_sum adds them in split order, so
+// the result does not depend on scheduling. Same-molecule pairs and bonded
+// partners are left to the special-pair pass.
+// ---------------------------------------------------------------------------
+#define AP_TPB 256
+__global__ void __launch_bounds__(AP_TPB) k_ap_coulomb_mixed(MtDev d, MtParams p, int want_e)
+{
+    __shared__ float4 tq[AP_TPB];
+    __shared__ int tm[AP_TPB];
+    const int t = threadIdx.x;
+    const int k = blockIdx.x * AP_TPB + t;
+    const bool act = k < d.n;
+    const int nsplit = gridDim.y, split = blockIdx.y;
+    const int per = (((d.n + nsplit - 1) / nsplit) + AP_TPB - 1) / AP_TPB * AP_TPB;
+    const int i0 = split * per, i1 = min(d.n, i0 + per);
+    float xk = 0.f, yk = 0.f, zk = 0.f, qk = 0.f;
+    int molk = -1;
+    bool xk_ = false;
+    if (act) {
+        const double4 r4 = d.pos[k];
+        xk = (float)r4.x; yk = (float)r4.y; zk = (float)r4.z;
+        qk = d.qf[k];
+        molk = d.has_molecules ? d.mol[k] : -1;
+        xk_ = d.has_xmol && d.xmol[d.perm[k]] != 0;
+    }
+    const float Lx = (float)p.L[0], Ly = (float)p.L[1], Lz = (float)p.L[2];
+    const float iLx = 1.0f / Lx, iLy = 1.0f / Ly, iLz = 1.0f / Lz;
+    double Fx = 0.0, Fy = 0.0, Fz = 0.0;
+    float U = 0.0f;
+    for (int base = i0; base < i1; base += AP_TPB) {
+        __syncthreads();
+        const int i = base + t;
+        if (i < i1) {
+            const double4 q = d.pos[i];
+            tq[t] = make_float4((float)q.x, (float)q.y, (float)q.z, d.qf[i]);
+            tm[t] = d.has_molecules ? d.mol[i] : -1;
+        }
+        __syncthreads();
+        if (!act || qk == 0.0f) continue;
+        const int lim = min(AP_TPB, i1 - base);
+        float fx = 0.f, fy = 0.f, fz = 0.f, u = 0.f;
+        for (int e = 0; e < lim; e++) {
+            const float4 q = tq[e];
+            float dx = xk - q.x, dy = yk - q.y, dz = zk - q.z;
+            dx -= Lx * rintf(dx * iLx);
+            dy -= Ly * rintf(dy * iLy);
+            dz -= Lz * rintf(dz * iLz);
+            const float r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 > 0.0f && q.w != 0.0f && (molk < 0 || tm[e] != molk)) {
+                if (xk_ && mt_is_partner(d, k, base + e)) continue;
+                const float rinv = rsqrtf(r2);
+                const float K = MT_COULOMB_K * qk * q.w;
+                const float sc = K * rinv * rinv * rinv;
+                fx += dx * sc; fy += dy * sc; fz += dz * sc;
+                u += K * rinv;
+            }
+        }
+        Fx += (double)fx; Fy += (double)fy; Fz += (double)fz;
+        U += 0.5f * u;
+    }
+    if (act) {
+        d.ap_part[(size_t)split * d.n + k] = make_double4(Fx, Fy, Fz, (double)U);
+    }
+}
+
+// FP64 engine: the reference's pair arithmetic (mt_coulomb_pair) on double positions
+__global__ void __launch_bounds__(AP_TPB) k_ap_coulomb_fp64(MtDev d, MtParams p, int want_e)
+{
+    __shared__ double sx[AP_TPB], sy[AP_TPB], sz[AP_TPB];
+    __shared__ float sq[AP_TPB];
+    __shared__ int sm[AP_TPB];
+    const int t = threadIdx.x;
+    const int k = blockIdx.x * AP_TPB + t;
+    const bool act = k < d.n;
+    const int nsplit = gridDim.y, split = blockIdx.y;
+    const int per = (((d.n + nsplit - 1) / nsplit) + AP_TPB - 1) / AP_TPB * AP_TPB;
+    const int i0 = split * per, i1 = min(d.n, i0 + per);
+    mt_d3 rk = {0.0, 0.0, 0.0};
+    float qk = 0.f;
+    int molk = -1;
+    bool xk_ = false;
+    if (act) {
+        const double4 r4 = d.pos[k];
+        rk = {r4.x, r4.y, r4.z};
+        qk = (float)r4.w;
+        molk = d.has_molecules ? d.mol[k] : -1;
+        xk_ = d.has_xmol && d.xmol[d.perm[k]] != 0;
+    }
+    mt_d3 F = {0.0, 0.0, 0.0};
+    float U = 0.0f;
+    for (int base = i0; base < i1; base += AP_TPB) {
+        __syncthreads();
+        const int i = base + t;
+        if (i < i1) {
+            const double4 q = d.pos[i];
+            sx[t] = q.x; sy[t] = q.y; sz[t] = q.z;
+            sq[t] = (float)q.w;
+            sm[t] = d.has_molecules ? d.mol[i] : -1;
+        }
+        __syncthreads();
+        if (!act || qk == 0.0f) continue;
+        const int lim = min(AP_TPB, i1 - base);
+        for (int e = 0; e < lim; e++) {
+            const float qi = sq[e];
+            if (qi == 0.0f || base + e == k) continue;
+            if (molk >= 0 && sm[e] == molk) continue;
+            if (xk_ && mt_is_partner(d, k, base + e)) continue;
+            const mt_d3 ri = mt_move_same_side(rk, {sx[e], sy[e], sz[e]}, p.wbox);
+            float u = 0.0f;
+            F = mt_add(F, mt_coulomb_pair(rk, ri, qk, qi, -1, 0, u));
+            U += 0.5f * u;
+        }
+    }
+    if (act) d.ap_part[(size_t)split * d.n + k] = make_double4(F.x, F.y, F.z, (double)U);
+}
+
+__global__ void k_ap_sum(MtDev d, int nsplit, int want_e)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= d.n || !d.mobile[k]) return;
+    double fx = 0.0, fy = 0.0, fz = 0.0, u = 0.0;
+    for (int sp = 0; sp < nsplit; sp++) {
+        const double4 v = d.ap_part[(size_t)sp * d.n + k];
+        fx += v.x; fy += v.y; fz += v.z; u += v.w;
+    }
+    const double4 f0 = d.frc[k];
+    d.frc[k] = make_double4(f0.x + fx, f0.y + fy, f0.z + fz, 0.0);
+    if (want_e) d.epot[k] += (float)u;
+}
+
 void mt_fast_forces(const MtDev& d, const MtParams& p, int mixed, int want_e, int block, hipStream_t s)
 {
-    const bool coul = p.coulomb_on != 0;
+    const bool coul_all = p.coulomb_on != 0;
+    const bool ap = coul_all && p.coulomb_allpairs;
+    const bool coul = coul_all && !ap;      // Coulomb inside the list kernels
     const size_t lds_tab = (size_t)d.T * d.T * MT_NPTS * sizeof(float2);
     if (mixed && !want_e && lds_tab <= 65536) {
         // persistent-style grid: enough workgroups to fill every CU a few times
@@ -565,9 +703,15 @@ void mt_fast_forces(const MtDev& d, const MtParams& p, int mixed, int want_e, in
             else hipLaunchKernelGGL((k_fast_force_fp64<false, false>), g, b, 0, s, d, p);
         }
     }
+    if (ap) {
+        dim3 g(nblk(d.n, AP_TPB), d.ap_nsplit);
+        if (mixed) hipLaunchKernelGGL(k_ap_coulomb_mixed, g, dim3(AP_TPB), 0, s, d, p, want_e);
+        else hipLaunchKernelGGL(k_ap_coulomb_fp64, g, dim3(AP_TPB), 0, s, d, p, want_e);
+        hipLaunchKernelGGL(k_ap_sum, dim3(nblk(d.n, TPB)), dim3(TPB), 0, s, d, d.ap_nsplit, want_e);
+    }
     if (d.bl.has_bonds || d.bl.has_angles || d.bl.has_dihedrals || d.sp.n_entries > 0)
         hipLaunchKernelGGL(k_fast_bonded_special, dim3(nblk(d.n, TPB)), dim3(TPB), 0, s, d, p, want_e,
-                           coul ? 1 : 0);
+                           coul_all ? 1 : 0);
 }
 
 // ---------------------------------------------------------------------------

@@ -307,8 +307,6 @@ static int setup(mtmd_handle* h, const mtmd_system* s)
     for (int k = 0; k < n; k++) if ((float)s->charge[k] != 0.0f) any_q = true;
     int cm = c.coulomb;
     if (cm == MTMD_COUL_AUTO) cm = h->fast ? MTMD_COUL_CUTOFF : MTMD_COUL_ALLPAIRS;
-    if (h->fast && cm == MTMD_COUL_ALLPAIRS)
-        return h->fail(MTMD_EINVAL, "coulomb allpairs is available in the exact engine only");
     p.coulomb_allpairs = (cm == MTMD_COUL_ALLPAIRS);
     p.coulomb_on = any_q ? 1 : 0;
 
@@ -616,6 +614,13 @@ static int setup(mtmd_handle* h, const mtmd_system* s)
         d.home = h->mem.alloc<int>(n);
         d.qf = h->mem.alloc<float>(n);
         d.qf2 = h->mem.alloc<float>(n);
+        if (p.coulomb_on && p.coulomb_allpairs) {
+            // enough (atom block, slice) workgroups to fill the chip: ~4 per CU
+            const int blocks = (n + 255) / 256;
+            d.ap_nsplit = std::max(1, std::min(64, (256 * 4 + blocks - 1) / blocks));
+            d.ap_part = h->mem.alloc<double4>((size_t)d.ap_nsplit * n);
+            if (!d.ap_part) return h->fail(MTMD_ENOMEM, "device allocation failed");
+        }
         std::vector<float> qf(n);
         for (int k = 0; k < n; k++) qf[k] = (float)s->charge[k];
         HIPCHK(h, hipMemcpy(d.qf, qf.data(), n * sizeof(float), hipMemcpyHostToDevice));

@@ -72,6 +72,8 @@ struct MtDev {
     float* qf; float* qf2;  // float charges in slot order
     double4* posref;       // positions at the last rebuild
     float4* posf; int* home;
+    double4* ap_part;      // all-pairs Coulomb partial sums [ap_nsplit][n] (FP64/MIXED)
+    int ap_nsplit;
     // reductions / scalars
     double* red;           // block partials
     int red_blocks;

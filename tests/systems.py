@@ -26,6 +26,7 @@ SYSTEMS = {
     "buckingham": (lambda: W.binary_lj(800, seed=12, buckingham=True), 2.5, 0.3),
     "water_300": (lambda: W.water_box(300, seed=5), 6.0, 0.5),
     "chains": (lambda: W.chain_molecules(30, 6, 24.0, seed=4), 8.0, 0.0),
+    "chains_rc7": (lambda: W.chain_molecules(30, 6, 24.0, seed=4), 7.0, 0.0),   # >= 3 cells of rc+skin
 }
 
 

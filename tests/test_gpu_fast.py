@@ -133,3 +133,45 @@ def test_nonfinite_position_reported(engine):
     e.set_state(pos=s.pos)
     F, _ = e.compute_forces()
     assert np.isfinite(F).all()
+
+
+@pytest.mark.parametrize("name", ["water_300", "chains_rc7"])
+@pytest.mark.parametrize("engine", [abi.ENGINE_FP64, abi.ENGINE_MIXED])
+def test_allpairs_coulomb_vs_oracle(oracle_mod, name, engine):
+    """All-pairs minimum-image Coulomb (the reference's own Coulomb) in the
+    sorted engines against the oracle's all-pairs forces."""
+    print(f"case allpairs {name} engine={engine}", flush=True)
+    s, c = make(name, engine=engine, coulomb=abi.COUL_ALLPAIRS)
+    co = make(name, coulomb=abi.COUL_ALLPAIRS)[1]
+    o, e = oracle_mod.Oracle(s, co), Engine(s, c)
+    pos = s.pos + np.random.default_rng(4).normal(0, 0.02, s.pos.shape)
+    o.set_state(pos=pos)
+    e.set_state(pos=pos)
+    Fo, Uo = o.compute_forces()
+    Fe, Ue = e.compute_forces()
+    if engine == abi.ENGINE_FP64:
+        scale = np.maximum(np.linalg.norm(Fo, axis=1), np.sqrt(np.mean(np.sum(Fo ** 2, 1))))
+        assert (np.linalg.norm(Fe - Fo, axis=1) / scale).max() < 1e-11
+        assert abs(Ue.astype(np.float64).sum() - Uo.astype(np.float64).sum()) <= 1e-5 * np.abs(Uo).sum()
+    else:
+        assert rms_rel(Fe, Fo) < 1e-5, rms_rel(Fe, Fo)
+        assert abs(Ue.astype(np.float64).sum() - Uo.astype(np.float64).sum()) <= 1e-4 * np.abs(Uo).sum()
+
+
+@pytest.mark.parametrize("engine", [abi.ENGINE_FP64, abi.ENGINE_MIXED])
+def test_water_nve_energy_allpairs(engine):
+    """Relaxed water with all-pairs Coulomb: 200 NVE steps conserve the total
+    energy (bonded, angle, LJ and Coulomb terms through the sorted engines)."""
+    from moltwister_amd import workloads as W
+    print(f"case water nve engine={engine}", flush=True)
+    s = W.water_box(300, seed=5)
+    c = make("water_300", engine=engine, coulomb=abi.COUL_ALLPAIRS)[1]
+    c.cutoff_force = 1e9
+    e = Engine(s, c)
+    e.optimize(300, 1e-4, 0.0)
+    e.step(20)
+    E0 = e.thermo()["Etot"]
+    e.step(200)
+    th = e.thermo()
+    assert np.isfinite(th["Etot"])
+    assert abs(th["Etot"] - E0) < 2e-3 * abs(E0), (th["Etot"], E0)

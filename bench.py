@@ -67,12 +67,43 @@ def rms_rel(F, Fr):
     return float(np.sqrt(np.sum((F - Fr) ** 2) / np.sum(Fr ** 2)))
 
 
+def energy_drift(th0, th1):
+    """|E(t) - E(0)| / |E(0)| of the total energy, and of Etot - U_cut (the
+    energy without the jumps of the unshifted truncated pair potential, which
+    the integrator conserves)."""
+    e0, e1 = th0["Etot"], th1["Etot"]
+    s0, s1 = e0 - th0["U_cut"], e1 - th1["U_cut"]
+    return abs(e1 - e0) / abs(e0), abs(s1 - s0) / abs(s0)
+
+
 def timed_steps(eng, k):
     eng.synchronize()
     t0 = time.perf_counter()
     eng.step(k)
     eng.synchronize()
     return time.perf_counter() - t0
+
+
+def kernels_only(args, warm, sysA) -> int:
+    """Average launch times (HIP events) of the dominant kernels after a short
+    warm-up: 50 NVE steps from the relaxed packing."""
+    from moltwister_amd import abi
+    from moltwister_amd.engine import Engine
+    warm.step(50)
+    sysA.pos, sysA.mom = warm.get_state()
+    warm.close()
+    out = {"workload": "1,048,576-atom binary LJ after 100 descent + 50 NVE steps"}
+    for name, eng_id in (("mixed", abi.ENGINE_MIXED), ("fp64", abi.ENGINE_FP64)):
+        c = lj_config(eng_id, abi.NVE)
+        c.block_size = args.block
+        e = Engine(sysA, c)
+        e.step(5)
+        out[name] = {"force_ms": e.bench_kernel(0, 5), "build_ms": e.bench_kernel(1, 3),
+                     "copy_ms": e.bench_kernel(3, 5)}
+        log(f"bench: {name} {out[name]}")
+        e.close()
+    print(json.dumps(out))
+    return 0
 
 
 def main() -> int:
@@ -83,6 +114,9 @@ def main() -> int:
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-water", action="store_true")
     ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="short warm-up, then only the kernel timings (force pass, list build, copy) of the "
+                         "mixed and FP64 engines; for profiler runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described in DESIGN.md "
@@ -108,6 +142,8 @@ def main() -> int:
     cw.block_size = args.block
     warm = Engine(sysA, cw)
     warm.optimize(100, 2e-5, 0.0)
+    if args.kernels_only:
+        return kernels_only(args, warm, sysA)
     for rnd in range(3):
         warm.step(Wm)
         pos, mom = warm.get_state()
@@ -130,8 +166,7 @@ def main() -> int:
     st1 = em.stats()
     value = n * K / dt_m
     rebuild_every = K / max(1, st1["rebuilds"] - st0["rebuilds"])
-    drift = abs(th1["Etot"] - th0["Etot"]) / abs(th0["Etot"])
-    drift_1e4 = drift * (1e4 / K)
+    drift, drift_s = energy_drift(th0, th1)
     log(f"bench: mixed {dt_m / K * 1e3:.3f} ms/step, {value:.4g} atom-steps/s, rebuild every {rebuild_every:.1f}")
 
     # dominant kernel: the pair-force pass, timed with HIP events on the engine's stream
@@ -159,9 +194,11 @@ def main() -> int:
     F64, _ = e64.compute_forces()
     force_rms = rms_rel(F_mixed, F64)
     e64.step(10)
+    th64_0 = e64.thermo()
     K64 = max(50, K // 5)
     dt_64 = timed_steps(e64, K64)
     value64 = n * K64 / dt_64
+    drift64, drift64_s = energy_drift(th64_0, e64.thermo())
     t_force64 = e64.bench_kernel(0, 10) * 1e-3
     e64.close()
     log(f"bench: fp64 {dt_64 / K64 * 1e3:.3f} ms/step, {value64:.4g} atom-steps/s; force rms rel {force_rms:.2e}")
@@ -256,7 +293,16 @@ def main() -> int:
         "cpu_baseline": cpu,
         "fp64": {"value": value64, "ms_per_step": dt_64 / K64 * 1e3, "steps": K64,
                  "force_kernel_ms": t_force64 * 1e3, "mixed_speedup": value / value64},
-        "precision": {"force_rms_rel_vs_fp64": force_rms, "nve_drift_rel": drift, "nve_drift_per_1e4_steps": drift_1e4},
+        "precision": {"force_rms_rel_vs_fp64": force_rms,
+                      "nve_drift_rel": drift, "nve_drift_steps": K,
+                      "nve_drift_shifted_rel": drift_s,
+                      "nve_drift_per_1e4_steps_linear": drift * 1e4 / K,
+                      "nve_drift_shifted_per_1e4_steps_linear": drift_s * 1e4 / K,
+                      "fp64_nve_drift_rel": drift64, "fp64_nve_drift_shifted_rel": drift64_s,
+                      "fp64_drift_steps": K64,
+                      "note": "drift = |E(K)-E(0)|/|E(0)| over the timed steps; 'shifted' uses Etot - U_cut, "
+                              "which has no jumps from pairs crossing the unshifted cutoff; per-1e4 values "
+                              "are linear extrapolations (an upper bound for a random walk)"},
         "step_roofline": {"bytes_per_atom_step": step_bytes, "frac": value * step_bytes / HBM_PEAK,
                           "rebuild_every_steps": rebuild_every, "list_build_ms": t_build * 1e3,
                           "copy_GBps": copy_bw / 1e9},

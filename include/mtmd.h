@@ -173,6 +173,12 @@ typedef struct mtmd_thermo {
     double Etot;     /* kJ/mol              */
     double sum_p2m;  /* sum p.p/m (reduced) */
     double sum_fr;   /* sum F.r   (reduced) */
+    double U_cut;    /* kJ/mol: sum over pairs inside the cutoff of the pair
+                        energy at the cutoff (scaled like the pair). The pair
+                        potentials are truncated without a shift, so U jumps
+                        by U(rc) when a pair crosses rc; Etot - U_cut does not
+                        jump and is what the integrator conserves. No
+                        counterpart in the reference.                        */
 } mtmd_thermo;
 
 typedef struct mtmd_handle mtmd_handle;

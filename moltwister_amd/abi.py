@@ -109,7 +109,7 @@ class MtmdSystem(C.Structure):
 
 
 class MtmdThermo(C.Structure):
-    _fields_ = [(k, C.c_double) for k in ("T", "P", "V", "U", "K", "Etot", "sum_p2m", "sum_fr")]
+    _fields_ = [(k, C.c_double) for k in ("T", "P", "V", "U", "K", "Etot", "sum_p2m", "sum_fr", "U_cut")]
 
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))

@@ -21,6 +21,25 @@
 
 static inline int nblk(int n, int b) { return (n + b - 1) / b; }
 
+// Workgroups of a launch are dispatched round-robin over the 8 XCDs of the
+// chip (workgroup b on XCD b % 8), and each XCD has its own L2. Atoms are
+// spatially sorted, so giving every XCD one contiguous range of logical blocks
+// keeps the positions its workgroups gather (a slab of the box plus a halo)
+// inside that XCD's L2. The map is a bijection on [0, nb).
+__device__ __forceinline__ int mt_xcd_block(int b, int nb)
+{
+    const int x = b & 7, q = nb >> 3, r = nb & 7;
+    return x * q + min(x, r) + (b >> 3);
+}
+
+// cell-offset shift (ox, oy, oz) * cell size of neighbour-word code 0..26
+__device__ __forceinline__ void mt_fill_shifts(float4* sh, const MtParams& p)
+{
+    for (int c = threadIdx.x; c < 27; c += blockDim.x)
+        sh[c] = make_float4((float)(c % 3 - 1) * p.cs[0], (float)((c / 3) % 3 - 1) * p.cs[1],
+                            (float)(c / 9 - 1) * p.cs[2], 0.f);
+}
+
 __device__ __forceinline__ bool mt_skip_rebuild(int cond, const unsigned int* flags, float thr)
 {
     return cond && !(__uint_as_float(flags[0]) > thr);
@@ -226,93 +245,190 @@ __device__ inline bool mt_is_partner(const MtDev& d, int k, int ia)
     return false;
 }
 
-// neighbour list at rlist. One 64-lane workgroup builds the lists of two home
-// cells, one per half-wave (a cell holds ~27 atoms at rho* = 1.2 with cells of
-// cutoff + skin). For each of the 27 cell offsets, the atoms of each half's
-// neighbouring cell are staged in that half of an LDS tile, already shifted by
-// the offset, and each lane (one atom of its home cell) scans its half with
-// broadcast reads. Same-molecule pairs and cross-molecule bonded partners are
-// left to the special-pair pass.
-#define NB_HALF 32
-__global__ void __launch_bounds__(64) k_fast_neigh_cell(MtDev d, MtParams p, int cond)
+// Neighbour list at rlist. One 256-thread workgroup builds the lists of a
+// 2x2x2 block of home cells, one cell per half-wave (a cell holds ~26 atoms at
+// rho* = 1.2 with cells of cutoff + skin). The 4x4x4 cells around the block
+// (the union of the eight 27-cell neighbourhoods) are staged once in LDS as
+// (x, y, z, slot) in cell order; every lane (one atom of a home cell) then
+// scans the 27 staged cells of its own neighbourhood with broadcast reads,
+// two candidates per iteration. Each lane keeps its last three hits in
+// registers and writes every fourth as one 16-byte chunk of the chunked
+// column-major layout. When the 64 staged cells hold more than NB_CAP atoms
+// they are staged in batches of whole cells.
+//
+// MODE 0: no molecules. MODE 1: same-molecule pairs are left out (molecule
+// ids staged in LDS beside the positions). MODE 2: as 1, and bonded partners
+// in other molecules are left out too (a global-memory search of the atom's
+// bonded lists). Left-out pairs belong to the special-pair pass. Keeping
+// global loads out of the scan loop matters: on this architecture the
+// counter that orders loads also counts stores, so a load in the loop would
+// make every iteration wait for the previous chunk store.
+#define NB_CAP 2048
+#define NB_PAD_CODE 13u      // offset code of (0, 0, 0): a pad word is the atom itself
+
+template <int MODE>
+__device__ __forceinline__ void mt_neigh_hit(const MtDev& d, uint4* nl4, float4 c, uint32_t code, int k, int molk,
+                                             const int* smol, int e, int n, int ncap, int& cnt, uint32_t& b0,
+                                             uint32_t& b1, uint32_t& b2)
+{
+    const int a = (int)__float_as_uint(c.w);
+    if (a == k) return;
+    if constexpr (MODE >= 1) {
+        if (smol[e] == molk) return;
+    }
+    if constexpr (MODE == 2) {
+        if (mt_is_partner(d, k, a)) return;
+    }
+    const uint32_t w = (uint32_t)a | code;
+    const int slot = cnt & 3;
+    b0 = slot == 0 ? w : b0;
+    b1 = slot == 1 ? w : b1;
+    b2 = slot == 2 ? w : b2;
+    if (slot == 3 && cnt < ncap) nl4[(size_t)(cnt >> 2) * n + k] = make_uint4(b0, b1, b2, w);
+    cnt++;
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(256) k_fast_neigh_block(MtDev d, MtParams p, int cond)
 {
     if (mt_skip_rebuild(cond, d.flags, p.half_skin2)) return;
+    __shared__ float4 spos[NB_CAP];
+    __shared__ int smol_buf[MODE >= 1 ? NB_CAP : 1];
+    __shared__ int s_first[64], s_cnt[64], s_off[65];
+    __shared__ int s_hfirst[8], s_hcnt[8];
+    int* smol = smol_buf;
     const int t = threadIdx.x;
-    const int h = t >> 5, l = t & 31;
-    const int c = 2 * blockIdx.x + h;
-    const bool cell_ok = c < d.ncells;
-    __shared__ float4 tile[64];
-    __shared__ uint32_t tword[64];
-    __shared__ int s_cnt[2];
-    if (t == 0 || t == 32) s_cnt[h] = cell_ok ? d.cell_count[c] : 0;
+    const int Nx = d.nc[0], Ny = d.nc[1], Nz = d.nc[2];
+    const int bx = (Nx + 1) >> 1, by = (Ny + 1) >> 1;
+    const int g = mt_xcd_block(blockIdx.x, gridDim.x);
+    const int x0 = 2 * (g % bx), y0 = 2 * ((g / bx) % by), z0 = 2 * (g / (bx * by));
+    // this half-wave's home cell: (x0 + hxo, y0 + hyo, z0 + hzo)
+    const int hc = t >> 5, l = t & 31;
+    const int hxo = hc & 1, hyo = (hc >> 1) & 1, hzo = hc >> 2;
+    const int hx = x0 + hxo, hy = y0 + hyo, hz = z0 + hzo;
+    const bool home_ok = hx < Nx && hy < Ny && hz < Nz;
+    if (t < 64) {
+        // staged cell q = (ox+1) + 4*((oy+1) + 4*(oz+1)), o* in -1..2 around (x0, y0, z0)
+        const int ox = (t & 3) - 1, oy = ((t >> 2) & 3) - 1, oz = (t >> 4) - 1;
+        int cx = x0 + ox, cy = y0 + oy, cz = z0 + oz;
+        cx = cx < 0 ? cx + Nx : (cx >= Nx ? cx - Nx : cx);
+        cy = cy < 0 ? cy + Ny : (cy >= Ny ? cy - Ny : cy);
+        cz = cz < 0 ? cz + Nz : (cz >= Nz ? cz - Nz : cz);
+        const int cc = cx + Nx * (cy + Ny * cz);
+        const int cn = d.cell_count[cc];
+        s_first[t] = d.cell_first[cc];
+        s_cnt[t] = cn;
+        // inclusive scan of the 64 counts across the wavefront
+        int v = cn;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int u = __shfl_up(v, o);
+            if (t >= o) v += u;
+        }
+        s_off[t + 1] = v;
+        if (t == 0) s_off[0] = 0;
+    }
+    if ((t & 31) == 0) {
+        const int hcell = home_ok ? hx + Nx * (hy + Ny * hz) : 0;
+        s_hfirst[hc] = home_ok ? d.cell_first[hcell] : 0;
+        s_hcnt[hc] = home_ok ? d.cell_count[hcell] : 0;
+    }
     __syncthreads();
-    const int cnt_c = s_cnt[h];
-    const int cnt_max = max(s_cnt[0], s_cnt[1]);
+    const int f0 = s_hfirst[hc], cnt_c = s_hcnt[hc];
+    int cnt_max = 0;
+    for (int i = 0; i < 8; i++) cnt_max = max(cnt_max, s_hcnt[i]);
     if (cnt_max == 0) return;
-    const int f0 = cell_ok ? d.cell_first[c] : 0;
     if (d.flags[3]) {
-        for (int i = l; i < cnt_c; i += NB_HALF) d.ncount[d.cell_list[f0 + i]] = 0;
+        for (int i = l; i < cnt_c; i += 32) d.ncount[d.cell_list[f0 + i]] = 0;
         return;
     }
-    const int Nx = d.nc[0], Ny = d.nc[1], Nz = d.nc[2];
-    const int cc0 = cell_ok ? c : 0;
-    const int hx = cc0 % Nx, hy = (cc0 / Nx) % Ny, hz = cc0 / (Nx * Ny);
-    const int ncap = d.ncap, n = d.n;
+    const int n = d.n, ncap = d.ncap;
     const float rl2 = p.rlist2;
-    for (int ib = 0; ib < cnt_max; ib += NB_HALF) {
+    uint4* __restrict__ nl4 = reinterpret_cast<uint4*>(d.nlist);
+    int staged_b = -1, staged_e = -1;
+    for (int ib = 0; ib < cnt_max; ib += 32) {
         const bool act = ib + l < cnt_c;
         const int k = act ? d.cell_list[f0 + ib + l] : 0;
         const float4 pk = act ? d.posf[k] : make_float4(0.f, 0.f, 0.f, 0.f);
-        const int molk = (act && d.has_molecules) ? d.mol[k] : -1;
-        const bool xk = act && d.has_xmol && d.xmol[d.perm[k]] != 0;
+        const int molk = (MODE >= 1 && act) ? d.mol[k] : -1;
+        // MODE 2: only atoms with a bonded partner outside their molecule search
+        const bool xk = MODE == 2 && act && d.xmol[d.perm[k]] != 0;
         int cnt = 0;
-        for (int oz = -1; oz <= 1; oz++) {
-            int cz = hz + oz; cz = cz < 0 ? cz + Nz : (cz >= Nz ? cz - Nz : cz);
-            for (int oy = -1; oy <= 1; oy++) {
-                int cy = hy + oy; cy = cy < 0 ? cy + Ny : (cy >= Ny ? cy - Ny : cy);
-                for (int ox = -1; ox <= 1; ox++) {
-                    int cx = hx + ox; cx = cx < 0 ? cx + Nx : (cx >= Nx ? cx - Nx : cx);
-                    const int cc = cx + Nx * (cy + Ny * cz);
-                    const uint32_t code = (uint32_t)((ox + 1) + 3 * (oy + 1) + 9 * (oz + 1)) << MT_NW_IDX_BITS;
-                    const float sx = (float)ox * p.cs[0], sy = (float)oy * p.cs[1], sz = (float)oz * p.cs[2];
-                    const int g0 = cell_ok ? d.cell_first[cc] : 0;
-                    const int gn = cell_ok ? d.cell_count[cc] : 0;
-                    __syncthreads();
-                    if (t == 0 || t == 32) s_cnt[h] = gn;
-                    __syncthreads();
-                    const int gmax = max(s_cnt[0], s_cnt[1]);
-                    for (int jb = 0; jb < gmax; jb += NB_HALF) {
-                        __syncthreads();
-                        if (jb + l < gn) {
-                            const int a = d.cell_list[g0 + jb + l];
-                            const float4 pa = d.posf[a];
-                            tile[t] = make_float4(pa.x + sx, pa.y + sy, pa.z + sz, 0.f);
-                            tword[t] = (uint32_t)a | code;
-                        }
-                        __syncthreads();
-                        if (act) {
-                            const int m = min(NB_HALF, gn - jb);
-                            const float4* tl = tile + (h << 5);
-                            const uint32_t* tw = tword + (h << 5);
-                            for (int e = 0; e < m; e++) {
-                                const float4 q = tl[e];
-                                const float dx = pk.x - q.x, dy = pk.y - q.y, dz = pk.z - q.z;
-                                if (dx * dx + dy * dy + dz * dz < rl2) {
-                                    const uint32_t w = tw[e];
-                                    const int a = (int)(w & MT_NW_IDX_MASK);
-                                    if (a == k) continue;
-                                    if (molk >= 0 && d.mol[a] == molk) continue;
-                                    if (xk && mt_is_partner(d, k, a)) continue;
-                                    if (cnt < ncap) d.nlist[(size_t)cnt * n + k] = w;
-                                    cnt++;
-                                }
+        uint32_t b0 = 0, b1 = 0, b2 = 0;
+        int cb = 0;
+        while (cb < 64) {
+            // batch of whole staged cells [cb, ce) that fits NB_CAP (uniform)
+            int ce = cb + 1;
+            while (ce < 64 && s_off[ce + 1] - s_off[cb] <= NB_CAP) ce++;
+            const int base = s_off[cb];
+            if (s_off[ce] - base > NB_CAP) {
+                // a single cell larger than the staging buffer
+                if (t == 0) d.flags[5] = 1u;
+                return;
+            }
+            if (cb != staged_b || ce != staged_e) {
+                __syncthreads();
+                for (int i = base + t; i < s_off[ce]; i += 256) {
+                    // staged cell of item i: binary search over s_off[cb..ce]
+                    int lo = cb, hi = ce;
+                    while (hi - lo > 1) {
+                        const int mid = (lo + hi) >> 1;
+                        if (s_off[mid] <= i) lo = mid; else hi = mid;
+                    }
+                    const int a = d.cell_list[s_first[lo] + (i - s_off[lo])];
+                    const float4 pa = d.posf[a];
+                    spos[i - base] = make_float4(pa.x, pa.y, pa.z, __uint_as_float((uint32_t)a));
+                    if constexpr (MODE >= 1) smol[i - base] = d.mol[a];
+                }
+                __syncthreads();
+                staged_b = cb;
+                staged_e = ce;
+            }
+            if (act) {
+                for (int oz = -1; oz <= 1; oz++)
+                    for (int oy = -1; oy <= 1; oy++)
+                        for (int ox = -1; ox <= 1; ox++) {
+                            const int q = (ox + 1 + hxo) + 4 * ((oy + 1 + hyo) + 4 * (oz + 1 + hzo));
+                            if (q < cb || q >= ce) continue;
+                            const uint32_t code = (uint32_t)((ox + 1) + 3 * (oy + 1) + 9 * (oz + 1)) << MT_NW_IDX_BITS;
+                            const float qx = pk.x - (float)ox * p.cs[0];
+                            const float qy = pk.y - (float)oy * p.cs[1];
+                            const float qz = pk.z - (float)oz * p.cs[2];
+                            const int e0 = s_off[q] - base, e1 = e0 + s_cnt[q];
+                            int e = e0;
+                            for (; e + 1 < e1; e += 2) {
+                                float4 c0 = spos[e], c1 = spos[e + 1];
+                                asm volatile("" : "+v"(c0.w), "+v"(c1.w));
+                                const float dx0 = qx - c0.x, dy0 = qy - c0.y, dz0 = qz - c0.z;
+                                const float dx1 = qx - c1.x, dy1 = qy - c1.y, dz1 = qz - c1.z;
+                                const bool h0 = dx0 * dx0 + dy0 * dy0 + dz0 * dz0 < rl2;
+                                const bool h1 = dx1 * dx1 + dy1 * dy1 + dz1 * dz1 < rl2;
+                                if (h0 && (MODE != 2 || !xk || !mt_is_partner(d, k, (int)__float_as_uint(c0.w))))
+                                    mt_neigh_hit<MODE == 2 ? 1 : MODE>(d, nl4, c0, code, k, molk, smol, e, n, ncap,
+                                                                         cnt, b0, b1, b2);
+                                if (h1 && (MODE != 2 || !xk || !mt_is_partner(d, k, (int)__float_as_uint(c1.w))))
+                                    mt_neigh_hit<MODE == 2 ? 1 : MODE>(d, nl4, c1, code, k, molk, smol, e + 1, n,
+                                                                         ncap, cnt, b0, b1, b2);
+                            }
+                            if (e < e1) {
+                                float4 c0 = spos[e];
+                                asm volatile("" : "+v"(c0.w));
+                                const float dx0 = qx - c0.x, dy0 = qy - c0.y, dz0 = qz - c0.z;
+                                if (dx0 * dx0 + dy0 * dy0 + dz0 * dz0 < rl2 &&
+                                    (MODE != 2 || !xk || !mt_is_partner(d, k, (int)__float_as_uint(c0.w))))
+                                    mt_neigh_hit<MODE == 2 ? 1 : MODE>(d, nl4, c0, code, k, molk, smol, e, n, ncap,
+                                                                         cnt, b0, b1, b2);
                             }
                         }
-                    }
-                }
             }
+            cb = ce;
         }
         if (act) {
+            // last partial chunk, padded with the atom itself (offset code 0,0,0)
+            const int slot = cnt & 3;
+            if (slot && cnt < ncap) {
+                const uint32_t pad = (uint32_t)k | (NB_PAD_CODE << MT_NW_IDX_BITS);
+                nl4[(size_t)(cnt >> 2) * n + k] = make_uint4(b0, slot > 1 ? b1 : pad, slot > 2 ? b2 : pad, pad);
+            }
             d.ncount[k] = cnt;
             if (cnt > ncap) { d.flags[2] = 1; atomicMax(d.nreq, cnt); }
         }
@@ -321,113 +437,150 @@ __global__ void __launch_bounds__(64) k_fast_neigh_cell(MtDev d, MtParams p, int
 
 void mt_fast_neighbors(const MtDev& d, const MtParams& p, int cond, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_fast_neigh_cell, dim3((d.ncells + 1) / 2), dim3(64), 0, s, d, p, cond);
+    const int nb = ((d.nc[0] + 1) / 2) * ((d.nc[1] + 1) / 2) * ((d.nc[2] + 1) / 2);
+    if (d.has_xmol)
+        hipLaunchKernelGGL(k_fast_neigh_block<2>, dim3(nb), dim3(256), 0, s, d, p, cond);
+    else if (d.has_molecules)
+        hipLaunchKernelGGL(k_fast_neigh_block<1>, dim3(nb), dim3(256), 0, s, d, p, cond);
+    else
+        hipLaunchKernelGGL(k_fast_neigh_block<0>, dim3(nb), dim3(256), 0, s, d, p, cond);
 }
 
 // ---------------------------------------------------------------------------
 // pair forces
 // ---------------------------------------------------------------------------
 
-// MIXED: FP32 pair math on local coordinates, FP64 per-atom accumulation
+// The FP64/MIXED neighbour lists are chunked column-major: word j of atom k
+// sits in 16-byte chunk j/4 at nlist[4 * ((j/4) * n + k) + j%4], so a
+// wavefront (consecutive atoms) loads one contiguous 1 KB line of chunks per
+// four neighbours. Unused words of an atom's last chunk hold the atom itself
+// with offset code (0, 0, 0), which gives r = 0 and no force.
+typedef unsigned int mt_u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint32_t mt_fast_word(const uint32_t* nl, int n, int k, int j)
+{
+    return nl[(((size_t)(j >> 2) * n + k) << 2) + (j & 3)];
+}
+
+// the float4 positions of a chunk's four neighbours
+__device__ __forceinline__ void mt_gather4(const float4* posf, mt_u32x4 w, float4& a0, float4& a1, float4& a2,
+                                           float4& a3)
+{
+    a0 = posf[w.x & MT_NW_IDX_MASK];
+    a1 = posf[w.y & MT_NW_IDX_MASK];
+    a2 = posf[w.z & MT_NW_IDX_MASK];
+    a3 = posf[w.w & MT_NW_IDX_MASK];
+}
+
+// one MIXED pair: FP32 geometry on local coordinates, segment coefficients
+// (a_f, b_f[, a_e, b_e]) of the pair's summed tables, FP64 accumulation
+template <bool WANT_E, bool COUL, typename CoefT>
+__device__ __forceinline__ void mt_mixed_pair(uint32_t w, const float4& pa, const float4& pk, const float4* sh,
+                                              const CoefT* row, float rc2, float x0, float invd, float qk,
+                                              const float* qf, double& Fx, double& Fy, double& Fz, float& U)
+{
+    const float4 o = sh[w >> MT_NW_IDX_BITS];
+    const float dx = pk.x - pa.x - o.x;
+    const float dy = pk.y - pa.y - o.y;
+    const float dz = pk.z - pa.z - o.z;
+    const float r2 = dx * dx + dy * dy + dz * dz;
+    if (r2 < rc2 && r2 >= 1.0e-6f) {
+        const float rinv = __builtin_amdgcn_rsqf(r2);
+        const float r = r2 * rinv;
+        float fr = 0.0f;
+        const int idx = (int)((r - x0) * invd);
+        if (idx >= 0 && idx < MT_NPTS - 1) {
+            const CoefT c = row[__float_as_int(pa.w) * MT_NPTS + idx];
+            fr = c.x + c.y * rinv;
+            if constexpr (WANT_E) U += 0.5f * (c.z * r + c.w);
+        }
+        if (COUL) {
+            const float K = MT_COULOMB_K * qk * qf[w & MT_NW_IDX_MASK];
+            fr += K * rinv * rinv * rinv;
+            if (WANT_E) U += 0.5f * K * rinv;
+        }
+        Fx += (double)(dx * fr);
+        Fy += (double)(dy * fr);
+        Fz += (double)(dz * fr);
+    }
+}
+
+// the pair loop of one atom over its chunked list: four gathers in flight
+// together, and the next chunk's words loaded while the current chunk is
+// computed
+template <bool WANT_E, bool COUL, typename CoefT>
+__device__ __forceinline__ void mt_mixed_atom(const MtDev& d, const MtParams& p, int k, const float4& pk,
+                                              const float4* sh, const CoefT* row, float qk, double& Fx,
+                                              double& Fy, double& Fz, float& U)
+{
+    const int nch = (min(d.ncount[k], d.ncap) + 3) >> 2;
+    if (nch == 0) return;
+    const mt_u32x4* __restrict__ nl4 = reinterpret_cast<const mt_u32x4*>(d.nlist) + k;
+    const size_t stride = (size_t)d.n;
+    const float rc2 = p.rc2f, x0 = p.x0, invd = p.inv_delta;
+    mt_u32x4 wn = __builtin_nontemporal_load(nl4);
+    for (int q = 0; q < nch; q++) {
+        const mt_u32x4 w = wn;
+        float4 a0, a1, a2, a3;
+        mt_gather4(d.posf, w, a0, a1, a2, a3);
+        // unconditional (the last iteration reloads its own chunk) so the
+        // load needs no branch and stays in flight across the pair math
+        wn = __builtin_nontemporal_load(nl4 + (size_t)min(q + 1, nch - 1) * stride);
+        // the empty asm makes the four whole 16-byte gathers complete here
+        // (while the next chunk's load stays in flight), so the type bits in
+        // .w are not fetched by a second, dependent load inside a pair branch
+        asm volatile("" : "+v"(a0.w), "+v"(a1.w), "+v"(a2.w), "+v"(a3.w));
+        mt_mixed_pair<WANT_E, COUL>(w.x, a0, pk, sh, row, rc2, x0, invd, qk, d.qf, Fx, Fy, Fz, U);
+        mt_mixed_pair<WANT_E, COUL>(w.y, a1, pk, sh, row, rc2, x0, invd, qk, d.qf, Fx, Fy, Fz, U);
+        mt_mixed_pair<WANT_E, COUL>(w.z, a2, pk, sh, row, rc2, x0, invd, qk, d.qf, Fx, Fy, Fz, U);
+        mt_mixed_pair<WANT_E, COUL>(w.w, a3, pk, sh, row, rc2, x0, invd, qk, d.qf, Fx, Fy, Fz, U);
+    }
+}
+
+// MIXED with energies: coefficients (a_f, b_f, a_e, b_e) read from global memory
 template <bool WANT_E, bool COUL>
 __global__ void __launch_bounds__(512) k_fast_force_mixed(MtDev d, MtParams p)
 {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ float4 sh[27];
+    mt_fill_shifts(sh, p);
+    __syncthreads();
+    int k = mt_xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (k >= d.n) return;
     const float4 pk = d.posf[k];
-    const int tk = __float_as_int(pk.w);
-    const int T = d.T;
-    const float4* __restrict__ coef = d.nbcoef + (size_t)tk * T * MT_NPTS;
-    const float csx = p.cs[0], csy = p.cs[1], csz = p.cs[2];
-    const float rc2 = p.rc2f, x0 = p.x0, invd = p.inv_delta;
+    const float4* __restrict__ row = d.nbcoef + (size_t)__float_as_int(pk.w) * d.T * MT_NPTS;
     const float qk = COUL ? d.qf[k] : 0.0f;
     double Fx = 0.0, Fy = 0.0, Fz = 0.0;
     float U = 0.0f;
-    const int cnt = min(d.ncount[k], d.ncap);
-    const uint32_t* __restrict__ nl = d.nlist + k;
-    const size_t stride = (size_t)d.n;
-    for (int j = 0; j < cnt; j++) {
-        const uint32_t w = nl[(size_t)j * stride];
-        const int a = (int)(w & MT_NW_IDX_MASK);
-        const int code = (int)(w >> MT_NW_IDX_BITS);
-        const int ox = code % 3 - 1, oy = (code / 3) % 3 - 1, oz = code / 9 - 1;
-        const float4 pa = d.posf[a];
-        const float dx = pk.x - pa.x - (float)ox * csx;
-        const float dy = pk.y - pa.y - (float)oy * csy;
-        const float dz = pk.z - pa.z - (float)oz * csz;
-        const float r2 = dx * dx + dy * dy + dz * dz;
-        if (r2 < rc2 && r2 >= 1.0e-6f) {
-            const float r = __fsqrt_rn(r2);
-            const float rinv = 1.0f / r;
-            float fr = 0.0f;
-            const int idx = (int)((r - x0) * invd);
-            if (idx >= 0 && idx < MT_NPTS - 1) {
-                const int ta = __float_as_int(pa.w);
-                const float4 c = coef[(size_t)ta * MT_NPTS + idx];
-                fr = (c.x * r + c.y) * rinv;
-                if (WANT_E) U += 0.5f * (c.z * r + c.w);
-            }
-            if (COUL) {
-                const float K = MT_COULOMB_K * qk * d.qf[a];
-                fr += K * rinv * rinv * rinv;
-                if (WANT_E) U += 0.5f * K * rinv;
-            }
-            Fx += (double)(dx * fr);
-            Fy += (double)(dy * fr);
-            Fz += (double)(dz * fr);
-        }
-    }
+    mt_mixed_atom<WANT_E, COUL>(d, p, k, pk, sh, row, qk, Fx, Fy, Fz, U);
     if (!d.mobile[k]) { Fx = Fy = Fz = 0.0; U = 0.0f; }
     d.frc[k] = make_double4(Fx, Fy, Fz, 0.0);
     if (WANT_E) d.epot[k] = U;
 }
 
 // MIXED with the force-only segment coefficients (a_f, b_f) of every type pair
-// staged in LDS once per workgroup; workgroups stride over the atoms. Used
-// when T^2 * 1000 * 8 bytes fits the LDS budget and energies are not needed.
+// staged in LDS once per workgroup; each workgroup owns one contiguous range
+// of atoms (ranges XCD-contiguous). Used when T^2 * 1000 * 8 bytes fits the
+// LDS budget and energies are not needed.
 template <bool COUL>
 __global__ void __launch_bounds__(512) k_fast_force_mixed_lds(MtDev d, MtParams p)
 {
     extern __shared__ float2 stab[];
+    __shared__ float4 sh[27];
     const int T = d.T;
     const int ntab = T * T * MT_NPTS;
     for (int i = threadIdx.x; i < ntab; i += blockDim.x) stab[i] = d.nbcoef2[i];
+    mt_fill_shifts(sh, p);
     __syncthreads();
-    const float csx = p.cs[0], csy = p.cs[1], csz = p.cs[2];
-    const float rc2 = p.rc2f, x0 = p.x0, invd = p.inv_delta;
-    const size_t stride = (size_t)d.n;
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < d.n; k += gridDim.x * blockDim.x) {
+    const int lw = mt_xcd_block(blockIdx.x, gridDim.x);
+    const int chunk = ((d.n + gridDim.x - 1) / gridDim.x + blockDim.x - 1) / blockDim.x * blockDim.x;
+    const int kb = lw * chunk, ke = min(d.n, kb + chunk);
+    for (int k = kb + threadIdx.x; k < ke; k += blockDim.x) {
         const float4 pk = d.posf[k];
         const float2* __restrict__ row = stab + __float_as_int(pk.w) * T * MT_NPTS;
         const float qk = COUL ? d.qf[k] : 0.0f;
         double Fx = 0.0, Fy = 0.0, Fz = 0.0;
-        const int cnt = min(d.ncount[k], d.ncap);
-        const uint32_t* __restrict__ nl = d.nlist + k;
-        for (int j = 0; j < cnt; j++) {
-            const uint32_t w = nl[(size_t)j * stride];
-            const int a = (int)(w & MT_NW_IDX_MASK);
-            const int code = (int)(w >> MT_NW_IDX_BITS);
-            const int ox = code % 3 - 1, oy = (code / 3) % 3 - 1, oz = code / 9 - 1;
-            const float4 pa = d.posf[a];
-            const float dx = pk.x - pa.x - (float)ox * csx;
-            const float dy = pk.y - pa.y - (float)oy * csy;
-            const float dz = pk.z - pa.z - (float)oz * csz;
-            const float r2 = dx * dx + dy * dy + dz * dz;
-            if (r2 < rc2 && r2 >= 1.0e-6f) {
-                const float r = __fsqrt_rn(r2);
-                const float rinv = 1.0f / r;
-                float fr = 0.0f;
-                const int idx = (int)((r - x0) * invd);
-                if (idx >= 0 && idx < MT_NPTS - 1) {
-                    const float2 c = row[__float_as_int(pa.w) * MT_NPTS + idx];
-                    fr = (c.x * r + c.y) * rinv;
-                }
-                if (COUL) fr += MT_COULOMB_K * qk * d.qf[a] * rinv * rinv * rinv;
-                Fx += (double)(dx * fr);
-                Fy += (double)(dy * fr);
-                Fz += (double)(dz * fr);
-            }
-        }
+        float U = 0.0f;
+        mt_mixed_atom<false, COUL>(d, p, k, pk, sh, row, qk, Fx, Fy, Fz, U);
         if (!d.mobile[k]) { Fx = Fy = Fz = 0.0; }
         d.frc[k] = make_double4(Fx, Fy, Fz, 0.0);
     }
@@ -438,7 +591,7 @@ __global__ void __launch_bounds__(512) k_fast_force_mixed_lds(MtDev d, MtParams 
 template <bool WANT_E, bool COUL>
 __global__ void __launch_bounds__(256) k_fast_force_fp64(MtDev d, MtParams p)
 {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    int k = mt_xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (k >= d.n) return;
     const double4 rk4 = d.pos[k];
     const mt_d3 rk = {rk4.x, rk4.y, rk4.z};
@@ -448,7 +601,7 @@ __global__ void __launch_bounds__(256) k_fast_force_fp64(MtDev d, MtParams p)
     float U = 0.0f;
     const int cnt = min(d.ncount[k], d.ncap);
     for (int j = 0; j < cnt; j++) {
-        const int a = (int)(d.nlist[(size_t)j * d.n + k] & MT_NW_IDX_MASK);
+        const int a = (int)(mt_fast_word(d.nlist, d.n, k, j) & MT_NW_IDX_MASK);
         const double4 ra4 = d.pos[a];
         const mt_d3 ra = {ra4.x, ra4.y, ra4.z};
         if (!(mt_dist2_pbc(rk, ra, p.wnl) < (double)p.rc2f)) continue;
@@ -679,7 +832,8 @@ void mt_fast_forces(const MtDev& d, const MtParams& p, int mixed, int want_e, in
         if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
         const int bs = 512;
         const int per_cu = lds_tab > 32768 ? 2 : 4;
-        const int g = std::max(1, std::min(nblk(d.n, bs), ncu * per_cu));
+        // a multiple of 8 workgroups so every XCD gets the same share
+        const int g = std::max(8, std::min(nblk(d.n, bs), ncu * per_cu) / 8 * 8);
         if (coul) hipLaunchKernelGGL((k_fast_force_mixed_lds<true>), dim3(g), dim3(bs), lds_tab, s, d, p);
         else hipLaunchKernelGGL((k_fast_force_mixed_lds<false>), dim3(g), dim3(bs), lds_tab, s, d, p);
     } else if (mixed) {
@@ -763,6 +917,63 @@ __global__ void __launch_bounds__(TPB) k_fast_kick_drift(MtDev d, MtParams p, in
 void mt_fast_kick_drift(const MtDev& d, const MtParams& p, int scale_slot, int mixed, hipStream_t s)
 {
     hipLaunchKernelGGL(k_fast_kick_drift, dim3(nblk(d.n, TPB)), dim3(TPB), 0, s, d, p, scale_slot, mixed);
+}
+
+// ---------------------------------------------------------------------------
+// U_cut = sum over the pairs inside rc of U(rc) of the pair's force fields,
+// scaled like the pair. The reference's pair potentials are truncated without
+// a shift, so U jumps by U(rc) whenever a pair crosses the cutoff while the
+// force does not: Etot - U_cut has no such jumps and is the energy velocity
+// Verlet conserves for these forces, so its drift measures the integration
+// (time step, precision) rather than the truncation. Every engine; the EXACT
+// lists hold plain atom indices at rc, the FP64/MIXED lists coded words at
+// rc + skin plus the special pairs.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(TPB) k_ucut(MtDev d, MtParams p, int exact)
+{
+    __shared__ double sh[TPB];
+    double u = 0.0;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < d.n; k += gridDim.x * blockDim.x) {
+        if (!d.mobile[k]) continue;
+        const double4 rk4 = d.pos[k];
+        const mt_d3 rk = {rk4.x, rk4.y, rk4.z};
+        const int T = d.T, row = d.type[k], molk = d.mol[k];
+        const int cnt = min(d.ncount[k], d.ncap);
+        for (int j = 0; j < cnt; j++) {
+            const int a = exact ? (int)d.nlist[(size_t)j * d.n + k]
+                                : (int)(mt_fast_word(d.nlist, d.n, k, j) & MT_NW_IDX_MASK);
+            const double4 ra4 = d.pos[a];
+            float e = d.ecut[row * T + d.type[a]];
+            if (exact) {
+                mt_d3 f = {0.0, 0.0, 0.0};
+                mt_scale_pair(d, p, f, e, a, molk, k);
+            } else if (!(mt_dist2_pbc(rk, {ra4.x, ra4.y, ra4.z}, p.wnl) < (double)p.rc2f)) {
+                continue;
+            }
+            u += 0.5 * (double)e;
+        }
+        if (!exact && d.sp.n_entries > 0) {
+            const int o = d.perm[k];
+            for (int j = 0; j < d.sp.num[o]; j++) {
+                const int a = d.inv[d.sp.partner[d.sp.first[o] + j]];
+                const double4 ra4 = d.pos[a];
+                if (mt_dist2_pbc(rk, {ra4.x, ra4.y, ra4.z}, p.wnl) < (double)p.rc2f)
+                    u += 0.5 * (double)(d.ecut[row * T + d.type[a]] * d.sp.scale[d.sp.first[o] + j]);
+            }
+        }
+    }
+    sh[threadIdx.x] = u;
+    __syncthreads();
+    for (int off = TPB / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) d.red[blockIdx.x] = sh[0];
+}
+
+void mt_ucut(const MtDev& d, const MtParams& p, int exact, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_ucut, dim3(mt_reduce_blocks(d.n)), dim3(TPB), 0, s, d, p, exact);
 }
 
 __global__ void k_fast_clear(unsigned int* flags) { flags[0] = 0u; }

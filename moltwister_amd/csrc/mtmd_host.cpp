@@ -574,6 +574,14 @@ static int setup(mtmd_handle* h, const mtmd_system* s)
     int rc2;
 #define UP(dst, src) if ((rc2 = upload(h, dst, src))) return rc2
     UP(d.pos, pos); UP(d.mom, mom); UP(d.type, type); UP(d.mol, mol); UP(d.mobile, mobile); UP(d.xmol, xmol);
+    {
+        // U at the last table point (x = rc) summed over each pair's force fields
+        std::vector<float> ecut((size_t)T * T, 0.0f);
+        for (int pr = 0; pr < T * T; pr++)
+            for (int ff = 0; ff < nbcnt[pr]; ff++)
+                ecut[pr] += nbtab[((size_t)pr * MT_MAXFF + ff) * MT_NPTS + MT_NPTS - 1].z;
+        UP(d.ecut, ecut);
+    }
     UP(d.nbtab, nbtab); UP(d.nbcnt, nbcnt); UP(d.btab, btab); UP(d.atab, atab); UP(d.dtab, dtab);
     UP(d.bl.b_first, b_first); UP(d.bl.b_num, b_num); UP(d.bl.b_j, b_j); UP(d.bl.b_type, b_type);
     UP(d.bl.a_first, a_first); UP(d.bl.a_num, a_num); UP(d.bl.a_i, a_i); UP(d.bl.a_j, a_j);
@@ -656,6 +664,7 @@ static int setup(mtmd_handle* h, const mtmd_system* s)
         d.ncap = std::max(16, (int)std::ceil(nexp * 1.35) + 16);
         if (!h->fast && (d.nc[0] < 3 || d.nc[1] < 3 || d.nc[2] < 3)) d.ncap *= 2;
         d.ncap = std::min(d.ncap, n * 27 + 16);
+        d.ncap = (d.ncap + 3) & ~3;          // whole 16-byte chunks (FP64/MIXED list layout)
         d.nlist = h->mem.alloc<uint32_t>((size_t)d.ncap * n);
         if (!d.nlist) return h->fail(MTMD_ENOMEM, "neighbour list allocation failed");
     }
@@ -827,7 +836,7 @@ extern "C" int mtmd_get_thermostat(mtmd_handle* h, double* eta, double* p_eta)
 static int grow_neighbors(mtmd_handle* h, int need)
 {
     MtDev& d = h->d;
-    int cap = need + need / 8 + 16;
+    const int cap = (need + need / 8 + 16 + 3) & ~3;
     void* old = d.nlist;
     uint32_t* nl = nullptr;
     HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -906,6 +915,8 @@ static int fast_check_capacity(mtmd_handle* h, bool* overflowed)
     HIPCHK(h, hipStreamSynchronize(h->stream));
     *overflowed = h->h_status[2] != 0;
     h->rebuilds = (int64_t)(unsigned)h->h_status[4];
+    if (h->h_status[5])
+        return h->fail(MTMD_EBOX, "a cell holds more atoms than the neighbour-list staging buffer (2560)");
     if (h->h_status[3]) {
         HIPCHK(h, hipMemsetAsync(d.flags + 3, 0, sizeof(unsigned int), h->stream));
         h->have_forces = false;
@@ -1131,6 +1142,14 @@ extern "C" int mtmd_get_thermo(mtmd_handle* h, mtmd_thermo* out)
     out->U = su;
     out->K = 0.5 * sp;
     out->Etot = su + 0.5 * sp;
+    // U_cut over the lists of the last force evaluation (EXACT: rebuilt at rc
+    // for those positions; FP64/MIXED: the skin list, valid until the next step)
+    mt_ucut(d, h->p, h->fast ? 0 : 1, h->stream);
+    HIPCHK(h, hipMemcpyAsync(part.data(), d.red, nb * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    double uc = 0;
+    for (int b = 0; b < nb; b++) uc += part[b];
+    out->U_cut = uc;
     return MTMD_OK;
 }
 

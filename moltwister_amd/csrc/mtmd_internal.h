@@ -57,6 +57,7 @@ struct MtDev {
     float4* nbtab; int* nbcnt; float4* btab; float4* atab; float4* dtab;
     float4* nbcoef;
     float2* nbcoef2;       // force-only a_f, b_f per segment (MIXED, staged in LDS)
+    float* ecut;           // [row*T + col] pair energy at the cutoff, summed over force fields
     MtBonded bl;
     MtSpecial sp;
     // cells
@@ -124,5 +125,7 @@ void mt_fast_forces(const MtDev& d, const MtParams& p, int mixed, int want_energ
 void mt_fast_kick_drift(const MtDev& d, const MtParams& p, int scale_slot, int mixed, hipStream_t s);
 void mt_fast_clear_flags(const MtDev& d, hipStream_t s);
 void mt_copy_d4(const double4* src, double4* dst, int n, hipStream_t s);
+// per-block partial sums of U_cut into d.red[0 .. mt_reduce_blocks(n))
+void mt_ucut(const MtDev& d, const MtParams& p, int exact, hipStream_t s);
 
 int mt_reduce_blocks(int n);

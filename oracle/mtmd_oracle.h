@@ -13,8 +13,8 @@
  *
  * Pinning (see tests/test_oracle_ref.py, tests/test_oracle_math.py):
  *  - profiles, cells, neighbour lists, per-atom forces, NH chain propagator:
- *    bit-compared against the reference's own sources compiled from
- *    /root/reference (oracle/_ref) and against committed fixtures
+ *    bit-compared against the reference's own sources compiled by
+ *    oracle/Makefile (oracle/_ref) and against committed fixtures
  *    (tests/golden) written from that build;
  *  - velocity-Verlet step, PBC wrap, thermo reductions, gradient-descent step:
  *    pinned by closed forms and invariants (time reversibility, exact

@@ -2,7 +2,7 @@
 
   Oracle    - oracle/liboracle.so, the C restatement (mtmd_oracle.c)
   Reference - oracle/_ref/libmtref.so, MolTwister's own CPU sources driven by
-              ref_driver.cpp (present only where /root/reference was built)
+              ref_driver.cpp (present only where the MolTwister source tree was built)
 
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use this
 module; the product never imports it.

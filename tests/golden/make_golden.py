@@ -1,5 +1,5 @@
 """Write the golden fixtures in this directory from MolTwister's own CPU
-sources (oracle/_ref/libmtref.so, built from /root/reference by
+sources (oracle/_ref/libmtref.so, built from the MolTwister sources by
 oracle/Makefile). Inputs and outputs are stored together, so the fixtures do
 not depend on the generators that made the inputs.
 

@@ -175,3 +175,34 @@ def test_water_nve_energy_allpairs(engine):
     th = e.thermo()
     assert np.isfinite(th["Etot"])
     assert abs(th["Etot"] - E0) < 2e-3 * abs(E0), (th["Etot"], E0)
+
+
+def test_ucut_brute_force_and_engines_agree():
+    """U_cut (pairs inside rc times the pair energy at rc) from all three
+    engines against an O(N^2) numpy count with the same float-box predicate."""
+    from moltwister_amd.engine import profile
+    print("case ucut binary_lj", flush=True)
+    s, c = make("binary_lj", engine=abi.ENGINE_EXACT)
+    pos = s.pos + np.random.default_rng(8).normal(0, 0.02, s.pos.shape)
+    L = np.array(s.box)
+    d = pos[:, None, :] - pos[None, :, :]
+    d -= L * np.round(d / L)
+    r2 = np.sum(d * d, axis=2)
+    inside = (r2 < float(np.float32(c.cutoff_radius)) ** 2) & ~np.eye(s.n_atoms, dtype=bool)
+    ti = s.types
+    T = max(s.n_types, max(max(a, b) + 1 for a, b, _, _ in s.nonbonded))
+    ecut = np.zeros((T, T))
+    for a, b, kind, prm in s.nonbonded:
+        _, _, e = profile(0, kind, prm, 0.01, float(np.float32(c.cutoff_radius)))
+        ecut[a, b] += float(e[-1])
+        if a != b:
+            ecut[b, a] += float(e[-1])
+    want = 0.5 * float(np.sum(ecut[ti[:, None], ti[None, :]] * inside))
+    got = {}
+    for eng in (abi.ENGINE_EXACT, abi.ENGINE_FP64, abi.ENGINE_MIXED):
+        e = Engine(s, make("binary_lj", engine=eng)[1])
+        e.set_state(pos=pos)
+        e.compute_forces()
+        got[eng] = e.thermo()["U_cut"]
+    for eng, v in got.items():
+        assert abs(v - want) <= 1e-9 * abs(want), (eng, v, want)

@@ -1,6 +1,6 @@
 """The oracle against MolTwister's own CPU sources (oracle/_ref/libmtref.so).
 
-The reference build exists only where /root/reference is mounted (it is built
+The reference build exists only where the MolTwister tree is mounted (it is built
 from those sources by oracle/Makefile); elsewhere these tests skip and the
 committed fixtures (test_oracle_golden.py) carry the same pins.
 Bar: bit-identical (np.array_equal) profiles, cells, neighbour lists,
@@ -15,7 +15,7 @@ from systems import SYSTEMS, make
 
 pytestmark = pytest.mark.skipif(
     not __import__("os").path.exists(__import__("oracle.pyoracle", fromlist=["x"]).REF_LIB),
-    reason="reference build oracle/_ref/libmtref.so not present (needs /root/reference)")
+    reason="reference build oracle/_ref/libmtref.so not present (needs the MolTwister source tree)")
 
 PROFILES = [
     (0, abi.NB_LJ, (1.0, 1.0, 1.0), 0.01, 2.5),

@@ -181,7 +181,8 @@ def main() -> int:
     b_step = 184 + 4 * n_full
     b_rb = 32 + 4 * n_full
     step_bytes = b_step + b_rb / rebuild_every
-    copy_bw = n * 64 / t_copy
+    cap = em.stats()["capacity"]
+    copy_bw = 2 * n * 4 * cap / t_copy                  # in-place pass over the list buffer
 
     # FP64 engine on the same state and the force error of MIXED against it
     pos_m, mom_m = em.get_state()
@@ -203,7 +204,7 @@ def main() -> int:
     e64.close()
     log(f"bench: fp64 {dt_64 / K64 * 1e3:.3f} ms/step, {value64:.4g} atom-steps/s; force rms rel {force_rms:.2e}")
 
-    water = {}
+    water, water_F = {}, {}
     if not args.skip_water:
         from moltwister_amd.engine import MtmdError
         sysW = W.water_box(10000, seed=2)
@@ -223,6 +224,8 @@ def main() -> int:
             cwt.block_size = args.block
             try:
                 ew = Engine(sysW, cwt)
+                if coul == abi.COUL_ALLPAIRS:
+                    water_F[eng_id] = ew.compute_forces()[0]
                 ew.step(20)
                 kw = 200 if eng_id == abi.ENGINE_MIXED else 100
                 dtw = timed_steps(ew, kw)
@@ -232,6 +235,9 @@ def main() -> int:
             except MtmdError as ex:
                 water[name] = None
                 water[name + "_error"] = str(ex)
+        if len(water_F) == 2:
+            water["force_rms_rel_mixed_vs_fp64_allpairs"] = rms_rel(water_F[abi.ENGINE_MIXED],
+                                                                    water_F[abi.ENGINE_FP64])
         log(f"bench: water {water}")
 
     cpu = None

@@ -268,8 +268,9 @@ int mtmd_synchronize(mtmd_handle* h);
  * the per-step wall clock of MDLoop.cpp:434-456): average duration in ms of
  * one launch of a kernel group, timed with HIP events on the handle's stream
  * over `reps` back-to-back launches. which: 0 pair-force pass, 1 full
- * neighbour-list build, 2 one MD step, 3 streaming copy of 32 bytes per atom
- * (FP64/MIXED engines; calibrates the HBM byte counters). */
+ * neighbour-list build, 2 one MD step, 3 in-place read + write of the whole
+ * neighbour-list buffer, 4 * capacity bytes per atom each way (FP64/MIXED
+ * engines; calibrates the HBM byte counters). */
 int mtmd_bench_kernel(mtmd_handle* h, int which, int reps, double* avg_ms);
 
 /* CMDSimulator::run (Simulator/MDSimulator.cpp:35-47): creates the engine,

@@ -16,6 +16,7 @@
 #include "mtmd_internal.h"
 #include "mtmd_device.h"
 #include <algorithm>
+#include <climits>
 
 #define TPB 256
 
@@ -52,93 +53,166 @@ __global__ void k_fast_fill(int* p, int v, int n, int cond, const unsigned int* 
     if (i < n) p[i] = v;
 }
 
-__global__ void k_fast_cell_keys(const double4* __restrict__ pos, int n, int Nx, int Ny, int Nz, double LX,
-                                 double LY, double LZ, int* __restrict__ key, int* __restrict__ count, int cond,
-                                 unsigned int* flags, float thr)
+// Lanes of a wavefront that hold equal keys in a row form a run; atoms are
+// spatially sorted, so runs are long and one atomic per run replaces one per
+// atom. Gives the run's head lane and length.
+__device__ __forceinline__ void mt_key_runs(int key, int& head_lane, int& len)
 {
-    if (mt_skip_rebuild(cond, flags, thr)) return;
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    double4 r = pos[k];
-    if (!(isfinite(r.x) && isfinite(r.y) && isfinite(r.z))) atomicOr(&flags[3], 1u);
-    int ix = mt_d2i(floor((r.x + LX / 2.0) * (double)Nx / LX));
-    int iy = mt_d2i(floor((r.y + LY / 2.0) * (double)Ny / LY));
-    int iz = mt_d2i(floor((r.z + LZ / 2.0) * (double)Nz / LZ));
-    ix = ix < 0 ? 0 : (ix >= Nx ? Nx - 1 : ix);
-    iy = iy < 0 ? 0 : (iy >= Ny ? Ny - 1 : iy);
-    iz = iz < 0 ? 0 : (iz >= Nz ? Nz - 1 : iz);
-    int c = ix + Nx * (iy + Ny * iz);
-    key[k] = c;
-    atomicAdd(&count[c], 1);
+    const int lane = threadIdx.x & 63;
+    const int prev = __shfl_up(key, 1);
+    const unsigned long long heads = __ballot(lane == 0 || prev != key);
+    head_lane = 63 - __clzll(heads & ((2ull << lane) - 1ull));
+    const unsigned long long above = heads & ~((2ull << head_lane) - 1ull);
+    len = (above ? __ffsll(above) - 1 : 64) - head_lane;
 }
 
-__global__ void __launch_bounds__(1024) k_fast_scan(const int* __restrict__ count, int* __restrict__ first,
-                                                    int* __restrict__ fill, int ncell, int cond,
-                                                    const unsigned int* flags, float thr)
+__global__ void __launch_bounds__(TPB) k_fast_cell_keys(const double4* __restrict__ pos, int n, int Nx, int Ny,
+                                                        int Nz, double LX, double LY, double LZ,
+                                                        int* __restrict__ key, int* __restrict__ count, int cond,
+                                                        unsigned int* flags, float thr)
 {
     if (mt_skip_rebuild(cond, flags, thr)) return;
-    __shared__ int sums[1024];
-    int t = threadIdx.x;
-    int chunk = (ncell + 1023) / 1024;
-    int b = t * chunk, e = min(b + chunk, ncell);
-    int s = 0;
-    for (int c = b; c < e; c++) s += count[c];
-    sums[t] = s;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    int c = -1;
+    if (k < n) {
+        const double4 r = pos[k];
+        if (!(isfinite(r.x) && isfinite(r.y) && isfinite(r.z))) atomicOr(&flags[3], 1u);
+        int ix = mt_d2i(floor((r.x + LX / 2.0) * (double)Nx / LX));
+        int iy = mt_d2i(floor((r.y + LY / 2.0) * (double)Ny / LY));
+        int iz = mt_d2i(floor((r.z + LZ / 2.0) * (double)Nz / LZ));
+        ix = ix < 0 ? 0 : (ix >= Nx ? Nx - 1 : ix);
+        iy = iy < 0 ? 0 : (iy >= Ny ? Ny - 1 : iy);
+        iz = iz < 0 ? 0 : (iz >= Nz ? Nz - 1 : iz);
+        c = ix + Nx * (iy + Ny * iz);
+        key[k] = c;
+    }
+    int hl, len;
+    mt_key_runs(c, hl, len);
+    if (c >= 0 && (int)(threadIdx.x & 63) == hl) atomicAdd(&count[c], len);
+}
+
+// exclusive scan of the cell counts in blocks of SCAN_B cells: block sums,
+// then each block adds the sums of the blocks before it to its own scan
+#define SCAN_B 1024
+__global__ void __launch_bounds__(SCAN_B) k_fast_scan_sums(const int* __restrict__ count, int* __restrict__ bsum,
+                                                           int ncell, int cond, const unsigned int* flags, float thr)
+{
+    if (mt_skip_rebuild(cond, flags, thr)) return;
+    __shared__ int ws[SCAN_B / 64];
+    const int c = blockIdx.x * SCAN_B + threadIdx.x;
+    int v = c < ncell ? count[c] : 0;
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = v;
     __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        int v = (t >= off) ? sums[t - off] : 0;
-        __syncthreads();
-        sums[t] += v;
-        __syncthreads();
-    }
-    int run = sums[t] - s;
-    for (int c = b; c < e; c++) {
-        first[c] = run;
-        fill[c] = 0;
-        run += count[c];
+    if (threadIdx.x == 0) {
+        int sum = 0;
+        for (int w = 0; w < SCAN_B / 64; w++) sum += ws[w];
+        bsum[blockIdx.x] = sum;
     }
 }
 
-__global__ void k_fast_scatter(const int* __restrict__ key, int n, const int* __restrict__ first,
-                               int* __restrict__ fill, int* __restrict__ list, int cond, const unsigned int* flags,
-                               float thr)
+__global__ void __launch_bounds__(SCAN_B) k_fast_scan_apply(const int* __restrict__ count,
+                                                            const int* __restrict__ bsum, int* __restrict__ first,
+                                                            int* __restrict__ fill, int ncell, int cond,
+                                                            const unsigned int* flags, float thr)
 {
     if (mt_skip_rebuild(cond, flags, thr)) return;
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    int c = key[k];
-    list[first[c] + atomicAdd(&fill[c], 1)] = k;
+    __shared__ int ws[SCAN_B / 64];
+    __shared__ int s_base;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    // sum of the block sums before this block
+    int pre = 0;
+    for (int j = t; j < (int)blockIdx.x; j += SCAN_B) pre += bsum[j];
+    for (int o = 32; o > 0; o >>= 1) pre += __shfl_xor(pre, o);
+    if (lane == 0) ws[w] = pre;
+    __syncthreads();
+    if (t == 0) {
+        int sum = 0;
+        for (int j = 0; j < SCAN_B / 64; j++) sum += ws[j];
+        s_base = sum;
+    }
+    __syncthreads();
+    // inclusive scan inside the wavefront, then across the block's wavefronts
+    const int c = blockIdx.x * SCAN_B + t;
+    const int v = c < ncell ? count[c] : 0;
+    int incl = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int u = __shfl_up(incl, o);
+        if (lane >= o) incl += u;
+    }
+    __syncthreads();
+    if (lane == 63) ws[w] = incl;
+    __syncthreads();
+    int wpre = 0;
+    for (int j = 0; j < w; j++) wpre += ws[j];
+    if (c < ncell) {
+        first[c] = s_base + wpre + incl - v;
+        fill[c] = 0;
+    }
 }
 
-// ascending slot order inside each cell, so lists are reproducible run to run
-__global__ void k_fast_cell_sort(const int* __restrict__ first, const int* __restrict__ count,
-                                 int* __restrict__ list, int ncell, int cond, const unsigned int* flags, float thr)
+__global__ void __launch_bounds__(TPB) k_fast_scatter(const int* __restrict__ key, int n,
+                                                      const int* __restrict__ first, int* __restrict__ fill,
+                                                      int* __restrict__ list, int cond, const unsigned int* flags,
+                                                      float thr)
+{
+    if (mt_skip_rebuild(cond, flags, thr)) return;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int c = k < n ? key[k] : -1;
+    int hl, len;
+    mt_key_runs(c, hl, len);
+    int base = 0;
+    if (c >= 0 && lane == hl) base = atomicAdd(&fill[c], len);
+    base = __shfl(base, hl);
+    if (c >= 0) list[first[c] + base + (lane - hl)] = k;
+}
+
+// ascending slot order inside each cell, so lists are reproducible run to run:
+// one wavefront per cell ranks its (distinct) entries with shuffles; cells of
+// more than 64 atoms fall back to an insertion sort by one lane
+__global__ void __launch_bounds__(TPB) k_fast_cell_sort(const int* __restrict__ first,
+                                                        const int* __restrict__ count, int* __restrict__ list,
+                                                        int ncell, int cond, const unsigned int* flags, float thr)
 {
     if (mt_skip_rebuild(cond, flags, thr) || flags[3]) return;
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
     if (c >= ncell) return;
-    int cnt = count[c];
+    const int cnt = count[c];
+    if (cnt <= 1) return;
     int* a = list + first[c];
-    for (int i = 1; i < cnt; i++) {
-        int v = a[i], j = i - 1;
-        while (j >= 0 && a[j] > v) { a[j + 1] = a[j]; j--; }
-        a[j + 1] = v;
+    if (cnt <= 64) {
+        const int v = lane < cnt ? a[lane] : INT_MAX;
+        int rank = 0;
+        for (int j = 0; j < cnt; j++) rank += __shfl(v, j) < v ? 1 : 0;
+        if (lane < cnt) a[rank] = v;
+    } else if (lane == 0) {
+        for (int i = 1; i < cnt; i++) {
+            const int v = a[i];
+            int j = i - 1;
+            while (j >= 0 && a[j] > v) { a[j + 1] = a[j]; j--; }
+            a[j + 1] = v;
+        }
     }
 }
 
 void mt_fast_cells(const MtDev& d, const MtParams& p, int cond, hipStream_t s)
 {
     const float thr = p.half_skin2;
+    const int nsb = nblk(d.ncells, SCAN_B);
     hipLaunchKernelGGL(k_fast_fill, dim3(nblk(d.ncells, TPB)), dim3(TPB), 0, s, d.cell_count, 0, d.ncells, cond,
                        d.flags, thr);
     hipLaunchKernelGGL(k_fast_cell_keys, dim3(nblk(d.n, TPB)), dim3(TPB), 0, s, d.pos, d.n, d.nc[0], d.nc[1],
                        d.nc[2], p.L[0], p.L[1], p.L[2], d.cell_key, d.cell_count, cond, d.flags, thr);
-    hipLaunchKernelGGL(k_fast_scan, dim3(1), dim3(1024), 0, s, d.cell_count, d.cell_first, d.cell_fill, d.ncells,
-                       cond, d.flags, thr);
+    hipLaunchKernelGGL(k_fast_scan_sums, dim3(nsb), dim3(SCAN_B), 0, s, d.cell_count, d.cell_bsum, d.ncells, cond,
+                       d.flags, thr);
+    hipLaunchKernelGGL(k_fast_scan_apply, dim3(nsb), dim3(SCAN_B), 0, s, d.cell_count, d.cell_bsum, d.cell_first,
+                       d.cell_fill, d.ncells, cond, d.flags, thr);
     hipLaunchKernelGGL(k_fast_scatter, dim3(nblk(d.n, TPB)), dim3(TPB), 0, s, d.cell_key, d.n, d.cell_first,
                        d.cell_fill, d.cell_list, cond, d.flags, thr);
-    hipLaunchKernelGGL(k_fast_cell_sort, dim3(nblk(d.ncells, TPB)), dim3(TPB), 0, s, d.cell_first, d.cell_count,
-                       d.cell_list, d.ncells, cond, d.flags, thr);
+    hipLaunchKernelGGL(k_fast_cell_sort, dim3(nblk(d.ncells, TPB / 64)), dim3(TPB), 0, s, d.cell_first,
+                       d.cell_count, d.cell_list, d.ncells, cond, d.flags, thr);
 }
 
 // gather every per-atom array into cell order: new slot s <- old slot list[s]
@@ -632,9 +706,72 @@ __global__ void __launch_bounds__(256) k_fast_force_fp64(MtDev d, MtParams p)
     if (WANT_E) d.epot[k] = U;
 }
 
+// geometry of the MIXED all-pairs term: box widths as the reference's
+// minimum image sees them (float L), their halves, and the band around the
+// image boundary where the image is decided in FP64 instead
+struct MtApGeom {
+    float w[3], iw[3], hw[3];
+    float edge;
+};
+
+__device__ __forceinline__ MtApGeom mt_ap_geom(const MtParams& p)
+{
+    MtApGeom g;
+    float wmax = 0.f;
+    for (int a = 0; a < 3; a++) {
+        g.w[a] = p.Lf[a];
+        g.iw[a] = 1.0f / p.Lf[a];
+        g.hw[a] = 0.5f * p.Lf[a];
+        wmax = fmaxf(wmax, p.Lf[a]);
+    }
+    // far above the FP32 rounding of a coordinate difference (~1e-7 * w)
+    g.edge = 1.0e-5f * wmax;
+    return g;
+}
+
+// One pair term of the MIXED all-pairs Coulomb sum, added to (fx, fy, fz, u):
+// F_k += K q_k q_i d / |d|^3, u += K q_k q_i / |d|, d = r_k - image(r_i), in
+// FP32 from FP32 positions, the image taken by rounding d / w. Within `edge` of
+// the image boundary (|d| = w/2, or 3w/2 for positions far outside the box)
+// the image is decided in FP64 from the double positions with the reference's
+// rule (C3DVector::moveToSameSideOfPBCAsThis, 3DVector.cpp:133-165), so FP32
+// rounding never picks the other image of a pair than the FP64 engine does.
+// Contraction is off and the FMAs are explicit: the special-pair pass
+// recomputes this term for subtraction and must get the same bits.
+__device__ __forceinline__ void mt_ap_term(float xk, float yk, float zk, const mt_d3& rk, float qks, float4 q, int i,
+                                           const double4* pos, const MtParams& p, const MtApGeom& g, float& fx,
+                                           float& fy, float& fz, float& u)
+{
+#pragma clang fp contract(off)
+    float dx = xk - q.x, dy = yk - q.y, dz = zk - q.z;
+    const float ax = fabsf(dx), ay = fabsf(dy), az = fabsf(dz);
+    const bool nearb = fabsf(ax - g.hw[0]) < g.edge || fabsf(ay - g.hw[1]) < g.edge || fabsf(az - g.hw[2]) < g.edge ||
+                       ax > 3.0f * g.hw[0] - g.edge || ay > 3.0f * g.hw[1] - g.edge || az > 3.0f * g.hw[2] - g.edge;
+    if (nearb) {
+        const double4 r4 = pos[i];
+        const mt_d3 ri = mt_move_same_side(rk, {r4.x, r4.y, r4.z}, p.wbox);
+        dx = (float)(rk.x - ri.x);
+        dy = (float)(rk.y - ri.y);
+        dz = (float)(rk.z - ri.z);
+    } else {
+        dx = fmaf(-g.w[0], rintf(dx * g.iw[0]), dx);
+        dy = fmaf(-g.w[1], rintf(dy * g.iw[1]), dy);
+        dz = fmaf(-g.w[2], rintf(dz * g.iw[2]), dz);
+    }
+    const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+    const float rinv = r2 > 0.0f ? __builtin_amdgcn_rsqf(r2) : 0.0f;   // the atom itself: 0
+    const float kr = (qks * q.w) * rinv;
+    const float sc = (kr * rinv) * rinv;
+    fx = fmaf(dx, sc, fx);
+    fy = fmaf(dy, sc, fy);
+    fz = fmaf(dz, sc, fz);
+    u += kr;
+}
+
 // bonded terms and special (same-molecule / scaled) pairs, added on top of the
 // pair forces with the exact FP64 helpers; one thread per atom with work
-__global__ void __launch_bounds__(TPB) k_fast_bonded_special(MtDev d, MtParams p, int want_e, int coul)
+__global__ void __launch_bounds__(TPB) k_fast_bonded_special(MtDev d, MtParams p, int want_e, int coul,
+                                                             int mixed)
 {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= d.n) return;
@@ -665,20 +802,35 @@ __global__ void __launch_bounds__(TPB) k_fast_bonded_special(MtDev d, MtParams p
             float u = 0.0f;
             mt_d3 f = {0.0, 0.0, 0.0};
             const float r = (float)mt_norm(mt_sub(rk, ri));
-            if (inside && !(r < 1E-3f)) {
+            if (inside && sc != 0.0f && !(r < 1E-3f)) {
                 const int col = d.type[a];
                 float mdU = 0.0f;
                 for (int ff = 0; ff < d.nbcnt[row * T + col]; ff++)
                     mt_table_eval(d.nbtab + ((size_t)(row * T + col) * MT_MAXFF + ff) * MT_NPTS, r, true, mdU, u);
                 f = mt_mul(mt_coeffs12(ri, rk), (double)mdU);
             }
-            if (coul_here) {
-                float uc = 0.0f;
-                f = mt_add(f, mt_coulomb_pair(rk, ri, (float)rk4.w, (float)ra4.w, -1, 0, uc));
-                u += uc;
-            }
             F = mt_add(F, mt_mul(f, (double)sc));
             U += 0.5f * (u * sc);
+            if (coul_here) {
+                float uc = 0.0f;
+                const mt_d3 fc = mt_coulomb_pair(rk, ri, (float)rk4.w, (float)ra4.w, -1, 0, uc);
+                if (p.coulomb_allpairs && mixed) {
+                    // the all-pairs sum holds this pair's FP32 term with weight 1:
+                    // take out exactly that term, put in the scaled FP64 one
+                    float tx = 0.f, ty = 0.f, tz = 0.f, tu = 0.f;
+                    mt_ap_term((float)rk4.x, (float)rk4.y, (float)rk4.z, rk, MT_COULOMB_K * d.qf[k],
+                               make_float4((float)ra4.x, (float)ra4.y, (float)ra4.z, d.qf[a]), a, d.pos, p,
+                               mt_ap_geom(p), tx, ty, tz, tu);
+                    F = mt_add(F, mt_mul(fc, (double)sc));
+                    F = {F.x - (double)tx, F.y - (double)ty, F.z - (double)tz};
+                    U += 0.5f * (uc * sc) - 0.5f * tu;
+                } else {
+                    // FP64 all-pairs holds the pair's FP64 term with weight 1
+                    const float cs = p.coulomb_allpairs ? sc - 1.0f : sc;
+                    F = mt_add(F, mt_mul(fc, (double)cs));
+                    U += 0.5f * (uc * cs);
+                }
+            }
         }
     }
     double4 f0 = d.frc[k];
@@ -691,68 +843,54 @@ __global__ void __launch_bounds__(TPB) k_fast_bonded_special(MtDev d, MtParams p
 // Coulomb, FunctorCalcForce.cpp:258-278, 514-535). Each workgroup owns 256
 // atoms k and one of `nsplit` slices of the atoms i, staged through LDS;
 // slice sums land in part[split][k] and k_ap_sum adds them in split order, so
-// the result does not depend on scheduling. Same-molecule pairs and bonded
-// partners are left to the special-pair pass.
+// the result does not depend on scheduling. Every pair is summed here with
+// weight 1, same-molecule pairs and bonded partners included: the special-pair
+// pass adds (scale - 1) times their Coulomb term, which keeps exclusion tests
+// out of the O(N^2) loop.
 // ---------------------------------------------------------------------------
 #define AP_TPB 256
+
 __global__ void __launch_bounds__(AP_TPB) k_ap_coulomb_mixed(MtDev d, MtParams p, int want_e)
 {
     __shared__ float4 tq[AP_TPB];
-    __shared__ int tm[AP_TPB];
     const int t = threadIdx.x;
     const int k = blockIdx.x * AP_TPB + t;
     const bool act = k < d.n;
     const int nsplit = gridDim.y, split = blockIdx.y;
     const int per = (((d.n + nsplit - 1) / nsplit) + AP_TPB - 1) / AP_TPB * AP_TPB;
     const int i0 = split * per, i1 = min(d.n, i0 + per);
-    float xk = 0.f, yk = 0.f, zk = 0.f, qk = 0.f;
-    int molk = -1;
-    bool xk_ = false;
+    const MtApGeom g = mt_ap_geom(p);
+    float xk = 0.f, yk = 0.f, zk = 0.f, qks = 0.f;
+    mt_d3 rk = {0.0, 0.0, 0.0};
     if (act) {
         const double4 r4 = d.pos[k];
+        rk = {r4.x, r4.y, r4.z};
         xk = (float)r4.x; yk = (float)r4.y; zk = (float)r4.z;
-        qk = d.qf[k];
-        molk = d.has_molecules ? d.mol[k] : -1;
-        xk_ = d.has_xmol && d.xmol[d.perm[k]] != 0;
+        qks = MT_COULOMB_K * d.qf[k];
     }
-    const float Lx = (float)p.L[0], Ly = (float)p.L[1], Lz = (float)p.L[2];
-    const float iLx = 1.0f / Lx, iLy = 1.0f / Ly, iLz = 1.0f / Lz;
-    double Fx = 0.0, Fy = 0.0, Fz = 0.0;
-    float U = 0.0f;
+    double Fx = 0.0, Fy = 0.0, Fz = 0.0, U = 0.0;
     for (int base = i0; base < i1; base += AP_TPB) {
         __syncthreads();
         const int i = base + t;
         if (i < i1) {
             const double4 q = d.pos[i];
             tq[t] = make_float4((float)q.x, (float)q.y, (float)q.z, d.qf[i]);
-            tm[t] = d.has_molecules ? d.mol[i] : -1;
+        } else {
+            tq[t] = make_float4(xk, yk, zk, 0.f);       // zero charge: no contribution
         }
         __syncthreads();
-        if (!act || qk == 0.0f) continue;
-        const int lim = min(AP_TPB, i1 - base);
+        if (qks == 0.0f) continue;
         float fx = 0.f, fy = 0.f, fz = 0.f, u = 0.f;
-        for (int e = 0; e < lim; e++) {
-            const float4 q = tq[e];
-            float dx = xk - q.x, dy = yk - q.y, dz = zk - q.z;
-            dx -= Lx * rintf(dx * iLx);
-            dy -= Ly * rintf(dy * iLy);
-            dz -= Lz * rintf(dz * iLz);
-            const float r2 = dx * dx + dy * dy + dz * dz;
-            if (r2 > 0.0f && q.w != 0.0f && (molk < 0 || tm[e] != molk)) {
-                if (xk_ && mt_is_partner(d, k, base + e)) continue;
-                const float rinv = rsqrtf(r2);
-                const float K = MT_COULOMB_K * qk * q.w;
-                const float sc = K * rinv * rinv * rinv;
-                fx += dx * sc; fy += dy * sc; fz += dz * sc;
-                u += K * rinv;
-            }
+        const int lim = min(AP_TPB, i1 - base);
+#pragma unroll 4
+        for (int e = 0; e < AP_TPB; e++) {
+            const int ie = e < lim ? base + e : k;       // padded entries: a zero charge at the atom itself
+            mt_ap_term(xk, yk, zk, rk, qks, tq[e], ie, d.pos, p, g, fx, fy, fz, u);
         }
         Fx += (double)fx; Fy += (double)fy; Fz += (double)fz;
-        U += 0.5f * u;
+        U += 0.5 * (double)u;
     }
-    if (act) {
-        d.ap_part[(size_t)split * d.n + k] = make_double4(Fx, Fy, Fz, (double)U);
-    }
+    if (act) d.ap_part[(size_t)split * d.n + k] = make_double4(Fx, Fy, Fz, want_e ? U : 0.0);
 }
 
 // FP64 engine: the reference's pair arithmetic (mt_coulomb_pair) on double positions
@@ -760,7 +898,6 @@ __global__ void __launch_bounds__(AP_TPB) k_ap_coulomb_fp64(MtDev d, MtParams p,
 {
     __shared__ double sx[AP_TPB], sy[AP_TPB], sz[AP_TPB];
     __shared__ float sq[AP_TPB];
-    __shared__ int sm[AP_TPB];
     const int t = threadIdx.x;
     const int k = blockIdx.x * AP_TPB + t;
     const bool act = k < d.n;
@@ -769,14 +906,10 @@ __global__ void __launch_bounds__(AP_TPB) k_ap_coulomb_fp64(MtDev d, MtParams p,
     const int i0 = split * per, i1 = min(d.n, i0 + per);
     mt_d3 rk = {0.0, 0.0, 0.0};
     float qk = 0.f;
-    int molk = -1;
-    bool xk_ = false;
     if (act) {
         const double4 r4 = d.pos[k];
         rk = {r4.x, r4.y, r4.z};
         qk = (float)r4.w;
-        molk = d.has_molecules ? d.mol[k] : -1;
-        xk_ = d.has_xmol && d.xmol[d.perm[k]] != 0;
     }
     mt_d3 F = {0.0, 0.0, 0.0};
     float U = 0.0f;
@@ -787,7 +920,6 @@ __global__ void __launch_bounds__(AP_TPB) k_ap_coulomb_fp64(MtDev d, MtParams p,
             const double4 q = d.pos[i];
             sx[t] = q.x; sy[t] = q.y; sz[t] = q.z;
             sq[t] = (float)q.w;
-            sm[t] = d.has_molecules ? d.mol[i] : -1;
         }
         __syncthreads();
         if (!act || qk == 0.0f) continue;
@@ -795,8 +927,6 @@ __global__ void __launch_bounds__(AP_TPB) k_ap_coulomb_fp64(MtDev d, MtParams p,
         for (int e = 0; e < lim; e++) {
             const float qi = sq[e];
             if (qi == 0.0f || base + e == k) continue;
-            if (molk >= 0 && sm[e] == molk) continue;
-            if (xk_ && mt_is_partner(d, k, base + e)) continue;
             const mt_d3 ri = mt_move_same_side(rk, {sx[e], sy[e], sz[e]}, p.wbox);
             float u = 0.0f;
             F = mt_add(F, mt_coulomb_pair(rk, ri, qk, qi, -1, 0, u));
@@ -865,7 +995,7 @@ void mt_fast_forces(const MtDev& d, const MtParams& p, int mixed, int want_e, in
     }
     if (d.bl.has_bonds || d.bl.has_angles || d.bl.has_dihedrals || d.sp.n_entries > 0)
         hipLaunchKernelGGL(k_fast_bonded_special, dim3(nblk(d.n, TPB)), dim3(TPB), 0, s, d, p, want_e,
-                           coul_all ? 1 : 0);
+                           coul_all ? 1 : 0, mixed);
 }
 
 // ---------------------------------------------------------------------------
@@ -983,15 +1113,23 @@ void mt_fast_clear_flags(const MtDev& d, hipStream_t s)
     hipLaunchKernelGGL(k_fast_clear, dim3(1), dim3(1), 0, s, d.flags);
 }
 
-// plain streaming copy (read 32 B + write 32 B per element): the reference
-// point for the HBM byte counters
-__global__ void __launch_bounds__(TPB) k_copy_d4(const double4* __restrict__ src, double4* __restrict__ dst, int n)
+// plain streaming pass over the neighbour-list buffer, read and written back
+// in place in 16-byte words (ncap * n * 4 bytes each way): the reference point
+// for the HBM byte counters and the achievable streaming bandwidth. The buffer
+// is far larger than the caches, so every byte crosses HBM.
+__global__ void __launch_bounds__(TPB) k_copy_u4(uint4* __restrict__ buf, size_t nwords)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) dst[i] = src[i];
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nwords; i += (size_t)gridDim.x * blockDim.x) {
+        uint4 v = buf[i];
+        // opaque to the compiler, so the write-back of the same value is kept
+        asm volatile("" : "+v"(v.x), "+v"(v.y), "+v"(v.z), "+v"(v.w));
+        buf[i] = v;
+    }
 }
 
-void mt_copy_d4(const double4* src, double4* dst, int n, hipStream_t s)
+void mt_copy_lists(const MtDev& d, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_copy_d4, dim3(nblk(n, TPB)), dim3(TPB), 0, s, src, dst, n);
+    const size_t nwords = (size_t)d.ncap * d.n / 4;
+    const int g = (int)std::min<size_t>((nwords + TPB - 1) / TPB, 65536);
+    hipLaunchKernelGGL(k_copy_u4, dim3(g), dim3(TPB), 0, s, reinterpret_cast<uint4*>(d.nlist), nwords);
 }

@@ -461,7 +461,8 @@ static int setup(mtmd_handle* h, const mtmd_system* s)
 
     // --- special pairs for the FP64/MIXED engines: every same-molecule pair and
     // every scaled cross-molecule partner, with the reference's scale
-    // (scaleForcesAndPotentials, FunctorCalcForce.cpp:95-157); zero scales dropped ---
+    // (scaleForcesAndPotentials, FunctorCalcForce.cpp:95-157); zero scales dropped
+    // unless all-pairs Coulomb needs them ---
     std::vector<int> sp_first(n, 0), sp_num(n, 0), sp_partner;
     std::vector<float> sp_scale;
     if (h->fast) {
@@ -498,7 +499,12 @@ static int setup(mtmd_handle* h, const mtmd_system* s)
                 for (int i : cand) {
                     if (i == k) continue;
                     float sc = scale_of(i);
-                    if (sc != 0.0f) { sp_partner.push_back(i); sp_scale.push_back(sc); }
+                    // all-pairs Coulomb sums every pair with weight 1, so a zero
+                    // scale is kept there: the special pass subtracts the pair
+                    if (sc != 0.0f || (p.coulomb_on && p.coulomb_allpairs)) {
+                        sp_partner.push_back(i);
+                        sp_scale.push_back(sc);
+                    }
                 }
                 sp_num[k] = (int)sp_partner.size() - sp_first[k];
             }
@@ -644,6 +650,7 @@ static int setup(mtmd_handle* h, const mtmd_system* s)
     d.cell_count = h->mem.alloc<int>(d.ncells);
     d.cell_first = h->mem.alloc<int>(d.ncells);
     d.cell_fill = h->mem.alloc<int>(d.ncells);
+    d.cell_bsum = h->mem.alloc<int>((d.ncells + 1023) / 1024 + 1);
     d.cell_list = h->mem.alloc<int>(n);
     d.ncount = h->mem.alloc<int>(n);
     d.nreq = h->mem.alloc<int>(1);
@@ -651,6 +658,7 @@ static int setup(mtmd_handle* h, const mtmd_system* s)
     d.red = h->mem.alloc<double>(d.red_blocks);
     d.flags = h->mem.alloc<unsigned int>(8);
     if (!d.frc || !d.epot || !d.cell_key || !d.cell_count || !d.cell_first || !d.cell_fill || !d.cell_list ||
+        !d.cell_bsum ||
         !d.ncount || !d.nreq || !d.red || !d.flags || (!h->fast && !d.cell_xyz))
         return h->fail(MTMD_ENOMEM, "device allocation failed");
     HIPCHK(h, hipMemset(d.frc, 0, n * sizeof(double4)));
@@ -1229,7 +1237,7 @@ extern "C" int mtmd_bench_kernel(mtmd_handle* h, int which, int reps, double* av
         } else if (which == 2) {
             rc = md_steps(h, 1, false);
         } else {
-            mt_copy_d4(h->d.frc, h->d.frc2, h->n, h->stream);
+            mt_copy_lists(h->d, h->stream);
             rc = MTMD_OK;
         }
         if (rc) break;

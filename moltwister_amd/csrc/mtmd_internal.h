@@ -64,6 +64,7 @@ struct MtDev {
     int nc[3]; int ncells;
     int* cell_key; int* cell_count; int* cell_first; int* cell_fill; int* cell_list;
     int* cell_xyz;         // EXACT: ix, iy, iz per atom
+    int* cell_bsum;        // FP64/MIXED: per-block sums of the cell-count scan
     // neighbours
     int ncap; int* ncount; uint32_t* nlist; int* nreq;
     // sorting (FP64/MIXED): perm[s] = original index of sorted slot s, inv[o] = slot
@@ -124,7 +125,8 @@ void mt_fast_neighbors(const MtDev& d, const MtParams& p, int conditional, hipSt
 void mt_fast_forces(const MtDev& d, const MtParams& p, int mixed, int want_energy, int block, hipStream_t s);
 void mt_fast_kick_drift(const MtDev& d, const MtParams& p, int scale_slot, int mixed, hipStream_t s);
 void mt_fast_clear_flags(const MtDev& d, hipStream_t s);
-void mt_copy_d4(const double4* src, double4* dst, int n, hipStream_t s);
+// in-place read + write of the whole neighbour-list buffer (HBM reference)
+void mt_copy_lists(const MtDev& d, hipStream_t s);
 // per-block partial sums of U_cut into d.red[0 .. mt_reduce_blocks(n))
 void mt_ucut(const MtDev& d, const MtParams& p, int exact, hipStream_t s);
 

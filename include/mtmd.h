@@ -73,6 +73,16 @@ enum { MTMD_COUL_AUTO = 0, MTMD_COUL_ALLPAIRS = 1, MTMD_COUL_CUTOFF = 2 };
 
 /* force-field kinds */
 enum { MTMD_NB_LJ = 1, MTMD_NB_BUCK = 2, MTMD_NB_LJ1208 = 3 };
+/* Any force field given as its tabulated profile instead of parameters: the
+ * MTMD_NUM_POINTS points (x, -dU/dx, U) that the reference's
+ * calc1DForceProfile / calc1DPotentialProfile give for it, as they are fed to
+ * the force matrices (MDFFMatrices.cpp:270-290 non-bonded, 296-382 bonded).
+ * `profile` then points at 3 * MTMD_NUM_POINTS floats: x[], f[], e[]. Every
+ * MDFF type of the reference can be passed this way, the tables are then
+ * bit-identical by construction. The x range is the one the reference
+ * samples: non-bonded and bonds 0.01 .. cutoff, angles and dihedrals
+ * 0 .. 2 pi (the FP64/MIXED engines check the non-bonded range). */
+#define MTMD_FF_PROFILE 100
 enum { MTMD_BOND_HARM = 1, MTMD_BOND_MORSE = 2 };
 enum { MTMD_ANGLE_HARM = 1 };
 enum { MTMD_DIH_HARM = 1 };
@@ -121,6 +131,7 @@ typedef struct mtmd_config {
 typedef struct mtmd_nonbonded_ff {
     int32_t type_a, type_b, kind, pad;
     double  p[4];
+    const float* profile;      /* kind == MTMD_FF_PROFILE only, else NULL   */
 } mtmd_nonbonded_ff;
 
 /* One bonded force-field type. Parameters:
@@ -131,6 +142,7 @@ typedef struct mtmd_nonbonded_ff {
 typedef struct mtmd_bonded_ff {
     int32_t kind, pad;
     double  p[4];
+    const float* profile;      /* kind == MTMD_FF_PROFILE only, else NULL   */
 } mtmd_bonded_ff;
 
 /* The system: the inputs CMDFFMatrices::prepareFFMatrices and CSimulationBox

@@ -27,6 +27,7 @@ ENGINE_EXACT, ENGINE_FP64, ENGINE_MIXED = 0, 1, 2
 COUL_AUTO, COUL_ALLPAIRS, COUL_CUTOFF = 0, 1, 2
 
 NB_LJ, NB_BUCK, NB_LJ1208 = 1, 2, 3
+FF_PROFILE = 100          # force field given as its (x, -dU/dx, U) profile
 BOND_HARM, BOND_MORSE = 1, 2
 ANGLE_HARM = 1
 DIH_HARM = 1
@@ -68,11 +69,12 @@ class MtmdConfig(C.Structure):
 
 class MtmdNonbondedFF(C.Structure):
     _fields_ = [("type_a", C.c_int32), ("type_b", C.c_int32), ("kind", C.c_int32),
-                ("pad", C.c_int32), ("p", C.c_double * 4)]
+                ("pad", C.c_int32), ("p", C.c_double * 4), ("profile", C.POINTER(C.c_float))]
 
 
 class MtmdBondedFF(C.Structure):
-    _fields_ = [("kind", C.c_int32), ("pad", C.c_int32), ("p", C.c_double * 4)]
+    _fields_ = [("kind", C.c_int32), ("pad", C.c_int32), ("p", C.c_double * 4),
+                ("profile", C.POINTER(C.c_float))]
 
 
 _dp = C.POINTER(C.c_double)

@@ -237,6 +237,14 @@ template <class T> int upload(mtmd_handle* h, T*& dst, const std::vector<T>& src
     return MTMD_OK;
 }
 
+// a caller-supplied profile: x[], f[], e[] of MT_NPTS floats each
+std::vector<float4> table_from_profile(const float* pr)
+{
+    std::vector<float4> t(MT_NPTS);
+    for (int i = 0; i < MT_NPTS; i++) t[i] = make_float4(pr[i], pr[MT_NPTS + i], pr[2 * MT_NPTS + i], 0.0f);
+    return t;
+}
+
 std::vector<float4> make_table(int which, int kind, const double* p, float x0, float x1)
 {
     std::vector<float> x(MT_NPTS), f(MT_NPTS), e(MT_NPTS);
@@ -323,9 +331,16 @@ static int setup(mtmd_handle* h, const mtmd_system* s)
                 if (cnt >= MT_MAXFF)
                     return h->fail(MTMD_EINVAL, "more than %d non-bonded force fields for types %d-%d", MT_MAXFF,
                                    row, col);
-                if (e.kind != MTMD_NB_LJ && e.kind != MTMD_NB_BUCK && e.kind != MTMD_NB_LJ1208)
+                if (e.kind != MTMD_NB_LJ && e.kind != MTMD_NB_BUCK && e.kind != MTMD_NB_LJ1208 &&
+                    e.kind != MTMD_FF_PROFILE)
                     return h->fail(MTMD_EINVAL, "non-bonded entry %d: unknown kind %d", i, e.kind);
-                auto t = make_table(0, e.kind, e.p, 0.01f, p.rc);
+                if (e.kind == MTMD_FF_PROFILE && !e.profile)
+                    return h->fail(MTMD_EINVAL, "non-bonded entry %d: profile kind without a profile", i);
+                if (e.kind == MTMD_FF_PROFILE && h->fast &&
+                    (e.profile[0] != 0.01f || e.profile[MT_NPTS - 1] != p.rc))
+                    return h->fail(MTMD_EINVAL, "non-bonded entry %d: the profile must span 0.01 .. cutoff", i);
+                auto t = e.kind == MTMD_FF_PROFILE ? table_from_profile(e.profile)
+                                                   : make_table(0, e.kind, e.p, 0.01f, p.rc);
                 std::copy(t.begin(), t.end(), nbtab.begin() + ((size_t)(row * T + col) * MT_MAXFF + cnt) * MT_NPTS);
                 cnt++;
             }
@@ -335,6 +350,14 @@ static int setup(mtmd_handle* h, const mtmd_system* s)
                              std::vector<float4>& out) -> int {
         out.assign((size_t)std::max(cnt, 1) * MT_NPTS, make_float4(0, 0, 0, 0));
         for (int i = 0; i < cnt; i++) {
+            if (ff[i].kind == MTMD_FF_PROFILE) {
+                if (!ff[i].profile)
+                    return h->fail(MTMD_EINVAL, "bonded force field %d (table %d): profile kind without a profile",
+                                   i, which);
+                auto t = table_from_profile(ff[i].profile);
+                std::copy(t.begin(), t.end(), out.begin() + (size_t)i * MT_NPTS);
+                continue;
+            }
             // a two-point profile with no outputs only validates the kind
             if (mtmd_profile(which, ff[i].kind, ff[i].p, x0, x1, 2, nullptr, nullptr, nullptr) != MTMD_OK)
                 return h->fail(MTMD_EINVAL, "bonded force field %d (table %d): unknown kind %d", i, which, ff[i].kind);

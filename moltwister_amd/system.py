@@ -79,9 +79,18 @@ class System:
         s.type = _ptr(self.types, C.c_int32)
         s.mobile = _ptr(self.mobile, C.c_uint8)
         s.box[:] = self.box
+        def profile_ptr(prm):
+            # FF_PROFILE entries carry a (3, 1000) array x, -dU/dx, U instead of parameters
+            arr = np.ascontiguousarray(np.asarray(prm, dtype=np.float32).reshape(3, -1))
+            keep.append(arr)
+            return arr.ctypes.data_as(C.POINTER(C.c_float))
+
         nb = (abi.MtmdNonbondedFF * max(1, len(self.nonbonded)))()
         for i, (a, b, kind, prm) in enumerate(self.nonbonded):
             nb[i].type_a, nb[i].type_b, nb[i].kind = a, b, kind
+            if kind == abi.FF_PROFILE:
+                nb[i].profile = profile_ptr(prm)
+                continue
             p = list(prm) + [0.0] * (4 - len(prm))
             if kind in (abi.NB_LJ, abi.NB_LJ1208) and len(prm) < 3:
                 p[2] = 1.0   # alpha defaults to 1 (MolTwisterMDFFNonBonded_LJ.h:29)
@@ -94,7 +103,10 @@ class System:
             arr = (abi.MtmdBondedFF * max(1, len(lst)))()
             for i, (kind, prm) in enumerate(lst):
                 arr[i].kind = kind
-                arr[i].p[:] = list(prm) + [0.0] * (4 - len(prm))
+                if kind == abi.FF_PROFILE:
+                    arr[i].profile = profile_ptr(prm)
+                else:
+                    arr[i].p[:] = list(prm) + [0.0] * (4 - len(prm))
             setattr(s, nfield, len(lst))
             setattr(s, pfield, C.cast(arr, C.POINTER(abi.MtmdBondedFF)))
             keep.append(arr)

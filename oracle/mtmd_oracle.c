@@ -486,6 +486,19 @@ static double calc_temp_reduced(const orc_md* o)
  * only equality of labels matters to the force functor (FCF:147-156) */
 static int uf_find(int* par, int a) { while (par[a] != a) { par[a] = par[par[a]]; a = par[a]; } return a; }
 
+/* a table from parameters (orc_profile) or, for MTMD_FF_PROFILE, the
+ * caller's own points (what MDFFMatrices.cpp:270-382 copies in) */
+static void orc_table(int which, int kind, const double* p, const float* profile, float x0, float x1, float* xs,
+                      float* fs, float* es)
+{
+    int j;
+    if (kind == MTMD_FF_PROFILE && profile) {
+        for (j = 0; j < NPTS; j++) { xs[j] = profile[j]; fs[j] = profile[NPTS + j]; es[j] = profile[2 * NPTS + j]; }
+        return;
+    }
+    orc_profile(which, kind, p, x0, x1, NPTS, xs, fs, es);
+}
+
 orc_md* orc_create(const mtmd_system* s, const mtmd_config* cfg)
 {
     int n = s->n_atoms, T = s->n_types, k, j, i, c, r;
@@ -531,7 +544,7 @@ orc_md* orc_create(const mtmd_system* s, const mtmd_config* cfg)
                 int match = (e->type_a == c && e->type_b == r) || (e->type_b == c && e->type_a == r);
                 if (!match || cnt >= MAXFF) continue;
                 float xs[NPTS], fs[NPTS], es[NPTS];
-                orc_profile(0, e->kind, e->p, 0.01f, o->rc, NPTS, xs, fs, es);
+                orc_table(0, e->kind, e->p, e->profile, 0.01f, o->rc, xs, fs, es);
                 for (j = 0; j < NPTS; j++) {
                     pt* d = &o->nb[(((size_t)r * T + c) * MAXFF + cnt) * NPTS + j];
                     d->x = xs[j]; d->f = fs[j]; d->e = es[j];
@@ -548,15 +561,17 @@ orc_md* orc_create(const mtmd_system* s, const mtmd_config* cfg)
         float xs[NPTS], fs[NPTS], es[NPTS];
         pt* dst;
         if (i < s->n_bond_ff) {
-            orc_profile(1, s->bond_ff[i].kind, s->bond_ff[i].p, 0.01f, o->rc, NPTS, xs, fs, es);
+            orc_table(1, s->bond_ff[i].kind, s->bond_ff[i].p, s->bond_ff[i].profile, 0.01f, o->rc, xs, fs, es);
             dst = o->bondtab + (size_t)i * NPTS;
         } else if (i < s->n_bond_ff + s->n_angle_ff) {
             int t = i - s->n_bond_ff;
-            orc_profile(2, s->angle_ff[t].kind, s->angle_ff[t].p, 0.0f, (float)(2.0 * M_PI), NPTS, xs, fs, es);
+            orc_table(2, s->angle_ff[t].kind, s->angle_ff[t].p, s->angle_ff[t].profile, 0.0f, (float)(2.0 * M_PI),
+                      xs, fs, es);
             dst = o->angtab + (size_t)t * NPTS;
         } else {
             int t = i - s->n_bond_ff - s->n_angle_ff;
-            orc_profile(3, s->dihedral_ff[t].kind, s->dihedral_ff[t].p, 0.0f, (float)(2.0 * M_PI), NPTS, xs, fs, es);
+            orc_table(3, s->dihedral_ff[t].kind, s->dihedral_ff[t].p, s->dihedral_ff[t].profile, 0.0f,
+                      (float)(2.0 * M_PI), xs, fs, es);
             dst = o->dihtab + (size_t)t * NPTS;
         }
         for (j = 0; j < NPTS; j++) { dst[j].x = xs[j]; dst[j].f = fs[j]; dst[j].e = es[j]; }

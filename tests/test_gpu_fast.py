@@ -206,3 +206,15 @@ def test_ucut_brute_force_and_engines_agree():
         got[eng] = e.thermo()["U_cut"]
     for eng, v in got.items():
         assert abs(v - want) <= 1e-9 * abs(want), (eng, v, want)
+
+
+@pytest.mark.parametrize("engine", [abi.ENGINE_EXACT, abi.ENGINE_FP64, abi.ENGINE_MIXED])
+def test_profile_kind_matches_parameters(engine):
+    """MTMD_FF_PROFILE force fields give bit-identical forces to the same force
+    fields given by parameters, in every engine."""
+    from test_oracle_golden import as_profiles
+    print(f"case profile kind engine={engine}", flush=True)
+    s, c = make("water_300", engine=engine, coulomb=abi.COUL_ALLPAIRS)
+    a = Engine(s, c).compute_forces()
+    b = Engine(as_profiles(s, c.cutoff_radius), c).compute_forces()
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])

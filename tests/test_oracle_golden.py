@@ -89,3 +89,29 @@ def test_library_profiles_match_golden():
         assert np.array_equal(x, z[f"p{i}_x"]) and np.array_equal(f, z[f"p{i}_f"], equal_nan=True)
         assert np.array_equal(e, z[f"p{i}_e"], equal_nan=True)
         i += 1
+
+
+def as_profiles(s, rc):
+    """The same system with every force field passed as its tabulated profile
+    (MTMD_FF_PROFILE), the way a MolTwister binding hands over the tables its
+    MDFF classes compute."""
+    import copy
+    from moltwister_amd.engine import profile
+    t = copy.copy(s)
+    rcf = float(np.float32(rc))
+    t.nonbonded = [(a, b, abi.FF_PROFILE, np.stack(profile(0, k, p, 0.01, rcf))) for a, b, k, p in s.nonbonded]
+    t.bond_ff = [(abi.FF_PROFILE, np.stack(profile(1, k, p, 0.01, rcf))) for k, p in s.bond_ff]
+    two_pi = float(np.float32(2 * np.pi))
+    t.angle_ff = [(abi.FF_PROFILE, np.stack(profile(2, k, p, 0.0, two_pi))) for k, p in s.angle_ff]
+    t.dihedral_ff = [(abi.FF_PROFILE, np.stack(profile(3, k, p, 0.0, two_pi))) for k, p in s.dihedral_ff]
+    return t
+
+
+@pytest.mark.parametrize("name", ["water_300", "chains", "binary_lj"])
+def test_oracle_profile_kind_matches_parameters(oracle_mod, name):
+    """Force fields passed as profiles give the same forces, bit for bit."""
+    from systems import make
+    s, c = make(name)
+    a = oracle_mod.Oracle(s, c).compute_forces()
+    b = oracle_mod.Oracle(as_profiles(s, c.cutoff_radius), c).compute_forces()
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])

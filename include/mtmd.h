@@ -294,6 +294,42 @@ int mtmd_run(const mtmd_config* cfg, const mtmd_system* sys, void* log_file);
 /* CMDSimulator::optimize (Simulator/MDSimulator.cpp:49-66). */
 int mtmd_run_optimize(const mtmd_config* cfg, const mtmd_system* sys, void* log_file);
 
+/* ---- batches of independent systems (replicas) ----
+ * No counterpart in the reference, which advances one system per `moldyn run`
+ * (one CSimulationBox + CMDLoop, SimulationBox.cpp:27-133, MDLoop.cpp:107-141).
+ * A batch advances n such systems in lockstep: the replicas share ensemble,
+ * time step, cutoff, skin, chain length, RESPA steps, force cap and thermostat
+ * relaxation time (cfgs[i] must agree on them; temperatures may differ), and
+ * each replica has its own box, atoms, momenta and Nose-Hoover chain, with the
+ * reference's per-step sequence (MDLoop.cpp:124-138). Arithmetic as the MIXED
+ * engine. Replicas take tabulated non-bonded force fields only (all replicas
+ * the same list), no charges, no bonded terms, all atoms mobile; boxes with
+ * fewer than three cells per axis are allowed (each distinct cell is searched
+ * once) but must exceed twice cutoff + skin. */
+typedef struct mtmd_batch mtmd_batch;
+
+int  mtmd_batch_create(int n_systems, const mtmd_system* systems, const mtmd_config* cfgs, mtmd_batch** out);
+void mtmd_batch_destroy(mtmd_batch* b);
+const char* mtmd_batch_last_error(const mtmd_batch* b);
+/* nsteps of every replica: one captured hipGraph of the whole batch's step,
+ * replayed; the last step runs eagerly and also computes energies */
+int mtmd_batch_step(mtmd_batch* b, int32_t nsteps);
+/* the same steps with every kernel launched replica by replica (the cost of
+ * running n separate systems); same results */
+int mtmd_batch_step_separate(mtmd_batch* b, int32_t nsteps);
+int mtmd_batch_synchronize(mtmd_batch* b);
+/* thermo of every replica, out[n_systems] (T, P, V, U, K, Etot, sums; U_cut 0) */
+int mtmd_batch_get_thermo(mtmd_batch* b, mtmd_thermo* out);
+/* state, forces (capped, as integrated) and per-atom energies of replica i */
+int mtmd_batch_get_state(mtmd_batch* b, int32_t i, double* pos, double* mom);
+int mtmd_batch_get_forces(mtmd_batch* b, int32_t i, double* F, float* U);
+/* block-size autotuner: times the pair-force kernel (64..512) and the list
+ * build (64..256) over the whole batch, `reps` launches each, keeps the
+ * fastest; returns the chosen kernels' average times */
+int mtmd_batch_autotune(mtmd_batch* b, int reps, double* force_ms, double* list_ms);
+int mtmd_batch_info(const mtmd_batch* b, int32_t* n_systems, int32_t* n_atoms, int32_t* force_block,
+                    int32_t* list_block);
+
 #ifdef __cplusplus
 }
 #endif

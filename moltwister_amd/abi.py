@@ -125,6 +125,9 @@ EXPORTED = [
     "mtmd_build_neighbors", "mtmd_compute_forces", "mtmd_step", "mtmd_get_thermo",
     "mtmd_get_forces", "mtmd_optimize", "mtmd_get_stats", "mtmd_synchronize", "mtmd_run",
     "mtmd_run_optimize", "mtmd_bench_kernel",
+    "mtmd_batch_create", "mtmd_batch_destroy", "mtmd_batch_last_error", "mtmd_batch_step",
+    "mtmd_batch_step_separate", "mtmd_batch_synchronize", "mtmd_batch_get_thermo", "mtmd_batch_get_state",
+    "mtmd_batch_get_forces", "mtmd_batch_autotune", "mtmd_batch_info",
 ]
 
 _lib = None
@@ -168,6 +171,18 @@ def load_library(path: str | None = None) -> C.CDLL:
         "mtmd_run": (C.c_int, [C.POINTER(MtmdConfig), C.POINTER(MtmdSystem), vp]),
         "mtmd_run_optimize": (C.c_int, [C.POINTER(MtmdConfig), C.POINTER(MtmdSystem), vp]),
         "mtmd_bench_kernel": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(C.c_double)]),
+        "mtmd_batch_create": (C.c_int, [C.c_int, C.POINTER(MtmdSystem), C.POINTER(MtmdConfig),
+                                        C.POINTER(C.c_void_p)]),
+        "mtmd_batch_destroy": (None, [vp]),
+        "mtmd_batch_last_error": (C.c_char_p, [vp]),
+        "mtmd_batch_step": (C.c_int, [vp, C.c_int32]),
+        "mtmd_batch_step_separate": (C.c_int, [vp, C.c_int32]),
+        "mtmd_batch_synchronize": (C.c_int, [vp]),
+        "mtmd_batch_get_thermo": (C.c_int, [vp, C.POINTER(MtmdThermo)]),
+        "mtmd_batch_get_state": (C.c_int, [vp, C.c_int32, _dp, _dp]),
+        "mtmd_batch_get_forces": (C.c_int, [vp, C.c_int32, _dp, C.POINTER(C.c_float)]),
+        "mtmd_batch_autotune": (C.c_int, [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+        "mtmd_batch_info": (C.c_int, [vp, _ip, _ip, _ip, _ip]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

@@ -1,0 +1,945 @@
+// mtmd_batch.hip - many independent systems (replicas) advanced together.
+//
+// A batch is n independent `moldyn` runs (each a CSimulationBox + CMDLoop of
+// the reference) that share the integration settings (time step, cutoff,
+// skin, ensemble, chain length) and differ in size, box, temperature and
+// state. Atoms of all replicas are concatenated; every kernel of a step
+// covers every replica in one launch (per-replica data is looked up through
+// the atom's replica id), and the whole step is captured once in a hipGraph
+// and replayed. For comparison the same kernels can be launched replica by
+// replica (mtmd_batch_step_separate), the way n separate engines would run.
+//
+// Precision is that of the MIXED engine: FP32 pair geometry and table
+// interpolation, FP64 force accumulation, integration, Nose-Hoover chains and
+// reductions. Pair vectors are minimum images in FP32 of FP32 positions
+// (replica boxes are small, |x| < 20 sigma for the target workload), so small
+// boxes with fewer than three cells per axis are handled without the
+// reference's duplicate-neighbour behaviour: each distinct cell is visited
+// once. Neighbour lists are skin lists rebuilt per replica on displacement.
+//
+// Supported per replica: tabulated non-bonded force fields (any kind, or
+// profiles), no charges, no bonded terms, all atoms mobile.
+#include "mtmd_internal.h"
+#include "mtmd_device.h"
+#include "../../include/mtmd.h"
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#define MB_TPB 256
+#define MB_MAXM 8
+
+static inline int mb_blk(int n, int b) { return (n + b - 1) / b; }
+static inline float mb_bits_as_float(int v) { float f; std::memcpy(&f, &v, sizeof f); return f; }
+
+// per-replica constants (device array)
+struct MbRep {
+    double L[3];            // box widths (centred on the origin)
+    float Lf[3], iLf[3];    // FP32 widths and their inverses (minimum image)
+    int nc[3];              // cell grid
+    int cell0, ncell;       // first global cell, number of cells
+    int atom0, n;           // atom range
+    double T;               // target temperature (reduced units)
+    double g;               // 3 n degrees of freedom
+};
+
+struct MbDev {
+    int n, nrep, ncells, T, M;
+    double4* pos; double4* mom; double4* frc; double4* posref;
+    float4* posf; float* epot; int* rid;
+    int* cell_key; int* cell_count; int* cell_first; int* cell_fill; int* cell_list; int* cell_rid;
+    int ncap; int* ncount; uint32_t* nlist; int* nreq;
+    float2* nbcoef2; float4* nbcoef;
+    MbRep* rep;
+    double* nh;             // per replica: eta[M], p_eta[M], Q[M], S, KE2  (stride 3M + 2)
+    double* red;            // per replica: sum p.p/m, sum F.r, sum U
+    unsigned int* rflag;    // per replica: rebuild the lists this step
+    unsigned int* flags;    // [0] list overflow, [1] non-finite position
+};
+
+struct MbPar {
+    double dt, Fcut;
+    float rc2, rlist2, x0, inv_delta, half_skin2;
+    int nresp;
+    double w[3];
+    int nvt;
+};
+
+// ---------------------------------------------------------------------------
+// kernels; ranges [a0, a1) atoms, [c0, c1) cells, [r0, r1) replicas
+// ---------------------------------------------------------------------------
+
+// per-replica sums (one workgroup per replica): p.p/m, F.r, U
+__global__ void __launch_bounds__(MB_TPB) k_mb_reduce(MbDev d, int r0, int want_all)
+{
+    __shared__ double sh[3][MB_TPB];
+    const int r = r0 + blockIdx.x;
+    const MbRep rp = d.rep[r];
+    double a = 0.0, b = 0.0, c = 0.0;
+    for (int k = rp.atom0 + threadIdx.x; k < rp.atom0 + rp.n; k += MB_TPB) {
+        const double4 pm = d.mom[k];
+        a += ((pm.x * pm.x) + (pm.y * pm.y) + (pm.z * pm.z)) / pm.w;
+        if (want_all) {
+            const double4 F = d.frc[k], x = d.pos[k];
+            b += (F.x * x.x) + (F.y * x.y) + (F.z * x.z);
+            c += (double)d.epot[k];
+        }
+    }
+    sh[0][threadIdx.x] = a; sh[1][threadIdx.x] = b; sh[2][threadIdx.x] = c;
+    __syncthreads();
+    for (int off = MB_TPB / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off)
+            for (int q = 0; q < 3; q++) sh[q][threadIdx.x] += sh[q][threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        d.red[3 * r] = sh[0][0];
+        if (want_all) { d.red[3 * r + 1] = sh[1][0]; d.red[3 * r + 2] = sh[2][0]; }
+    }
+}
+
+// one Nose-Hoover chain propagation per replica (one thread per replica)
+__global__ void k_mb_nh(MbDev d, MbPar p, int r0, int r1)
+{
+    const int r = r0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= r1) return;
+    const int M = d.M, stride = 3 * M + 2;
+    double* nh = d.nh + (size_t)r * stride;
+    const MbRep rp = d.rep[r];
+    nh[3 * M] = mt_nh_propagate(nh, nh + M, nh + 2 * M, M, p.nresp, rp.T, p.dt, rp.g, p.w, d.red[3 * r]);
+}
+
+// thermostat scale, half kick, drift, FP32 copy, per-replica displacement flag
+__global__ void __launch_bounds__(MB_TPB) k_mb_kick_drift(MbDev d, MbPar p, int a0, int a1)
+{
+    const int k = a0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= a1) return;
+    const int r = d.rid[k];
+    double4 pm = d.mom[k], x = d.pos[k];
+    const double4 F = d.frc[k];
+    if (p.nvt) {
+        const double S = d.nh[(size_t)r * (3 * d.M + 2) + 3 * d.M];
+        pm.x *= S; pm.y *= S; pm.z *= S;
+    }
+    const double dt_2 = p.dt / 2.0;
+    pm.x += F.x * dt_2; pm.y += F.y * dt_2; pm.z += F.z * dt_2;
+    const double c = p.dt / pm.w;
+    x.x += pm.x * c; x.y += pm.y * c; x.z += pm.z * c;
+    d.mom[k] = pm;
+    d.pos[k] = x;
+    const float4 old = d.posf[k];
+    d.posf[k] = make_float4((float)x.x, (float)x.y, (float)x.z, old.w);
+    const double4 q = d.posref[k];
+    const MbRep& rp = d.rep[r];
+    double dx = x.x - q.x, dy = x.y - q.y, dz = x.z - q.z;
+    dx -= rp.L[0] * rint(dx / rp.L[0]);
+    dy -= rp.L[1] * rint(dy / rp.L[1]);
+    dz -= rp.L[2] * rint(dz / rp.L[2]);
+    if ((float)(dx * dx + dy * dy + dz * dz) > p.half_skin2 && d.rflag[r] == 0u) d.rflag[r] = 1u;
+}
+
+// ---- conditional list rebuild (replicas with rflag set) ----
+
+__global__ void __launch_bounds__(MB_TPB) k_mb_cells_clear(MbDev d, int c0, int c1)
+{
+    const int c = c0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= c1 || !d.rflag[d.cell_rid[c]]) return;
+    d.cell_count[c] = 0;
+    d.cell_fill[c] = 0;
+}
+
+__device__ __forceinline__ void mb_key_runs(int key, int& head_lane, int& len)
+{
+    const int lane = threadIdx.x & 63;
+    const int prev = __shfl_up(key, 1);
+    const unsigned long long heads = __ballot(lane == 0 || prev != key);
+    head_lane = 63 - __clzll(heads & ((2ull << lane) - 1ull));
+    const unsigned long long above = heads & ~((2ull << head_lane) - 1ull);
+    len = (above ? __ffsll(above) - 1 : 64) - head_lane;
+}
+
+__global__ void __launch_bounds__(MB_TPB) k_mb_cell_keys(MbDev d, int a0, int a1)
+{
+    const int k = a0 + blockIdx.x * blockDim.x + threadIdx.x;
+    int key = -1;
+    if (k < a1) {
+        const int r = d.rid[k];
+        if (d.rflag[r]) {
+            const MbRep& rp = d.rep[r];
+            const double4 x = d.pos[k];
+            if (!(isfinite(x.x) && isfinite(x.y) && isfinite(x.z))) atomicOr(&d.flags[1], 1u);
+            int i3[3];
+            const double xs[3] = {x.x, x.y, x.z};
+            for (int a = 0; a < 3; a++) {
+                // position folded into the box first: between wraps an atom can sit just outside
+                double u = xs[a] - rp.L[a] * floor(xs[a] / rp.L[a] + 0.5);
+                int ic = (int)floor((u + rp.L[a] / 2.0) * (double)rp.nc[a] / rp.L[a]);
+                i3[a] = ic < 0 ? 0 : (ic >= rp.nc[a] ? rp.nc[a] - 1 : ic);
+            }
+            key = rp.cell0 + i3[0] + rp.nc[0] * (i3[1] + rp.nc[1] * i3[2]);
+            d.cell_key[k] = key;
+        }
+    }
+    int hl, len;
+    mb_key_runs(key, hl, len);
+    if (key >= 0 && (int)(threadIdx.x & 63) == hl) atomicAdd(&d.cell_count[key], len);
+}
+
+// exclusive scan of one replica's cell counts per workgroup; the cells of
+// replica r hold exactly its atoms, so its first slot is atom0
+__global__ void __launch_bounds__(MB_TPB) k_mb_scan(MbDev d, int r0)
+{
+    __shared__ int ws[MB_TPB / 64];
+    __shared__ int carry;
+    const int r = r0 + blockIdx.x;
+    if (!d.rflag[r]) return;
+    const MbRep rp = d.rep[r];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    if (t == 0) carry = rp.atom0;
+    __syncthreads();
+    for (int b = 0; b < rp.ncell; b += MB_TPB) {
+        const int c = b + t;
+        const int v = c < rp.ncell ? d.cell_count[rp.cell0 + c] : 0;
+        int incl = v;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int u = __shfl_up(incl, o);
+            if (lane >= o) incl += u;
+        }
+        if (lane == 63) ws[w] = incl;
+        __syncthreads();
+        int wpre = 0;
+        for (int j = 0; j < w; j++) wpre += ws[j];
+        if (c < rp.ncell) d.cell_first[rp.cell0 + c] = carry + wpre + incl - v;
+        __syncthreads();
+        if (t == MB_TPB - 1) carry += wpre + incl;
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(MB_TPB) k_mb_scatter(MbDev d, int a0, int a1)
+{
+    const int k = a0 + blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int key = (k < a1 && d.rflag[d.rid[k]]) ? d.cell_key[k] : -1;
+    int hl, len;
+    mb_key_runs(key, hl, len);
+    int base = 0;
+    if (key >= 0 && lane == hl) base = atomicAdd(&d.cell_fill[key], len);
+    base = __shfl(base, hl);
+    if (key >= 0) d.cell_list[d.cell_first[key] + base + (lane - hl)] = k;
+}
+
+// ascending atom order inside each cell (reproducible lists): one wavefront per cell
+__global__ void __launch_bounds__(MB_TPB) k_mb_cell_sort(MbDev d, int c0, int c1)
+{
+    const int c = c0 + (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const int lane = threadIdx.x & 63;
+    if (c >= c1 || !d.rflag[d.cell_rid[c]] || d.flags[1]) return;
+    const int cnt = d.cell_count[c];
+    if (cnt <= 1) return;
+    int* a = d.cell_list + d.cell_first[c];
+    if (cnt <= 64) {
+        const int v = lane < cnt ? a[lane] : INT_MAX;
+        int rank = 0;
+        for (int j = 0; j < cnt; j++) rank += __shfl(v, j) < v ? 1 : 0;
+        if (lane < cnt) a[rank] = v;
+    } else if (lane == 0) {
+        for (int i = 1; i < cnt; i++) {
+            const int v = a[i];
+            int j = i - 1;
+            while (j >= 0 && a[j] > v) { a[j + 1] = a[j]; j--; }
+            a[j + 1] = v;
+        }
+    }
+}
+
+__device__ __forceinline__ float mb_min_image(float d, float L, float iL) { return fmaf(-L, rintf(d * iL), d); }
+
+// skin list of one atom: every distinct cell within one cell of its own (per
+// axis: three offsets when the replica has three or more cells, else every
+// cell once), FP32 minimum-image distances; chunked column-major output
+// (word j of atom k at nlist[4 * ((j / 4) * n + k) + j % 4], unused words of
+// the last chunk hold k itself, which gives r = 0 and no force)
+__global__ void __launch_bounds__(MB_TPB) k_mb_neigh(MbDev d, MbPar p, int a0, int a1)
+{
+    const int k = a0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= a1) return;
+    const int r = d.rid[k];
+    if (!d.rflag[r]) return;
+    if (d.flags[1]) { d.ncount[k] = 0; return; }
+    const MbRep& rp = d.rep[r];
+    const float4 pk = d.posf[k];
+    const int key = d.cell_key[k] - rp.cell0;
+    const int hx = key % rp.nc[0], hy = (key / rp.nc[0]) % rp.nc[1], hz = key / (rp.nc[0] * rp.nc[1]);
+    int lo[3], hi[3];
+    for (int a = 0; a < 3; a++) {
+        if (rp.nc[a] >= 3) { lo[a] = -1; hi[a] = 1; }
+        else { lo[a] = 0; hi[a] = rp.nc[a] - 1; }
+    }
+    const float Lx = rp.Lf[0], Ly = rp.Lf[1], Lz = rp.Lf[2];
+    const float iLx = rp.iLf[0], iLy = rp.iLf[1], iLz = rp.iLf[2];
+    const int n = d.n, ncap = d.ncap;
+    uint4* __restrict__ nl4 = reinterpret_cast<uint4*>(d.nlist);
+    int cnt = 0;
+    uint32_t b0 = 0, b1 = 0, b2 = 0;
+    for (int oz = lo[2]; oz <= hi[2]; oz++) {
+        int cz = hz + oz; cz = cz < 0 ? cz + rp.nc[2] : (cz >= rp.nc[2] ? cz - rp.nc[2] : cz);
+        for (int oy = lo[1]; oy <= hi[1]; oy++) {
+            int cy = hy + oy; cy = cy < 0 ? cy + rp.nc[1] : (cy >= rp.nc[1] ? cy - rp.nc[1] : cy);
+            for (int ox = lo[0]; ox <= hi[0]; ox++) {
+                int cx = hx + ox; cx = cx < 0 ? cx + rp.nc[0] : (cx >= rp.nc[0] ? cx - rp.nc[0] : cx);
+                const int cc = rp.cell0 + cx + rp.nc[0] * (cy + rp.nc[1] * cz);
+                const int f = d.cell_first[cc], m = d.cell_count[cc];
+                for (int e = 0; e < m; e++) {
+                    const int a = d.cell_list[f + e];
+                    const float4 pa = d.posf[a];
+                    const float dx = mb_min_image(pk.x - pa.x, Lx, iLx);
+                    const float dy = mb_min_image(pk.y - pa.y, Ly, iLy);
+                    const float dz = mb_min_image(pk.z - pa.z, Lz, iLz);
+                    const bool hit = a != k && dx * dx + dy * dy + dz * dz < p.rlist2;
+                    if (hit) {
+                        const uint32_t w = (uint32_t)a;
+                        const int slot = cnt & 3;
+                        b0 = slot == 0 ? w : b0;
+                        b1 = slot == 1 ? w : b1;
+                        b2 = slot == 2 ? w : b2;
+                        if (slot == 3 && cnt < ncap) nl4[(size_t)(cnt >> 2) * n + k] = make_uint4(b0, b1, b2, w);
+                        cnt++;
+                    }
+                }
+            }
+        }
+    }
+    const int slot = cnt & 3;
+    if (slot && cnt < ncap) {
+        const uint32_t pad = (uint32_t)k;
+        nl4[(size_t)(cnt >> 2) * n + k] = make_uint4(b0, slot > 1 ? b1 : pad, slot > 2 ? b2 : pad, pad);
+    }
+    d.ncount[k] = cnt;
+    d.posref[k] = d.pos[k];
+    if (cnt > ncap) { atomicOr(&d.flags[0], 1u); atomicMax(d.nreq, cnt); }
+}
+
+__global__ void k_mb_clear_rflags(MbDev d, int r0, int r1)
+{
+    const int r = r0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < r1) d.rflag[r] = 0u;
+}
+
+// pair forces over the chunked list: FP32 minimum image and table
+// interpolation, FP64 accumulation; four gathers in flight per chunk
+template <bool WANT_E>
+__global__ void __launch_bounds__(512) k_mb_force(MbDev d, MbPar p, int a0, int a1)
+{
+    extern __shared__ float2 stab[];
+    const int T = d.T;
+    if (!WANT_E) {
+        for (int i = threadIdx.x; i < T * T * MT_NPTS; i += blockDim.x) stab[i] = d.nbcoef2[i];
+        __syncthreads();
+    }
+    const int k = a0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= a1) return;
+    const MbRep& rp = d.rep[d.rid[k]];
+    const float Lx = rp.Lf[0], Ly = rp.Lf[1], Lz = rp.Lf[2];
+    const float iLx = rp.iLf[0], iLy = rp.iLf[1], iLz = rp.iLf[2];
+    const float4 pk = d.posf[k];
+    const int rowoff = __float_as_int(pk.w) * T * MT_NPTS;
+    double Fx = 0.0, Fy = 0.0, Fz = 0.0;
+    float U = 0.0f;
+    const int nch = (min(d.ncount[k], d.ncap) + 3) >> 2;
+    typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+    const u32x4* __restrict__ nl4 = reinterpret_cast<const u32x4*>(d.nlist) + k;
+    const size_t stride = (size_t)d.n;
+    for (int q = 0; q < nch; q++) {
+        const u32x4 w = __builtin_nontemporal_load(nl4 + (size_t)q * stride);
+        float4 a4[4] = {d.posf[w.x], d.posf[w.y], d.posf[w.z], d.posf[w.w]};
+        asm volatile("" : "+v"(a4[0].w), "+v"(a4[1].w), "+v"(a4[2].w), "+v"(a4[3].w));
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const float4 pa = a4[j];
+            const float dx = mb_min_image(pk.x - pa.x, Lx, iLx);
+            const float dy = mb_min_image(pk.y - pa.y, Ly, iLy);
+            const float dz = mb_min_image(pk.z - pa.z, Lz, iLz);
+            const float r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 < p.rc2 && r2 >= 1.0e-6f) {
+                const float rinv = __builtin_amdgcn_rsqf(r2);
+                const float r = r2 * rinv;
+                const int idx = (int)((r - p.x0) * p.inv_delta);
+                if (idx >= 0 && idx < MT_NPTS - 1) {
+                    const int ci = rowoff + __float_as_int(pa.w) * MT_NPTS + idx;
+                    float fr;
+                    if (WANT_E) {
+                        const float4 c = d.nbcoef[ci];
+                        fr = c.x + c.y * rinv;
+                        U += 0.5f * (c.z * r + c.w);
+                    } else {
+                        const float2 c = stab[ci];
+                        fr = c.x + c.y * rinv;
+                    }
+                    Fx += (double)(dx * fr);
+                    Fy += (double)(dy * fr);
+                    Fz += (double)(dz * fr);
+                }
+            }
+        }
+    }
+    d.frc[k] = make_double4(Fx, Fy, Fz, 0.0);
+    if (WANT_E) d.epot[k] = U;
+}
+
+// force cap (cutForces) and half kick
+__global__ void __launch_bounds__(MB_TPB) k_mb_cap_kick(MbDev d, MbPar p, int a0, int a1)
+{
+    const int k = a0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= a1) return;
+    double4 F = d.frc[k], pm = d.mom[k];
+    if (p.Fcut > 0) {
+        if (fabs(F.x) > p.Fcut) F.x = ((F.x >= 0.0) ? 1.0 : -1.0) * p.Fcut;
+        if (fabs(F.y) > p.Fcut) F.y = ((F.y >= 0.0) ? 1.0 : -1.0) * p.Fcut;
+        if (fabs(F.z) > p.Fcut) F.z = ((F.z >= 0.0) ? 1.0 : -1.0) * p.Fcut;
+        d.frc[k] = F;
+    }
+    const double dt_2 = p.dt / 2.0;
+    pm.x += F.x * dt_2; pm.y += F.y * dt_2; pm.z += F.z * dt_2;
+    d.mom[k] = pm;
+}
+
+// thermostat scale and PBC wrap into the replica's box
+__global__ void __launch_bounds__(MB_TPB) k_mb_scale_wrap(MbDev d, MbPar p, int a0, int a1)
+{
+    const int k = a0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= a1) return;
+    const int r = d.rid[k];
+    const MbRep& rp = d.rep[r];
+    double4 x = d.pos[k];
+    double* xs[3] = {&x.x, &x.y, &x.z};
+    for (int a = 0; a < 3; a++) {
+        const double h = rp.L[a] / 2.0;
+        if (*xs[a] >= h || *xs[a] < -h) *xs[a] -= rp.L[a] * floor(*xs[a] / rp.L[a] + 0.5);
+    }
+    d.pos[k] = x;
+    if (p.nvt) {
+        const double S = d.nh[(size_t)r * (3 * d.M + 2) + 3 * d.M];
+        double4 pm = d.mom[k];
+        pm.x *= S; pm.y *= S; pm.z *= S;
+        d.mom[k] = pm;
+    }
+}
+
+__global__ void k_mb_fill_u32(unsigned int* a, unsigned int v, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+namespace {
+
+thread_local std::string g_batch_err;
+
+struct DevMem {
+    std::vector<void*> ptrs;
+    template <class T> T* alloc(size_t n)
+    {
+        void* p = nullptr;
+        if (hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess) return nullptr;
+        ptrs.push_back(p);
+        return (T*)p;
+    }
+    ~DevMem() { for (void* p : ptrs) (void)hipFree(p); }
+};
+
+} // namespace
+
+std::vector<float4> mtmd_host_table(int which, int kind, const double* prm, const float* profile, float x0, float x1);
+void mtmd_host_segment(const float4* t, int i, float& af, float& bf, float& ae, float& be);
+
+struct mtmd_batch {
+    DevMem mem;
+    MbDev d{};
+    MbPar p{};
+    hipStream_t stream = nullptr;
+    std::vector<MbRep> rep;
+    int nrep = 0, n = 0;
+    int force_block = 256, list_block = 128;
+    bool have_forces = false, ke_valid = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    int64_t steps = 0;
+    std::string err;
+    int fail(int code, const char* fmt, ...)
+    {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof buf, fmt, ap);
+        va_end(ap);
+        err = buf;
+        return code;
+    }
+    ~mtmd_batch()
+    {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+#define MB_HIP(b, call)                                                                                       \
+    do {                                                                                                      \
+        hipError_t e_ = (call);                                                                               \
+        if (e_ != hipSuccess) return (b)->fail(MTMD_EHIP, "%s: %s", #call, hipGetErrorString(e_));            \
+    } while (0)
+
+// launches of one step piece over replicas [r0, r1) (all replicas: batched)
+static void mb_rebuild(mtmd_batch* b, int r0, int r1, hipStream_t s)
+{
+    const MbDev& d = b->d;
+    const int a0 = b->rep[r0].atom0, a1 = b->rep[r1 - 1].atom0 + b->rep[r1 - 1].n;
+    const int c0 = b->rep[r0].cell0, c1 = b->rep[r1 - 1].cell0 + b->rep[r1 - 1].ncell;
+    const int na = a1 - a0, nc = c1 - c0;
+    hipLaunchKernelGGL(k_mb_cells_clear, dim3(mb_blk(nc, MB_TPB)), dim3(MB_TPB), 0, s, d, c0, c1);
+    hipLaunchKernelGGL(k_mb_cell_keys, dim3(mb_blk(na, MB_TPB)), dim3(MB_TPB), 0, s, d, a0, a1);
+    hipLaunchKernelGGL(k_mb_scan, dim3(r1 - r0), dim3(MB_TPB), 0, s, d, r0);
+    hipLaunchKernelGGL(k_mb_scatter, dim3(mb_blk(na, MB_TPB)), dim3(MB_TPB), 0, s, d, a0, a1);
+    hipLaunchKernelGGL(k_mb_cell_sort, dim3(mb_blk(nc, MB_TPB / 64)), dim3(MB_TPB), 0, s, d, c0, c1);
+    hipLaunchKernelGGL(k_mb_neigh, dim3(mb_blk(na, b->list_block)), dim3(b->list_block), 0, s, d, b->p, a0, a1);
+    hipLaunchKernelGGL(k_mb_clear_rflags, dim3(mb_blk(r1 - r0, 64)), dim3(64), 0, s, d, r0, r1);
+}
+
+static void mb_forces(mtmd_batch* b, int r0, int r1, bool want_e, hipStream_t s)
+{
+    const MbDev& d = b->d;
+    const int a0 = b->rep[r0].atom0, a1 = b->rep[r1 - 1].atom0 + b->rep[r1 - 1].n;
+    const int bs = b->force_block;
+    if (want_e)
+        hipLaunchKernelGGL(k_mb_force<true>, dim3(mb_blk(a1 - a0, bs)), dim3(bs), 0, s, d, b->p, a0, a1);
+    else
+        hipLaunchKernelGGL(k_mb_force<false>, dim3(mb_blk(a1 - a0, bs)), dim3(bs),
+                           (size_t)d.T * d.T * MT_NPTS * sizeof(float2), s, d, b->p, a0, a1);
+}
+
+// one MD step of replicas [r0, r1) (MDLoop.cpp:124-138 for each replica)
+static void mb_step(mtmd_batch* b, int r0, int r1, bool want_e, hipStream_t s)
+{
+    const MbDev& d = b->d;
+    const int a0 = b->rep[r0].atom0, a1 = b->rep[r1 - 1].atom0 + b->rep[r1 - 1].n, na = a1 - a0;
+    const int nr = r1 - r0;
+    if (b->p.nvt) hipLaunchKernelGGL(k_mb_nh, dim3(mb_blk(nr, 64)), dim3(64), 0, s, d, b->p, r0, r1);
+    hipLaunchKernelGGL(k_mb_kick_drift, dim3(mb_blk(na, MB_TPB)), dim3(MB_TPB), 0, s, d, b->p, a0, a1);
+    mb_rebuild(b, r0, r1, s);
+    mb_forces(b, r0, r1, want_e, s);
+    hipLaunchKernelGGL(k_mb_cap_kick, dim3(mb_blk(na, MB_TPB)), dim3(MB_TPB), 0, s, d, b->p, a0, a1);
+    if (b->p.nvt) {
+        hipLaunchKernelGGL(k_mb_reduce, dim3(nr), dim3(MB_TPB), 0, s, d, r0, 0);
+        hipLaunchKernelGGL(k_mb_nh, dim3(mb_blk(nr, 64)), dim3(64), 0, s, d, b->p, r0, r1);
+    }
+    hipLaunchKernelGGL(k_mb_scale_wrap, dim3(mb_blk(na, MB_TPB)), dim3(MB_TPB), 0, s, d, b->p, a0, a1);
+    if (b->p.nvt) hipLaunchKernelGGL(k_mb_reduce, dim3(nr), dim3(MB_TPB), 0, s, d, r0, 0);
+}
+
+static int mb_check(mtmd_batch* b)
+{
+    unsigned int fl[2];
+    MB_HIP(b, hipMemcpyAsync(fl, b->d.flags, sizeof fl, hipMemcpyDeviceToHost, b->stream));
+    MB_HIP(b, hipStreamSynchronize(b->stream));
+    if (fl[1]) return b->fail(MTMD_ENAN, "non-finite atom position");
+    if (fl[0]) {
+        int need = 0;
+        MB_HIP(b, hipMemcpy(&need, b->d.nreq, sizeof(int), hipMemcpyDeviceToHost));
+        return b->fail(MTMD_ENEIGH, "neighbour list overflow (capacity %d, needed %d)", b->d.ncap, need);
+    }
+    return MTMD_OK;
+}
+
+// initial lists, forces and kinetic energies
+static int mb_prime(mtmd_batch* b)
+{
+    MbDev& d = b->d;
+    hipLaunchKernelGGL(k_mb_fill_u32, dim3(mb_blk(b->nrep, 64)), dim3(64), 0, b->stream, d.rflag, 1u, b->nrep);
+    mb_rebuild(b, 0, b->nrep, b->stream);
+    mb_forces(b, 0, b->nrep, true, b->stream);
+    hipLaunchKernelGGL(k_mb_reduce, dim3(b->nrep), dim3(MB_TPB), 0, b->stream, d, 0, 0);
+    MB_HIP(b, hipGetLastError());
+    int rc = mb_check(b);
+    if (rc) return rc;
+    b->have_forces = true;
+    return MTMD_OK;
+}
+
+extern "C" const char* mtmd_batch_last_error(const mtmd_batch* b)
+{
+    return b ? b->err.c_str() : g_batch_err.c_str();
+}
+
+extern "C" int mtmd_batch_create(int n_systems, const mtmd_system* systems, const mtmd_config* cfgs,
+                                 mtmd_batch** out)
+{
+    auto bad = [&](int code, const char* msg) { g_batch_err = msg; return code; };
+    if (!out || !systems || !cfgs || n_systems < 1) return bad(MTMD_EINVAL, "invalid arguments");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return bad(MTMD_ENODEV, "no HIP device");
+    const mtmd_config& c0 = cfgs[0];
+    for (int i = 0; i < n_systems; i++) {
+        const mtmd_config& c = cfgs[i];
+        if (c.ensemble != c0.ensemble || c.timestep != c0.timestep || c.cutoff_radius != c0.cutoff_radius ||
+            c.skin != c0.skin || c.temperature_nh_len != c0.temperature_nh_len ||
+            c.temperature_respa != c0.temperature_respa || c.cutoff_force != c0.cutoff_force)
+            return bad(MTMD_EINVAL, "replicas must share ensemble, timestep, cutoff, skin, chain length, RESPA "
+                                    "steps and force cap");
+        if (c.temperature_relax != c0.temperature_relax)
+            return bad(MTMD_EINVAL, "replicas must share the thermostat relaxation time");
+    }
+    if (c0.ensemble == MTMD_NPT) return bad(MTMD_EINVAL, "ensemble npt is not supported");
+    if (c0.temperature_nh_len < 0 || c0.temperature_nh_len > MB_MAXM) return bad(MTMD_EINVAL, "chain length 0..8");
+    if (!(c0.skin > 0.0) || !(c0.cutoff_radius > 0.0) || !(c0.timestep > 0.0))
+        return bad(MTMD_EINVAL, "cutoff, skin and time step must be positive");
+
+    auto b = new mtmd_batch();
+    std::unique_ptr<mtmd_batch> guard(b);
+    b->nrep = n_systems;
+    const float rc = (float)c0.cutoff_radius;
+    const float rlist = (float)(c0.cutoff_radius + c0.skin);
+    b->p.dt = c0.timestep / MT_CONV_t;
+    b->p.Fcut = c0.cutoff_force;
+    b->p.rc2 = rc * rc;
+    b->p.rlist2 = rlist * rlist;
+    b->p.half_skin2 = (float)((c0.skin / 2.0) * (c0.skin / 2.0));
+    b->p.x0 = 0.01f;
+    b->p.inv_delta = 1.0f / ((rc - 0.01f) / float(MT_NPTS - 1));
+    b->p.nresp = c0.temperature_respa;
+    {
+        const double w0 = 1.0 / (1.0 - std::pow(2.0, 1.0 / 3.0));   // NHChain.cpp:45-52
+        b->p.w[0] = w0; b->p.w[1] = 1.0 - 2.0 * w0; b->p.w[2] = w0;
+    }
+    b->p.nvt = (c0.ensemble == MTMD_NVT && c0.temperature_nh_len > 0) ? 1 : 0;
+    const int M = b->p.nvt ? c0.temperature_nh_len : 0;
+
+    // the union of the type sets: every replica must define the same type pairs
+    int T = 0;
+    for (int i = 0; i < n_systems; i++) T = std::max(T, systems[i].n_types);
+    if (T < 1) return bad(MTMD_EINVAL, "no atom types");
+    std::vector<float4> nbtab((size_t)T * T * MT_MAXFF * MT_NPTS, make_float4(0, 0, 0, 0));
+    std::vector<int> nbcnt((size_t)T * T, 0);
+    {
+        const mtmd_system& s = systems[0];
+        for (int i = 0; i < s.n_nonbonded; i++) {
+            const auto& e = s.nonbonded[i];
+            if (e.type_a < 0 || e.type_b < 0 || e.type_a >= T || e.type_b >= T)
+                return bad(MTMD_EINVAL, "non-bonded entry: type out of range");
+            if (e.kind == MTMD_FF_PROFILE && (!e.profile || e.profile[0] != 0.01f || e.profile[MT_NPTS - 1] != rc))
+                return bad(MTMD_EINVAL, "non-bonded profile must span 0.01 .. cutoff");
+            auto t = mtmd_host_table(0, e.kind, e.p, e.profile, 0.01f, rc);
+            if (t.empty()) return bad(MTMD_EINVAL, "non-bonded entry: unknown kind");
+            const int pairs[2][2] = {{e.type_a, e.type_b}, {e.type_b, e.type_a}};
+            for (int q = 0; q < (e.type_a == e.type_b ? 1 : 2); q++) {
+                const int pr = pairs[q][0] * T + pairs[q][1];
+                if (nbcnt[pr] >= MT_MAXFF) return bad(MTMD_EINVAL, "more than 5 force fields for a type pair");
+                std::copy(t.begin(), t.end(), nbtab.begin() + ((size_t)pr * MT_MAXFF + nbcnt[pr]) * MT_NPTS);
+                nbcnt[pr]++;
+            }
+        }
+    }
+    std::vector<float4> coef((size_t)T * T * MT_NPTS, make_float4(0, 0, 0, 0));
+    for (int pr = 0; pr < T * T; pr++)
+        for (int ff = 0; ff < nbcnt[pr]; ff++)
+            for (int i = 0; i < MT_NPTS - 1; i++) {
+                float af, bf, ae, be;
+                mtmd_host_segment(&nbtab[((size_t)pr * MT_MAXFF + ff) * MT_NPTS], i, af, bf, ae, be);
+                float4& o = coef[(size_t)pr * MT_NPTS + i];
+                o.x += af; o.y += bf; o.z += ae; o.w += be;
+            }
+    std::vector<float2> coef2(coef.size());
+    for (size_t i = 0; i < coef.size(); i++) coef2[i] = make_float2(coef[i].x, coef[i].y);
+    if (coef2.size() * sizeof(float2) > 65536) return bad(MTMD_EINVAL, "too many atom types for the batch engine");
+
+    // replicas: boxes, cell grids, atoms
+    std::vector<double4> pos, mom;
+    std::vector<float4> posf;
+    std::vector<int> rid, cell_rid;
+    int natoms = 0, ncells = 0;
+    double nexp_max = 0.0;
+    for (int r = 0; r < n_systems; r++) {
+        const mtmd_system& s = systems[r];
+        if (s.n_atoms < 1 || !s.pos || !s.mass || !s.type) return bad(MTMD_EINVAL, "replica without atoms");
+        if (s.n_bonds || s.n_angles || s.n_dihedrals) return bad(MTMD_EINVAL, "the batch engine takes no bonded terms");
+        if (s.n_nonbonded != systems[0].n_nonbonded) return bad(MTMD_EINVAL, "replicas must share the force fields");
+        for (int k = 0; k < s.n_atoms; k++) {
+            if (s.charge && s.charge[k] != 0.0) return bad(MTMD_EINVAL, "the batch engine takes no charges");
+            if (s.mobile && !s.mobile[k]) return bad(MTMD_EINVAL, "the batch engine takes mobile atoms only");
+            if (s.type[k] < 0 || s.type[k] >= T) return bad(MTMD_EINVAL, "atom type out of range");
+            if (!(s.mass[k] > 0.0)) return bad(MTMD_EINVAL, "atom mass must be positive");
+        }
+        MbRep rp{};
+        for (int a = 0; a < 3; a++) {
+            if (!(s.box[a] > 2.0 * rlist)) return bad(MTMD_EBOX, "replica box below twice the list radius");
+            rp.L[a] = s.box[a];
+            rp.Lf[a] = (float)s.box[a];
+            rp.iLf[a] = 1.0f / rp.Lf[a];
+            rp.nc[a] = std::max(1, (int)std::floor(s.box[a] / (double)rlist));
+        }
+        rp.cell0 = ncells;
+        rp.ncell = rp.nc[0] * rp.nc[1] * rp.nc[2];
+        rp.atom0 = natoms;
+        rp.n = s.n_atoms;
+        rp.T = cfgs[r].temperature / MT_CONV_T;
+        rp.g = 3.0 * s.n_atoms;
+        b->rep.push_back(rp);
+        const double V = s.box[0] * s.box[1] * s.box[2];
+        nexp_max = std::max(nexp_max, (double)s.n_atoms / V * 4.0 / 3.0 * M_PI * rlist * rlist * rlist);
+        // momenta: given, or uniform draws scaled to T (SimulationBox.cpp:109-133 in spirit, seeded)
+        std::vector<double> p3(3 * (size_t)s.n_atoms);
+        if (s.mom) {
+            std::copy(s.mom, s.mom + 3 * (size_t)s.n_atoms, p3.begin());
+        } else {
+            uint64_t st = s.seed * 6364136223846793005ull + 1442695040888963407ull;
+            auto u = [&]() {
+                st = st * 6364136223846793005ull + 1442695040888963407ull;
+                return (double)(st >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+            };
+            double e2 = 0.0;
+            for (int k = 0; k < s.n_atoms; k++)
+                for (int a = 0; a < 3; a++) { p3[3 * k + a] = u(); e2 += p3[3 * k + a] * p3[3 * k + a] / s.mass[k]; }
+            const double sc = e2 > 0.0 ? std::sqrt(3.0 * s.n_atoms * rp.T / e2) : 0.0;
+            for (double& v : p3) v *= sc;
+        }
+        for (int k = 0; k < s.n_atoms; k++) {
+            double x[3] = {s.pos[3 * k], s.pos[3 * k + 1], s.pos[3 * k + 2]};
+            for (int a = 0; a < 3; a++) x[a] -= s.box[a] * std::floor(x[a] / s.box[a] + 0.5);
+            pos.push_back(make_double4(x[0], x[1], x[2], 0.0));
+            mom.push_back(make_double4(p3[3 * k], p3[3 * k + 1], p3[3 * k + 2], s.mass[k]));
+            posf.push_back(make_float4((float)x[0], (float)x[1], (float)x[2], mb_bits_as_float(s.type[k])));
+            rid.push_back(r);
+        }
+        for (int c = 0; c < rp.ncell; c++) cell_rid.push_back(r);
+        natoms += s.n_atoms;
+        ncells += rp.ncell;
+    }
+    if (natoms >= (1 << 30)) return bad(MTMD_EINVAL, "too many atoms");
+    b->n = natoms;
+
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
+        return bad(MTMD_EHIP, "stream creation failed");
+    MbDev& d = b->d;
+    d.n = natoms; d.nrep = n_systems; d.ncells = ncells; d.T = T; d.M = M;
+    d.pos = b->mem.alloc<double4>(natoms); d.mom = b->mem.alloc<double4>(natoms);
+    d.frc = b->mem.alloc<double4>(natoms); d.posref = b->mem.alloc<double4>(natoms);
+    d.posf = b->mem.alloc<float4>(natoms); d.epot = b->mem.alloc<float>(natoms); d.rid = b->mem.alloc<int>(natoms);
+    d.cell_key = b->mem.alloc<int>(natoms); d.cell_list = b->mem.alloc<int>(natoms);
+    d.cell_count = b->mem.alloc<int>(ncells); d.cell_first = b->mem.alloc<int>(ncells);
+    d.cell_fill = b->mem.alloc<int>(ncells); d.cell_rid = b->mem.alloc<int>(ncells);
+    d.ncount = b->mem.alloc<int>(natoms); d.nreq = b->mem.alloc<int>(1);
+    d.ncap = ((int)std::ceil(nexp_max * 1.35) + 16 + 3) & ~3;
+    d.nlist = b->mem.alloc<uint32_t>((size_t)d.ncap * natoms);
+    d.nbcoef2 = b->mem.alloc<float2>(coef2.size()); d.nbcoef = b->mem.alloc<float4>(coef.size());
+    d.rep = b->mem.alloc<MbRep>(n_systems);
+    const int stride = 3 * M + 2;
+    d.nh = b->mem.alloc<double>((size_t)n_systems * stride);
+    d.red = b->mem.alloc<double>(3 * (size_t)n_systems);
+    d.rflag = b->mem.alloc<unsigned int>(n_systems);
+    d.flags = b->mem.alloc<unsigned int>(4);
+    if (!d.pos || !d.mom || !d.frc || !d.posref || !d.posf || !d.epot || !d.rid || !d.cell_key || !d.cell_list ||
+        !d.cell_count || !d.cell_first || !d.cell_fill || !d.cell_rid || !d.ncount || !d.nreq || !d.nlist ||
+        !d.nbcoef2 || !d.nbcoef || !d.rep || !d.nh || !d.red || !d.rflag || !d.flags)
+        return bad(MTMD_ENOMEM, "device allocation failed");
+    std::vector<double> nh((size_t)n_systems * stride, 0.0);
+    for (int r = 0; r < n_systems; r++) {
+        // chain at rest, masses of CNHChain (NHChain.cpp:115-143): Q0 = g T tau^2, Qj = T tau^2
+        const double tau = c0.temperature_relax / MT_CONV_t;
+        double* q = nh.data() + (size_t)r * stride + 2 * M;
+        for (int j = 0; j < M; j++) q[j] = (j == 0 ? b->rep[r].g : 1.0) * b->rep[r].T * tau * tau;
+        nh[(size_t)r * stride + 3 * M] = 1.0;
+    }
+    auto up = [&](void* dst, const void* src, size_t bytes) {
+        return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
+    };
+    bool ok = up(d.pos, pos.data(), natoms * sizeof(double4)) && up(d.posref, pos.data(), natoms * sizeof(double4)) &&
+              up(d.mom, mom.data(), natoms * sizeof(double4)) && up(d.posf, posf.data(), natoms * sizeof(float4)) &&
+              up(d.rid, rid.data(), natoms * sizeof(int)) && up(d.cell_rid, cell_rid.data(), ncells * sizeof(int)) &&
+              up(d.nbcoef2, coef2.data(), coef2.size() * sizeof(float2)) &&
+              up(d.nbcoef, coef.data(), coef.size() * sizeof(float4)) &&
+              up(d.rep, b->rep.data(), n_systems * sizeof(MbRep)) && up(d.nh, nh.data(), nh.size() * sizeof(double)) &&
+              hipMemset(d.flags, 0, 4 * sizeof(unsigned int)) == hipSuccess &&
+              hipMemset(d.nreq, 0, sizeof(int)) == hipSuccess &&
+              hipMemset(d.frc, 0, natoms * sizeof(double4)) == hipSuccess &&
+              hipMemset(d.epot, 0, natoms * sizeof(float)) == hipSuccess;
+    if (!ok) return bad(MTMD_EHIP, "upload failed");
+    int rc0 = mb_prime(b);
+    if (rc0) { g_batch_err = b->err; return rc0; }
+    *out = guard.release();
+    return MTMD_OK;
+}
+
+extern "C" void mtmd_batch_destroy(mtmd_batch* b) { delete b; }
+
+// nsteps of every replica; steps before the last are replays of one captured
+// step graph, the last step runs eagerly with energies
+extern "C" int mtmd_batch_step(mtmd_batch* b, int32_t nsteps)
+{
+    if (!b || nsteps < 0) return MTMD_EINVAL;
+    if (nsteps == 0) return MTMD_OK;
+    if (!b->graph_exec) {
+        MB_HIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        mb_step(b, 0, b->nrep, false, b->stream);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(b->stream, &g);
+        if (e != hipSuccess) return b->fail(MTMD_EHIP, "step capture failed: %s", hipGetErrorString(e));
+        b->graph = g;
+        MB_HIP(b, hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0));
+    }
+    for (int t = 0; t < nsteps - 1; t++) MB_HIP(b, hipGraphLaunch(b->graph_exec, b->stream));
+    mb_step(b, 0, b->nrep, true, b->stream);
+    MB_HIP(b, hipGetLastError());
+    b->steps += nsteps;
+    return mb_check(b);
+}
+
+// the same steps with every kernel launched once per replica, in turn
+extern "C" int mtmd_batch_step_separate(mtmd_batch* b, int32_t nsteps)
+{
+    if (!b || nsteps < 0) return MTMD_EINVAL;
+    for (int t = 0; t < nsteps; t++)
+        for (int r = 0; r < b->nrep; r++) mb_step(b, r, r + 1, t == nsteps - 1, b->stream);
+    MB_HIP(b, hipGetLastError());
+    b->steps += nsteps;
+    return mb_check(b);
+}
+
+extern "C" int mtmd_batch_synchronize(mtmd_batch* b)
+{
+    if (!b) return MTMD_EINVAL;
+    MB_HIP(b, hipStreamSynchronize(b->stream));
+    return MTMD_OK;
+}
+
+extern "C" int mtmd_batch_info(const mtmd_batch* b, int32_t* n_systems, int32_t* n_atoms, int32_t* force_block,
+                               int32_t* list_block)
+{
+    if (!b) return MTMD_EINVAL;
+    if (n_systems) *n_systems = b->nrep;
+    if (n_atoms) *n_atoms = b->n;
+    if (force_block) *force_block = b->force_block;
+    if (list_block) *list_block = b->list_block;
+    return MTMD_OK;
+}
+
+// thermo of every replica (out[n_systems]) on the current state and the
+// forces and energies of the last step
+extern "C" int mtmd_batch_get_thermo(mtmd_batch* b, mtmd_thermo* out)
+{
+    if (!b || !out) return MTMD_EINVAL;
+    hipLaunchKernelGGL(k_mb_reduce, dim3(b->nrep), dim3(MB_TPB), 0, b->stream, b->d, 0, 1);
+    MB_HIP(b, hipGetLastError());
+    std::vector<double> red(3 * (size_t)b->nrep);
+    MB_HIP(b, hipMemcpyAsync(red.data(), b->d.red, red.size() * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+    MB_HIP(b, hipStreamSynchronize(b->stream));
+    for (int r = 0; r < b->nrep; r++) {
+        const MbRep& rp = b->rep[r];
+        const double V = rp.L[0] * rp.L[1] * rp.L[2];
+        const double sp = red[3 * r], sf = red[3 * r + 1], su = red[3 * r + 2];
+        mtmd_thermo& t = out[r];
+        std::memset(&t, 0, sizeof t);
+        t.sum_p2m = sp; t.sum_fr = sf;
+        t.T = sp / (3.0 * rp.n) * MT_CONV_T;
+        t.P = (1.0 / (3.0 * V)) * (sp + sf) * MT_CONV_PRESS;
+        t.V = V; t.U = su; t.K = 0.5 * sp; t.Etot = su + 0.5 * sp;
+    }
+    return MTMD_OK;
+}
+
+// state of replica i (layouts [3 * n_i])
+extern "C" int mtmd_batch_get_state(mtmd_batch* b, int32_t i, double* pos, double* mom)
+{
+    if (!b || i < 0 || i >= b->nrep) return MTMD_EINVAL;
+    const MbRep& rp = b->rep[i];
+    std::vector<double4> x(rp.n), p(rp.n);
+    MB_HIP(b, hipMemcpyAsync(x.data(), b->d.pos + rp.atom0, rp.n * sizeof(double4), hipMemcpyDeviceToHost, b->stream));
+    MB_HIP(b, hipMemcpyAsync(p.data(), b->d.mom + rp.atom0, rp.n * sizeof(double4), hipMemcpyDeviceToHost, b->stream));
+    MB_HIP(b, hipStreamSynchronize(b->stream));
+    for (int k = 0; k < rp.n; k++) {
+        if (pos) { pos[3 * k] = x[k].x; pos[3 * k + 1] = x[k].y; pos[3 * k + 2] = x[k].z; }
+        if (mom) { mom[3 * k] = p[k].x; mom[3 * k + 1] = p[k].y; mom[3 * k + 2] = p[k].z; }
+    }
+    return MTMD_OK;
+}
+
+// forces and per-atom energies of replica i from the last evaluation (capped
+// as the integrator used them); layouts [3 * n_i], [n_i]
+extern "C" int mtmd_batch_get_forces(mtmd_batch* b, int32_t i, double* F, float* U)
+{
+    if (!b || i < 0 || i >= b->nrep) return MTMD_EINVAL;
+    const MbRep& rp = b->rep[i];
+    std::vector<double4> f(rp.n);
+    MB_HIP(b, hipMemcpyAsync(f.data(), b->d.frc + rp.atom0, rp.n * sizeof(double4), hipMemcpyDeviceToHost, b->stream));
+    if (U) MB_HIP(b, hipMemcpyAsync(U, b->d.epot + rp.atom0, rp.n * sizeof(float), hipMemcpyDeviceToHost, b->stream));
+    MB_HIP(b, hipStreamSynchronize(b->stream));
+    if (F)
+        for (int k = 0; k < rp.n; k++) { F[3 * k] = f[k].x; F[3 * k + 1] = f[k].y; F[3 * k + 2] = f[k].z; }
+    return MTMD_OK;
+}
+
+// Block-size autotuner: times the pair-force and list-build kernels of the
+// whole batch for each candidate workgroup size (HIP events, `reps` launches
+// each, lists forced to rebuild) and keeps the fastest. Drops the captured
+// step graph, which is re-captured with the new shapes on the next step.
+extern "C" int mtmd_batch_autotune(mtmd_batch* b, int reps, double* force_ms, double* list_ms)
+{
+    if (!b || reps < 1) return MTMD_EINVAL;
+    hipEvent_t e0, e1;
+    MB_HIP(b, hipEventCreate(&e0));
+    MB_HIP(b, hipEventCreate(&e1));
+    auto time_it = [&](auto&& launch, double& ms_out) -> int {
+        launch();   // warm
+        MB_HIP(b, hipEventRecord(e0, b->stream));
+        for (int i = 0; i < reps; i++) launch();
+        MB_HIP(b, hipEventRecord(e1, b->stream));
+        MB_HIP(b, hipEventSynchronize(e1));
+        float ms = 0.f;
+        MB_HIP(b, hipEventElapsedTime(&ms, e0, e1));
+        ms_out = ms / reps;
+        return MTMD_OK;
+    };
+    const int fcand[4] = {64, 128, 256, 512}, lcand[3] = {64, 128, 256};
+    double best = 1e30, t = 0.0;
+    int bestb = b->force_block;
+    for (int bs : fcand) {
+        b->force_block = bs;
+        int rc = time_it([&]() { mb_forces(b, 0, b->nrep, false, b->stream); }, t);
+        if (rc) return rc;
+        if (t < best) { best = t; bestb = bs; }
+    }
+    b->force_block = bestb;
+    if (force_ms) *force_ms = best;
+    best = 1e30;
+    bestb = b->list_block;
+    for (int bs : lcand) {
+        b->list_block = bs;
+        int rc = time_it([&]() {
+            hipLaunchKernelGGL(k_mb_fill_u32, dim3(mb_blk(b->nrep, 64)), dim3(64), 0, b->stream, b->d.rflag, 1u,
+                               b->nrep);
+            mb_rebuild(b, 0, b->nrep, b->stream);
+        }, t);
+        if (rc) return rc;
+        if (t < best) { best = t; bestb = bs; }
+    }
+    b->list_block = bestb;
+    if (list_ms) *list_ms = best;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (b->graph_exec) { (void)hipGraphExecDestroy(b->graph_exec); b->graph_exec = nullptr; }
+    if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
+    // the timed force passes ran without energies: refresh them for the thermo
+    mb_forces(b, 0, b->nrep, true, b->stream);
+    MB_HIP(b, hipGetLastError());
+    return mb_check(b);
+}

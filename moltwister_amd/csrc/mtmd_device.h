@@ -364,3 +364,50 @@ __device__ inline void mt_exact_bonded(const MtDev& d, const MtParams& p, int k,
     }
 #undef MT_POS
 }
+
+// CMt::exp (Cmd/MolTwisterCmdMolDyn/Integrators/Math.cpp:29-38)
+__device__ inline double mt_cmt_exp(double x) { return (x > 1.0) ? 2.71828 * x : exp(x); }
+
+// One call of the Nose-Hoover chain propagator (NHChain.cpp:54-113) with
+// CFctT (MDLoop.cpp:29-60): Suzuki-Yoshida weights w[3], nresp RESPA steps.
+// Every momentum scaling of the chain multiplies all momenta by the same
+// factor, so sum p.p/m after a scaling by c is c^2 times the sum before: the
+// chain runs on the scalar KE2 = sum p.p/m and returns the accumulated factor
+// S, which the next streaming kernel applies to the momenta.
+__device__ inline double mt_nh_propagate(double* eta, double* peta, const double* Q, int M, int nresp, double T,
+                                         double dt, double g, const double* w, double KE2)
+{
+    MT_EXACT_FP
+    double S = 1.0;
+    const double beta = 1.0 / T;
+    if (M > 0 && peta[0] > 15.0) peta[0] = 10.0;
+    if (M <= 0) return S;
+    for (int mu = 0; mu < 3; mu++) {
+        const double Dt = w[mu] * dt / (double)nresp;
+        for (int i = 0; i < nresp; i++) {
+#define MT_G(j) ((j) == 0 ? (KE2 - (g / beta)) : ((peta[(j) - 1] * peta[(j) - 1] / Q[(j) - 1]) - (1.0 / beta)))
+            peta[M - 1] += (Dt / 4.0) * MT_G(M - 1);
+            for (int j = 0; j < M - 1; j++) {
+                double c = mt_cmt_exp(-(Dt / 8.0) * (peta[j + 1] / Q[j + 1]));
+                peta[j] *= c;
+                peta[j] += (Dt / 4.0) * MT_G(j);
+                c = mt_cmt_exp(-(Dt / 8.0) * (peta[j + 1] / Q[j + 1]));
+                peta[j] *= c;
+            }
+            for (int j = 0; j < M; j++) eta[j] += (Dt / 2.0) * (peta[j] / Q[j]);
+            const double c0 = mt_cmt_exp(-((Dt / 2.0) * (peta[0] / Q[0])));
+            S *= c0;
+            KE2 *= c0 * c0;
+            for (int j = M - 2; j >= 0; j--) {
+                double c = mt_cmt_exp(-(Dt / 8.0) * (peta[j + 1] / Q[j + 1]));
+                peta[j] *= c;
+                peta[j] += (Dt / 4.0) * MT_G(j);
+                c = mt_cmt_exp(-(Dt / 8.0) * (peta[j + 1] / Q[j + 1]));
+                peta[j] *= c;
+            }
+            peta[M - 1] += (Dt / 4.0) * MT_G(M - 1);
+#undef MT_G
+        }
+    }
+    return S;
+}

@@ -372,14 +372,9 @@ void mt_cap_kick(const MtDev& d, const MtParams& p, int want_ke, hipStream_t s)
                        want_ke);
 }
 
-// CMt::exp (Cmd/MolTwisterCmdMolDyn/Integrators/Math.cpp:29-38)
-__device__ inline double mt_cmt_exp(double x) { return (x > 1.0) ? 2.71828 * x : exp(x); }
-
-// Nose-Hoover chain propagator (NHChain.cpp:54-113) with CFctT (MDLoop.cpp:29-60).
-// Every momentum scaling of the chain multiplies all momenta by the same
-// factor, so sum p.p/m after a scaling by c is c^2 times the sum before: the
-// chain runs on the scalar KE2 and the accumulated factor is applied once by
-// the next streaming kernel. nh layout: eta[M], p_eta[M], Q[M], scale.
+// Nose-Hoover chain propagator (NHChain.cpp:54-113) with CFctT (MDLoop.cpp:29-60),
+// mt_nh_propagate in mtmd_device.h; one chain, nh layout: eta[M], p_eta[M],
+// Q[M], scale. Sums the block partials of p.p/m first.
 __global__ void k_nh_chain(double* nh, const double* red, int nblocks, int M, int nresp, double T, double dt,
                            double g, double w0, double w1, double w2)
 {
@@ -391,43 +386,8 @@ __global__ void k_nh_chain(double* nh, const double* red, int nblocks, int M, in
     if (threadIdx.x != 0) return;
     double KE2 = 0.0;
     for (int t = 0; t < 64; t++) KE2 += sh[t];
-    double* eta = nh;
-    double* peta = nh + M;
-    const double* Q = nh + 2 * M;
-    double S = 1.0;
-    double beta = 1.0 / T;
     const double w[3] = {w0, w1, w2};
-    if (M > 0 && peta[0] > 15.0) peta[0] = 10.0;
-    if (M > 0) {
-        for (int mu = 0; mu < 3; mu++) {
-            double Dt = w[mu] * dt / (double)nresp;
-            for (int i = 0; i < nresp; i++) {
-#define MT_G(j) ((j) == 0 ? (KE2 - (g / beta)) : ((peta[(j) - 1] * peta[(j) - 1] / Q[(j) - 1]) - (1.0 / beta)))
-                peta[M - 1] += (Dt / 4.0) * MT_G(M - 1);
-                for (int j = 0; j < M - 1; j++) {
-                    double c = mt_cmt_exp(-(Dt / 8.0) * (peta[j + 1] / Q[j + 1]));
-                    peta[j] *= c;
-                    peta[j] += (Dt / 4.0) * MT_G(j);
-                    c = mt_cmt_exp(-(Dt / 8.0) * (peta[j + 1] / Q[j + 1]));
-                    peta[j] *= c;
-                }
-                for (int j = 0; j < M; j++) eta[j] += (Dt / 2.0) * (peta[j] / Q[j]);
-                double c0 = mt_cmt_exp(-((Dt / 2.0) * (peta[0] / Q[0])));
-                S *= c0;
-                KE2 *= c0 * c0;
-                for (int j = M - 2; j >= 0; j--) {
-                    double c = mt_cmt_exp(-(Dt / 8.0) * (peta[j + 1] / Q[j + 1]));
-                    peta[j] *= c;
-                    peta[j] += (Dt / 4.0) * MT_G(j);
-                    c = mt_cmt_exp(-(Dt / 8.0) * (peta[j + 1] / Q[j + 1]));
-                    peta[j] *= c;
-                }
-                peta[M - 1] += (Dt / 4.0) * MT_G(M - 1);
-#undef MT_G
-            }
-        }
-    }
-    nh[3 * M] = S;
+    nh[3 * M] = mt_nh_propagate(nh, nh + M, nh + 2 * M, M, nresp, T, dt, g, w, KE2);
 }
 
 void mt_nh_chain(const MtDev& d, int M, int nresp, double T, double dt, double g, const double* w, int nblocks,

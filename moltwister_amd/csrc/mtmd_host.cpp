@@ -56,6 +56,7 @@ double urand(uint64_t& s) { return double(splitmix64(s) >> 11) * (1.0 / 90071992
 
 } // namespace
 
+
 struct mtmd_handle {
     int n = 0, T = 0;
     mtmd_config cfg{};
@@ -267,6 +268,20 @@ void segment_coeffs(const float4* t, int i, float& af, float& bf, float& ae, flo
 }
 
 } // namespace
+
+// table builders shared with the batch engine (mtmd_batch.hip): a profile
+// from parameters or from the caller's points; empty for an unknown kind
+std::vector<float4> mtmd_host_table(int which, int kind, const double* prm, const float* profile, float x0, float x1)
+{
+    if (kind == MTMD_FF_PROFILE) return profile ? table_from_profile(profile) : std::vector<float4>();
+    if (mtmd_profile(which, kind, prm, x0, x1, 2, nullptr, nullptr, nullptr) != MTMD_OK) return {};
+    return make_table(which, kind, prm, x0, x1);
+}
+
+void mtmd_host_segment(const float4* t, int i, float& af, float& bf, float& ae, float& be)
+{
+    segment_coeffs(t, i, af, bf, ae, be);
+}
 
 static int setup(mtmd_handle* h, const mtmd_system* s)
 {

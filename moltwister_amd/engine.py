@@ -148,3 +148,70 @@ class Engine:
 
     def synchronize(self):
         self._chk(self.lib.mtmd_synchronize(self.h))
+
+
+class BatchEngine:
+    """Many independent systems advanced together (mtmd_batch_*): one
+    captured step graph for all replicas, or per-replica launches."""
+
+    def __init__(self, systems: list, cfgs: list):
+        self.lib = abi.load_library()
+        self.systems = systems
+        self.n_systems = len(systems)
+        self._csys = [s.to_c() for s in systems]
+        arr = (abi.MtmdSystem * self.n_systems)(*self._csys)
+        carr = (abi.MtmdConfig * self.n_systems)(*cfgs)
+        self._keep = (arr, carr)
+        h = C.c_void_p()
+        rc = self.lib.mtmd_batch_create(self.n_systems, arr, carr, C.byref(h))
+        if rc:
+            raise MtmdError(rc, self.lib.mtmd_batch_last_error(None).decode())
+        self.h = h
+
+    def _chk(self, rc):
+        if rc:
+            raise MtmdError(rc, self.lib.mtmd_batch_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.mtmd_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def step(self, n: int = 1):
+        self._chk(self.lib.mtmd_batch_step(self.h, int(n)))
+
+    def step_separate(self, n: int = 1):
+        self._chk(self.lib.mtmd_batch_step_separate(self.h, int(n)))
+
+    def synchronize(self):
+        self._chk(self.lib.mtmd_batch_synchronize(self.h))
+
+    def thermo(self) -> list:
+        t = (abi.MtmdThermo * self.n_systems)()
+        self._chk(self.lib.mtmd_batch_get_thermo(self.h, t))
+        return [{k: getattr(x, k) for k, _ in abi.MtmdThermo._fields_} for x in t]
+
+    def get_state(self, i: int):
+        n = self.systems[i].n_atoms
+        pos, mom = np.zeros((n, 3)), np.zeros((n, 3))
+        self._chk(self.lib.mtmd_batch_get_state(self.h, i, _f64(pos), _f64(mom)))
+        return pos, mom
+
+    def get_forces(self, i: int):
+        n = self.systems[i].n_atoms
+        F, U = np.zeros((n, 3)), np.zeros(n, np.float32)
+        self._chk(self.lib.mtmd_batch_get_forces(self.h, i, _f64(F), _f32(U)))
+        return F, U
+
+    def autotune(self, reps: int = 3) -> dict:
+        f, l = C.c_double(0), C.c_double(0)
+        self._chk(self.lib.mtmd_batch_autotune(self.h, reps, C.byref(f), C.byref(l)))
+        return dict(force_ms=f.value, list_ms=l.value, **self.info())
+
+    def info(self) -> dict:
+        v = [C.c_int32(0) for _ in range(4)]
+        self._chk(self.lib.mtmd_batch_info(self.h, *[C.byref(x) for x in v]))
+        return dict(n_systems=v[0].value, n_atoms=v[1].value, force_block=v[2].value, list_block=v[3].value)

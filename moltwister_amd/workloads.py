@@ -311,3 +311,36 @@ def chain_molecules(n_chain: int = 60, length: int = 6, L: float = 24.0, seed: i
                   angle_ff=[(abi.ANGLE_HARM, (250.0, 114.0))],
                   dihedral_ff=[(abi.DIH_HARM, (5.0, 1, 3))],
                   bonds=np.array(bonds), angles=np.array(angles), dihedrals=np.array(dih), mom=mom, seed=seed)
+
+
+def replica_lj(n_atoms: int, density: float, T_star: float, seed: int, jitter: float = 0.05) -> System:
+    """One LJ replica (epsilon = sigma = m = 1): n_atoms on randomly chosen
+    sites of an FCC lattice filling the cube of side (n/density)^(1/3),
+    jittered by `jitter` lattice constants, Maxwell-Boltzmann momenta at T*."""
+    rng = np.random.default_rng(seed)
+    L = (n_atoms / density) ** (1.0 / 3.0)
+    ncell = int(math.ceil((n_atoms / 4.0) ** (1.0 / 3.0) - 1e-9))
+    a = L / ncell
+    base = np.array([[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5], [0, 0.5, 0.5]], float)
+    g = np.arange(ncell, dtype=float)
+    cells = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    sites = ((cells[:, None, :] + base[None, :, :]).reshape(-1, 3) + 0.25) * a - L / 2.0
+    pick = np.sort(rng.choice(sites.shape[0], n_atoms, replace=False))
+    pos = sites[pick] + rng.uniform(-jitter * a, jitter * a, (n_atoms, 3))
+    mass = np.ones(n_atoms)
+    mom = maxwell_boltzmann(mass, T_star * CONV_T, rng)
+    return System(pos=pos, mass=mass, charge=np.zeros(n_atoms), types=np.zeros(n_atoms, np.int32),
+                  box=(L, L, L), type_names=["Ar"], nonbonded=[(0, 0, abi.NB_LJ, (1.0, 1.0, 1.0))],
+                  mom=mom, seed=seed)
+
+
+def replicas(n: int = 512, seed: int = 7, n_min: int = 256, n_max: int = 4096):
+    """Configuration 4: n independent LJ systems with sizes uniform in
+    [n_min, n_max], densities uniform in [0.6, 1.0] and temperatures T*
+    uniform in [0.6, 1.5] (seeded). Returns (systems, T_star list)."""
+    rng = np.random.default_rng(seed)
+    sizes = rng.integers(n_min, n_max + 1, n)
+    dens = rng.uniform(0.6, 1.0, n)
+    temps = rng.uniform(0.6, 1.5, n)
+    systems = [replica_lj(int(sizes[i]), float(dens[i]), float(temps[i]), seed * 100003 + i) for i in range(n)]
+    return systems, [float(t) for t in temps]

@@ -1,0 +1,149 @@
+"""GPU checks of the batch engine (many independent LJ replicas advanced in
+one captured step graph).
+
+Forces of every replica are checked against an O(N^2) numpy evaluation of the
+same tabulated force (the table of mtmd_profile, linear interpolation as in
+FunctorCalcForce.cpp:457-512) with the minimum image, which also covers boxes
+with fewer than three cells per axis (where the reference search visits cells
+twice); replicas with three or more cells are also checked against the oracle.
+The MIXED arithmetic bar applies: force RMS <= 1e-5 relative.
+"""
+import numpy as np
+import pytest
+
+from moltwister_amd import abi, config, workloads as W
+from moltwister_amd.engine import BatchEngine, Engine, profile
+
+pytestmark = pytest.mark.gpu
+
+
+def lj_cfg(T_star, ensemble=abi.NVT):
+    c = config.defaults()
+    c.cutoff_radius, c.neigh_shell, c.skin = 2.5, 0.3, 0.3
+    c.timestep, c.temperature, c.temperature_relax = 0.5, T_star * 120.272, 20.0
+    c.ensemble = ensemble
+    c.coulomb = abi.COUL_CUTOFF
+    return c
+
+
+def table_forces(s, rc=2.5):
+    x, f, _ = profile(0, abi.NB_LJ, (1.0, 1.0, 1.0), 0.01, float(np.float32(rc)))
+    x, f = x.astype(np.float64), f.astype(np.float64)
+    L = np.array(s.box)
+    d = s.pos[:, None, :] - s.pos[None, :, :]
+    d -= L * np.round(d / L)
+    r = np.sqrt(np.sum(d * d, axis=2))
+    np.fill_diagonal(r, 1e9)
+    delta = (x[-1] - x[0]) / 999.0
+    idx = np.floor((r - x[0]) / delta).astype(np.int64)
+    ok = (r < np.float32(rc)) & (idx >= 0) & (idx < 999)
+    i = np.clip(idx, 0, 998)
+    fr = f[i] + (f[i + 1] - f[i]) * (r - x[i]) / (x[i + 1] - x[i])
+    fr = np.where(ok, fr, 0.0) / r
+    return np.sum(d * fr[:, :, None], axis=1)
+
+
+def rms_rel(F, Fr):
+    return float(np.sqrt(np.sum((F - Fr) ** 2) / np.sum(Fr ** 2)))
+
+
+def small_batch():
+    # two small boxes (2 cells per axis: 6.35 and 6.84 sigma) and two with 4-5 cells
+    specs = [(256, 1.0, 0.7), (300, 0.9, 1.2), (1500, 0.8, 0.9), (2200, 0.65, 1.4)]
+    systems = [W.replica_lj(n, rho, T, seed=21 + i) for i, (n, rho, T) in enumerate(specs)]
+    return systems, [t for _, _, t in specs]
+
+
+def test_batch_forces_vs_brute_force_and_oracle(oracle_mod):
+    print("case batch forces", flush=True)
+    systems, temps = small_batch()
+    b = BatchEngine(systems, [lj_cfg(t) for t in temps])
+    for i, s in enumerate(systems):
+        F, _ = b.get_forces(i)
+        Fr = table_forces(s)
+        assert rms_rel(F, Fr) < 1e-5, (i, rms_rel(F, Fr))
+        if s.box[0] >= 3 * 2.8:
+            o = oracle_mod.Oracle(s, lj_cfg(temps[i]))
+            Fo, _ = o.compute_forces()
+            assert rms_rel(F, Fo) < 1e-5, (i, rms_rel(F, Fo))
+    b.close()
+
+
+def test_batch_graph_equals_separate_launches():
+    """The captured whole-batch step and per-replica launches run the same
+    kernels on the same data: identical states."""
+    print("case batch graph vs separate", flush=True)
+    systems, temps = small_batch()
+    cfgs = [lj_cfg(t) for t in temps]
+    a, b = BatchEngine(systems, cfgs), BatchEngine(systems, cfgs)
+    a.step(40)
+    b.step_separate(40)
+    for i in range(len(systems)):
+        pa, ma = a.get_state(i)
+        pb, mb = b.get_state(i)
+        assert np.array_equal(pa, pb) and np.array_equal(ma, mb), i
+    ta, tb = a.thermo(), b.thermo()
+    for x, y in zip(ta, tb):
+        assert x["Etot"] == y["Etot"] and x["T"] == y["T"]
+    a.close()
+    b.close()
+
+
+def test_batch_nvt_replica_matches_single_system_engine():
+    """A replica of the batch follows the same NVT trajectory as the MIXED
+    single-system engine on the same system (same chain at rest, same
+    momenta), up to FP32 pair rounding, over 200 steps."""
+    print("case batch nvt vs single", flush=True)
+    systems, temps = small_batch()
+    i = 2
+    b = BatchEngine(systems, [lj_cfg(t) for t in temps])
+    c = lj_cfg(temps[i])
+    c.engine = abi.ENGINE_MIXED
+    e = Engine(systems[i], c)
+    for _ in range(4):
+        b.step(50)
+        e.step(50)
+        tb, te = b.thermo()[i], e.thermo()
+        assert abs(tb["T"] - te["T"]) < 2e-3 * te["T"], (tb["T"], te["T"])
+        assert abs(tb["Etot"] - te["Etot"]) < 2e-3 * abs(te["Etot"]), (tb["Etot"], te["Etot"])
+    b.close()
+
+
+def test_batch_nvt_holds_each_replica_temperature():
+    print("case batch temperatures", flush=True)
+    systems, temps = W.replicas(12, seed=3, n_min=256, n_max=1200)
+    b = BatchEngine(systems, [lj_cfg(t) for t in temps])
+    b.step(600)
+    for t, th in zip(temps, b.thermo()):
+        assert np.isfinite(th["Etot"])
+        assert abs(th["T"] / 120.272 - t) < 0.35 * t, (th["T"] / 120.272, t)
+    b.close()
+
+
+def test_batch_autotune_and_nve_energy():
+    """After the autotuner, an NVE batch started from thermalised states keeps
+    every replica's total energy (truncation jumps of the unshifted cutoff
+    allowed for)."""
+    print("case batch autotune + nve", flush=True)
+    systems, temps = small_batch()
+    warm = BatchEngine(systems, [lj_cfg(t) for t in temps])
+    warm.step(400)
+    for i, s in enumerate(systems):
+        s.pos, s.mom = warm.get_state(i)
+    warm.close()
+    b = BatchEngine(systems, [lj_cfg(t, abi.NVE) for t in temps])
+    info = b.autotune(2)
+    assert info["force_block"] in (64, 128, 256, 512) and info["list_block"] in (64, 128, 256)
+    e0 = [t["Etot"] for t in b.thermo()]
+    b.step(200)
+    e1 = [t["Etot"] for t in b.thermo()]
+    for x, y in zip(e0, e1):
+        assert abs(y - x) < 5e-3 * abs(x), (x, y)
+    b.close()
+
+
+def test_batch_rejects_charges():
+    systems, temps = small_batch()
+    systems[1].charge[:] = 0.1
+    with pytest.raises(Exception, match="charges"):
+        BatchEngine(systems, [lj_cfg(t) for t in temps])

@@ -113,6 +113,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-water", action="store_true")
+    ap.add_argument("--skip-batch", action="store_true")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--kernels-only", action="store_true",
                     help="short warm-up, then only the kernel timings (force pass, list build, copy) of the "
@@ -240,6 +241,46 @@ def main() -> int:
                                                                     water_F[abi.ENGINE_FP64])
         log(f"bench: water {water}")
 
+    batch = {}
+    if not args.skip_batch:
+        from moltwister_amd.engine import BatchEngine
+        t0 = time.perf_counter()
+        systems, temps = W.replicas(512, seed=7)
+        cfgs = []
+        for t in temps:
+            cb = lj_config(abi.ENGINE_MIXED, abi.NVT)
+            cb.temperature = t * 120.272
+            cfgs.append(cb)
+        be = BatchEngine(systems, cfgs)
+        tune = be.autotune(3)
+        be.step(200)                                  # capture + warm-up
+        Kb = max(100, K // 4)
+        be.synchronize()
+        t1 = time.perf_counter()
+        be.step(Kb)
+        be.synchronize()
+        dt_b = time.perf_counter() - t1
+        Ks = 4
+        t1 = time.perf_counter()
+        be.step_separate(Ks)
+        be.synchronize()
+        dt_s = time.perf_counter() - t1
+        th = be.thermo()
+        natoms = sum(x.n_atoms for x in systems)
+        batch = {"workload": "512 LJ replicas, N uniform in [256, 4096], rho* in [0.6, 1.0], T* in [0.6, 1.5], "
+                             "NH NVT, rc 2.5, skin 0.3, dt 0.005 tau",
+                 "atoms": natoms, "steps": Kb,
+                 "replica_steps_per_s": 512 * Kb / dt_b, "ms_per_step": dt_b / Kb * 1e3,
+                 "atom_steps_per_s": natoms * Kb / dt_b,
+                 "separate_launches": {"steps": Ks, "replica_steps_per_s": 512 * Ks / dt_s,
+                                       "ms_per_step": dt_s / Ks * 1e3},
+                 "graph_speedup": (512 * Kb / dt_b) / (512 * Ks / dt_s),
+                 "autotune": tune,
+                 "T_err_rel_mean": float(np.mean([abs(x["T"] / 120.272 - t) / t for x, t in zip(th, temps)])),
+                 "setup_s": t1 - t0}
+        be.close()
+        log(f"bench: batch {batch}")
+
     cpu = None
     if not args.skip_cpu:
         from oracle import pyoracle
@@ -313,6 +354,7 @@ def main() -> int:
                           "rebuild_every_steps": rebuild_every, "list_build_ms": t_build * 1e3,
                           "copy_GBps": copy_bw / 1e9},
         "water_30k": water,
+        "replicas_512": batch,
     }
     print(json.dumps(out))
     return 0

@@ -114,6 +114,8 @@ def main() -> int:
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-water", action="store_true")
     ap.add_argument("--skip-batch", action="store_true")
+    ap.add_argument("--skip-fm", action="store_true")
+    ap.add_argument("--fm-frames", type=int, default=2048)
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--kernels-only", action="store_true",
                     help="short warm-up, then only the kernel timings (force pass, list build, copy) of the "
@@ -281,6 +283,51 @@ def main() -> int:
         be.close()
         log(f"bench: batch {batch}")
 
+    fmres = {}
+    if not args.skip_fm:
+        from moltwister_amd.forcematch import ForceMatch, params_of, fit
+        t0 = time.perf_counter()
+        n_frames = args.fm_frames
+        sysF, frames = W.water_frames(1000, n_frames, 100, seed=4,
+                                      progress=lambda f: log(f"bench: fm frames {f}/{n_frames} "
+                                                             f"({time.perf_counter() - t0:.1f}s)"))
+        t_gen = time.perf_counter() - t0
+        fmh = ForceMatch(sysF, 10.0, frames)
+        p0 = params_of(sysF)
+        truth = p0 * np.random.default_rng(11).uniform(0.9, 1.1, p0.shape)
+        fmh.target_from_params(truth)
+        fmh.loss_grad(p0)                                   # warm
+        reps = 5
+        t1 = time.perf_counter()
+        for _ in range(reps):
+            L32, g32 = fmh.loss_grad(p0, fp64=False)
+        t_32 = (time.perf_counter() - t1) / reps
+        t1 = time.perf_counter()
+        L64, g64 = fmh.loss_grad(p0, fp64=True)
+        t_64 = time.perf_counter() - t1
+        fd_err = 0.0
+        for k in range(len(p0)):
+            h = 1e-6 * abs(p0[k])
+            pp, pm = p0.copy(), p0.copy()
+            pp[k] += h
+            pm[k] -= h
+            fd = (fmh.loss_grad(pp, fp64=True)[0] - fmh.loss_grad(pm, fp64=True)[0]) / (2 * h)
+            fd_err = max(fd_err, abs(g64[k] - fd) / abs(fd))
+        t1 = time.perf_counter()
+        theta, hist = fit(fmh, p0, iters=500, lr=2e-2)
+        t_fit = time.perf_counter() - t1
+        fmres = {"workload": f"1000 waters (3000 atoms), {n_frames} frames every 100 steps of 298 K NVT, "
+                             "targets from parameters scaled by U[0.9, 1.1]",
+                 "frames": n_frames, "frame_gen_s": t_gen,
+                 "loss_grad_ms_fp32": t_32 * 1e3, "loss_grad_ms_fp64": t_64 * 1e3,
+                 "frames_per_s_fp32": n_frames / t_32,
+                 "grad_rel_err_fp32_vs_fp64": float(np.max(np.abs(g32 - g64) / np.abs(g64))),
+                 "grad_rel_err_fp64_vs_fd": fd_err,
+                 "adam_500": {"loss_start": hist[0], "loss_end": hist[-1], "seconds": t_fit,
+                              "param_rel_err": [float(x) for x in np.abs(theta / truth - 1)]}}
+        fmh.close()
+        log(f"bench: force matching {fmres}")
+
     cpu = None
     if not args.skip_cpu:
         from oracle import pyoracle
@@ -355,6 +402,7 @@ def main() -> int:
                           "copy_GBps": copy_bw / 1e9},
         "water_30k": water,
         "replicas_512": batch,
+        "force_matching": fmres,
     }
     print(json.dumps(out))
     return 0

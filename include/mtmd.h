@@ -330,6 +330,35 @@ int mtmd_batch_autotune(mtmd_batch* b, int reps, double* force_ms, double* list_
 int mtmd_batch_info(const mtmd_batch* b, int32_t* n_systems, int32_t* n_atoms, int32_t* force_block,
                     int32_t* list_block);
 
+/* ---- force matching ----
+ * No counterpart in the reference (it has no gradients). Loss
+ *   L = sum_frames sum_i |F_i(theta) - F_i^ref|^2 / (3 N n_frames)
+ * and dL/dtheta by a hand-written adjoint, for the analytic forms of the
+ * reference's MDFF classes: LJ 4 eps [(s/r)^12 - (s/r)^6] between atoms of
+ * different molecules within `cutoff` (MolTwisterMDFFNonBonded_LJ.cpp:46-60),
+ * harmonic bonds k (r - r0)^2 (MolTwisterMDFFBond_Harm.cpp:43-51) and angles
+ * k (theta - theta0)^2 (MolTwisterMDFFAngle_Harm.cpp:47-55), minimum image.
+ * The system gives topology, types and box (its force-field parameters are
+ * ignored; kinds must be LJ / BOND_HARM / ANGLE_HARM, at most 4 of each).
+ * theta = for each non-bonded entry (eps, sigma), for each bond type (k, r0),
+ * for each angle type (k, theta0 in degrees). frames and target are
+ * [n_frames][3 n] and stay in device memory; pairs are searched again in
+ * every pass (no stored lists). fp64 = 0: FP32 pair terms with FP64
+ * accumulation; fp64 = 1: FP64 throughout. */
+typedef struct mtmd_fm mtmd_fm;
+
+int  mtmd_fm_create(const mtmd_system* sys, double cutoff, int n_frames, const double* frames,
+                    const double* target, mtmd_fm** out);
+void mtmd_fm_destroy(mtmd_fm* h);
+const char* mtmd_fm_last_error(const mtmd_fm* h);
+int  mtmd_fm_num_params(const mtmd_fm* h);
+int  mtmd_fm_set_target(mtmd_fm* h, const double* target);
+/* target := model forces of every frame at `params` (synthetic references) */
+int  mtmd_fm_target_from_params(mtmd_fm* h, const double* params);
+/* model forces of one frame, [3 n] */
+int  mtmd_fm_forces(mtmd_fm* h, const double* params, int32_t frame, int32_t fp64, double* F);
+int  mtmd_fm_loss_grad(mtmd_fm* h, const double* params, int32_t fp64, double* loss, double* grad);
+
 #ifdef __cplusplus
 }
 #endif

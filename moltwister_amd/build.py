@@ -49,17 +49,18 @@ def build(verbose: bool = False) -> str:
          os.path.join(SRC, "mtmd_exact.hip")],
         [HIPCC, *HIP_FLAGS, "-c", "-o", os.path.join(OUT, "mtmd_fast.o"), os.path.join(SRC, "mtmd_fast.hip")],
         [HIPCC, *HIP_FLAGS, "-c", "-o", os.path.join(OUT, "mtmd_batch.o"), os.path.join(SRC, "mtmd_batch.hip")],
+        [HIPCC, *HIP_FLAGS, "-c", "-o", os.path.join(OUT, "mtmd_fm.o"), os.path.join(SRC, "mtmd_fm.hip")],
         [HIPCC, *HIP_FLAGS, "-x", "hip", "-c", "-o", os.path.join(OUT, "mtmd_host.o"),
          os.path.join(SRC, "mtmd_host.cpp")],
         [shutil.which("g++") or "g++", "-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-c",
          "-o", os.path.join(OUT, "mtmd_tables.o"), os.path.join(SRC, "mtmd_tables.cpp")],
     ]
-    with ThreadPoolExecutor(max_workers=5) as ex:
+    with ThreadPoolExecutor(max_workers=6) as ex:
         for r in ex.map(_run, jobs):
             if verbose and (r.stdout or r.stderr):
                 sys.stderr.write(r.stdout + r.stderr)
-    objs = [os.path.join(OUT, f) for f in ("mtmd_exact.o", "mtmd_fast.o", "mtmd_batch.o", "mtmd_host.o",
-                                           "mtmd_tables.o")]
+    objs = [os.path.join(OUT, f) for f in ("mtmd_exact.o", "mtmd_fast.o", "mtmd_batch.o", "mtmd_fm.o",
+                                           "mtmd_host.o", "mtmd_tables.o")]
     # workload generator helpers (host only)
     _run([shutil.which("gcc") or "gcc", "-O2", "-fPIC", "-shared", "-o", GENLIB, os.path.join(SRC, "mtmd_gen.c"), "-lm"])
     tmp = LIB + ".tmp"

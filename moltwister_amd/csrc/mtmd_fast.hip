@@ -509,8 +509,116 @@ __global__ void __launch_bounds__(256) k_fast_neigh_block(MtDev d, MtParams p, i
     }
 }
 
+// Neighbour list for dense cells (tens to hundreds of atoms per cell, e.g.
+// water with a 10 A cutoff): one 256-thread workgroup per home cell, lanes =
+// home atoms; the 27 neighbour cells are staged in LDS one at a time (in
+// chunks of NBW_CAP atoms) and every lane scans each with broadcast reads.
+// Same output, order and exclusion rules as k_fast_neigh_block.
+#define NBW_CAP 1024
+template <int MODE>
+__global__ void __launch_bounds__(256) k_fast_neigh_wide(MtDev d, MtParams p, int cond)
+{
+    if (mt_skip_rebuild(cond, d.flags, p.half_skin2)) return;
+    __shared__ float4 spos[NBW_CAP];
+    __shared__ int smol_buf[MODE >= 1 ? NBW_CAP : 1];
+    int* smol = smol_buf;
+    const int t = threadIdx.x;
+    const int Nx = d.nc[0], Ny = d.nc[1], Nz = d.nc[2];
+    const int c = mt_xcd_block(blockIdx.x, gridDim.x);
+    if (c >= d.ncells) return;
+    const int hx = c % Nx, hy = (c / Nx) % Ny, hz = c / (Nx * Ny);
+    const int f0 = d.cell_first[c], cnt_c = d.cell_count[c];
+    if (cnt_c == 0) return;
+    if (d.flags[3]) {
+        for (int i = t; i < cnt_c; i += 256) d.ncount[d.cell_list[f0 + i]] = 0;
+        return;
+    }
+    const int n = d.n, ncap = d.ncap;
+    const float rl2 = p.rlist2;
+    uint4* __restrict__ nl4 = reinterpret_cast<uint4*>(d.nlist);
+    for (int ib = 0; ib < cnt_c; ib += 256) {
+        const bool act = ib + t < cnt_c;
+        const int k = act ? d.cell_list[f0 + ib + t] : 0;
+        const float4 pk = act ? d.posf[k] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const int molk = (MODE >= 1 && act) ? d.mol[k] : -1;
+        const bool xk = MODE == 2 && act && d.xmol[d.perm[k]] != 0;
+        int cnt = 0;
+        uint32_t b0 = 0, b1 = 0, b2 = 0;
+        for (int oz = -1; oz <= 1; oz++) {
+            int cz = hz + oz; cz = cz < 0 ? cz + Nz : (cz >= Nz ? cz - Nz : cz);
+            for (int oy = -1; oy <= 1; oy++) {
+                int cy = hy + oy; cy = cy < 0 ? cy + Ny : (cy >= Ny ? cy - Ny : cy);
+                for (int ox = -1; ox <= 1; ox++) {
+                    int cx = hx + ox; cx = cx < 0 ? cx + Nx : (cx >= Nx ? cx - Nx : cx);
+                    const int cc = cx + Nx * (cy + Ny * cz);
+                    const uint32_t code = (uint32_t)((ox + 1) + 3 * (oy + 1) + 9 * (oz + 1)) << MT_NW_IDX_BITS;
+                    const float qx = pk.x - (float)ox * p.cs[0];
+                    const float qy = pk.y - (float)oy * p.cs[1];
+                    const float qz = pk.z - (float)oz * p.cs[2];
+                    const int g0 = d.cell_first[cc], gn = d.cell_count[cc];
+                    for (int jb = 0; jb < gn; jb += NBW_CAP) {
+                        const int m = min(NBW_CAP, gn - jb);
+                        __syncthreads();
+                        for (int e = t; e < m; e += 256) {
+                            const int a = d.cell_list[g0 + jb + e];
+                            const float4 pa = d.posf[a];
+                            spos[e] = make_float4(pa.x, pa.y, pa.z, __uint_as_float((uint32_t)a));
+                            if constexpr (MODE >= 1) smol[e] = d.mol[a];
+                        }
+                        __syncthreads();
+                        if (!act) continue;
+                        int e = 0;
+                        for (; e + 1 < m; e += 2) {
+                            float4 c0 = spos[e], c1 = spos[e + 1];
+                            asm volatile("" : "+v"(c0.w), "+v"(c1.w));
+                            const float dx0 = qx - c0.x, dy0 = qy - c0.y, dz0 = qz - c0.z;
+                            const float dx1 = qx - c1.x, dy1 = qy - c1.y, dz1 = qz - c1.z;
+                            const bool h0 = dx0 * dx0 + dy0 * dy0 + dz0 * dz0 < rl2;
+                            const bool h1 = dx1 * dx1 + dy1 * dy1 + dz1 * dz1 < rl2;
+                            if (h0 && (MODE != 2 || !xk || !mt_is_partner(d, k, (int)__float_as_uint(c0.w))))
+                                mt_neigh_hit<MODE == 2 ? 1 : MODE>(d, nl4, c0, code, k, molk, smol, e, n, ncap, cnt,
+                                                                     b0, b1, b2);
+                            if (h1 && (MODE != 2 || !xk || !mt_is_partner(d, k, (int)__float_as_uint(c1.w))))
+                                mt_neigh_hit<MODE == 2 ? 1 : MODE>(d, nl4, c1, code, k, molk, smol, e + 1, n, ncap,
+                                                                     cnt, b0, b1, b2);
+                        }
+                        if (e < m) {
+                            float4 c0 = spos[e];
+                            asm volatile("" : "+v"(c0.w));
+                            const float dx0 = qx - c0.x, dy0 = qy - c0.y, dz0 = qz - c0.z;
+                            if (dx0 * dx0 + dy0 * dy0 + dz0 * dz0 < rl2 &&
+                                (MODE != 2 || !xk || !mt_is_partner(d, k, (int)__float_as_uint(c0.w))))
+                                mt_neigh_hit<MODE == 2 ? 1 : MODE>(d, nl4, c0, code, k, molk, smol, e, n, ncap, cnt,
+                                                                     b0, b1, b2);
+                        }
+                    }
+                }
+            }
+        }
+        if (act) {
+            const int slot = cnt & 3;
+            if (slot && cnt < ncap) {
+                const uint32_t pad = (uint32_t)k | (NB_PAD_CODE << MT_NW_IDX_BITS);
+                nl4[(size_t)(cnt >> 2) * n + k] = make_uint4(b0, slot > 1 ? b1 : pad, slot > 2 ? b2 : pad, pad);
+            }
+            d.ncount[k] = cnt;
+            if (cnt > ncap) { d.flags[2] = 1; atomicMax(d.nreq, cnt); }
+        }
+    }
+}
+
 void mt_fast_neighbors(const MtDev& d, const MtParams& p, int cond, hipStream_t s)
 {
+    if (d.n > 40 * d.ncells) {
+        // dense cells: one workgroup per home cell
+        if (d.has_xmol)
+            hipLaunchKernelGGL(k_fast_neigh_wide<2>, dim3(d.ncells), dim3(256), 0, s, d, p, cond);
+        else if (d.has_molecules)
+            hipLaunchKernelGGL(k_fast_neigh_wide<1>, dim3(d.ncells), dim3(256), 0, s, d, p, cond);
+        else
+            hipLaunchKernelGGL(k_fast_neigh_wide<0>, dim3(d.ncells), dim3(256), 0, s, d, p, cond);
+        return;
+    }
     const int nb = ((d.nc[0] + 1) / 2) * ((d.nc[1] + 1) / 2) * ((d.nc[2] + 1) / 2);
     if (d.has_xmol)
         hipLaunchKernelGGL(k_fast_neigh_block<2>, dim3(nb), dim3(256), 0, s, d, p, cond);
@@ -548,17 +656,27 @@ __device__ __forceinline__ void mt_gather4(const float4* posf, mt_u32x4 w, float
 
 // one MIXED pair: FP32 geometry on local coordinates, segment coefficients
 // (a_f, b_f[, a_e, b_e]) of the pair's summed tables, FP64 accumulation
+// The pair potentials are truncated at rc without a shift, so a pair within
+// FP32 rounding of rc would be counted or not depending on precision: in a
+// narrow band around rc the inside test is taken in FP64 with the FP64
+// engine's (the reference's) predicate, from the double positions.
 template <bool WANT_E, bool COUL, typename CoefT>
 __device__ __forceinline__ void mt_mixed_pair(uint32_t w, const float4& pa, const float4& pk, const float4* sh,
                                               const CoefT* row, float rc2, float x0, float invd, float qk,
-                                              const float* qf, double& Fx, double& Fy, double& Fz, float& U)
+                                              const float* qf, double& Fx, double& Fy, double& Fz, float& U,
+                                              const MtDev& d, const MtParams& p, int k)
 {
     const float4 o = sh[w >> MT_NW_IDX_BITS];
     const float dx = pk.x - pa.x - o.x;
     const float dy = pk.y - pa.y - o.y;
     const float dz = pk.z - pa.z - o.z;
     const float r2 = dx * dx + dy * dy + dz * dz;
-    if (r2 < rc2 && r2 >= 1.0e-6f) {
+    bool inside = r2 < rc2;
+    if (fabsf(r2 - rc2) < 2.0e-5f * rc2) {
+        const double4 rk4 = d.pos[k], ra4 = d.pos[w & MT_NW_IDX_MASK];
+        inside = mt_dist2_pbc({rk4.x, rk4.y, rk4.z}, {ra4.x, ra4.y, ra4.z}, p.wnl) < (double)p.rc2f;
+    }
+    if (inside && r2 >= 1.0e-6f) {
         const float rinv = __builtin_amdgcn_rsqf(r2);
         const float r = r2 * rinv;
         float fr = 0.0f;
@@ -604,10 +722,10 @@ __device__ __forceinline__ void mt_mixed_atom(const MtDev& d, const MtParams& p,
         // (while the next chunk's load stays in flight), so the type bits in
         // .w are not fetched by a second, dependent load inside a pair branch
         asm volatile("" : "+v"(a0.w), "+v"(a1.w), "+v"(a2.w), "+v"(a3.w));
-        mt_mixed_pair<WANT_E, COUL>(w.x, a0, pk, sh, row, rc2, x0, invd, qk, d.qf, Fx, Fy, Fz, U);
-        mt_mixed_pair<WANT_E, COUL>(w.y, a1, pk, sh, row, rc2, x0, invd, qk, d.qf, Fx, Fy, Fz, U);
-        mt_mixed_pair<WANT_E, COUL>(w.z, a2, pk, sh, row, rc2, x0, invd, qk, d.qf, Fx, Fy, Fz, U);
-        mt_mixed_pair<WANT_E, COUL>(w.w, a3, pk, sh, row, rc2, x0, invd, qk, d.qf, Fx, Fy, Fz, U);
+        mt_mixed_pair<WANT_E, COUL>(w.x, a0, pk, sh, row, rc2, x0, invd, qk, d.qf, Fx, Fy, Fz, U, d, p, k);
+        mt_mixed_pair<WANT_E, COUL>(w.y, a1, pk, sh, row, rc2, x0, invd, qk, d.qf, Fx, Fy, Fz, U, d, p, k);
+        mt_mixed_pair<WANT_E, COUL>(w.z, a2, pk, sh, row, rc2, x0, invd, qk, d.qf, Fx, Fy, Fz, U, d, p, k);
+        mt_mixed_pair<WANT_E, COUL>(w.w, a3, pk, sh, row, rc2, x0, invd, qk, d.qf, Fx, Fy, Fz, U, d, p, k);
     }
 }
 

@@ -344,3 +344,30 @@ def replicas(n: int = 512, seed: int = 7, n_min: int = 256, n_max: int = 4096):
     temps = rng.uniform(0.6, 1.5, n)
     systems = [replica_lj(int(sizes[i]), float(dens[i]), float(temps[i]), seed * 100003 + i) for i in range(n)]
     return systems, [float(t) for t in temps]
+
+
+def water_frames(n_mol: int, n_frames: int, every: int, seed: int = 2, T_kelvin: float = 298.0,
+                 cutoff: float = 10.0, skin: float = 0.3, relax_steps: int = 300, equil_steps: int = 200,
+                 progress=None):
+    """Configuration 5 frames: a flexible-water box (configuration 2 force
+    field) relaxed by gradient descent, then Nose-Hoover NVT with all-pairs
+    Coulomb on the MIXED engine, sampled every `every` steps after
+    `equil_steps`. Returns (system, frames[n_frames, 3 n_atoms])."""
+    from .engine import Engine
+    from . import config
+    s = water_box(n_mol, seed=seed)
+    c = config.defaults()
+    c.engine, c.coulomb = abi.ENGINE_MIXED, abi.COUL_ALLPAIRS
+    c.cutoff_radius, c.neigh_shell, c.skin = cutoff, skin, skin
+    c.timestep, c.temperature, c.temperature_relax, c.ensemble = 0.5, T_kelvin, 20.0, abi.NVT
+    e = Engine(s, c)
+    e.optimize(relax_steps, 1e-4, 0.0)
+    e.step(equil_steps)
+    frames = np.zeros((n_frames, s.n_atoms * 3))
+    for f in range(n_frames):
+        e.step(every)
+        frames[f] = e.get_state()[0].reshape(-1)
+        if progress and (f + 1) % 128 == 0:
+            progress(f + 1)
+    e.close()
+    return s, frames

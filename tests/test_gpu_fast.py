@@ -218,3 +218,28 @@ def test_profile_kind_matches_parameters(engine):
     a = Engine(s, c).compute_forces()
     b = Engine(as_profiles(s, c.cutoff_radius), c).compute_forces()
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+@pytest.mark.parametrize("coul", [abi.COUL_ALLPAIRS, abi.COUL_CUTOFF])
+@pytest.mark.parametrize("engine", [abi.ENGINE_FP64, abi.ENGINE_MIXED])
+def test_dense_cells_water_1000(oracle_mod, engine, coul):
+    """1000 waters with a 10 A cutoff: ~110 atoms per cell, the dense-cell
+    list builder (one workgroup per home cell)."""
+    from moltwister_amd import workloads as W
+    from systems import cfg
+    print(f"case dense water engine={engine} coul={coul}", flush=True)
+    s = W.water_box(1000, seed=6)
+    c = cfg(10.0, 0.3, engine=engine, coulomb=coul, skin=0.3)
+    co = cfg(10.0, 0.3, coulomb=coul)
+    pos = s.pos + np.random.default_rng(7).normal(0, 0.02, s.pos.shape)
+    o, e = oracle_mod.Oracle(s, co), Engine(s, c)
+    o.set_state(pos=pos)
+    e.set_state(pos=pos)
+    Fo, Uo = o.compute_forces()
+    Fe, Ue = e.compute_forces()
+    if engine == abi.ENGINE_FP64:
+        scale = np.maximum(np.linalg.norm(Fo, axis=1), np.sqrt(np.mean(np.sum(Fo ** 2, 1))))
+        assert (np.linalg.norm(Fe - Fo, axis=1) / scale).max() < 1e-11
+    else:
+        assert rms_rel(Fe, Fo) < 1e-5, rms_rel(Fe, Fo)
+    assert abs(Ue.astype(np.float64).sum() - Uo.astype(np.float64).sum()) <= 1e-4 * np.abs(Uo).sum()

@@ -67,6 +67,18 @@ def rms_rel(F, Fr):
     return float(np.sqrt(np.sum((F - Fr) ** 2) / np.sum(Fr ** 2)))
 
 
+def profiled_traffic():
+    """HBM bytes per launch of the MIXED force kernel from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by the copy reference),
+    or None when it is absent."""
+    import re
+    path = os.path.join(ROOT, "profiles", "traffic_force_mixed_binary_lj_1M.md")
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.]+)", open(path).read())
+    return float(m.group(1)) if m else None
+
+
 def energy_drift(th0, th1):
     """|E(t) - E(0)| / |E(0)| of the total energy, and of Etot - U_cut (the
     energy without the jumps of the unshifted truncated pair potential, which
@@ -379,7 +391,9 @@ def main() -> int:
             "peak": HBM_PEAK / 1e9,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK,
-            "traffic": None,
+            "traffic": profiled_traffic(),
+            "traffic_source": "profiles/traffic_force_mixed_binary_lj_1M.md (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; "
+                              "reads scaled x2.0, writes x1.0 by the in-place list-buffer copy)",
             "kernel": "k_fast_force_mixed_lds",
             "kernel_ms": t_force * 1e3,
             "bytes_per_atom": b_force,

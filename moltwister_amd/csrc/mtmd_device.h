@@ -411,3 +411,10 @@ __device__ inline double mt_nh_propagate(double* eta, double* peta, const double
     }
     return S;
 }
+
+// CSimulationBox::pbcAdd (SimulationBox.cpp:135-166): one wrap step
+__device__ inline void mt_pbc_add(double& pos, double L, double Lm)
+{
+    MT_EXACT_FP
+    pos -= (double)mt_d2i((pos + L) / Lm) * Lm;
+}

@@ -398,7 +398,6 @@ void mt_nh_chain(const MtDev& d, int M, int nresp, double T, double dt, double g
 }
 
 // p *= S; pbcWrap (SimulationBox.cpp:135-166); optional partials of p.p/m
-__device__ inline void mt_pbc_add(double& pos, double L, double Lm) { pos -= (double)mt_d2i((pos + L) / Lm) * Lm; }
 
 __global__ void __launch_bounds__(TPB) k_scale_wrap(double4* __restrict__ pos, double4* __restrict__ mom, int n,
                                                     const double* nh, int scale_slot, double LX, double LY,

@@ -11,8 +11,9 @@
 // word carries the home-cell offset code of the pair, so the pair vector is
 //   d = xl_k - xl_i - offset * cell_size
 // in FP32 from two small numbers: no minimum-image branch and no loss of
-// precision to the box size. Pair forces and table interpolation run in FP32;
-// the per-atom force sum, integration and reductions stay in FP64.
+// precision to the box size. Pair forces and table interpolation run in FP32,
+// the four pairs of a list chunk are summed in FP32, and the per-atom sum of
+// the chunk sums, integration and reductions are FP64.
 #include "mtmd_internal.h"
 #include "mtmd_device.h"
 #include <algorithm>
@@ -663,7 +664,7 @@ __device__ __forceinline__ void mt_gather4(const float4* posf, mt_u32x4 w, float
 template <bool WANT_E, bool COUL, typename CoefT>
 __device__ __forceinline__ void mt_mixed_pair(uint32_t w, const float4& pa, const float4& pk, const float4* sh,
                                               const CoefT* row, float rc2, float x0, float invd, float qk,
-                                              const float* qf, double& Fx, double& Fy, double& Fz, float& U,
+                                              const float* qf, float& Fx, float& Fy, float& Fz, float& U,
                                               const MtDev& d, const MtParams& p, int k)
 {
     const float4 o = sh[w >> MT_NW_IDX_BITS];
@@ -691,9 +692,9 @@ __device__ __forceinline__ void mt_mixed_pair(uint32_t w, const float4& pa, cons
             fr += K * rinv * rinv * rinv;
             if (WANT_E) U += 0.5f * K * rinv;
         }
-        Fx += (double)(dx * fr);
-        Fy += (double)(dy * fr);
-        Fz += (double)(dz * fr);
+        Fx = fmaf(dx, fr, Fx);
+        Fy = fmaf(dy, fr, Fy);
+        Fz = fmaf(dz, fr, Fz);
     }
 }
 
@@ -722,10 +723,15 @@ __device__ __forceinline__ void mt_mixed_atom(const MtDev& d, const MtParams& p,
         // (while the next chunk's load stays in flight), so the type bits in
         // .w are not fetched by a second, dependent load inside a pair branch
         asm volatile("" : "+v"(a0.w), "+v"(a1.w), "+v"(a2.w), "+v"(a3.w));
-        mt_mixed_pair<WANT_E, COUL>(w.x, a0, pk, sh, row, rc2, x0, invd, qk, d.qf, Fx, Fy, Fz, U, d, p, k);
-        mt_mixed_pair<WANT_E, COUL>(w.y, a1, pk, sh, row, rc2, x0, invd, qk, d.qf, Fx, Fy, Fz, U, d, p, k);
-        mt_mixed_pair<WANT_E, COUL>(w.z, a2, pk, sh, row, rc2, x0, invd, qk, d.qf, Fx, Fy, Fz, U, d, p, k);
-        mt_mixed_pair<WANT_E, COUL>(w.w, a3, pk, sh, row, rc2, x0, invd, qk, d.qf, Fx, Fy, Fz, U, d, p, k);
+        // the four pair forces of a chunk are summed in FP32, the chunk sums in FP64
+        float cx = 0.f, cy = 0.f, cz = 0.f;
+        mt_mixed_pair<WANT_E, COUL>(w.x, a0, pk, sh, row, rc2, x0, invd, qk, d.qf, cx, cy, cz, U, d, p, k);
+        mt_mixed_pair<WANT_E, COUL>(w.y, a1, pk, sh, row, rc2, x0, invd, qk, d.qf, cx, cy, cz, U, d, p, k);
+        mt_mixed_pair<WANT_E, COUL>(w.z, a2, pk, sh, row, rc2, x0, invd, qk, d.qf, cx, cy, cz, U, d, p, k);
+        mt_mixed_pair<WANT_E, COUL>(w.w, a3, pk, sh, row, rc2, x0, invd, qk, d.qf, cx, cy, cz, U, d, p, k);
+        Fx += (double)cx;
+        Fy += (double)cy;
+        Fz += (double)cz;
     }
 }
 
@@ -1120,46 +1126,101 @@ void mt_fast_forces(const MtDev& d, const MtParams& p, int mixed, int want_e, in
 // integrator front half for the sorted engines: thermostat scale, kick, drift,
 // minimum-image displacement since the last rebuild, MIXED local coordinates
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(TPB) k_fast_kick_drift(MtDev d, MtParams p, int scale_slot, int mixed)
+// the per-atom arithmetic of the front half, shared with the fused NVE kernel
+// (contraction off in both, so fused and separate steps agree bit for bit)
+__device__ __forceinline__ float mt_fast_front(const MtDev& d, const MtParams& p, int k, double4& pm, double4& r,
+                                               const double4& F, int mixed)
 {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    MT_EXACT_FP
+    const double dt_2 = p.dt / 2.0;
+    pm.x += F.x * dt_2; pm.y += F.y * dt_2; pm.z += F.z * dt_2;
+    const double c = p.dt / pm.w;
+    r.x += pm.x * c; r.y += pm.y * c; r.z += pm.z * c;
+    const double4 q = d.posref[k];
+    double dx = r.x - q.x, dy = r.y - q.y, dz = r.z - q.z;
+    dx -= p.L[0] * rint(dx / p.L[0]);
+    dy -= p.L[1] * rint(dy / p.L[1]);
+    dz -= p.L[2] * rint(dz / p.L[2]);
+    if (mixed) {
+        const int c0 = d.home[k];
+        const int ix = c0 % d.nc[0], iy = (c0 / d.nc[0]) % d.nc[1], iz = c0 / (d.nc[0] * d.nc[1]);
+        double lx = r.x - (-p.L[0] / 2.0 + ((double)ix + 0.5) * (p.L[0] / d.nc[0]));
+        double ly = r.y - (-p.L[1] / 2.0 + ((double)iy + 0.5) * (p.L[1] / d.nc[1]));
+        double lz = r.z - (-p.L[2] / 2.0 + ((double)iz + 0.5) * (p.L[2] / d.nc[2]));
+        lx -= p.L[0] * rint(lx / p.L[0]);
+        ly -= p.L[1] * rint(ly / p.L[1]);
+        lz -= p.L[2] * rint(lz / p.L[2]);
+        const float4 old = d.posf[k];
+        d.posf[k] = make_float4((float)lx, (float)ly, (float)lz, old.w);
+    }
+    return (float)(dx * dx + dy * dy + dz * dz);
+}
+
+// only "has any atom moved beyond skin/2" matters: one atomic per wavefront
+// that crossed the threshold, and none once the flag is already raised
+__device__ __forceinline__ void mt_fast_flag_disp(const MtDev& d, const MtParams& p, float d2)
+{
+    for (int off = 32; off > 0; off >>= 1) d2 = fmaxf(d2, __shfl_xor(d2, off));
+    if ((threadIdx.x & 63) == 0 && d2 > p.half_skin2 && !(__uint_as_float(d.flags[0]) > p.half_skin2))
+        atomicMax(&d.flags[0], __float_as_uint(d2));
+}
+
+// NVE steps inside one call: the back half of step t (force cap, half kick,
+// PBC wrap; CVelVerlet + pbcWrap, VelVerlet.cpp:96-122, SimulationBox.cpp:135-166)
+// and the front half of step t + 1 (half kick, drift) in one pass. The
+// capped force is written back only where the cap changed it.
+__global__ void __launch_bounds__(TPB) k_fast_nve_fused(MtDev d, MtParams p, int mixed)
+{
+    MT_EXACT_FP
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
     float d2 = 0.0f;
     if (k < d.n) {
-        double4 pm = d.mom[k], r = d.pos[k], F = d.frc[k];
+        double4 F = d.frc[k], pm = d.mom[k], r = d.pos[k];
+        if (p.Fcut > 0) {
+            bool capped = false;
+            if (fabs(F.x) > p.Fcut) { F.x = ((F.x >= 0.0) ? 1.0 : -1.0) * p.Fcut; capped = true; }
+            if (fabs(F.y) > p.Fcut) { F.y = ((F.y >= 0.0) ? 1.0 : -1.0) * p.Fcut; capped = true; }
+            if (fabs(F.z) > p.Fcut) { F.z = ((F.z >= 0.0) ? 1.0 : -1.0) * p.Fcut; capped = true; }
+            if (capped) d.frc[k] = F;
+        }
         const double dt_2 = p.dt / 2.0;
+        pm.x = pm.x + F.x * dt_2; pm.y = pm.y + F.y * dt_2; pm.z = pm.z + F.z * dt_2;
+        const double X2 = p.L[0] / 2.0, Y2 = p.L[1] / 2.0, Z2 = p.L[2] / 2.0;
+        if (r.x >= X2) mt_pbc_add(r.x, X2, p.L[0]);
+        if (r.x < -X2) mt_pbc_add(r.x, -X2, p.L[0]);
+        if (r.y >= Y2) mt_pbc_add(r.y, Y2, p.L[1]);
+        if (r.y < -Y2) mt_pbc_add(r.y, -Y2, p.L[1]);
+        if (r.z >= Z2) mt_pbc_add(r.z, Z2, p.L[2]);
+        if (r.z < -Z2) mt_pbc_add(r.z, -Z2, p.L[2]);
+        d2 = mt_fast_front(d, p, k, pm, r, F, mixed);
+        d.mom[k] = pm;
+        d.pos[k] = r;
+    }
+    mt_fast_flag_disp(d, p, d2);
+}
+
+void mt_fast_nve_fused(const MtDev& d, const MtParams& p, int mixed, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fast_nve_fused, dim3(nblk(d.n, TPB)), dim3(TPB), 0, s, d, p, mixed);
+}
+
+__global__ void __launch_bounds__(TPB) k_fast_kick_drift(MtDev d, MtParams p, int scale_slot, int mixed)
+{
+    MT_EXACT_FP
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    float d2 = 0.0f;
+    if (k < d.n) {
+        double4 pm = d.mom[k], r = d.pos[k];
+        const double4 F = d.frc[k];
         if (scale_slot >= 0) {
             const double S = d.nh[scale_slot];
             pm.x *= S; pm.y *= S; pm.z *= S;
         }
-        pm.x += F.x * dt_2; pm.y += F.y * dt_2; pm.z += F.z * dt_2;
-        const double c = p.dt / pm.w;
-        r.x += pm.x * c; r.y += pm.y * c; r.z += pm.z * c;
+        d2 = mt_fast_front(d, p, k, pm, r, F, mixed);
         d.mom[k] = pm;
         d.pos[k] = r;
-        const double4 q = d.posref[k];
-        double dx = r.x - q.x, dy = r.y - q.y, dz = r.z - q.z;
-        dx -= p.L[0] * rint(dx / p.L[0]);
-        dy -= p.L[1] * rint(dy / p.L[1]);
-        dz -= p.L[2] * rint(dz / p.L[2]);
-        d2 = (float)(dx * dx + dy * dy + dz * dz);
-        if (mixed) {
-            const int c0 = d.home[k];
-            const int ix = c0 % d.nc[0], iy = (c0 / d.nc[0]) % d.nc[1], iz = c0 / (d.nc[0] * d.nc[1]);
-            double lx = r.x - (-p.L[0] / 2.0 + ((double)ix + 0.5) * (p.L[0] / d.nc[0]));
-            double ly = r.y - (-p.L[1] / 2.0 + ((double)iy + 0.5) * (p.L[1] / d.nc[1]));
-            double lz = r.z - (-p.L[2] / 2.0 + ((double)iz + 0.5) * (p.L[2] / d.nc[2]));
-            lx -= p.L[0] * rint(lx / p.L[0]);
-            ly -= p.L[1] * rint(ly / p.L[1]);
-            lz -= p.L[2] * rint(lz / p.L[2]);
-            const float4 old = d.posf[k];
-            d.posf[k] = make_float4((float)lx, (float)ly, (float)lz, old.w);
-        }
     }
-    // only "has any atom moved beyond skin/2" matters: one atomic per wavefront
-    // that crossed the threshold, and none once the flag is already raised
-    for (int off = 32; off > 0; off >>= 1) d2 = fmaxf(d2, __shfl_xor(d2, off));
-    if ((threadIdx.x & 63) == 0 && d2 > p.half_skin2 && !(__uint_as_float(d.flags[0]) > p.half_skin2))
-        atomicMax(&d.flags[0], __float_as_uint(d2));
+    mt_fast_flag_disp(d, p, d2);
 }
 
 void mt_fast_kick_drift(const MtDev& d, const MtParams& p, int scale_slot, int mixed, hipStream_t s)

@@ -1118,6 +1118,8 @@ static int md_steps(mtmd_handle* h, int nsteps, bool energy_last)
         // re-sort now so the step loop runs on spatially ordered atoms
         if ((rc = fast_full_forces(h))) return rc;
     }
+    const int mixed = h->engine == MTMD_ENGINE_MIXED;
+    bool front_done = false;        // NVE, FP64/MIXED: this step's front half ran fused with the last back half
     for (int t = 0; t < nsteps; t++) {
         const bool last = (t == nsteps - 1);
         if (nvt) {
@@ -1129,10 +1131,22 @@ static int md_steps(mtmd_handle* h, int nsteps, bool energy_last)
             if ((rc = exact_forces(h))) return rc;
         } else {
             const bool sort_now = (++h->steps_since_sort >= h->sort_every);
-            mt_fast_clear_flags(d, h->stream);
-            mt_fast_kick_drift(d, h->p, nvt ? h->nh_scale_slot : -1, h->engine == MTMD_ENGINE_MIXED, h->stream);
+            if (!front_done) {
+                mt_fast_clear_flags(d, h->stream);
+                mt_fast_kick_drift(d, h->p, nvt ? h->nh_scale_slot : -1, mixed, h->stream);
+            }
             if ((rc = fast_rebuild(h, sort_now, sort_now))) return rc;
             if ((rc = fast_forces(h, last && energy_last))) return rc;
+            if (!nvt && !last) {
+                // back half of this step and front half of the next in one pass
+                mt_fast_clear_flags(d, h->stream);
+                mt_fast_nve_fused(d, h->p, mixed, h->stream);
+                front_done = true;
+                h->ke_valid = false;
+                h->steps++;
+                continue;
+            }
+            front_done = false;
         }
         mt_cap_kick(d, h->p, nvt ? 1 : 0, h->stream);
         if (nvt) mt_nh_chain(d, h->M, h->nresp, h->Tred, h->p.dt, h->g, h->w, nblocks, h->stream);

@@ -125,6 +125,8 @@ void mt_fast_neighbors(const MtDev& d, const MtParams& p, int conditional, hipSt
 void mt_fast_forces(const MtDev& d, const MtParams& p, int mixed, int want_energy, int block, hipStream_t s);
 void mt_fast_kick_drift(const MtDev& d, const MtParams& p, int scale_slot, int mixed, hipStream_t s);
 void mt_fast_clear_flags(const MtDev& d, hipStream_t s);
+// NVE: back half of one step and front half of the next in one pass
+void mt_fast_nve_fused(const MtDev& d, const MtParams& p, int mixed, hipStream_t s);
 // in-place read + write of the whole neighbour-list buffer (HBM reference)
 void mt_copy_lists(const MtDev& d, hipStream_t s);
 // per-block partial sums of U_cut into d.red[0 .. mt_reduce_blocks(n))

@@ -243,3 +243,21 @@ def test_dense_cells_water_1000(oracle_mod, engine, coul):
     else:
         assert rms_rel(Fe, Fo) < 1e-5, rms_rel(Fe, Fo)
     assert abs(Ue.astype(np.float64).sum() - Uo.astype(np.float64).sum()) <= 1e-4 * np.abs(Uo).sum()
+
+
+@pytest.mark.parametrize("engine", [abi.ENGINE_FP64, abi.ENGINE_MIXED])
+def test_fused_nve_steps_equal_single_steps(engine):
+    """NVE steps inside one call run the back half of a step and the front half
+    of the next in one kernel; the trajectory is bit-identical to the same
+    steps taken one call at a time (separate kernels)."""
+    print(f"case fused nve engine={engine}", flush=True)
+    s, c = make("binary_lj", engine=engine)
+    s.pos = s.pos + np.random.default_rng(12).normal(0, 0.01, s.pos.shape)
+    a, b = Engine(s, c), Engine(s, c)
+    a.step(60)
+    for _ in range(60):
+        b.step(1)
+    pa, ma = a.get_state()
+    pb, mb = b.get_state()
+    assert np.array_equal(pa, pb) and np.array_equal(ma, mb)
+    assert a.thermo()["Etot"] == b.thermo()["Etot"]

@@ -255,6 +255,24 @@ def main() -> int:
                                                                     water_F[abi.ENGINE_FP64])
         log(f"bench: water {water}")
 
+    # configuration 1: the reference's CPU-runnable case on the GPU engines
+    cfg1 = {}
+    sys1 = W.fcc_lj(10)
+    for name, eng_id in (("exact", abi.ENGINE_EXACT), ("fp64", abi.ENGINE_FP64), ("mixed", abi.ENGINE_MIXED)):
+        c1 = lj_config(eng_id, abi.NVE)
+        c1.temperature = 86.596
+        e1 = Engine(sys1, c1)
+        e1.step(10)
+        th_a = e1.thermo()
+        k1 = 1000
+        dt1 = timed_steps(e1, k1)
+        th_b = e1.thermo()
+        cfg1[name] = {"steps_per_s": k1 / dt1, "atom_steps_per_s": sys1.n_atoms * k1 / dt1,
+                      "nve_drift_shifted_rel": energy_drift(th_a, th_b)[1]}
+        e1.close()
+    cfg1["workload"] = "FCC LJ, 4000 atoms, rho*=0.8442, T*=0.72, rc 2.5, NVE, dt 0.005 tau, 1000 timed steps"
+    log(f"bench: configuration 1 {cfg1}")
+
     batch = {}
     if not args.skip_batch:
         from moltwister_amd.engine import BatchEngine
@@ -365,6 +383,24 @@ def main() -> int:
                          f"thread, lists rebuilt at rc every step, its all-pairs Coulomb pass skipped because "
                          f"every charge is zero; the first force call ({t_first:.1f}s) not counted"}
         log(f"bench: cpu oracle {t_cpu / nst:.2f} s/step on {sub.n_atoms} atoms")
+        # MolTwister's own force path (oracle/_ref, built from its sources) on
+        # configuration 1, where it runs in seconds: cell list, neighbour list
+        # and the force functor including its all-pairs Coulomb loop
+        if os.path.exists(pyoracle.REF_LIB):
+            c_ref1 = lj_config(abi.ENGINE_EXACT, abi.NVE)
+            r1 = pyoracle.Reference(sys1, c_ref1)
+            r1.compute_forces()
+            t0 = time.perf_counter()
+            nrep = 5
+            for _ in range(nrep):
+                r1.compute_forces()
+            t_r1 = (time.perf_counter() - t0) / nrep
+            cpu["reference_config1"] = {"value": sys1.n_atoms / t_r1, "unit": "atom-force-evaluations/s",
+                                        "cores": 1, "kind": "reference",
+                                        "sample": f"{nrep} force calls of MolTwister's CPU functors on the 4000-atom "
+                                                  "FCC system (cells, neighbour list, functor with its all-pairs "
+                                                  "Coulomb loop); the O(N) integrator is not included"}
+            log(f"bench: reference functor {t_r1 * 1e3:.1f} ms per force call on {sys1.n_atoms} atoms")
 
     out = {
         "metric": METRIC,
@@ -414,6 +450,7 @@ def main() -> int:
         "step_roofline": {"bytes_per_atom_step": step_bytes, "frac": value * step_bytes / HBM_PEAK,
                           "rebuild_every_steps": rebuild_every, "list_build_ms": t_build * 1e3,
                           "copy_GBps": copy_bw / 1e9},
+        "configuration_1": cfg1,
         "water_30k": water,
         "replicas_512": batch,
         "force_matching": fmres,

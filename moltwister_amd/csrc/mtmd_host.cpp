@@ -85,6 +85,21 @@ struct mtmd_handle {
     std::vector<int> perm_host;   // valid after download_perm()
     int* h_status = nullptr;      // pinned status words
 
+    // captured plain steps (no re-sort, not the last of a call), one per
+    // buffer parity (the re-sort swaps the slot buffers, which the kernels
+    // take by value) and ensemble; dropped when the neighbour buffer moves
+    struct StepGraph { const void* key = nullptr; hipGraph_t g = nullptr; hipGraphExec_t x = nullptr; };
+    StepGraph graphs[2];
+    bool use_graphs = true;
+    void drop_graphs()
+    {
+        for (auto& sg : graphs) {
+            if (sg.x) (void)hipGraphExecDestroy(sg.x);
+            if (sg.g) (void)hipGraphDestroy(sg.g);
+            sg = StepGraph();
+        }
+    }
+
     int fail(int code, const char* fmt, ...)
     {
         char buf[512];
@@ -776,6 +791,7 @@ extern "C" void mtmd_destroy(mtmd_handle* h)
 {
     if (!h) return;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
+    h->drop_graphs();
     h->mem.release();
     if (h->h_status) (void)hipHostFree(h->h_status);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -891,6 +907,7 @@ static int grow_neighbors(mtmd_handle* h, int need)
     for (auto& p : h->mem.ptrs) if (p == old) { (void)hipFree(p); p = nl; }
     d.nlist = nl;
     d.ncap = cap;
+    h->drop_graphs();
     return MTMD_OK;
 }
 
@@ -1119,40 +1136,97 @@ static int md_steps(mtmd_handle* h, int nsteps, bool energy_last)
         if ((rc = fast_full_forces(h))) return rc;
     }
     const int mixed = h->engine == MTMD_ENGINE_MIXED;
-    bool front_done = false;        // NVE, FP64/MIXED: this step's front half ran fused with the last back half
-    for (int t = 0; t < nsteps; t++) {
-        const bool last = (t == nsteps - 1);
-        if (nvt) {
-            if (!h->ke_valid) mt_reduce_ke(d, h->stream);
-            mt_nh_chain(d, h->M, h->nresp, h->Tred, h->p.dt, h->g, h->w, nblocks, h->stream);
-        }
-        if (!h->fast) {
+    if (!h->fast) {
+        for (int t = 0; t < nsteps; t++) {
+            if (nvt) {
+                if (!h->ke_valid) mt_reduce_ke(d, h->stream);
+                mt_nh_chain(d, h->M, h->nresp, h->Tred, h->p.dt, h->g, h->w, nblocks, h->stream);
+            }
             mt_kick_drift(d, h->p, nvt ? h->nh_scale_slot : -1, 0, h->stream);
             if ((rc = exact_forces(h))) return rc;
-        } else {
-            const bool sort_now = (++h->steps_since_sort >= h->sort_every);
-            if (!front_done) {
-                mt_fast_clear_flags(d, h->stream);
-                mt_fast_kick_drift(d, h->p, nvt ? h->nh_scale_slot : -1, mixed, h->stream);
-            }
-            if ((rc = fast_rebuild(h, sort_now, sort_now))) return rc;
-            if ((rc = fast_forces(h, last && energy_last))) return rc;
-            if (!nvt && !last) {
-                // back half of this step and front half of the next in one pass
+            mt_cap_kick(d, h->p, nvt ? 1 : 0, h->stream);
+            if (nvt) mt_nh_chain(d, h->M, h->nresp, h->Tred, h->p.dt, h->g, h->w, nblocks, h->stream);
+            mt_scale_wrap(d, h->p, nvt ? h->nh_scale_slot : -1, nvt ? 1 : 0, h->stream);
+            h->ke_valid = nvt;
+            h->steps++;
+        }
+    } else {
+        // FP64/MIXED. NVE: inside a call the back half of step t and the front
+        // half of step t + 1 run as one kernel. Plain steps (no re-sort, not
+        // the last of the call) replay a captured graph of their kernels.
+        auto nvt_front = [&]() {
+            if (!h->ke_valid) mt_reduce_ke(d, h->stream);
+            mt_nh_chain(d, h->M, h->nresp, h->Tred, h->p.dt, h->g, h->w, nblocks, h->stream);
+            mt_fast_clear_flags(d, h->stream);
+            mt_fast_kick_drift(d, h->p, h->nh_scale_slot, mixed, h->stream);
+        };
+        auto nvt_back = [&]() {
+            mt_cap_kick(d, h->p, 1, h->stream);
+            mt_nh_chain(d, h->M, h->nresp, h->Tred, h->p.dt, h->g, h->w, nblocks, h->stream);
+            mt_scale_wrap(d, h->p, h->nh_scale_slot, 1, h->stream);
+        };
+        // the kernels of one plain step after its front half (NVE) or whole (NVT)
+        auto plain_step = [&]() -> int {
+            if (nvt) nvt_front();
+            int r = fast_rebuild(h, false, false);
+            if (r) return r;
+            if ((r = fast_forces(h, false))) return r;
+            if (nvt) {
+                nvt_back();
+            } else {
                 mt_fast_clear_flags(d, h->stream);
                 mt_fast_nve_fused(d, h->p, mixed, h->stream);
-                front_done = true;
-                h->ke_valid = false;
+            }
+            return MTMD_OK;
+        };
+        auto replay = [&]() -> int {
+            // graph slot by ensemble; key = current slot buffer (parity)
+            mtmd_handle::StepGraph& sg = h->graphs[nvt ? 1 : 0];
+            if (sg.key != (const void*)d.pos) {
+                if (sg.x) (void)hipGraphExecDestroy(sg.x);
+                if (sg.g) (void)hipGraphDestroy(sg.g);
+                sg = mtmd_handle::StepGraph();
+                HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+                int r = plain_step();
+                hipGraph_t g = nullptr;
+                const hipError_t e = hipStreamEndCapture(h->stream, &g);
+                if (r) { if (g) (void)hipGraphDestroy(g); return r; }
+                if (e != hipSuccess) return h->fail(MTMD_EHIP, "step capture: %s", hipGetErrorString(e));
+                sg.g = g;
+                HIPCHK(h, hipGraphInstantiate(&sg.x, g, nullptr, nullptr, 0));
+                sg.key = (const void*)d.pos;
+            }
+            HIPCHK(h, hipGraphLaunch(sg.x, h->stream));
+            return MTMD_OK;
+        };
+        if (!nvt) {
+            mt_fast_clear_flags(d, h->stream);
+            mt_fast_kick_drift(d, h->p, -1, mixed, h->stream);
+        }
+        for (int t = 0; t < nsteps; t++) {
+            const bool last = (t == nsteps - 1);
+            const bool sort_now = (++h->steps_since_sort >= h->sort_every);
+            if (!sort_now && !last && h->use_graphs && (!nvt || h->ke_valid)) {
+                if ((rc = replay())) return rc;
+                h->ke_valid = nvt;
                 h->steps++;
                 continue;
             }
-            front_done = false;
+            if (nvt) nvt_front();
+            if ((rc = fast_rebuild(h, sort_now, sort_now))) return rc;
+            if ((rc = fast_forces(h, last && energy_last))) return rc;
+            if (nvt) {
+                nvt_back();
+            } else if (last) {
+                mt_cap_kick(d, h->p, 0, h->stream);
+                mt_scale_wrap(d, h->p, -1, 0, h->stream);
+            } else {
+                mt_fast_clear_flags(d, h->stream);
+                mt_fast_nve_fused(d, h->p, mixed, h->stream);
+            }
+            h->ke_valid = nvt;
+            h->steps++;
         }
-        mt_cap_kick(d, h->p, nvt ? 1 : 0, h->stream);
-        if (nvt) mt_nh_chain(d, h->M, h->nresp, h->Tred, h->p.dt, h->g, h->w, nblocks, h->stream);
-        mt_scale_wrap(d, h->p, nvt ? h->nh_scale_slot : -1, nvt ? 1 : 0, h->stream);
-        h->ke_valid = nvt;
-        h->steps++;
     }
     HIPCHK(h, hipGetLastError());
     if (h->fast) {

@@ -261,3 +261,23 @@ def test_fused_nve_steps_equal_single_steps(engine):
     pb, mb = b.get_state()
     assert np.array_equal(pa, pb) and np.array_equal(ma, mb)
     assert a.thermo()["Etot"] == b.thermo()["Etot"]
+
+
+@pytest.mark.parametrize("engine", [abi.ENGINE_FP64, abi.ENGINE_MIXED])
+def test_graph_replayed_nvt_steps_equal_single_steps(engine):
+    """Plain NVT steps inside one call replay a captured step graph (re-sorts
+    every 100 steps swap the buffers and re-capture); the trajectory equals
+    the same steps taken one call at a time, bit for bit."""
+    print(f"case graph nvt engine={engine}", flush=True)
+    s, c = make("fcc_4000", engine=engine, ensemble=abi.NVT, T=86.596)
+    s.pos = s.pos + np.random.default_rng(13).normal(0, 0.01, s.pos.shape)
+    a, b = Engine(s, c), Engine(s, c)
+    a.step(230)
+    for _ in range(230):
+        b.step(1)
+    pa, ma = a.get_state()
+    pb, mb = b.get_state()
+    assert np.array_equal(pa, pb) and np.array_equal(ma, mb)
+    ea, pea = a.get_thermostat()
+    eb, peb = b.get_thermostat()
+    assert np.array_equal(ea, eb) and np.array_equal(pea, peb)

@@ -59,8 +59,9 @@ enum { MTMD_NVT = 0, MTMD_NPT = 1, MTMD_NVE = 2 };
  *          and accumulation, the reference summation order per atom.
  *  FP64  - same arithmetic as EXACT per pair, but neighbour lists at rc + skin
  *          rebuilt on displacement, atoms spatially sorted at each rebuild.
- *  MIXED - as FP64 but pair geometry and table interpolation in FP32,
- *          per-atom accumulation, integration and reductions in FP64. */
+ *  MIXED - as FP64 but pair geometry and table interpolation in FP32, the
+ *          four pairs of a list chunk summed in FP32 and the chunk sums per
+ *          atom in FP64; integration and reductions in FP64. */
 enum { MTMD_ENGINE_EXACT = 0, MTMD_ENGINE_FP64 = 1, MTMD_ENGINE_MIXED = 2 };
 
 /* Coulomb treatment.

@@ -239,25 +239,12 @@ __global__ void __launch_bounds__(MB_TPB) k_mb_scatter(MbDev d, int a0, int a1)
 // ascending atom order inside each cell (reproducible lists): one wavefront per cell
 __global__ void __launch_bounds__(MB_TPB) k_mb_cell_sort(MbDev d, int c0, int c1)
 {
+    __shared__ __align__(16) int lds[(MB_TPB / 64) * MT_CELL_SORT_LDS];
+    if (d.flags[1]) return;                                      // uniform over the grid
     const int c = c0 + (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-    const int lane = threadIdx.x & 63;
-    if (c >= c1 || !d.rflag[d.cell_rid[c]] || d.flags[1]) return;
-    const int cnt = d.cell_count[c];
-    if (cnt <= 1) return;
-    int* a = d.cell_list + d.cell_first[c];
-    if (cnt <= 64) {
-        const int v = lane < cnt ? a[lane] : INT_MAX;
-        int rank = 0;
-        for (int j = 0; j < cnt; j++) rank += __shfl(v, j) < v ? 1 : 0;
-        if (lane < cnt) a[rank] = v;
-    } else if (lane == 0) {
-        for (int i = 1; i < cnt; i++) {
-            const int v = a[i];
-            int j = i - 1;
-            while (j >= 0 && a[j] > v) { a[j + 1] = a[j]; j--; }
-            a[j + 1] = v;
-        }
-    }
+    const bool live = c < c1 && d.rflag[d.cell_rid[c]];
+    mt_wave_sort_cell(d.cell_list + (live ? d.cell_first[c] : 0), live ? d.cell_count[c] : 0,
+                      lds + (threadIdx.x >> 6) * MT_CELL_SORT_LDS);
 }
 
 __device__ __forceinline__ float mb_min_image(float d, float L, float iL) { return fmaf(-L, rintf(d * iL), d); }

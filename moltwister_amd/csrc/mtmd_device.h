@@ -418,3 +418,45 @@ __device__ inline void mt_pbc_add(double& pos, double L, double Lm)
     MT_EXACT_FP
     pos -= (double)mt_d2i((pos + L) / Lm) * Lm;
 }
+
+// Ascending sort of one cell's list range a[0..cnt) of distinct atom indices
+// by one wavefront. The lists are filled by atomics, so their order is
+// arbitrary until sorted. Up to 64 entries are ranked in registers; up to
+// MT_CELL_SORT_LDS entries are copied into this wave's share of LDS and each
+// lane ranks its entries against all of them (broadcast reads, 4 per int4);
+// a larger cell falls back to an insertion sort by lane 0. Holds a block
+// barrier, so every thread of the block calls it (cnt = 0 for no cell).
+#define MT_CELL_SORT_LDS 2048
+__device__ inline void mt_wave_sort_cell(int* __restrict__ a, int cnt, int* __restrict__ lds)
+{
+    const int lane = threadIdx.x & 63;
+    const bool in_lds = cnt > 64 && cnt <= MT_CELL_SORT_LDS;
+    const int n4 = (cnt + 3) >> 2;
+    if (in_lds)
+        for (int i = lane; i < 4 * n4; i += 64) lds[i] = i < cnt ? a[i] : INT_MAX;
+    __syncthreads();
+    if (cnt > 1 && cnt <= 64) {
+        const int v = lane < cnt ? a[lane] : INT_MAX;
+        int rank = 0;
+        for (int j = 0; j < cnt; j++) rank += __shfl(v, j) < v ? 1 : 0;
+        if (lane < cnt) a[rank] = v;
+    } else if (in_lds) {
+        const int4* l4 = reinterpret_cast<const int4*>(lds);
+        for (int i = lane; i < cnt; i += 64) {
+            const int v = lds[i];
+            int rank = 0;
+            for (int j = 0; j < n4; j++) {
+                const int4 w = l4[j];
+                rank += (w.x < v) + (w.y < v) + (w.z < v) + (w.w < v);
+            }
+            a[rank] = v;
+        }
+    } else if (cnt > MT_CELL_SORT_LDS && lane == 0) {
+        for (int i = 1; i < cnt; i++) {
+            const int v = a[i];
+            int j = i - 1;
+            while (j >= 0 && a[j] > v) { a[j + 1] = a[j]; j--; }
+            a[j + 1] = v;
+        }
+    }
+}

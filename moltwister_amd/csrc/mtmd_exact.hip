@@ -94,19 +94,14 @@ __global__ void k_cell_scatter(const int* __restrict__ key, int n, const int* __
 }
 
 // ascending atom index inside each cell (the reference fills cells in atom order)
-__global__ void k_cell_sort(const int* __restrict__ first, const int* __restrict__ count,
-                            int* __restrict__ list, int ncell, const unsigned int* flags)
+__global__ void __launch_bounds__(TPB) k_cell_sort(const int* __restrict__ first, const int* __restrict__ count,
+                                                   int* __restrict__ list, int ncell, const unsigned int* flags)
 {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncell || flags[3]) return;
-    int cnt = count[c];
-    if (cnt < 2) return;
-    int* a = list + first[c];
-    for (int i = 1; i < cnt; i++) {
-        int v = a[i], j = i - 1;
-        while (j >= 0 && a[j] > v) { a[j + 1] = a[j]; j--; }
-        a[j + 1] = v;
-    }
+    __shared__ __align__(16) int lds[(TPB / 64) * MT_CELL_SORT_LDS];
+    if (flags[3]) return;                                        // uniform over the grid
+    const int c = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int cnt = c < ncell ? count[c] : 0;
+    mt_wave_sort_cell(list + (c < ncell ? first[c] : 0), cnt, lds + (threadIdx.x >> 6) * MT_CELL_SORT_LDS);
 }
 
 void mt_exact_cells(const MtDev& d, const MtParams& p, hipStream_t s)
@@ -118,7 +113,7 @@ void mt_exact_cells(const MtDev& d, const MtParams& p, hipStream_t s)
     hipLaunchKernelGGL(k_scan_cells, dim3(1), dim3(1024), 0, s, d.cell_count, d.cell_first, d.cell_fill, d.ncells);
     hipLaunchKernelGGL(k_cell_scatter, dim3(nblk(d.n, TPB)), dim3(TPB), 0, s, d.cell_key, d.n, d.cell_first,
                        d.cell_fill, d.cell_list);
-    hipLaunchKernelGGL(k_cell_sort, dim3(nblk(d.ncells, TPB)), dim3(TPB), 0, s, d.cell_first, d.cell_count,
+    hipLaunchKernelGGL(k_cell_sort, dim3(nblk(d.ncells, TPB / 64)), dim3(TPB), 0, s, d.cell_first, d.cell_count,
                        d.cell_list, d.ncells, d.flags);
 }
 

@@ -176,26 +176,11 @@ __global__ void __launch_bounds__(TPB) k_fast_cell_sort(const int* __restrict__ 
                                                         const int* __restrict__ count, int* __restrict__ list,
                                                         int ncell, int cond, const unsigned int* flags, float thr)
 {
-    if (mt_skip_rebuild(cond, flags, thr) || flags[3]) return;
+    __shared__ __align__(16) int lds[(TPB / 64) * MT_CELL_SORT_LDS];
+    if (mt_skip_rebuild(cond, flags, thr) || flags[3]) return;   // uniform over the grid
     const int c = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int lane = threadIdx.x & 63;
-    if (c >= ncell) return;
-    const int cnt = count[c];
-    if (cnt <= 1) return;
-    int* a = list + first[c];
-    if (cnt <= 64) {
-        const int v = lane < cnt ? a[lane] : INT_MAX;
-        int rank = 0;
-        for (int j = 0; j < cnt; j++) rank += __shfl(v, j) < v ? 1 : 0;
-        if (lane < cnt) a[rank] = v;
-    } else if (lane == 0) {
-        for (int i = 1; i < cnt; i++) {
-            const int v = a[i];
-            int j = i - 1;
-            while (j >= 0 && a[j] > v) { a[j + 1] = a[j]; j--; }
-            a[j + 1] = v;
-        }
-    }
+    const int cnt = c < ncell ? count[c] : 0;
+    mt_wave_sort_cell(list + (c < ncell ? first[c] : 0), cnt, lds + (threadIdx.x >> 6) * MT_CELL_SORT_LDS);
 }
 
 void mt_fast_cells(const MtDev& d, const MtParams& p, int cond, hipStream_t s)

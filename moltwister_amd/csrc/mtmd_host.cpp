@@ -305,6 +305,11 @@ static int setup(mtmd_handle* h, const mtmd_system* s)
     h->n = n; h->T = T;
     h->engine = c.engine;
     h->fast = (c.engine == MTMD_ENGINE_FP64 || c.engine == MTMD_ENGINE_MIXED);
+    {
+        // MTMD_GRAPHS=0 launches every step's kernels directly (same results)
+        const char* g = std::getenv("MTMD_GRAPHS");
+        h->use_graphs = !(g && std::strcmp(g, "0") == 0);
+    }
     if (n <= 0 || T <= 0 || !s->pos || !s->mass || !s->charge || !s->type)
         return h->fail(MTMD_EINVAL, "system needs atoms, types, positions, masses and charges");
     if (n >= (1 << MT_NW_IDX_BITS)) return h->fail(MTMD_EINVAL, "at most %d atoms", (1 << MT_NW_IDX_BITS) - 1);

@@ -430,7 +430,7 @@ def main() -> int:
             "traffic": profiled_traffic(),
             "traffic_source": "profiles/traffic_force_mixed_binary_lj_1M.md (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; "
                               "reads scaled x2.0, writes x1.0 by the in-place list-buffer copy)",
-            "kernel": "k_fast_force_mixed_lds",
+            "kernel": "k_fast_force_tile",
             "kernel_ms": t_force * 1e3,
             "bytes_per_atom": b_force,
         },

@@ -261,13 +261,17 @@ void mt_fast_identity_perm(const MtDev& d, hipStream_t s)
 __global__ void k_fast_home(MtDev d, MtParams p, int cond)
 {
     if (mt_skip_rebuild(cond, d.flags, p.half_skin2)) return;
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s == 0) d.flags[4] += 1;
-    if (s >= d.n) return;
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u == 0) d.flags[4] += 1;
+    if (u >= d.n) return;
+    // u-th atom in cell order
+    const int s = d.cell_list[u];
     int c = d.cell_key[s];
     double4 r = d.pos[s];
     d.home[s] = c;
     d.posref[s] = r;
+    d.sidx[s] = u;
+    d.qfs[u] = d.qf[s];
     int ix = c % d.nc[0], iy = (c / d.nc[0]) % d.nc[1], iz = c / (d.nc[0] * d.nc[1]);
     double cx = -p.L[0] / 2.0 + ((double)ix + 0.5) * (p.L[0] / d.nc[0]);
     double cy = -p.L[1] / 2.0 + ((double)iy + 0.5) * (p.L[1] / d.nc[1]);
@@ -276,7 +280,7 @@ __global__ void k_fast_home(MtDev d, MtParams p, int cond)
     dx -= p.L[0] * rint(dx / p.L[0]);
     dy -= p.L[1] * rint(dy / p.L[1]);
     dz -= p.L[2] * rint(dz / p.L[2]);
-    d.posf[s] = make_float4((float)dx, (float)dy, (float)dz, __int_as_float(d.type[s]));
+    d.posf[u] = make_float4((float)dx, (float)dy, (float)dz, __int_as_float(d.type[s]));
 }
 
 void mt_fast_home(const MtDev& d, const MtParams& p, int cond, hipStream_t s)
@@ -326,18 +330,22 @@ __device__ inline bool mt_is_partner(const MtDev& d, int k, int ia)
 #define NB_CAP 2048
 #define NB_PAD_CODE 13u      // offset code of (0, 0, 0): a pad word is the atom itself
 
+// slot of a list-word index (the index itself unless the words are in cell order)
+__device__ __forceinline__ int mt_id_slot(const MtDev& d, int id) { return d.wsorted ? d.cell_list[id] : id; }
+
+// k: the home atom's slot (its list column); self: its own word index
 template <int MODE>
-__device__ __forceinline__ void mt_neigh_hit(const MtDev& d, uint4* nl4, float4 c, uint32_t code, int k, int molk,
-                                             const int* smol, int e, int n, int ncap, int& cnt, uint32_t& b0,
-                                             uint32_t& b1, uint32_t& b2)
+__device__ __forceinline__ void mt_neigh_hit(const MtDev& d, uint4* nl4, float4 c, uint32_t code, int k, int self,
+                                             int molk, const int* smol, int e, int n, int ncap, int& cnt,
+                                             uint32_t& b0, uint32_t& b1, uint32_t& b2)
 {
     const int a = (int)__float_as_uint(c.w);
-    if (a == k) return;
+    if (a == self) return;
     if constexpr (MODE >= 1) {
         if (smol[e] == molk) return;
     }
     if constexpr (MODE == 2) {
-        if (mt_is_partner(d, k, a)) return;
+        if (mt_is_partner(d, k, mt_id_slot(d, a))) return;
     }
     const uint32_t w = (uint32_t)a | code;
     const int slot = cnt & 3;
@@ -407,8 +415,10 @@ __global__ void __launch_bounds__(256) k_fast_neigh_block(MtDev d, MtParams p, i
     int staged_b = -1, staged_e = -1;
     for (int ib = 0; ib < cnt_max; ib += 32) {
         const bool act = ib + l < cnt_c;
-        const int k = act ? d.cell_list[f0 + ib + l] : 0;
-        const float4 pk = act ? d.posf[k] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const int uk = f0 + ib + l;                 // place in cell order
+        const int k = act ? d.cell_list[uk] : 0;
+        const int self = d.wsorted ? uk : k;
+        const float4 pk = act ? d.posf[uk] : make_float4(0.f, 0.f, 0.f, 0.f);
         const int molk = (MODE >= 1 && act) ? d.mol[k] : -1;
         // MODE 2: only atoms with a bonded partner outside their molecule search
         const bool xk = MODE == 2 && act && d.xmol[d.perm[k]] != 0;
@@ -434,9 +444,10 @@ __global__ void __launch_bounds__(256) k_fast_neigh_block(MtDev d, MtParams p, i
                         const int mid = (lo + hi) >> 1;
                         if (s_off[mid] <= i) lo = mid; else hi = mid;
                     }
-                    const int a = d.cell_list[s_first[lo] + (i - s_off[lo])];
-                    const float4 pa = d.posf[a];
-                    spos[i - base] = make_float4(pa.x, pa.y, pa.z, __uint_as_float((uint32_t)a));
+                    const int ui = s_first[lo] + (i - s_off[lo]);
+                    const float4 pa = d.posf[ui];
+                    const int a = (MODE >= 1 || !d.wsorted) ? d.cell_list[ui] : ui;
+                    spos[i - base] = make_float4(pa.x, pa.y, pa.z, __uint_as_float((uint32_t)(d.wsorted ? ui : a)));
                     if constexpr (MODE >= 1) smol[i - base] = d.mol[a];
                 }
                 __syncthreads();
@@ -462,11 +473,11 @@ __global__ void __launch_bounds__(256) k_fast_neigh_block(MtDev d, MtParams p, i
                                 const float dx1 = qx - c1.x, dy1 = qy - c1.y, dz1 = qz - c1.z;
                                 const bool h0 = dx0 * dx0 + dy0 * dy0 + dz0 * dz0 < rl2;
                                 const bool h1 = dx1 * dx1 + dy1 * dy1 + dz1 * dz1 < rl2;
-                                if (h0 && (MODE != 2 || !xk || !mt_is_partner(d, k, (int)__float_as_uint(c0.w))))
-                                    mt_neigh_hit<MODE == 2 ? 1 : MODE>(d, nl4, c0, code, k, molk, smol, e, n, ncap,
+                                if (h0 && (MODE != 2 || !xk || !mt_is_partner(d, k, mt_id_slot(d, (int)__float_as_uint(c0.w)))))
+                                    mt_neigh_hit<MODE == 2 ? 1 : MODE>(d, nl4, c0, code, k, self, molk, smol, e, n, ncap,
                                                                          cnt, b0, b1, b2);
-                                if (h1 && (MODE != 2 || !xk || !mt_is_partner(d, k, (int)__float_as_uint(c1.w))))
-                                    mt_neigh_hit<MODE == 2 ? 1 : MODE>(d, nl4, c1, code, k, molk, smol, e + 1, n,
+                                if (h1 && (MODE != 2 || !xk || !mt_is_partner(d, k, mt_id_slot(d, (int)__float_as_uint(c1.w)))))
+                                    mt_neigh_hit<MODE == 2 ? 1 : MODE>(d, nl4, c1, code, k, self, molk, smol, e + 1, n,
                                                                          ncap, cnt, b0, b1, b2);
                             }
                             if (e < e1) {
@@ -474,8 +485,8 @@ __global__ void __launch_bounds__(256) k_fast_neigh_block(MtDev d, MtParams p, i
                                 asm volatile("" : "+v"(c0.w));
                                 const float dx0 = qx - c0.x, dy0 = qy - c0.y, dz0 = qz - c0.z;
                                 if (dx0 * dx0 + dy0 * dy0 + dz0 * dz0 < rl2 &&
-                                    (MODE != 2 || !xk || !mt_is_partner(d, k, (int)__float_as_uint(c0.w))))
-                                    mt_neigh_hit<MODE == 2 ? 1 : MODE>(d, nl4, c0, code, k, molk, smol, e, n, ncap,
+                                    (MODE != 2 || !xk || !mt_is_partner(d, k, mt_id_slot(d, (int)__float_as_uint(c0.w)))))
+                                    mt_neigh_hit<MODE == 2 ? 1 : MODE>(d, nl4, c0, code, k, self, molk, smol, e, n, ncap,
                                                                          cnt, b0, b1, b2);
                             }
                         }
@@ -486,7 +497,7 @@ __global__ void __launch_bounds__(256) k_fast_neigh_block(MtDev d, MtParams p, i
             // last partial chunk, padded with the atom itself (offset code 0,0,0)
             const int slot = cnt & 3;
             if (slot && cnt < ncap) {
-                const uint32_t pad = (uint32_t)k | (NB_PAD_CODE << MT_NW_IDX_BITS);
+                const uint32_t pad = (uint32_t)self | (NB_PAD_CODE << MT_NW_IDX_BITS);
                 nl4[(size_t)(cnt >> 2) * n + k] = make_uint4(b0, slot > 1 ? b1 : pad, slot > 2 ? b2 : pad, pad);
             }
             d.ncount[k] = cnt;
@@ -524,8 +535,10 @@ __global__ void __launch_bounds__(256) k_fast_neigh_wide(MtDev d, MtParams p, in
     uint4* __restrict__ nl4 = reinterpret_cast<uint4*>(d.nlist);
     for (int ib = 0; ib < cnt_c; ib += 256) {
         const bool act = ib + t < cnt_c;
-        const int k = act ? d.cell_list[f0 + ib + t] : 0;
-        const float4 pk = act ? d.posf[k] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const int uk = f0 + ib + t;                 // place in cell order
+        const int k = act ? d.cell_list[uk] : 0;
+        const int self = d.wsorted ? uk : k;
+        const float4 pk = act ? d.posf[uk] : make_float4(0.f, 0.f, 0.f, 0.f);
         const int molk = (MODE >= 1 && act) ? d.mol[k] : -1;
         const bool xk = MODE == 2 && act && d.xmol[d.perm[k]] != 0;
         int cnt = 0;
@@ -546,9 +559,10 @@ __global__ void __launch_bounds__(256) k_fast_neigh_wide(MtDev d, MtParams p, in
                         const int m = min(NBW_CAP, gn - jb);
                         __syncthreads();
                         for (int e = t; e < m; e += 256) {
-                            const int a = d.cell_list[g0 + jb + e];
-                            const float4 pa = d.posf[a];
-                            spos[e] = make_float4(pa.x, pa.y, pa.z, __uint_as_float((uint32_t)a));
+                            const int ui = g0 + jb + e;
+                            const float4 pa = d.posf[ui];
+                            const int a = (MODE >= 1 || !d.wsorted) ? d.cell_list[ui] : ui;
+                            spos[e] = make_float4(pa.x, pa.y, pa.z, __uint_as_float((uint32_t)(d.wsorted ? ui : a)));
                             if constexpr (MODE >= 1) smol[e] = d.mol[a];
                         }
                         __syncthreads();
@@ -561,11 +575,11 @@ __global__ void __launch_bounds__(256) k_fast_neigh_wide(MtDev d, MtParams p, in
                             const float dx1 = qx - c1.x, dy1 = qy - c1.y, dz1 = qz - c1.z;
                             const bool h0 = dx0 * dx0 + dy0 * dy0 + dz0 * dz0 < rl2;
                             const bool h1 = dx1 * dx1 + dy1 * dy1 + dz1 * dz1 < rl2;
-                            if (h0 && (MODE != 2 || !xk || !mt_is_partner(d, k, (int)__float_as_uint(c0.w))))
-                                mt_neigh_hit<MODE == 2 ? 1 : MODE>(d, nl4, c0, code, k, molk, smol, e, n, ncap, cnt,
+                            if (h0 && (MODE != 2 || !xk || !mt_is_partner(d, k, mt_id_slot(d, (int)__float_as_uint(c0.w)))))
+                                mt_neigh_hit<MODE == 2 ? 1 : MODE>(d, nl4, c0, code, k, self, molk, smol, e, n, ncap, cnt,
                                                                      b0, b1, b2);
-                            if (h1 && (MODE != 2 || !xk || !mt_is_partner(d, k, (int)__float_as_uint(c1.w))))
-                                mt_neigh_hit<MODE == 2 ? 1 : MODE>(d, nl4, c1, code, k, molk, smol, e + 1, n, ncap,
+                            if (h1 && (MODE != 2 || !xk || !mt_is_partner(d, k, mt_id_slot(d, (int)__float_as_uint(c1.w)))))
+                                mt_neigh_hit<MODE == 2 ? 1 : MODE>(d, nl4, c1, code, k, self, molk, smol, e + 1, n, ncap,
                                                                      cnt, b0, b1, b2);
                         }
                         if (e < m) {
@@ -573,8 +587,8 @@ __global__ void __launch_bounds__(256) k_fast_neigh_wide(MtDev d, MtParams p, in
                             asm volatile("" : "+v"(c0.w));
                             const float dx0 = qx - c0.x, dy0 = qy - c0.y, dz0 = qz - c0.z;
                             if (dx0 * dx0 + dy0 * dy0 + dz0 * dz0 < rl2 &&
-                                (MODE != 2 || !xk || !mt_is_partner(d, k, (int)__float_as_uint(c0.w))))
-                                mt_neigh_hit<MODE == 2 ? 1 : MODE>(d, nl4, c0, code, k, molk, smol, e, n, ncap, cnt,
+                                (MODE != 2 || !xk || !mt_is_partner(d, k, mt_id_slot(d, (int)__float_as_uint(c0.w)))))
+                                mt_neigh_hit<MODE == 2 ? 1 : MODE>(d, nl4, c0, code, k, self, molk, smol, e, n, ncap, cnt,
                                                                      b0, b1, b2);
                         }
                     }
@@ -584,7 +598,7 @@ __global__ void __launch_bounds__(256) k_fast_neigh_wide(MtDev d, MtParams p, in
         if (act) {
             const int slot = cnt & 3;
             if (slot && cnt < ncap) {
-                const uint32_t pad = (uint32_t)k | (NB_PAD_CODE << MT_NW_IDX_BITS);
+                const uint32_t pad = (uint32_t)self | (NB_PAD_CODE << MT_NW_IDX_BITS);
                 nl4[(size_t)(cnt >> 2) * n + k] = make_uint4(b0, slot > 1 ? b1 : pad, slot > 2 ? b2 : pad, pad);
             }
             d.ncount[k] = cnt;
@@ -640,111 +654,176 @@ __device__ __forceinline__ void mt_gather4(const float4* posf, mt_u32x4 w, float
     a3 = posf[w.w & MT_NW_IDX_MASK];
 }
 
-// one MIXED pair: FP32 geometry on local coordinates, segment coefficients
-// (a_f, b_f[, a_e, b_e]) of the pair's summed tables, FP64 accumulation
-// The pair potentials are truncated at rc without a shift, so a pair within
-// FP32 rounding of rc would be counted or not depending on precision: in a
-// narrow band around rc the inside test is taken in FP64 with the FP64
-// engine's (the reference's) predicate, from the double positions.
+// force (and half energy) of one MIXED pair inside rc from its FP32
+// separation: segment coefficients (a_f, b_f[, a_e, b_e]) of the pair's summed
+// tables, plus the cutoff Coulomb term
 template <bool WANT_E, bool COUL, typename CoefT>
-__device__ __forceinline__ void mt_mixed_pair(uint32_t w, const float4& pa, const float4& pk, const float4* sh,
-                                              const CoefT* row, float rc2, float x0, float invd, float qk,
-                                              const float* qf, float& Fx, float& Fy, float& Fz, float& U,
-                                              const MtDev& d, const MtParams& p, int k)
+__device__ __forceinline__ void mt_mixed_force(float dx, float dy, float dz, float r2, const float4& pa, float qa,
+                                               const CoefT* row, float x0, float invd, float qk, float& Fx,
+                                               float& Fy, float& Fz, float& U)
 {
+    if (!(r2 >= 1.0e-6f)) return;
+    const float rinv = __builtin_amdgcn_rsqf(r2);
+    const float r = r2 * rinv;
+    float fr = 0.0f;
+    const int idx = (int)((r - x0) * invd);
+    if (idx >= 0 && idx < MT_NPTS - 1) {
+        // type indices are small: a 24-bit multiply (full rate) forms the row offset
+        const CoefT c = row[__umul24(__float_as_uint(pa.w), MT_NPTS) + idx];
+        fr = c.x + c.y * rinv;
+        if constexpr (WANT_E) U += 0.5f * (c.z * r + c.w);
+    }
+    if (COUL) {
+        const float K = MT_COULOMB_K * qk * qa;
+        fr += K * rinv * rinv * rinv;
+        if (WANT_E) U += 0.5f * K * rinv;
+    }
+    Fx = fmaf(dx, fr, Fx);
+    Fy = fmaf(dy, fr, Fy);
+    Fz = fmaf(dz, fr, Fz);
+}
+
+// one MIXED pair: FP32 geometry on local coordinates, FP32 force. The pair
+// potentials are truncated at rc without a shift, so a pair within FP32
+// rounding of rc would be counted or not depending on precision. A pair in a
+// narrow band around rc is therefore left out here and the call returns true;
+// mt_mixed_band later takes its inside test in FP64 with the FP64 engine's
+// (the reference's) predicate, after the chunk's other pairs, so the double
+// positions are loaded only for those rare pairs.
+template <bool WANT_E, bool COUL, typename CoefT>
+__device__ __forceinline__ bool mt_mixed_pair_o(const float4& pa, const float4& o, float qa, const float4& pk,
+                                                const CoefT* row, float rc2, float x0, float invd, float qk,
+                                                float& Fx, float& Fy, float& Fz, float& U)
+{
+    const float dx = pk.x - pa.x - o.x;
+    const float dy = pk.y - pa.y - o.y;
+    const float dz = pk.z - pa.z - o.z;
+    const float r2 = dx * dx + dy * dy + dz * dz;
+    const bool band = fabsf(r2 - rc2) < 2.0e-5f * rc2;
+    if (r2 < rc2 && !band) mt_mixed_force<WANT_E, COUL>(dx, dy, dz, r2, pa, qa, row, x0, invd, qk, Fx, Fy, Fz, U);
+    return band;
+}
+
+// the same with the shift of the word's offset code and the charge from
+// global memory
+template <bool WANT_E, bool COUL, typename CoefT>
+__device__ __forceinline__ bool mt_mixed_pair(uint32_t w, const float4& pa, const float4& pk, const float4* sh,
+                                              const CoefT* row, float rc2, float x0, float invd, float qk,
+                                              const float* qf, float& Fx, float& Fy, float& Fz, float& U)
+{
+    const float qa = COUL ? qf[w & MT_NW_IDX_MASK] : 0.0f;
+    return mt_mixed_pair_o<WANT_E, COUL>(pa, sh[w >> MT_NW_IDX_BITS], qa, pk, row, rc2, x0, invd, qk, Fx, Fy, Fz,
+                                         U);
+}
+
+// a band pair of mt_mixed_pair: FP64 inside test from the double positions,
+// then the same FP32 force
+template <bool WANT_E, bool COUL, typename CoefT>
+__device__ __forceinline__ void mt_mixed_band(uint32_t w, const float4& pk, const float4* sh, const CoefT* row,
+                                           float x0, float invd, float qk, float& Fx, float& Fy, float& Fz,
+                                           float& U, const MtDev& d, const MtParams& p, int k)
+{
+    const int u = (int)(w & MT_NW_IDX_MASK);          // place in cell order
+    const double4 rk4 = d.pos[k], ra4 = d.pos[d.cell_list[u]];
+    if (!(mt_dist2_pbc({rk4.x, rk4.y, rk4.z}, {ra4.x, ra4.y, ra4.z}, p.wnl) < (double)p.rc2f)) return;
+    const float4 pa = d.posf[u];
     const float4 o = sh[w >> MT_NW_IDX_BITS];
     const float dx = pk.x - pa.x - o.x;
     const float dy = pk.y - pa.y - o.y;
     const float dz = pk.z - pa.z - o.z;
     const float r2 = dx * dx + dy * dy + dz * dz;
-    bool inside = r2 < rc2;
-    if (fabsf(r2 - rc2) < 2.0e-5f * rc2) {
-        const double4 rk4 = d.pos[k], ra4 = d.pos[w & MT_NW_IDX_MASK];
-        inside = mt_dist2_pbc({rk4.x, rk4.y, rk4.z}, {ra4.x, ra4.y, ra4.z}, p.wnl) < (double)p.rc2f;
-    }
-    if (inside && r2 >= 1.0e-6f) {
-        const float rinv = __builtin_amdgcn_rsqf(r2);
-        const float r = r2 * rinv;
-        float fr = 0.0f;
-        const int idx = (int)((r - x0) * invd);
-        if (idx >= 0 && idx < MT_NPTS - 1) {
-            const CoefT c = row[__float_as_int(pa.w) * MT_NPTS + idx];
-            fr = c.x + c.y * rinv;
-            if constexpr (WANT_E) U += 0.5f * (c.z * r + c.w);
-        }
-        if (COUL) {
-            const float K = MT_COULOMB_K * qk * qf[w & MT_NW_IDX_MASK];
-            fr += K * rinv * rinv * rinv;
-            if (WANT_E) U += 0.5f * K * rinv;
-        }
-        Fx = fmaf(dx, fr, Fx);
-        Fy = fmaf(dy, fr, Fy);
-        Fz = fmaf(dz, fr, Fz);
-    }
+    mt_mixed_force<WANT_E, COUL>(dx, dy, dz, r2, pa, COUL ? d.qfs[u] : 0.0f, row, x0, invd, qk, Fx, Fy, Fz, U);
 }
 
-// the pair loop of one atom over its chunked list: four gathers in flight
-// together, and the next chunk's words loaded while the current chunk is
-// computed
+// the pair loop of one atom over chunks q0, q0 + qs, ... of its chunked list
+// (qs lanes share an atom when the system is too small to fill the GPU with
+// one lane per atom): four gathers in flight together, and the next chunk's
+// words loaded while the current chunk is computed
 template <bool WANT_E, bool COUL, typename CoefT>
 __device__ __forceinline__ void mt_mixed_atom(const MtDev& d, const MtParams& p, int k, const float4& pk,
                                               const float4* sh, const CoefT* row, float qk, double& Fx,
-                                              double& Fy, double& Fz, float& U)
+                                              double& Fy, double& Fz, float& U, int q0, int qs)
 {
     const int nch = (min(d.ncount[k], d.ncap) + 3) >> 2;
-    if (nch == 0) return;
+    if (q0 >= nch) return;
     const mt_u32x4* __restrict__ nl4 = reinterpret_cast<const mt_u32x4*>(d.nlist) + k;
     const size_t stride = (size_t)d.n;
     const float rc2 = p.rc2f, x0 = p.x0, invd = p.inv_delta;
-    mt_u32x4 wn = __builtin_nontemporal_load(nl4);
-    for (int q = 0; q < nch; q++) {
+    mt_u32x4 wn = __builtin_nontemporal_load(nl4 + (size_t)q0 * stride);
+    for (int q = q0; q < nch; q += qs) {
         const mt_u32x4 w = wn;
         float4 a0, a1, a2, a3;
         mt_gather4(d.posf, w, a0, a1, a2, a3);
-        // unconditional (the last iteration reloads its own chunk) so the
-        // load needs no branch and stays in flight across the pair math
-        wn = __builtin_nontemporal_load(nl4 + (size_t)min(q + 1, nch - 1) * stride);
+        // unconditional (the last iteration reloads a chunk of this atom's
+        // list) so the load needs no branch and stays in flight across the
+        // pair math
+        wn = __builtin_nontemporal_load(nl4 + (size_t)min(q + qs, nch - 1) * stride);
         // the empty asm makes the four whole 16-byte gathers complete here
         // (while the next chunk's load stays in flight), so the type bits in
         // .w are not fetched by a second, dependent load inside a pair branch
         asm volatile("" : "+v"(a0.w), "+v"(a1.w), "+v"(a2.w), "+v"(a3.w));
         // the four pair forces of a chunk are summed in FP32, the chunk sums in FP64
         float cx = 0.f, cy = 0.f, cz = 0.f;
-        mt_mixed_pair<WANT_E, COUL>(w.x, a0, pk, sh, row, rc2, x0, invd, qk, d.qf, cx, cy, cz, U, d, p, k);
-        mt_mixed_pair<WANT_E, COUL>(w.y, a1, pk, sh, row, rc2, x0, invd, qk, d.qf, cx, cy, cz, U, d, p, k);
-        mt_mixed_pair<WANT_E, COUL>(w.z, a2, pk, sh, row, rc2, x0, invd, qk, d.qf, cx, cy, cz, U, d, p, k);
-        mt_mixed_pair<WANT_E, COUL>(w.w, a3, pk, sh, row, rc2, x0, invd, qk, d.qf, cx, cy, cz, U, d, p, k);
+        const bool b0 = mt_mixed_pair<WANT_E, COUL>(w.x, a0, pk, sh, row, rc2, x0, invd, qk, d.qfs, cx, cy, cz, U);
+        const bool b1 = mt_mixed_pair<WANT_E, COUL>(w.y, a1, pk, sh, row, rc2, x0, invd, qk, d.qfs, cx, cy, cz, U);
+        const bool b2 = mt_mixed_pair<WANT_E, COUL>(w.z, a2, pk, sh, row, rc2, x0, invd, qk, d.qfs, cx, cy, cz, U);
+        const bool b3 = mt_mixed_pair<WANT_E, COUL>(w.w, a3, pk, sh, row, rc2, x0, invd, qk, d.qfs, cx, cy, cz, U);
+        if (b0 | b1 | b2 | b3) {
+            if (b0) mt_mixed_band<WANT_E, COUL>(w.x, pk, sh, row, x0, invd, qk, cx, cy, cz, U, d, p, k);
+            if (b1) mt_mixed_band<WANT_E, COUL>(w.y, pk, sh, row, x0, invd, qk, cx, cy, cz, U, d, p, k);
+            if (b2) mt_mixed_band<WANT_E, COUL>(w.z, pk, sh, row, x0, invd, qk, cx, cy, cz, U, d, p, k);
+            if (b3) mt_mixed_band<WANT_E, COUL>(w.w, pk, sh, row, x0, invd, qk, cx, cy, cz, U, d, p, k);
+        }
         Fx += (double)cx;
         Fy += (double)cy;
         Fz += (double)cz;
     }
 }
 
-// MIXED with energies: coefficients (a_f, b_f, a_e, b_e) read from global memory
-template <bool WANT_E, bool COUL>
+// sum over the G consecutive lanes that share an atom (butterfly, fixed
+// order, so every run gives the same bits); every lane ends with the total
+template <int G>
+__device__ __forceinline__ void mt_lane_sum(double& x, double& y, double& z, float& u)
+{
+#pragma unroll
+    for (int m = 1; m < G; m <<= 1) {
+        x += __shfl_xor(x, m);
+        y += __shfl_xor(y, m);
+        z += __shfl_xor(z, m);
+        u += __shfl_xor(u, m);
+    }
+}
+
+// MIXED with energies: coefficients (a_f, b_f, a_e, b_e) read from global
+// memory; G lanes per atom
+template <bool WANT_E, bool COUL, int G>
 __global__ void __launch_bounds__(512) k_fast_force_mixed(MtDev d, MtParams p)
 {
     __shared__ float4 sh[27];
     mt_fill_shifts(sh, p);
     __syncthreads();
-    int k = mt_xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (k >= d.n) return;
-    const float4 pk = d.posf[k];
+    const int t = mt_xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const int k = t / G, g = t % G;
+    if (k >= d.n) return;                       // all G lanes of an atom leave together
+    const float4 pk = d.posf[d.sidx[k]];
     const float4* __restrict__ row = d.nbcoef + (size_t)__float_as_int(pk.w) * d.T * MT_NPTS;
     const float qk = COUL ? d.qf[k] : 0.0f;
     double Fx = 0.0, Fy = 0.0, Fz = 0.0;
     float U = 0.0f;
-    mt_mixed_atom<WANT_E, COUL>(d, p, k, pk, sh, row, qk, Fx, Fy, Fz, U);
+    mt_mixed_atom<WANT_E, COUL>(d, p, k, pk, sh, row, qk, Fx, Fy, Fz, U, g, G);
+    mt_lane_sum<G>(Fx, Fy, Fz, U);
+    if (g != 0) return;
     if (!d.mobile[k]) { Fx = Fy = Fz = 0.0; U = 0.0f; }
     d.frc[k] = make_double4(Fx, Fy, Fz, 0.0);
     if (WANT_E) d.epot[k] = U;
 }
 
 // MIXED with the force-only segment coefficients (a_f, b_f) of every type pair
-// staged in LDS once per workgroup; each workgroup owns one contiguous range
-// of atoms (ranges XCD-contiguous). Used when T^2 * 1000 * 8 bytes fits the
-// LDS budget and energies are not needed.
-template <bool COUL>
+// staged in LDS once per workgroup. The grid holds as many workgroups as can
+// be resident at once; workgroup w owns atoms [w n / g, (w+1) n / g), and the
+// ranges are XCD-contiguous. G lanes per atom. Used when T^2 * 1000 * 8 bytes
+// fits the LDS budget and energies are not needed.
+template <bool COUL, int G>
 __global__ void __launch_bounds__(512) k_fast_force_mixed_lds(MtDev d, MtParams p)
 {
     extern __shared__ float2 stab[];
@@ -755,27 +834,203 @@ __global__ void __launch_bounds__(512) k_fast_force_mixed_lds(MtDev d, MtParams 
     mt_fill_shifts(sh, p);
     __syncthreads();
     const int lw = mt_xcd_block(blockIdx.x, gridDim.x);
-    const int chunk = ((d.n + gridDim.x - 1) / gridDim.x + blockDim.x - 1) / blockDim.x * blockDim.x;
-    const int kb = lw * chunk, ke = min(d.n, kb + chunk);
-    for (int k = kb + threadIdx.x; k < ke; k += blockDim.x) {
-        const float4 pk = d.posf[k];
+    const int kb = (int)((long long)lw * d.n / gridDim.x), ke = (int)((long long)(lw + 1) * d.n / gridDim.x);
+    const int g = threadIdx.x % G;
+    for (int k = kb + (int)threadIdx.x / G; k < ke; k += blockDim.x / G) {
+        const float4 pk = d.posf[d.sidx[k]];
         const float2* __restrict__ row = stab + __float_as_int(pk.w) * T * MT_NPTS;
         const float qk = COUL ? d.qf[k] : 0.0f;
         double Fx = 0.0, Fy = 0.0, Fz = 0.0;
         float U = 0.0f;
-        mt_mixed_atom<false, COUL>(d, p, k, pk, sh, row, qk, Fx, Fy, Fz, U);
-        if (!d.mobile[k]) { Fx = Fy = Fz = 0.0; }
-        d.frc[k] = make_double4(Fx, Fy, Fz, 0.0);
+        mt_mixed_atom<false, COUL>(d, p, k, pk, sh, row, qk, Fx, Fy, Fz, U, g, G);
+        mt_lane_sum<G>(Fx, Fy, Fz, U);
+        if (g == 0) {
+            if (!d.mobile[k]) { Fx = Fy = Fz = 0.0; }
+            d.frc[k] = make_double4(Fx, Fy, Fz, 0.0);
+        }
+    }
+}
+
+// MIXED pair forces with neighbour-cell tiles in LDS. One workgroup per
+// 2x2x2 block of home cells (the list build's blocking). The positions (and
+// charges) of the 4x4x4 cells around the block, the union of the eight
+// 27-cell neighbourhoods, are staged once in LDS in cell order; the list
+// words carry an atom's place u in cell order, so the neighbour of a word
+// with offset code c sits in LDS at u + delta[h][c] for home cell h, where
+// delta maps the staged cell's first cell-order index to its LDS offset
+// (kept in .w of the per-home-cell shift table). Each pair then costs LDS
+// reads instead of a scattered 16-byte global load, whose L1 tag lookup (one
+// per lane) is what bounds the global-gather kernel. A workgroup whose staged
+// cells hold more than FT_CAP atoms gathers from global memory instead.
+// G lanes per home atom.
+#define FT_CAP 2048
+#ifndef MT_TILE_DEPTH
+#define MT_TILE_DEPTH 1                               // list chunks in flight per lane
+#endif
+template <bool COUL, bool FROM_LDS>
+__device__ __forceinline__ void mt_tile_atom(const MtDev& d, const MtParams& p, int k, const float4& pk,
+                                             const float4* shd, const float4* spos, const float* sq,
+                                             const float2* row, float qk, double& Fx, double& Fy, double& Fz,
+                                             int q0, int qs)
+{
+    const int nch = (min(d.ncount[k], d.ncap) + 3) >> 2;
+    if (q0 >= nch) return;
+    const mt_u32x4* __restrict__ nl4 = reinterpret_cast<const mt_u32x4*>(d.nlist) + k;
+    const size_t stride = (size_t)d.n;
+    const float rc2 = p.rc2f, x0 = p.x0, invd = p.inv_delta;
+    float U = 0.0f;
+    mt_u32x4 wn[MT_TILE_DEPTH];
+#pragma unroll
+    for (int i = 0; i < MT_TILE_DEPTH; i++)
+        wn[i] = __builtin_nontemporal_load(nl4 + (size_t)min(q0 + i * qs, nch - 1) * stride);
+    for (int q = q0; q < nch; q += qs) {
+        const mt_u32x4 w = wn[0];
+        const float4 o0 = shd[w.x >> MT_NW_IDX_BITS], o1 = shd[w.y >> MT_NW_IDX_BITS];
+        const float4 o2 = shd[w.z >> MT_NW_IDX_BITS], o3 = shd[w.w >> MT_NW_IDX_BITS];
+        const int i0 = (int)(w.x & MT_NW_IDX_MASK), i1 = (int)(w.y & MT_NW_IDX_MASK);
+        const int i2 = (int)(w.z & MT_NW_IDX_MASK), i3 = (int)(w.w & MT_NW_IDX_MASK);
+        float4 a0, a1, a2, a3;
+        float c0 = 0.f, c1 = 0.f, c2 = 0.f, c3 = 0.f;
+        if constexpr (FROM_LDS) {
+            const int l0 = i0 + __float_as_int(o0.w), l1 = i1 + __float_as_int(o1.w);
+            const int l2 = i2 + __float_as_int(o2.w), l3 = i3 + __float_as_int(o3.w);
+            a0 = spos[l0]; a1 = spos[l1]; a2 = spos[l2]; a3 = spos[l3];
+            if constexpr (COUL) { c0 = sq[l0]; c1 = sq[l1]; c2 = sq[l2]; c3 = sq[l3]; }
+        } else {
+            a0 = d.posf[i0]; a1 = d.posf[i1]; a2 = d.posf[i2]; a3 = d.posf[i3];
+            if constexpr (COUL) { c0 = d.qfs[i0]; c1 = d.qfs[i1]; c2 = d.qfs[i2]; c3 = d.qfs[i3]; }
+        }
+        // the next chunks' words stay in flight across the pair math
+#pragma unroll
+        for (int i = 0; i + 1 < MT_TILE_DEPTH; i++) wn[i] = wn[i + 1];
+        wn[MT_TILE_DEPTH - 1] = __builtin_nontemporal_load(nl4 + (size_t)min(q + MT_TILE_DEPTH * qs, nch - 1) * stride);
+        float cx = 0.f, cy = 0.f, cz = 0.f;
+        const bool b0 = mt_mixed_pair_o<false, COUL>(a0, o0, c0, pk, row, rc2, x0, invd, qk, cx, cy, cz, U);
+        const bool b1 = mt_mixed_pair_o<false, COUL>(a1, o1, c1, pk, row, rc2, x0, invd, qk, cx, cy, cz, U);
+        const bool b2 = mt_mixed_pair_o<false, COUL>(a2, o2, c2, pk, row, rc2, x0, invd, qk, cx, cy, cz, U);
+        const bool b3 = mt_mixed_pair_o<false, COUL>(a3, o3, c3, pk, row, rc2, x0, invd, qk, cx, cy, cz, U);
+        if (b0 | b1 | b2 | b3) {
+            if (b0) mt_mixed_band<false, COUL>(w.x, pk, shd, row, x0, invd, qk, cx, cy, cz, U, d, p, k);
+            if (b1) mt_mixed_band<false, COUL>(w.y, pk, shd, row, x0, invd, qk, cx, cy, cz, U, d, p, k);
+            if (b2) mt_mixed_band<false, COUL>(w.z, pk, shd, row, x0, invd, qk, cx, cy, cz, U, d, p, k);
+            if (b3) mt_mixed_band<false, COUL>(w.w, pk, shd, row, x0, invd, qk, cx, cy, cz, U, d, p, k);
+        }
+        Fx += (double)cx;
+        Fy += (double)cy;
+        Fz += (double)cz;
+    }
+}
+
+template <bool COUL, int G>
+__global__ void __launch_bounds__(256 * G) k_fast_force_tile(MtDev d, MtParams p)
+{
+    extern __shared__ float2 stab[];                  // T * T * MT_NPTS segment coefficients
+    __shared__ float4 spos[FT_CAP];
+    __shared__ float sq_buf[COUL ? FT_CAP : 1];
+    __shared__ float4 shd[8 * 27];                    // per home cell: shift xyz, LDS delta (.w bits)
+    __shared__ int s_first[64], s_cnt[64], s_off[65];
+    __shared__ int s_hfirst[8], s_hoff[9];
+    const int t = threadIdx.x;
+    const int T = d.T;
+    const int ntab = T * T * MT_NPTS;
+    for (int i = t; i < ntab; i += blockDim.x) stab[i] = d.nbcoef2[i];
+    const int Nx = d.nc[0], Ny = d.nc[1], Nz = d.nc[2];
+    const int bx = (Nx + 1) >> 1, by = (Ny + 1) >> 1;
+    const int g = mt_xcd_block(blockIdx.x, gridDim.x);
+    const int x0 = 2 * (g % bx), y0 = 2 * ((g / bx) % by), z0 = 2 * (g / (bx * by));
+    if (t < 64) {
+        // staged cell q = (ox+1) + 4*((oy+1) + 4*(oz+1)), o* in -1..2 around (x0, y0, z0)
+        const int ox = (t & 3) - 1, oy = ((t >> 2) & 3) - 1, oz = (t >> 4) - 1;
+        int cx = x0 + ox, cy = y0 + oy, cz = z0 + oz;
+        cx = cx < 0 ? cx + Nx : (cx >= Nx ? cx - Nx : cx);
+        cy = cy < 0 ? cy + Ny : (cy >= Ny ? cy - Ny : cy);
+        cz = cz < 0 ? cz + Nz : (cz >= Nz ? cz - Nz : cz);
+        const int cc = cx + Nx * (cy + Ny * cz);
+        const int cn = d.cell_count[cc];
+        s_first[t] = d.cell_first[cc];
+        s_cnt[t] = cn;
+        int v = cn;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int u = __shfl_up(v, o);
+            if (t >= o) v += u;
+        }
+        s_off[t + 1] = v;
+        if (t == 0) s_off[0] = 0;
+        // home cells (the first eight lanes): counts scanned across them
+        int hn = 0, hf = 0;
+        if (t < 8) {
+            const int hx = x0 + (t & 1), hy = y0 + ((t >> 1) & 1), hz = z0 + (t >> 2);
+            if (hx < Nx && hy < Ny && hz < Nz) {
+                const int hcell = hx + Nx * (hy + Ny * hz);
+                hn = d.cell_count[hcell];
+                hf = d.cell_first[hcell];
+            }
+        }
+        int hv = hn;
+        for (int o = 1; o < 8; o <<= 1) {
+            const int u = __shfl_up(hv, o);
+            if (t >= o) hv += u;
+        }
+        if (t < 8) {
+            s_hfirst[t] = hf;
+            s_hoff[t + 1] = hv;
+            if (t == 0) s_hoff[0] = 0;
+        }
+    }
+    __syncthreads();
+    for (int e = t; e < 8 * 27; e += blockDim.x) {
+        const int h = e / 27, c = e % 27;
+        const int ox = c % 3 - 1, oy = (c / 3) % 3 - 1, oz = c / 9 - 1;
+        const int q = (ox + 1 + (h & 1)) + 4 * ((oy + 1 + ((h >> 1) & 1)) + 4 * (oz + 1 + (h >> 2)));
+        shd[e] = make_float4((float)ox * p.cs[0], (float)oy * p.cs[1], (float)oz * p.cs[2],
+                             __int_as_float(s_off[q] - s_first[q]));
+    }
+    const int total = s_off[64];
+    const bool lds = total <= FT_CAP;                 // uniform over the workgroup
+    if (lds) {
+        for (int i = t; i < total; i += blockDim.x) {
+            int lo = 0, hi = 64;                       // staged cell of item i
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (s_off[mid] <= i) lo = mid; else hi = mid;
+            }
+            const int u = s_first[lo] + (i - s_off[lo]);
+            spos[i] = d.posf[u];
+            if constexpr (COUL) sq_buf[i] = d.qfs[u];
+        }
+    }
+    __syncthreads();
+    const int nh = s_hoff[8];
+    const int gl = t % G;
+    for (int v = t / G; v < nh; v += (int)blockDim.x / G) {
+        int h = 0;
+        while (h < 7 && v >= s_hoff[h + 1]) h++;
+        const int u = s_hfirst[h] + (v - s_hoff[h]);
+        const int k = d.cell_list[u];
+        const float4 pk = d.posf[u];
+        const float2* __restrict__ row = stab + __float_as_int(pk.w) * T * MT_NPTS;
+        const float qk = COUL ? d.qf[k] : 0.0f;
+        double Fx = 0.0, Fy = 0.0, Fz = 0.0;
+        if (lds) mt_tile_atom<COUL, true>(d, p, k, pk, shd + 27 * h, spos, sq_buf, row, qk, Fx, Fy, Fz, gl, G);
+        else mt_tile_atom<COUL, false>(d, p, k, pk, shd + 27 * h, spos, sq_buf, row, qk, Fx, Fy, Fz, gl, G);
+        float U = 0.0f;
+        mt_lane_sum<G>(Fx, Fy, Fz, U);
+        if (gl == 0) {
+            if (!d.mobile[k]) { Fx = Fy = Fz = 0.0; }
+            d.frc[k] = make_double4(Fx, Fy, Fz, 0.0);
+        }
     }
 }
 
 // FP64: the reference's per-pair arithmetic (FunctorCalcForce.cpp:467-535)
-// on the skin list, restricted to pairs inside rc by the reference predicate
-template <bool WANT_E, bool COUL>
+// on the skin list, restricted to pairs inside rc by the reference predicate.
+// G lanes per atom: lane g takes the list's 4-word chunks g, g + G, ...
+template <bool WANT_E, bool COUL, int G>
 __global__ void __launch_bounds__(256) k_fast_force_fp64(MtDev d, MtParams p)
 {
-    int k = mt_xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (k >= d.n) return;
+    const int t = mt_xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const int k = t / G, g = t % G;
+    if (k >= d.n) return;                       // all G lanes of an atom leave together
     const double4 rk4 = d.pos[k];
     const mt_d3 rk = {rk4.x, rk4.y, rk4.z};
     const int row = d.type[k], T = d.T;
@@ -783,7 +1038,7 @@ __global__ void __launch_bounds__(256) k_fast_force_fp64(MtDev d, MtParams p)
     mt_d3 F = {0.0, 0.0, 0.0};
     float U = 0.0f;
     const int cnt = min(d.ncount[k], d.ncap);
-    for (int j = 0; j < cnt; j++) {
+    for (int j = 4 * g; j < cnt; j += ((j & 3) == 3) ? 4 * G - 3 : 1) {
         const int a = (int)(mt_fast_word(d.nlist, d.n, k, j) & MT_NW_IDX_MASK);
         const double4 ra4 = d.pos[a];
         const mt_d3 ra = {ra4.x, ra4.y, ra4.z};
@@ -810,6 +1065,8 @@ __global__ void __launch_bounds__(256) k_fast_force_fp64(MtDev d, MtParams p)
         }
         if (WANT_E) U += 0.5f * u;
     }
+    mt_lane_sum<G>(F.x, F.y, F.z, U);
+    if (g != 0) return;
     if (!d.mobile[k]) { F = {0.0, 0.0, 0.0}; U = 0.0f; }
     d.frc[k] = make_double4(F.x, F.y, F.z, 0.0);
     if (WANT_E) d.epot[k] = U;
@@ -1059,43 +1316,123 @@ __global__ void k_ap_sum(MtDev d, int nsplit, int want_e)
     if (want_e) d.epot[k] += (float)u;
 }
 
+// Lanes per atom for the pair-force kernels: one while the atoms alone fill
+// the GPU (32 waves per CU), else 4 or 16 so that the lanes share out the
+// atoms' list chunks, keeping at least ~4 chunks per lane. The choice depends
+// only on n, the list capacity and the CU count, so it is fixed for a run of
+// steps (the per-atom sums, and so the bits, depend on it).
+static int mt_lanes_per_atom(int n, int ncap, int ncu)
+{
+    const long long fill = (long long)ncu * 2048;
+    const int chunks = (ncap + 3) / 4;
+    if ((long long)n >= fill || chunks < 16) return 1;
+    if ((long long)n * 4 >= fill || chunks < 64) return 4;
+    return 16;
+}
+
+static int mt_cu_count()
+{
+    static int ncu = 0;
+    if (ncu == 0) {
+        int dev = 0, v = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0)
+            ncu = v;
+        else
+            return 256;
+    }
+    return ncu;
+}
+
+#ifndef MT_FORCE_ITERS
+#define MT_FORCE_ITERS 2
+#endif
+template <bool COUL, int G>
+static void mt_launch_mixed_lds(const MtDev& d, const MtParams& p, size_t lds_tab, hipStream_t s)
+{
+    const int bs = 512;
+    // each workgroup takes MT_FORCE_ITERS passes of bs / G atoms; the LDS
+    // table is loaded once per workgroup
+    const long long need = ((long long)d.n * G + bs - 1) / bs;
+    const long long per = (need + MT_FORCE_ITERS - 1) / MT_FORCE_ITERS;
+    // a multiple of 8 workgroups so every XCD gets the same share
+    const int g = (int)std::max<long long>(8, (per + 7) / 8 * 8);
+    hipLaunchKernelGGL((k_fast_force_mixed_lds<COUL, G>), dim3(g), dim3(bs), lds_tab, s, d, p);
+}
+
+template <bool WANT_E, bool COUL, int G>
+static void mt_launch_mixed(const MtDev& d, const MtParams& p, int bs, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_fast_force_mixed<WANT_E, COUL, G>), dim3(nblk((int)((long long)d.n * G), bs)), dim3(bs), 0,
+                       s, d, p);
+}
+
+template <bool WANT_E, bool COUL, int G>
+static void mt_launch_fp64(const MtDev& d, const MtParams& p, int bs, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_fast_force_fp64<WANT_E, COUL, G>), dim3(nblk((int)((long long)d.n * G), bs)), dim3(bs), 0, s,
+                       d, p);
+}
+
+#ifndef MT_TILE_G
+#define MT_TILE_G 2
+#endif
+// the LDS-tiled MIXED kernel: force-only passes of systems with sparse cells
+// (the block list build's regime), the coefficient table within 32 KB of
+// LDS, and enough 2x2x2 cell blocks to fill the GPU
+static bool mt_use_tile(const MtDev& d, int mixed, int want_e, size_t lds_tab)
+{
+    if (!mixed || want_e || !d.wsorted || lds_tab > 32768 || d.n > 40 * d.ncells) return false;
+    const long long nb = (long long)((d.nc[0] + 1) / 2) * ((d.nc[1] + 1) / 2) * ((d.nc[2] + 1) / 2);
+    return nb >= 2LL * mt_cu_count();
+}
+
+template <int G>
+static void mt_launch_pairs(const MtDev& d, const MtParams& p, int mixed, int want_e, bool coul, int block,
+                            size_t lds_tab, hipStream_t s)
+{
+    if (mt_use_tile(d, mixed, want_e, lds_tab)) {
+        const int nb = ((d.nc[0] + 1) / 2) * ((d.nc[1] + 1) / 2) * ((d.nc[2] + 1) / 2);
+        if (coul) hipLaunchKernelGGL((k_fast_force_tile<true, MT_TILE_G>), dim3(nb), dim3(256 * MT_TILE_G), lds_tab, s, d, p);
+        else hipLaunchKernelGGL((k_fast_force_tile<false, MT_TILE_G>), dim3(nb), dim3(256 * MT_TILE_G), lds_tab, s, d, p);
+        return;
+    }
+    if (mixed && !want_e && lds_tab <= 65536) {
+        if (coul) mt_launch_mixed_lds<true, G>(d, p, lds_tab, s);
+        else mt_launch_mixed_lds<false, G>(d, p, lds_tab, s);
+    } else if (mixed) {
+        const int bs = (block == 64 || block == 128 || block == 256 || block == 512) ? block : 256;
+        if (want_e) {
+            if (coul) mt_launch_mixed<true, true, G>(d, p, bs, s);
+            else mt_launch_mixed<true, false, G>(d, p, bs, s);
+        } else {
+            if (coul) mt_launch_mixed<false, true, G>(d, p, bs, s);
+            else mt_launch_mixed<false, false, G>(d, p, bs, s);
+        }
+    } else {
+        const int bs = (block == 64 || block == 128 || block == 256) ? block : 256;
+        if (want_e) {
+            if (coul) mt_launch_fp64<true, true, G>(d, p, bs, s);
+            else mt_launch_fp64<true, false, G>(d, p, bs, s);
+        } else {
+            if (coul) mt_launch_fp64<false, true, G>(d, p, bs, s);
+            else mt_launch_fp64<false, false, G>(d, p, bs, s);
+        }
+    }
+}
+
+int mt_fast_lanes(const MtDev& d) { return mt_lanes_per_atom(d.n, d.ncap, mt_cu_count()); }
+
 void mt_fast_forces(const MtDev& d, const MtParams& p, int mixed, int want_e, int block, hipStream_t s)
 {
     const bool coul_all = p.coulomb_on != 0;
     const bool ap = coul_all && p.coulomb_allpairs;
     const bool coul = coul_all && !ap;      // Coulomb inside the list kernels
     const size_t lds_tab = (size_t)d.T * d.T * MT_NPTS * sizeof(float2);
-    if (mixed && !want_e && lds_tab <= 65536) {
-        // persistent-style grid: enough workgroups to fill every CU a few times
-        int dev = 0, ncu = 256;
-        if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-        const int bs = 512;
-        const int per_cu = lds_tab > 32768 ? 2 : 4;
-        // a multiple of 8 workgroups so every XCD gets the same share
-        const int g = std::max(8, std::min(nblk(d.n, bs), ncu * per_cu) / 8 * 8);
-        if (coul) hipLaunchKernelGGL((k_fast_force_mixed_lds<true>), dim3(g), dim3(bs), lds_tab, s, d, p);
-        else hipLaunchKernelGGL((k_fast_force_mixed_lds<false>), dim3(g), dim3(bs), lds_tab, s, d, p);
-    } else if (mixed) {
-        int bs = (block == 64 || block == 128 || block == 256 || block == 512) ? block : 256;
-        dim3 g(nblk(d.n, bs)), b(bs);
-        if (want_e) {
-            if (coul) hipLaunchKernelGGL((k_fast_force_mixed<true, true>), g, b, 0, s, d, p);
-            else hipLaunchKernelGGL((k_fast_force_mixed<true, false>), g, b, 0, s, d, p);
-        } else {
-            if (coul) hipLaunchKernelGGL((k_fast_force_mixed<false, true>), g, b, 0, s, d, p);
-            else hipLaunchKernelGGL((k_fast_force_mixed<false, false>), g, b, 0, s, d, p);
-        }
-    } else {
-        int bs = (block == 64 || block == 128 || block == 256) ? block : 256;
-        dim3 g(nblk(d.n, bs)), b(bs);
-        if (want_e) {
-            if (coul) hipLaunchKernelGGL((k_fast_force_fp64<true, true>), g, b, 0, s, d, p);
-            else hipLaunchKernelGGL((k_fast_force_fp64<true, false>), g, b, 0, s, d, p);
-        } else {
-            if (coul) hipLaunchKernelGGL((k_fast_force_fp64<false, true>), g, b, 0, s, d, p);
-            else hipLaunchKernelGGL((k_fast_force_fp64<false, false>), g, b, 0, s, d, p);
-        }
-    }
+    const int G = mt_fast_lanes(d);
+    if (G == 16) mt_launch_pairs<16>(d, p, mixed, want_e, coul, block, lds_tab, s);
+    else if (G == 4) mt_launch_pairs<4>(d, p, mixed, want_e, coul, block, lds_tab, s);
+    else mt_launch_pairs<1>(d, p, mixed, want_e, coul, block, lds_tab, s);
     if (ap) {
         dim3 g(nblk(d.n, AP_TPB), d.ap_nsplit);
         if (mixed) hipLaunchKernelGGL(k_ap_coulomb_mixed, g, dim3(AP_TPB), 0, s, d, p, want_e);
@@ -1135,8 +1472,9 @@ __device__ __forceinline__ float mt_fast_front(const MtDev& d, const MtParams& p
         lx -= p.L[0] * rint(lx / p.L[0]);
         ly -= p.L[1] * rint(ly / p.L[1]);
         lz -= p.L[2] * rint(lz / p.L[2]);
-        const float4 old = d.posf[k];
-        d.posf[k] = make_float4((float)lx, (float)ly, (float)lz, old.w);
+        const int u = d.sidx[k];
+        const float4 old = d.posf[u];
+        d.posf[u] = make_float4((float)lx, (float)ly, (float)lz, old.w);
     }
     return (float)(dx * dx + dy * dy + dz * dz);
 }
@@ -1235,7 +1573,7 @@ __global__ void __launch_bounds__(TPB) k_ucut(MtDev d, MtParams p, int exact)
         const int cnt = min(d.ncount[k], d.ncap);
         for (int j = 0; j < cnt; j++) {
             const int a = exact ? (int)d.nlist[(size_t)j * d.n + k]
-                                : (int)(mt_fast_word(d.nlist, d.n, k, j) & MT_NW_IDX_MASK);
+                                : mt_id_slot(d, (int)(mt_fast_word(d.nlist, d.n, k, j) & MT_NW_IDX_MASK));
             const double4 ra4 = d.pos[a];
             float e = d.ecut[row * T + d.type[a]];
             if (exact) {

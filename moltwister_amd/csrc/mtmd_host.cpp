@@ -684,6 +684,9 @@ static int setup(mtmd_handle* h, const mtmd_system* s)
         d.posref = h->mem.alloc<double4>(n);
         d.posf = h->mem.alloc<float4>(n);
         d.home = h->mem.alloc<int>(n);
+        d.sidx = h->mem.alloc<int>(n);
+        d.qfs = h->mem.alloc<float>(n);
+        d.wsorted = (c.engine == MTMD_ENGINE_MIXED) ? 1 : 0;
         d.qf = h->mem.alloc<float>(n);
         d.qf2 = h->mem.alloc<float>(n);
         if (p.coulomb_on && p.coulomb_allpairs) {
@@ -696,7 +699,7 @@ static int setup(mtmd_handle* h, const mtmd_system* s)
         std::vector<float> qf(n);
         for (int k = 0; k < n; k++) qf[k] = (float)s->charge[k];
         HIPCHK(h, hipMemcpy(d.qf, qf.data(), n * sizeof(float), hipMemcpyHostToDevice));
-        if (!d.pos2 || !d.mom2 || !d.type2 || !d.mol2 || !d.mobile2 || !d.posref || !d.posf || !d.home || !d.qf ||
+        if (!d.pos2 || !d.mom2 || !d.type2 || !d.mol2 || !d.mobile2 || !d.posref || !d.posf || !d.home || !d.qf || !d.sidx || !d.qfs ||
             !d.qf2 || !d.perm2 || !d.frc2 || !d.epot2)
             return h->fail(MTMD_ENOMEM, "device allocation failed");
     }

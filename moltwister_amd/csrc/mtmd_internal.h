@@ -74,6 +74,12 @@ struct MtDev {
     float* qf; float* qf2;  // float charges in slot order
     double4* posref;       // positions at the last rebuild
     float4* posf; int* home;
+    // posf is kept in cell order: posf[u] belongs to atom cell_list[u] (u is
+    // the atom's place in the cell lists of the last rebuild, sidx[slot] = u),
+    // so the atoms of a cell are one contiguous run of posf; qfs = charges in
+    // the same order. With wsorted set (MIXED) the neighbour-list words carry u
+    // instead of the slot.
+    int* sidx; float* qfs; int wsorted;
     double4* ap_part;      // all-pairs Coulomb partial sums [ap_nsplit][n] (FP64/MIXED)
     int ap_nsplit;
     // reductions / scalars
@@ -123,6 +129,8 @@ void mt_fast_identity_perm(const MtDev& d, hipStream_t s);
 void mt_fast_home(const MtDev& d, const MtParams& p, int conditional, hipStream_t s);
 void mt_fast_neighbors(const MtDev& d, const MtParams& p, int conditional, hipStream_t s);
 void mt_fast_forces(const MtDev& d, const MtParams& p, int mixed, int want_energy, int block, hipStream_t s);
+// lanes per atom the pair-force kernels use for this system (1, 4 or 16)
+int mt_fast_lanes(const MtDev& d);
 void mt_fast_kick_drift(const MtDev& d, const MtParams& p, int scale_slot, int mixed, hipStream_t s);
 void mt_fast_clear_flags(const MtDev& d, hipStream_t s);
 // NVE: back half of one step and front half of the next in one pass

@@ -426,8 +426,13 @@ __device__ inline void mt_pbc_add(double& pos, double L, double Lm)
 // lane ranks its entries against all of them (broadcast reads, 4 per int4);
 // a larger cell falls back to an insertion sort by lane 0. Holds a block
 // barrier, so every thread of the block calls it (cnt = 0 for no cell).
+// on(v, rank) is called once for every entry v with its sorted position.
+struct MtNoSortHook {
+    __device__ void operator()(int, int) const {}
+};
 #define MT_CELL_SORT_LDS 2048
-__device__ inline void mt_wave_sort_cell(int* __restrict__ a, int cnt, int* __restrict__ lds)
+template <typename Hook = MtNoSortHook>
+__device__ inline void mt_wave_sort_cell(int* __restrict__ a, int cnt, int* __restrict__ lds, Hook on = Hook())
 {
     const int lane = threadIdx.x & 63;
     const bool in_lds = cnt > 64 && cnt <= MT_CELL_SORT_LDS;
@@ -435,11 +440,14 @@ __device__ inline void mt_wave_sort_cell(int* __restrict__ a, int cnt, int* __re
     if (in_lds)
         for (int i = lane; i < 4 * n4; i += 64) lds[i] = i < cnt ? a[i] : INT_MAX;
     __syncthreads();
-    if (cnt > 1 && cnt <= 64) {
+    if (cnt >= 1 && cnt <= 64) {
         const int v = lane < cnt ? a[lane] : INT_MAX;
         int rank = 0;
         for (int j = 0; j < cnt; j++) rank += __shfl(v, j) < v ? 1 : 0;
-        if (lane < cnt) a[rank] = v;
+        if (lane < cnt) {
+            a[rank] = v;
+            on(v, rank);
+        }
     } else if (in_lds) {
         const int4* l4 = reinterpret_cast<const int4*>(lds);
         for (int i = lane; i < cnt; i += 64) {
@@ -450,6 +458,7 @@ __device__ inline void mt_wave_sort_cell(int* __restrict__ a, int cnt, int* __re
                 rank += (w.x < v) + (w.y < v) + (w.z < v) + (w.w < v);
             }
             a[rank] = v;
+            on(v, rank);
         }
     } else if (cnt > MT_CELL_SORT_LDS && lane == 0) {
         for (int i = 1; i < cnt; i++) {
@@ -458,5 +467,6 @@ __device__ inline void mt_wave_sort_cell(int* __restrict__ a, int cnt, int* __re
             while (j >= 0 && a[j] > v) { a[j + 1] = a[j]; j--; }
             a[j + 1] = v;
         }
+        for (int i = 0; i < cnt; i++) on(a[i], i);
     }
 }

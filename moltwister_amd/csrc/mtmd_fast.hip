@@ -3,8 +3,10 @@
 // gathered per atom over column-major lists (no atomics, deterministic).
 //
 // Per step the conditional rebuild kernels read flags[0] (the largest squared
-// displacement since the last rebuild, written by k_fast_kick_drift) and
-// return at once when it is below (skin/2)^2, so a step never waits on the host.
+// displacement since the last rebuild, written by the integrator's front
+// half) and return at once when it is below (skin/2)^2, so a step never waits
+// on the host. The pair-force kernel, which runs after the last of them,
+// resets flags[0] for the next step's front half (MT_RESET_DISP).
 //
 // MIXED keeps an FP32 copy of every position relative to the centre of the
 // atom's home cell (the cell it sat in at the last rebuild). Each neighbour
@@ -41,6 +43,11 @@ __device__ __forceinline__ void mt_fill_shifts(float4* sh, const MtParams& p)
         sh[c] = make_float4((float)(c % 3 - 1) * p.cs[0], (float)((c / 3) % 3 - 1) * p.cs[1],
                             (float)(c / 9 - 1) * p.cs[2], 0.f);
 }
+
+// first statement of every pair-force kernel: one thread clears the
+// displacement maximum; no kernel of the step reads it after the list build
+#define MT_RESET_DISP(d)                                   \
+    if (blockIdx.x == 0 && threadIdx.x == 0) (d).flags[0] = 0u
 
 __device__ __forceinline__ bool mt_skip_rebuild(int cond, const unsigned int* flags, float thr)
 {
@@ -169,21 +176,49 @@ __global__ void __launch_bounds__(TPB) k_fast_scatter(const int* __restrict__ ke
     if (c >= 0) list[first[c] + base + (lane - hl)] = k;
 }
 
-// ascending slot order inside each cell, so lists are reproducible run to run:
-// one wavefront per cell ranks its (distinct) entries with shuffles; cells of
-// more than 64 atoms fall back to an insertion sort by one lane
-__global__ void __launch_bounds__(TPB) k_fast_cell_sort(const int* __restrict__ first,
-                                                        const int* __restrict__ count, int* __restrict__ list,
-                                                        int ncell, int cond, const unsigned int* flags, float thr)
+// the per-atom set-up of a rebuild for atom s, the rank-th atom of cell c in
+// cell order (u = first[c] + rank): home cell, reference position for the
+// displacement test, its place in cell order, and the MIXED cell-local float
+// position (.w: the bits of type * MT_NPTS, the type's offset in a row of
+// segment coefficients) and charge at that place
+__device__ __forceinline__ void mt_fast_home_atom(const MtDev& d, const MtParams& p, int s, int c, int u)
 {
-    __shared__ __align__(16) int lds[(TPB / 64) * MT_CELL_SORT_LDS];
-    if (mt_skip_rebuild(cond, flags, thr) || flags[3]) return;   // uniform over the grid
-    const int c = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int cnt = c < ncell ? count[c] : 0;
-    mt_wave_sort_cell(list + (c < ncell ? first[c] : 0), cnt, lds + (threadIdx.x >> 6) * MT_CELL_SORT_LDS);
+    const double4 r = d.pos[s];
+    d.home[s] = c;
+    d.posref[s] = r;
+    d.sidx[s] = u;
+    d.qfs[u] = d.qf[s];
+    const int ix = c % d.nc[0], iy = (c / d.nc[0]) % d.nc[1], iz = c / (d.nc[0] * d.nc[1]);
+    const double cx = -p.L[0] / 2.0 + ((double)ix + 0.5) * (p.L[0] / d.nc[0]);
+    const double cy = -p.L[1] / 2.0 + ((double)iy + 0.5) * (p.L[1] / d.nc[1]);
+    const double cz = -p.L[2] / 2.0 + ((double)iz + 0.5) * (p.L[2] / d.nc[2]);
+    double dx = r.x - cx, dy = r.y - cy, dz = r.z - cz;
+    dx -= p.L[0] * rint(dx / p.L[0]);
+    dy -= p.L[1] * rint(dy / p.L[1]);
+    dz -= p.L[2] * rint(dz / p.L[2]);
+    d.posf[u] = make_float4((float)dx, (float)dy, (float)dz, __int_as_float(d.type[s] * MT_NPTS));
 }
 
-void mt_fast_cells(const MtDev& d, const MtParams& p, int cond, hipStream_t s)
+// ascending slot order inside each cell, so lists are reproducible run to run
+// (one wavefront per cell, mt_wave_sort_cell); with `home` set the same
+// wavefront then does the per-atom set-up of every atom of the cell, which
+// holds its sorted position already
+__global__ void __launch_bounds__(TPB) k_fast_cell_sort(MtDev d, MtParams p, int cond, int home)
+{
+    __shared__ __align__(16) int lds[(TPB / 64) * MT_CELL_SORT_LDS];
+    if (mt_skip_rebuild(cond, d.flags, p.half_skin2) || d.flags[3]) return;   // uniform over the grid
+    if (home && blockIdx.x == 0 && threadIdx.x == 0) d.flags[4] += 1;        // rebuild count
+    const int c = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int cnt = c < d.ncells ? d.cell_count[c] : 0;
+    const int f = c < d.ncells ? d.cell_first[c] : 0;
+    int* lw = lds + (threadIdx.x >> 6) * MT_CELL_SORT_LDS;
+    if (home)
+        mt_wave_sort_cell(d.cell_list + f, cnt, lw, [&](int v, int rank) { mt_fast_home_atom(d, p, v, c, f + rank); });
+    else
+        mt_wave_sort_cell(d.cell_list + f, cnt, lw);
+}
+
+void mt_fast_cells(const MtDev& d, const MtParams& p, int cond, int with_home, hipStream_t s)
 {
     const float thr = p.half_skin2;
     const int nsb = nblk(d.ncells, SCAN_B);
@@ -197,8 +232,7 @@ void mt_fast_cells(const MtDev& d, const MtParams& p, int cond, hipStream_t s)
                        d.cell_fill, d.ncells, cond, d.flags, thr);
     hipLaunchKernelGGL(k_fast_scatter, dim3(nblk(d.n, TPB)), dim3(TPB), 0, s, d.cell_key, d.n, d.cell_first,
                        d.cell_fill, d.cell_list, cond, d.flags, thr);
-    hipLaunchKernelGGL(k_fast_cell_sort, dim3(nblk(d.ncells, TPB / 64)), dim3(TPB), 0, s, d.cell_first,
-                       d.cell_count, d.cell_list, d.ncells, cond, d.flags, thr);
+    hipLaunchKernelGGL(k_fast_cell_sort, dim3(nblk(d.ncells, TPB / 64)), dim3(TPB), 0, s, d, p, cond, with_home);
 }
 
 // gather every per-atom array into cell order: new slot s <- old slot list[s]
@@ -258,36 +292,6 @@ void mt_fast_identity_perm(const MtDev& d, hipStream_t s)
 }
 
 // home cell, reference position and local coordinates at a rebuild
-__global__ void k_fast_home(MtDev d, MtParams p, int cond)
-{
-    if (mt_skip_rebuild(cond, d.flags, p.half_skin2)) return;
-    const int u = blockIdx.x * blockDim.x + threadIdx.x;
-    if (u == 0) d.flags[4] += 1;
-    if (u >= d.n) return;
-    // u-th atom in cell order
-    const int s = d.cell_list[u];
-    int c = d.cell_key[s];
-    double4 r = d.pos[s];
-    d.home[s] = c;
-    d.posref[s] = r;
-    d.sidx[s] = u;
-    d.qfs[u] = d.qf[s];
-    int ix = c % d.nc[0], iy = (c / d.nc[0]) % d.nc[1], iz = c / (d.nc[0] * d.nc[1]);
-    double cx = -p.L[0] / 2.0 + ((double)ix + 0.5) * (p.L[0] / d.nc[0]);
-    double cy = -p.L[1] / 2.0 + ((double)iy + 0.5) * (p.L[1] / d.nc[1]);
-    double cz = -p.L[2] / 2.0 + ((double)iz + 0.5) * (p.L[2] / d.nc[2]);
-    double dx = r.x - cx, dy = r.y - cy, dz = r.z - cz;
-    dx -= p.L[0] * rint(dx / p.L[0]);
-    dy -= p.L[1] * rint(dy / p.L[1]);
-    dz -= p.L[2] * rint(dz / p.L[2]);
-    d.posf[u] = make_float4((float)dx, (float)dy, (float)dz, __int_as_float(d.type[s]));
-}
-
-void mt_fast_home(const MtDev& d, const MtParams& p, int cond, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_fast_home, dim3(nblk(d.n, TPB)), dim3(TPB), 0, s, d, p, cond);
-}
-
 // is atom `ia` (slot) a 1-2/1-3/1-4 partner of atom k (slot)? k's rows are in
 // original indexing (FunctorCalcForce.cpp:95-145 lists)
 __device__ inline bool mt_is_partner(const MtDev& d, int k, int ia)
@@ -313,11 +317,11 @@ __device__ inline bool mt_is_partner(const MtDev& d, int k, int ia)
 // 2x2x2 block of home cells, one cell per half-wave (a cell holds ~26 atoms at
 // rho* = 1.2 with cells of cutoff + skin). The 4x4x4 cells around the block
 // (the union of the eight 27-cell neighbourhoods) are staged once in LDS as
-// (x, y, z, slot) in cell order; every lane (one atom of a home cell) then
-// scans the 27 staged cells of its own neighbourhood with broadcast reads,
-// two candidates per iteration. Each lane keeps its last three hits in
-// registers and writes every fourth as one 16-byte chunk of the chunked
-// column-major layout. When the 64 staged cells hold more than NB_CAP atoms
+// (x, y, z, word index) in cell order; every lane (one atom of a home cell)
+// then scans the 27 staged cells of its own neighbourhood with broadcast
+// reads, collecting hits as bit masks (mt_scan_cell). Each lane keeps its last
+// three hits in registers and writes every fourth as one 16-byte chunk of the
+// chunked column-major layout. When the 64 staged cells hold more than NB_CAP atoms
 // they are staged in batches of whole cells.
 //
 // MODE 0: no molecules. MODE 1: same-molecule pairs are left out (molecule
@@ -668,8 +672,8 @@ __device__ __forceinline__ void mt_mixed_force(float dx, float dy, float dz, flo
     float fr = 0.0f;
     const int idx = (int)((r - x0) * invd);
     if (idx >= 0 && idx < MT_NPTS - 1) {
-        // type indices are small: a 24-bit multiply (full rate) forms the row offset
-        const CoefT c = row[__umul24(__float_as_uint(pa.w), MT_NPTS) + idx];
+        // pa.w holds the neighbour's type times MT_NPTS: its offset in the row
+        const CoefT c = row[__float_as_int(pa.w) + idx];
         fr = c.x + c.y * rinv;
         if constexpr (WANT_E) U += 0.5f * (c.z * r + c.w);
     }
@@ -799,6 +803,7 @@ __device__ __forceinline__ void mt_lane_sum(double& x, double& y, double& z, flo
 template <bool WANT_E, bool COUL, int G>
 __global__ void __launch_bounds__(512) k_fast_force_mixed(MtDev d, MtParams p)
 {
+    MT_RESET_DISP(d);
     __shared__ float4 sh[27];
     mt_fill_shifts(sh, p);
     __syncthreads();
@@ -806,7 +811,7 @@ __global__ void __launch_bounds__(512) k_fast_force_mixed(MtDev d, MtParams p)
     const int k = t / G, g = t % G;
     if (k >= d.n) return;                       // all G lanes of an atom leave together
     const float4 pk = d.posf[d.sidx[k]];
-    const float4* __restrict__ row = d.nbcoef + (size_t)__float_as_int(pk.w) * d.T * MT_NPTS;
+    const float4* __restrict__ row = d.nbcoef + (size_t)__float_as_int(pk.w) * d.T;
     const float qk = COUL ? d.qf[k] : 0.0f;
     double Fx = 0.0, Fy = 0.0, Fz = 0.0;
     float U = 0.0f;
@@ -826,6 +831,7 @@ __global__ void __launch_bounds__(512) k_fast_force_mixed(MtDev d, MtParams p)
 template <bool COUL, int G>
 __global__ void __launch_bounds__(512) k_fast_force_mixed_lds(MtDev d, MtParams p)
 {
+    MT_RESET_DISP(d);
     extern __shared__ float2 stab[];
     __shared__ float4 sh[27];
     const int T = d.T;
@@ -838,7 +844,7 @@ __global__ void __launch_bounds__(512) k_fast_force_mixed_lds(MtDev d, MtParams 
     const int g = threadIdx.x % G;
     for (int k = kb + (int)threadIdx.x / G; k < ke; k += blockDim.x / G) {
         const float4 pk = d.posf[d.sidx[k]];
-        const float2* __restrict__ row = stab + __float_as_int(pk.w) * T * MT_NPTS;
+        const float2* __restrict__ row = stab + __float_as_int(pk.w) * T;
         const float qk = COUL ? d.qf[k] : 0.0f;
         double Fx = 0.0, Fy = 0.0, Fz = 0.0;
         float U = 0.0f;
@@ -885,8 +891,10 @@ __device__ __forceinline__ void mt_tile_atom(const MtDev& d, const MtParams& p, 
         wn[i] = __builtin_nontemporal_load(nl4 + (size_t)min(q0 + i * qs, nch - 1) * stride);
     for (int q = q0; q < nch; q += qs) {
         const mt_u32x4 w = wn[0];
-        const float4 o0 = shd[w.x >> MT_NW_IDX_BITS], o1 = shd[w.y >> MT_NW_IDX_BITS];
-        const float4 o2 = shd[w.z >> MT_NW_IDX_BITS], o3 = shd[w.w >> MT_NW_IDX_BITS];
+        const float4 o0 = shd[__builtin_amdgcn_ubfe(w.x, MT_NW_IDX_BITS, 5)];
+        const float4 o1 = shd[__builtin_amdgcn_ubfe(w.y, MT_NW_IDX_BITS, 5)];
+        const float4 o2 = shd[__builtin_amdgcn_ubfe(w.z, MT_NW_IDX_BITS, 5)];
+        const float4 o3 = shd[__builtin_amdgcn_ubfe(w.w, MT_NW_IDX_BITS, 5)];
         const int i0 = (int)(w.x & MT_NW_IDX_MASK), i1 = (int)(w.y & MT_NW_IDX_MASK);
         const int i2 = (int)(w.z & MT_NW_IDX_MASK), i3 = (int)(w.w & MT_NW_IDX_MASK);
         float4 a0, a1, a2, a3;
@@ -924,6 +932,7 @@ __device__ __forceinline__ void mt_tile_atom(const MtDev& d, const MtParams& p, 
 template <bool COUL, int G>
 __global__ void __launch_bounds__(256 * G) k_fast_force_tile(MtDev d, MtParams p)
 {
+    MT_RESET_DISP(d);
     extern __shared__ float2 stab[];                  // T * T * MT_NPTS segment coefficients
     __shared__ float4 spos[FT_CAP];
     __shared__ float sq_buf[COUL ? FT_CAP : 1];
@@ -1008,7 +1017,7 @@ __global__ void __launch_bounds__(256 * G) k_fast_force_tile(MtDev d, MtParams p
         const int u = s_hfirst[h] + (v - s_hoff[h]);
         const int k = d.cell_list[u];
         const float4 pk = d.posf[u];
-        const float2* __restrict__ row = stab + __float_as_int(pk.w) * T * MT_NPTS;
+        const float2* __restrict__ row = stab + __float_as_int(pk.w) * T;
         const float qk = COUL ? d.qf[k] : 0.0f;
         double Fx = 0.0, Fy = 0.0, Fz = 0.0;
         if (lds) mt_tile_atom<COUL, true>(d, p, k, pk, shd + 27 * h, spos, sq_buf, row, qk, Fx, Fy, Fz, gl, G);
@@ -1028,6 +1037,7 @@ __global__ void __launch_bounds__(256 * G) k_fast_force_tile(MtDev d, MtParams p
 template <bool WANT_E, bool COUL, int G>
 __global__ void __launch_bounds__(256) k_fast_force_fp64(MtDev d, MtParams p)
 {
+    MT_RESET_DISP(d);
     const int t = mt_xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const int k = t / G, g = t % G;
     if (k >= d.n) return;                       // all G lanes of an atom leave together

@@ -966,14 +966,13 @@ static int fast_rebuild(mtmd_handle* h, bool force_rebuild, bool sort)
     const int cond = force_rebuild ? 0 : 1;
     if (sort) {
         // spatial re-sort: order atoms by cell (ascending original slot inside a cell)
-        mt_fast_cells(d, h->p, 0, h->stream);
+        mt_fast_cells(d, h->p, 0, 0, h->stream);
         mt_fast_sort_apply(d, h->stream);
         swap_slots(d);
         h->steps_since_sort = 0;
         h->need_sort = false;
     }
-    mt_fast_cells(d, h->p, cond, h->stream);
-    mt_fast_home(d, h->p, cond, h->stream);
+    mt_fast_cells(d, h->p, cond, 1, h->stream);
     mt_fast_neighbors(d, h->p, cond, h->stream);
     HIPCHK(h, hipGetLastError());
     return MTMD_OK;
@@ -1165,7 +1164,6 @@ static int md_steps(mtmd_handle* h, int nsteps, bool energy_last)
         auto nvt_front = [&]() {
             if (!h->ke_valid) mt_reduce_ke(d, h->stream);
             mt_nh_chain(d, h->M, h->nresp, h->Tred, h->p.dt, h->g, h->w, nblocks, h->stream);
-            mt_fast_clear_flags(d, h->stream);
             mt_fast_kick_drift(d, h->p, h->nh_scale_slot, mixed, h->stream);
         };
         auto nvt_back = [&]() {
@@ -1179,12 +1177,8 @@ static int md_steps(mtmd_handle* h, int nsteps, bool energy_last)
             int r = fast_rebuild(h, false, false);
             if (r) return r;
             if ((r = fast_forces(h, false))) return r;
-            if (nvt) {
-                nvt_back();
-            } else {
-                mt_fast_clear_flags(d, h->stream);
-                mt_fast_nve_fused(d, h->p, mixed, h->stream);
-            }
+            if (nvt) nvt_back();
+            else mt_fast_nve_fused(d, h->p, mixed, h->stream);
             return MTMD_OK;
         };
         auto replay = [&]() -> int {
@@ -1207,10 +1201,10 @@ static int md_steps(mtmd_handle* h, int nsteps, bool energy_last)
             HIPCHK(h, hipGraphLaunch(sg.x, h->stream));
             return MTMD_OK;
         };
-        if (!nvt) {
-            mt_fast_clear_flags(d, h->stream);
-            mt_fast_kick_drift(d, h->p, -1, mixed, h->stream);
-        }
+        // the pair-force kernels reset the displacement maximum for the next
+        // step's front half; this call's first front half gets a clear of its own
+        mt_fast_clear_flags(d, h->stream);
+        if (!nvt) mt_fast_kick_drift(d, h->p, -1, mixed, h->stream);
         for (int t = 0; t < nsteps; t++) {
             const bool last = (t == nsteps - 1);
             const bool sort_now = (++h->steps_since_sort >= h->sort_every);
@@ -1229,7 +1223,6 @@ static int md_steps(mtmd_handle* h, int nsteps, bool energy_last)
                 mt_cap_kick(d, h->p, 0, h->stream);
                 mt_scale_wrap(d, h->p, -1, 0, h->stream);
             } else {
-                mt_fast_clear_flags(d, h->stream);
                 mt_fast_nve_fused(d, h->p, mixed, h->stream);
             }
             h->ke_valid = nvt;

@@ -6,9 +6,13 @@
 //   frc    double4  Fx, Fy, Fz, 0         (F == Fpi in every engine)
 //   epot   float    per-atom potential energy
 //   type, mol int32; mobile uint8
-//   posf   float4   xl, yl, zl, type-bits (MIXED: position relative to the
-//                   centre of the atom's home cell, see k_refresh_local)
+//   posf   float4   xl, yl, zl, bits of type*NPTS, in CELL ORDER (entry u is
+//                   the atom cell_list[u] of the last rebuild): position
+//                   relative to the centre of the atom's home cell, written
+//                   at each rebuild and by the integrator's front half
 //   home   int32    home cell of the atom at the last neighbour rebuild
+//   sidx   int32    the atom's place u in cell order; qfs float: charges in
+//                   cell order
 // Tables (float4 per point, 16-byte aligned):
 //   nbtab  [((row*T + col)*MAXFF + ff)*NPTS + pt]  x, -dU/dx, U, 0
 //   nbcnt  [row*T + col]
@@ -16,7 +20,10 @@
 //   nbcoef [(row*T + col)*NPTS + pt]   a_f, b_f, a_e, b_e of the segment
 //          starting at pt, summed over the pair's force fields (MIXED)
 // Neighbour lists are column-major: nl[j*ncap_stride + k], so the lanes of a
-// wavefront (consecutive atoms) read consecutive words at every j.
+// wavefront (consecutive atoms) read consecutive words at every j. FP64/MIXED
+// lists are chunked (4 words per 16-byte chunk) and their words carry a 5-bit
+// cell-offset code above a 27-bit index: the neighbour's slot (FP64) or its
+// place in cell order (MIXED).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -122,11 +129,12 @@ void mt_gradient_step(const MtDev& d, double lr, hipStream_t s);
 // ---- launchers (mtmd_fast.hip) ----
 // `conditional` kernels return at once unless this step's maximum
 // displacement since the last rebuild exceeds half the skin (flags[0] vs p.half_skin2)
-void mt_fast_cells(const MtDev& d, const MtParams& p, int conditional, hipStream_t s);
+// with_home: the cell sort also does the per-atom set-up of the rebuild (home
+// cell, reference position, cell-order place, MIXED float position)
+void mt_fast_cells(const MtDev& d, const MtParams& p, int conditional, int with_home, hipStream_t s);
 void mt_fast_sort_apply(const MtDev& d, hipStream_t s);   // writes the *2 buffers, perm2 and inv
 void mt_fast_unsort(const MtDev& d, hipStream_t s);       // writes the *2 buffers in original order
 void mt_fast_identity_perm(const MtDev& d, hipStream_t s);
-void mt_fast_home(const MtDev& d, const MtParams& p, int conditional, hipStream_t s);
 void mt_fast_neighbors(const MtDev& d, const MtParams& p, int conditional, hipStream_t s);
 void mt_fast_forces(const MtDev& d, const MtParams& p, int mixed, int want_energy, int block, hipStream_t s);
 // lanes per atom the pair-force kernels use for this system (1, 4 or 16)

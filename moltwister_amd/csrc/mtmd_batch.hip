@@ -113,7 +113,7 @@ __global__ void k_mb_nh(MbDev d, MbPar p, int r0, int r1)
     const int M = d.M, stride = 3 * M + 2;
     double* nh = d.nh + (size_t)r * stride;
     const MbRep rp = d.rep[r];
-    nh[3 * M] = mt_nh_propagate(nh, nh + M, nh + 2 * M, M, p.nresp, rp.T, p.dt, rp.g, p.w, d.red[3 * r]);
+    nh[3 * M] = mt_nh_step(nh, nh + M, nh + 2 * M, M, p.nresp, rp.T, p.dt, rp.g, p.w, d.red[3 * r]);
 }
 
 // thermostat scale, half kick, drift, FP32 copy, per-replica displacement flag

@@ -382,7 +382,7 @@ __global__ void k_nh_chain(double* nh, const double* red, int nblocks, int M, in
     double KE2 = 0.0;
     for (int t = 0; t < 64; t++) KE2 += sh[t];
     const double w[3] = {w0, w1, w2};
-    nh[3 * M] = mt_nh_propagate(nh, nh + M, nh + 2 * M, M, nresp, T, dt, g, w, KE2);
+    nh[3 * M] = mt_nh_step(nh, nh + M, nh + 2 * M, M, nresp, T, dt, g, w, KE2);
 }
 
 void mt_nh_chain(const MtDev& d, int M, int nresp, double T, double dt, double g, const double* w, int nblocks,

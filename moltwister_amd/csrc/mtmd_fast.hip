@@ -510,43 +510,51 @@ __global__ void __launch_bounds__(256) k_fast_neigh_block(MtDev d, MtParams p, i
     }
 }
 
-// Neighbour list for dense cells (tens to hundreds of atoms per cell, e.g.
-// water with a 10 A cutoff): one 256-thread workgroup per home cell, lanes =
-// home atoms; the 27 neighbour cells are staged in LDS one at a time (in
-// chunks of NBW_CAP atoms) and every lane scans each with broadcast reads.
-// Same output, order and exclusion rules as k_fast_neigh_block.
 #define NBW_CAP 1024
+
+// Neighbour list for dense cells (tens to hundreds of atoms per cell, e.g.
+// water with a 10 A cutoff), with enough workgroups to fill the GPU: a
+// workgroup takes a batch of NBD_ATOMS home atoms of one cell (grid: cells x
+// batches, each workgroup looping over the batches y, y + gridDim.y, ...),
+// NBD_G lanes per home atom. The 27 neighbour cells are staged in LDS one at
+// a time (in chunks of NBW_CAP atoms); the NBD_G lanes of an atom test NBD_G
+// consecutive candidates per step, and a wavefront ballot gives every hit its
+// rank among the atom's hits of that step, so the words land in candidate
+// order: the list is the one k_fast_neigh_block's sequential scan would write.
+#define NBD_G 16
+#define NBD_ATOMS (256 / NBD_G)
 template <int MODE>
-__global__ void __launch_bounds__(256) k_fast_neigh_wide(MtDev d, MtParams p, int cond)
+__global__ void __launch_bounds__(256) k_fast_neigh_dense(MtDev d, MtParams p, int cond)
 {
     if (mt_skip_rebuild(cond, d.flags, p.half_skin2)) return;
     __shared__ float4 spos[NBW_CAP];
     __shared__ int smol_buf[MODE >= 1 ? NBW_CAP : 1];
-    int* smol = smol_buf;
-    const int t = threadIdx.x;
+    const int t = threadIdx.x, lane = t & 63;
+    const int a = t / NBD_G, g = t % NBD_G;
     const int Nx = d.nc[0], Ny = d.nc[1], Nz = d.nc[2];
-    const int c = mt_xcd_block(blockIdx.x, gridDim.x);
-    if (c >= d.ncells) return;
+    const int c = blockIdx.x;
     const int hx = c % Nx, hy = (c / Nx) % Ny, hz = c / (Nx * Ny);
     const int f0 = d.cell_first[c], cnt_c = d.cell_count[c];
-    if (cnt_c == 0) return;
     if (d.flags[3]) {
-        for (int i = t; i < cnt_c; i += 256) d.ncount[d.cell_list[f0 + i]] = 0;
+        if (blockIdx.y == 0)
+            for (int i = t; i < cnt_c; i += 256) d.ncount[d.cell_list[f0 + i]] = 0;
         return;
     }
     const int n = d.n, ncap = d.ncap;
     const float rl2 = p.rlist2;
-    uint4* __restrict__ nl4 = reinterpret_cast<uint4*>(d.nlist);
-    for (int ib = 0; ib < cnt_c; ib += 256) {
-        const bool act = ib + t < cnt_c;
-        const int uk = f0 + ib + t;                 // place in cell order
+    uint32_t* __restrict__ nl = d.nlist;
+    // the lanes of this atom's group within the wavefront, and those below g
+    const int gbase = lane & ~(NBD_G - 1);
+    const uint64_t below = (1ull << g) - 1ull;
+    for (int hb = (int)blockIdx.y * NBD_ATOMS; hb < cnt_c; hb += (int)gridDim.y * NBD_ATOMS) {
+        const bool act = hb + a < cnt_c;
+        const int uk = f0 + hb + a;
         const int k = act ? d.cell_list[uk] : 0;
         const int self = d.wsorted ? uk : k;
         const float4 pk = act ? d.posf[uk] : make_float4(0.f, 0.f, 0.f, 0.f);
         const int molk = (MODE >= 1 && act) ? d.mol[k] : -1;
         const bool xk = MODE == 2 && act && d.xmol[d.perm[k]] != 0;
         int cnt = 0;
-        uint32_t b0 = 0, b1 = 0, b2 = 0;
         for (int oz = -1; oz <= 1; oz++) {
             int cz = hz + oz; cz = cz < 0 ? cz + Nz : (cz >= Nz ? cz - Nz : cz);
             for (int oy = -1; oy <= 1; oy++) {
@@ -565,48 +573,45 @@ __global__ void __launch_bounds__(256) k_fast_neigh_wide(MtDev d, MtParams p, in
                         for (int e = t; e < m; e += 256) {
                             const int ui = g0 + jb + e;
                             const float4 pa = d.posf[ui];
-                            const int a = (MODE >= 1 || !d.wsorted) ? d.cell_list[ui] : ui;
-                            spos[e] = make_float4(pa.x, pa.y, pa.z, __uint_as_float((uint32_t)(d.wsorted ? ui : a)));
-                            if constexpr (MODE >= 1) smol[e] = d.mol[a];
+                            const int at = (MODE >= 1 || !d.wsorted) ? d.cell_list[ui] : ui;
+                            spos[e] = make_float4(pa.x, pa.y, pa.z, __uint_as_float((uint32_t)(d.wsorted ? ui : at)));
+                            if constexpr (MODE >= 1) smol_buf[e] = d.mol[at];
                         }
                         __syncthreads();
-                        if (!act) continue;
-                        int e = 0;
-                        for (; e + 1 < m; e += 2) {
-                            float4 c0 = spos[e], c1 = spos[e + 1];
-                            asm volatile("" : "+v"(c0.w), "+v"(c1.w));
-                            const float dx0 = qx - c0.x, dy0 = qy - c0.y, dz0 = qz - c0.z;
-                            const float dx1 = qx - c1.x, dy1 = qy - c1.y, dz1 = qz - c1.z;
-                            const bool h0 = dx0 * dx0 + dy0 * dy0 + dz0 * dz0 < rl2;
-                            const bool h1 = dx1 * dx1 + dy1 * dy1 + dz1 * dz1 < rl2;
-                            if (h0 && (MODE != 2 || !xk || !mt_is_partner(d, k, mt_id_slot(d, (int)__float_as_uint(c0.w)))))
-                                mt_neigh_hit<MODE == 2 ? 1 : MODE>(d, nl4, c0, code, k, self, molk, smol, e, n, ncap, cnt,
-                                                                     b0, b1, b2);
-                            if (h1 && (MODE != 2 || !xk || !mt_is_partner(d, k, mt_id_slot(d, (int)__float_as_uint(c1.w)))))
-                                mt_neigh_hit<MODE == 2 ? 1 : MODE>(d, nl4, c1, code, k, self, molk, smol, e + 1, n, ncap,
-                                                                     cnt, b0, b1, b2);
-                        }
-                        if (e < m) {
-                            float4 c0 = spos[e];
-                            asm volatile("" : "+v"(c0.w));
-                            const float dx0 = qx - c0.x, dy0 = qy - c0.y, dz0 = qz - c0.z;
-                            if (dx0 * dx0 + dy0 * dy0 + dz0 * dz0 < rl2 &&
-                                (MODE != 2 || !xk || !mt_is_partner(d, k, mt_id_slot(d, (int)__float_as_uint(c0.w)))))
-                                mt_neigh_hit<MODE == 2 ? 1 : MODE>(d, nl4, c0, code, k, self, molk, smol, e, n, ncap, cnt,
-                                                                     b0, b1, b2);
+                        for (int e0 = 0; e0 < m; e0 += NBD_G) {
+                            const int e = e0 + g;
+                            bool hit = false;
+                            uint32_t id = 0u;
+                            if (act && e < m) {
+                                const float4 q4 = spos[e];
+                                const float dx = qx - q4.x, dy = qy - q4.y, dz = qz - q4.z;
+                                id = __float_as_uint(q4.w);
+                                hit = dx * dx + dy * dy + dz * dz < rl2 && (int)id != self;
+                                if constexpr (MODE >= 1) hit = hit && smol_buf[e] != molk;
+                                if constexpr (MODE == 2)
+                                    hit = hit && !(xk && mt_is_partner(d, k, mt_id_slot(d, (int)id)));
+                            }
+                            const uint64_t bal = __ballot(hit);
+                            const uint64_t grp = (bal >> gbase) & ((NBD_G == 64) ? ~0ull : ((1ull << NBD_G) - 1ull));
+                            if (hit) {
+                                const int j = cnt + __popcll(grp & below);
+                                if (j < ncap) nl[(((size_t)(j >> 2) * n + k) << 2) + (j & 3)] = id | code;
+                            }
+                            cnt += __popcll(grp);
                         }
                     }
                 }
             }
         }
         if (act) {
-            const int slot = cnt & 3;
-            if (slot && cnt < ncap) {
-                const uint32_t pad = (uint32_t)self | (NB_PAD_CODE << MT_NW_IDX_BITS);
-                nl4[(size_t)(cnt >> 2) * n + k] = make_uint4(b0, slot > 1 ? b1 : pad, slot > 2 ? b2 : pad, pad);
+            // the last partial chunk, padded with the atom itself (offset code 0,0,0)
+            const int j = cnt + g;
+            if (g < ((4 - (cnt & 3)) & 3) && j < ncap)
+                nl[(((size_t)(j >> 2) * n + k) << 2) + (j & 3)] = (uint32_t)self | (NB_PAD_CODE << MT_NW_IDX_BITS);
+            if (g == 0) {
+                d.ncount[k] = cnt;
+                if (cnt > ncap) { d.flags[2] = 1; atomicMax(d.nreq, cnt); }
             }
-            d.ncount[k] = cnt;
-            if (cnt > ncap) { d.flags[2] = 1; atomicMax(d.nreq, cnt); }
         }
     }
 }
@@ -614,13 +619,16 @@ __global__ void __launch_bounds__(256) k_fast_neigh_wide(MtDev d, MtParams p, in
 void mt_fast_neighbors(const MtDev& d, const MtParams& p, int cond, hipStream_t s)
 {
     if (d.n > 40 * d.ncells) {
-        // dense cells: one workgroup per home cell
+        // dense cells: workgroups of NBD_ATOMS home atoms, about 1.5x the mean
+        // cell's batches per cell (a fuller cell loops)
+        const int mean = (d.n + d.ncells - 1) / d.ncells;
+        const dim3 grid(d.ncells, std::max(1, (3 * mean / 2 + NBD_ATOMS - 1) / NBD_ATOMS));
         if (d.has_xmol)
-            hipLaunchKernelGGL(k_fast_neigh_wide<2>, dim3(d.ncells), dim3(256), 0, s, d, p, cond);
+            hipLaunchKernelGGL(k_fast_neigh_dense<2>, grid, dim3(256), 0, s, d, p, cond);
         else if (d.has_molecules)
-            hipLaunchKernelGGL(k_fast_neigh_wide<1>, dim3(d.ncells), dim3(256), 0, s, d, p, cond);
+            hipLaunchKernelGGL(k_fast_neigh_dense<1>, grid, dim3(256), 0, s, d, p, cond);
         else
-            hipLaunchKernelGGL(k_fast_neigh_wide<0>, dim3(d.ncells), dim3(256), 0, s, d, p, cond);
+            hipLaunchKernelGGL(k_fast_neigh_dense<0>, grid, dim3(256), 0, s, d, p, cond);
         return;
     }
     const int nb = ((d.nc[0] + 1) / 2) * ((d.nc[1] + 1) / 2) * ((d.nc[2] + 1) / 2);

@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define MTMD_ABI_VERSION 1
+#define MTMD_ABI_VERSION 2
 
 /* Number of points in every force/energy profile and the maximum number of
  * non-bonded force fields per atom-type pair
@@ -115,6 +115,23 @@ typedef struct mtmd_config {
     double  opt_accuracy;
     int32_t verbose;
     char    info_file[256];         /* thermo table written by mtmd_run        */
+    /* output files of mtmd_run / mtmd_run_optimize (CMDLoop, MDLoop.cpp:84-102,
+     * 370-431); paths as the `moldyn cfg set` keys xyzfile, dcdfile, xtcfile,
+     * pdistrfile, vdistrfile (the last two are prefixes: <prefix>_p_x.dat ...,
+     * <prefix>_v.dat) */
+    char    xyz_file[256];
+    char    dcd_file[256];
+    char    xtc_file[256];
+    char    pdistr_file[256];
+    char    vdistr_file[256];
+    int32_t include_xyz;            /* default 0                                */
+    int32_t include_dcd;            /* default 1                                */
+    int32_t include_xtc;            /* default 0; no XTC library: writes DCD    */
+    float   xtc_precision;          /* default 1000                             */
+    int32_t include_pdistr;         /* momentum histograms after step 1000      */
+    int32_t include_vdistr;         /* volume histogram after step 1000         */
+    double  max_pdistr;             /* AA*g/(fs*mol), default 0.3               */
+    double  max_vdistr;             /* AA^3, default 1e5                        */
     /* engine options (no counterpart in the reference) */
     int32_t engine;                 /* MTMD_ENGINE_*                           */
     int32_t coulomb;                /* MTMD_COUL_*                             */
@@ -172,6 +189,10 @@ typedef struct mtmd_system {
     int32_t n_angles;      const int32_t* angles;     /* [4*n]: a, centre, c, ff  */
     int32_t n_dihedrals;   const int32_t* dihedrals;  /* [5*n]: a, b, c, d, ff    */
     uint64_t seed;             /* for random momenta when mom == NULL            */
+    /* [n_types] atom-type names (the atom IDs of CMolTwisterState, which
+     * define the types) for the XYZ writer, or NULL: types are then named by
+     * their index */
+    const char* const* type_names;
 } mtmd_system;
 
 /* Thermodynamic output (SimulationBox.cpp:168-231, MDLoop.cpp:391-404).
@@ -286,13 +307,29 @@ int mtmd_synchronize(mtmd_handle* h);
  * engines; calibrates the HBM byte counters). */
 int mtmd_bench_kernel(mtmd_handle* h, int which, int reps, double* avg_ms);
 
+/* CMDLoop::appendToXYZFile (MDLoop/MDLoop.cpp:229-246): appends one frame of
+ * the current positions, headed "Frame <t>", to the text file at path. */
+int mtmd_append_xyz(mtmd_handle* h, const char* path, int32_t t);
+
+/* CMDLoop::appendToDCDFile (MDLoop/MDLoop.cpp:248-258) with
+ * CDCDFile::createDCDFileIfNotExists / appendToDCDFile
+ * (Utilities/DCDFile.cpp:670-729): writes the CHARMM header (n_sets =
+ * n_steps / stride, write frequency = stride, time step in fs) when the file
+ * does not exist, then appends one record of the current positions as float
+ * with the current box widths narrowed to int. */
+int mtmd_append_dcd(mtmd_handle* h, const char* path, int32_t n_steps, int32_t stride);
+
 /* CMDSimulator::run (Simulator/MDSimulator.cpp:35-47): creates the engine,
  * computes initial forces and runs cfg->n_steps, writing the reference's
  * thermo table every out_stride steps to cfg->info_file (and to `log` when
- * not NULL). */
+ * not NULL), the XYZ / DCD frames at the same steps when enabled, and the
+ * momentum and volume histograms (150 bins, accumulated every step after
+ * step 1000) at the end (MDLoop.cpp:107-141,370-431). */
 int mtmd_run(const mtmd_config* cfg, const mtmd_system* sys, void* log_file);
 
-/* CMDSimulator::optimize (Simulator/MDSimulator.cpp:49-66). */
+/* CMDSimulator::optimize (Simulator/MDSimulator.cpp:49-66): gradient descent
+ * (MDLoop.cpp:143-173) with a thermo line and the XYZ / DCD frames every
+ * out_stride steps. */
 int mtmd_run_optimize(const mtmd_config* cfg, const mtmd_system* sys, void* log_file);
 
 /* ---- batches of independent systems (replicas) ----

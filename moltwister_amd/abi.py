@@ -59,6 +59,19 @@ class MtmdConfig(C.Structure):
         ("opt_accuracy", C.c_double),
         ("verbose", C.c_int32),
         ("info_file", C.c_char * 256),
+        ("xyz_file", C.c_char * 256),
+        ("dcd_file", C.c_char * 256),
+        ("xtc_file", C.c_char * 256),
+        ("pdistr_file", C.c_char * 256),
+        ("vdistr_file", C.c_char * 256),
+        ("include_xyz", C.c_int32),
+        ("include_dcd", C.c_int32),
+        ("include_xtc", C.c_int32),
+        ("xtc_precision", C.c_float),
+        ("include_pdistr", C.c_int32),
+        ("include_vdistr", C.c_int32),
+        ("max_pdistr", C.c_double),
+        ("max_vdistr", C.c_double),
         ("engine", C.c_int32),
         ("coulomb", C.c_int32),
         ("skin", C.c_double),
@@ -107,6 +120,7 @@ class MtmdSystem(C.Structure):
         ("n_dihedrals", C.c_int32),
         ("dihedrals", _ip),
         ("seed", C.c_uint64),
+        ("type_names", C.POINTER(C.c_char_p)),
     ]
 
 
@@ -124,7 +138,7 @@ EXPORTED = [
     "mtmd_get_state", "mtmd_set_thermostat", "mtmd_get_thermostat", "mtmd_build_cells",
     "mtmd_build_neighbors", "mtmd_compute_forces", "mtmd_step", "mtmd_get_thermo",
     "mtmd_get_forces", "mtmd_optimize", "mtmd_get_stats", "mtmd_synchronize", "mtmd_run",
-    "mtmd_run_optimize", "mtmd_bench_kernel",
+    "mtmd_run_optimize", "mtmd_bench_kernel", "mtmd_append_xyz", "mtmd_append_dcd",
     "mtmd_batch_create", "mtmd_batch_destroy", "mtmd_batch_last_error", "mtmd_batch_step",
     "mtmd_batch_step_separate", "mtmd_batch_synchronize", "mtmd_batch_get_thermo", "mtmd_batch_get_state",
     "mtmd_batch_get_forces", "mtmd_batch_autotune", "mtmd_batch_info",
@@ -173,6 +187,8 @@ def load_library(path: str | None = None) -> C.CDLL:
         "mtmd_run": (C.c_int, [C.POINTER(MtmdConfig), C.POINTER(MtmdSystem), vp]),
         "mtmd_run_optimize": (C.c_int, [C.POINTER(MtmdConfig), C.POINTER(MtmdSystem), vp]),
         "mtmd_bench_kernel": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(C.c_double)]),
+        "mtmd_append_xyz": (C.c_int, [vp, C.c_char_p, C.c_int32]),
+        "mtmd_append_dcd": (C.c_int, [vp, C.c_char_p, C.c_int32, C.c_int32]),
         "mtmd_batch_create": (C.c_int, [C.c_int, C.POINTER(MtmdSystem), C.POINTER(MtmdConfig),
                                         C.POINTER(C.c_void_p)]),
         "mtmd_batch_destroy": (None, [vp]),

@@ -9,6 +9,7 @@ Translation units:
   mtmd_tables.cpp  force/energy profiles, g++ -ffp-contract=off so the float
                    tables are the ones a default x86-64 build of the reference
                    computes
+  mtmd_output.cpp  XYZ / DCD trajectory and histogram files of mtmd_run (g++)
   mtmd_gen.c       seeded workload generators (libmtgen.so, host only)
 Every HIP compile line carries ``-Xclang -target-feature -Xclang -packed-fp32-ops``
 (no packed FP32 instructions).
@@ -54,13 +55,15 @@ def build(verbose: bool = False) -> str:
          os.path.join(SRC, "mtmd_host.cpp")],
         [shutil.which("g++") or "g++", "-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-c",
          "-o", os.path.join(OUT, "mtmd_tables.o"), os.path.join(SRC, "mtmd_tables.cpp")],
+        [shutil.which("g++") or "g++", "-O2", "-fPIC", "-std=c++17", "-Wall", "-c",
+         "-o", os.path.join(OUT, "mtmd_output.o"), os.path.join(SRC, "mtmd_output.cpp")],
     ]
-    with ThreadPoolExecutor(max_workers=6) as ex:
+    with ThreadPoolExecutor(max_workers=7) as ex:
         for r in ex.map(_run, jobs):
             if verbose and (r.stdout or r.stderr):
                 sys.stderr.write(r.stdout + r.stderr)
     objs = [os.path.join(OUT, f) for f in ("mtmd_exact.o", "mtmd_fast.o", "mtmd_batch.o", "mtmd_fm.o",
-                                           "mtmd_host.o", "mtmd_tables.o")]
+                                           "mtmd_host.o", "mtmd_tables.o", "mtmd_output.o")]
     # workload generator helpers (host only)
     _run([shutil.which("gcc") or "gcc", "-O2", "-fPIC", "-shared", "-o", GENLIB, os.path.join(SRC, "mtmd_gen.c"), "-lm"])
     tmp = LIB + ".tmp"

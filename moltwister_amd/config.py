@@ -48,9 +48,11 @@ _NUMERIC = {
     "skin": ("skin", float),
     "blocksize": ("block_size", int),
 }
-_YESNO = ["includexyzfile", "includedcdfile", "includextcfile", "includepdistrfile",
-          "includevdistrfile", "verboseoutput"]
-_FILES = ["infofile", "xyzfile", "dcdfile", "xtcfile", "pdistrfile", "vdistrfile"]
+# key -> mtmd_config field
+_YESNO = {"includexyzfile": "include_xyz", "includedcdfile": "include_dcd", "includextcfile": "include_xtc",
+          "includepdistrfile": "include_pdistr", "includevdistrfile": "include_vdistr", "verboseoutput": "verbose"}
+_FILES = {"infofile": "info_file", "xyzfile": "xyz_file", "dcdfile": "dcd_file", "xtcfile": "xtc_file",
+          "pdistrfile": "pdistr_file", "vdistrfile": "vdistr_file"}
 
 
 def _atoi(s: str) -> int:
@@ -96,6 +98,19 @@ def defaults() -> abi.MtmdConfig:
     c.cutoff_force = 1000.0
     c.n_steps = 50000
     c.info_file = b"out.txt"
+    c.xyz_file = b"traj.xyz"
+    c.dcd_file = b"traj.dcd"
+    c.xtc_file = b"traj.xtc"
+    c.pdistr_file = b"distr"
+    c.vdistr_file = b"distr"
+    c.include_xyz = 0
+    c.include_dcd = 1
+    c.include_xtc = 0
+    c.xtc_precision = 1000.0
+    c.include_pdistr = 0
+    c.include_vdistr = 0
+    c.max_pdistr = 0.3
+    c.max_vdistr = 100.0e3
     c.verbose = 0
     c.scale12 = 0.0
     c.scale13 = 0.0
@@ -116,13 +131,6 @@ class MolDynConfig:
 
     def __init__(self) -> None:
         self.cfg = defaults()
-        self.files = {"infofile": "out.txt", "xyzfile": "traj.xyz", "dcdfile": "traj.dcd",
-                      "xtcfile": "traj.xtc", "pdistrfile": "distr", "vdistrfile": "distr"}
-        self.flags = {"includexyzfile": False, "includedcdfile": True, "includextcfile": False,
-                      "includepdistrfile": False, "includevdistrfile": False, "verboseoutput": False}
-        self.xtcprecision = 1000.0
-        self.maxpdistr = 0.3
-        self.maxvdistr = 100.0e3
 
     def reset_to_defaults(self) -> None:
         self.__init__()
@@ -139,24 +147,20 @@ class MolDynConfig:
                 return "Error: unknown ensemble!"
             c.ensemble = m[value]
         elif parameter in _FILES:
-            self.files[parameter] = value
-            if parameter == "infofile":
-                c.info_file = value.encode()[:255]
+            setattr(c, _FILES[parameter], value.encode()[:255])
         elif parameter in _YESNO:
             if value == "yes":
-                self.flags[parameter] = True
+                setattr(c, _YESNO[parameter], 1)
             elif value == "no":
-                self.flags[parameter] = False
+                setattr(c, _YESNO[parameter], 0)
             else:
                 return f"Error: '{parameter}' parameter should be either 'yes' or 'no'!"
-            if parameter == "verboseoutput":
-                c.verbose = int(self.flags[parameter])
         elif parameter == "xtcprecision":
-            self.xtcprecision = _atof(value)
+            c.xtc_precision = _atof(value)
         elif parameter == "maxpdistr":
-            self.maxpdistr = _atof(value)
+            c.max_pdistr = _atof(value)
         elif parameter == "maxvdistr":
-            self.maxvdistr = _atof(value)
+            c.max_vdistr = _atof(value)
         elif parameter == "engine":
             m = {"exact": abi.ENGINE_EXACT, "fp64": abi.ENGINE_FP64, "mixed": abi.ENGINE_MIXED}
             if value not in m:

@@ -482,3 +482,44 @@ void mt_gradient_step(const MtDev& d, double lr, hipStream_t s)
 {
     hipLaunchKernelGGL(k_gradient_step, dim3(nblk(d.n, TPB)), dim3(TPB), 0, s, d.pos, d.frc, d.n, lr);
 }
+
+// CMDLoop::appendToMomentumDistribution (MDLoop.cpp:304-322), one step: each
+// momentum component of every atom goes to bin int(((p + maxP)/(2 maxP))*nbins)
+// of its axis (hist[axis*nbins + bin], counted when the bin is in range), and
+// the first thermostat momentum, when peta is given, to hist[3*nbins + bin]
+// with its own maximum. Counts gather in LDS per workgroup, then one atomic
+// per non-empty bin goes to global memory.
+#define MT_DISTR_MAXBINS 256
+__global__ void __launch_bounds__(TPB) k_pdistr(const double4* __restrict__ mom, int n, double maxP, int nbins,
+                                                const double* __restrict__ peta, double maxPeta,
+                                                int* __restrict__ hist)
+{
+    __shared__ int sh[3 * MT_DISTR_MAXBINS];
+    for (int i = threadIdx.x; i < 3 * nbins; i += blockDim.x) sh[i] = 0;
+    __syncthreads();
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const double4 pm = mom[k];
+        const double c[3] = {pm.x, pm.y, pm.z};
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            const int b = mt_d2i(((c[a] + maxP) / (2.0 * maxP)) * (double)nbins);
+            if (b >= 0 && b < nbins) atomicAdd(&sh[a * nbins + b], 1);
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < 3 * nbins; i += blockDim.x)
+        if (sh[i]) atomicAdd(&hist[i], sh[i]);
+    if (peta && blockIdx.x == 0 && threadIdx.x == 0) {
+        const int b = mt_d2i(((peta[0] + maxPeta) / (2.0 * maxPeta)) * (double)nbins);
+        if (b >= 0 && b < nbins) atomicAdd(&hist[3 * nbins + b], 1);
+    }
+}
+
+int mt_pdistr(const MtDev& d, double maxP, int nbins, const double* peta, double maxPeta, int* hist, hipStream_t s)
+{
+    if (nbins < 1 || nbins > MT_DISTR_MAXBINS) return -1;
+    int nb = nblk(d.n, TPB);
+    nb = nb < 1 ? 1 : (nb > 1024 ? 1024 : nb);
+    hipLaunchKernelGGL(k_pdistr, dim3(nb), dim3(TPB), 0, s, d.mom, d.n, maxP, nbins, peta, maxPeta, hist);
+    return 0;
+}

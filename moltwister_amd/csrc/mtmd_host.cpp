@@ -23,6 +23,7 @@
 
 #include "../../include/mtmd.h"
 #include "mtmd_internal.h"
+#include "mtmd_output.h"
 
 namespace {
 
@@ -66,6 +67,10 @@ struct mtmd_handle {
     hipStream_t stream = nullptr;
     DevBuf mem;
     std::string err;
+
+    // atom types and type names (XYZ frames)
+    std::vector<int32_t> type_host;
+    std::vector<std::string> type_names;
 
     // thermostat (NHChain.h:55-66)
     int M = 0, nresp = 0;
@@ -164,6 +169,19 @@ extern "C" void mtmd_config_defaults(mtmd_config* c)
     c->cutoff_force = 1000.0;
     c->n_steps = 50000;
     std::snprintf(c->info_file, sizeof c->info_file, "%s", "out.txt");
+    std::snprintf(c->xyz_file, sizeof c->xyz_file, "%s", "traj.xyz");
+    std::snprintf(c->dcd_file, sizeof c->dcd_file, "%s", "traj.dcd");
+    std::snprintf(c->xtc_file, sizeof c->xtc_file, "%s", "traj.xtc");
+    std::snprintf(c->pdistr_file, sizeof c->pdistr_file, "%s", "distr");
+    std::snprintf(c->vdistr_file, sizeof c->vdistr_file, "%s", "distr");
+    c->include_xyz = 0;
+    c->include_dcd = 1;
+    c->include_xtc = 0;
+    c->xtc_precision = 1000.0f;
+    c->include_pdistr = 0;
+    c->include_vdistr = 0;
+    c->max_pdistr = 0.3;
+    c->max_vdistr = 100.0E3;
     c->scale12 = 0.0;
     c->scale13 = 0.0;
     c->scale14 = 0.5;
@@ -176,9 +194,8 @@ extern "C" void mtmd_config_defaults(mtmd_config* c)
     c->skin = 0.3;
 }
 
-// CMolDynConfig::set (Config/MolDynConfig.cpp:95-287); output-file keys the
-// engine does not consume are accepted and ignored, as strings the reference
-// only stores.
+// CMolDynConfig::set (Config/MolDynConfig.cpp:95-287): the reference's keys,
+// atoi/atof parsing and error strings, plus the engine options.
 extern "C" int mtmd_config_set(mtmd_config* c, const char* key, const char* value, char* err, size_t err_len)
 {
     auto fail = [&](const char* msg) {
@@ -209,18 +226,28 @@ extern "C" int mtmd_config_set(mtmd_config* c, const char* key, const char* valu
         else return fail("Error: unknown ensemble!");
         return MTMD_OK;
     }
-    if (k == "infofile") { std::snprintf(c->info_file, sizeof c->info_file, "%s", value); return MTMD_OK; }
-    if (k == "xyzfile" || k == "dcdfile" || k == "xtcfile" || k == "pdistrfile" || k == "vdistrfile" ||
-        k == "xtcprecision" || k == "maxpdistr" || k == "maxvdistr")
-        return MTMD_OK;
-    if (k == "includexyzfile" || k == "includedcdfile" || k == "includextcfile" || k == "includepdistrfile" ||
-        k == "includevdistrfile" || k == "verboseoutput") {
+    struct FileKey { const char* name; char* field; size_t size; };
+    const FileKey fk[] = {{"infofile", c->info_file, sizeof c->info_file}, {"xyzfile", c->xyz_file, sizeof c->xyz_file},
+                          {"dcdfile", c->dcd_file, sizeof c->dcd_file}, {"xtcfile", c->xtc_file, sizeof c->xtc_file},
+                          {"pdistrfile", c->pdistr_file, sizeof c->pdistr_file},
+                          {"vdistrfile", c->vdistr_file, sizeof c->vdistr_file}};
+    for (const auto& e : fk)
+        if (k == e.name) { std::snprintf(e.field, e.size, "%s", value); return MTMD_OK; }
+    if (k == "xtcprecision") { c->xtc_precision = (float)std::atof(value); return MTMD_OK; }
+    if (k == "maxpdistr") { c->max_pdistr = std::atof(value); return MTMD_OK; }
+    if (k == "maxvdistr") { c->max_vdistr = std::atof(value); return MTMD_OK; }
+    struct FlagKey { const char* name; int32_t* field; };
+    const FlagKey gk[] = {{"includexyzfile", &c->include_xyz}, {"includedcdfile", &c->include_dcd},
+                          {"includextcfile", &c->include_xtc}, {"includepdistrfile", &c->include_pdistr},
+                          {"includevdistrfile", &c->include_vdistr}, {"verboseoutput", &c->verbose}};
+    for (const auto& e : gk) {
+        if (k != e.name) continue;
         if (v != "yes" && v != "no") {
             char msg[160];
             std::snprintf(msg, sizeof msg, "Error: '%s' parameter should be either 'yes' or 'no'!", key);
             return fail(msg);
         }
-        if (k == "verboseoutput") c->verbose = (v == "yes");
+        *e.field = (v == "yes");
         return MTMD_OK;
     }
     if (k == "engine") {
@@ -761,6 +788,10 @@ static int setup(mtmd_handle* h, const mtmd_system* s)
     }
     nh[3 * M] = 1.0;
     h->nh_scale_slot = 3 * M;
+    h->type_host.assign(s->type, s->type + n);
+    h->type_names.assign(T, std::string());
+    for (int t = 0; t < T; t++)
+        h->type_names[t] = (s->type_names && s->type_names[t]) ? std::string(s->type_names[t]) : std::to_string(t);
     if ((rc2 = upload(h, d.nh, nh))) return rc2;
     h->h_status = nullptr;
     HIPCHK(h, hipHostMalloc((void**)&h->h_status, 16 * sizeof(int), hipHostMallocDefault));
@@ -1288,9 +1319,14 @@ extern "C" int mtmd_get_thermo(mtmd_handle* h, mtmd_thermo* out)
     return MTMD_OK;
 }
 
-extern "C" int mtmd_optimize(mtmd_handle* h, int32_t max_steps, double lr, double accuracy, int32_t* steps_done)
+// CMDLoop::runOptimization (MDLoop.cpp:143-173): per step, forces at the current
+// positions, r += lr F (CSimulationBox::gradientDescentStep), U of those forces;
+// stop when |dU| < accuracy after more than out_every steps. per_step(i, seconds)
+// runs after step i (the reference's updateOutput); a non-zero return ends the loop.
+template <class PerStep>
+static int optimize_loop(mtmd_handle* h, int max_steps, double lr, double accuracy, int out_every,
+                         int32_t* steps_done, PerStep&& per_step)
 {
-    if (!h) return MTMD_EINVAL;
     int rc;
     auto utot = [&](double& u) -> int {
         mt_reduce_thermo(h->d, h->stream);
@@ -1307,18 +1343,30 @@ extern "C" int mtmd_optimize(mtmd_handle* h, int32_t max_steps, double lr, doubl
     if ((rc = utot(Un))) return rc;
     int i = 0;
     for (i = 0; i < max_steps; i++) {
+        const auto t0 = std::chrono::steady_clock::now();
         if ((rc = compute_forces_now(h))) return rc;
         mt_gradient_step(h->d, lr, h->stream);
         double Unp1 = 0.0;
         if ((rc = utot(Unp1))) return rc;
         const double dU = std::fabs(Unp1 - Un);
         Un = Unp1;
-        if (dU < accuracy && i > h->cfg.out_stride) { i++; break; }
+        const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        // the forces stay those of the positions before the step, as the
+        // reference's F (its thermo line and the next U use them)
+        h->ke_valid = false;
+        if ((rc = per_step(i, sec))) return rc;
+        if (dU < accuracy && i > out_every) { i++; break; }
     }
     h->have_forces = false;
     h->ke_valid = false;
     if (steps_done) *steps_done = i;
     return MTMD_OK;
+}
+
+extern "C" int mtmd_optimize(mtmd_handle* h, int32_t max_steps, double lr, double accuracy, int32_t* steps_done)
+{
+    if (!h) return MTMD_EINVAL;
+    return optimize_loop(h, max_steps, lr, accuracy, h->cfg.out_stride, steps_done, [](int, double) { return 0; });
 }
 
 extern "C" int mtmd_get_stats(mtmd_handle* h, int64_t* rebuilds, int64_t* steps, int32_t* max_neighbors)
@@ -1391,26 +1439,153 @@ static void tee(FILE* a, FILE* b, const char* fmt, ...)
     va_start(ap, fmt);
     vsnprintf(buf, sizeof buf, fmt, ap);
     va_end(ap);
-    if (a) std::fputs(buf, a);
+    if (a) { std::fputs(buf, a); std::fflush(a); }
     if (b) std::fputs(buf, b);
 }
+
+// CMDLoop::appendToXYZFile (MDLoop.cpp:229-246)
+extern "C" int mtmd_append_xyz(mtmd_handle* h, const char* path, int32_t t)
+{
+    if (!h || !path) return MTMD_EINVAL;
+    std::vector<double> xyz(3 * (size_t)h->n);
+    int rc = mtmd_get_state(h, xyz.data(), nullptr);
+    if (rc) return rc;
+    std::vector<const char*> names(h->T);
+    for (int i = 0; i < h->T; i++) names[i] = h->type_names[i].c_str();
+    if (mt_xyz_append(path, h->n, t, xyz.data(), h->type_host.data(), names.data(), h->T))
+        return h->fail(MTMD_EINVAL, "cannot open %s", path);
+    return MTMD_OK;
+}
+
+// CMDLoop::appendToDCDFile (MDLoop.cpp:248-258), DCDFile.cpp:670-729
+extern "C" int mtmd_append_dcd(mtmd_handle* h, const char* path, int32_t n_steps, int32_t stride)
+{
+    if (!h || !path) return MTMD_EINVAL;
+    std::vector<double> xyz(3 * (size_t)h->n);
+    int rc = mtmd_get_state(h, xyz.data(), nullptr);
+    if (rc) return rc;
+    // the record's box is the current simulation box narrowed to int
+    // (std::tuple<int, int, int> parameter of CDCDFile::appendToDCDFile)
+    const int box[3] = {(int)h->p.L[0], (int)h->p.L[1], (int)h->p.L[2]};
+    if (mt_dcd_create_if_missing(path, n_steps, stride, h->cfg.timestep, h->n) ||
+        mt_dcd_append(path, h->n, box, xyz.data()))
+        return h->fail(MTMD_EINVAL, "cannot write %s", path);
+    return MTMD_OK;
+}
+
+// Trajectory frames of CMDLoop::updateOutput (MDLoop.cpp:384-386). Without the
+// Gromacs XTC library the reference prints an error and writes the frame to
+// the DCD file instead (MDLoop.cpp:288-292), which is what happens here.
+static int write_frames(mtmd_handle* h, const mtmd_config* cfg, int t, int n_steps, int stride)
+{
+    int rc = MTMD_OK;
+    if (cfg->include_xyz && (rc = mtmd_append_xyz(h, cfg->xyz_file, t))) return rc;
+    if (cfg->include_dcd && (rc = mtmd_append_dcd(h, cfg->dcd_file, n_steps, stride))) return rc;
+    if (cfg->include_xtc) {
+        std::printf("Error, missing library. MolTwister was not built with the Gromacs XTC library, likely because "
+                    "it was not available during the MolTwister build. Using DCD instead!\r\n");
+        if ((rc = mtmd_append_dcd(h, cfg->dcd_file, n_steps, stride))) return rc;
+    }
+    return rc;
+}
+
+// Histograms of CMDLoop (MDLoop.cpp:104-140,296-366,407-432): momenta of every
+// atom per axis and the first thermostat momentum, and the volume, every step
+// after the equilibration steps; written when the run ends.
+struct Distributions {
+    static constexpr int kBins = 150;          // numPDistrBins (MDLoop.cpp:111)
+    static constexpr int kEquilib = 1000;      // equilibSteps (MDLoop.cpp:112)
+    bool p_on = false, v_on = false;
+    double maxP = 0, maxPeta = 0, maxV = 0;
+    int* dev = nullptr;                        // [4 * kBins] on the device
+    std::vector<int> vol;
+
+    int init(mtmd_handle* h, const mtmd_config* cfg)
+    {
+        p_on = cfg->include_pdistr != 0;
+        v_on = cfg->include_vdistr != 0;
+        maxP = cfg->max_pdistr / 1.0E-2;                     // Conv_P (Constants.h:28)
+        // calcMaxPetaDistrOutput (MDLoop.cpp:429-432): 2 Q_0 / tau of the
+        // temperature chain, Q_0 = dim N T tau^2 (NHChain.cpp:115-124)
+        const double T = cfg->temperature / MT_CONV_T, tau = cfg->temperature_relax / MT_CONV_t;
+        maxPeta = 2.0 * ((double)(3 * h->n) * T * tau * tau) / tau;
+        maxV = cfg->max_vdistr;
+        vol.assign(kBins, 0);
+        if (p_on) {
+            dev = h->mem.alloc<int>(4 * kBins);
+            if (!dev) return h->fail(MTMD_ENOMEM, "histogram allocation failed");
+            HIPCHK(h, hipMemsetAsync(dev, 0, 4 * kBins * sizeof(int), h->stream));
+        }
+        return MTMD_OK;
+    }
+    bool active(int t) const { return (p_on || v_on) && t > kEquilib; }
+    int accumulate(mtmd_handle* h)
+    {
+        if (p_on) {
+            // NVE has no active chain; its thermostat momentum stays at rest (0)
+            const double* peta = (h->M > 0) ? h->d.nh + h->M : nullptr;
+            if (mt_pdistr(h->d, maxP, kBins, peta, maxPeta, dev, h->stream))
+                return h->fail(MTMD_EINVAL, "histogram bins");
+            if (!peta) {
+                const int b = (int)(((0.0 + maxPeta) / (2.0 * maxPeta)) * (double)kBins);
+                peta_rest_bin_hits += (b >= 0 && b < kBins) ? 1 : 0;
+                peta_rest_bin = b;
+            }
+        }
+        if (v_on) {
+            // CMDLoop::appendToVolumeDistribution (MDLoop.cpp:324-328)
+            const double V = h->p.L[0] * h->p.L[1] * h->p.L[2];
+            const int i = (int)((V / maxV) * (double)kBins);
+            if (i < kBins) vol[i]++;
+        }
+        return MTMD_OK;
+    }
+    int finish(mtmd_handle* h, const mtmd_config* cfg)
+    {
+        if (p_on) {
+            std::vector<int> hp(4 * kBins);
+            HIPCHK(h, hipMemcpyAsync(hp.data(), dev, hp.size() * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+            HIPCHK(h, hipStreamSynchronize(h->stream));
+            if (peta_rest_bin_hits) hp[3 * kBins + peta_rest_bin] += peta_rest_bin_hits;
+            const char* suffix[4] = {"_p_x.dat", "_p_y.dat", "_p_z.dat", "_p_eta.dat"};
+            for (int a = 0; a < 4; a++) {
+                const std::string f = std::string(cfg->pdistr_file) + suffix[a];
+                (void)mt_store_pdistr(f.c_str(), hp.data() + a * kBins, kBins, a < 3 ? maxP : maxPeta, a);
+            }
+        }
+        if (v_on) {
+            const std::string f = std::string(cfg->vdistr_file) + "_v.dat";
+            (void)mt_store_vdistr(f.c_str(), vol.data(), kBins, maxV);
+        }
+        return MTMD_OK;
+    }
+    int peta_rest_bin = 0, peta_rest_bin_hits = 0;
+};
 
 static int run_common(const mtmd_config* cfg, const mtmd_system* sys, void* log_file, bool optimize)
 {
     mtmd_handle* h = nullptr;
     int rc = mtmd_create(sys, cfg, &h);
     if (rc) return rc;
+    // COut::setOutputFile prints a warning when the info file cannot be opened (Printf.cpp:46-50)
     FILE* info = cfg->info_file[0] ? std::fopen(cfg->info_file, "w") : nullptr;
     FILE* log = (FILE*)log_file;
+    if (!info) tee(nullptr, log, "Warning! Could not open output file!\r\n");
     const int stride = std::max(1, cfg->out_stride);
-    const char* ens = (cfg->ensemble == MTMD_NVT) ? "NVT" : "NVE";
+    const char* ens = (cfg->ensemble == MTMD_NPT) ? "NPT" : ((cfg->ensemble == MTMD_NVT) ? "NVT" : "NVE");
+    const bool thermostat = (cfg->ensemble == MTMD_NVT || cfg->ensemble == MTMD_NPT);
     tee(info, log, "\r\n\t----------------------------\r\n\t%s\r\n\t----------------------------\r\n",
         optimize ? "Optimization" : "Molecular dynamics simulation");
     if (!optimize) {
         tee(info, log, "\t Ensemble = %s\r\n", ens);
-        if (cfg->ensemble == MTMD_NVT) tee(info, log, "\t Temperature, T = %g K\r\n", cfg->temperature);
-        if (cfg->ensemble == MTMD_NVT) tee(info, log, "\t NH T relax, tau = %g fs\r\n", cfg->temperature_relax);
-        tee(info, log, "\t Timestep, dt = %g fs\r\n", cfg->timestep);
+        if (thermostat) tee(info, log, "\t Temperature, T = %g K\r\n", (cfg->temperature / MT_CONV_T) * MT_CONV_T);
+        if (cfg->ensemble == MTMD_NPT)
+            tee(info, log, "\t Pressure, P = %g atm\r\n", (cfg->pressure / MT_CONV_PRESS) * MT_CONV_PRESS);
+        if (thermostat)
+            tee(info, log, "\t NH T relax, tau = %g fs\r\n", (cfg->temperature_relax / MT_CONV_t) * MT_CONV_t);
+        if (cfg->ensemble == MTMD_NPT)
+            tee(info, log, "\t NH P relax, tau = %g fs\r\n", (cfg->pressure_relax / MT_CONV_t) * MT_CONV_t);
+        tee(info, log, "\t Timestep, dt = %g fs\r\n", (cfg->timestep / MT_CONV_t) * MT_CONV_t);
         tee(info, log, "\t Particles, N = %i\r\n", sys->n_atoms);
         tee(info, log, "\t NH RESPA, n = %i\r\n", cfg->temperature_respa);
         tee(info, log, "\t NH Chain length, M = %i\r\n", cfg->temperature_nh_len);
@@ -1422,34 +1597,59 @@ static int run_common(const mtmd_config* cfg, const mtmd_system* sys, void* log_
     tee(info, log, "\t----------------------------\r\n\r\n");
     tee(info, log, "\t%-15s%-17s%-15s%-15s%-20s%-15s%-15s%-15s\r\n", optimize ? "Step" : "Timestep", "Avg.steptime[s]",
         "Temp[K]", "Press[atm]", "Vol[AA^3]", "U[kJ/mol]", "K[kJ/mol]", "Etot[kJ/mol]");
-    auto line = [&](int t, double avg) {
-        mtmd_thermo th{};
-        int r = mtmd_get_thermo(h, &th);
+    // one output of CMDLoop::updateOutput (MDLoop.cpp:380-404): frames, then the thermo line
+    // Avg.steptime: mean wall time of the steps since the last line (CMDLoop::readTimerAverage)
+    double acc_s = 0.0;
+    int acc_n = 0;
+    auto output = [&](int t, int total) -> int {
+        const double avg = acc_n ? acc_s / acc_n : -1.0;
+        acc_s = 0.0;
+        acc_n = 0;
+        int r = write_frames(h, cfg, t, total, stride);
         if (r) return r;
-        // the reference passes (avg, t) to a "%-15i%-17.8f" format (MDLoop.cpp:402); the
-        // intended columns are printed here
+        mtmd_thermo th{};
+        if ((r = mtmd_get_thermo(h, &th))) return r;
+        // the reference passes (avg, t) to a "%-15i%-17.8f" format (MDLoop.cpp:402),
+        // which the x86-64 calling convention prints as these columns
         tee(info, log, "\t%-15i%-17.8f%-15g%-15g%-20g%-15g%-15g%-15g\r\n", t, avg, th.T, th.P, th.V, th.U, th.K, th.Etot);
         return 0;
     };
     if (!optimize) {
+        Distributions dist;
+        rc = dist.init(h, cfg);
         const int total = cfg->n_steps;
         int done = 0;
         while (done < total && !rc) {
-            // steps t = done .. done+chunk-1; output after step t when t % stride == 0
-            int next = (done % stride == 0) ? done : done + (stride - done % stride);
-            int chunk = std::min(next, total - 1) - done + 1;
+            // steps t = done .. done+chunk-1; output after step t when t % stride == 0;
+            // past the equilibration steps with histograms on, one step at a time
+            int chunk;
+            if (dist.active(done)) {
+                chunk = 1;
+            } else {
+                int next = (done % stride == 0) ? done : done + (stride - done % stride);
+                if ((dist.p_on || dist.v_on) && done <= Distributions::kEquilib)
+                    next = std::min(next, Distributions::kEquilib);
+                chunk = std::min(next, total - 1) - done + 1;
+            }
             auto t0 = std::chrono::steady_clock::now();
             rc = md_steps(h, chunk, true);
             if (rc) break;
             (void)mtmd_synchronize(h);
-            double avg = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() / chunk;
+            acc_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            acc_n += chunk;
             done += chunk;
-            if ((done - 1) % stride == 0) rc = line(done - 1, avg);
+            const int t = done - 1;
+            if (dist.active(t) && (rc = dist.accumulate(h))) break;
+            if (t % stride == 0) rc = output(t, total);
         }
+        if (!rc) rc = dist.finish(h, cfg);
     } else {
-        int steps = 0;
-        rc = mtmd_optimize(h, cfg->opt_max_steps, cfg->opt_learning_rate, cfg->opt_accuracy, &steps);
-        if (!rc) rc = line(steps, 0.0);
+        rc = optimize_loop(h, cfg->opt_max_steps, cfg->opt_learning_rate, cfg->opt_accuracy, stride, nullptr,
+                           [&](int i, double sec) {
+                               acc_s += sec;
+                               acc_n++;
+                               return (i % stride == 0) ? output(i, cfg->opt_max_steps) : 0;
+                           });
     }
     if (rc && log) std::fprintf(log, "Error: %s\r\n", h->err.c_str());
     if (info) std::fclose(info);

@@ -125,6 +125,9 @@ void mt_scale_wrap(const MtDev& d, const MtParams& p, int scale_slot, int want_k
 void mt_reduce_ke(const MtDev& d, hipStream_t s);
 void mt_reduce_thermo(const MtDev& d, hipStream_t s);
 void mt_gradient_step(const MtDev& d, double lr, hipStream_t s);
+// one step of the momentum histograms of mtmd_run (hist [4*nbins] ints on the
+// device, nbins <= 256; peta NULL skips the thermostat bin); -1 on bad nbins
+int mt_pdistr(const MtDev& d, double maxP, int nbins, const double* peta, double maxPeta, int* hist, hipStream_t s);
 
 // ---- launchers (mtmd_fast.hip) ----
 // `conditional` kernels return at once unless this step's maximum

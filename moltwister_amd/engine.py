@@ -149,6 +149,24 @@ class Engine:
     def synchronize(self):
         self._chk(self.lib.mtmd_synchronize(self.h))
 
+    def append_xyz(self, path: str, t: int):
+        """CMDLoop::appendToXYZFile: one frame of the current positions."""
+        self._chk(self.lib.mtmd_append_xyz(self.h, path.encode(), int(t)))
+
+    def append_dcd(self, path: str, n_steps: int, stride: int):
+        """CMDLoop::appendToDCDFile: header when the file is new, then one frame."""
+        self._chk(self.lib.mtmd_append_dcd(self.h, path.encode(), int(n_steps), int(stride)))
+
+
+def run(system: System, cfg: abi.MtmdConfig, optimize: bool = False) -> None:
+    """CMDSimulator::run / optimize through mtmd_run / mtmd_run_optimize: the
+    files named by cfg (thermo table, XYZ, DCD, histograms) are written."""
+    lib = abi.load_library()
+    cs = system.to_c()
+    rc = (lib.mtmd_run_optimize if optimize else lib.mtmd_run)(C.byref(cfg), C.byref(cs), None)
+    if rc:
+        raise MtmdError(rc, lib.mtmd_last_error(None).decode() or "mtmd_run failed")
+
 
 class BatchEngine:
     """Many independent systems advanced together (mtmd_batch_*): one

@@ -114,6 +114,10 @@ class System:
         s.n_angles, s.angles = len(self.angles), _ptr(self.angles, C.c_int32)
         s.n_dihedrals, s.dihedrals = len(self.dihedrals), _ptr(self.dihedrals, C.c_int32)
         s.seed = int(self.seed) & 0xFFFFFFFFFFFFFFFF
+        if self.type_names:
+            names = (C.c_char_p * len(self.type_names))(*[str(x).encode() for x in self.type_names])
+            s.type_names = C.cast(names, C.POINTER(C.c_char_p))
+            keep.append(names)
         keep.extend([self.pos, self.mom, self.mass, self.charge, self.types, self.mobile,
                      self.bonds, self.angles, self.dihedrals])
         s._keep = keep

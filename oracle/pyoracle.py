@@ -110,6 +110,8 @@ def _ref_lib():
         L.ref_cmt_exp.argtypes = [C.c_double]
         L.ref_cmt_sinhxoverx.restype = C.c_double
         L.ref_cmt_sinhxoverx.argtypes = [C.c_double]
+        L.ref_dcd_append.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_double, C.c_int, C.c_double, C.c_double,
+                                     C.c_double, _dp]
         _rlib = L
     return _rlib
 
@@ -123,6 +125,14 @@ def profile(which: int, kind: int, params, x0: float, x1: float, n: int = 1000, 
     rc = fn(which, kind, _f64(p), x0, x1, n, _f32(x), _f32(f), _f32(e))
     assert rc == 0
     return x, f, e
+
+
+def ref_dcd_append(path: str, n_steps: int, stride: int, timestep: float, xyz, box) -> None:
+    """MolTwister's DCD writer (CDCDFile, as CMDLoop::appendToDCDFile calls it):
+    header if the file is new, then one frame of xyz (n, 3)."""
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+    _ref_lib().ref_dcd_append(path.encode(), int(n_steps), int(stride), float(timestep), xyz.shape[0],
+                              float(box[0]), float(box[1]), float(box[2]), _f64(xyz))
 
 
 class _Base:

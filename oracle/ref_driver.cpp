@@ -11,6 +11,8 @@
 //   - neighbours:CFunctorGenNeighList::operator() with the grow-and-retry loop
 //   - forces:    CFunctorCalcForce::operator()
 //   - NH chain:  CNHChain::propagator, CMt::exp/sinhXoverX
+//   - DCD files: CDCDFile::createDCDFileIfNotExists + appendToDCDFile, called
+//                as CMDLoop::appendToDCDFile calls them (MDLoop.cpp:248-258)
 //
 // What the driver restates (the reference code for it needs CMolTwisterState,
 // whose C3DView translation unit needs GLUT, absent in this image):
@@ -45,6 +47,7 @@
 #include "MDFF/Bonds/MolTwisterMDFFBond_Morse.h"
 #include "MDFF/Angles/MolTwisterMDFFAngle_Harm.h"
 #include "MDFF/Dihedrals/MolTwisterMDFFDih_Harm.h"
+#include "Utilities/DCDFile.h"
 #undef private
 #undef protected
 
@@ -492,4 +495,16 @@ EXPORT void ref_nh_propagate(ref_md* o, double* mom_out)
     DrvFctT f(o);
     o->nh.propagator(o->n, 3, o->dt, f);
     if (mom_out) for (int i = 0; i < 3 * o->n; i++) mom_out[i] = o->p[i];
+}
+
+// CMDLoop::appendToDCDFile (MDLoop.cpp:248-258): header when the file is new,
+// then one record of the positions xyz[3n] with the box narrowed to int
+EXPORT void ref_dcd_append(const char* path, int n_steps, int stride, double timestep, int n, double box_x,
+                           double box_y, double box_z, const double* xyz)
+{
+    std::function<std::tuple<double, double, double>(const int&)> get = [xyz](const int& i) {
+        return std::tuple<double, double, double>(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
+    };
+    CDCDFile::createDCDFileIfNotExists(path, n_steps, stride, timestep, n);
+    CDCDFile::appendToDCDFile(path, n, {box_x, box_y, box_z}, get);
 }

@@ -60,6 +60,18 @@ def main():
         prof[f"p{i}_x"], prof[f"p{i}_f"], prof[f"p{i}_e"] = x, f, e
         prof[f"p{i}_args"] = np.array([which, kind, *(list(prm) + [0.0] * (3 - len(prm))), x0, x1], float)
     np.savez_compressed(os.path.join(HERE, "profiles.npz"), **prof)
+    # two DCD frames written by MolTwister's CDCDFile (header + records)
+    import tempfile
+    rng = np.random.default_rng(5)
+    frames = rng.uniform(-6.0, 6.0, (2, 11, 3))
+    box = np.array([12.75, 13.2, 9.99])
+    with tempfile.TemporaryDirectory() as tmp:
+        path = os.path.join(tmp, "ref.dcd")
+        for f in frames:
+            pyoracle.ref_dcd_append(path, 1000, 100, 0.5, f, box)
+        data = np.frombuffer(open(path, "rb").read(), dtype=np.uint8)
+    np.savez_compressed(os.path.join(HERE, "dcd_frames.npz"), frames=frames, box=box,
+                        args=np.array([1000, 100, 0.5]), dcd=data)
 
 
 if __name__ == "__main__":

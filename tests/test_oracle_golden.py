@@ -12,7 +12,7 @@ from moltwister_amd import abi, config
 from moltwister_amd.system import System
 
 HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
-CASES = sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(HERE, "*.npz")) if "profiles" not in p)
+CASES = sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(HERE, "*.npz")) if "profiles" not in p and "dcd_frames" not in p)
 
 
 def load_case(name):

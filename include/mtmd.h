@@ -252,6 +252,16 @@ int mtmd_get_state(mtmd_handle* h, double* pos, double* mom);
 int mtmd_set_thermostat(mtmd_handle* h, const double* eta, const double* p_eta);
 int mtmd_get_thermostat(mtmd_handle* h, double* eta, double* p_eta);
 
+/* NPT barostat of CVelVerlet (VelVerlet.h: eps_, p_eps_) and its chain NH_P_
+ * (NHChain.h:59-61, arrays of pressure_nh_len). The reference draws p_eps and
+ * the chain at random (VelVerlet.cpp:176-183, NHChain.cpp:126-143); a handle
+ * starts them at rest, and these calls set or read any state. Setting eps
+ * rescales the box to Lmax (V0 exp(3 eps) / V0)^(1/3) (getSimBox,
+ * VelVerlet.cpp:199-211); box[3] receives the current simulation box.
+ * mtmd_set_barostat fails outside the npt ensemble. */
+int mtmd_set_barostat(mtmd_handle* h, double eps, double p_eps, const double* eta, const double* p_eta);
+int mtmd_get_barostat(mtmd_handle* h, double* eps, double* p_eps, double* eta, double* p_eta, double* box);
+
 /* CMDFFMatrices::genNeighList, cell half (MDFFMatrices.cpp:151-159,
  * FunctorGenCellList.cpp:49-169) on the current positions. Any output may be
  * NULL. ixyz [3*n]: cell of each atom; cell_first/cell_count [ncells]
@@ -272,7 +282,8 @@ int mtmd_build_neighbors(mtmd_handle* h, int32_t max_per_atom, int32_t* count,
 int mtmd_compute_forces(mtmd_handle* h, double* F, double* Fpi, float* U);
 
 /* CMDLoop::runSimulation inner loop (MDLoop.cpp:124-138): nsteps of
- * NH_P, NH_T, velocity Verlet (with force cap), NH_T, NH_P, PBC wrap.
+ * NH_P, NH_T, velocity Verlet (with force cap in NVT/NVE; the MTK barostat
+ * in NPT, VelVerlet.cpp:57-71), NH_T, NH_P, PBC wrap in the current box.
  * The first call computes the initial forces (MDLoop.cpp:121). */
 int mtmd_step(mtmd_handle* h, int32_t nsteps);
 

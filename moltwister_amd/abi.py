@@ -139,6 +139,7 @@ EXPORTED = [
     "mtmd_build_neighbors", "mtmd_compute_forces", "mtmd_step", "mtmd_get_thermo",
     "mtmd_get_forces", "mtmd_optimize", "mtmd_get_stats", "mtmd_synchronize", "mtmd_run",
     "mtmd_run_optimize", "mtmd_bench_kernel", "mtmd_append_xyz", "mtmd_append_dcd",
+    "mtmd_set_barostat", "mtmd_get_barostat",
     "mtmd_batch_create", "mtmd_batch_destroy", "mtmd_batch_last_error", "mtmd_batch_step",
     "mtmd_batch_step_separate", "mtmd_batch_synchronize", "mtmd_batch_get_thermo", "mtmd_batch_get_state",
     "mtmd_batch_get_forces", "mtmd_batch_autotune", "mtmd_batch_info",
@@ -188,6 +189,8 @@ def load_library(path: str | None = None) -> C.CDLL:
         "mtmd_run_optimize": (C.c_int, [C.POINTER(MtmdConfig), C.POINTER(MtmdSystem), vp]),
         "mtmd_bench_kernel": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(C.c_double)]),
         "mtmd_append_xyz": (C.c_int, [vp, C.c_char_p, C.c_int32]),
+        "mtmd_set_barostat": (C.c_int, [vp, C.c_double, C.c_double, _dp, _dp]),
+        "mtmd_get_barostat": (C.c_int, [vp, _dp, _dp, _dp, _dp, _dp]),
         "mtmd_append_dcd": (C.c_int, [vp, C.c_char_p, C.c_int32, C.c_int32]),
         "mtmd_batch_create": (C.c_int, [C.c_int, C.POINTER(MtmdSystem), C.POINTER(MtmdConfig),
                                         C.POINTER(C.c_void_p)]),

@@ -523,3 +523,95 @@ int mt_pdistr(const MtDev& d, double maxP, int nbins, const double* peta, double
     hipLaunchKernelGGL(k_pdistr, dim3(nb), dim3(TPB), 0, s, d.mom, d.n, maxP, nbins, peta, maxPeta, hist);
     return 0;
 }
+
+// ---------------------------------------------------------------------------
+// NPT: the particle half of CVelVerlet::propagator's MTK branch
+// (VelVerlet.cpp:57-71,124-174); the barostat scalars (p_eps, eps, box) are
+// advanced on the host between these kernels
+// ---------------------------------------------------------------------------
+// the NH_T scale applied to the momenta, then block partials of sum p.p/m
+// (red[b]) and sum Fpi.r (red[gridDim.x + b]) for G_eps (VelVerlet.cpp:160-174)
+__global__ void __launch_bounds__(TPB) k_npt_scale_sums(double4* __restrict__ mom, const double4* __restrict__ pos,
+                                                        const double4* __restrict__ frc, int n, const double* nh,
+                                                        int scale_slot, double* red)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    double a = 0.0, b = 0.0;
+    if (k < n) {
+        double4 pm = mom[k];
+        if (scale_slot >= 0) {
+            const double S = nh[scale_slot];
+            pm.x *= S; pm.y *= S; pm.z *= S;
+            mom[k] = pm;
+        }
+        const double4 r = pos[k], F = frc[k];
+        a = ((pm.x * pm.x) + (pm.y * pm.y) + (pm.z * pm.z)) / pm.w;
+        b = (F.x * r.x) + (F.y * r.y) + (F.z * r.z);
+    }
+    block_sum_store(a, &red[blockIdx.x]);
+    __syncthreads();
+    block_sum_store(b, &red[gridDim.x + blockIdx.x]);
+}
+
+// prop_p then prop_r (VelVerlet.cpp:124-150): p = p c2 + F cf, r = r r2 + (p/m) rr
+__global__ void __launch_bounds__(TPB) k_npt_prop_pr(double4* __restrict__ pos, double4* __restrict__ mom,
+                                                     const double4* __restrict__ frc, int n, double cp2, double cpf,
+                                                     double cr2, double crr)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    double4 pm = mom[k], r = pos[k];
+    const double4 F = frc[k];
+    pm.x = pm.x * cp2 + F.x * cpf;
+    pm.y = pm.y * cp2 + F.y * cpf;
+    pm.z = pm.z * cp2 + F.z * cpf;
+    const double im = 1.0 / pm.w;
+    const double ux = pm.x * im, uy = pm.y * im, uz = pm.z * im;
+    r.x = r.x * cr2 + ux * crr;
+    r.y = r.y * cr2 + uy * crr;
+    r.z = r.z * cr2 + uz * crr;
+    mom[k] = pm;
+    pos[k] = r;
+}
+
+// prop_p after the force call, then the partials of G_eps as k_npt_scale_sums
+__global__ void __launch_bounds__(TPB) k_npt_prop_p_sums(double4* __restrict__ mom, const double4* __restrict__ pos,
+                                                         const double4* __restrict__ frc, int n, double cp2,
+                                                         double cpf, double* red)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    double a = 0.0, b = 0.0;
+    if (k < n) {
+        double4 pm = mom[k];
+        const double4 r = pos[k], F = frc[k];
+        pm.x = pm.x * cp2 + F.x * cpf;
+        pm.y = pm.y * cp2 + F.y * cpf;
+        pm.z = pm.z * cp2 + F.z * cpf;
+        mom[k] = pm;
+        a = ((pm.x * pm.x) + (pm.y * pm.y) + (pm.z * pm.z)) / pm.w;
+        b = (F.x * r.x) + (F.y * r.y) + (F.z * r.z);
+    }
+    block_sum_store(a, &red[blockIdx.x]);
+    __syncthreads();
+    block_sum_store(b, &red[gridDim.x + blockIdx.x]);
+}
+
+int mt_npt_blocks(int n) { return nblk(n, TPB); }
+
+void mt_npt_scale_sums(const MtDev& d, int scale_slot, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_npt_scale_sums, dim3(nblk(d.n, TPB)), dim3(TPB), 0, s, d.mom, d.pos, d.frc, d.n, d.nh,
+                       scale_slot, d.red);
+}
+
+void mt_npt_prop_pr(const MtDev& d, double cp2, double cpf, double cr2, double crr, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_npt_prop_pr, dim3(nblk(d.n, TPB)), dim3(TPB), 0, s, d.pos, d.mom, d.frc, d.n, cp2, cpf,
+                       cr2, crr);
+}
+
+void mt_npt_prop_p_sums(const MtDev& d, double cp2, double cpf, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_npt_prop_p_sums, dim3(nblk(d.n, TPB)), dim3(TPB), 0, s, d.mom, d.pos, d.frc, d.n, cp2, cpf,
+                       d.red);
+}

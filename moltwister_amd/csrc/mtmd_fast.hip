@@ -1468,14 +1468,12 @@ void mt_fast_forces(const MtDev& d, const MtParams& p, int mixed, int want_e, in
 // ---------------------------------------------------------------------------
 // the per-atom arithmetic of the front half, shared with the fused NVE kernel
 // (contraction off in both, so fused and separate steps agree bit for bit)
-__device__ __forceinline__ float mt_fast_front(const MtDev& d, const MtParams& p, int k, double4& pm, double4& r,
-                                               const double4& F, int mixed)
+// displacement since the last rebuild (squared, minimum image) and, MIXED,
+// the position relative to the centre of the atom's home cell
+__device__ __forceinline__ float mt_fast_locate_atom(const MtDev& d, const MtParams& p, int k, const double4& r,
+                                                     int mixed)
 {
     MT_EXACT_FP
-    const double dt_2 = p.dt / 2.0;
-    pm.x += F.x * dt_2; pm.y += F.y * dt_2; pm.z += F.z * dt_2;
-    const double c = p.dt / pm.w;
-    r.x += pm.x * c; r.y += pm.y * c; r.z += pm.z * c;
     const double4 q = d.posref[k];
     double dx = r.x - q.x, dy = r.y - q.y, dz = r.z - q.z;
     dx -= p.L[0] * rint(dx / p.L[0]);
@@ -1495,6 +1493,17 @@ __device__ __forceinline__ float mt_fast_front(const MtDev& d, const MtParams& p
         d.posf[u] = make_float4((float)lx, (float)ly, (float)lz, old.w);
     }
     return (float)(dx * dx + dy * dy + dz * dz);
+}
+
+__device__ __forceinline__ float mt_fast_front(const MtDev& d, const MtParams& p, int k, double4& pm, double4& r,
+                                               const double4& F, int mixed)
+{
+    MT_EXACT_FP
+    const double dt_2 = p.dt / 2.0;
+    pm.x += F.x * dt_2; pm.y += F.y * dt_2; pm.z += F.z * dt_2;
+    const double c = p.dt / pm.w;
+    r.x += pm.x * c; r.y += pm.y * c; r.z += pm.z * c;
+    return mt_fast_locate_atom(d, p, k, r, mixed);
 }
 
 // only "has any atom moved beyond skin/2" matters: one atomic per wavefront
@@ -1538,6 +1547,19 @@ __global__ void __launch_bounds__(TPB) k_fast_nve_fused(MtDev d, MtParams p, int
         d.pos[k] = r;
     }
     mt_fast_flag_disp(d, p, d2);
+}
+
+__global__ void __launch_bounds__(TPB) k_fast_locate(MtDev d, MtParams p, int mixed)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    float d2 = 0.0f;
+    if (k < d.n) d2 = mt_fast_locate_atom(d, p, k, d.pos[k], mixed);
+    mt_fast_flag_disp(d, p, d2);
+}
+
+void mt_fast_locate(const MtDev& d, const MtParams& p, int mixed, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fast_locate, dim3(nblk(d.n, TPB)), dim3(TPB), 0, s, d, p, mixed);
 }
 
 void mt_fast_nve_fused(const MtDev& d, const MtParams& p, int mixed, hipStream_t s)

@@ -68,6 +68,17 @@ struct mtmd_handle {
     DevBuf mem;
     std::string err;
 
+    // NPT: the barostat of CVelVerlet (VelVerlet.h: P_, p_eps_, eps_, W_, V0_)
+    // and its chain NH_P_, advanced on the host with the reference's scalar
+    // arithmetic; Lmax is the initial box, Lrb the box of the last list build
+    bool npt = false;
+    double Lmax[3] = {0, 0, 0}, Lrb[3] = {0, 0, 0};
+    double Pext = 0, W = 0, V0 = 0, eps = 0, peps = 0;
+    int Mp = 0, nrespP = 0;
+    double tauP = 0;
+    std::vector<double> etaP, petaP, QP;
+    unsigned int rebuild_count = 0;
+
     // atom types and type names (XYZ frames)
     std::vector<int32_t> type_host;
     std::vector<std::string> type_names;
@@ -340,14 +351,16 @@ static int setup(mtmd_handle* h, const mtmd_system* s)
     if (n <= 0 || T <= 0 || !s->pos || !s->mass || !s->charge || !s->type)
         return h->fail(MTMD_EINVAL, "system needs atoms, types, positions, masses and charges");
     if (n >= (1 << MT_NW_IDX_BITS)) return h->fail(MTMD_EINVAL, "at most %d atoms", (1 << MT_NW_IDX_BITS) - 1);
-    if (c.ensemble == MTMD_NPT) return h->fail(MTMD_EINVAL, "ensemble npt is not supported by this engine");
-    if (c.ensemble != MTMD_NVT && c.ensemble != MTMD_NVE) return h->fail(MTMD_EINVAL, "unknown ensemble");
+    if (c.ensemble != MTMD_NVT && c.ensemble != MTMD_NVE && c.ensemble != MTMD_NPT)
+        return h->fail(MTMD_EINVAL, "unknown ensemble");
     if (c.cutoff_radius <= 0) return h->fail(MTMD_EINVAL, "cutoffradius must be positive");
     if (c.timestep <= 0) return h->fail(MTMD_EINVAL, "timestep must be positive");
-    if (c.ensemble == MTMD_NVT && (c.temperature_nh_len < 1 || c.temperature_respa < 1 || c.temperature <= 0 ||
+    if (c.ensemble != MTMD_NVE && (c.temperature_nh_len < 1 || c.temperature_respa < 1 || c.temperature <= 0 ||
                                    c.temperature_relax <= 0))
-        return h->fail(MTMD_EINVAL, "NVT needs temperature, temperaturerelax, temperaturenhlen and "
+        return h->fail(MTMD_EINVAL, "NVT and NPT need temperature, temperaturerelax, temperaturenhlen and "
                                     "temperaturerespa > 0");
+    if (c.ensemble == MTMD_NPT && (c.pressure_nh_len < 1 || c.pressure_respa < 1 || c.pressure_relax <= 0))
+        return h->fail(MTMD_EINVAL, "NPT needs pressurerelax, pressurenhlen and pressurerespa > 0");
     for (int k = 0; k < n; k++)
         if (s->type[k] < 0 || s->type[k] >= T || !(s->mass[k] > 0) || !std::isfinite(s->pos[3 * k]) ||
             !std::isfinite(s->pos[3 * k + 1]) || !std::isfinite(s->pos[3 * k + 2]))
@@ -742,7 +755,7 @@ static int setup(mtmd_handle* h, const mtmd_system* s)
     d.cell_list = h->mem.alloc<int>(n);
     d.ncount = h->mem.alloc<int>(n);
     d.nreq = h->mem.alloc<int>(1);
-    d.red_blocks = std::max((n + 255) / 256, 3 * 1024);
+    d.red_blocks = std::max(2 * ((n + 255) / 256), 3 * 1024);
     d.red = h->mem.alloc<double>(d.red_blocks);
     d.flags = h->mem.alloc<unsigned int>(8);
     if (!d.frc || !d.epot || !d.cell_key || !d.cell_count || !d.cell_first || !d.cell_fill || !d.cell_list ||
@@ -766,7 +779,7 @@ static int setup(mtmd_handle* h, const mtmd_system* s)
     }
 
     // --- thermostat (SimulationBox.cpp:53-79, NHChain.cpp:32-52,115-143) ---
-    h->M = (c.ensemble == MTMD_NVT) ? c.temperature_nh_len : 0;
+    h->M = (c.ensemble == MTMD_NVT || c.ensemble == MTMD_NPT) ? c.temperature_nh_len : 0;
     h->nresp = c.temperature_respa;
     h->Tred = c.temperature / MT_CONV_T;
     h->tau = c.temperature_relax / MT_CONV_t;
@@ -788,6 +801,23 @@ static int setup(mtmd_handle* h, const mtmd_system* s)
     }
     nh[3 * M] = 1.0;
     h->nh_scale_slot = 3 * M;
+    // --- barostat (SimulationBox.cpp:53-79, VelVerlet.cpp:31-43, NHChain.cpp:115-124
+    // with prepareArrays(1, 1)): p_eps and the pressure chain start at rest
+    // (the reference draws them at random); mtmd_set_barostat sets any state
+    h->npt = (c.ensemble == MTMD_NPT);
+    for (int i = 0; i < 3; i++) h->Lmax[i] = h->Lrb[i] = s->box[i];
+    if (h->npt) {
+        h->Mp = c.pressure_nh_len;
+        h->nrespP = c.pressure_respa;
+        h->tauP = c.pressure_relax / MT_CONV_t;
+        h->W = h->Tred * h->tauP * h->tauP;
+        h->Pext = c.pressure / MT_CONV_PRESS;
+        h->V0 = h->Lmax[0] * h->Lmax[1] * h->Lmax[2];
+        h->etaP.assign(h->Mp, 0.0);
+        h->petaP.assign(h->Mp, 0.0);
+        h->QP.assign(h->Mp, h->Tred * h->tauP * h->tauP);
+        h->QP[0] = (double)(1 * 1) * h->Tred * h->tauP * h->tauP;
+    }
     h->type_host.assign(s->type, s->type + n);
     h->type_names.assign(T, std::string());
     for (int t = 0; t < T; t++)
@@ -918,6 +948,148 @@ extern "C" int mtmd_set_thermostat(mtmd_handle* h, const double* eta, const doub
     HIPCHK(h, hipStreamSynchronize(h->stream));
     if (eta) HIPCHK(h, hipMemcpy(h->d.nh, eta, h->M * sizeof(double), hipMemcpyHostToDevice));
     if (p_eta) HIPCHK(h, hipMemcpy(h->d.nh + h->M, p_eta, h->M * sizeof(double), hipMemcpyHostToDevice));
+    return MTMD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// NPT (MDLoop.cpp:124-138 with VelVerlet.cpp:57-71,124-174,192-211)
+// ---------------------------------------------------------------------------
+#pragma clang fp contract(off)
+// the box every kernel sees: widths as the functors take them (float) and as
+// the integrator and the wrap take them (double)
+static void set_box(mtmd_handle* h, const double L[3])
+{
+    MtParams& p = h->p;
+    for (int i = 0; i < 3; i++) {
+        p.L[i] = L[i];
+        p.Lf[i] = (float)L[i];
+        p.lowf[i] = -p.Lf[i] / 2.0f;
+        const double Lf = (double)p.Lf[i];
+        p.wbox[i] = (Lf / 2.0) - (-Lf / 2.0);
+        p.wnl[i] = (double)(p.lowf[i] + p.Lf[i]) - (double)p.lowf[i];
+        if (h->fast) p.cs[i] = (float)(L[i] / h->d.nc[i]);
+    }
+}
+
+// CMt::exp / CMt::sinhXoverX (Integrators/Math.cpp:27-54)
+static double cmt_exp(double x) { return (x > 1.0) ? 2.71828 * x : std::exp(x); }
+static double cmt_sinhxoverx(double x)
+{
+    static const double a[6] = {1.0, 1.0 / 6.0, 1.0 / 120.0, 1.0 / 5040.0, 1.0 / 362880.0, 1.0 / 39916800.0};
+    double sum = 0.0, xn = 1.0;
+    for (int n = 0; n < 6; n++) { sum += a[n] * xn; xn *= (x * x); }
+    return sum;
+}
+
+static double npt_volume(const mtmd_handle* h) { return h->V0 * std::exp(3.0 * h->eps); }   // getV
+
+static void npt_box(const mtmd_handle* h, double L[3])                                      // getSimBox
+{
+    const double Vmax = h->Lmax[0] * h->Lmax[1] * h->Lmax[2];
+    double etaCube = npt_volume(h) / Vmax;
+    etaCube = std::pow(etaCube, 1.0 / 3.0);
+    for (int i = 0; i < 3; i++) L[i] = etaCube * h->Lmax[i];
+}
+
+// G_eps (VelVerlet.cpp:160-174) from the sums of p.p/m and Fpi.r
+static double npt_G(const mtmd_handle* h, double sum_p, double sum_f)
+{
+    return (1.0 + 1.0 / (double)h->n) * sum_p + sum_f - 3.0 * h->Pext * npt_volume(h);
+}
+
+// NH_P_.propagator with CFctP (NHChain.cpp:54-113, MDLoop.cpp:63-83): a chain
+// on the barostat momentum, G(0) = p_eps^2/W - T, scaleMomentum scales p_eps
+static void npt_nhp(mtmd_handle* h)
+{
+    const int M = h->Mp;
+    double* peta = h->petaP.data();
+    double* eta = h->etaP.data();
+    const double* Q = h->QP.data();
+    const double beta = 1.0 / h->Tred;
+    auto G = [&](int j) {
+        if (j == 0) return h->peps * h->peps / h->W - (1.0 / beta);
+        return (peta[j - 1] * peta[j - 1] / Q[j - 1]) - (1.0 / beta);
+    };
+    if (M > 0 && peta[0] > 15.0) peta[0] = 10.0;
+    if (M <= 0) return;
+    for (int mu = 0; mu < 3; mu++) {
+        const double Dt = h->w[mu] * h->p.dt / (double)h->nrespP;
+        for (int i = 0; i < h->nrespP; i++) {
+            peta[M - 1] += (Dt / 4.0) * G(M - 1);
+            for (int j = 0; j < M - 1; j++) {
+                peta[j] *= cmt_exp(-(Dt / 8.0) * (peta[j + 1] / Q[j + 1]));
+                peta[j] += (Dt / 4.0) * G(j);
+                peta[j] *= cmt_exp(-(Dt / 8.0) * (peta[j + 1] / Q[j + 1]));
+            }
+            for (int j = 0; j < M; j++) eta[j] += (Dt / 2.0) * (peta[j] / Q[j]);
+            h->peps *= cmt_exp(-((Dt / 2.0) * (peta[0] / Q[0])));
+            for (int j = M - 2; j >= 0; j--) {
+                peta[j] *= cmt_exp(-(Dt / 8.0) * (peta[j + 1] / Q[j + 1]));
+                peta[j] += (Dt / 4.0) * G(j);
+                peta[j] *= cmt_exp(-(Dt / 8.0) * (peta[j + 1] / Q[j + 1]));
+            }
+            peta[M - 1] += (Dt / 4.0) * G(M - 1);
+        }
+    }
+}
+#pragma clang fp contract(on)
+
+// FP64/MIXED under NPT: coarsen the cell grid when a cell of the current box
+// would be narrower than the list radius (the device arrays were sized for the
+// initial, finer grid)
+static int npt_regrid(mtmd_handle* h)
+{
+    MtDev& d = h->d;
+    bool changed = false;
+    for (int i = 0; i < 3; i++) {
+        const int nc = (int)std::floor(h->p.L[i] / (double)h->p.rlist);
+        if (nc < 3) return h->fail(MTMD_EBOX, "the box shrank below 3 cells of cutoff+skin per axis");
+        if (nc < d.nc[i]) { d.nc[i] = nc; changed = true; }
+    }
+    if (changed) {
+        d.ncells = d.nc[0] * d.nc[1] * d.nc[2];
+        set_box(h, h->p.L);                          // cell sizes of the new grid
+        h->drop_graphs();
+    }
+    return MTMD_OK;
+}
+
+extern "C" int mtmd_set_barostat(mtmd_handle* h, double eps, double p_eps, const double* eta, const double* p_eta)
+{
+    if (!h) return MTMD_EINVAL;
+    if (!h->npt) return h->fail(MTMD_EINVAL, "the barostat exists in the npt ensemble only");
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    h->eps = eps;
+    h->peps = p_eps;
+    for (int j = 0; j < h->Mp; j++) {
+        if (eta) h->etaP[j] = eta[j];
+        if (p_eta) h->petaP[j] = p_eta[j];
+    }
+    // the box follows eps (getSimBox); forces of the old box are stale
+    double box[3];
+    npt_box(h, box);
+    set_box(h, box);
+    if (h->fast) {
+        int rc = npt_regrid(h);
+        if (rc) return rc;
+    }
+    h->have_forces = false;
+    h->need_sort = h->fast;
+    return MTMD_OK;
+}
+
+extern "C" int mtmd_get_barostat(mtmd_handle* h, double* eps, double* p_eps, double* eta, double* p_eta,
+                                 double* box)
+{
+    if (!h) return MTMD_EINVAL;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (eps) *eps = h->eps;
+    if (p_eps) *p_eps = h->peps;
+    for (int j = 0; j < h->Mp; j++) {
+        if (eta) eta[j] = h->etaP[j];
+        if (p_eta) p_eta[j] = h->petaP[j];
+    }
+    if (box) for (int i = 0; i < 3; i++) box[i] = h->p.L[i];
     return MTMD_OK;
 }
 
@@ -1157,6 +1329,103 @@ extern "C" int mtmd_build_neighbors(mtmd_handle* h, int32_t max_per_atom, int32_
                                        : MTMD_OK;
 }
 
+
+// NPT steps, all engines: NH_P and the barostat scalars on the host, the NH_T
+// chain and the particles on the device. Two host waits per step read the
+// block partials of G_eps (and, FP64/MIXED, the list-build counter).
+// FP64/MIXED lists: a step rebuilds when an atom moved more than skin/2 less
+// the box change since the last build (which bounds the error of the
+// minimum-image displacement of atoms wrapped in between), when the box
+// changed by more than skin/4 since that build, or when a cell would drop
+// below the list radius (the grid is then coarsened).
+static int npt_steps(mtmd_handle* h, int nsteps, bool energy_last)
+{
+    MtDev& d = h->d;
+    MtParams& p = h->p;
+    const int nb = mt_npt_blocks(h->n);
+    const int mixed = h->engine == MTMD_ENGINE_MIXED;
+    const double skin = h->cfg.skin;
+    std::vector<double> part(2 * (size_t)nb);
+    int rc;
+    auto sums = [&](double& sp, double& sf) -> int {
+        HIPCHK(h, hipMemcpyAsync(part.data(), d.red, part.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        if (h->fast)
+            HIPCHK(h, hipMemcpyAsync(h->h_status, d.flags, 8 * sizeof(unsigned int), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        sp = 0.0;
+        sf = 0.0;
+        for (int b = 0; b < nb; b++) { sp += part[b]; sf += part[nb + b]; }
+        return MTMD_OK;
+    };
+    if (h->fast) {
+        // the device's list-build counter at the start of the call
+        HIPCHK(h, hipMemcpyAsync(h->h_status, d.flags, 8 * sizeof(unsigned int), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        h->rebuild_count = h->h_status[4];
+        mt_fast_clear_flags(d, h->stream);
+    }
+    for (int t = 0; t < nsteps; t++) {
+        const bool last = (t == nsteps - 1);
+        npt_nhp(h);
+        if (!h->ke_valid) mt_reduce_ke(d, h->stream);
+        mt_nh_chain(d, h->M, h->nresp, h->Tred, p.dt, h->g, h->w, nb, h->stream);
+        mt_npt_scale_sums(d, h->nh_scale_slot, h->stream);
+        double sp, sf;
+        if ((rc = sums(sp, sf))) return rc;
+        h->peps += ((p.dt / 2.0) * npt_G(h, sp, sf));
+        // prop_p (VelVerlet.cpp:124-136) and prop_r (:138-150) coefficients
+        const double u_eps = h->peps / h->W;
+        const double parm_p = (1.0 + 1.0 / (double)h->n) * u_eps * p.dt / 4.0;
+        const double cp = cmt_exp(-parm_p);
+        const double cp2 = cp * cp, cpf = (p.dt / 2.0) * cp * cmt_sinhxoverx(parm_p);
+        const double parm_r = u_eps * p.dt / 2.0;
+        const double cr = cmt_exp(parm_r);
+        const double cr2 = cr * cr, crr = p.dt * (cmt_sinhxoverx(parm_r) * cr);
+        h->eps += (p.dt * h->peps / h->W);
+        double box[3];
+        npt_box(h, box);
+        set_box(h, box);
+        mt_npt_prop_pr(d, cp2, cpf, cr2, crr, h->stream);
+        if (!h->fast) {
+            if ((rc = exact_forces(h))) return rc;
+        } else {
+            double dL = 0.0;
+            bool regrid = false;
+            for (int i = 0; i < 3; i++) {
+                dL = std::max(dL, std::fabs(box[i] - h->Lrb[i]));
+                if (box[i] / d.nc[i] < (double)p.rlist) regrid = true;
+            }
+            const bool sort = (++h->steps_since_sort >= h->sort_every);
+            const bool force = regrid || sort || dL > skin / 4.0;
+            if (regrid && (rc = npt_regrid(h))) return rc;
+            const double thr = std::max(0.0, skin / 2.0 - dL);
+            p.half_skin2 = (float)(thr * thr);
+            mt_fast_locate(d, p, mixed, h->stream);
+            if ((rc = fast_rebuild(h, force, sort))) return rc;
+            if (force) for (int i = 0; i < 3; i++) h->Lrb[i] = box[i];
+            if ((rc = fast_forces(h, last && energy_last))) return rc;
+        }
+        mt_npt_prop_p_sums(d, cp2, cpf, h->stream);
+        if ((rc = sums(sp, sf))) return rc;
+        h->peps += ((p.dt / 2.0) * npt_G(h, sp, sf));
+        if (h->fast) {
+            // a list build on the device (the displacement test) this step
+            if (h->h_status[4] != h->rebuild_count) {
+                h->rebuild_count = h->h_status[4];
+                for (int i = 0; i < 3; i++) h->Lrb[i] = box[i];
+            }
+            if (h->h_status[2]) break;                // list overflow: reported below
+        }
+        mt_nh_chain(d, h->M, h->nresp, h->Tred, p.dt, h->g, h->w, nb, h->stream);
+        npt_nhp(h);
+        mt_scale_wrap(d, p, h->nh_scale_slot, 1, h->stream);
+        h->ke_valid = true;
+        h->steps++;
+    }
+    HIPCHK(h, hipGetLastError());
+    return MTMD_OK;
+}
+
 // ---------------------------------------------------------------------------
 // MD loop (MDLoop.cpp:124-138): NH_T, velocity Verlet, NH_T, wrap
 // ---------------------------------------------------------------------------
@@ -1172,9 +1441,12 @@ static int md_steps(mtmd_handle* h, int nsteps, bool energy_last)
     if (h->fast && h->need_sort) {
         // re-sort now so the step loop runs on spatially ordered atoms
         if ((rc = fast_full_forces(h))) return rc;
+        for (int i = 0; i < 3; i++) h->Lrb[i] = h->p.L[i];
     }
     const int mixed = h->engine == MTMD_ENGINE_MIXED;
-    if (!h->fast) {
+    if (h->npt) {
+        if ((rc = npt_steps(h, nsteps, energy_last))) return rc;
+    } else if (!h->fast) {
         for (int t = 0; t < nsteps; t++) {
             if (nvt) {
                 if (!h->ke_valid) mt_reduce_ke(d, h->stream);
@@ -1299,7 +1571,8 @@ extern "C" int mtmd_get_thermo(mtmd_handle* h, mtmd_thermo* out)
     h->ke_valid = false;   // d.red now holds thermo partials
     double sp = 0, sf = 0, su = 0;
     for (int b = 0; b < nb; b++) { sp += part[3 * b]; sf += part[3 * b + 1]; su += part[3 * b + 2]; }
-    const double V = h->p.L[0] * h->p.L[1] * h->p.L[2];
+    // calcV / getV (SimulationBox.h:44, VelVerlet.cpp:192-197)
+    const double V = h->npt ? npt_volume(h) : h->p.L[0] * h->p.L[1] * h->p.L[2];
     out->sum_p2m = sp;
     out->sum_fr = sf;
     out->T = sp / (3 * (double)h->n) * MT_CONV_T;
@@ -1534,7 +1807,7 @@ struct Distributions {
         }
         if (v_on) {
             // CMDLoop::appendToVolumeDistribution (MDLoop.cpp:324-328)
-            const double V = h->p.L[0] * h->p.L[1] * h->p.L[2];
+            const double V = h->npt ? npt_volume(h) : h->p.L[0] * h->p.L[1] * h->p.L[2];
             const int i = (int)((V / maxV) * (double)kBins);
             if (i < kBins) vol[i]++;
         }

@@ -125,6 +125,12 @@ void mt_scale_wrap(const MtDev& d, const MtParams& p, int scale_slot, int want_k
 void mt_reduce_ke(const MtDev& d, hipStream_t s);
 void mt_reduce_thermo(const MtDev& d, hipStream_t s);
 void mt_gradient_step(const MtDev& d, double lr, hipStream_t s);
+// NPT particle kernels (all engines): partials of sum p.p/m in red[0, nb) and
+// sum F.r in red[nb, 2 nb), nb = mt_npt_blocks(n)
+int mt_npt_blocks(int n);
+void mt_npt_scale_sums(const MtDev& d, int scale_slot, hipStream_t s);
+void mt_npt_prop_pr(const MtDev& d, double cp2, double cpf, double cr2, double crr, hipStream_t s);
+void mt_npt_prop_p_sums(const MtDev& d, double cp2, double cpf, hipStream_t s);
 // one step of the momentum histograms of mtmd_run (hist [4*nbins] ints on the
 // device, nbins <= 256; peta NULL skips the thermostat bin); -1 on bad nbins
 int mt_pdistr(const MtDev& d, double maxP, int nbins, const double* peta, double maxPeta, int* hist, hipStream_t s);
@@ -144,6 +150,9 @@ void mt_fast_forces(const MtDev& d, const MtParams& p, int mixed, int want_energ
 int mt_fast_lanes(const MtDev& d);
 void mt_fast_kick_drift(const MtDev& d, const MtParams& p, int scale_slot, int mixed, hipStream_t s);
 void mt_fast_clear_flags(const MtDev& d, hipStream_t s);
+// NPT: displacement since the last rebuild and MIXED local coordinates of the
+// current positions (the locating half of the front-half kernel)
+void mt_fast_locate(const MtDev& d, const MtParams& p, int mixed, hipStream_t s);
 // NVE: back half of one step and front half of the next in one pass
 void mt_fast_nve_fused(const MtDev& d, const MtParams& p, int mixed, hipStream_t s);
 // in-place read + write of the whole neighbour-list buffer (HBM reference)

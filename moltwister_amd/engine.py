@@ -93,6 +93,19 @@ class Engine:
         self._chk(self.lib.mtmd_get_thermostat(self.h, _f64(eta), _f64(p_eta)))
         return eta, p_eta
 
+    def set_barostat(self, eps, p_eps, eta=None, p_eta=None):
+        eta = None if eta is None else np.ascontiguousarray(eta, np.float64)
+        p_eta = None if p_eta is None else np.ascontiguousarray(p_eta, np.float64)
+        self._chk(self.lib.mtmd_set_barostat(self.h, float(eps), float(p_eps), _f64(eta), _f64(p_eta)))
+
+    def get_barostat(self) -> dict:
+        M = self.cfg.pressure_nh_len
+        eps, peps = C.c_double(0), C.c_double(0)
+        eta, p_eta, box = np.zeros(M), np.zeros(M), np.zeros(3)
+        self._chk(self.lib.mtmd_get_barostat(self.h, C.byref(eps), C.byref(peps), _f64(eta), _f64(p_eta),
+                                             _f64(box)))
+        return dict(eps=eps.value, p_eps=peps.value, eta=eta, p_eta=p_eta, box=box)
+
     def build_cells(self):
         nc = np.zeros(3, np.int32)
         self._chk(self.lib.mtmd_build_cells(self.h, _i32(nc), None, None, None, None))

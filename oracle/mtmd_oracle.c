@@ -443,6 +443,12 @@ struct orc_md {
     double T, tau;
     double *eta, *peta, *Q;
     double w[3];
+    /* NPT: barostat (VelVerlet.h: P_, p_eps_, eps_, W_, V0_) and its chain NH_P_ */
+    double Lmax[3];
+    double Pext, W, V0, eps, peps;
+    int Mp, nrespP;
+    double tauP;
+    double *etaP, *petaP, *QP;
 };
 
 int orc_natoms(const orc_md* o) { return o->n; }
@@ -693,6 +699,26 @@ orc_md* orc_create(const mtmd_system* s, const mtmd_config* cfg)
         for (j = 1; j < o->M; j++) o->Q[j] = o->T * o->tau * o->tau;
     }
 
+    /* NPT barostat (SB:53-79, VV:31-43, NHC:115-124 with prepareArrays(1, 1)):
+     * W = T tau_P^2, P in reduced units, V0 = Lx Ly Lz, eps = 0; p_eps and
+     * the pressure chain start at rest (orc_set_barostat sets any state) */
+    o->Lmax[0] = s->box[0]; o->Lmax[1] = s->box[1]; o->Lmax[2] = s->box[2];
+    o->Mp = (o->ensemble == MTMD_NPT) ? cfg->pressure_nh_len : 0;
+    o->nrespP = cfg->pressure_respa;
+    o->tauP = cfg->pressure_relax / CONV_t;
+    o->W = o->T * o->tauP * o->tauP;
+    o->Pext = cfg->pressure / CONV_PRESS;
+    o->V0 = o->Lmax[0] * o->Lmax[1] * o->Lmax[2];
+    o->eps = 0.0;
+    o->peps = 0.0;
+    o->etaP = (double*)calloc(o->Mp + 1, sizeof(double));
+    o->petaP = (double*)calloc(o->Mp + 1, sizeof(double));
+    o->QP = (double*)calloc(o->Mp + 1, sizeof(double));
+    if (o->Mp > 0) {
+        o->QP[0] = (double)(1 * 1) * o->T * o->tauP * o->tauP;
+        for (j = 1; j < o->Mp; j++) o->QP[j] = o->T * o->tauP * o->tauP;
+    }
+
     /* momenta: given, or uniform in [-1,1] per component, immobile zeroed,
      * rescaled to T (SB:98-133) */
     if (s->mom) {
@@ -725,6 +751,7 @@ void orc_destroy(orc_md* o)
     free(o->dfirst); free(o->dnum); free(o->di); free(o->dj); free(o->dl); free(o->dtype); free(o->dcenter);
     free(o->cix); free(o->cfirst); free(o->ccount); free(o->clist); free(o->ncount); free(o->nlist);
     free(o->eta); free(o->peta); free(o->Q);
+    free(o->etaP); free(o->petaP); free(o->QP);
     free(o);
 }
 
@@ -757,6 +784,23 @@ void orc_get_thermostat(const orc_md* o, double* eta, double* p_eta)
 {
     int j;
     for (j = 0; j < o->M; j++) { if (eta) eta[j] = o->eta[j]; if (p_eta) p_eta[j] = o->peta[j]; }
+}
+
+void orc_set_barostat(orc_md* o, double eps, double p_eps, const double* eta, const double* p_eta)
+{
+    int j;
+    o->eps = eps;
+    o->peps = p_eps;
+    for (j = 0; j < o->Mp; j++) { if (eta) o->etaP[j] = eta[j]; if (p_eta) o->petaP[j] = p_eta[j]; }
+}
+
+void orc_get_barostat(const orc_md* o, double* eps, double* p_eps, double* eta, double* p_eta, double* box)
+{
+    int j;
+    if (eps) *eps = o->eps;
+    if (p_eps) *p_eps = o->peps;
+    for (j = 0; j < o->Mp; j++) { if (eta) eta[j] = o->etaP[j]; if (p_eta) p_eta[j] = o->petaP[j]; }
+    if (box) { box[0] = o->L[0]; box[1] = o->L[1]; box[2] = o->L[2]; }
 }
 
 void orc_get_mol(const orc_md* o, int32_t* mol) { memcpy(mol, o->mol, sizeof(int) * o->n); }
@@ -1119,16 +1163,14 @@ void orc_get_forces(const orc_md* o, double* F, float* U)
 /* ------------------------------------------------------------------ */
 /* Nose-Hoover chain (NHC:54-113) with CFctT (MDL:29-60)                */
 /* ------------------------------------------------------------------ */
-static double fctT_G(const orc_md* o, int j, double beta)
+/* G of CFctT (MDL:29-52): j = 0 from the particles' kinetic energy */
+static double fctT_G0(orc_md* o, double beta)
 {
     const double g = (double)(3 * o->n);
-    if (j == 0) {
-        double s = 0.0;
-        int k;
-        for (k = 0; k < o->n; k++) s += vdot(o->p[k], o->p[k]) / o->m[k];
-        return s - (g / beta);
-    }
-    return (o->peta[j - 1] * o->peta[j - 1] / o->Q[j - 1]) - (1.0 / beta);
+    double s = 0.0;
+    int k;
+    for (k = 0; k < o->n; k++) s += vdot(o->p[k], o->p[k]) / o->m[k];
+    return s - (g / beta);
 }
 
 static void fctT_scale(orc_md* o, double coeff)
@@ -1137,36 +1179,54 @@ static void fctT_scale(orc_md* o, double coeff)
     for (k = 0; k < o->n; k++) o->p[k] = vmul(o->p[k], coeff);
 }
 
-void orc_nh_propagate(orc_md* o)
+/* G and scaleMomentum of CFctP (MDL:63-83): the barostat momentum, g = 1 */
+static double fctP_G0(orc_md* o, double beta)
 {
-    double beta = 1.0 / o->T, coeff;
-    int M = o->M, mu, i, j;
-    if (M > 0 && o->peta[0] > 15.0) o->peta[0] = 10.0;
+    return o->peps * o->peps / o->W - (1.0 / beta);
+}
+
+static void fctP_scale(orc_md* o, double coeff) { o->peps *= coeff; }
+
+/* CNHChain::propagator (NHC:54-113) for a chain of length M acting through
+ * G0 / scale; G(j > 0) = p_eta[j-1]^2 / Q[j-1] - 1/beta for both functors */
+static void chain_propagate(orc_md* o, int M, int nresp, double T, double* eta, double* peta, const double* Q,
+                            double (*G0)(orc_md*, double), void (*scale)(orc_md*, double))
+{
+    double beta = 1.0 / T, coeff;
+    int mu, i, j;
+#define CHAIN_G(jj) ((jj) == 0 ? G0(o, beta) : (peta[(jj) - 1] * peta[(jj) - 1] / Q[(jj) - 1]) - (1.0 / beta))
+    if (M > 0 && peta[0] > 15.0) peta[0] = 10.0;
     if (M <= 0) return;
     for (mu = 0; mu < 3; mu++) {
-        double Dt = o->w[mu] * o->dt / (double)o->nresp;
-        for (i = 0; i < o->nresp; i++) {
-            o->peta[M - 1] += (Dt / 4.0) * fctT_G(o, M - 1, beta);
+        double Dt = o->w[mu] * o->dt / (double)nresp;
+        for (i = 0; i < nresp; i++) {
+            peta[M - 1] += (Dt / 4.0) * CHAIN_G(M - 1);
             for (j = 0; j < M - 1; j++) {
-                coeff = orc_cmt_exp(-(Dt / 8.0) * (o->peta[j + 1] / o->Q[j + 1]));
-                o->peta[j] *= coeff;
-                o->peta[j] += (Dt / 4.0) * fctT_G(o, j, beta);
-                coeff = orc_cmt_exp(-(Dt / 8.0) * (o->peta[j + 1] / o->Q[j + 1]));
-                o->peta[j] *= coeff;
+                coeff = orc_cmt_exp(-(Dt / 8.0) * (peta[j + 1] / Q[j + 1]));
+                peta[j] *= coeff;
+                peta[j] += (Dt / 4.0) * CHAIN_G(j);
+                coeff = orc_cmt_exp(-(Dt / 8.0) * (peta[j + 1] / Q[j + 1]));
+                peta[j] *= coeff;
             }
-            for (j = 0; j < M; j++) o->eta[j] += (Dt / 2.0) * (o->peta[j] / o->Q[j]);
-            coeff = orc_cmt_exp(-((Dt / 2.0) * (o->peta[0] / o->Q[0])));
-            fctT_scale(o, coeff);
+            for (j = 0; j < M; j++) eta[j] += (Dt / 2.0) * (peta[j] / Q[j]);
+            coeff = orc_cmt_exp(-((Dt / 2.0) * (peta[0] / Q[0])));
+            scale(o, coeff);
             for (j = M - 2; j >= 0; j--) {
-                coeff = orc_cmt_exp(-(Dt / 8.0) * (o->peta[j + 1] / o->Q[j + 1]));
-                o->peta[j] *= coeff;
-                o->peta[j] += (Dt / 4.0) * fctT_G(o, j, beta);
-                coeff = orc_cmt_exp(-(Dt / 8.0) * (o->peta[j + 1] / o->Q[j + 1]));
-                o->peta[j] *= coeff;
+                coeff = orc_cmt_exp(-(Dt / 8.0) * (peta[j + 1] / Q[j + 1]));
+                peta[j] *= coeff;
+                peta[j] += (Dt / 4.0) * CHAIN_G(j);
+                coeff = orc_cmt_exp(-(Dt / 8.0) * (peta[j + 1] / Q[j + 1]));
+                peta[j] *= coeff;
             }
-            o->peta[M - 1] += (Dt / 4.0) * fctT_G(o, M - 1, beta);
+            peta[M - 1] += (Dt / 4.0) * CHAIN_G(M - 1);
         }
     }
+#undef CHAIN_G
+}
+
+void orc_nh_propagate(orc_md* o)
+{
+    chain_propagate(o, o->M, o->nresp, o->T, o->eta, o->peta, o->Q, fctT_G0, fctT_scale);
 }
 
 /* ------------------------------------------------------------------ */
@@ -1216,15 +1276,89 @@ static void vv_step(orc_md* o)
     for (k = 0; k < o->n; k++) o->p[k] = vadd(o->p[k], vmul(o->F[k], dt_2));
 }
 
+/* NPT (VV:57-71,124-150,160-174,192-211): the barostat variables and the
+ * box follow CVelVerlet operation for operation */
+static double npt_volume(const orc_md* o) { return o->V0 * exp(3.0 * o->eps); }   /* getV, VV:192-197 */
+
+static void npt_box(orc_md* o)                                                      /* getSimBox, VV:199-211 */
+{
+    double Vmax = o->Lmax[0] * o->Lmax[1] * o->Lmax[2];
+    double etaCube = npt_volume(o) / Vmax;
+    etaCube = pow(etaCube, 1.0 / 3.0);
+    o->L[0] = etaCube * o->Lmax[0];
+    o->L[1] = etaCube * o->Lmax[1];
+    o->L[2] = etaCube * o->Lmax[2];
+}
+
+static double npt_G_eps(const orc_md* o)                                            /* G_eps, VV:160-174 */
+{
+    double V = npt_volume(o), sum_p = 0.0, sum_f = 0.0;
+    int k;
+    for (k = 0; k < o->n; k++) {
+        double p2 = vdot(o->p[k], o->p[k]);
+        sum_p += (p2 / o->m[k]);
+        sum_f += vdot(o->F[k], o->r[k]);
+    }
+    return (1.0 + 1.0 / (double)o->n) * sum_p + sum_f - 3.0 * o->Pext * V;
+}
+
+static void npt_prop_p(orc_md* o)                                                   /* prop_p, VV:124-136 */
+{
+    double u_eps = o->peps / o->W;
+    double alpha = (1.0 + 1.0 / (double)o->n);
+    double parm = alpha * u_eps * o->dt / 4.0;
+    double coeff = orc_cmt_exp(-parm);
+    double coeff2 = coeff * coeff;
+    double coeff3 = orc_cmt_sinhxoverx(parm);
+    double cf = (o->dt / 2.0) * coeff * coeff3;
+    int k;
+    for (k = 0; k < o->n; k++) o->p[k] = vadd(vmul(o->p[k], coeff2), vmul(o->F[k], cf));
+}
+
+static void npt_prop_r(orc_md* o)                                                   /* prop_r, VV:138-150 */
+{
+    double u_eps = o->peps / o->W;
+    double parm = u_eps * o->dt / 2.0;
+    double coeff = orc_cmt_exp(parm);
+    double coeff2 = coeff * coeff;
+    double coeff3 = orc_cmt_sinhxoverx(parm) * coeff;
+    double cr = o->dt * coeff3;
+    int k;
+    for (k = 0; k < o->n; k++) {
+        v3 u = vmul(o->p[k], 1.0 / o->m[k]);
+        o->r[k] = vadd(vmul(o->r[k], coeff2), vmul(u, cr));
+    }
+}
+
+static void npt_step(orc_md* o)                                                     /* VV:57-71 */
+{
+    o->peps += ((o->dt / 2.0) * npt_G_eps(o));
+    npt_prop_p(o);
+    npt_prop_r(o);
+    o->eps += (o->dt * o->peps / o->W);
+    npt_box(o);
+    compute_forces_internal(o);
+    npt_prop_p(o);
+    o->peps += ((o->dt / 2.0) * npt_G_eps(o));
+}
+
+static void orc_nhp_propagate(orc_md* o)
+{
+    chain_propagate(o, o->Mp, o->nrespP, o->T, o->etaP, o->petaP, o->QP, fctP_G0, fctP_scale);
+}
+
 void orc_step(orc_md* o, int nsteps)
 {
     int t;
+    const int npt = (o->ensemble == MTMD_NPT), nh_t = (o->ensemble == MTMD_NVT || npt);
     if (!o->have_forces) compute_forces_internal(o);   /* calcInitialForces, MDL:189-192 */
     for (t = 0; t < nsteps; t++) {
-        if (o->ensemble == MTMD_NVT) orc_nh_propagate(o);
-        vv_step(o);
-        if (o->ensemble == MTMD_NVT) orc_nh_propagate(o);
-        pbc_wrap(o);
+        if (npt) orc_nhp_propagate(o);
+        if (nh_t) orc_nh_propagate(o);
+        if (npt) npt_step(o); else vv_step(o);
+        if (nh_t) orc_nh_propagate(o);
+        if (npt) orc_nhp_propagate(o);
+        pbc_wrap(o);                                   /* with calcSimBox: o->L is the current box */
     }
 }
 
@@ -1236,7 +1370,7 @@ void orc_thermo(orc_md* o, double* out)
     if (!o->have_forces) compute_forces_internal(o);
     for (k = 0; k < o->n; k++) sp += vdot(o->p[k], o->p[k]) / o->m[k];
     Tred = sp / (3 * (double)o->n);
-    V = o->L[0] * o->L[1] * o->L[2];
+    V = (o->ensemble == MTMD_NPT) ? npt_volume(o) : o->L[0] * o->L[1] * o->L[2];
     for (k = 0; k < o->n; k++) {
         double p2 = vdot(o->p[k], o->p[k]) / o->m[k];
         double f = vdot(o->F[k], o->r[k]);

@@ -49,6 +49,10 @@ void orc_set_state(orc_md* o, const double* pos, const double* mom);
 void orc_get_state(const orc_md* o, double* pos, double* mom);
 void orc_set_thermostat(orc_md* o, const double* eta, const double* p_eta);
 void orc_get_thermostat(const orc_md* o, double* eta, double* p_eta);
+/* NPT barostat: eps, p_eps and the pressure chain (arrays of pressure_nh_len);
+ * box receives the current simulation box (getSimBox) */
+void orc_set_barostat(orc_md* o, double eps, double p_eps, const double* eta, const double* p_eta);
+void orc_get_barostat(const orc_md* o, double* eps, double* p_eps, double* eta, double* p_eta, double* box);
 /* raw per-atom data, for tests */
 void orc_get_mol(const orc_md* o, int32_t* mol);
 

@@ -72,6 +72,8 @@ def _oracle_lib():
         L.orc_set_thermostat.argtypes = [_vp, _dp, _dp]
         L.orc_get_thermostat.argtypes = [_vp, _dp, _dp]
         L.orc_get_mol.argtypes = [_vp, _ip]
+        L.orc_set_barostat.argtypes = [_vp, C.c_double, C.c_double, _dp, _dp]
+        L.orc_get_barostat.argtypes = [_vp, _dp, _dp, _dp, _dp, _dp]
         L.orc_build_cells.argtypes = [_vp, _ip, _ip, _ip, _ip, _ip]
         L.orc_build_neighbors.argtypes = [_vp, C.c_int32, _ip, _ip]
         L.orc_compute_forces.argtypes = [_vp, _dp, _fp]
@@ -180,6 +182,19 @@ class Oracle(_Base):
         m = np.zeros(self.n, np.int32)
         self.lib.orc_get_mol(self.h, _i32(m))
         return m
+
+    def set_barostat(self, eps, p_eps, eta=None, p_eta=None):
+        eta = None if eta is None else np.ascontiguousarray(eta, np.float64)
+        p_eta = None if p_eta is None else np.ascontiguousarray(p_eta, np.float64)
+        self.lib.orc_set_barostat(self.h, float(eps), float(p_eps), _f64(eta) if eta is not None else None,
+                                  _f64(p_eta) if p_eta is not None else None)
+
+    def get_barostat(self):
+        M = self.cfg.pressure_nh_len
+        eps, peps = C.c_double(0), C.c_double(0)
+        eta, p_eta, box = np.zeros(M), np.zeros(M), np.zeros(3)
+        self.lib.orc_get_barostat(self.h, C.byref(eps), C.byref(peps), _f64(eta), _f64(p_eta), _f64(box))
+        return dict(eps=eps.value, p_eps=peps.value, eta=eta, p_eta=p_eta, box=box)
 
     def build_cells(self):
         nc = np.zeros(3, np.int32)

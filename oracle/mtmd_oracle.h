@@ -21,6 +21,9 @@
  *    harmonic orbits, sums against numpy) because the reference's
  *    CVelVerlet/CSimulationBox need CMolTwisterState + C3DView, whose
  *    translation unit needs GLUT, absent here;
+ *  - NVT and NPT steps (thermostat and barostat chains, MTK barostat): the
+ *    trajectory converges with order 2 in the time step to the MTK equations
+ *    of motion integrated independently (tests/test_oracle_npt.py);
  *  - per-atom bonded lists and molecule indices: restated twice (here and in
  *    the reference driver, MDFFMatrices.cpp:384-520), pinned through the
  *    force comparison with the reference functor.

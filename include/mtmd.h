@@ -73,7 +73,7 @@ enum { MTMD_ENGINE_EXACT = 0, MTMD_ENGINE_FP64 = 1, MTMD_ENGINE_MIXED = 2 };
 enum { MTMD_COUL_AUTO = 0, MTMD_COUL_ALLPAIRS = 1, MTMD_COUL_CUTOFF = 2 };
 
 /* force-field kinds */
-enum { MTMD_NB_LJ = 1, MTMD_NB_BUCK = 2, MTMD_NB_LJ1208 = 3 };
+enum { MTMD_NB_LJ = 1, MTMD_NB_BUCK = 2, MTMD_NB_LJ1208 = 3, MTMD_NB_LJBUCK = 4 };
 /* Any force field given as its tabulated profile instead of parameters: the
  * MTMD_NUM_POINTS points (x, -dU/dx, U) that the reference's
  * calc1DForceProfile / calc1DPotentialProfile give for it, as they are fed to
@@ -85,8 +85,8 @@ enum { MTMD_NB_LJ = 1, MTMD_NB_BUCK = 2, MTMD_NB_LJ1208 = 3 };
  * 0 .. 2 pi (the FP64/MIXED engines check the non-bonded range). */
 #define MTMD_FF_PROFILE 100
 enum { MTMD_BOND_HARM = 1, MTMD_BOND_MORSE = 2 };
-enum { MTMD_ANGLE_HARM = 1 };
-enum { MTMD_DIH_HARM = 1 };
+enum { MTMD_ANGLE_HARM = 1, MTMD_ANGLE_CLASS2 = 2 };
+enum { MTMD_DIH_HARM = 1, MTMD_DIH_FOURIER4T = 2 };
 
 /* Run configuration: the fields of SMolDynConfigStruct
  * (Cmd/MolTwisterCmdMolDyn/Config/MolDynConfigStruct.h:24-65) that the engine
@@ -145,7 +145,12 @@ typedef struct mtmd_config {
  *          (MDFF/Non-Bonded/MolTwisterMDFFNonBonded_LJ.cpp:131-162)
  *  BUCK:   p[0]=A, p[1]=rho, p[2]=C (MolTwisterMDFFNonBonded_Buck.cpp:116-143)
  *  LJ1208: p[0]=epsilon, p[1]=sigma, p[2]=alpha (default 1)
- *          (MolTwisterMDFFNonBonded_LJ1208.cpp:131-170) */
+ *          (MolTwisterMDFFNonBonded_LJ1208.cpp:131-170)
+ *  LJBUCK: p[0]=0 (LJ: p[1]=epsilon, p[2]=sigma) or 1 (Buckingham: p[1]=A,
+ *          p[2]=rho, p[3]=C) (MolTwisterMDFFNonBonded_LJBuck.cpp:181-251).
+ *          The reference's calcForces zeroes both forces at its end
+ *          (:250), so the force profile of this kind is 0 and only the
+ *          energy is tabulated; the engines reproduce that. */
 typedef struct mtmd_nonbonded_ff {
     int32_t type_a, type_b, kind, pad;
     double  p[4];
@@ -153,13 +158,20 @@ typedef struct mtmd_nonbonded_ff {
 } mtmd_nonbonded_ff;
 
 /* One bonded force-field type. Parameters:
- *  BOND_HARM:  p[0]=k, p[1]=r0          U = k (r - r0)^2
- *  BOND_MORSE: p[0]=D, p[1]=alpha, p[2]=r0
- *  ANGLE_HARM: p[0]=k, p[1]=theta0[deg] U = k (theta - theta0)^2
- *  DIH_HARM:   p[0]=K, p[1]=D, p[2]=N   U = K (1 + D cos(N phi)) */
+ *  BOND_HARM:    p[0]=k, p[1]=r0          U = k (r - r0)^2
+ *  BOND_MORSE:   p[0]=D, p[1]=alpha, p[2]=r0
+ *  ANGLE_HARM:   p[0]=k, p[1]=theta0[deg] U = k (theta - theta0)^2
+ *  ANGLE_CLASS2: p[0]=theta0[deg], p[1..3]=K2, K3, K4 (Ea); p[4..6]=M, r1, r2
+ *                (Ebb); p[7..10]=N1, N2, r1, r2 (Eba)
+ *                (MDFF/Angles/MolTwisterMDFFAngle_Class2.cpp:145-296)
+ *  DIH_HARM:     p[0]=K, p[1]=D, p[2]=N   U = K (1 + D cos(N phi))
+ *  DIH_FOURIER4T: p[0..3]=V1..V4          U = (V1 (1 + cos phi) + V2 (1 - cos 2phi)
+ *                + V3 (1 + cos 3phi) + V4 (1 - cos 4phi)) / 2
+ *                (MDFF/Dihedrals/MolTwisterMDFFDih_Fourier4t.cpp:98-142) */
+#define MTMD_BONDED_PARAMS 12
 typedef struct mtmd_bonded_ff {
     int32_t kind, pad;
-    double  p[4];
+    double  p[MTMD_BONDED_PARAMS];
     const float* profile;      /* kind == MTMD_FF_PROFILE only, else NULL   */
 } mtmd_bonded_ff;
 
@@ -233,7 +245,8 @@ int mtmd_config_set(mtmd_config* cfg, const char* key, const char* value,
 /* Profiles: CMDFF*::calc1DForceProfile / calc1DPotentialProfile
  * (MolTwisterMDFFNonBonded.cpp:126-172, MolTwisterMDFFBond.cpp:133-179,
  * MolTwisterMDFFAngle.cpp:210-257, MolTwisterMDFFDih.cpp:272-320).
- * which: 0 non-bonded, 1 bond, 2 angle, 3 dihedral. Writes n points of
+ * which: 0 non-bonded, 1 bond, 2 angle, 3 dihedral; params holds 4 values
+ * (non-bonded) or MTMD_BONDED_PARAMS values (bonded). Writes n points of
  * x, -dU/dx and U as float. This is also what `moldyn ff *profile` prints. */
 int mtmd_profile(int which, int kind, const double* params, float x0, float x1,
                  int n, float* x, float* f, float* e);

@@ -26,11 +26,12 @@ NVT, NPT, NVE = 0, 1, 2
 ENGINE_EXACT, ENGINE_FP64, ENGINE_MIXED = 0, 1, 2
 COUL_AUTO, COUL_ALLPAIRS, COUL_CUTOFF = 0, 1, 2
 
-NB_LJ, NB_BUCK, NB_LJ1208 = 1, 2, 3
+NB_LJ, NB_BUCK, NB_LJ1208, NB_LJBUCK = 1, 2, 3, 4
 FF_PROFILE = 100          # force field given as its (x, -dU/dx, U) profile
 BOND_HARM, BOND_MORSE = 1, 2
-ANGLE_HARM = 1
-DIH_HARM = 1
+ANGLE_HARM, ANGLE_CLASS2 = 1, 2
+DIH_HARM, DIH_FOURIER4T = 1, 2
+BONDED_PARAMS = 12
 
 
 class MtmdConfig(C.Structure):
@@ -86,7 +87,7 @@ class MtmdNonbondedFF(C.Structure):
 
 
 class MtmdBondedFF(C.Structure):
-    _fields_ = [("kind", C.c_int32), ("pad", C.c_int32), ("p", C.c_double * 4),
+    _fields_ = [("kind", C.c_int32), ("pad", C.c_int32), ("p", C.c_double * BONDED_PARAMS),
                 ("profile", C.POINTER(C.c_float))]
 
 

@@ -407,7 +407,7 @@ static int setup(mtmd_handle* h, const mtmd_system* s)
                     return h->fail(MTMD_EINVAL, "more than %d non-bonded force fields for types %d-%d", MT_MAXFF,
                                    row, col);
                 if (e.kind != MTMD_NB_LJ && e.kind != MTMD_NB_BUCK && e.kind != MTMD_NB_LJ1208 &&
-                    e.kind != MTMD_FF_PROFILE)
+                    e.kind != MTMD_NB_LJBUCK && e.kind != MTMD_FF_PROFILE)
                     return h->fail(MTMD_EINVAL, "non-bonded entry %d: unknown kind %d", i, e.kind);
                 if (e.kind == MTMD_FF_PROFILE && !e.profile)
                     return h->fail(MTMD_EINVAL, "non-bonded entry %d: profile kind without a profile", i);

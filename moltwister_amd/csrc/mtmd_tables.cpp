@@ -47,6 +47,9 @@ struct NonBonded {
             const double s2 = p[1] * p[1], s4 = s2 * s2, s6 = s4 * s2, s8 = s4 * s4, s12 = s6 * s6;
             return 4.0 * p[0] * RInv2 * (12.0 * s12 * RInv12 - 8.0 * p[2] * s8 * RInv8);
         }
+        case MTMD_NB_LJBUCK:
+            // CMDFFNonBonded_LJBuck::calcForces ends by zeroing both forces (_LJBuck.cpp:250)
+            return 0.0;
         }
         return 0.0;
     }
@@ -68,6 +71,20 @@ struct NonBonded {
             const double RInv4 = RInv2 * RInv2, RInv6 = RInv4 * RInv2, RInv8 = RInv4 * RInv4, RInv12 = RInv6 * RInv6;
             const double s2 = p[1] * p[1], s4 = s2 * s2, s6 = s4 * s2, s8 = s4 * s4, s12 = s6 * s6;
             return 4.0 * p[0] * (s12 * RInv12 - p[2] * s8 * RInv8);
+        }
+        case MTMD_NB_LJBUCK: {
+            // CMDFFNonBonded_LJBuck::calcPotential (_LJBuck.cpp:181-211): LJ (alpha 1) or Buckingham
+            if (p[0] == 0.0) {
+                const double RInv6 = RInv2 * RInv2 * RInv2, RInv12 = RInv6 * RInv6;
+                const double s2 = p[2] * p[2], s6 = s2 * s2 * s2, s12 = s6 * s6;
+                return 4.0 * p[1] * (s12 * RInv12 - s6 * RInv6);
+            }
+            if (p[0] == 1.0) {
+                const double RInv4 = RInv2 * RInv2, RInv6 = RInv4 * RInv2;
+                const double rhoInv = (p[2] == 0.0) ? 1E-10 : 1.0 / p[2];
+                return p[1] * std::exp(-R * rhoInv) - p[3] * RInv6;
+            }
+            return 0.0;
         }
         }
         return 0.0;
@@ -133,7 +150,61 @@ V3 dihedral_dphi_dr4(const V3& r1, const V3& r2, const V3& r3, const V3& r4)
     return {g * dx, g * dy, g * dz};
 }
 
-constexpr double kToRad = 0.017453292519943295;   // MolTwisterMDFFAngle_Harm.h:36
+constexpr double kToRad = 0.017453292519943295;   // MolTwisterMDFFAngle_Harm.h:36, _Class2.h:76
+
+// theta - theta0 at (r1, r2, r3) as the angle classes compute it
+double angle_delta(const V3& r1, const V3& r2, const V3& r3, double theta0_deg)
+{
+    const V3 r21 = r1 - r2, r23 = r3 - r2;
+    const double cosT = r21.dot(r23) * (inv_or_big(r21.norm()) * inv_or_big(r23.norm()));
+    return std::acos(cosT) - theta0_deg * kToRad;
+}
+
+// CMDFFAngle_Class2 (MolTwisterMDFFAngle_Class2.cpp:145-296): the force on r3 and
+// the energy of Ea (degrees 2, 3, 4), Ebb and Eba, summed in the class's order.
+// q: theta0, K2, K3, K4, M, r1, r2 (Ebb), N1, N2, r1, r2 (Eba)
+V3 class2_force3(const V3& r1, const V3& r2, const V3& r3, const double* q)
+{
+    V3 f3{0.0, 0.0, 0.0};
+    auto add = [&](const V3& a) { f3 = {f3.x + a.x, f3.y + a.y, f3.z + a.z}; };
+    const V3 c13 = angle_dtheta_dr3(r1, r2, r3);
+    for (int degree = 2; degree <= 4; degree++) {                  // calcForcesHarm
+        const double dT = angle_delta(r1, r2, r3, q[0]);
+        double dU = -double(degree) * q[degree - 1];
+        for (int i = 0; i < degree - 1; i++) dU *= dT;
+        add(c13 * dU);
+    }
+    const V3 r12 = r2 - r1, r23 = r3 - r2;
+    const double R12 = r12.norm(), R23 = r23.norm();
+    const double RInv12 = (R12 == 0.0) ? 1.0 / 1E-10 : 1.0 / R12, RInv23 = (R23 == 0.0) ? 1.0 / 1E-10 : 1.0 / R23;
+    const V3 u23 = r23 * RInv23;
+    add(u23 * ((-1.0) * q[4] * (R12 - q[5])));                      // calcForcesCartesian
+    {                                                               // calcForcesMix
+        const V3 r21 = r12 * (-1.0);
+        const double coeff = q[7] * (R12 - q[9]) + q[8] * (R23 - q[10]);
+        const double cosT = r21.dot(r23) * (RInv12 * RInv23);
+        const double dT = std::acos(cosT) - q[0] * kToRad;
+        const V3 a = u23 * ((-1.0) * q[8] * dT), b = c13 * coeff;
+        add({a.x - b.x, a.y - b.y, a.z - b.z});
+    }
+    return f3;
+}
+
+double class2_energy(const V3& r1, const V3& r2, const V3& r3, const double* q)
+{
+    double U = 0.0;
+    for (int degree = 2; degree <= 4; degree++) {                  // calcPotentialHarm
+        const double dT = angle_delta(r1, r2, r3, q[0]);
+        double dN = 1.0;
+        for (int i = 0; i < degree; i++) dN *= dT;
+        U += q[degree - 1] * dN;
+    }
+    const double R12 = (r2 - r1).norm(), R23 = (r3 - r2).norm();
+    U += q[4] * (R12 - q[5]) * (R23 - q[6]);                        // calcPotentialCartesian
+    const double R21 = (r1 - r2).norm();
+    U += (q[7] * (R21 - q[9]) + q[8] * (R23 - q[10])) * angle_delta(r1, r2, r3, q[0]);   // calcPotentialMix
+    return U;
+}
 
 } // namespace
 
@@ -141,12 +212,16 @@ extern "C" int mtmd_profile(int which, int kind, const double* params, float x0,
                             float* fo, float* eo)
 {
     if (n <= 0 || !params) return MTMD_EINVAL;
-    std::array<double, 4> p{params[0], params[1], params[2], params[3]};
-    if (which == 0 && kind != MTMD_NB_LJ && kind != MTMD_NB_BUCK && kind != MTMD_NB_LJ1208) return MTMD_EINVAL;
+    if (which == 0 && kind != MTMD_NB_LJ && kind != MTMD_NB_BUCK && kind != MTMD_NB_LJ1208 && kind != MTMD_NB_LJBUCK)
+        return MTMD_EINVAL;
     if (which == 1 && kind != MTMD_BOND_HARM && kind != MTMD_BOND_MORSE) return MTMD_EINVAL;
-    if (which == 2 && kind != MTMD_ANGLE_HARM) return MTMD_EINVAL;
-    if (which == 3 && kind != MTMD_DIH_HARM) return MTMD_EINVAL;
+    if (which == 2 && kind != MTMD_ANGLE_HARM && kind != MTMD_ANGLE_CLASS2) return MTMD_EINVAL;
+    if (which == 3 && kind != MTMD_DIH_HARM && kind != MTMD_DIH_FOURIER4T) return MTMD_EINVAL;
     if (which < 0 || which > 3) return MTMD_EINVAL;
+    // non-bonded kinds take 4 parameters, bonded kinds MTMD_BONDED_PARAMS
+    double q[MTMD_BONDED_PARAMS] = {0.0};
+    for (int i = 0; i < (which == 0 ? 4 : MTMD_BONDED_PARAMS); i++) q[i] = params[i];
+    std::array<double, 4> p{q[0], q[1], q[2], q[3]};
     const float step = (x1 - x0) / float(n - 1);
     for (int i = 0; i < n; i++) {
         const float x = x0 + float(i) * step;
@@ -160,6 +235,12 @@ extern "C" int mtmd_profile(int which, int kind, const double* params, float x0,
             const V3 u = r12 * inv_or_big(r12.norm());
             fval = (u.x != 0.0) ? F.x / u.x : 0.0;
             e = (which == 0) ? NonBonded{kind, p}.energy(r12) : Bond{kind, p}.energy(r12);
+        } else if (which == 2 && kind == MTMD_ANGLE_CLASS2) {
+            const V3 r1{-1.0, 0.0, 0.0}, r2{0.0, 0.0, 0.0}, r3{-std::cos(double(x)), std::sin(double(x)), 0.0};
+            const V3 f3 = class2_force3(r1, r2, r3, q);
+            const V3 c = angle_dtheta_dr3(r1, r2, r3);
+            fval = (c.x != 0.0) ? f3.x / c.x : 0.0;
+            e = class2_energy(r1, r2, r3, q);
         } else if (which == 2) {
             // CMDFFAngle_Harm (MolTwisterMDFFAngle_Harm.cpp:94-130) at r1=(-1,0,0), r2=0, r3=(-cos, sin, 0)
             const V3 r1{-1.0, 0.0, 0.0}, r2{0.0, 0.0, 0.0}, r3{-std::cos(double(x)), std::sin(double(x)), 0.0};
@@ -181,9 +262,20 @@ extern "C" int mtmd_profile(int which, int kind, const double* params, float x0,
             const double cosPhi = (nn == 0.0) ? n1.dot(n2) / 1E-20 : n1.dot(n2) / nn;
             const double phi = std::acos(cosPhi);
             const V3 c = dihedral_dphi_dr4(r1, r2, r3, r4);
-            const V3 f4 = c * (-p[0] * double(D * N) * std::sin(double(N) * phi));
-            fval = (c.x != 0.0) ? f4.x / c.x : 0.0;
-            e = p[0] * (1.0 + double(D) * std::cos(double(N) * phi));
+            if (kind == MTMD_DIH_FOURIER4T) {
+                // CMDFFDih_Fourier4t (MolTwisterMDFFDih_Fourier4t.cpp:98-142)
+                const double dU = -0.5 * p[0] * std::sin(phi) + p[1] * std::sin(2.0 * phi) -
+                                  1.5 * p[2] * std::sin(3.0 * phi) + 2.0 * p[3] * std::sin(4.0 * phi);
+                const V3 f4 = c * dU;
+                fval = (c.x != 0.0) ? f4.x / c.x : 0.0;
+                const double U1 = p[0] * (1.0 + std::cos(phi)), U2 = p[1] * (1.0 - std::cos(2.0 * phi));
+                const double U3 = p[2] * (1.0 + std::cos(3.0 * phi)), U4 = p[3] * (1.0 - std::cos(4.0 * phi));
+                e = 0.5 * (U1 + U2 + U3 + U4);
+            } else {
+                const V3 f4 = c * (-p[0] * double(D * N) * std::sin(double(N) * phi));
+                fval = (c.x != 0.0) ? f4.x / c.x : 0.0;
+                e = p[0] * (1.0 + double(D) * std::cos(double(N) * phi));
+            }
         }
         if (xo) xo[i] = x;
         if (fo) fo[i] = float(fval);

@@ -39,7 +39,7 @@ def _f32(a):
 def profile(which: int, kind: int, params, x0: float, x1: float, n: int = 1000):
     """CMDFF*::calc1DForceProfile/calc1DPotentialProfile through mtmd_profile."""
     lib = abi.load_library()
-    p = np.zeros(4)
+    p = np.zeros(abi.BONDED_PARAMS)          # 4 used by non-bonded kinds
     p[:len(params)] = params
     x, f, e = (np.zeros(n, np.float32) for _ in range(3))
     rc = lib.mtmd_profile(which, kind, _f64(p), x0, x1, n, _f32(x), _f32(f), _f32(e))

@@ -106,7 +106,7 @@ class System:
                 if kind == abi.FF_PROFILE:
                     arr[i].profile = profile_ptr(prm)
                 else:
-                    arr[i].p[:] = list(prm) + [0.0] * (4 - len(prm))
+                    arr[i].p[:] = list(prm) + [0.0] * (abi.BONDED_PARAMS - len(prm))
             setattr(s, nfield, len(lst))
             setattr(s, pfield, C.cast(arr, C.POINTER(abi.MtmdBondedFF)))
             keep.append(arr)

@@ -132,6 +132,9 @@ static v3 nb_force(int kind, const double* p, v3 r1, v3 r2)
         double RInv12 = RInv6 * RInv6;
         double s2 = sig * sig, s4 = s2 * s2, s6 = s4 * s2, s8 = s4 * s4, s12 = s6 * s6;
         c = 4.0 * eps * RInv2 * (12.0 * s12 * RInv12 - 8.0 * alpha * s8 * RInv8);
+    } else if (kind == MTMD_NB_LJBUCK) {
+        /* MolTwisterMDFFNonBonded_LJBuck.cpp:214-251: f1 = f2 = 0 at the end */
+        return V(0.0, 0.0, 0.0);
     }
     return vmul(r12, c);
 }
@@ -162,6 +165,20 @@ static double nb_pot(int kind, const double* p, v3 r1, v3 r2)
         double RInv12 = RInv6 * RInv6;
         double s2 = sig * sig, s4 = s2 * s2, s6 = s4 * s2, s8 = s4 * s4, s12 = s6 * s6;
         return 4.0 * eps * (s12 * RInv12 - alpha * s8 * RInv8);
+    } else if (kind == MTMD_NB_LJBUCK) {
+        /* MolTwisterMDFFNonBonded_LJBuck.cpp:181-211; p[0] selects LJ (0) or Buckingham (1) */
+        if (p[0] == 0.0) {
+            double eps = p[1], sig = p[2];
+            double RInv6 = RInv2 * RInv2 * RInv2, RInv12 = RInv6 * RInv6;
+            double s2 = sig * sig, s6 = s2 * s2 * s2, s12 = s6 * s6;
+            return 4.0 * eps * (s12 * RInv12 - s6 * RInv6);
+        }
+        if (p[0] == 1.0) {
+            double A = p[1], rho = p[2], C = p[3];
+            double RInv4 = RInv2 * RInv2, RInv6 = RInv4 * RInv2;
+            double rhoInv = (rho == 0.0) ? 1E-10 : 1.0 / rho;
+            return A * exp(-R * rhoInv) - C * RInv6;
+        }
     }
     return 0.0;
 }
@@ -342,16 +359,69 @@ static double dihedral_phi(v3 r1, v3 r2, v3 r3, v3 r4)
     return acos(cosPhi);
 }
 
+/* CMDFFAngle_Class2 (MolTwisterMDFFAngle_Class2.cpp:145-296) at (r1, r2, r3):
+ * the force on r3 and the energy, the terms summed in the class's order.
+ * q: theta0, K2, K3, K4 (Ea), M, r1, r2 (Ebb), N1, N2, r1, r2 (Eba) */
+static double class2_dtheta(v3 r1, v3 r2, v3 r3, double theta0)
+{
+    /* Theta - Theta0 of calcPotentialHarm/calcForcesHarm (:145-190) */
+    double R21 = vnorm(vsub(r1, r2)), R23 = vnorm(vsub(r3, r2));
+    double R21Inv = (R21 == 0.0) ? 1.0 / 1E-10 : 1.0 / R21;
+    double R23Inv = (R23 == 0.0) ? 1.0 / 1E-10 : 1.0 / R23;
+    double cosTheta = vdot(vsub(r1, r2), vsub(r3, r2)) * (R21Inv * R23Inv);
+    return acos(cosTheta) - theta0 * TO_RAD;
+}
+
+static v3 class2_f3(v3 r1, v3 r2, v3 r3, const double* q)
+{
+    v3 f3 = V(0.0, 0.0, 0.0), r12, r23, r21, u23, c13 = ang_coeffs13(r1, r2, r3);
+    double R12, R23, RInv12, RInv23, coeff, cosTheta, dT;
+    int degree, i;
+    for (degree = 2; degree <= 4; degree++) {                         /* calcForcesHarm */
+        double d = class2_dtheta(r1, r2, r3, q[0]);
+        double dU = -(double)degree * q[degree - 1];
+        for (i = 0; i < degree - 1; i++) dU *= d;
+        f3 = vadd(f3, vmul(c13, dU));
+    }
+    r12 = vsub(r2, r1); r23 = vsub(r3, r2);
+    R12 = vnorm(r12); R23 = vnorm(r23);
+    RInv12 = (R12 == 0.0) ? 1.0 / 1E-10 : 1.0 / R12;
+    RInv23 = (R23 == 0.0) ? 1.0 / 1E-10 : 1.0 / R23;
+    u23 = vmul(r23, RInv23);
+    f3 = vadd(f3, vmul(u23, (-1.0) * q[4] * (R12 - q[5])));           /* calcForcesCartesian */
+    r21 = vmul(r12, -1.0);                                             /* calcForcesMix */
+    coeff = q[7] * (R12 - q[9]) + q[8] * (R23 - q[10]);
+    cosTheta = vdot(r21, r23) * (RInv12 * RInv23);
+    dT = acos(cosTheta) - q[0] * TO_RAD;
+    f3 = vadd(f3, vsub(vmul(u23, (-1.0) * q[8] * dT), vmul(c13, coeff)));
+    return f3;
+}
+
+static double class2_u(v3 r1, v3 r2, v3 r3, const double* q)
+{
+    double U = 0.0, R12, R23, R21;
+    int degree, i;
+    for (degree = 2; degree <= 4; degree++) {                         /* calcPotentialHarm */
+        double d = class2_dtheta(r1, r2, r3, q[0]), dN = 1.0;
+        for (i = 0; i < degree; i++) dN *= d;
+        U += q[degree - 1] * dN;
+    }
+    R12 = vnorm(vsub(r2, r1)); R23 = vnorm(vsub(r3, r2)); R21 = vnorm(vsub(r1, r2));
+    U += q[4] * (R12 - q[5]) * (R23 - q[6]);                           /* calcPotentialCartesian */
+    U += (q[7] * (R21 - q[9]) + q[8] * (R23 - q[10])) * class2_dtheta(r1, r2, r3, q[0]);   /* calcPotentialMix */
+    return U;
+}
+
 /* ------------------------------------------------------------------ */
 /* Profiles: calc1DForceProfile / calc1DPotentialProfile               */
 /* ------------------------------------------------------------------ */
 int orc_profile(int which, int kind, const double* params, float x0, float x1,
                 int n, float* xo, float* fo, float* eo)
 {
-    double p[4] = {0, 0, 0, 0};
+    double p[MTMD_BONDED_PARAMS];
     int i;
     if (n <= 0) return -1;
-    for (i = 0; i < 4; i++) p[i] = params[i];
+    for (i = 0; i < MTMD_BONDED_PARAMS; i++) p[i] = (which == 0 && i >= 4) ? 0.0 : params[i];
     float delta = (x1 - x0) / (float)(n - 1);
     for (i = 0; i < n; i++) {
         float x = x0 + (float)i * delta;
@@ -375,6 +445,11 @@ int orc_profile(int which, int kind, const double* params, float x0, float x1,
                 v3 c = ang_coeffs13(r1, r2, r3);
                 fval = (c.x != 0.0) ? f3.x / c.x : 0.0;
                 e = p[0] * deltaTheta * deltaTheta;
+            } else if (kind == MTMD_ANGLE_CLASS2) {
+                v3 f3 = class2_f3(r1, r2, r3, p);
+                v3 c = ang_coeffs13(r1, r2, r3);
+                fval = (c.x != 0.0) ? f3.x / c.x : 0.0;
+                e = class2_u(r1, r2, r3, p);
             }
         } else if (which == 3) {
             /* MolTwisterMDFFDih.cpp:272-320 with CMDFFDih_Harm */
@@ -389,6 +464,17 @@ int orc_profile(int which, int kind, const double* params, float x0, float x1,
                 v3 c = dih_coeffs14(r1, r2, r3, r4);
                 fval = (c.x != 0.0) ? f4.x / c.x : 0.0;
                 e = K * (1.0 + (double)D * cos((double)N * phi));
+            } else if (kind == MTMD_DIH_FOURIER4T) {
+                /* MolTwisterMDFFDih_Fourier4t.cpp:98-142 */
+                double phi = dihedral_phi(r1, r2, r3, r4);
+                double d1 = -0.5 * p[0] * sin(phi), d2 = p[1] * sin(2.0 * phi);
+                double d3 = -1.5 * p[2] * sin(3.0 * phi), d4 = 2.0 * p[3] * sin(4.0 * phi);
+                v3 f4 = vmul(dih_coeffs14(r1, r2, r3, r4), d1 + d2 + d3 + d4);
+                v3 c = dih_coeffs14(r1, r2, r3, r4);
+                double U1 = p[0] * (1.0 + cos(phi)), U2 = p[1] * (1.0 - cos(2.0 * phi));
+                double U3 = p[2] * (1.0 + cos(3.0 * phi)), U4 = p[3] * (1.0 - cos(4.0 * phi));
+                fval = (c.x != 0.0) ? f4.x / c.x : 0.0;
+                e = 0.5 * (U1 + U2 + U3 + U4);
             }
         } else {
             return -1;

@@ -119,7 +119,7 @@ def _ref_lib():
 
 
 def profile(which: int, kind: int, params, x0: float, x1: float, n: int = 1000, ref: bool = False):
-    p = np.zeros(4)
+    p = np.zeros(abi.BONDED_PARAMS)          # 4 used by non-bonded kinds
     p[:len(params)] = params
     x, f, e = (np.zeros(n, np.float32) for _ in range(3))
     lib = _ref_lib() if ref else _oracle_lib()

@@ -5,8 +5,9 @@
 // the MolTwister tree (paths below are relative to it). It exposes a small C
 // API so tests can compare the oracle (oracle/mtmd_oracle.c) and the HIP
 // engine with the reference's own arithmetic:
-//   - profiles:  CMDFFNonBonded_LJ/_Buck/_LJ1208::calc1D*Profile,
-//                CMDFFBond_Harm/_Morse, CMDFFAngle_Harm, CMDFFDih_Harm
+//   - profiles:  CMDFFNonBonded_LJ/_Buck/_LJ1208/_LJBuck::calc1D*Profile,
+//                CMDFFBond_Harm/_Morse, CMDFFAngle_Harm/_Class2,
+//                CMDFFDih_Harm/_Fourier4t
 //   - cells:     CFunctorGenCellList::operator() + assembleCellList
 //   - neighbours:CFunctorGenNeighList::operator() with the grow-and-retry loop
 //   - forces:    CFunctorCalcForce::operator()
@@ -47,6 +48,9 @@
 #include "MDFF/Bonds/MolTwisterMDFFBond_Morse.h"
 #include "MDFF/Angles/MolTwisterMDFFAngle_Harm.h"
 #include "MDFF/Dihedrals/MolTwisterMDFFDih_Harm.h"
+#include "MDFF/Non-Bonded/MolTwisterMDFFNonBonded_LJBuck.h"
+#include "MDFF/Angles/MolTwisterMDFFAngle_Class2.h"
+#include "MDFF/Dihedrals/MolTwisterMDFFDih_Fourier4t.h"
 #include "Utilities/DCDFile.h"
 #undef private
 #undef protected
@@ -78,6 +82,13 @@ static std::vector<std::pair<float, float>> profile_of(int which, int kind, cons
             CMDFFNonBonded_LJ1208 f; f.epsilon_ = p[0]; f.sigma_ = p[1]; f.alpha_ = p[2];
             return energy ? f.calc1DPotentialProfile(x0, x1, n) : f.calc1DForceProfile(x0, x1, n);
         }
+        if (kind == MTMD_NB_LJBUCK) {
+            CMDFFNonBonded_LJBuck f;
+            f.ljBuckType_ = (int)p[0];
+            if (f.ljBuckType_ == 0) { f.epsilon_ = p[1]; f.sigma_ = p[2]; }
+            else { f.A_ = p[1]; f.rho_ = p[2]; f.C_ = p[3]; }
+            return energy ? f.calc1DPotentialProfile(x0, x1, n) : f.calc1DForceProfile(x0, x1, n);
+        }
     } else if (which == 1) {
         if (kind == MTMD_BOND_HARM) {
             CMDFFBond_Harm f; f.k_ = p[0]; f.r0_ = p[1];
@@ -92,9 +103,21 @@ static std::vector<std::pair<float, float>> profile_of(int which, int kind, cons
             CMDFFAngle_Harm f; f.k_ = p[0]; f.theta0_ = p[1];
             return energy ? f.calc1DPotentialProfile(x0, x1, n) : f.calc1DForceProfile(x0, x1, n);
         }
+        if (kind == MTMD_ANGLE_CLASS2) {
+            CMDFFAngle_Class2 f;
+            f.Ea_.theta0_ = p[0]; f.Ea_.K2_ = p[1]; f.Ea_.K3_ = p[2]; f.Ea_.K4_ = p[3];
+            f.Ebb_.M_ = p[4]; f.Ebb_.r1_ = p[5]; f.Ebb_.r2_ = p[6];
+            f.Eba_.N1_ = p[7]; f.Eba_.N2_ = p[8]; f.Eba_.r1_ = p[9]; f.Eba_.r2_ = p[10];
+            return energy ? f.calc1DPotentialProfile(x0, x1, n) : f.calc1DForceProfile(x0, x1, n);
+        }
     } else if (which == 3) {
         if (kind == MTMD_DIH_HARM) {
             CMDFFDih_Harm f; f.K_ = p[0]; f.D_ = (int)p[1]; f.N_ = (int)p[2];
+            return energy ? f.calc1DPotentialProfile(x0, x1, n) : f.calc1DForceProfile(x0, x1, n);
+        }
+        if (kind == MTMD_DIH_FOURIER4T) {
+            CMDFFDih_Fourier4t f;
+            for (int i = 0; i < 4; i++) f.V_[i] = p[i];
             return energy ? f.calc1DPotentialProfile(x0, x1, n) : f.calc1DForceProfile(x0, x1, n);
         }
     }

@@ -3,7 +3,7 @@ sources (oracle/_ref/libmtref.so, built from the MolTwister sources by
 oracle/Makefile). Inputs and outputs are stored together, so the fixtures do
 not depend on the generators that made the inputs.
 
-  python tests/golden/make_golden.py
+  python tests/golden/make_golden.py [--profiles-only]
 """
 import os
 import sys
@@ -35,6 +35,8 @@ def system_arrays(s, c):
 
 def main():
     pyoracle.build(with_ref=True)
+    if "--profiles-only" in sys.argv:
+        return profiles_and_dcd()
     for name in CASES:
         s, c = make(name, ensemble=abi.NVT)
         r = pyoracle.Reference(s, c)
@@ -53,12 +55,17 @@ def main():
                    nh_eta_in=eta, nh_peta_in=pe, nh_mom_out=mom_nh, nh_eta_out=eta2, nh_peta_out=pe2)
         np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **out)
         print(name, s.n_atoms, os.path.getsize(os.path.join(HERE, f"{name}.npz")))
+    profiles_and_dcd()
+
+
+def profiles_and_dcd():
     prof = {}
     from test_oracle_ref import PROFILES
     for i, (which, kind, prm, x0, x1) in enumerate(PROFILES):
         x, f, e = pyoracle.profile(which, kind, prm, x0, x1, ref=True)
         prof[f"p{i}_x"], prof[f"p{i}_f"], prof[f"p{i}_e"] = x, f, e
-        prof[f"p{i}_args"] = np.array([which, kind, *(list(prm) + [0.0] * (3 - len(prm))), x0, x1], float)
+        prof[f"p{i}_args"] = np.array([which, kind, *(list(prm[:3]) + [0.0] * (3 - len(prm[:3]))), x0, x1], float)
+        prof[f"p{i}_prm"] = np.array(prm, float)
     np.savez_compressed(os.path.join(HERE, "profiles.npz"), **prof)
     # two DCD frames written by MolTwister's CDCDFile (header + records)
     import tempfile

@@ -18,6 +18,16 @@ def cfg(rc, shell, ensemble=abi.NVE, dt=0.5, T=298.0, coulomb=abi.COUL_AUTO, eng
     return c
 
 
+def chains_class2():
+    """The chain molecules with the other MDFF kinds: Class2 angles, Fourier4t
+    dihedrals and an LJBuck pair (whose reference force is zero)."""
+    s = W.chain_molecules(30, 6, 24.0, seed=4)
+    s.angle_ff = [(abi.ANGLE_CLASS2, (114.0, 250.0, -30.0, 20.0, 5.0, 1.54, 1.54, 8.0, 8.0, 1.54, 1.54))]
+    s.dihedral_ff = [(abi.DIH_FOURIER4T, (2.0, -0.5, 3.0, 0.1))]
+    s.nonbonded = s.nonbonded[:2] + [(1, 1, abi.NB_LJBUCK, (0, 0.3908, 3.95))]
+    return s
+
+
 # name -> (builder, rc, shell)
 SYSTEMS = {
     "fcc_2cells": (lambda: W.fcc_lj(5), 2.5, 0.3),          # 2 cells/axis: duplicated neighbours
@@ -27,6 +37,7 @@ SYSTEMS = {
     "water_300": (lambda: W.water_box(300, seed=5), 6.0, 0.5),
     "chains": (lambda: W.chain_molecules(30, 6, 24.0, seed=4), 8.0, 0.0),
     "chains_rc7": (lambda: W.chain_molecules(30, 6, 24.0, seed=4), 7.0, 0.0),   # >= 3 cells of rc+skin
+    "chains_class2": (chains_class2, 7.0, 0.0),
 }
 
 

@@ -16,7 +16,7 @@ from systems import make
 
 pytestmark = pytest.mark.gpu
 
-FAST = ["binary_lj", "buckingham", "water_300", "fcc_4000"]
+FAST = ["binary_lj", "buckingham", "water_300", "fcc_4000", "chains_class2"]
 
 
 def rms_rel(F, Fr):

@@ -69,13 +69,13 @@ def test_profiles_match_golden(oracle_mod):
     while f"p{i}_x" in z:
         a = z[f"p{i}_args"]
         which, kind = int(a[0]), int(a[1])
-        prm = tuple(a[2:5])
+        prm = tuple(z[f"p{i}_prm"]) if f"p{i}_prm" in z else tuple(a[2:5])
         x0, x1 = float(np.float32(a[5])), float(np.float32(a[6]))
         x, f, e = oracle_mod.profile(which, kind, prm, x0, x1)
         assert np.array_equal(x, z[f"p{i}_x"]) and np.array_equal(f, z[f"p{i}_f"], equal_nan=True)
         assert np.array_equal(e, z[f"p{i}_e"], equal_nan=True)
         i += 1
-    assert i >= 8
+    assert i >= 14
 
 
 def test_library_profiles_match_golden():
@@ -85,7 +85,8 @@ def test_library_profiles_match_golden():
     i = 0
     while f"p{i}_x" in z:
         a = z[f"p{i}_args"]
-        x, f, e = profile(int(a[0]), int(a[1]), tuple(a[2:5]), float(np.float32(a[5])), float(np.float32(a[6])))
+        prm = tuple(z[f"p{i}_prm"]) if f"p{i}_prm" in z else tuple(a[2:5])
+        x, f, e = profile(int(a[0]), int(a[1]), prm, float(np.float32(a[5])), float(np.float32(a[6])))
         assert np.array_equal(x, z[f"p{i}_x"]) and np.array_equal(f, z[f"p{i}_f"], equal_nan=True)
         assert np.array_equal(e, z[f"p{i}_e"], equal_nan=True)
         i += 1

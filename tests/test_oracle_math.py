@@ -239,3 +239,88 @@ def test_harmonic_angle_known_answer(oracle_mod):
             assert F[i, d] == pytest.approx(-(energy(pp) - energy(pm)) / (2 * h), abs=2e-3 * k)
     assert np.abs(F.sum(axis=0)).max() < 1e-6 * k
     assert U.sum() == pytest.approx(energy(pos), rel=2e-3)
+
+
+def _normals_angle(p):
+    """Angle in [0, pi] between the normals r23 x r21 and r32 x r34 of the
+    planes (1,2,3) and (2,3,4): pi for a cis torsion, 0 for trans."""
+    b1, b2, b3 = p[1] - p[0], p[2] - p[1], p[3] - p[2]
+    n1, n2 = np.cross(b2, p[0] - p[1]), np.cross(-b2, b3)
+    return np.arccos(n1 @ n2 / np.linalg.norm(n1) / np.linalg.norm(n2))
+
+
+@pytest.mark.parametrize("kind,prm,U", [
+    (abi.DIH_HARM, (3.0, 1, 3), lambda f: 3.0 * (1.0 + np.cos(3 * f))),
+    (abi.DIH_FOURIER4T, (1.5, -2.0, 0.7, 0.3),
+     lambda f: 0.5 * (1.5 * (1 + np.cos(f)) - 2.0 * (1 - np.cos(2 * f)) + 0.7 * (1 + np.cos(3 * f))
+                      + 0.3 * (1 - np.cos(4 * f)))),
+])
+def test_dihedral_forces_known_answer(oracle_mod, kind, prm, U):
+    """Forces of a torsion on four atoms are -dE/dr of the closed-form energy
+    E = U(phi) (central differences) to table accuracy, they sum to zero, and
+    the energies of the two centres add up to E. phi is the dihedral with cis
+    = 0, i.e. pi minus the angle between the plane normals: the profile
+    geometry (MolTwisterMDFFDih.cpp:272-320: r1 = (0,1,0), r4 = (sin x,
+    cos x, 1)) is cis at x = 0, where the classes' normals are antiparallel."""
+    from moltwister_amd.system import System
+    pos = np.array([[0.1, 1.2, -0.1], [0.0, 0.0, 0.0], [0.05, -0.1, 1.3], [0.9, 0.6, 1.5]])
+    s = System(pos=pos, mass=[1, 1, 1, 1], charge=[0, 0, 0, 0], types=[0, 0, 0, 0], box=(30, 30, 30),
+               nonbonded=[], dihedral_ff=[(kind, prm)], dihedrals=[[0, 1, 2, 3, 0]])
+    c = cfg(5.0, 0.5)
+    c.scale14 = 0.0
+    o = oracle_mod.Oracle(s, c)
+    F, Ua = o.compute_forces()
+
+    def energy(p):
+        return U(np.pi - _normals_angle(p))
+
+    h = 1e-6
+    Fmax = 0.0
+    for i in range(4):
+        for d in range(3):
+            pp, pm = pos.copy(), pos.copy()
+            pp[i, d] += h
+            pm[i, d] -= h
+            ref = -(energy(pp) - energy(pm)) / (2 * h)
+            Fmax = max(Fmax, abs(ref))
+            assert F[i, d] == pytest.approx(ref, abs=2e-3 * max(1.0, abs(ref)))
+    assert Fmax > 0.5
+    assert np.abs(F.sum(axis=0)).max() < 1e-6 * Fmax
+    assert float(Ua.sum()) == pytest.approx(energy(pos), rel=1e-4, abs=1e-5)
+
+
+def test_class2_angle_tables_closed_form(oracle_mod):
+    """CMDFFAngle_Class2 (MolTwisterMDFFAngle_Class2.cpp:145-296) at the
+    profile's unit bond lengths: U(theta) = K2 d^2 + K3 d^3 + K4 d^4
+    + M (1 - r1)(1 - r2) + (N1 (1 - r1') + N2 (1 - r2')) d, d = theta - theta0;
+    with only the Ea terms the force column is -dU/dtheta, and Ea with K3 =
+    K4 = 0 is the harmonic angle."""
+    two_pi = float(np.float32(2 * np.pi))
+    q = (112.0, 40.0, -20.0, 10.0, 5.0, 1.1, 1.3, 3.0, 4.0, 1.2, 0.9)
+    x, f, e = oracle_mod.profile(2, abi.ANGLE_CLASS2, q, 0.0, two_pi)
+    th = x.astype(np.float64)
+    sel = (th > 0.05) & (th < np.pi - 0.05)
+    d = th - np.radians(q[0])
+    U = q[1] * d ** 2 + q[2] * d ** 3 + q[3] * d ** 4 + q[4] * (1 - q[5]) * (1 - q[6]) \
+        + (q[7] * (1 - q[9]) + q[8] * (1 - q[10])) * d
+    assert np.allclose(e[sel], U[sel], rtol=2e-6, atol=2e-5)
+    qa = q[:4] + (0.0,) * 7
+    _, fa, _ = oracle_mod.profile(2, abi.ANGLE_CLASS2, qa, 0.0, two_pi)
+    dU = 2 * q[1] * d + 3 * q[2] * d ** 2 + 4 * q[3] * d ** 3
+    assert np.allclose(fa[sel], -dU[sel], rtol=1e-5, atol=1e-3)
+    _, f2, e2 = oracle_mod.profile(2, abi.ANGLE_CLASS2, (104.5, 75.0) + (0.0,) * 9, 0.0, two_pi)
+    _, fh, eh = oracle_mod.profile(2, abi.ANGLE_HARM, (75.0, 104.5), 0.0, two_pi)
+    assert np.array_equal(f2, fh)
+    assert np.allclose(e2, eh, rtol=1e-6, atol=1e-6)
+
+
+def test_ljbuck_tables(oracle_mod):
+    """CMDFFNonBonded_LJBuck: the energy column of its LJ (alpha 1) and
+    Buckingham forms, and a force column of zeros (its calcForces zeroes both
+    forces at the end, MolTwisterMDFFNonBonded_LJBuck.cpp:250)."""
+    for lb, plain in (((0, 1.2, 3.1), (abi.NB_LJ, (1.2, 3.1, 1.0))),
+                      ((1, 1000.0, 0.2, 5.0), (abi.NB_BUCK, (1000.0, 0.2, 5.0)))):
+        x, f, e = oracle_mod.profile(0, abi.NB_LJBUCK, lb, 0.01, 10.0)
+        xp, fp, ep = oracle_mod.profile(0, plain[0], plain[1], 0.01, 10.0)
+        assert np.array_equal(x, xp) and np.array_equal(e, ep)
+        assert np.all(f == 0.0) and np.any(fp != 0.0)

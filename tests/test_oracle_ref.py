@@ -28,6 +28,11 @@ PROFILES = [
     (2, abi.ANGLE_HARM, (158.78, 113.24), 0.0, float(np.float32(2 * np.pi))),
     (3, abi.DIH_HARM, (5.0, 1, 3), 0.0, float(np.float32(2 * np.pi))),
     (3, abi.DIH_HARM, (2.0, -1, 2), 0.0, float(np.float32(2 * np.pi))),
+    (0, abi.NB_LJBUCK, (0, 1.2, 3.1), 0.01, 10.0),
+    (0, abi.NB_LJBUCK, (1, 1000.0, 0.2, 5.0), 0.01, 8.0),
+    (2, abi.ANGLE_CLASS2, (112.0, 40.0, -20.0, 10.0, 5.0, 1.1, 1.3, 3.0, 4.0, 1.2, 0.9), 0.0,
+     float(np.float32(2 * np.pi))),
+    (3, abi.DIH_FOURIER4T, (1.5, -2.0, 0.7, 0.3), 0.0, float(np.float32(2 * np.pi))),
 ]
 
 

@@ -356,6 +356,33 @@ int mtmd_run(const mtmd_config* cfg, const mtmd_system* sys, void* log_file);
  * out_stride steps. */
 int mtmd_run_optimize(const mtmd_config* cfg, const mtmd_system* sys, void* log_file);
 
+/* ---- topology search ----
+ * CMolTwisterStateTools::getAllMDBondsInSystem / getAllMDAnglesInSystem /
+ * getAllMDDihedralsInSystem (Cmd/Tools/MolTwisterStateTools.cpp:544-724,
+ * 767-920, 993-1140) for force fields detected with the all_r_less_than
+ * criterion: chains of atoms of the rule's types, each link shorter than R0,
+ * in the reference's order (rule, then first, second, ... atom ascending),
+ * duplicates and reversed duplicates removed keeping the first
+ * (MolecularSystemTools.cpp:25-200). which: 1 bonds, 2 angles, 3 dihedrals;
+ * rule r gives the types of its atoms (the first which + 1 entries) and R0.
+ * box: periodic widths for the folded distance of
+ * CAtom::getDistanceTo2UsingPBC (what the force-field matrices use), or NULL
+ * for the plain distance. out: [(which + 2) * max_out] entries laid out as
+ * mtmd_system's bonds / angles / dihedrals (atoms, then the rule index), or
+ * NULL to count only; *n_out receives the count. Returns MTMD_EINVAL when out
+ * is too short. The molecule and visible-bond criteria walk the caller's own
+ * bond graph and are not searched here. Runs on the first HIP device;
+ * mtmd_topology_last_error() gives the message of a failed call. */
+typedef struct mtmd_topo_rule {
+    int32_t types[4];
+    double  R0;
+} mtmd_topo_rule;
+
+int mtmd_find_topology(int32_t which, int32_t n_atoms, const double* pos, const int32_t* type, const double* box,
+                       int32_t n_rules, const mtmd_topo_rule* rules, int32_t max_out, int32_t* out,
+                       int32_t* n_out);
+const char* mtmd_topology_last_error(void);
+
 /* ---- batches of independent systems (replicas) ----
  * No counterpart in the reference, which advances one system per `moldyn run`
  * (one CSimulationBox + CMDLoop, SimulationBox.cpp:27-133, MDLoop.cpp:107-141).

@@ -125,6 +125,10 @@ class MtmdSystem(C.Structure):
     ]
 
 
+class MtmdTopoRule(C.Structure):
+    _fields_ = [("types", C.c_int32 * 4), ("R0", C.c_double)]
+
+
 class MtmdThermo(C.Structure):
     _fields_ = [(k, C.c_double) for k in ("T", "P", "V", "U", "K", "Etot", "sum_p2m", "sum_fr", "U_cut")]
 
@@ -140,7 +144,7 @@ EXPORTED = [
     "mtmd_build_neighbors", "mtmd_compute_forces", "mtmd_step", "mtmd_get_thermo",
     "mtmd_get_forces", "mtmd_optimize", "mtmd_get_stats", "mtmd_synchronize", "mtmd_run",
     "mtmd_run_optimize", "mtmd_bench_kernel", "mtmd_append_xyz", "mtmd_append_dcd",
-    "mtmd_set_barostat", "mtmd_get_barostat",
+    "mtmd_set_barostat", "mtmd_get_barostat", "mtmd_find_topology", "mtmd_topology_last_error",
     "mtmd_batch_create", "mtmd_batch_destroy", "mtmd_batch_last_error", "mtmd_batch_step",
     "mtmd_batch_step_separate", "mtmd_batch_synchronize", "mtmd_batch_get_thermo", "mtmd_batch_get_state",
     "mtmd_batch_get_forces", "mtmd_batch_autotune", "mtmd_batch_info",
@@ -191,6 +195,9 @@ def load_library(path: str | None = None) -> C.CDLL:
         "mtmd_bench_kernel": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(C.c_double)]),
         "mtmd_append_xyz": (C.c_int, [vp, C.c_char_p, C.c_int32]),
         "mtmd_set_barostat": (C.c_int, [vp, C.c_double, C.c_double, _dp, _dp]),
+        "mtmd_find_topology": (C.c_int, [C.c_int32, C.c_int32, _dp, _ip, _dp, C.c_int32, C.POINTER(MtmdTopoRule),
+                                         C.c_int32, _ip, _ip]),
+        "mtmd_topology_last_error": (C.c_char_p, []),
         "mtmd_get_barostat": (C.c_int, [vp, _dp, _dp, _dp, _dp, _dp]),
         "mtmd_append_dcd": (C.c_int, [vp, C.c_char_p, C.c_int32, C.c_int32]),
         "mtmd_batch_create": (C.c_int, [C.c_int, C.POINTER(MtmdSystem), C.POINTER(MtmdConfig),

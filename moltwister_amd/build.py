@@ -6,6 +6,8 @@ Translation units:
   mtmd_fast.hip    FP64/MIXED engine kernels (contraction allowed; the exact
                    device helpers disable it locally)
   mtmd_host.cpp    C ABI, set-up and step driver (hipcc, host code)
+  mtmd_topo.hip    topology search (bonds, angles, dihedrals by distance
+                   criteria), -ffp-contract=off
   mtmd_tables.cpp  force/energy profiles, g++ -ffp-contract=off so the float
                    tables are the ones a default x86-64 build of the reference
                    computes
@@ -51,6 +53,8 @@ def build(verbose: bool = False) -> str:
         [HIPCC, *HIP_FLAGS, "-c", "-o", os.path.join(OUT, "mtmd_fast.o"), os.path.join(SRC, "mtmd_fast.hip")],
         [HIPCC, *HIP_FLAGS, "-c", "-o", os.path.join(OUT, "mtmd_batch.o"), os.path.join(SRC, "mtmd_batch.hip")],
         [HIPCC, *HIP_FLAGS, "-c", "-o", os.path.join(OUT, "mtmd_fm.o"), os.path.join(SRC, "mtmd_fm.hip")],
+        [HIPCC, *HIP_FLAGS, "-ffp-contract=off", "-c", "-o", os.path.join(OUT, "mtmd_topo.o"),
+         os.path.join(SRC, "mtmd_topo.hip")],
         [HIPCC, *HIP_FLAGS, "-x", "hip", "-c", "-o", os.path.join(OUT, "mtmd_host.o"),
          os.path.join(SRC, "mtmd_host.cpp")],
         [shutil.which("g++") or "g++", "-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-c",
@@ -58,11 +62,11 @@ def build(verbose: bool = False) -> str:
         [shutil.which("g++") or "g++", "-O2", "-fPIC", "-std=c++17", "-Wall", "-c",
          "-o", os.path.join(OUT, "mtmd_output.o"), os.path.join(SRC, "mtmd_output.cpp")],
     ]
-    with ThreadPoolExecutor(max_workers=7) as ex:
+    with ThreadPoolExecutor(max_workers=8) as ex:
         for r in ex.map(_run, jobs):
             if verbose and (r.stdout or r.stderr):
                 sys.stderr.write(r.stdout + r.stderr)
-    objs = [os.path.join(OUT, f) for f in ("mtmd_exact.o", "mtmd_fast.o", "mtmd_batch.o", "mtmd_fm.o",
+    objs = [os.path.join(OUT, f) for f in ("mtmd_exact.o", "mtmd_fast.o", "mtmd_batch.o", "mtmd_fm.o", "mtmd_topo.o",
                                            "mtmd_host.o", "mtmd_tables.o", "mtmd_output.o")]
     # workload generator helpers (host only)
     _run([shutil.which("gcc") or "gcc", "-O2", "-fPIC", "-shared", "-o", GENLIB, os.path.join(SRC, "mtmd_gen.c"), "-lm"])

@@ -171,6 +171,30 @@ class Engine:
         self._chk(self.lib.mtmd_append_dcd(self.h, path.encode(), int(n_steps), int(stride)))
 
 
+def find_topology(which: int, pos, types, box, rules) -> np.ndarray:
+    """mtmd_find_topology: bonds (which 1), angles (2) or dihedrals (3) of the
+    all_r_less_than rules [(types tuple, R0)], rows as mtmd_system's lists
+    (atoms, then the rule index); box None for the plain distance."""
+    lib = abi.load_library()
+    pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
+    types = np.ascontiguousarray(types, np.int32)
+    arr = (abi.MtmdTopoRule * max(1, len(rules)))()
+    for i, (t, r0) in enumerate(rules):
+        arr[i].types[:] = list(t) + [0] * (4 - len(t))
+        arr[i].R0 = r0
+    b = None if box is None else np.ascontiguousarray(box, np.float64)
+    n = C.c_int32(0)
+    args = (which, pos.shape[0], _f64(pos), _i32(types), _f64(b), len(rules), arr)
+    rc = lib.mtmd_find_topology(*args, 0, None, C.byref(n))
+    if rc:
+        raise MtmdError(rc, lib.mtmd_topology_last_error().decode())
+    out = np.zeros((max(n.value, 1), which + 2), np.int32)
+    rc = lib.mtmd_find_topology(*args, n.value, _i32(out), C.byref(n))
+    if rc:
+        raise MtmdError(rc, lib.mtmd_topology_last_error().decode())
+    return out[:n.value]
+
+
 def run(system: System, cfg: abi.MtmdConfig, optimize: bool = False) -> None:
     """CMDSimulator::run / optimize through mtmd_run / mtmd_run_optimize: the
     files named by cfg (thermo table, XYZ, DCD, histograms) are written."""

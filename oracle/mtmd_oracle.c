@@ -1499,3 +1499,93 @@ int orc_optimize(orc_md* o, int max_steps, double lr, double accuracy, int out_s
     o->have_forces = 0;
     return i;
 }
+
+/* ------------------------------------------------------------------ */
+/* Topology search with the all_r_less_than criterion                   */
+/* (MolTwisterStateTools.cpp:544-724,767-920,993-1140): nested loops    */
+/* over the atoms of each type in index order, then the duplicate       */
+/* removal of MolecularSystemTools.cpp:25-200 (first occurrence kept,   */
+/* an entry equals its reverse). Brute force: small systems only.       */
+/* ------------------------------------------------------------------ */
+static double topo_d2(const double* pos, int a, int b, const double* w)
+{
+    double dx = pos[3 * a] - pos[3 * b], dy = pos[3 * a + 1] - pos[3 * b + 1], dz = pos[3 * a + 2] - pos[3 * b + 2];
+    if (w) {   /* CAtom::getDistanceTo2UsingPBC, MolTwisterAtom.cpp:374-394 */
+        dx = (dx < 0.0) ? -dx : dx;
+        dy = (dy < 0.0) ? -dy : dy;
+        dz = (dz < 0.0) ? -dz : dz;
+        if (dx > (w[0] / 2.0)) dx = w[0] - dx;
+        if (dy > (w[1] / 2.0)) dy = w[1] - dy;
+        if (dz > (w[2] / 2.0)) dz = w[2] - dz;
+    }
+    return (dx * dx + dy * dy + dz * dz);
+}
+
+static int topo_same(const int* a, const int* b, int na)
+{
+    int q, fwd = 1, rev = 1;
+    for (q = 0; q < na; q++) { fwd &= (a[q] == b[q]); rev &= (a[q] == b[na - 1 - q]); }
+    return fwd || rev;
+}
+
+int orc_find_topology(int which, int n, const double* pos, const int32_t* type, const double* box, int n_rules,
+                      const mtmd_topo_rule* rules, int max_out, int32_t* out)
+{
+    const int na = which + 1, width = which + 2;
+    int cap = 1024, cnt = 0, r, j, k, l, m, e, f;
+    int* buf = (int*)malloc(sizeof(int) * (size_t)cap * width);
+    int* lists[4];
+    int nlist[4];
+    for (r = 0; r < n_rules; r++) {
+        const double R02 = rules[r].R0 * rules[r].R0;
+        int q;
+        for (q = 0; q < na; q++) {          /* CMolTwisterState::getAtomsWithID (MolTwisterState.cpp:584-596) */
+            lists[q] = (int*)malloc(sizeof(int) * (size_t)(n + 1));
+            nlist[q] = 0;
+            for (j = 0; j < n; j++) if (type[j] == rules[r].types[q]) lists[q][nlist[q]++] = j;
+        }
+#define EMIT(...) do {                                                                   \
+            int v_[5] = {__VA_ARGS__};                                                   \
+            if (cnt == cap) { cap *= 2; buf = (int*)realloc(buf, sizeof(int) * (size_t)cap * width); } \
+            memcpy(buf + (size_t)cnt * width, v_, sizeof(int) * width);                  \
+            cnt++;                                                                       \
+        } while (0)
+        for (j = 0; j < nlist[0]; j++) {
+            int a = lists[0][j];
+            for (k = 0; k < nlist[1]; k++) {
+                int b = lists[1][k];
+                if (a == b) continue;
+                if (!(topo_d2(pos, a, b, box) < R02)) continue;
+                if (which == 1) { EMIT(a, b, r, 0, 0); continue; }
+                for (l = 0; l < nlist[2]; l++) {
+                    int c = lists[2][l];
+                    if (c == b || c == a) continue;
+                    if (!(topo_d2(pos, b, c, box) < R02)) continue;
+                    if (which == 2) { EMIT(a, b, c, r, 0); continue; }
+                    for (m = 0; m < nlist[3]; m++) {
+                        int d = lists[3][m];
+                        if (d == c || d == b || d == a) continue;
+                        if (topo_d2(pos, c, d, box) < R02) EMIT(a, b, c, d, r);
+                    }
+                }
+            }
+        }
+#undef EMIT
+        for (q = 0; q < na; q++) free(lists[q]);
+    }
+    /* removeDuplicate*Indices: keep the first of equal or reversed entries */
+    {
+        int kept = 0;
+        for (e = 0; e < cnt; e++) {
+            int dup = 0;
+            for (f = 0; f < kept && !dup; f++) dup = topo_same(buf + (size_t)f * width, buf + (size_t)e * width, na);
+            if (dup) continue;
+            if (kept != e) memmove(buf + (size_t)kept * width, buf + (size_t)e * width, sizeof(int) * width);
+            kept++;
+        }
+        cnt = kept;
+    }
+    if (out) memcpy(out, buf, sizeof(int) * (size_t)(cnt < max_out ? cnt : max_out) * width);
+    free(buf);
+    return cnt;
+}

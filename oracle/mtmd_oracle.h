@@ -77,6 +77,11 @@ void orc_get_forces(const orc_md* o, double* F, float* U);
 /* gradient descent (MDLoop.cpp:143-173); returns steps done */
 int  orc_optimize(orc_md* o, int max_steps, double lr, double accuracy, int out_stride);
 
+/* topology search with the all_r_less_than criterion (brute force);
+ * returns the number of entries, writes at most max_out of them */
+int orc_find_topology(int which, int n, const double* pos, const int32_t* type, const double* box, int n_rules,
+                      const mtmd_topo_rule* rules, int max_out, int32_t* out);
+
 /* reference scalar helpers */
 double orc_cmt_exp(double x);
 double orc_cmt_sinhxoverx(double x);

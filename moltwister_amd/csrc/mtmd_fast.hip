@@ -337,11 +337,15 @@ __device__ inline bool mt_is_partner(const MtDev& d, int k, int ia)
 // slot of a list-word index (the index itself unless the words are in cell order)
 __device__ __forceinline__ int mt_id_slot(const MtDev& d, int id) { return d.wsorted ? d.cell_list[id] : id; }
 
-// k: the home atom's slot (its list column); self: its own word index
-template <int MODE>
+// k: the home atom's slot (its list column); self: its own word index; e: the
+// candidate's place in the staged cells. 32-bit words: the candidate's word
+// index | offset code, four to a chunk (b0..b2 hold the chunk's first three).
+// W16: e itself, eight to a chunk (pend holds an even word until its pair
+// is complete, b0..b2 the chunk's first three pairs).
+template <int MODE, bool W16>
 __device__ __forceinline__ void mt_neigh_hit(const MtDev& d, uint4* nl4, float4 c, uint32_t code, int k, int self,
-                                             int molk, const int* smol, int e, int n, int ncap, int& cnt,
-                                             uint32_t& b0, uint32_t& b1, uint32_t& b2)
+                                             int molk, const int* smol, int e, int n, int cap, int& cnt,
+                                             uint32_t& b0, uint32_t& b1, uint32_t& b2, uint32_t& pend)
 {
     const int a = (int)__float_as_uint(c.w);
     if (a == self) return;
@@ -351,71 +355,39 @@ __device__ __forceinline__ void mt_neigh_hit(const MtDev& d, uint4* nl4, float4 
     if constexpr (MODE == 2) {
         if (mt_is_partner(d, k, mt_id_slot(d, a))) return;
     }
-    const uint32_t w = (uint32_t)a | code;
-    const int slot = cnt & 3;
-    b0 = slot == 0 ? w : b0;
-    b1 = slot == 1 ? w : b1;
-    b2 = slot == 2 ? w : b2;
-    if (slot == 3 && cnt < ncap) nl4[(size_t)(cnt >> 2) * n + k] = make_uint4(b0, b1, b2, w);
+    if constexpr (W16) {
+        const int slot = cnt & 7;
+        const uint32_t pair = pend | ((uint32_t)e << 16);
+        pend = (slot & 1) ? pend : (uint32_t)e;
+        b0 = slot == 1 ? pair : b0;
+        b1 = slot == 3 ? pair : b1;
+        b2 = slot == 5 ? pair : b2;
+        if (slot == 7 && cnt < cap) nl4[(size_t)(cnt >> 3) * n + k] = make_uint4(b0, b1, b2, pair);
+    } else {
+        const uint32_t w = (uint32_t)a | code;
+        const int slot = cnt & 3;
+        b0 = slot == 0 ? w : b0;
+        b1 = slot == 1 ? w : b1;
+        b2 = slot == 2 ? w : b2;
+        if (slot == 3 && cnt < cap) nl4[(size_t)(cnt >> 2) * n + k] = make_uint4(b0, b1, b2, w);
+    }
     cnt++;
 }
 
-template <int MODE>
-__global__ void __launch_bounds__(256) k_fast_neigh_block(MtDev d, MtParams p, int cond)
+// the lists of one workgroup's home atoms (the scan below the staging set-up
+// of k_fast_neigh_block); W16 is uniform over the workgroup
+template <int MODE, bool W16>
+__device__ __forceinline__ void mt_neigh_block_atoms(const MtDev& d, const MtParams& p, float4* spos, int* smol,
+                                                     const int* s_first, const int* s_cnt, const int* s_off, int hxo,
+                                                     int hyo, int hzo, int l, int f0, int cnt_c, int cnt_max)
 {
-    if (mt_skip_rebuild(cond, d.flags, p.half_skin2)) return;
-    __shared__ float4 spos[NB_CAP];
-    __shared__ int smol_buf[MODE >= 1 ? NB_CAP : 1];
-    __shared__ int s_first[64], s_cnt[64], s_off[65];
-    __shared__ int s_hfirst[8], s_hcnt[8];
-    int* smol = smol_buf;
     const int t = threadIdx.x;
-    const int Nx = d.nc[0], Ny = d.nc[1], Nz = d.nc[2];
-    const int bx = (Nx + 1) >> 1, by = (Ny + 1) >> 1;
-    const int g = mt_xcd_block(blockIdx.x, gridDim.x);
-    const int x0 = 2 * (g % bx), y0 = 2 * ((g / bx) % by), z0 = 2 * (g / (bx * by));
-    // this half-wave's home cell: (x0 + hxo, y0 + hyo, z0 + hzo)
-    const int hc = t >> 5, l = t & 31;
-    const int hxo = hc & 1, hyo = (hc >> 1) & 1, hzo = hc >> 2;
-    const int hx = x0 + hxo, hy = y0 + hyo, hz = z0 + hzo;
-    const bool home_ok = hx < Nx && hy < Ny && hz < Nz;
-    if (t < 64) {
-        // staged cell q = (ox+1) + 4*((oy+1) + 4*(oz+1)), o* in -1..2 around (x0, y0, z0)
-        const int ox = (t & 3) - 1, oy = ((t >> 2) & 3) - 1, oz = (t >> 4) - 1;
-        int cx = x0 + ox, cy = y0 + oy, cz = z0 + oz;
-        cx = cx < 0 ? cx + Nx : (cx >= Nx ? cx - Nx : cx);
-        cy = cy < 0 ? cy + Ny : (cy >= Ny ? cy - Ny : cy);
-        cz = cz < 0 ? cz + Nz : (cz >= Nz ? cz - Nz : cz);
-        const int cc = cx + Nx * (cy + Ny * cz);
-        const int cn = d.cell_count[cc];
-        s_first[t] = d.cell_first[cc];
-        s_cnt[t] = cn;
-        // inclusive scan of the 64 counts across the wavefront
-        int v = cn;
-        for (int o = 1; o < 64; o <<= 1) {
-            const int u = __shfl_up(v, o);
-            if (t >= o) v += u;
-        }
-        s_off[t + 1] = v;
-        if (t == 0) s_off[0] = 0;
-    }
-    if ((t & 31) == 0) {
-        const int hcell = home_ok ? hx + Nx * (hy + Ny * hz) : 0;
-        s_hfirst[hc] = home_ok ? d.cell_first[hcell] : 0;
-        s_hcnt[hc] = home_ok ? d.cell_count[hcell] : 0;
-    }
-    __syncthreads();
-    const int f0 = s_hfirst[hc], cnt_c = s_hcnt[hc];
-    int cnt_max = 0;
-    for (int i = 0; i < 8; i++) cnt_max = max(cnt_max, s_hcnt[i]);
-    if (cnt_max == 0) return;
-    if (d.flags[3]) {
-        for (int i = l; i < cnt_c; i += 32) d.ncount[d.cell_list[f0 + i]] = 0;
-        return;
-    }
-    const int n = d.n, ncap = d.ncap;
+    const int n = d.n;
+    const int cap = W16 ? 2 * d.ncap : d.ncap;          // words a row holds
     const float rl2 = p.rlist2;
     uint4* __restrict__ nl4 = reinterpret_cast<uint4*>(d.nlist);
+    // the home cell's place among the staged cells (W16 pads: the atom itself)
+    const int qh = (hxo + 1) + 4 * ((hyo + 1) + 4 * (hzo + 1));
     int staged_b = -1, staged_e = -1;
     for (int ib = 0; ib < cnt_max; ib += 32) {
         const bool act = ib + l < cnt_c;
@@ -427,10 +399,11 @@ __global__ void __launch_bounds__(256) k_fast_neigh_block(MtDev d, MtParams p, i
         // MODE 2: only atoms with a bonded partner outside their molecule search
         const bool xk = MODE == 2 && act && d.xmol[d.perm[k]] != 0;
         int cnt = 0;
-        uint32_t b0 = 0, b1 = 0, b2 = 0;
+        uint32_t b0 = 0, b1 = 0, b2 = 0, pend = 0;
         int cb = 0;
         while (cb < 64) {
-            // batch of whole staged cells [cb, ce) that fits NB_CAP (uniform)
+            // batch of whole staged cells [cb, ce) that fits NB_CAP (uniform;
+            // a W16 block is one batch)
             int ce = cb + 1;
             while (ce < 64 && s_off[ce + 1] - s_off[cb] <= NB_CAP) ce++;
             const int base = s_off[cb];
@@ -478,11 +451,11 @@ __global__ void __launch_bounds__(256) k_fast_neigh_block(MtDev d, MtParams p, i
                                 const bool h0 = dx0 * dx0 + dy0 * dy0 + dz0 * dz0 < rl2;
                                 const bool h1 = dx1 * dx1 + dy1 * dy1 + dz1 * dz1 < rl2;
                                 if (h0 && (MODE != 2 || !xk || !mt_is_partner(d, k, mt_id_slot(d, (int)__float_as_uint(c0.w)))))
-                                    mt_neigh_hit<MODE == 2 ? 1 : MODE>(d, nl4, c0, code, k, self, molk, smol, e, n, ncap,
-                                                                         cnt, b0, b1, b2);
+                                    mt_neigh_hit<MODE == 2 ? 1 : MODE, W16>(d, nl4, c0, code, k, self, molk, smol, e, n,
+                                                                              cap, cnt, b0, b1, b2, pend);
                                 if (h1 && (MODE != 2 || !xk || !mt_is_partner(d, k, mt_id_slot(d, (int)__float_as_uint(c1.w)))))
-                                    mt_neigh_hit<MODE == 2 ? 1 : MODE>(d, nl4, c1, code, k, self, molk, smol, e + 1, n,
-                                                                         ncap, cnt, b0, b1, b2);
+                                    mt_neigh_hit<MODE == 2 ? 1 : MODE, W16>(d, nl4, c1, code, k, self, molk, smol, e + 1,
+                                                                              n, cap, cnt, b0, b1, b2, pend);
                             }
                             if (e < e1) {
                                 float4 c0 = spos[e];
@@ -490,24 +463,100 @@ __global__ void __launch_bounds__(256) k_fast_neigh_block(MtDev d, MtParams p, i
                                 const float dx0 = qx - c0.x, dy0 = qy - c0.y, dz0 = qz - c0.z;
                                 if (dx0 * dx0 + dy0 * dy0 + dz0 * dz0 < rl2 &&
                                     (MODE != 2 || !xk || !mt_is_partner(d, k, mt_id_slot(d, (int)__float_as_uint(c0.w)))))
-                                    mt_neigh_hit<MODE == 2 ? 1 : MODE>(d, nl4, c0, code, k, self, molk, smol, e, n, ncap,
-                                                                         cnt, b0, b1, b2);
+                                    mt_neigh_hit<MODE == 2 ? 1 : MODE, W16>(d, nl4, c0, code, k, self, molk, smol, e, n,
+                                                                              cap, cnt, b0, b1, b2, pend);
                             }
                         }
             }
             cb = ce;
         }
         if (act) {
-            // last partial chunk, padded with the atom itself (offset code 0,0,0)
-            const int slot = cnt & 3;
-            if (slot && cnt < ncap) {
-                const uint32_t pad = (uint32_t)self | (NB_PAD_CODE << MT_NW_IDX_BITS);
-                nl4[(size_t)(cnt >> 2) * n + k] = make_uint4(b0, slot > 1 ? b1 : pad, slot > 2 ? b2 : pad, pad);
+            // last partial chunk, padded with the atom itself
+            if constexpr (W16) {
+                const int slot = cnt & 7;
+                if (slot && cnt < cap) {
+                    const uint32_t pad = (uint32_t)(s_off[qh] + ib + l), pp = pad | (pad << 16);
+                    // the pair holding word `slot`: its even word (if filled) and a pad
+                    const int ri = slot >> 1;
+                    const uint32_t last = (slot & 1) ? (pend | (pad << 16)) : pp;
+                    nl4[(size_t)(cnt >> 3) * n + k] =
+                        make_uint4(ri > 0 ? b0 : last, ri > 1 ? b1 : (ri == 1 ? last : pp),
+                                   ri > 2 ? b2 : (ri == 2 ? last : pp), ri == 3 ? last : pp);
+                }
+            } else {
+                const int slot = cnt & 3;
+                if (slot && cnt < cap) {
+                    const uint32_t pad = (uint32_t)self | (NB_PAD_CODE << MT_NW_IDX_BITS);
+                    nl4[(size_t)(cnt >> 2) * n + k] = make_uint4(b0, slot > 1 ? b1 : pad, slot > 2 ? b2 : pad, pad);
+                }
             }
             d.ncount[k] = cnt;
-            if (cnt > ncap) { d.flags[2] = 1; atomicMax(d.nreq, cnt); }
+            // the capacity request is in 32-bit words
+            if (cnt > cap) { d.flags[2] = 1; atomicMax(d.nreq, W16 ? (cnt + 1) >> 1 : cnt); }
         }
     }
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(256) k_fast_neigh_block(MtDev d, MtParams p, int cond)
+{
+    if (mt_skip_rebuild(cond, d.flags, p.half_skin2)) return;
+    __shared__ float4 spos[NB_CAP];
+    __shared__ int smol_buf[MODE >= 1 ? NB_CAP : 1];
+    __shared__ int s_first[64], s_cnt[64], s_off[65];
+    __shared__ int s_hfirst[8], s_hcnt[8];
+    const int t = threadIdx.x;
+    const int Nx = d.nc[0], Ny = d.nc[1], Nz = d.nc[2];
+    const int bx = (Nx + 1) >> 1, by = (Ny + 1) >> 1;
+    const int g = mt_xcd_block(blockIdx.x, gridDim.x);
+    const int x0 = 2 * (g % bx), y0 = 2 * ((g / bx) % by), z0 = 2 * (g / (bx * by));
+    // this half-wave's home cell: (x0 + hxo, y0 + hyo, z0 + hzo)
+    const int hc = t >> 5, l = t & 31;
+    const int hxo = hc & 1, hyo = (hc >> 1) & 1, hzo = hc >> 2;
+    const int hx = x0 + hxo, hy = y0 + hyo, hz = z0 + hzo;
+    const bool home_ok = hx < Nx && hy < Ny && hz < Nz;
+    if (t < 64) {
+        // staged cell q = (ox+1) + 4*((oy+1) + 4*(oz+1)), o* in -1..2 around (x0, y0, z0)
+        const int ox = (t & 3) - 1, oy = ((t >> 2) & 3) - 1, oz = (t >> 4) - 1;
+        int cx = x0 + ox, cy = y0 + oy, cz = z0 + oz;
+        cx = cx < 0 ? cx + Nx : (cx >= Nx ? cx - Nx : cx);
+        cy = cy < 0 ? cy + Ny : (cy >= Ny ? cy - Ny : cy);
+        cz = cz < 0 ? cz + Nz : (cz >= Nz ? cz - Nz : cz);
+        const int cc = cx + Nx * (cy + Ny * cz);
+        const int cn = d.cell_count[cc];
+        s_first[t] = d.cell_first[cc];
+        s_cnt[t] = cn;
+        // inclusive scan of the 64 counts across the wavefront
+        int v = cn;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int u = __shfl_up(v, o);
+            if (t >= o) v += u;
+        }
+        s_off[t + 1] = v;
+        if (t == 0) s_off[0] = 0;
+    }
+    if ((t & 31) == 0) {
+        const int hcell = home_ok ? hx + Nx * (hy + Ny * hz) : 0;
+        s_hfirst[hc] = home_ok ? d.cell_first[hcell] : 0;
+        s_hcnt[hc] = home_ok ? d.cell_count[hcell] : 0;
+    }
+    __syncthreads();
+    const int f0 = s_hfirst[hc], cnt_c = s_hcnt[hc];
+    int cnt_max = 0;
+    for (int i = 0; i < 8; i++) cnt_max = max(cnt_max, s_hcnt[i]);
+    if (cnt_max == 0) return;
+    if (d.flags[3]) {
+        for (int i = l; i < cnt_c; i += 32) d.ncount[d.cell_list[f0 + i]] = 0;
+        return;
+    }
+    // the word format of this block's lists (uniform; k_fast_force_tile makes
+    // the same choice from the same counts)
+    if (d.tile16 && s_off[64] <= NB_CAP)
+        mt_neigh_block_atoms<MODE, true>(d, p, spos, smol_buf, s_first, s_cnt, s_off, hxo, hyo, hzo, l, f0, cnt_c,
+                                         cnt_max);
+    else
+        mt_neigh_block_atoms<MODE, false>(d, p, spos, smol_buf, s_first, s_cnt, s_off, hxo, hyo, hzo, l, f0, cnt_c,
+                                          cnt_max);
 }
 
 #define NBW_CAP 1024
@@ -873,68 +922,180 @@ __global__ void __launch_bounds__(512) k_fast_force_mixed_lds(MtDev d, MtParams 
 // MIXED pair forces with neighbour-cell tiles in LDS. One workgroup per
 // 2x2x2 block of home cells (the list build's blocking). The positions (and
 // charges) of the 4x4x4 cells around the block, the union of the eight
-// 27-cell neighbourhoods, are staged once in LDS in cell order; the list
-// words carry an atom's place u in cell order, so the neighbour of a word
-// with offset code c sits in LDS at u + delta[h][c] for home cell h, where
-// delta maps the staged cell's first cell-order index to its LDS offset
-// (kept in .w of the per-home-cell shift table). Each pair then costs LDS
-// reads instead of a scattered 16-byte global load, whose L1 tag lookup (one
-// per lane) is what bounds the global-gather kernel. A workgroup whose staged
-// cells hold more than FT_CAP atoms gathers from global memory instead.
-// G lanes per home atom.
+// 27-cell neighbourhoods, are staged once in LDS in cell order, moved from
+// cell-centre coordinates into the frame of the centre of the block's first
+// cell (x0, y0, z0): the staged cells at offsets -1..2 are shifted by
+// -1..2 cell widths, products that are exact in FP32, so the separation of
+// a pair is one difference of staged positions with no per-pair shift. The
+// list words of such a block (k_fast_neigh_block, W16) are the neighbour's
+// LDS place itself, 16 bits, eight to a 16-byte chunk: half the list bytes
+// of the coded 32-bit words, and one dependent LDS read per pair. A
+// workgroup whose staged cells hold more than FT_CAP atoms has 32-bit coded
+// words and gathers from global memory instead (per-offset shifts). G lanes
+// per home atom.
 #define FT_CAP 2048
-#ifndef MT_TILE_DEPTH
-#define MT_TILE_DEPTH 1                               // list chunks in flight per lane
+#if FT_CAP != NB_CAP
+#error the force tiles and the list build must stage the same blocks
 #endif
-template <bool COUL, bool FROM_LDS>
-__device__ __forceinline__ void mt_tile_atom(const MtDev& d, const MtParams& p, int k, const float4& pk,
-                                             const float4* shd, const float4* spos, const float* sq,
-                                             const float2* row, float qk, double& Fx, double& Fy, double& Fz,
-                                             int q0, int qs)
+
+// four pairs of a lane, without a branch on the common path: a pair outside
+// rc (or the atom itself) still reads a segment and its force is selected
+// away, and a pair outside the table reads segment MT_NPTS - 1 of its row,
+// which is zero, so the four coefficient reads are issued together and
+// waited for once. The arithmetic per pair is mt_mixed_pair_o's. Pairs in
+// the band around rc come back as bits 0..3.
+template <bool WANT_E, bool COUL, bool SHIFT, typename CoefT>
+__device__ __forceinline__ uint32_t mt_tile_pairs4(const float4 (&a)[4], const float4 (&o)[4], const float (&qa)[4],
+                                                   const float4& pk, const CoefT* row, float lo, float hi, float x0,
+                                                   float invd, float qk, float& cx, float& cy, float& cz, float& U)
+{
+    float dx[4], dy[4], dz[4], rinv[4], r[4];
+    bool in[4];
+    int ix[4];
+    uint32_t band = 0u;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        dx[i] = SHIFT ? pk.x - a[i].x - o[i].x : pk.x - a[i].x;
+        dy[i] = SHIFT ? pk.y - a[i].y - o[i].y : pk.y - a[i].y;
+        dz[i] = SHIFT ? pk.z - a[i].z - o[i].z : pk.z - a[i].z;
+        const float r2 = dx[i] * dx[i] + dy[i] * dy[i] + dz[i] * dz[i];
+        in[i] = r2 < lo && r2 >= 1.0e-6f;
+        band |= (r2 >= lo && r2 < hi) ? 1u << i : 0u;
+        rinv[i] = __builtin_amdgcn_rsqf(fmaxf(r2, 1.0e-6f));
+        r[i] = r2 * rinv[i];
+        const int idx = (int)((r[i] - x0) * invd);
+        // the neighbour's type times MT_NPTS (in .w) is its offset in the row
+        ix[i] = __float_as_int(a[i].w) + (int)min((unsigned)idx, (unsigned)(MT_NPTS - 1));
+    }
+    CoefT c[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) c[i] = row[ix[i]];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        float fr = c[i].x + c[i].y * rinv[i];
+        float K = 0.0f;
+        if constexpr (COUL) {
+            K = MT_COULOMB_K * qk * qa[i];
+            fr += K * rinv[i] * rinv[i] * rinv[i];
+        }
+        fr = in[i] ? fr : 0.0f;
+        if constexpr (WANT_E) {
+            U += in[i] ? 0.5f * (c[i].z * r[i] + c[i].w) : 0.0f;
+            if constexpr (COUL) U += in[i] ? 0.5f * K * rinv[i] : 0.0f;
+        }
+        cx = fmaf(dx[i], fr, cx);
+        cy = fmaf(dy[i], fr, cy);
+        cz = fmaf(dz[i], fr, cz);
+    }
+    return band;
+}
+
+// a band pair of a 16-bit block list: the staged cell of LDS place l gives
+// the neighbour's cell-order place, the inside test is taken in FP64 with the
+// reference predicate (as mt_mixed_band), the force from the staged positions
+template <bool WANT_E, bool COUL, typename CoefT>
+__device__ __forceinline__ void mt_tile_band16(int l, const float4& a, float qa, const float4& pk, const CoefT* row,
+                                               float x0, float invd, float qk, float& cx, float& cy, float& cz,
+                                               float& U, const MtDev& d, const MtParams& p, int k,
+                                               const int* s_first, const int* s_off)
+{
+    int lo = 0, hi = 64;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (s_off[mid] <= l) lo = mid; else hi = mid;
+    }
+    const int u = s_first[lo] + (l - s_off[lo]);
+    const double4 rk4 = d.pos[k], ra4 = d.pos[d.cell_list[u]];
+    if (!(mt_dist2_pbc({rk4.x, rk4.y, rk4.z}, {ra4.x, ra4.y, ra4.z}, p.wnl) < (double)p.rc2f)) return;
+    const float dx = pk.x - a.x, dy = pk.y - a.y, dz = pk.z - a.z;
+    const float r2 = dx * dx + dy * dy + dz * dz;
+    mt_mixed_force<WANT_E, COUL>(dx, dy, dz, r2, a, qa, row, x0, invd, qk, cx, cy, cz, U);
+}
+
+// the pair loop of a home atom of a 16-bit block over chunks q0, q0 + qs, ...
+// (the next chunk's words in flight across the current chunk's pairs); the
+// pair forces of four pairs are summed in FP32, those sums in FP64
+template <bool WANT_E, bool COUL, typename CoefT>
+__device__ __forceinline__ void mt_tile_atom16(const MtDev& d, const MtParams& p, int k, const float4& pk,
+                                               const float4* spos, const float* sq, const int* s_first,
+                                               const int* s_off, const CoefT* row, float qk, double& Fx, double& Fy,
+                                               double& Fz, float& U, int q0, int qs)
+{
+    const int nch = (min(d.ncount[k], 2 * d.ncap) + 7) >> 3;
+    if (q0 >= nch) return;
+    const mt_u32x4* __restrict__ nl4 = reinterpret_cast<const mt_u32x4*>(d.nlist) + k;
+    const size_t stride = (size_t)d.n;
+    const float x0 = p.x0, invd = p.inv_delta;
+    const float lo = p.rc2f * (1.0f - 2.0e-5f), hi = p.rc2f * (1.0f + 2.0e-5f);
+    mt_u32x4 wn = __builtin_nontemporal_load(nl4 + (size_t)q0 * stride);
+    for (int q = q0; q < nch; q += qs) {
+        const mt_u32x4 w = wn;
+        const int li[8] = {(int)(w.x & 0xffffu), (int)(w.x >> 16), (int)(w.y & 0xffffu), (int)(w.y >> 16),
+                           (int)(w.z & 0xffffu), (int)(w.z >> 16), (int)(w.w & 0xffffu), (int)(w.w >> 16)};
+        float4 a[8];
+        float qa[8];
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            a[i] = spos[li[i]];
+            qa[i] = COUL ? sq[li[i]] : 0.0f;
+        }
+        // unconditional (the last iteration reloads a chunk of this atom's list)
+        wn = __builtin_nontemporal_load(nl4 + (size_t)min(q + qs, nch - 1) * stride);
+#pragma unroll
+        for (int hh = 0; hh < 2; hh++) {
+            const float4 ah[4] = {a[4 * hh], a[4 * hh + 1], a[4 * hh + 2], a[4 * hh + 3]};
+            const float qh[4] = {qa[4 * hh], qa[4 * hh + 1], qa[4 * hh + 2], qa[4 * hh + 3]};
+            float cx = 0.f, cy = 0.f, cz = 0.f;
+            const uint32_t band =
+                mt_tile_pairs4<WANT_E, COUL, false>(ah, ah, qh, pk, row, lo, hi, x0, invd, qk, cx, cy, cz, U);
+            if (band) {
+#pragma unroll
+                for (int i = 0; i < 4; i++)
+                    if (band & (1u << i))
+                        mt_tile_band16<WANT_E, COUL>(li[4 * hh + i], ah[i], qh[i], pk, row, x0, invd, qk, cx, cy, cz,
+                                                     U, d, p, k, s_first, s_off);
+            }
+            Fx += (double)cx;
+            Fy += (double)cy;
+            Fz += (double)cz;
+        }
+    }
+}
+
+// the same for a block over FT_CAP atoms: 32-bit coded words, positions and
+// charges gathered from global memory, the offset's shift from shd
+template <bool WANT_E, bool COUL, typename CoefT>
+__device__ __forceinline__ void mt_tile_atom32(const MtDev& d, const MtParams& p, int k, const float4& pk,
+                                               const float4* shd, const CoefT* row, float qk, double& Fx, double& Fy,
+                                               double& Fz, float& U, int q0, int qs)
 {
     const int nch = (min(d.ncount[k], d.ncap) + 3) >> 2;
     if (q0 >= nch) return;
     const mt_u32x4* __restrict__ nl4 = reinterpret_cast<const mt_u32x4*>(d.nlist) + k;
     const size_t stride = (size_t)d.n;
-    const float rc2 = p.rc2f, x0 = p.x0, invd = p.inv_delta;
-    float U = 0.0f;
-    mt_u32x4 wn[MT_TILE_DEPTH];
-#pragma unroll
-    for (int i = 0; i < MT_TILE_DEPTH; i++)
-        wn[i] = __builtin_nontemporal_load(nl4 + (size_t)min(q0 + i * qs, nch - 1) * stride);
+    const float x0 = p.x0, invd = p.inv_delta;
+    const float lo = p.rc2f * (1.0f - 2.0e-5f), hi = p.rc2f * (1.0f + 2.0e-5f);
+    mt_u32x4 wn = __builtin_nontemporal_load(nl4 + (size_t)q0 * stride);
     for (int q = q0; q < nch; q += qs) {
-        const mt_u32x4 w = wn[0];
-        const float4 o0 = shd[__builtin_amdgcn_ubfe(w.x, MT_NW_IDX_BITS, 5)];
-        const float4 o1 = shd[__builtin_amdgcn_ubfe(w.y, MT_NW_IDX_BITS, 5)];
-        const float4 o2 = shd[__builtin_amdgcn_ubfe(w.z, MT_NW_IDX_BITS, 5)];
-        const float4 o3 = shd[__builtin_amdgcn_ubfe(w.w, MT_NW_IDX_BITS, 5)];
-        const int i0 = (int)(w.x & MT_NW_IDX_MASK), i1 = (int)(w.y & MT_NW_IDX_MASK);
-        const int i2 = (int)(w.z & MT_NW_IDX_MASK), i3 = (int)(w.w & MT_NW_IDX_MASK);
-        float4 a0, a1, a2, a3;
-        float c0 = 0.f, c1 = 0.f, c2 = 0.f, c3 = 0.f;
-        if constexpr (FROM_LDS) {
-            const int l0 = i0 + __float_as_int(o0.w), l1 = i1 + __float_as_int(o1.w);
-            const int l2 = i2 + __float_as_int(o2.w), l3 = i3 + __float_as_int(o3.w);
-            a0 = spos[l0]; a1 = spos[l1]; a2 = spos[l2]; a3 = spos[l3];
-            if constexpr (COUL) { c0 = sq[l0]; c1 = sq[l1]; c2 = sq[l2]; c3 = sq[l3]; }
-        } else {
-            a0 = d.posf[i0]; a1 = d.posf[i1]; a2 = d.posf[i2]; a3 = d.posf[i3];
-            if constexpr (COUL) { c0 = d.qfs[i0]; c1 = d.qfs[i1]; c2 = d.qfs[i2]; c3 = d.qfs[i3]; }
-        }
-        // the next chunks' words stay in flight across the pair math
+        const mt_u32x4 w = wn;
+        const uint32_t wv[4] = {w.x, w.y, w.z, w.w};
+        float4 a[4], o[4];
+        float qa[4];
 #pragma unroll
-        for (int i = 0; i + 1 < MT_TILE_DEPTH; i++) wn[i] = wn[i + 1];
-        wn[MT_TILE_DEPTH - 1] = __builtin_nontemporal_load(nl4 + (size_t)min(q + MT_TILE_DEPTH * qs, nch - 1) * stride);
+        for (int i = 0; i < 4; i++) {
+            const int ia = (int)(wv[i] & MT_NW_IDX_MASK);
+            o[i] = shd[__builtin_amdgcn_ubfe(wv[i], MT_NW_IDX_BITS, 5)];
+            a[i] = d.posf[ia];
+            qa[i] = COUL ? d.qfs[ia] : 0.0f;
+        }
+        wn = __builtin_nontemporal_load(nl4 + (size_t)min(q + qs, nch - 1) * stride);
         float cx = 0.f, cy = 0.f, cz = 0.f;
-        const bool b0 = mt_mixed_pair_o<false, COUL>(a0, o0, c0, pk, row, rc2, x0, invd, qk, cx, cy, cz, U);
-        const bool b1 = mt_mixed_pair_o<false, COUL>(a1, o1, c1, pk, row, rc2, x0, invd, qk, cx, cy, cz, U);
-        const bool b2 = mt_mixed_pair_o<false, COUL>(a2, o2, c2, pk, row, rc2, x0, invd, qk, cx, cy, cz, U);
-        const bool b3 = mt_mixed_pair_o<false, COUL>(a3, o3, c3, pk, row, rc2, x0, invd, qk, cx, cy, cz, U);
-        if (b0 | b1 | b2 | b3) {
-            if (b0) mt_mixed_band<false, COUL>(w.x, pk, shd, row, x0, invd, qk, cx, cy, cz, U, d, p, k);
-            if (b1) mt_mixed_band<false, COUL>(w.y, pk, shd, row, x0, invd, qk, cx, cy, cz, U, d, p, k);
-            if (b2) mt_mixed_band<false, COUL>(w.z, pk, shd, row, x0, invd, qk, cx, cy, cz, U, d, p, k);
-            if (b3) mt_mixed_band<false, COUL>(w.w, pk, shd, row, x0, invd, qk, cx, cy, cz, U, d, p, k);
+        const uint32_t band = mt_tile_pairs4<WANT_E, COUL, true>(a, o, qa, pk, row, lo, hi, x0, invd, qk, cx, cy, cz, U);
+        if (band) {
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+                if (band & (1u << i))
+                    mt_mixed_band<WANT_E, COUL>(wv[i], pk, shd, row, x0, invd, qk, cx, cy, cz, U, d, p, k);
         }
         Fx += (double)cx;
         Fy += (double)cy;
@@ -942,13 +1103,83 @@ __device__ __forceinline__ void mt_tile_atom(const MtDev& d, const MtParams& p, 
     }
 }
 
-template <bool COUL, int G>
+// the staged cells of the block whose first cell is (x0, y0, z0): staged cell
+// q = (ox+1) + 4 (oy+1) + 16 (oz+1), o* in -1..2, has its first cell-order
+// place in s_first[q], its count in s_cnt[q] and its first staged place in
+// s_off[q] (s_off[64] = total); home cell h = (h&1, (h>>1)&1, h>>2) has
+// first place s_hfirst[h] and first home index s_hoff[h] (s_hoff[8] = home
+// atoms). Threads 0..63 write; the caller synchronises.
+__device__ __forceinline__ void mt_tile_cells(const MtDev& d, int x0, int y0, int z0, int* s_first, int* s_cnt,
+                                              int* s_off, int* s_hfirst, int* s_hoff)
+{
+    const int t = threadIdx.x;
+    if (t >= 64) return;
+    const int Nx = d.nc[0], Ny = d.nc[1], Nz = d.nc[2];
+    const int ox = (t & 3) - 1, oy = ((t >> 2) & 3) - 1, oz = (t >> 4) - 1;
+    int cx = x0 + ox, cy = y0 + oy, cz = z0 + oz;
+    cx = cx < 0 ? cx + Nx : (cx >= Nx ? cx - Nx : cx);
+    cy = cy < 0 ? cy + Ny : (cy >= Ny ? cy - Ny : cy);
+    cz = cz < 0 ? cz + Nz : (cz >= Nz ? cz - Nz : cz);
+    const int cc = cx + Nx * (cy + Ny * cz);
+    const int cn = d.cell_count[cc];
+    s_first[t] = d.cell_first[cc];
+    s_cnt[t] = cn;
+    int v = cn;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int u = __shfl_up(v, o);
+        if (t >= o) v += u;
+    }
+    s_off[t + 1] = v;
+    if (t == 0) s_off[0] = 0;
+    // home cells (the first eight lanes): counts scanned across them
+    int hn = 0, hf = 0;
+    if (t < 8) {
+        const int hx = x0 + (t & 1), hy = y0 + ((t >> 1) & 1), hz = z0 + (t >> 2);
+        if (hx < Nx && hy < Ny && hz < Nz) {
+            const int hcell = hx + Nx * (hy + Ny * hz);
+            hn = d.cell_count[hcell];
+            hf = d.cell_first[hcell];
+        }
+    }
+    int hv = hn;
+    for (int o = 1; o < 8; o <<= 1) {
+        const int u = __shfl_up(hv, o);
+        if (t >= o) hv += u;
+    }
+    if (t < 8) {
+        s_hfirst[t] = hf;
+        s_hoff[t + 1] = hv;
+        if (t == 0) s_hoff[0] = 0;
+    }
+}
+
+// the block of workgroup blockIdx.x (XCD-contiguous order) and its first cell
+__device__ __forceinline__ void mt_tile_block(const MtDev& d, int& x0, int& y0, int& z0)
+{
+    const int bx = (d.nc[0] + 1) >> 1, by = (d.nc[1] + 1) >> 1;
+    const int g = mt_xcd_block(blockIdx.x, gridDim.x);
+    x0 = 2 * (g % bx);
+    y0 = 2 * ((g / bx) % by);
+    z0 = 2 * (g / (bx * by));
+}
+
+// home atom v of a block: its home cell
+__device__ __forceinline__ int mt_tile_home(const int* s_hoff, int v)
+{
+    int h = 0;
+    while (h < 7 && v >= s_hoff[h + 1]) h++;
+    return h;
+}
+
+// WANT_E: per-atom energies too, with the (a_f, b_f, a_e, b_e) coefficients
+// read from global memory; otherwise (a_f, b_f) of every type pair staged in LDS
+template <bool WANT_E, bool COUL, int G>
 __global__ void __launch_bounds__(256 * G) k_fast_force_tile(MtDev d, MtParams p)
 {
     MT_RESET_DISP(d);
     __shared__ float4 spos[FT_CAP];
     __shared__ float sq_buf[COUL ? FT_CAP : 1];
-    __shared__ float4 shd[8 * 27];                    // per home cell: shift xyz, LDS delta (.w bits)
+    __shared__ float4 shd[27];                        // 32-bit blocks: shift of each offset code
     __shared__ int s_first[64], s_cnt[64], s_off[65];
     __shared__ int s_hfirst[8], s_hoff[9];
     const int t = threadIdx.x;
@@ -956,62 +1187,17 @@ __global__ void __launch_bounds__(256 * G) k_fast_force_tile(MtDev d, MtParams p
     // the segment coefficients in LDS: read from global memory instead (L1/L2
     // hits), the pass measured 0.39 ms instead of 0.23 ms on the 1M-atom mixture
     extern __shared__ float2 stab[];                  // T * T * MT_NPTS segment coefficients
-    const int ntab = T * T * MT_NPTS;
-    for (int i = t; i < ntab; i += blockDim.x) stab[i] = d.nbcoef2[i];
-    const int Nx = d.nc[0], Ny = d.nc[1], Nz = d.nc[2];
-    const int bx = (Nx + 1) >> 1, by = (Ny + 1) >> 1;
-    const int g = mt_xcd_block(blockIdx.x, gridDim.x);
-    const int x0 = 2 * (g % bx), y0 = 2 * ((g / bx) % by), z0 = 2 * (g / (bx * by));
-    if (t < 64) {
-        // staged cell q = (ox+1) + 4*((oy+1) + 4*(oz+1)), o* in -1..2 around (x0, y0, z0)
-        const int ox = (t & 3) - 1, oy = ((t >> 2) & 3) - 1, oz = (t >> 4) - 1;
-        int cx = x0 + ox, cy = y0 + oy, cz = z0 + oz;
-        cx = cx < 0 ? cx + Nx : (cx >= Nx ? cx - Nx : cx);
-        cy = cy < 0 ? cy + Ny : (cy >= Ny ? cy - Ny : cy);
-        cz = cz < 0 ? cz + Nz : (cz >= Nz ? cz - Nz : cz);
-        const int cc = cx + Nx * (cy + Ny * cz);
-        const int cn = d.cell_count[cc];
-        s_first[t] = d.cell_first[cc];
-        s_cnt[t] = cn;
-        int v = cn;
-        for (int o = 1; o < 64; o <<= 1) {
-            const int u = __shfl_up(v, o);
-            if (t >= o) v += u;
-        }
-        s_off[t + 1] = v;
-        if (t == 0) s_off[0] = 0;
-        // home cells (the first eight lanes): counts scanned across them
-        int hn = 0, hf = 0;
-        if (t < 8) {
-            const int hx = x0 + (t & 1), hy = y0 + ((t >> 1) & 1), hz = z0 + (t >> 2);
-            if (hx < Nx && hy < Ny && hz < Nz) {
-                const int hcell = hx + Nx * (hy + Ny * hz);
-                hn = d.cell_count[hcell];
-                hf = d.cell_first[hcell];
-            }
-        }
-        int hv = hn;
-        for (int o = 1; o < 8; o <<= 1) {
-            const int u = __shfl_up(hv, o);
-            if (t >= o) hv += u;
-        }
-        if (t < 8) {
-            s_hfirst[t] = hf;
-            s_hoff[t + 1] = hv;
-            if (t == 0) s_hoff[0] = 0;
-        }
+    if constexpr (!WANT_E) {
+        const int ntab = T * T * MT_NPTS;
+        for (int i = t; i < ntab; i += blockDim.x) stab[i] = d.nbcoef2[i];
     }
+    int x0, y0, z0;
+    mt_tile_block(d, x0, y0, z0);
+    mt_tile_cells(d, x0, y0, z0, s_first, s_cnt, s_off, s_hfirst, s_hoff);
     __syncthreads();
-    for (int e = t; e < 8 * 27; e += blockDim.x) {
-        const int h = e / 27, c = e % 27;
-        const int ox = c % 3 - 1, oy = (c / 3) % 3 - 1, oz = c / 9 - 1;
-        const int q = (ox + 1 + (h & 1)) + 4 * ((oy + 1 + ((h >> 1) & 1)) + 4 * (oz + 1 + (h >> 2)));
-        shd[e] = make_float4((float)ox * p.cs[0], (float)oy * p.cs[1], (float)oz * p.cs[2],
-                             __int_as_float(s_off[q] - s_first[q]));
-    }
     const int total = s_off[64];
-    const bool lds = total <= FT_CAP;                 // uniform over the workgroup
-    if (lds) {
+    const bool w16 = total <= FT_CAP;                 // uniform: the list build's choice for this block
+    if (w16) {
         for (int i = t; i < total; i += blockDim.x) {
             int lo = 0, hi = 64;                       // staged cell of item i
             while (hi - lo > 1) {
@@ -1019,29 +1205,49 @@ __global__ void __launch_bounds__(256 * G) k_fast_force_tile(MtDev d, MtParams p
                 if (s_off[mid] <= i) lo = mid; else hi = mid;
             }
             const int u = s_first[lo] + (i - s_off[lo]);
-            spos[i] = d.posf[u];
+            const float4 a = d.posf[u];
+            spos[i] = make_float4(a.x + (float)((lo & 3) - 1) * p.cs[0], a.y + (float)(((lo >> 2) & 3) - 1) * p.cs[1],
+                                  a.z + (float)((lo >> 4) - 1) * p.cs[2], a.w);
             if constexpr (COUL) sq_buf[i] = d.qfs[u];
         }
+    } else if (t < 27) {
+        shd[t] = make_float4((float)(t % 3 - 1) * p.cs[0], (float)((t / 3) % 3 - 1) * p.cs[1],
+                             (float)(t / 9 - 1) * p.cs[2], 0.0f);
     }
     __syncthreads();
     const int nh = s_hoff[8];
     const int gl = t % G;
     for (int v = t / G; v < nh; v += (int)blockDim.x / G) {
-        int h = 0;
-        while (h < 7 && v >= s_hoff[h + 1]) h++;
+        const int h = mt_tile_home(s_hoff, v);
         const int u = s_hfirst[h] + (v - s_hoff[h]);
         const int k = d.cell_list[u];
-        const float4 pk = d.posf[u];
-        const float2* __restrict__ row = stab + __float_as_int(pk.w) * T;
         const float qk = COUL ? d.qf[k] : 0.0f;
         double Fx = 0.0, Fy = 0.0, Fz = 0.0;
-        if (lds) mt_tile_atom<COUL, true>(d, p, k, pk, shd + 27 * h, spos, sq_buf, row, qk, Fx, Fy, Fz, gl, G);
-        else mt_tile_atom<COUL, false>(d, p, k, pk, shd + 27 * h, spos, sq_buf, row, qk, Fx, Fy, Fz, gl, G);
         float U = 0.0f;
+        if (w16) {
+            // the home atom's own staged entry (home cell h is staged cell qh)
+            const int qh = (1 + (h & 1)) + 4 * ((1 + ((h >> 1) & 1)) + 4 * (1 + (h >> 2)));
+            const float4 pk = spos[s_off[qh] + (v - s_hoff[h])];
+            if constexpr (WANT_E)
+                mt_tile_atom16<true, COUL>(d, p, k, pk, spos, sq_buf, s_first, s_off,
+                                           d.nbcoef + (size_t)__float_as_int(pk.w) * T, qk, Fx, Fy, Fz, U, gl, G);
+            else
+                mt_tile_atom16<false, COUL>(d, p, k, pk, spos, sq_buf, s_first, s_off,
+                                            stab + __float_as_int(pk.w) * T, qk, Fx, Fy, Fz, U, gl, G);
+        } else {
+            const float4 pk = d.posf[u];
+            if constexpr (WANT_E)
+                mt_tile_atom32<true, COUL>(d, p, k, pk, shd, d.nbcoef + (size_t)__float_as_int(pk.w) * T, qk, Fx, Fy,
+                                           Fz, U, gl, G);
+            else
+                mt_tile_atom32<false, COUL>(d, p, k, pk, shd, stab + __float_as_int(pk.w) * T, qk, Fx, Fy, Fz, U, gl,
+                                            G);
+        }
         mt_lane_sum<G>(Fx, Fy, Fz, U);
         if (gl == 0) {
-            if (!d.mobile[k]) { Fx = Fy = Fz = 0.0; }
+            if (!d.mobile[k]) { Fx = Fy = Fz = 0.0; U = 0.0f; }
             d.frc[k] = make_double4(Fx, Fy, Fz, 0.0);
+            if (WANT_E) d.epot[k] = U;
         }
     }
 }
@@ -1402,24 +1608,32 @@ static void mt_launch_fp64(const MtDev& d, const MtParams& p, int bs, hipStream_
 #ifndef MT_TILE_G
 #define MT_TILE_G 2
 #endif
-// the LDS-tiled MIXED kernel: force-only passes of systems with sparse cells
-// (the block list build's regime), the coefficient table within 32 KB of
-// LDS, and enough 2x2x2 cell blocks to fill the GPU
-static bool mt_use_tile(const MtDev& d, int mixed, int want_e, size_t lds_tab)
+// the LDS-tiled MIXED kernel (k_fast_force_tile and the block lists it reads):
+// sparse cells (the block list build's regime), the coefficient table within
+// 32 KB of LDS, and enough 2x2x2 cell blocks to fill the GPU
+int mt_fast_tile_mode(const MtDev& d, int mixed)
 {
-    if (!mixed || want_e || !d.wsorted || lds_tab > 32768 || d.n > 40 * d.ncells) return false;
+    const size_t lds_tab = (size_t)d.T * d.T * MT_NPTS * sizeof(float2);
+    if (!mixed || !d.wsorted || lds_tab > 32768 || d.n > 40 * d.ncells) return 0;
     const long long nb = (long long)((d.nc[0] + 1) / 2) * ((d.nc[1] + 1) / 2) * ((d.nc[2] + 1) / 2);
-    return nb >= 2LL * mt_cu_count();
+    return nb >= 2LL * mt_cu_count() ? 1 : 0;
 }
 
 template <int G>
 static void mt_launch_pairs(const MtDev& d, const MtParams& p, int mixed, int want_e, bool coul, int block,
                             size_t lds_tab, hipStream_t s)
 {
-    if (mt_use_tile(d, mixed, want_e, lds_tab)) {
+    if (mixed && d.tile16) {
+        // every MIXED pass of these lists, energies included
         const int nb = ((d.nc[0] + 1) / 2) * ((d.nc[1] + 1) / 2) * ((d.nc[2] + 1) / 2);
-        if (coul) hipLaunchKernelGGL((k_fast_force_tile<true, MT_TILE_G>), dim3(nb), dim3(256 * MT_TILE_G), lds_tab, s, d, p);
-        else hipLaunchKernelGGL((k_fast_force_tile<false, MT_TILE_G>), dim3(nb), dim3(256 * MT_TILE_G), lds_tab, s, d, p);
+        const dim3 grid(nb), tpb(256 * MT_TILE_G);
+        if (want_e) {
+            if (coul) hipLaunchKernelGGL((k_fast_force_tile<true, true, MT_TILE_G>), grid, tpb, 0, s, d, p);
+            else hipLaunchKernelGGL((k_fast_force_tile<true, false, MT_TILE_G>), grid, tpb, 0, s, d, p);
+        } else {
+            if (coul) hipLaunchKernelGGL((k_fast_force_tile<false, true, MT_TILE_G>), grid, tpb, lds_tab, s, d, p);
+            else hipLaunchKernelGGL((k_fast_force_tile<false, false, MT_TILE_G>), grid, tpb, lds_tab, s, d, p);
+        }
         return;
     }
     if (mixed && !want_e && lds_tab <= 65536) {
@@ -1608,9 +1822,36 @@ void mt_fast_kick_drift(const MtDev& d, const MtParams& p, int scale_slot, int m
 // lists hold plain atom indices at rc, the FP64/MIXED lists coded words at
 // rc + skin plus the special pairs.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(TPB) k_ucut(MtDev d, MtParams p, int exact)
+// the special pairs (1-2/1-3/1-4 and same-molecule pairs) of slot k inside rc
+__device__ __forceinline__ double mt_ucut_special(const MtDev& d, const MtParams& p, int k, const mt_d3& rk, int row)
+{
+    double u = 0.0;
+    if (d.sp.n_entries > 0) {
+        const int o = d.perm[k];
+        for (int j = 0; j < d.sp.num[o]; j++) {
+            const int a = d.inv[d.sp.partner[d.sp.first[o] + j]];
+            const double4 ra4 = d.pos[a];
+            if (mt_dist2_pbc(rk, {ra4.x, ra4.y, ra4.z}, p.wnl) < (double)p.rc2f)
+                u += 0.5 * (double)(d.ecut[row * d.T + d.type[a]] * d.sp.scale[d.sp.first[o] + j]);
+        }
+    }
+    return u;
+}
+
+__device__ __forceinline__ void mt_block_sum_to(double u, double* out)
 {
     __shared__ double sh[TPB];
+    sh[threadIdx.x] = u;
+    __syncthreads();
+    for (int off = TPB / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = sh[0];
+}
+
+__global__ void __launch_bounds__(TPB) k_ucut(MtDev d, MtParams p, int exact)
+{
     double u = 0.0;
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < d.n; k += gridDim.x * blockDim.x) {
         if (!d.mobile[k]) continue;
@@ -1631,28 +1872,70 @@ __global__ void __launch_bounds__(TPB) k_ucut(MtDev d, MtParams p, int exact)
             }
             u += 0.5 * (double)e;
         }
-        if (!exact && d.sp.n_entries > 0) {
-            const int o = d.perm[k];
-            for (int j = 0; j < d.sp.num[o]; j++) {
-                const int a = d.inv[d.sp.partner[d.sp.first[o] + j]];
-                const double4 ra4 = d.pos[a];
-                if (mt_dist2_pbc(rk, {ra4.x, ra4.y, ra4.z}, p.wnl) < (double)p.rc2f)
-                    u += 0.5 * (double)(d.ecut[row * T + d.type[a]] * d.sp.scale[d.sp.first[o] + j]);
-            }
-        }
+        if (!exact) u += mt_ucut_special(d, p, k, rk, row);
     }
-    sh[threadIdx.x] = u;
-    __syncthreads();
-    for (int off = TPB / 2; off > 0; off >>= 1) {
-        if (threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) d.red[blockIdx.x] = sh[0];
+    mt_block_sum_to(u, d.red + blockIdx.x);
 }
 
-void mt_ucut(const MtDev& d, const MtParams& p, int exact, hipStream_t s)
+// the same over the MIXED block lists (d.tile16): workgroups take the 2x2x2
+// blocks g = blockIdx.x, blockIdx.x + gridDim.x, ... and decode a 16-bit
+// word through the block's staged cells
+__global__ void __launch_bounds__(TPB) k_ucut_tile(MtDev d, MtParams p)
 {
+    __shared__ int s_first[64], s_cnt[64], s_off[65];
+    __shared__ int s_hfirst[8], s_hoff[9];
+    const int bx = (d.nc[0] + 1) >> 1, by = (d.nc[1] + 1) >> 1, bz = (d.nc[2] + 1) >> 1;
+    const int nb = bx * by * bz;
+    const uint16_t* __restrict__ nl16 = reinterpret_cast<const uint16_t*>(d.nlist);
+    double u = 0.0;
+    for (int g = blockIdx.x; g < nb; g += gridDim.x) {
+        __syncthreads();                              // the previous block's tables are read
+        mt_tile_cells(d, 2 * (g % bx), 2 * ((g / bx) % by), 2 * (g / (bx * by)), s_first, s_cnt, s_off, s_hfirst,
+                      s_hoff);
+        __syncthreads();
+        const bool w16 = s_off[64] <= FT_CAP;
+        const int nh = s_hoff[8];
+        for (int v = threadIdx.x; v < nh; v += blockDim.x) {
+            const int h = mt_tile_home(s_hoff, v);
+            const int k = d.cell_list[s_hfirst[h] + (v - s_hoff[h])];
+            if (!d.mobile[k]) continue;
+            const double4 rk4 = d.pos[k];
+            const mt_d3 rk = {rk4.x, rk4.y, rk4.z};
+            const int row = d.type[k];
+            const int cnt = min(d.ncount[k], w16 ? 2 * d.ncap : d.ncap);
+            for (int j = 0; j < cnt; j++) {
+                int a;
+                if (w16) {
+                    const int l = nl16[(((size_t)(j >> 3) * d.n + k) << 3) + (j & 7)];
+                    int lo = 0, hi = 64;
+                    while (hi - lo > 1) {
+                        const int mid = (lo + hi) >> 1;
+                        if (s_off[mid] <= l) lo = mid; else hi = mid;
+                    }
+                    a = d.cell_list[s_first[lo] + (l - s_off[lo])];
+                } else {
+                    a = mt_id_slot(d, (int)(mt_fast_word(d.nlist, d.n, k, j) & MT_NW_IDX_MASK));
+                }
+                const double4 ra4 = d.pos[a];
+                if (mt_dist2_pbc(rk, {ra4.x, ra4.y, ra4.z}, p.wnl) < (double)p.rc2f)
+                    u += 0.5 * (double)d.ecut[row * d.T + d.type[a]];
+            }
+            u += mt_ucut_special(d, p, k, rk, row);
+        }
+    }
+    mt_block_sum_to(u, d.red + blockIdx.x);
+}
+
+int mt_ucut(const MtDev& d, const MtParams& p, int exact, hipStream_t s)
+{
+    if (!exact && d.tile16) {
+        const int nb = ((d.nc[0] + 1) / 2) * ((d.nc[1] + 1) / 2) * ((d.nc[2] + 1) / 2);
+        const int g = std::max(1, std::min(nb, mt_reduce_blocks(d.n)));
+        hipLaunchKernelGGL(k_ucut_tile, dim3(g), dim3(TPB), 0, s, d, p);
+        return g;
+    }
     hipLaunchKernelGGL(k_ucut, dim3(mt_reduce_blocks(d.n)), dim3(TPB), 0, s, d, p, exact);
+    return mt_reduce_blocks(d.n);
 }
 
 __global__ void k_fast_clear(unsigned int* flags) { flags[0] = 0u; }

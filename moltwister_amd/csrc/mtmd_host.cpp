@@ -727,6 +727,7 @@ static int setup(mtmd_handle* h, const mtmd_system* s)
         d.sidx = h->mem.alloc<int>(n);
         d.qfs = h->mem.alloc<float>(n);
         d.wsorted = (c.engine == MTMD_ENGINE_MIXED) ? 1 : 0;
+        d.tile16 = mt_fast_tile_mode(d, c.engine == MTMD_ENGINE_MIXED);
         d.qf = h->mem.alloc<float>(n);
         d.qf2 = h->mem.alloc<float>(n);
         if (p.coulomb_on && p.coulomb_allpairs) {
@@ -1048,6 +1049,7 @@ static int npt_regrid(mtmd_handle* h)
     }
     if (changed) {
         d.ncells = d.nc[0] * d.nc[1] * d.nc[2];
+        d.tile16 = mt_fast_tile_mode(d, h->engine == MTMD_ENGINE_MIXED);   // lists of the new grid
         set_box(h, h->p.L);                          // cell sizes of the new grid
         h->drop_graphs();
     }
@@ -1583,11 +1585,11 @@ extern "C" int mtmd_get_thermo(mtmd_handle* h, mtmd_thermo* out)
     out->Etot = su + 0.5 * sp;
     // U_cut over the lists of the last force evaluation (EXACT: rebuilt at rc
     // for those positions; FP64/MIXED: the skin list, valid until the next step)
-    mt_ucut(d, h->p, h->fast ? 0 : 1, h->stream);
-    HIPCHK(h, hipMemcpyAsync(part.data(), d.red, nb * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    const int nbu = mt_ucut(d, h->p, h->fast ? 0 : 1, h->stream);
+    HIPCHK(h, hipMemcpyAsync(part.data(), d.red, nbu * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
     double uc = 0;
-    for (int b = 0; b < nb; b++) uc += part[b];
+    for (int b = 0; b < nbu; b++) uc += part[b];
     out->U_cut = uc;
     return MTMD_OK;
 }

@@ -74,6 +74,11 @@ struct MtDev {
     int* cell_bsum;        // FP64/MIXED: per-block sums of the cell-count scan
     // neighbours
     int ncap; int* ncount; uint32_t* nlist; int* nreq;
+    // MIXED block lists (mt_fast_tile_mode): the lists of every 2x2x2 block of
+    // home cells whose 4x4x4 staged cells hold at most 2048 atoms are 16-bit
+    // words, the neighbour's place in those staged cells (8 words a chunk,
+    // 2 ncap words a row); the other blocks keep the 32-bit coded words
+    int tile16;
     // sorting (FP64/MIXED): perm[s] = original index of sorted slot s, inv[o] = slot
     int* perm; int* inv;
     int* perm2;
@@ -145,6 +150,10 @@ void mt_fast_sort_apply(const MtDev& d, hipStream_t s);   // writes the *2 buffe
 void mt_fast_unsort(const MtDev& d, hipStream_t s);       // writes the *2 buffers in original order
 void mt_fast_identity_perm(const MtDev& d, hipStream_t s);
 void mt_fast_neighbors(const MtDev& d, const MtParams& p, int conditional, hipStream_t s);
+// 1 when the MIXED pair forces run on 2x2x2 cell blocks staged in LDS (sparse
+// cells, the coefficient table within 32 KB, enough blocks to fill the GPU);
+// the host stores it in d.tile16 whenever the cell grid is set, before a build
+int mt_fast_tile_mode(const MtDev& d, int mixed);
 void mt_fast_forces(const MtDev& d, const MtParams& p, int mixed, int want_energy, int block, hipStream_t s);
 // lanes per atom the pair-force kernels use for this system (1, 4 or 16)
 int mt_fast_lanes(const MtDev& d);
@@ -157,7 +166,7 @@ void mt_fast_locate(const MtDev& d, const MtParams& p, int mixed, hipStream_t s)
 void mt_fast_nve_fused(const MtDev& d, const MtParams& p, int mixed, hipStream_t s);
 // in-place read + write of the whole neighbour-list buffer (HBM reference)
 void mt_copy_lists(const MtDev& d, hipStream_t s);
-// per-block partial sums of U_cut into d.red[0 .. mt_reduce_blocks(n))
-void mt_ucut(const MtDev& d, const MtParams& p, int exact, hipStream_t s);
+// per-block partial sums of U_cut into d.red[0 .. returned count)
+int mt_ucut(const MtDev& d, const MtParams& p, int exact, hipStream_t s);
 
 int mt_reduce_blocks(int n);

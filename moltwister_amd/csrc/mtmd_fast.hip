@@ -337,40 +337,46 @@ __device__ inline bool mt_is_partner(const MtDev& d, int k, int ia)
 // slot of a list-word index (the index itself unless the words are in cell order)
 __device__ __forceinline__ int mt_id_slot(const MtDev& d, int id) { return d.wsorted ? d.cell_list[id] : id; }
 
-// k: the home atom's slot (its list column); self: its own word index; e: the
-// candidate's place in the staged cells. 32-bit words: the candidate's word
-// index | offset code, four to a chunk (b0..b2 hold the chunk's first three).
-// W16: e itself, eight to a chunk (pend holds an even word until its pair
-// is complete, b0..b2 the chunk's first three pairs).
+// the chunk being filled is a window of the last words written, r0..r3 (the
+// oldest lowest): a word enters at the top and the window moves down one
+// word, so after a chunk's last word the window is the chunk itself
+template <bool W16>
+__device__ __forceinline__ void mt_neigh_push(uint32_t w, uint32_t& r0, uint32_t& r1, uint32_t& r2, uint32_t& r3)
+{
+    if constexpr (W16) {
+        // eight 16-bit words: ((hi:lo) >> 16) moves each register down one word
+        r0 = __builtin_amdgcn_alignbit(r1, r0, 16);
+        r1 = __builtin_amdgcn_alignbit(r2, r1, 16);
+        r2 = __builtin_amdgcn_alignbit(r3, r2, 16);
+        r3 = __builtin_amdgcn_alignbit(w, r3, 16);
+    } else {
+        r0 = r1;
+        r1 = r2;
+        r2 = r3;
+        r3 = w;
+    }
+}
+
+// a candidate inside rlist that is not the atom itself. k: the home atom's
+// slot (its list column); e: the candidate's place in the staged cells.
+// 32-bit words: the candidate's word index | offset code, four to a chunk.
+// W16: e itself, eight to a chunk.
 template <int MODE, bool W16>
-__device__ __forceinline__ void mt_neigh_hit(const MtDev& d, uint4* nl4, float4 c, uint32_t code, int k, int self,
-                                             int molk, const int* smol, int e, int n, int cap, int& cnt,
-                                             uint32_t& b0, uint32_t& b1, uint32_t& b2, uint32_t& pend)
+__device__ __forceinline__ void mt_neigh_hit(const MtDev& d, uint4* nl4, float4 c, uint32_t code, int k, int molk,
+                                             const int* smol, int e, int n, int cap, int& cnt, uint32_t& r0,
+                                             uint32_t& r1, uint32_t& r2, uint32_t& r3)
 {
     const int a = (int)__float_as_uint(c.w);
-    if (a == self) return;
     if constexpr (MODE >= 1) {
         if (smol[e] == molk) return;
     }
     if constexpr (MODE == 2) {
         if (mt_is_partner(d, k, mt_id_slot(d, a))) return;
     }
-    if constexpr (W16) {
-        const int slot = cnt & 7;
-        const uint32_t pair = pend | ((uint32_t)e << 16);
-        pend = (slot & 1) ? pend : (uint32_t)e;
-        b0 = slot == 1 ? pair : b0;
-        b1 = slot == 3 ? pair : b1;
-        b2 = slot == 5 ? pair : b2;
-        if (slot == 7 && cnt < cap) nl4[(size_t)(cnt >> 3) * n + k] = make_uint4(b0, b1, b2, pair);
-    } else {
-        const uint32_t w = (uint32_t)a | code;
-        const int slot = cnt & 3;
-        b0 = slot == 0 ? w : b0;
-        b1 = slot == 1 ? w : b1;
-        b2 = slot == 2 ? w : b2;
-        if (slot == 3 && cnt < cap) nl4[(size_t)(cnt >> 2) * n + k] = make_uint4(b0, b1, b2, w);
-    }
+    constexpr int lg = W16 ? 3 : 2;                   // log2 of the words a chunk
+    mt_neigh_push<W16>(W16 ? (uint32_t)e : ((uint32_t)a | code), r0, r1, r2, r3);
+    if ((cnt & ((1 << lg) - 1)) == (1 << lg) - 1 && cnt < cap)
+        nl4[(size_t)(cnt >> lg) * n + k] = make_uint4(r0, r1, r2, r3);
     cnt++;
 }
 
@@ -399,7 +405,7 @@ __device__ __forceinline__ void mt_neigh_block_atoms(const MtDev& d, const MtPar
         // MODE 2: only atoms with a bonded partner outside their molecule search
         const bool xk = MODE == 2 && act && d.xmol[d.perm[k]] != 0;
         int cnt = 0;
-        uint32_t b0 = 0, b1 = 0, b2 = 0, pend = 0;
+        uint32_t r0 = 0, r1 = 0, r2 = 0, r3 = 0;
         int cb = 0;
         while (cb < 64) {
             // batch of whole staged cells [cb, ce) that fits NB_CAP (uniform;
@@ -442,53 +448,50 @@ __device__ __forceinline__ void mt_neigh_block_atoms(const MtDev& d, const MtPar
                             const float qy = pk.y - (float)oy * p.cs[1];
                             const float qz = pk.z - (float)oz * p.cs[2];
                             const int e0 = s_off[q] - base, e1 = e0 + s_cnt[q];
+                            // four candidates a pass: their LDS reads are issued
+                            // together, and the hit test (inside rlist, not the
+                            // atom itself) is one predicate per candidate
                             int e = e0;
-                            for (; e + 1 < e1; e += 2) {
-                                float4 c0 = spos[e], c1 = spos[e + 1];
-                                asm volatile("" : "+v"(c0.w), "+v"(c1.w));
-                                const float dx0 = qx - c0.x, dy0 = qy - c0.y, dz0 = qz - c0.z;
-                                const float dx1 = qx - c1.x, dy1 = qy - c1.y, dz1 = qz - c1.z;
-                                const bool h0 = dx0 * dx0 + dy0 * dy0 + dz0 * dz0 < rl2;
-                                const bool h1 = dx1 * dx1 + dy1 * dy1 + dz1 * dz1 < rl2;
-                                if (h0 && (MODE != 2 || !xk || !mt_is_partner(d, k, mt_id_slot(d, (int)__float_as_uint(c0.w)))))
-                                    mt_neigh_hit<MODE == 2 ? 1 : MODE, W16>(d, nl4, c0, code, k, self, molk, smol, e, n,
-                                                                              cap, cnt, b0, b1, b2, pend);
-                                if (h1 && (MODE != 2 || !xk || !mt_is_partner(d, k, mt_id_slot(d, (int)__float_as_uint(c1.w)))))
-                                    mt_neigh_hit<MODE == 2 ? 1 : MODE, W16>(d, nl4, c1, code, k, self, molk, smol, e + 1,
-                                                                              n, cap, cnt, b0, b1, b2, pend);
+                            for (; e + 3 < e1; e += 4) {
+                                float4 c[4];
+#pragma unroll
+                                for (int i = 0; i < 4; i++) c[i] = spos[e + i];
+                                bool hv[4];
+#pragma unroll
+                                for (int i = 0; i < 4; i++) {
+                                    const float dx = qx - c[i].x, dy = qy - c[i].y, dz = qz - c[i].z;
+                                    hv[i] = dx * dx + dy * dy + dz * dz < rl2 && __float_as_int(c[i].w) != self;
+                                }
+#pragma unroll
+                                for (int i = 0; i < 4; i++)
+                                    if (hv[i] && (MODE != 2 || !xk ||
+                                                  !mt_is_partner(d, k, mt_id_slot(d, __float_as_int(c[i].w)))))
+                                        mt_neigh_hit<MODE == 2 ? 1 : MODE, W16>(d, nl4, c[i], code, k, molk, smol,
+                                                                                  e + i, n, cap, cnt, r0, r1, r2,
+                                                                                  r3);
                             }
-                            if (e < e1) {
-                                float4 c0 = spos[e];
-                                asm volatile("" : "+v"(c0.w));
-                                const float dx0 = qx - c0.x, dy0 = qy - c0.y, dz0 = qz - c0.z;
-                                if (dx0 * dx0 + dy0 * dy0 + dz0 * dz0 < rl2 &&
-                                    (MODE != 2 || !xk || !mt_is_partner(d, k, mt_id_slot(d, (int)__float_as_uint(c0.w)))))
-                                    mt_neigh_hit<MODE == 2 ? 1 : MODE, W16>(d, nl4, c0, code, k, self, molk, smol, e, n,
-                                                                              cap, cnt, b0, b1, b2, pend);
+                            for (; e < e1; e++) {
+                                const float4 c0 = spos[e];
+                                const float dx = qx - c0.x, dy = qy - c0.y, dz = qz - c0.z;
+                                if (dx * dx + dy * dy + dz * dz < rl2 && __float_as_int(c0.w) != self &&
+                                    (MODE != 2 || !xk || !mt_is_partner(d, k, mt_id_slot(d, __float_as_int(c0.w)))))
+                                    mt_neigh_hit<MODE == 2 ? 1 : MODE, W16>(d, nl4, c0, code, k, molk, smol, e, n,
+                                                                              cap, cnt, r0, r1, r2, r3);
                             }
                         }
             }
             cb = ce;
         }
         if (act) {
-            // last partial chunk, padded with the atom itself
-            if constexpr (W16) {
-                const int slot = cnt & 7;
-                if (slot && cnt < cap) {
-                    const uint32_t pad = (uint32_t)(s_off[qh] + ib + l), pp = pad | (pad << 16);
-                    // the pair holding word `slot`: its even word (if filled) and a pad
-                    const int ri = slot >> 1;
-                    const uint32_t last = (slot & 1) ? (pend | (pad << 16)) : pp;
-                    nl4[(size_t)(cnt >> 3) * n + k] =
-                        make_uint4(ri > 0 ? b0 : last, ri > 1 ? b1 : (ri == 1 ? last : pp),
-                                   ri > 2 ? b2 : (ri == 2 ? last : pp), ri == 3 ? last : pp);
-                }
-            } else {
-                const int slot = cnt & 3;
-                if (slot && cnt < cap) {
-                    const uint32_t pad = (uint32_t)self | (NB_PAD_CODE << MT_NW_IDX_BITS);
-                    nl4[(size_t)(cnt >> 2) * n + k] = make_uint4(b0, slot > 1 ? b1 : pad, slot > 2 ? b2 : pad, pad);
-                }
+            // last partial chunk, padded with the atom itself (W16: its own
+            // staged place; 32-bit: its word with the offset code of (0, 0, 0))
+            constexpr int lg = W16 ? 3 : 2;
+            const int slot = cnt & ((1 << lg) - 1);
+            if (slot && cnt < cap) {
+                const uint32_t pad = W16 ? (uint32_t)(s_off[qh] + ib + l)
+                                         : ((uint32_t)self | (NB_PAD_CODE << MT_NW_IDX_BITS));
+                for (int i = slot; i < (1 << lg); i++) mt_neigh_push<W16>(pad, r0, r1, r2, r3);
+                nl4[(size_t)(cnt >> lg) * n + k] = make_uint4(r0, r1, r2, r3);
             }
             d.ncount[k] = cnt;
             // the capacity request is in 32-bit words

@@ -316,13 +316,15 @@ __device__ inline bool mt_is_partner(const MtDev& d, int k, int ia)
 // Neighbour list at rlist. One 256-thread workgroup builds the lists of a
 // 2x2x2 block of home cells, one cell per half-wave (a cell holds ~26 atoms at
 // rho* = 1.2 with cells of cutoff + skin). The 4x4x4 cells around the block
-// (the union of the eight 27-cell neighbourhoods) are staged once in LDS as
+// (the union of the eight 27-cell neighbourhoods) are staged in LDS as
 // (x, y, z, word index) in cell order; every lane (one atom of a home cell)
-// then scans the 27 staged cells of its own neighbourhood with broadcast
-// reads, collecting hits as bit masks (mt_scan_cell). Each lane keeps its last
-// three hits in registers and writes every fourth as one 16-byte chunk of the
-// chunked column-major layout. When the 64 staged cells hold more than NB_CAP atoms
-// they are staged in batches of whole cells.
+// then scans the staged cells of its own neighbourhood with broadcast reads,
+// four candidates at a time, and writes its hits as 16-byte chunks of the
+// chunked column-major layout. In the MIXED tile mode a block whose staged
+// cells fit NB_CAP gets 16-bit words (the neighbour's staged place, eight a
+// chunk, mt_neigh_block_atoms16); every other block gets 32-bit words (word
+// index | offset code, four a chunk, mt_neigh_block_atoms), its cells staged
+// in batches of whole cells when they hold more than NB_CAP atoms.
 //
 // MODE 0: no molecules. MODE 1: same-molecule pairs are left out (molecule
 // ids staged in LDS beside the positions). MODE 2: as 1, and bonded partners
@@ -380,20 +382,18 @@ __device__ __forceinline__ void mt_neigh_hit(const MtDev& d, uint4* nl4, float4 
     cnt++;
 }
 
-// the lists of one workgroup's home atoms (the scan below the staging set-up
-// of k_fast_neigh_block); W16 is uniform over the workgroup
-template <int MODE, bool W16>
+// the 32-bit lists of one workgroup's home atoms (the scan below the staging
+// set-up of k_fast_neigh_block), staging the 64 cells in batches that fit NB_CAP
+template <int MODE>
 __device__ __forceinline__ void mt_neigh_block_atoms(const MtDev& d, const MtParams& p, float4* spos, int* smol,
                                                      const int* s_first, const int* s_cnt, const int* s_off, int hxo,
                                                      int hyo, int hzo, int l, int f0, int cnt_c, int cnt_max)
 {
     const int t = threadIdx.x;
     const int n = d.n;
-    const int cap = W16 ? 2 * d.ncap : d.ncap;          // words a row holds
+    const int cap = d.ncap;                           // words a row holds
     const float rl2 = p.rlist2;
     uint4* __restrict__ nl4 = reinterpret_cast<uint4*>(d.nlist);
-    // the home cell's place among the staged cells (W16 pads: the atom itself)
-    const int qh = (hxo + 1) + 4 * ((hyo + 1) + 4 * (hzo + 1));
     int staged_b = -1, staged_e = -1;
     for (int ib = 0; ib < cnt_max; ib += 32) {
         const bool act = ib + l < cnt_c;
@@ -408,8 +408,7 @@ __device__ __forceinline__ void mt_neigh_block_atoms(const MtDev& d, const MtPar
         uint32_t r0 = 0, r1 = 0, r2 = 0, r3 = 0;
         int cb = 0;
         while (cb < 64) {
-            // batch of whole staged cells [cb, ce) that fits NB_CAP (uniform;
-            // a W16 block is one batch)
+            // batch of whole staged cells [cb, ce) that fits NB_CAP (uniform)
             int ce = cb + 1;
             while (ce < 64 && s_off[ce + 1] - s_off[cb] <= NB_CAP) ce++;
             const int base = s_off[cb];
@@ -466,7 +465,7 @@ __device__ __forceinline__ void mt_neigh_block_atoms(const MtDev& d, const MtPar
                                 for (int i = 0; i < 4; i++)
                                     if (hv[i] && (MODE != 2 || !xk ||
                                                   !mt_is_partner(d, k, mt_id_slot(d, __float_as_int(c[i].w)))))
-                                        mt_neigh_hit<MODE == 2 ? 1 : MODE, W16>(d, nl4, c[i], code, k, molk, smol,
+                                        mt_neigh_hit<MODE == 2 ? 1 : MODE, false>(d, nl4, c[i], code, k, molk, smol,
                                                                                   e + i, n, cap, cnt, r0, r1, r2,
                                                                                   r3);
                             }
@@ -475,7 +474,7 @@ __device__ __forceinline__ void mt_neigh_block_atoms(const MtDev& d, const MtPar
                                 const float dx = qx - c0.x, dy = qy - c0.y, dz = qz - c0.z;
                                 if (dx * dx + dy * dy + dz * dz < rl2 && __float_as_int(c0.w) != self &&
                                     (MODE != 2 || !xk || !mt_is_partner(d, k, mt_id_slot(d, __float_as_int(c0.w)))))
-                                    mt_neigh_hit<MODE == 2 ? 1 : MODE, W16>(d, nl4, c0, code, k, molk, smol, e, n,
+                                    mt_neigh_hit<MODE == 2 ? 1 : MODE, false>(d, nl4, c0, code, k, molk, smol, e, n,
                                                                               cap, cnt, r0, r1, r2, r3);
                             }
                         }
@@ -483,20 +482,105 @@ __device__ __forceinline__ void mt_neigh_block_atoms(const MtDev& d, const MtPar
             cb = ce;
         }
         if (act) {
-            // last partial chunk, padded with the atom itself (W16: its own
-            // staged place; 32-bit: its word with the offset code of (0, 0, 0))
-            constexpr int lg = W16 ? 3 : 2;
-            const int slot = cnt & ((1 << lg) - 1);
+            // last partial chunk, padded with the atom itself (its word with
+            // the offset code of (0, 0, 0))
+            const int slot = cnt & 3;
             if (slot && cnt < cap) {
-                const uint32_t pad = W16 ? (uint32_t)(s_off[qh] + ib + l)
-                                         : ((uint32_t)self | (NB_PAD_CODE << MT_NW_IDX_BITS));
-                for (int i = slot; i < (1 << lg); i++) mt_neigh_push<W16>(pad, r0, r1, r2, r3);
-                nl4[(size_t)(cnt >> lg) * n + k] = make_uint4(r0, r1, r2, r3);
+                const uint32_t pad = (uint32_t)self | (NB_PAD_CODE << MT_NW_IDX_BITS);
+                for (int i = slot; i < 4; i++) mt_neigh_push<false>(pad, r0, r1, r2, r3);
+                nl4[(size_t)(cnt >> 2) * n + k] = make_uint4(r0, r1, r2, r3);
             }
             d.ncount[k] = cnt;
-            // the capacity request is in 32-bit words
-            if (cnt > cap) { d.flags[2] = 1; atomicMax(d.nreq, W16 ? (cnt + 1) >> 1 : cnt); }
+            if (cnt > cap) { d.flags[2] = 1; atomicMax(d.nreq, cnt); }
         }
+    }
+}
+
+// the 16-bit lists of a block whose 64 staged cells fit NB_CAP: the cells are
+// staged once, in the frame of the centre of the block's first cell (as
+// k_fast_force_tile stages them: cell offsets times cell widths, exact in
+// FP32), so the three cells ox = -1..1 of a (oy, oz) row of a home cell's
+// neighbourhood are one contiguous run of staged atoms with no per-cell
+// shift: nine runs a home atom.
+template <int MODE>
+__device__ __forceinline__ void mt_neigh_block_atoms16(const MtDev& d, const MtParams& p, float4* spos, int* smol,
+                                                       const int* s_first, const int* s_off, int hxo, int hyo,
+                                                       int hzo, int l, int f0, int cnt_c, int cnt_max)
+{
+    const int t = threadIdx.x;
+    const int n = d.n;
+    const int cap = 2 * d.ncap;                       // 16-bit words a row holds
+    const float rl2 = p.rlist2;
+    uint4* __restrict__ nl4 = reinterpret_cast<uint4*>(d.nlist);
+    const int total = s_off[64];
+    for (int i = t; i < total; i += 256) {
+        int lo = 0, hi = 64;                          // staged cell of item i
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (s_off[mid] <= i) lo = mid; else hi = mid;
+        }
+        const int ui = s_first[lo] + (i - s_off[lo]);
+        const float4 pa = d.posf[ui];
+        const int a = (MODE >= 1 || !d.wsorted) ? d.cell_list[ui] : ui;
+        spos[i] = make_float4(pa.x + (float)((lo & 3) - 1) * p.cs[0], pa.y + (float)(((lo >> 2) & 3) - 1) * p.cs[1],
+                              pa.z + (float)((lo >> 4) - 1) * p.cs[2], __uint_as_float((uint32_t)(d.wsorted ? ui : a)));
+        if constexpr (MODE >= 1) smol[i] = d.mol[a];
+    }
+    __syncthreads();
+    // the home cell's place among the staged cells
+    const int qh = (hxo + 1) + 4 * ((hyo + 1) + 4 * (hzo + 1));
+    for (int ib = 0; ib < cnt_max; ib += 32) {
+        const bool act = ib + l < cnt_c;
+        if (!act) continue;
+        const int uk = f0 + ib + l;                   // place in cell order
+        const int k = d.cell_list[uk];
+        const int self = d.wsorted ? uk : k;
+        const int lself = s_off[qh] + ib + l;         // own staged place (the pad word)
+        const float4 pk = spos[lself];
+        const int molk = MODE >= 1 ? d.mol[k] : -1;
+        const bool xk = MODE == 2 && d.xmol[d.perm[k]] != 0;
+        int cnt = 0;
+        uint32_t r0 = 0, r1 = 0, r2 = 0, r3 = 0;
+        for (int oz = 0; oz < 3; oz++)
+            for (int oy = 0; oy < 3; oy++) {
+                const int q = hxo + 4 * ((oy + hyo) + 4 * (oz + hzo));     // the row's ox = -1 cell
+                const int e1 = s_off[q + 3];
+                int e = s_off[q];
+                for (; e + 3 < e1; e += 4) {
+                    float4 c[4];
+#pragma unroll
+                    for (int i = 0; i < 4; i++) c[i] = spos[e + i];
+                    bool hv[4];
+#pragma unroll
+                    for (int i = 0; i < 4; i++) {
+                        const float dx = pk.x - c[i].x, dy = pk.y - c[i].y, dz = pk.z - c[i].z;
+                        hv[i] = dx * dx + dy * dy + dz * dz < rl2 && __float_as_int(c[i].w) != self;
+                    }
+#pragma unroll
+                    for (int i = 0; i < 4; i++)
+                        if (hv[i] &&
+                            (MODE != 2 || !xk || !mt_is_partner(d, k, mt_id_slot(d, __float_as_int(c[i].w)))))
+                            mt_neigh_hit<MODE == 2 ? 1 : MODE, true>(d, nl4, c[i], 0u, k, molk, smol, e + i, n, cap,
+                                                                      cnt, r0, r1, r2, r3);
+                }
+                for (; e < e1; e++) {
+                    const float4 c0 = spos[e];
+                    const float dx = pk.x - c0.x, dy = pk.y - c0.y, dz = pk.z - c0.z;
+                    if (dx * dx + dy * dy + dz * dz < rl2 && __float_as_int(c0.w) != self &&
+                        (MODE != 2 || !xk || !mt_is_partner(d, k, mt_id_slot(d, __float_as_int(c0.w)))))
+                        mt_neigh_hit<MODE == 2 ? 1 : MODE, true>(d, nl4, c0, 0u, k, molk, smol, e, n, cap, cnt, r0,
+                                                                  r1, r2, r3);
+                }
+            }
+        // last partial chunk, padded with the atom's own staged place
+        const int slot = cnt & 7;
+        if (slot && cnt < cap) {
+            for (int i = slot; i < 8; i++) mt_neigh_push<true>((uint32_t)lself, r0, r1, r2, r3);
+            nl4[(size_t)(cnt >> 3) * n + k] = make_uint4(r0, r1, r2, r3);
+        }
+        d.ncount[k] = cnt;
+        // the capacity request is in 32-bit words
+        if (cnt > cap) { d.flags[2] = 1; atomicMax(d.nreq, (cnt + 1) >> 1); }
     }
 }
 
@@ -555,11 +639,9 @@ __global__ void __launch_bounds__(256) k_fast_neigh_block(MtDev d, MtParams p, i
     // the word format of this block's lists (uniform; k_fast_force_tile makes
     // the same choice from the same counts)
     if (d.tile16 && s_off[64] <= NB_CAP)
-        mt_neigh_block_atoms<MODE, true>(d, p, spos, smol_buf, s_first, s_cnt, s_off, hxo, hyo, hzo, l, f0, cnt_c,
-                                         cnt_max);
+        mt_neigh_block_atoms16<MODE>(d, p, spos, smol_buf, s_first, s_off, hxo, hyo, hzo, l, f0, cnt_c, cnt_max);
     else
-        mt_neigh_block_atoms<MODE, false>(d, p, spos, smol_buf, s_first, s_cnt, s_off, hxo, hyo, hzo, l, f0, cnt_c,
-                                          cnt_max);
+        mt_neigh_block_atoms<MODE>(d, p, spos, smol_buf, s_first, s_cnt, s_off, hxo, hyo, hzo, l, f0, cnt_c, cnt_max);
 }
 
 #define NBW_CAP 1024

@@ -79,6 +79,30 @@ def profiled_traffic():
     return float(m.group(1)) if m else None
 
 
+def profiled_valu():
+    """VALU instructions (wave64) per launch of the MIXED force kernel from the
+    committed rocprofv3 SQ_INSTS_VALU summary, or None when it is absent."""
+    import re
+    path = os.path.join(ROOT, "profiles", "valu_force_mixed_binary_lj_1M.md")
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"valu_insts_per_launch:\s*([0-9.]+)", open(path).read())
+    return float(m.group(1)) if m else None
+
+
+def valu_issue(t_force):
+    """The vector-ALU side of the force kernel: a wave64 VALU instruction
+    occupies a 16-lane SIMD for 4 cycles, so the profiled instruction count
+    over 4 SIMDs x 256 CUs at 2.4 GHz is the issue time the kernel cannot go
+    below without fewer instructions (transcendentals take longer still)."""
+    insts = profiled_valu()
+    if insts is None:
+        return None
+    t_issue = insts * 4.0 / (4 * 256 * 2.4e9)
+    return {"insts_per_launch": insts, "issue_ms": t_issue * 1e3, "issue_frac": t_issue / t_force,
+            "source": "profiles/valu_force_mixed_binary_lj_1M.md (rocprofv3 --pmc SQ_INSTS_VALU)"}
+
+
 def energy_drift(th0, th1):
     """|E(t) - E(0)| / |E(0)| of the total energy, and of Etot - U_cut (the
     energy without the jumps of the unshifted truncated pair potential, which
@@ -191,10 +215,12 @@ def main() -> int:
     rho = n / (sysA.box[0] ** 3)
     r_list = cm.cutoff_radius + cm.skin
     n_full = rho * 4.0 / 3.0 * math.pi * r_list ** 3
-    b_force = 16 + 4 + 4 * n_full + 24                  # own position, count, indices, force
+    # the MIXED tile path's lists hold 16-bit words (the neighbour's place in
+    # its block's staged cells)
+    b_force = 16 + 4 + 2 * n_full + 24                  # own position, count, list words, force
     achieved = n * b_force / t_force
-    b_step = 184 + 4 * n_full
-    b_rb = 32 + 4 * n_full
+    b_step = 184 + 2 * n_full
+    b_rb = 32 + 2 * n_full
     step_bytes = b_step + b_rb / rebuild_every
     cap = em.stats()["capacity"]
     copy_bw = 2 * n * 4 * cap / t_copy                  # in-place pass over the list buffer
@@ -433,6 +459,7 @@ def main() -> int:
             "kernel": "k_fast_force_tile",
             "kernel_ms": t_force * 1e3,
             "bytes_per_atom": b_force,
+            "valu": valu_issue(t_force),
         },
         "cpu_baseline": cpu,
         "fp64": {"value": value64, "ms_per_step": dt_64 / K64 * 1e3, "steps": K64,

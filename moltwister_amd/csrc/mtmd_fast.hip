@@ -1044,17 +1044,21 @@ __device__ __forceinline__ uint32_t mt_tile_pairs4(const float4 (&a)[4], const f
         dy[i] = SHIFT ? pk.y - a[i].y - o[i].y : pk.y - a[i].y;
         dz[i] = SHIFT ? pk.z - a[i].z - o[i].z : pk.z - a[i].z;
         const float r2 = dx[i] * dx[i] + dy[i] * dy[i] + dz[i] * dz[i];
-        in[i] = r2 < lo && r2 >= 1.0e-6f;
-        band |= (r2 >= lo && r2 < hi) ? 1u << i : 0u;
+        const bool lt = r2 < lo;
+        in[i] = lt && r2 >= 1.0e-6f;
+        band |= (!lt && r2 < hi) ? 1u << i : 0u;
         rinv[i] = __builtin_amdgcn_rsqf(fmaxf(r2, 1.0e-6f));
         r[i] = r2 * rinv[i];
         const int idx = (int)((r[i] - x0) * invd);
-        // the neighbour's type times MT_NPTS (in .w) is its offset in the row
-        ix[i] = __float_as_int(a[i].w) + (int)min((unsigned)idx, (unsigned)(MT_NPTS - 1));
+        const int seg = (int)min((unsigned)idx, (unsigned)(MT_NPTS - 1));
+        // .w: the neighbour's type times MT_NPTS, its offset in the row; in
+        // elements for global neighbours (SHIFT), in bytes for staged ones
+        ix[i] = __float_as_int(a[i].w) + (SHIFT ? seg : seg * (int)sizeof(CoefT));
     }
     CoefT c[4];
 #pragma unroll
-    for (int i = 0; i < 4; i++) c[i] = row[ix[i]];
+    for (int i = 0; i < 4; i++)
+        c[i] = SHIFT ? row[ix[i]] : *reinterpret_cast<const CoefT*>(reinterpret_cast<const char*>(row) + ix[i]);
 #pragma unroll
     for (int i = 0; i < 4; i++) {
         float fr = c[i].x + c[i].y * rinv[i];
@@ -1094,7 +1098,9 @@ __device__ __forceinline__ void mt_tile_band16(int l, const float4& a, float qa,
     if (!(mt_dist2_pbc({rk4.x, rk4.y, rk4.z}, {ra4.x, ra4.y, ra4.z}, p.wnl) < (double)p.rc2f)) return;
     const float dx = pk.x - a.x, dy = pk.y - a.y, dz = pk.z - a.z;
     const float r2 = dx * dx + dy * dy + dz * dz;
-    mt_mixed_force<WANT_E, COUL>(dx, dy, dz, r2, a, qa, row, x0, invd, qk, cx, cy, cz, U);
+    // the staged type offset is in bytes, mt_mixed_force's in elements
+    const float4 ae = make_float4(a.x, a.y, a.z, __int_as_float(__float_as_int(a.w) / (int)sizeof(CoefT)));
+    mt_mixed_force<WANT_E, COUL>(dx, dy, dz, r2, ae, qa, row, x0, invd, qk, cx, cy, cz, U);
 }
 
 // the pair loop of a home atom of a 16-bit block over chunks q0, q0 + qs, ...
@@ -1291,8 +1297,10 @@ __global__ void __launch_bounds__(256 * G) k_fast_force_tile(MtDev d, MtParams p
             }
             const int u = s_first[lo] + (i - s_off[lo]);
             const float4 a = d.posf[u];
+            // .w: the type's offset in a coefficient row, in bytes
             spos[i] = make_float4(a.x + (float)((lo & 3) - 1) * p.cs[0], a.y + (float)(((lo >> 2) & 3) - 1) * p.cs[1],
-                                  a.z + (float)((lo >> 4) - 1) * p.cs[2], a.w);
+                                  a.z + (float)((lo >> 4) - 1) * p.cs[2],
+                                  __int_as_float(__float_as_int(a.w) * (WANT_E ? (int)sizeof(float4) : (int)sizeof(float2))));
             if constexpr (COUL) sq_buf[i] = d.qfs[u];
         }
     } else if (t < 27) {
@@ -1313,12 +1321,16 @@ __global__ void __launch_bounds__(256 * G) k_fast_force_tile(MtDev d, MtParams p
             // the home atom's own staged entry (home cell h is staged cell qh)
             const int qh = (1 + (h & 1)) + 4 * ((1 + ((h >> 1) & 1)) + 4 * (1 + (h >> 2)));
             const float4 pk = spos[s_off[qh] + (v - s_hoff[h])];
+            // the home atom's row: its type offset (bytes) times T
+            const size_t rowb = (size_t)__float_as_int(pk.w) * T;
             if constexpr (WANT_E)
                 mt_tile_atom16<true, COUL>(d, p, k, pk, spos, sq_buf, s_first, s_off,
-                                           d.nbcoef + (size_t)__float_as_int(pk.w) * T, qk, Fx, Fy, Fz, U, gl, G);
+                                           reinterpret_cast<const float4*>(reinterpret_cast<const char*>(d.nbcoef) + rowb),
+                                           qk, Fx, Fy, Fz, U, gl, G);
             else
                 mt_tile_atom16<false, COUL>(d, p, k, pk, spos, sq_buf, s_first, s_off,
-                                            stab + __float_as_int(pk.w) * T, qk, Fx, Fy, Fz, U, gl, G);
+                                            reinterpret_cast<const float2*>(reinterpret_cast<const char*>(stab) + rowb),
+                                            qk, Fx, Fy, Fz, U, gl, G);
         } else {
             const float4 pk = d.posf[u];
             if constexpr (WANT_E)

@@ -504,27 +504,26 @@ __device__ __forceinline__ void mt_neigh_block_atoms(const MtDev& d, const MtPar
 // shift: nine runs a home atom.
 template <int MODE>
 __device__ __forceinline__ void mt_neigh_block_atoms16(const MtDev& d, const MtParams& p, float4* spos, int* smol,
-                                                       const int* s_first, const int* s_off, int hxo, int hyo,
-                                                       int hzo, int l, int f0, int cnt_c, int cnt_max)
+                                                       const int* s_first, const int* s_cnt, const int* s_off, int hxo,
+                                                       int hyo, int hzo, int l, int f0, int cnt_c, int cnt_max)
 {
     const int t = threadIdx.x;
     const int n = d.n;
     const int cap = 2 * d.ncap;                       // 16-bit words a row holds
     const float rl2 = p.rlist2;
     uint4* __restrict__ nl4 = reinterpret_cast<uint4*>(d.nlist);
-    const int total = s_off[64];
-    for (int i = t; i < total; i += 256) {
-        int lo = 0, hi = 64;                          // staged cell of item i
-        while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (s_off[mid] <= i) lo = mid; else hi = mid;
+    {
+        // four threads a staged cell, each taking every fourth atom
+        const int q = t >> 2;
+        const float sx = (float)((q & 3) - 1) * p.cs[0], sy = (float)(((q >> 2) & 3) - 1) * p.cs[1];
+        const float sz = (float)((q >> 4) - 1) * p.cs[2];
+        for (int m = t & 3; m < s_cnt[q]; m += 4) {
+            const int ui = s_first[q] + m, i = s_off[q] + m;
+            const float4 pa = d.posf[ui];
+            const int a = (MODE >= 1 || !d.wsorted) ? d.cell_list[ui] : ui;
+            spos[i] = make_float4(pa.x + sx, pa.y + sy, pa.z + sz, __uint_as_float((uint32_t)(d.wsorted ? ui : a)));
+            if constexpr (MODE >= 1) smol[i] = d.mol[a];
         }
-        const int ui = s_first[lo] + (i - s_off[lo]);
-        const float4 pa = d.posf[ui];
-        const int a = (MODE >= 1 || !d.wsorted) ? d.cell_list[ui] : ui;
-        spos[i] = make_float4(pa.x + (float)((lo & 3) - 1) * p.cs[0], pa.y + (float)(((lo >> 2) & 3) - 1) * p.cs[1],
-                              pa.z + (float)((lo >> 4) - 1) * p.cs[2], __uint_as_float((uint32_t)(d.wsorted ? ui : a)));
-        if constexpr (MODE >= 1) smol[i] = d.mol[a];
     }
     __syncthreads();
     // the home cell's place among the staged cells
@@ -639,7 +638,8 @@ __global__ void __launch_bounds__(256) k_fast_neigh_block(MtDev d, MtParams p, i
     // the word format of this block's lists (uniform; k_fast_force_tile makes
     // the same choice from the same counts)
     if (d.tile16 && s_off[64] <= NB_CAP)
-        mt_neigh_block_atoms16<MODE>(d, p, spos, smol_buf, s_first, s_off, hxo, hyo, hzo, l, f0, cnt_c, cnt_max);
+        mt_neigh_block_atoms16<MODE>(d, p, spos, smol_buf, s_first, s_cnt, s_off, hxo, hyo, hzo, l, f0, cnt_c,
+                                     cnt_max);
     else
         mt_neigh_block_atoms<MODE>(d, p, spos, smol_buf, s_first, s_cnt, s_off, hxo, hyo, hzo, l, f0, cnt_c, cnt_max);
 }
@@ -1105,7 +1105,7 @@ __device__ __forceinline__ void mt_tile_band16(int l, const float4& a, float qa,
 
 // the pair loop of a home atom of a 16-bit block over chunks q0, q0 + qs, ...
 // (the next chunk's words in flight across the current chunk's pairs); the
-// pair forces of four pairs are summed in FP32, those sums in FP64
+// pair forces of a chunk's eight pairs are summed in FP32, those sums in FP64
 template <bool WANT_E, bool COUL, typename CoefT>
 __device__ __forceinline__ void mt_tile_atom16(const MtDev& d, const MtParams& p, int k, const float4& pk,
                                                const float4* spos, const float* sq, const int* s_first,
@@ -1132,11 +1132,11 @@ __device__ __forceinline__ void mt_tile_atom16(const MtDev& d, const MtParams& p
         }
         // unconditional (the last iteration reloads a chunk of this atom's list)
         wn = __builtin_nontemporal_load(nl4 + (size_t)min(q + qs, nch - 1) * stride);
+        float cx = 0.f, cy = 0.f, cz = 0.f;
 #pragma unroll
         for (int hh = 0; hh < 2; hh++) {
             const float4 ah[4] = {a[4 * hh], a[4 * hh + 1], a[4 * hh + 2], a[4 * hh + 3]};
             const float qh[4] = {qa[4 * hh], qa[4 * hh + 1], qa[4 * hh + 2], qa[4 * hh + 3]};
-            float cx = 0.f, cy = 0.f, cz = 0.f;
             const uint32_t band =
                 mt_tile_pairs4<WANT_E, COUL, false>(ah, ah, qh, pk, row, lo, hi, x0, invd, qk, cx, cy, cz, U);
             if (band) {
@@ -1146,10 +1146,10 @@ __device__ __forceinline__ void mt_tile_atom16(const MtDev& d, const MtParams& p
                         mt_tile_band16<WANT_E, COUL>(li[4 * hh + i], ah[i], qh[i], pk, row, x0, invd, qk, cx, cy, cz,
                                                      U, d, p, k, s_first, s_off);
             }
-            Fx += (double)cx;
-            Fy += (double)cy;
-            Fz += (double)cz;
         }
+        Fx += (double)cx;
+        Fy += (double)cy;
+        Fz += (double)cz;
     }
 }
 
@@ -1289,17 +1289,15 @@ __global__ void __launch_bounds__(256 * G) k_fast_force_tile(MtDev d, MtParams p
     const int total = s_off[64];
     const bool w16 = total <= FT_CAP;                 // uniform: the list build's choice for this block
     if (w16) {
-        for (int i = t; i < total; i += blockDim.x) {
-            int lo = 0, hi = 64;                       // staged cell of item i
-            while (hi - lo > 1) {
-                const int mid = (lo + hi) >> 1;
-                if (s_off[mid] <= i) lo = mid; else hi = mid;
-            }
-            const int u = s_first[lo] + (i - s_off[lo]);
+        // blockDim / 64 threads a staged cell, each taking every (blockDim / 64)-th atom
+        const int per = (int)blockDim.x >> 6, q = t / per;
+        const float sx = (float)((q & 3) - 1) * p.cs[0], sy = (float)(((q >> 2) & 3) - 1) * p.cs[1];
+        const float sz = (float)((q >> 4) - 1) * p.cs[2];
+        for (int m = t % per; m < s_cnt[q]; m += per) {
+            const int u = s_first[q] + m, i = s_off[q] + m;
             const float4 a = d.posf[u];
             // .w: the type's offset in a coefficient row, in bytes
-            spos[i] = make_float4(a.x + (float)((lo & 3) - 1) * p.cs[0], a.y + (float)(((lo >> 2) & 3) - 1) * p.cs[1],
-                                  a.z + (float)((lo >> 4) - 1) * p.cs[2],
+            spos[i] = make_float4(a.x + sx, a.y + sy, a.z + sz,
                                   __int_as_float(__float_as_int(a.w) * (WANT_E ? (int)sizeof(float4) : (int)sizeof(float2))));
             if constexpr (COUL) sq_buf[i] = d.qfs[u];
         }

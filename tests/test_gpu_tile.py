@@ -136,3 +136,95 @@ def test_tile_cutoff_coulomb(mixture):
     assert np.abs(F0 - Fm).max() <= 1e-12 * np.abs(Fm).max()
     em.close()
     ef.close()
+
+
+def water_cfg(engine, ensemble=abi.NVT):
+    c = config.defaults()
+    c.engine = engine
+    c.ensemble = ensemble
+    c.cutoff_radius = 5.5
+    c.neigh_shell = 1.0
+    c.skin = 1.0
+    c.timestep = 0.5
+    c.temperature = 298.0
+    c.temperature_relax = 20.0
+    c.coulomb = abi.COUL_CUTOFF
+    return c
+
+
+def test_tile_water_molecules():
+    """Water (bonds and angles: same-molecule pairs left to the special-pair
+    pass) on the tile path: 114,000 atoms in a 104 A box, 16 cells of 6.5 A a
+    side, 512 blocks, every block on 16-bit words."""
+    print("case tile water", flush=True)
+    s = W.water_box(38000, seed=3)
+    tot = staged_totals(s.pos, s.box[0], 6.5)
+    assert tot.size == 512 and tot.max() <= FT_CAP
+    em, ef = Engine(s, water_cfg(abi.ENGINE_MIXED)), Engine(s, water_cfg(abi.ENGINE_FP64))
+    Fm, Um = em.compute_forces()
+    Ff, Uf = ef.compute_forces()
+    assert rms_rel(Fm, Ff) < 1e-5, rms_rel(Fm, Ff)
+    assert abs(Um.sum(dtype=np.float64) - Uf.sum(dtype=np.float64)) <= 1e-5 * np.abs(Uf).sum(dtype=np.float64)
+    em.bench_kernel(0, 1)
+    F0, _ = em.get_forces()
+    assert np.abs(F0 - Fm).max() <= 1e-12 * np.abs(Fm).max()
+    assert em.thermo()["U_cut"] == pytest.approx(ef.thermo()["U_cut"], rel=1e-12)
+    for e in (em, ef):
+        e.step(20)
+    tm, tf = em.thermo(), ef.thermo()
+    for k in ("T", "Etot"):
+        assert tm[k] == pytest.approx(tf[k], rel=1e-4), (k, tm[k], tf[k])
+    pm, _ = em.get_state()
+    pf, _ = ef.get_state()
+    L = np.array(s.box)
+    d = pm - pf
+    d -= L * np.rint(d / L)
+    assert np.abs(d).max() < 1e-3, np.abs(d).max()
+    em.close()
+    ef.close()
+
+
+def test_tile_npt_follows_fp64(mixture):
+    """NPT on the tile path: the staged block frame follows the cell width of
+    the current box every step, and a regrid (the box shrinking below 17 cells
+    of cutoff + skin) switches blocks between word formats; the run tracks the
+    FP64 engine. The barostat mass W = T tau_P^2 has no factor N, so a
+    131,072-atom box needs a long tau_P; it starts with an inward p_eps at the
+    current pressure."""
+    print("case tile npt", flush=True)
+    relax = Engine(mixture, lj_cfg(abi.ENGINE_MIXED, 0.3))
+    relax.optimize(40, 2e-5, 0.0)
+    pos, mom = relax.get_state()
+    P0 = relax.thermo()["P"]
+    relax.close()
+    T = 120.272 / 120.272
+    tauP = 50000.0 / 100.0
+    Wb = T * tauP * tauP
+    runs = []
+    for eng in (abi.ENGINE_FP64, abi.ENGINE_MIXED):
+        s2 = W.binary_lj_large(N, tiles=2, seed=5)
+        s2.pos, s2.mom = pos, mom
+        c = lj_cfg(eng, 0.3, ensemble=abi.NPT)
+        c.pressure = P0
+        c.pressure_relax = 50000.0
+        c.temperature_relax = 50.0
+        c.cutoff_force = 0.0
+        e = Engine(s2, c)
+        e.set_barostat(0.0, -15.0 * Wb / tauP, None, None)
+        trace = []
+        for _ in range(4):
+            e.step(25)
+            th, b = e.thermo(), e.get_barostat()
+            trace.append((th["V"], th["T"], b["box"][0]))
+        print(f"engine {eng}: P0 {P0:.1f}, (V, T, L) {trace}", flush=True)
+        runs.append((trace, e.get_barostat(), e.stats()))
+        e.close()
+    (trf, bf, sf), (trm, bm, sm) = runs
+    for (vf, tf_, lf), (vm, tm_, lm) in zip(trf, trm):
+        assert vm == pytest.approx(vf, rel=3e-5)
+        assert tm_ == pytest.approx(tf_, rel=1e-3)
+    # the box moved through a regrid: below 17 cells of 2.8 sigma (47.6 sigma)
+    assert min(lf for _, _, lf in trf) < 17 * 2.8
+    assert abs(bf["eps"]) > 1e-4
+    assert np.allclose(bm["box"], bf["box"], rtol=1e-5, atol=0)
+    assert sm["rebuilds"] > 1

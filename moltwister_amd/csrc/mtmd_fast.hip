@@ -496,12 +496,60 @@ __device__ __forceinline__ void mt_neigh_block_atoms(const MtDev& d, const MtPar
     }
 }
 
+// FP16 pair of (x, y), for the 16-bit list build's distance screen
+typedef _Float16 mt_h2 __attribute__((ext_vector_type(2)));
+
+// half an FP16 ulp at magnitude v > 0 (2^(exponent(v) - 11))
+__device__ __forceinline__ float mt_half_ulp16(float v)
+{
+    return __uint_as_float((uint32_t)((int)(__float_as_uint(v) >> 23) - 11) << 23);
+}
+
 // the 16-bit lists of a block whose 64 staged cells fit NB_CAP: the cells are
 // staged once, in the frame of the centre of the block's first cell (as
 // k_fast_force_tile stages them: cell offsets times cell widths, exact in
 // FP32), so the three cells ox = -1..1 of a (oy, oz) row of a home cell's
 // neighbourhood are one contiguous run of staged atoms with no per-cell
-// shift: nine runs a home atom.
+// shift: nine runs a home atom. The distance screen takes x and y in FP16
+// (one packed subtraction and one dot product with an FP32 z term, five
+// instructions a candidate) against rlist^2 widened by a margin that covers
+// the FP16 rounding, so a list holds every pair inside rlist and a few just
+// beyond it, which the force kernels see as outside rc.
+template <int MODE, bool SELF>
+__device__ __forceinline__ void mt_neigh_run16(const MtDev& d, uint4* nl4, const uint2* sxz, const int* sword,
+                                               const int* smol, int e, int e1, mt_h2 pxy, float pz, float thr,
+                                               int lself, int k, int molk, bool xk, int n, int cap, int& cnt,
+                                               uint32_t& r0, uint32_t& r1, uint32_t& r2, uint32_t& r3)
+{
+    const float4 dummy = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (; e + 3 < e1; e += 4) {
+        uint2 c[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) c[i] = sxz[e + i];
+        bool hv[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const mt_h2 dxy = pxy - __builtin_bit_cast(mt_h2, c[i].x);
+            const float dz = pz - __uint_as_float(c[i].y);
+            hv[i] = __builtin_amdgcn_fdot2(dxy, dxy, dz * dz, false) < thr && (!SELF || e + i != lself);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+            if (hv[i] && (MODE != 2 || !xk || !mt_is_partner(d, k, mt_id_slot(d, sword[e + i]))))
+                mt_neigh_hit<MODE == 2 ? 1 : MODE, true>(d, nl4, dummy, 0u, k, molk, smol, e + i, n, cap, cnt, r0,
+                                                          r1, r2, r3);
+    }
+    for (; e < e1; e++) {
+        const uint2 c0 = sxz[e];
+        const mt_h2 dxy = pxy - __builtin_bit_cast(mt_h2, c0.x);
+        const float dz = pz - __uint_as_float(c0.y);
+        if (__builtin_amdgcn_fdot2(dxy, dxy, dz * dz, false) < thr && (!SELF || e != lself) &&
+            (MODE != 2 || !xk || !mt_is_partner(d, k, mt_id_slot(d, sword[e]))))
+            mt_neigh_hit<MODE == 2 ? 1 : MODE, true>(d, nl4, dummy, 0u, k, molk, smol, e, n, cap, cnt, r0, r1, r2,
+                                                      r3);
+    }
+}
+
 template <int MODE>
 __device__ __forceinline__ void mt_neigh_block_atoms16(const MtDev& d, const MtParams& p, float4* spos, int* smol,
                                                        const int* s_first, const int* s_cnt, const int* s_off, int hxo,
@@ -510,8 +558,18 @@ __device__ __forceinline__ void mt_neigh_block_atoms16(const MtDev& d, const MtP
     const int t = threadIdx.x;
     const int n = d.n;
     const int cap = 2 * d.ncap;                       // 16-bit words a row holds
-    const float rl2 = p.rlist2;
     uint4* __restrict__ nl4 = reinterpret_cast<uint4*>(d.nlist);
+    // the staging buffer as (FP16 x | FP16 y, FP32 z) and, for MODE 2, the word indices
+    uint2* sxz = reinterpret_cast<uint2*>(spos);
+    int* sword = reinterpret_cast<int*>(spos) + 2 * NB_CAP;
+    // the screen's threshold: staged |x|, |y| <= 2.5 cs (cells at offsets -1..2
+    // around the first cell's centre, atoms within half a cell of their
+    // cell's centre at a build) round by at most h(2.6 cs) each in FP16, the
+    // difference of a pair inside rlist by h(rlist), so |error of dx, dy| <=
+    // delta and |error of r^2| <= 2 sqrt(2) rlist delta + 2 delta^2
+    const float csm = fmaxf(p.cs[0], fmaxf(p.cs[1], p.cs[2]));
+    const float delta = 2.0f * mt_half_ulp16(2.6f * csm) + mt_half_ulp16(p.rlist);
+    const float thr = p.rlist2 + 3.0f * p.rlist * delta + 2.0f * delta * delta;
     {
         // four threads a staged cell, each taking every fourth atom
         const int q = t >> 2;
@@ -520,9 +578,13 @@ __device__ __forceinline__ void mt_neigh_block_atoms16(const MtDev& d, const MtP
         for (int m = t & 3; m < s_cnt[q]; m += 4) {
             const int ui = s_first[q] + m, i = s_off[q] + m;
             const float4 pa = d.posf[ui];
-            const int a = (MODE >= 1 || !d.wsorted) ? d.cell_list[ui] : ui;
-            spos[i] = make_float4(pa.x + sx, pa.y + sy, pa.z + sz, __uint_as_float((uint32_t)(d.wsorted ? ui : a)));
-            if constexpr (MODE >= 1) smol[i] = d.mol[a];
+            const mt_h2 xy = {(_Float16)(pa.x + sx), (_Float16)(pa.y + sy)};
+            sxz[i] = make_uint2(__builtin_bit_cast(uint32_t, xy), __float_as_uint(pa.z + sz));
+            if constexpr (MODE >= 1) {
+                const int a = d.cell_list[ui];
+                smol[i] = d.mol[a];
+                if constexpr (MODE == 2) sword[i] = d.wsorted ? ui : a;
+            }
         }
     }
     __syncthreads();
@@ -533,9 +595,10 @@ __device__ __forceinline__ void mt_neigh_block_atoms16(const MtDev& d, const MtP
         if (!act) continue;
         const int uk = f0 + ib + l;                   // place in cell order
         const int k = d.cell_list[uk];
-        const int self = d.wsorted ? uk : k;
         const int lself = s_off[qh] + ib + l;         // own staged place (the pad word)
-        const float4 pk = spos[lself];
+        const uint2 hv = sxz[lself];
+        const mt_h2 pxy = __builtin_bit_cast(mt_h2, hv.x);
+        const float pz = __uint_as_float(hv.y);
         const int molk = MODE >= 1 ? d.mol[k] : -1;
         const bool xk = MODE == 2 && d.xmol[d.perm[k]] != 0;
         int cnt = 0;
@@ -543,33 +606,13 @@ __device__ __forceinline__ void mt_neigh_block_atoms16(const MtDev& d, const MtP
         for (int oz = 0; oz < 3; oz++)
             for (int oy = 0; oy < 3; oy++) {
                 const int q = hxo + 4 * ((oy + hyo) + 4 * (oz + hzo));     // the row's ox = -1 cell
-                const int e1 = s_off[q + 3];
-                int e = s_off[q];
-                for (; e + 3 < e1; e += 4) {
-                    float4 c[4];
-#pragma unroll
-                    for (int i = 0; i < 4; i++) c[i] = spos[e + i];
-                    bool hv[4];
-#pragma unroll
-                    for (int i = 0; i < 4; i++) {
-                        const float dx = pk.x - c[i].x, dy = pk.y - c[i].y, dz = pk.z - c[i].z;
-                        hv[i] = dx * dx + dy * dy + dz * dz < rl2 && __float_as_int(c[i].w) != self;
-                    }
-#pragma unroll
-                    for (int i = 0; i < 4; i++)
-                        if (hv[i] &&
-                            (MODE != 2 || !xk || !mt_is_partner(d, k, mt_id_slot(d, __float_as_int(c[i].w)))))
-                            mt_neigh_hit<MODE == 2 ? 1 : MODE, true>(d, nl4, c[i], 0u, k, molk, smol, e + i, n, cap,
-                                                                      cnt, r0, r1, r2, r3);
-                }
-                for (; e < e1; e++) {
-                    const float4 c0 = spos[e];
-                    const float dx = pk.x - c0.x, dy = pk.y - c0.y, dz = pk.z - c0.z;
-                    if (dx * dx + dy * dy + dz * dz < rl2 && __float_as_int(c0.w) != self &&
-                        (MODE != 2 || !xk || !mt_is_partner(d, k, mt_id_slot(d, __float_as_int(c0.w)))))
-                        mt_neigh_hit<MODE == 2 ? 1 : MODE, true>(d, nl4, c0, 0u, k, molk, smol, e, n, cap, cnt, r0,
-                                                                  r1, r2, r3);
-                }
+                // only the middle run holds the atom itself
+                if (oy == 1 && oz == 1)
+                    mt_neigh_run16<MODE, true>(d, nl4, sxz, sword, smol, s_off[q], s_off[q + 3], pxy, pz, thr, lself,
+                                               k, molk, xk, n, cap, cnt, r0, r1, r2, r3);
+                else
+                    mt_neigh_run16<MODE, false>(d, nl4, sxz, sword, smol, s_off[q], s_off[q + 3], pxy, pz, thr, lself,
+                                                k, molk, xk, n, cap, cnt, r0, r1, r2, r3);
             }
         // last partial chunk, padded with the atom's own staged place
         const int slot = cnt & 7;

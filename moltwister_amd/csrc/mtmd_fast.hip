@@ -1318,8 +1318,8 @@ __global__ void __launch_bounds__(256 * G) k_fast_force_tile(MtDev d, MtParams p
     __shared__ int s_hfirst[8], s_hoff[9];
     const int t = threadIdx.x;
     const int T = d.T;
-    // the segment coefficients in LDS: read from global memory instead (L1/L2
-    // hits), the pass measured 0.39 ms instead of 0.23 ms on the 1M-atom mixture
+    // the segment coefficients in LDS (read from global memory instead, L1/L2
+    // hits, the pass measured 1.7x slower on the 1M-atom mixture)
     extern __shared__ float2 stab[];                  // T * T * MT_NPTS segment coefficients
     if constexpr (!WANT_E) {
         const int ntab = T * T * MT_NPTS;

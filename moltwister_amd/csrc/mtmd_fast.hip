@@ -2016,12 +2016,16 @@ __global__ void __launch_bounds__(TPB) k_ucut(MtDev d, MtParams p, int exact)
 }
 
 // the same over the MIXED block lists (d.tile16): workgroups take the 2x2x2
-// blocks g = blockIdx.x, blockIdx.x + gridDim.x, ... and decode a 16-bit
-// word through the block's staged cells
+// blocks g = blockIdx.x, blockIdx.x + gridDim.x, ...; a block on 16-bit words
+// has the FP64 positions and types of its staged cells copied to LDS, so a
+// word is two LDS reads
 __global__ void __launch_bounds__(TPB) k_ucut_tile(MtDev d, MtParams p)
 {
+    __shared__ double s_x[FT_CAP], s_y[FT_CAP], s_z[FT_CAP];
+    __shared__ int s_ty[FT_CAP];
     __shared__ int s_first[64], s_cnt[64], s_off[65];
     __shared__ int s_hfirst[8], s_hoff[9];
+    const int t = threadIdx.x;
     const int bx = (d.nc[0] + 1) >> 1, by = (d.nc[1] + 1) >> 1, bz = (d.nc[2] + 1) >> 1;
     const int nb = bx * by * bz;
     const uint16_t* __restrict__ nl16 = reinterpret_cast<const uint16_t*>(d.nlist);
@@ -2032,8 +2036,21 @@ __global__ void __launch_bounds__(TPB) k_ucut_tile(MtDev d, MtParams p)
                       s_hoff);
         __syncthreads();
         const bool w16 = s_off[64] <= FT_CAP;
+        if (w16) {
+            // TPB / 64 threads a staged cell
+            const int per = TPB >> 6, q = t / per;
+            for (int m = t % per; m < s_cnt[q]; m += per) {
+                const int a = d.cell_list[s_first[q] + m], i = s_off[q] + m;
+                const double4 r = d.pos[a];
+                s_x[i] = r.x;
+                s_y[i] = r.y;
+                s_z[i] = r.z;
+                s_ty[i] = d.type[a];
+            }
+            __syncthreads();
+        }
         const int nh = s_hoff[8];
-        for (int v = threadIdx.x; v < nh; v += blockDim.x) {
+        for (int v = t; v < nh; v += blockDim.x) {
             const int h = mt_tile_home(s_hoff, v);
             const int k = d.cell_list[s_hfirst[h] + (v - s_hoff[h])];
             if (!d.mobile[k]) continue;
@@ -2042,21 +2059,19 @@ __global__ void __launch_bounds__(TPB) k_ucut_tile(MtDev d, MtParams p)
             const int row = d.type[k];
             const int cnt = min(d.ncount[k], w16 ? 2 * d.ncap : d.ncap);
             for (int j = 0; j < cnt; j++) {
-                int a;
+                mt_d3 ra;
+                int ta;
                 if (w16) {
                     const int l = nl16[(((size_t)(j >> 3) * d.n + k) << 3) + (j & 7)];
-                    int lo = 0, hi = 64;
-                    while (hi - lo > 1) {
-                        const int mid = (lo + hi) >> 1;
-                        if (s_off[mid] <= l) lo = mid; else hi = mid;
-                    }
-                    a = d.cell_list[s_first[lo] + (l - s_off[lo])];
+                    ra = {s_x[l], s_y[l], s_z[l]};
+                    ta = s_ty[l];
                 } else {
-                    a = mt_id_slot(d, (int)(mt_fast_word(d.nlist, d.n, k, j) & MT_NW_IDX_MASK));
+                    const int a = mt_id_slot(d, (int)(mt_fast_word(d.nlist, d.n, k, j) & MT_NW_IDX_MASK));
+                    const double4 ra4 = d.pos[a];
+                    ra = {ra4.x, ra4.y, ra4.z};
+                    ta = d.type[a];
                 }
-                const double4 ra4 = d.pos[a];
-                if (mt_dist2_pbc(rk, {ra4.x, ra4.y, ra4.z}, p.wnl) < (double)p.rc2f)
-                    u += 0.5 * (double)d.ecut[row * d.T + d.type[a]];
+                if (mt_dist2_pbc(rk, ra, p.wnl) < (double)p.rc2f) u += 0.5 * (double)d.ecut[row * d.T + ta];
             }
             u += mt_ucut_special(d, p, k, rk, row);
         }

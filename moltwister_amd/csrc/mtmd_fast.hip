@@ -2015,6 +2015,10 @@ __global__ void __launch_bounds__(TPB) k_ucut(MtDev d, MtParams p, int exact)
     mt_block_sum_to(u, d.red + blockIdx.x);
 }
 
+// type pairs whose U(rc) k_ucut_tile keeps in LDS; the tile mode's 32 KB
+// coefficient table allows at most 4 (two types)
+#define MT_UCUT_TT 64
+
 // the same over the MIXED block lists (d.tile16): workgroups take the 2x2x2
 // blocks g = blockIdx.x, blockIdx.x + gridDim.x, ...; a block on 16-bit words
 // has the FP64 positions and types of its staged cells copied to LDS, so a
@@ -2025,10 +2029,11 @@ __global__ void __launch_bounds__(TPB) k_ucut_tile(MtDev d, MtParams p)
     __shared__ int s_ty[FT_CAP];
     __shared__ int s_first[64], s_cnt[64], s_off[65];
     __shared__ int s_hfirst[8], s_hoff[9];
+    __shared__ float s_ecut[MT_UCUT_TT];
     const int t = threadIdx.x;
     const int bx = (d.nc[0] + 1) >> 1, by = (d.nc[1] + 1) >> 1, bz = (d.nc[2] + 1) >> 1;
     const int nb = bx * by * bz;
-    const uint16_t* __restrict__ nl16 = reinterpret_cast<const uint16_t*>(d.nlist);
+    for (int i = t; i < d.T * d.T; i += blockDim.x) s_ecut[i] = d.ecut[i];
     double u = 0.0;
     for (int g = blockIdx.x; g < nb; g += gridDim.x) {
         __syncthreads();                              // the previous block's tables are read
@@ -2058,20 +2063,27 @@ __global__ void __launch_bounds__(TPB) k_ucut_tile(MtDev d, MtParams p)
             const mt_d3 rk = {rk4.x, rk4.y, rk4.z};
             const int row = d.type[k];
             const int cnt = min(d.ncount[k], w16 ? 2 * d.ncap : d.ncap);
-            for (int j = 0; j < cnt; j++) {
-                mt_d3 ra;
-                int ta;
-                if (w16) {
-                    const int l = nl16[(((size_t)(j >> 3) * d.n + k) << 3) + (j & 7)];
-                    ra = {s_x[l], s_y[l], s_z[l]};
-                    ta = s_ty[l];
-                } else {
+            if (w16) {
+                // a chunk of eight words a load; U(rc) of the pair's types from LDS
+                const uint4* __restrict__ nl4 = reinterpret_cast<const uint4*>(d.nlist) + k;
+                for (int j0 = 0; j0 < cnt; j0 += 8) {
+                    const uint4 w = nl4[(size_t)(j0 >> 3) * d.n];
+                    const uint32_t wv[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+                    for (int i = 0; i < 8; i++) {
+                        if (j0 + i >= cnt) break;
+                        const int l = (int)((wv[i >> 1] >> (16 * (i & 1))) & 0xffffu);
+                        if (mt_dist2_pbc(rk, {s_x[l], s_y[l], s_z[l]}, p.wnl) < (double)p.rc2f)
+                            u += 0.5 * (double)s_ecut[row * d.T + s_ty[l]];
+                    }
+                }
+            } else {
+                for (int j = 0; j < cnt; j++) {
                     const int a = mt_id_slot(d, (int)(mt_fast_word(d.nlist, d.n, k, j) & MT_NW_IDX_MASK));
                     const double4 ra4 = d.pos[a];
-                    ra = {ra4.x, ra4.y, ra4.z};
-                    ta = d.type[a];
+                    if (mt_dist2_pbc(rk, {ra4.x, ra4.y, ra4.z}, p.wnl) < (double)p.rc2f)
+                        u += 0.5 * (double)d.ecut[row * d.T + d.type[a]];
                 }
-                if (mt_dist2_pbc(rk, ra, p.wnl) < (double)p.rc2f) u += 0.5 * (double)d.ecut[row * d.T + ta];
             }
             u += mt_ucut_special(d, p, k, rk, row);
         }

@@ -1749,10 +1749,14 @@ static void mt_launch_fp64(const MtDev& d, const MtParams& p, int bs, hipStream_
 // the LDS-tiled MIXED kernel (k_fast_force_tile and the block lists it reads):
 // sparse cells (the block list build's regime), the coefficient table within
 // 32 KB of LDS, and enough 2x2x2 cell blocks to fill the GPU
-int mt_fast_tile_mode(const MtDev& d, int mixed)
+int mt_fast_tile_mode(const MtDev& d, const double* L, int mixed)
 {
     const size_t lds_tab = (size_t)d.T * d.T * MT_NPTS * sizeof(float2);
     if (!mixed || !d.wsorted || lds_tab > 32768 || d.n > 40 * d.ncells) return 0;
+    // the 16-bit list build screens staged coordinates (up to 2.5 cell widths)
+    // in FP16: keep them far inside its range
+    for (int i = 0; i < 3; i++)
+        if (!(2.6 * L[i] / d.nc[i] < 16384.0)) return 0;
     const long long nb = (long long)((d.nc[0] + 1) / 2) * ((d.nc[1] + 1) / 2) * ((d.nc[2] + 1) / 2);
     return nb >= 2LL * mt_cu_count() ? 1 : 0;
 }

@@ -727,7 +727,7 @@ static int setup(mtmd_handle* h, const mtmd_system* s)
         d.sidx = h->mem.alloc<int>(n);
         d.qfs = h->mem.alloc<float>(n);
         d.wsorted = (c.engine == MTMD_ENGINE_MIXED) ? 1 : 0;
-        d.tile16 = mt_fast_tile_mode(d, c.engine == MTMD_ENGINE_MIXED);
+        d.tile16 = mt_fast_tile_mode(d, p.L, c.engine == MTMD_ENGINE_MIXED);
         d.qf = h->mem.alloc<float>(n);
         d.qf2 = h->mem.alloc<float>(n);
         if (p.coulomb_on && p.coulomb_allpairs) {
@@ -1049,7 +1049,7 @@ static int npt_regrid(mtmd_handle* h)
     }
     if (changed) {
         d.ncells = d.nc[0] * d.nc[1] * d.nc[2];
-        d.tile16 = mt_fast_tile_mode(d, h->engine == MTMD_ENGINE_MIXED);   // lists of the new grid
+        d.tile16 = mt_fast_tile_mode(d, h->p.L, h->engine == MTMD_ENGINE_MIXED);   // lists of the new grid
         set_box(h, h->p.L);                          // cell sizes of the new grid
         h->drop_graphs();
     }

@@ -151,9 +151,10 @@ void mt_fast_unsort(const MtDev& d, hipStream_t s);       // writes the *2 buffe
 void mt_fast_identity_perm(const MtDev& d, hipStream_t s);
 void mt_fast_neighbors(const MtDev& d, const MtParams& p, int conditional, hipStream_t s);
 // 1 when the MIXED pair forces run on 2x2x2 cell blocks staged in LDS (sparse
-// cells, the coefficient table within 32 KB, enough blocks to fill the GPU);
+// cells, the coefficient table within 32 KB, enough blocks to fill the GPU,
+// cell widths of box L within the FP16 range of the list build's screen);
 // the host stores it in d.tile16 whenever the cell grid is set, before a build
-int mt_fast_tile_mode(const MtDev& d, int mixed);
+int mt_fast_tile_mode(const MtDev& d, const double* L, int mixed);
 void mt_fast_forces(const MtDev& d, const MtParams& p, int mixed, int want_energy, int block, hipStream_t s);
 // lanes per atom the pair-force kernels use for this system (1, 4 or 16)
 int mt_fast_lanes(const MtDev& d);

@@ -116,17 +116,21 @@ def test_tile_nve_tracks_fp64(mixture):
     assert sm["rebuilds"] > 1 and sf["rebuilds"] > 1
 
 
-def test_tile_cutoff_coulomb(mixture):
-    """Charged mixture (random +-q, neutral) with cutoff Coulomb: the charges
-    staged beside the positions."""
-    print("case tile coulomb", flush=True)
+@pytest.mark.parametrize("coul", [abi.COUL_CUTOFF, abi.COUL_ALLPAIRS])
+def test_tile_coulomb(mixture, coul):
+    """Charged mixture (random +-q, neutral): cutoff Coulomb with the charges
+    staged beside the positions, and the reference's all-pairs Coulomb (its
+    own N^2 pass beside the tile kernel)."""
+    print(f"case tile coulomb {coul}", flush=True)
     s = W.binary_lj_large(N, tiles=2, seed=5)
     q = np.full(N, 0.3)
     q[N // 2:] = -0.3
     np.random.default_rng(9).shuffle(q)
     s.charge = q
-    em = Engine(s, lj_cfg(abi.ENGINE_MIXED, 0.3))
-    ef = Engine(s, lj_cfg(abi.ENGINE_FP64, 0.3))
+    cm, cf = lj_cfg(abi.ENGINE_MIXED, 0.3), lj_cfg(abi.ENGINE_FP64, 0.3)
+    cm.coulomb = cf.coulomb = coul
+    em = Engine(s, cm)
+    ef = Engine(s, cf)
     Fm, Um = em.compute_forces()
     Ff, Uf = ef.compute_forces()
     assert rms_rel(Fm, Ff) < 1e-5, rms_rel(Fm, Ff)

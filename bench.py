@@ -361,14 +361,21 @@ def main() -> int:
         t1 = time.perf_counter()
         L64, g64 = fmh.loss_grad(p0, fp64=True)
         t_64 = time.perf_counter() - t1
-        fd_err = 0.0
-        for k in range(len(p0)):
-            h = 1e-6 * abs(p0[k])
+        # central differences of the FP64 loss at h = 1e-6 relative (the stated
+        # check) and at h = 1e-5: their difference is the differences' own
+        # rounding error, which at 1e-6 is of the order of the bar itself
+        def central(k, rel):
+            h = rel * abs(p0[k])
             pp, pm = p0.copy(), p0.copy()
             pp[k] += h
             pm[k] -= h
-            fd = (fmh.loss_grad(pp, fp64=True)[0] - fmh.loss_grad(pm, fp64=True)[0]) / (2 * h)
-            fd_err = max(fd_err, abs(g64[k] - fd) / abs(fd))
+            return (fmh.loss_grad(pp, fp64=True)[0] - fmh.loss_grad(pm, fp64=True)[0]) / (2 * h)
+        fd_err = fd_err5 = fd_noise = 0.0
+        for k in range(len(p0)):
+            fd6, fd5 = central(k, 1e-6), central(k, 1e-5)
+            fd_err = max(fd_err, abs(g64[k] - fd6) / abs(fd6))
+            fd_err5 = max(fd_err5, abs(g64[k] - fd5) / abs(fd5))
+            fd_noise = max(fd_noise, abs(fd6 - fd5) / abs(fd5))
         t1 = time.perf_counter()
         theta, hist = fit(fmh, p0, iters=500, lr=2e-2)
         t_fit = time.perf_counter() - t1
@@ -379,6 +386,8 @@ def main() -> int:
                  "frames_per_s_fp32": n_frames / t_32,
                  "grad_rel_err_fp32_vs_fp64": float(np.max(np.abs(g32 - g64) / np.abs(g64))),
                  "grad_rel_err_fp64_vs_fd": fd_err,
+                 "grad_rel_err_fp64_vs_fd_h1e-5": fd_err5,
+                 "fd_h1e-6_vs_h1e-5": fd_noise,
                  "adam_500": {"loss_start": hist[0], "loss_end": hist[-1], "seconds": t_fit,
                               "param_rel_err": [float(x) for x in np.abs(theta / truth - 1)]}}
         fmh.close()

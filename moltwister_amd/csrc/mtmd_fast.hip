@@ -1851,9 +1851,10 @@ __device__ __forceinline__ float mt_fast_locate_atom(const MtDev& d, const MtPar
         lx -= p.L[0] * rint(lx / p.L[0]);
         ly -= p.L[1] * rint(ly / p.L[1]);
         lz -= p.L[2] * rint(lz / p.L[2]);
-        const int u = d.sidx[k];
-        const float4 old = d.posf[u];
-        d.posf[u] = make_float4((float)lx, (float)ly, (float)lz, old.w);
+        // x, y, z only: .w (the type offset) stays, and the line is not read
+        float* pf = reinterpret_cast<float*>(d.posf + d.sidx[k]);
+        *reinterpret_cast<float2*>(pf) = make_float2((float)lx, (float)ly);
+        pf[2] = (float)lz;
     }
     return (float)(dx * dx + dy * dy + dz * dz);
 }

@@ -412,22 +412,23 @@ __device__ inline double mt_nh_propagate(double* eta, double* peta, const double
     return S;
 }
 
-// mt_nh_propagate for chains of up to MAXM links with the chain state held in
-// registers: the same operations in the same order (so the same bits), with
-// every loop unrolled over MAXM and guarded by M so each array index is a
-// constant. Working on eta, p_eta and Q in global memory instead costs one
-// dependent memory round trip per update, tens of microseconds per call.
-template <int MAXM>
-__device__ inline double mt_nh_propagate_reg(double* eta_g, double* peta_g, const double* Q_g, int M, int nresp,
-                                             double T, double dt, double g, const double* w, double KE2)
+// mt_nh_propagate for a chain of exactly M links with the chain state held in
+// registers: the same operations in the same order (so the same bits), every
+// loop over the links unrolled with constant indices and no guard. Working on
+// eta, p_eta and Q in global memory instead costs one dependent memory round
+// trip per update, and loops guarded for a run-time M keep a branch around
+// every update; either way the serial chain takes several times longer.
+template <int M>
+__device__ inline double mt_nh_propagate_fixed(double* eta_g, double* peta_g, const double* Q_g, int nresp, double T,
+                                               double dt, double g, const double* w, double KE2)
 {
     MT_EXACT_FP
-    double eta[MAXM], peta[MAXM], Q[MAXM];
+    double eta[M], peta[M], Q[M];
 #pragma unroll
-    for (int j = 0; j < MAXM; j++) {
-        eta[j] = j < M ? eta_g[j] : 0.0;
-        peta[j] = j < M ? peta_g[j] : 0.0;
-        Q[j] = j < M ? Q_g[j] : 1.0;
+    for (int j = 0; j < M; j++) {
+        eta[j] = eta_g[j];
+        peta[j] = peta_g[j];
+        Q[j] = Q_g[j];
     }
     double S = 1.0;
     const double beta = 1.0 / T;
@@ -436,47 +437,37 @@ __device__ inline double mt_nh_propagate_reg(double* eta_g, double* peta_g, cons
         const double Dt = w[mu] * dt / (double)nresp;
         for (int i = 0; i < nresp; i++) {
 #define MT_G(j) ((j) == 0 ? (KE2 - (g / beta)) : ((peta[(j) - 1] * peta[(j) - 1] / Q[(j) - 1]) - (1.0 / beta)))
+            peta[M - 1] += (Dt / 4.0) * MT_G(M - 1);
 #pragma unroll
-            for (int j = 0; j < MAXM; j++)
-                if (j == M - 1) peta[j] += (Dt / 4.0) * MT_G(j);
-#pragma unroll
-            for (int j = 0; j < MAXM - 1; j++) {
-                if (j < M - 1) {
-                    double c = mt_cmt_exp(-(Dt / 8.0) * (peta[j + 1] / Q[j + 1]));
-                    peta[j] *= c;
-                    peta[j] += (Dt / 4.0) * MT_G(j);
-                    c = mt_cmt_exp(-(Dt / 8.0) * (peta[j + 1] / Q[j + 1]));
-                    peta[j] *= c;
-                }
+            for (int j = 0; j < M - 1; j++) {
+                double c = mt_cmt_exp(-(Dt / 8.0) * (peta[j + 1] / Q[j + 1]));
+                peta[j] *= c;
+                peta[j] += (Dt / 4.0) * MT_G(j);
+                c = mt_cmt_exp(-(Dt / 8.0) * (peta[j + 1] / Q[j + 1]));
+                peta[j] *= c;
             }
 #pragma unroll
-            for (int j = 0; j < MAXM; j++)
-                if (j < M) eta[j] += (Dt / 2.0) * (peta[j] / Q[j]);
+            for (int j = 0; j < M; j++) eta[j] += (Dt / 2.0) * (peta[j] / Q[j]);
             const double c0 = mt_cmt_exp(-((Dt / 2.0) * (peta[0] / Q[0])));
             S *= c0;
             KE2 *= c0 * c0;
 #pragma unroll
-            for (int j = MAXM - 2; j >= 0; j--) {
-                if (j < M - 1) {
-                    double c = mt_cmt_exp(-(Dt / 8.0) * (peta[j + 1] / Q[j + 1]));
-                    peta[j] *= c;
-                    peta[j] += (Dt / 4.0) * MT_G(j);
-                    c = mt_cmt_exp(-(Dt / 8.0) * (peta[j + 1] / Q[j + 1]));
-                    peta[j] *= c;
-                }
+            for (int j = M - 2; j >= 0; j--) {
+                double c = mt_cmt_exp(-(Dt / 8.0) * (peta[j + 1] / Q[j + 1]));
+                peta[j] *= c;
+                peta[j] += (Dt / 4.0) * MT_G(j);
+                c = mt_cmt_exp(-(Dt / 8.0) * (peta[j + 1] / Q[j + 1]));
+                peta[j] *= c;
             }
-#pragma unroll
-            for (int j = 0; j < MAXM; j++)
-                if (j == M - 1) peta[j] += (Dt / 4.0) * MT_G(j);
+            peta[M - 1] += (Dt / 4.0) * MT_G(M - 1);
 #undef MT_G
         }
     }
 #pragma unroll
-    for (int j = 0; j < MAXM; j++)
-        if (j < M) {
-            eta_g[j] = eta[j];
-            peta_g[j] = peta[j];
-        }
+    for (int j = 0; j < M; j++) {
+        eta_g[j] = eta[j];
+        peta_g[j] = peta[j];
+    }
     return S;
 }
 
@@ -485,9 +476,17 @@ __device__ inline double mt_nh_propagate_reg(double* eta_g, double* peta_g, cons
 __device__ inline double mt_nh_step(double* eta, double* peta, const double* Q, int M, int nresp, double T,
                                     double dt, double g, const double* w, double KE2)
 {
-    if (M <= 0) return 1.0;
-    if (M <= 8) return mt_nh_propagate_reg<8>(eta, peta, Q, M, nresp, T, dt, g, w, KE2);
-    return mt_nh_propagate(eta, peta, Q, M, nresp, T, dt, g, w, KE2);
+    switch (M) {
+    case 1: return mt_nh_propagate_fixed<1>(eta, peta, Q, nresp, T, dt, g, w, KE2);
+    case 2: return mt_nh_propagate_fixed<2>(eta, peta, Q, nresp, T, dt, g, w, KE2);
+    case 3: return mt_nh_propagate_fixed<3>(eta, peta, Q, nresp, T, dt, g, w, KE2);
+    case 4: return mt_nh_propagate_fixed<4>(eta, peta, Q, nresp, T, dt, g, w, KE2);
+    case 5: return mt_nh_propagate_fixed<5>(eta, peta, Q, nresp, T, dt, g, w, KE2);
+    case 6: return mt_nh_propagate_fixed<6>(eta, peta, Q, nresp, T, dt, g, w, KE2);
+    case 7: return mt_nh_propagate_fixed<7>(eta, peta, Q, nresp, T, dt, g, w, KE2);
+    case 8: return mt_nh_propagate_fixed<8>(eta, peta, Q, nresp, T, dt, g, w, KE2);
+    default: return M <= 0 ? 1.0 : mt_nh_propagate(eta, peta, Q, M, nresp, T, dt, g, w, KE2);
+    }
 }
 
 // CSimulationBox::pbcAdd (SimulationBox.cpp:135-166): one wrap step

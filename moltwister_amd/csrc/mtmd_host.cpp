@@ -1613,13 +1613,30 @@ static int optimize_loop(mtmd_handle* h, int max_steps, double lr, double accura
         for (int b = 0; b < nb; b++) u += part[3 * b + 2];
         return MTMD_OK;
     };
+    // FP64/MIXED (outside NPT): after the first full build the skin lists are
+    // rebuilt only when an atom moved more than half the skin, as in an MD
+    // step (the lists then hold every pair inside rc, as the reference's
+    // per-step rebuild at rc does); EXACT rebuilds at rc every step
+    const bool skin_lists = h->fast && !h->npt;
     if ((rc = compute_forces_now(h))) return rc;
     double Un = 0.0;
     if ((rc = utot(Un))) return rc;
     int i = 0;
     for (i = 0; i < max_steps; i++) {
         const auto t0 = std::chrono::steady_clock::now();
-        if ((rc = compute_forces_now(h))) return rc;
+        if (!skin_lists) {
+            if ((rc = compute_forces_now(h))) return rc;
+        } else if (i > 0) {
+            // positions moved by the last gradient step: displacement test (and
+            // MIXED cell-local coordinates), conditional rebuild, forces
+            mt_fast_locate(h->d, h->p, h->engine == MTMD_ENGINE_MIXED, h->stream);
+            if ((rc = fast_rebuild(h, false, false))) return rc;
+            if ((rc = fast_forces(h, true))) return rc;
+            bool ov = false;
+            if ((rc = fast_check_capacity(h, &ov))) return rc;
+            if (ov && (rc = compute_forces_now(h))) return rc;       // regrow and rebuild in full
+            h->have_forces = true;
+        }
         mt_gradient_step(h->d, lr, h->stream);
         double Unp1 = 0.0;
         if ((rc = utot(Unp1))) return rc;

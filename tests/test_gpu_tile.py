@@ -232,3 +232,23 @@ def test_tile_npt_follows_fp64(mixture):
     assert abs(bf["eps"]) > 1e-4
     assert np.allclose(bm["box"], bf["box"], rtol=1e-5, atol=0)
     assert sm["rebuilds"] > 1
+
+
+def test_tile_monatomic_fcc():
+    """One atom type (T = 1): a 256,000-atom FCC LJ crystal (configuration 1's
+    lattice at 40^3 unit cells, 1728 cell blocks), positions perturbed off the
+    lattice sites, forces and energies against FP64."""
+    print("case tile fcc", flush=True)
+    s = W.fcc_lj(40)
+    s.pos = s.pos + np.random.default_rng(8).normal(0.0, 0.03, s.pos.shape)
+    em, ef = Engine(s, lj_cfg(abi.ENGINE_MIXED, 0.3)), Engine(s, lj_cfg(abi.ENGINE_FP64, 0.3))
+    Fm, Um = em.compute_forces()
+    Ff, Uf = ef.compute_forces()
+    assert rms_rel(Fm, Ff) < 1e-5, rms_rel(Fm, Ff)
+    assert abs(Um.sum(dtype=np.float64) - Uf.sum(dtype=np.float64)) <= 1e-5 * np.abs(Uf).sum(dtype=np.float64)
+    em.bench_kernel(0, 1)
+    F0, _ = em.get_forces()
+    assert np.abs(F0 - Fm).max() <= 1e-12 * np.abs(Fm).max()
+    assert em.thermo()["U_cut"] == pytest.approx(ef.thermo()["U_cut"], rel=1e-12)
+    em.close()
+    ef.close()

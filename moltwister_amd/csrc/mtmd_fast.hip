@@ -552,8 +552,8 @@ __device__ __forceinline__ void mt_neigh_run16(const MtDev& d, uint4* nl4, const
 
 template <int MODE>
 __device__ __forceinline__ void mt_neigh_block_atoms16(const MtDev& d, const MtParams& p, float4* spos, int* smol,
-                                                       const int* s_first, const int* s_cnt, const int* s_off, int hxo,
-                                                       int hyo, int hzo, int l, int f0, int cnt_c, int cnt_max)
+                                                       const int* s_first, const int* s_cnt, const int* s_off,
+                                                       const int* s_hfirst, const int* s_hcnt)
 {
     const int t = threadIdx.x;
     const int n = d.n;
@@ -588,14 +588,23 @@ __device__ __forceinline__ void mt_neigh_block_atoms16(const MtDev& d, const MtP
         }
     }
     __syncthreads();
-    // the home cell's place among the staged cells
-    const int qh = (hxo + 1) + 4 * ((hyo + 1) + 4 * (hzo + 1));
-    for (int ib = 0; ib < cnt_max; ib += 32) {
-        const bool act = ib + l < cnt_c;
-        if (!act) continue;
-        const int uk = f0 + ib + l;                   // place in cell order
+    // the block's home atoms, cell after cell, over all the workgroup's lanes
+    // (a lane per atom, as k_fast_force_tile takes them): a home cell with
+    // more atoms than a half-wave holds costs no extra pass of the wavefront
+    int hoff[9];
+    hoff[0] = 0;
+#pragma unroll
+    for (int h = 0; h < 8; h++) hoff[h + 1] = hoff[h] + s_hcnt[h];
+    for (int v = t; v < hoff[8]; v += blockDim.x) {
+        int h = 0;
+        while (h < 7 && v >= hoff[h + 1]) h++;
+        const int m = v - hoff[h];                    // place in the home cell
+        const int hxo = h & 1, hyo = (h >> 1) & 1, hzo = h >> 2;
+        // the home cell's place among the staged cells
+        const int qh = (hxo + 1) + 4 * ((hyo + 1) + 4 * (hzo + 1));
+        const int uk = s_hfirst[h] + m;               // place in cell order
         const int k = d.cell_list[uk];
-        const int lself = s_off[qh] + ib + l;         // own staged place (the pad word)
+        const int lself = s_off[qh] + m;              // own staged place (the pad word)
         const uint2 hv = sxz[lself];
         const mt_h2 pxy = __builtin_bit_cast(mt_h2, hv.x);
         const float pz = __uint_as_float(hv.y);
@@ -681,8 +690,7 @@ __global__ void __launch_bounds__(256) k_fast_neigh_block(MtDev d, MtParams p, i
     // the word format of this block's lists (uniform; k_fast_force_tile makes
     // the same choice from the same counts)
     if (d.tile16 && s_off[64] <= NB_CAP)
-        mt_neigh_block_atoms16<MODE>(d, p, spos, smol_buf, s_first, s_cnt, s_off, hxo, hyo, hzo, l, f0, cnt_c,
-                                     cnt_max);
+        mt_neigh_block_atoms16<MODE>(d, p, spos, smol_buf, s_first, s_cnt, s_off, s_hfirst, s_hcnt);
     else
         mt_neigh_block_atoms<MODE>(d, p, spos, smol_buf, s_first, s_cnt, s_off, hxo, hyo, hzo, l, f0, cnt_c, cnt_max);
 }

@@ -26,6 +26,9 @@ import numpy as np  # noqa: E402
 
 METRIC = "atom-steps/s (1M LJ, 30k water), mixed vs FP64 on same GPU at stated drift"
 HBM_PEAK = 8.0e12          # MI355X HBM3E, bytes/s
+EQ_ROUNDS, EQ_STEPS = 10, 200      # equilibration of the 1M-atom mixture (NVE rounds rescaled to T* = 1)
+DRIFT_STEPS, DRIFT_EVERY = 10000, 1000
+WATER_EQ_ROUNDS, WATER_EQ_STEPS, WATER_DRIFT_STEPS = 5, 200, 1000
 
 
 def log(*a):
@@ -112,6 +115,23 @@ def energy_drift(th0, th1):
     return abs(e1 - e0) / abs(e0), abs(s1 - s0) / abs(s0)
 
 
+def host_threads() -> int:
+    """Host threads for the all-core CPU baseline: OMP_NUM_THREADS when set
+    (the share of the host this process may use), else the CPU count."""
+    v = os.environ.get("OMP_NUM_THREADS", "")
+    return int(v) if v.isdigit() and int(v) > 0 else (os.cpu_count() or 1)
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def timed_steps(eng, k):
     eng.synchronize()
     t0 = time.perf_counter()
@@ -171,41 +191,42 @@ def main() -> int:
     n = sysA.n_atoms
     log(f"bench: generated {n} atoms in {time.perf_counter() - t_gen:.1f}s, box {sysA.box[0]:.3f}")
 
-    # warm-up: gradient descent on the random packing (moldyn optimizeenergy),
-    # W capped NVE steps, then the momenta rescaled to T* = 1 with the
-    # reference's initial-temperature scaling (SimulationBox.cpp:124-132).
-    # (An NVT warm-up is avoided: the reference's Nose-Hoover chain, kept here
-    # operation for operation, resets p_eta[0] > 15 to 10 whatever N is and
-    # becomes unstable far from equilibrium in large systems.)
+    # equilibration, the same whatever --steps/--warmup say: gradient descent
+    # on the random packing (moldyn optimizeenergy), then EQ_ROUNDS rounds of
+    # EQ_STEPS NVE steps, each followed by a rescale of the momenta to T* = 1
+    # with the reference's initial-temperature scaling (SimulationBox.cpp:124-132).
+    # (No NVT here: the reference's Nose-Hoover chain, kept operation for
+    # operation, resets p_eta[0] > 15 to 10 whatever N is, which leaves a
+    # million-atom system almost unthermostatted.)
     cw = lj_config(abi.ENGINE_MIXED, abi.NVE)
     cw.block_size = args.block
     warm = Engine(sysA, cw)
     warm.optimize(100, 2e-5, 0.0)
     if args.kernels_only:
         return kernels_only(args, warm, sysA)
-    for rnd in range(3):
-        warm.step(Wm)
+    t_eq = time.perf_counter()
+    for rnd in range(EQ_ROUNDS):
+        warm.step(EQ_STEPS)
         pos, mom = warm.get_state()
         thw = warm.thermo()
         mom *= math.sqrt(120.272 / thw["T"])
         warm.set_state(mom=mom)
-        log(f"bench: warm-up round {rnd}: {Wm} NVE steps, T = {thw['T']:.2f} K -> rescaled to 120.272 K")
+    log(f"bench: equilibrated: {EQ_ROUNDS} x {EQ_STEPS} NVE steps rescaled to T* = 1 "
+        f"(last T before rescale {thw['T']:.2f} K) in {time.perf_counter() - t_eq:.1f}s")
     warm.close()
     sysA.pos, sysA.mom = pos, mom
+    pos_eq, mom_eq = pos.copy(), mom.copy()
 
-    # headline: MIXED engine, NVE (configuration 3), timed K steps
+    # headline: MIXED engine, NVE (configuration 3): W untimed steps, K timed
     cm = lj_config(abi.ENGINE_MIXED, abi.NVE)
     cm.block_size = args.block
     em = Engine(sysA, cm)
-    em.step(20)                                     # first list build, sort, caches
-    th0 = em.thermo()
+    em.step(max(1, Wm))                             # first list build, sort, graph capture
     st0 = em.stats()
     dt_m = timed_steps(em, K)
-    th1 = em.thermo()
     st1 = em.stats()
     value = n * K / dt_m
     rebuild_every = K / max(1, st1["rebuilds"] - st0["rebuilds"])
-    drift, drift_s = energy_drift(th0, th1)
     log(f"bench: mixed {dt_m / K * 1e3:.3f} ms/step, {value:.4g} atom-steps/s, rebuild every {rebuild_every:.1f}")
 
     # dominant kernel: the pair-force pass, timed with HIP events on the engine's stream
@@ -227,7 +248,7 @@ def main() -> int:
 
     # FP64 engine on the same state and the force error of MIXED against it
     pos_m, mom_m = em.get_state()
-    F_mixed, _ = em.compute_forces()
+    F_mixed, U_mixed = em.compute_forces()
     em.close()
     c64 = lj_config(abi.ENGINE_FP64, abi.NVE)
     c64.block_size = args.block
@@ -236,16 +257,38 @@ def main() -> int:
     F64, _ = e64.compute_forces()
     force_rms = rms_rel(F_mixed, F64)
     e64.step(10)
-    th64_0 = e64.thermo()
     K64 = max(50, K // 5)
     dt_64 = timed_steps(e64, K64)
     value64 = n * K64 / dt_64
-    drift64, drift64_s = energy_drift(th64_0, e64.thermo())
     t_force64 = e64.bench_kernel(0, 10) * 1e-3
     e64.close()
     log(f"bench: fp64 {dt_64 / K64 * 1e3:.3f} ms/step, {value64:.4g} atom-steps/s; force rms rel {force_rms:.2e}")
 
-    water, water_F = {}, {}
+    # NVE drift: DRIFT_STEPS steps of each engine from the same equilibrated
+    # state, the energies read every DRIFT_EVERY steps
+    drift = {}
+    for name, eng_id in (("mixed", abi.ENGINE_MIXED), ("fp64", abi.ENGINE_FP64)):
+        sysA.pos, sysA.mom = pos_eq, mom_eq
+        cd = lj_config(eng_id, abi.NVE)
+        cd.block_size = args.block
+        ed = Engine(sysA, cd)
+        th0 = ed.thermo()
+        worst, worst_s = 0.0, 0.0
+        t0 = time.perf_counter()
+        for blk in range(DRIFT_STEPS // DRIFT_EVERY):
+            ed.step(DRIFT_EVERY)
+            d_raw, d_shift = energy_drift(th0, ed.thermo())
+            worst, worst_s = max(worst, d_raw), max(worst_s, d_shift)
+        th1 = ed.thermo()
+        d_raw, d_shift = energy_drift(th0, th1)
+        drift[name] = {"steps": DRIFT_STEPS, "drift_rel": d_raw, "drift_shifted_rel": d_shift,
+                       "max_over_trace_rel": worst, "max_over_trace_shifted_rel": worst_s,
+                       "T_start_K": th0["T"], "T_end_K": th1["T"], "seconds": time.perf_counter() - t0}
+        ed.close()
+        log(f"bench: {name} NVE drift over {DRIFT_STEPS} steps: {d_raw:.2e} (shifted {d_shift:.2e}), "
+            f"T {th0['T']:.1f} -> {th1['T']:.1f} K")
+
+    water = {}
     if not args.skip_water:
         from moltwister_amd.engine import MtmdError
         sysW = W.water_box(10000, seed=2)
@@ -254,31 +297,59 @@ def main() -> int:
         cwr.coulomb = abi.COUL_ALLPAIRS
         relax = Engine(sysW, cwr)
         relax.optimize(300, 1e-4, 0.0)
-        sysW.pos, _ = relax.get_state()
+        pos_w0, _ = relax.get_state()
         relax.close()
-        for eng_id, coul, name in ((abi.ENGINE_MIXED, abi.COUL_ALLPAIRS, "mixed_allpairs"),
-                                   (abi.ENGINE_FP64, abi.COUL_ALLPAIRS, "fp64_allpairs"),
-                                   (abi.ENGINE_MIXED, abi.COUL_CUTOFF, "mixed_cutoff10"),
-                                   (abi.ENGINE_FP64, abi.COUL_CUTOFF, "fp64_cutoff10")):
-            cwt = water_config(eng_id)
-            cwt.coulomb = coul
-            cwt.block_size = args.block
-            try:
-                ew = Engine(sysW, cwt)
-                if coul == abi.COUL_ALLPAIRS:
-                    water_F[eng_id] = ew.compute_forces()[0]
-                ew.step(20)
-                kw = 200 if eng_id == abi.ENGINE_MIXED else 100
-                dtw = timed_steps(ew, kw)
-                water[name] = sysW.n_atoms * kw / dtw
-                water[name + "_T"] = ew.thermo()["T"]
-                ew.close()
-            except MtmdError as ex:
-                water[name] = None
-                water[name + "_error"] = str(ex)
-        if len(water_F) == 2:
-            water["force_rms_rel_mixed_vs_fp64_allpairs"] = rms_rel(water_F[abi.ENGINE_MIXED],
-                                                                    water_F[abi.ENGINE_FP64])
+        for coul, mode in ((abi.COUL_ALLPAIRS, "allpairs"), (abi.COUL_CUTOFF, "cutoff10")):
+            # thermalise at 298 K in this Coulomb mode: NVE rounds on the MIXED
+            # engine, each followed by a rescale of the momenta to 298 K
+            sysW.pos = pos_w0
+            cth = water_config(abi.ENGINE_MIXED)
+            cth.coulomb = coul
+            cth.ensemble = abi.NVE
+            eth = Engine(sysW, cth)
+            for _ in range(WATER_EQ_ROUNDS):
+                eth.step(WATER_EQ_STEPS)
+                pw, mw = eth.get_state()
+                mw *= math.sqrt(298.0 / eth.thermo()["T"])
+                eth.set_state(mom=mw)
+            eth.close()
+            sysW.pos, sysW.mom = pw, mw
+            F_w = {}
+            for eng_id, ename in ((abi.ENGINE_MIXED, "mixed"), (abi.ENGINE_FP64, "fp64")):
+                key = f"{ename}_{mode}"
+                cwt = water_config(eng_id)
+                cwt.coulomb = coul
+                cwt.block_size = args.block
+                try:
+                    ew = Engine(sysW, cwt)
+                    F_w[ename] = ew.compute_forces()[0]
+                    T0 = ew.thermo()["T"]
+                    ew.step(20)
+                    kw = 200 if eng_id == abi.ENGINE_MIXED else 100
+                    dtw = timed_steps(ew, kw)
+                    water[key] = {"atom_steps_per_s": sysW.n_atoms * kw / dtw, "ms_per_step": dtw / kw * 1e3,
+                                  "nvt_steps": kw, "T_start_K": T0, "T_end_K": ew.thermo()["T"]}
+                    ew.close()
+                    # NVE drift from the same thermalised state
+                    cwe = water_config(eng_id)
+                    cwe.coulomb = coul
+                    cwe.ensemble = abi.NVE
+                    cwe.block_size = args.block
+                    ee = Engine(sysW, cwe)
+                    th0 = ee.thermo()
+                    ee.step(WATER_DRIFT_STEPS)
+                    d_raw, d_shift = energy_drift(th0, ee.thermo())
+                    water[key].update({"nve_drift_steps": WATER_DRIFT_STEPS, "nve_drift_rel": d_raw,
+                                       "nve_drift_shifted_rel": d_shift})
+                    ee.close()
+                except MtmdError as ex:
+                    water[key] = {"error": str(ex)}
+            if len(F_w) == 2:
+                water[f"force_rms_rel_mixed_vs_fp64_{mode}"] = rms_rel(F_w["mixed"], F_w["fp64"])
+        water["workload"] = ("10,000 flexible 3-site waters (30,000 atoms), 1 g/cm^3, L = 66.88 A; 300 descent "
+                             f"steps, then per Coulomb mode {WATER_EQ_ROUNDS} x {WATER_EQ_STEPS} NVE steps each "
+                             "rescaled to 298 K; timed Nose-Hoover NVT at 298 K (tau 20 fs, M 4, n 4), dt 0.5 fs; "
+                             f"NVE drift over {WATER_DRIFT_STEPS} steps from the same state")
         log(f"bench: water {water}")
 
     # configuration 1: the reference's CPU-runnable case on the GPU engines
@@ -394,30 +465,53 @@ def main() -> int:
         log(f"bench: force matching {fmres}")
 
     cpu = None
+    oracle_check = None
     if not args.skip_cpu:
         from oracle import pyoracle
         c_ref = lj_config(abi.ENGINE_EXACT, abi.NVE)
+        threads = host_threads()
+        # the MIXED and FP64 forces of the bench state (the timed run's end)
+        # against the oracle's, at full size
+        sysA.pos, sysA.mom = pos_m, mom_m
+        o1m = pyoracle.Oracle(sysA, c_ref)
+        o1m.set_threads(threads)
+        t0 = time.perf_counter()
+        F_orc, U_orc = o1m.compute_forces()
+        t_orc = time.perf_counter() - t0
+        del o1m
+        scale = np.maximum(np.linalg.norm(F_orc, axis=1), np.sqrt(np.mean(np.sum(F_orc ** 2, 1))))
+        oracle_check = {"mixed_force_rms_rel": rms_rel(F_mixed, F_orc),
+                        "mixed_force_max_rel_per_atom": float(np.max(np.linalg.norm(F_mixed - F_orc, axis=1) / scale)),
+                        "fp64_force_max_rel_per_atom": float(np.max(np.linalg.norm(F64 - F_orc, axis=1) / scale)),
+                        "mixed_U_rel": abs(float(U_mixed.sum(dtype=np.float64)) - float(U_orc.sum(dtype=np.float64)))
+                        / float(np.abs(U_orc).sum(dtype=np.float64)),
+                        "oracle_seconds": t_orc, "oracle_threads": threads}
+        log(f"bench: 1M state vs oracle {oracle_check}")
         # bounded sample: the 1/8 sub-box (47.8 sigma, 131,072 atoms at the same
-        # density) of the warmed-up state, so the per-atom work is that of the workload
+        # density) of the timed run's end state, so the per-atom work is that of the workload
         Lh = sysA.box[0] / 2.0
         sel = np.all((pos_m >= -Lh) & (pos_m < 0.0), axis=1)
         from moltwister_amd.system import System
         sub = System(pos=pos_m[sel] + Lh / 2.0, mom=mom_m[sel], mass=sysA.mass[sel], charge=sysA.charge[sel],
                      types=sysA.types[sel], box=(Lh, Lh, Lh), nonbonded=sysA.nonbonded)
-        o = pyoracle.Oracle(sub, c_ref)
-        t0 = time.perf_counter()
-        o.compute_forces()
-        t_first = time.perf_counter() - t0
-        nst = 2
-        t0 = time.perf_counter()
-        o.step(nst)
-        t_cpu = time.perf_counter() - t0
-        cpu = {"value": sub.n_atoms * nst / t_cpu, "unit": "atom-steps/s", "cores": 1, "kind": "port",
-               "sample": f"{nst} velocity-Verlet steps of a {sub.n_atoms}-atom sub-box (1/8 of the warmed-up "
-                         f"1,048,576-atom state, same density) with the oracle: the reference algorithm, single "
-                         f"thread, lists rebuilt at rc every step, its all-pairs Coulomb pass skipped because "
-                         f"every charge is zero; the first force call ({t_first:.1f}s) not counted"}
-        log(f"bench: cpu oracle {t_cpu / nst:.2f} s/step on {sub.n_atoms} atoms")
+        legs = {}
+        for nth, nst in ((1, 2), (threads, 8)):
+            o = pyoracle.Oracle(sub, c_ref)
+            o.set_threads(nth)
+            o.compute_forces()
+            t0 = time.perf_counter()
+            o.step(nst)
+            legs[nth] = (sub.n_atoms * nst / (time.perf_counter() - t0), nst)
+            del o
+        cpu = {"value": legs[1][0], "unit": "atom-steps/s", "cores": 1, "kind": "port", "cpu_model": cpu_model(),
+               "sample": f"{legs[1][1]} velocity-Verlet steps of a {sub.n_atoms}-atom sub-box (1/8 of the timed "
+                         f"run's 1,048,576-atom end state, same density) with the oracle: the reference algorithm, "
+                         f"single thread, lists rebuilt at rc every step, its all-pairs Coulomb pass skipped "
+                         f"because every charge is zero; the first force call not counted",
+               "all_cores": {"value": legs[threads][0], "unit": "atom-steps/s", "cores": threads, "kind": "port",
+                             "sample": f"{legs[threads][1]} steps of the same sub-box, the oracle's per-atom loops "
+                                       f"(neighbour rows, forces) on {threads} OpenMP threads, same results"}}
+        log(f"bench: cpu oracle {cpu['value']:.4g} atom-steps/s on 1 core, {legs[threads][0]:.4g} on {threads}")
         # MolTwister's own force path (oracle/_ref, built from its sources) on
         # configuration 1, where it runs in seconds: cell list, neighbour list
         # and the force functor including its all-pairs Coulomb loop
@@ -451,8 +545,9 @@ def main() -> int:
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": "binary LJ 80:20, 1,048,576 atoms, rho*=1.2, rc=2.5 sigma, skin 0.3, NVE, dt=0.005 tau, "
-                        "RSA start (min sep 0.8 sigma), 100 descent steps, 3 x W NVE steps each rescaled to T*=1",
+            "workload": "binary LJ 80:20, 1,048,576 atoms, rho*=1.2, rc=2.5 sigma, skin 0.3, NVE, dt=0.005 tau; "
+                        "start: a 131,072-atom RSA box (min sep 0.8 sigma) tiled 2x2x2 with per-atom types and "
+                        "momenta, 100 descent steps, 10 x 200 NVE steps each rescaled to T*=1, then W untimed steps",
             "engine": "mixed: f32 pair math, f64 accumulation/integration",
             "parallelism": "single GPU",
         },
@@ -469,20 +564,20 @@ def main() -> int:
             "kernel_ms": t_force * 1e3,
             "bytes_per_atom": b_force,
             "valu": valu_issue(t_force),
+            "limiter": "VALU issue and LDS reads of the staged positions (see valu); the fraction above is of "
+                       "the HBM peak, the roofline the kernel's algorithmic bytes are quoted against",
         },
         "cpu_baseline": cpu,
         "fp64": {"value": value64, "ms_per_step": dt_64 / K64 * 1e3, "steps": K64,
                  "force_kernel_ms": t_force64 * 1e3, "mixed_speedup": value / value64},
         "precision": {"force_rms_rel_vs_fp64": force_rms,
-                      "nve_drift_rel": drift, "nve_drift_steps": K,
-                      "nve_drift_shifted_rel": drift_s,
-                      "nve_drift_per_1e4_steps_linear": drift * 1e4 / K,
-                      "nve_drift_shifted_per_1e4_steps_linear": drift_s * 1e4 / K,
-                      "fp64_nve_drift_rel": drift64, "fp64_nve_drift_shifted_rel": drift64_s,
-                      "fp64_drift_steps": K64,
-                      "note": "drift = |E(K)-E(0)|/|E(0)| over the timed steps; 'shifted' uses Etot - U_cut, "
-                              "which has no jumps from pairs crossing the unshifted cutoff; per-1e4 values "
-                              "are linear extrapolations (an upper bound for a random walk)"},
+                      "vs_oracle": oracle_check,
+                      "nve_drift_steps": DRIFT_STEPS,
+                      "mixed": drift["mixed"], "fp64": drift["fp64"],
+                      "note": "drift = |E(t)-E(0)|/|E(0)| after DRIFT_STEPS NVE steps of each engine from the same "
+                              "equilibrated state (max_over_trace: the largest value at the 1000-step checkpoints); "
+                              "'shifted' uses Etot - U_cut, which has no jumps from pairs crossing the unshifted "
+                              "cutoff"},
         "step_roofline": {"bytes_per_atom_step": step_bytes, "frac": value * step_bytes / HBM_PEAK,
                           "rebuild_every_steps": rebuild_every, "list_build_ms": t_build * 1e3,
                           "copy_GBps": copy_bw / 1e9},

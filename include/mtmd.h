@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define MTMD_ABI_VERSION 2
+#define MTMD_ABI_VERSION 3
 
 /* Number of points in every force/energy profile and the maximum number of
  * non-bonded force fields per atom-type pair
@@ -220,11 +220,13 @@ typedef struct mtmd_thermo {
     double sum_p2m;  /* sum p.p/m (reduced) */
     double sum_fr;   /* sum F.r   (reduced) */
     double U_cut;    /* kJ/mol: sum over pairs inside the cutoff of the pair
-                        energy at the cutoff (scaled like the pair). The pair
-                        potentials are truncated without a shift, so U jumps
-                        by U(rc) when a pair crosses rc; Etot - U_cut does not
-                        jump and is what the integrator conserves. No
-                        counterpart in the reference.                        */
+                        energy at the cutoff (scaled like the pair): the
+                        tabulated terms and, with MTMD_COUL_CUTOFF, the
+                        Coulomb term K qi qj / rc. The pair potentials are
+                        truncated without a shift, so U jumps by U(rc) when a
+                        pair crosses rc; Etot - U_cut does not jump and is
+                        what the integrator conserves. No counterpart in the
+                        reference.                                           */
 } mtmd_thermo;
 
 typedef struct mtmd_handle mtmd_handle;
@@ -318,6 +320,17 @@ int mtmd_optimize(mtmd_handle* h, int32_t max_steps, double learning_rate,
 /* Diagnostics: neighbour-list rebuilds since create, current list capacity. */
 int mtmd_get_stats(mtmd_handle* h, int64_t* rebuilds, int64_t* steps,
                    int32_t* max_neighbors);
+
+/* Diagnostics, no counterpart in the reference (whose lists are the public
+ * members cellList_ / neighList_ of CMDFFMatrices, MDFFMatrices.h:228-242):
+ * the FP64/MIXED engine's own neighbour lists as of its last build, at the
+ * list radius cutoff + skin, in the engine's word order, as the caller's atom
+ * indices (16-bit block-list words are decoded through their block's staged
+ * cells). count [n]: words of each row; list [n*max_per_atom] row-major or
+ * NULL; *required receives the largest count. Builds the lists first when
+ * there are none. MTMD_EINVAL on the EXACT engine (mtmd_build_neighbors gives
+ * its lists); MTMD_ENEIGH when a row does not fit max_per_atom. */
+int mtmd_get_skin_lists(mtmd_handle* h, int32_t max_per_atom, int32_t* count, int32_t* list, int32_t* required);
 
 /* Blocks until all queued device work of the handle is done. */
 int mtmd_synchronize(mtmd_handle* h);

@@ -1973,8 +1973,17 @@ void mt_fast_kick_drift(const MtDev& d, const MtParams& p, int scale_slot, int m
 // lists hold plain atom indices at rc, the FP64/MIXED lists coded words at
 // rc + skin plus the special pairs.
 // ---------------------------------------------------------------------------
+// the cutoff Coulomb energy of a pair at the cutoff, K qk qa / rc in FP32 as
+// the pair term forms it (FunctorCalcForce.cpp:258-278); 0 unless Coulomb is
+// on and truncated at rc
+__device__ __forceinline__ float mt_ucut_coul(const MtParams& p, float qk, float qa)
+{
+    return (p.coulomb_on && !p.coulomb_allpairs) ? MT_COULOMB_K * qk * qa / p.rc : 0.0f;
+}
+
 // the special pairs (1-2/1-3/1-4 and same-molecule pairs) of slot k inside rc
-__device__ __forceinline__ double mt_ucut_special(const MtDev& d, const MtParams& p, int k, const mt_d3& rk, int row)
+__device__ __forceinline__ double mt_ucut_special(const MtDev& d, const MtParams& p, int k, const mt_d3& rk, int row,
+                                                  float qk)
 {
     double u = 0.0;
     if (d.sp.n_entries > 0) {
@@ -1983,7 +1992,8 @@ __device__ __forceinline__ double mt_ucut_special(const MtDev& d, const MtParams
             const int a = d.inv[d.sp.partner[d.sp.first[o] + j]];
             const double4 ra4 = d.pos[a];
             if (mt_dist2_pbc(rk, {ra4.x, ra4.y, ra4.z}, p.wnl) < (double)p.rc2f)
-                u += 0.5 * (double)(d.ecut[row * d.T + d.type[a]] * d.sp.scale[d.sp.first[o] + j]);
+                u += 0.5 * (double)((d.ecut[row * d.T + d.type[a]] + mt_ucut_coul(p, qk, (float)ra4.w)) *
+                                    d.sp.scale[d.sp.first[o] + j]);
         }
     }
     return u;
@@ -2008,13 +2018,14 @@ __global__ void __launch_bounds__(TPB) k_ucut(MtDev d, MtParams p, int exact)
         if (!d.mobile[k]) continue;
         const double4 rk4 = d.pos[k];
         const mt_d3 rk = {rk4.x, rk4.y, rk4.z};
+        const float qk = (float)rk4.w;
         const int T = d.T, row = d.type[k], molk = d.mol[k];
         const int cnt = min(d.ncount[k], d.ncap);
         for (int j = 0; j < cnt; j++) {
             const int a = exact ? (int)d.nlist[(size_t)j * d.n + k]
                                 : mt_id_slot(d, (int)(mt_fast_word(d.nlist, d.n, k, j) & MT_NW_IDX_MASK));
             const double4 ra4 = d.pos[a];
-            float e = d.ecut[row * T + d.type[a]];
+            float e = d.ecut[row * T + d.type[a]] + mt_ucut_coul(p, qk, (float)ra4.w);
             if (exact) {
                 mt_d3 f = {0.0, 0.0, 0.0};
                 mt_scale_pair(d, p, f, e, a, molk, k);
@@ -2023,7 +2034,7 @@ __global__ void __launch_bounds__(TPB) k_ucut(MtDev d, MtParams p, int exact)
             }
             u += 0.5 * (double)e;
         }
-        if (!exact) u += mt_ucut_special(d, p, k, rk, row);
+        if (!exact) u += mt_ucut_special(d, p, k, rk, row, qk);
     }
     mt_block_sum_to(u, d.red + blockIdx.x);
 }
@@ -2040,6 +2051,7 @@ __global__ void __launch_bounds__(TPB) k_ucut_tile(MtDev d, MtParams p)
 {
     __shared__ double s_x[FT_CAP], s_y[FT_CAP], s_z[FT_CAP];
     __shared__ int s_ty[FT_CAP];
+    __shared__ float s_q[FT_CAP];
     __shared__ int s_first[64], s_cnt[64], s_off[65];
     __shared__ int s_hfirst[8], s_hoff[9];
     __shared__ float s_ecut[MT_UCUT_TT];
@@ -2064,6 +2076,7 @@ __global__ void __launch_bounds__(TPB) k_ucut_tile(MtDev d, MtParams p)
                 s_y[i] = r.y;
                 s_z[i] = r.z;
                 s_ty[i] = d.type[a];
+                s_q[i] = (float)r.w;
             }
             __syncthreads();
         }
@@ -2074,6 +2087,7 @@ __global__ void __launch_bounds__(TPB) k_ucut_tile(MtDev d, MtParams p)
             if (!d.mobile[k]) continue;
             const double4 rk4 = d.pos[k];
             const mt_d3 rk = {rk4.x, rk4.y, rk4.z};
+            const float qk = (float)rk4.w;
             const int row = d.type[k];
             const int cnt = min(d.ncount[k], w16 ? 2 * d.ncap : d.ncap);
             if (w16) {
@@ -2087,7 +2101,7 @@ __global__ void __launch_bounds__(TPB) k_ucut_tile(MtDev d, MtParams p)
                         if (j0 + i >= cnt) break;
                         const int l = (int)((wv[i >> 1] >> (16 * (i & 1))) & 0xffffu);
                         if (mt_dist2_pbc(rk, {s_x[l], s_y[l], s_z[l]}, p.wnl) < (double)p.rc2f)
-                            u += 0.5 * (double)s_ecut[row * d.T + s_ty[l]];
+                            u += 0.5 * (double)(s_ecut[row * d.T + s_ty[l]] + mt_ucut_coul(p, qk, s_q[l]));
                     }
                 }
             } else {
@@ -2095,10 +2109,10 @@ __global__ void __launch_bounds__(TPB) k_ucut_tile(MtDev d, MtParams p)
                     const int a = mt_id_slot(d, (int)(mt_fast_word(d.nlist, d.n, k, j) & MT_NW_IDX_MASK));
                     const double4 ra4 = d.pos[a];
                     if (mt_dist2_pbc(rk, {ra4.x, ra4.y, ra4.z}, p.wnl) < (double)p.rc2f)
-                        u += 0.5 * (double)d.ecut[row * d.T + d.type[a]];
+                        u += 0.5 * (double)(d.ecut[row * d.T + d.type[a]] + mt_ucut_coul(p, qk, (float)ra4.w));
                 }
             }
-            u += mt_ucut_special(d, p, k, rk, row);
+            u += mt_ucut_special(d, p, k, rk, row, qk);
         }
     }
     mt_block_sum_to(u, d.red + blockIdx.x);
@@ -2114,6 +2128,62 @@ int mt_ucut(const MtDev& d, const MtParams& p, int exact, hipStream_t s)
     }
     hipLaunchKernelGGL(k_ucut, dim3(mt_reduce_blocks(d.n)), dim3(TPB), 0, s, d, p, exact);
     return mt_reduce_blocks(d.n);
+}
+
+// The lists of the last build in the caller's atom indices: row perm[k] of
+// `out` (max_per_atom entries) receives atom k's neighbours in list order,
+// cnt[perm[k]] its word count. Workgroups take the 2x2x2 blocks as the list
+// build does, so a block's word format (16-bit staged places or 32-bit coded
+// words) is decided from the same cell counts; a 16-bit word is mapped back
+// through the block's staged cells to its cell-order place, then its slot.
+__global__ void __launch_bounds__(TPB) k_fast_decode_lists(MtDev d, int* __restrict__ out, int* __restrict__ cnt_out,
+                                                           int maxp)
+{
+    __shared__ int s_first[64], s_cnt[64], s_off[65];
+    __shared__ int s_hfirst[8], s_hoff[9];
+    const int t = threadIdx.x;
+    const int bx = (d.nc[0] + 1) >> 1, by = (d.nc[1] + 1) >> 1, bz = (d.nc[2] + 1) >> 1;
+    const int nb = bx * by * bz;
+    for (int g = blockIdx.x; g < nb; g += gridDim.x) {
+        __syncthreads();                              // the previous block's tables are read
+        mt_tile_cells(d, 2 * (g % bx), 2 * ((g / bx) % by), 2 * (g / (bx * by)), s_first, s_cnt, s_off, s_hfirst,
+                      s_hoff);
+        __syncthreads();
+        const bool w16 = d.tile16 && s_off[64] <= NB_CAP;
+        const int nh = s_hoff[8];
+        for (int v = t; v < nh; v += blockDim.x) {
+            const int h = mt_tile_home(s_hoff, v);
+            const int k = d.cell_list[s_hfirst[h] + (v - s_hoff[h])];
+            const int ok = d.perm[k];
+            const int c = d.ncount[k];
+            cnt_out[ok] = c;
+            const int m = min(min(c, w16 ? 2 * d.ncap : d.ncap), maxp);
+            int* row = out + (size_t)ok * maxp;
+            for (int j = 0; j < m; j++) {
+                int slot;
+                if (w16) {
+                    const uint32_t w = d.nlist[(((size_t)(j >> 3) * d.n + k) << 2) + ((j & 7) >> 1)];
+                    const int l = (int)((w >> (16 * (j & 1))) & 0xffffu);
+                    int lo = 0, hi = 64;
+                    while (hi - lo > 1) {
+                        const int mid = (lo + hi) >> 1;
+                        if (s_off[mid] <= l) lo = mid; else hi = mid;
+                    }
+                    slot = d.cell_list[s_first[lo] + (l - s_off[lo])];
+                } else {
+                    slot = mt_id_slot(d, (int)(mt_fast_word(d.nlist, d.n, k, j) & MT_NW_IDX_MASK));
+                }
+                row[j] = d.perm[slot];
+            }
+        }
+    }
+}
+
+void mt_fast_decode_lists(const MtDev& d, int* out, int* cnt, int maxp, hipStream_t s)
+{
+    const int nb = ((d.nc[0] + 1) / 2) * ((d.nc[1] + 1) / 2) * ((d.nc[2] + 1) / 2);
+    hipLaunchKernelGGL(k_fast_decode_lists, dim3(std::max(1, std::min(nb, 4096))), dim3(TPB), 0, s, d, out, cnt,
+                       maxp);
 }
 
 __global__ void k_fast_clear(unsigned int* flags) { flags[0] = 0u; }

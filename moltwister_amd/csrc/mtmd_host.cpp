@@ -1675,6 +1675,42 @@ extern "C" int mtmd_get_stats(mtmd_handle* h, int64_t* rebuilds, int64_t* steps,
     return MTMD_OK;
 }
 
+extern "C" int mtmd_get_skin_lists(mtmd_handle* h, int32_t max_per_atom, int32_t* count, int32_t* list,
+                                   int32_t* required)
+{
+    if (!h || max_per_atom < 0 || (list && max_per_atom == 0)) return MTMD_EINVAL;
+    if (!h->fast) return h->fail(MTMD_EINVAL, "skin lists belong to the fp64/mixed engines");
+    int rc;
+    if (!h->have_forces || h->need_sort) {
+        if ((rc = compute_forces_now(h))) return rc;
+    }
+    const int n = h->n;
+    const int maxp = list ? max_per_atom : 0;
+    int* d_out = nullptr;
+    int* d_cnt = nullptr;
+    if (hipMalloc((void**)&d_cnt, (size_t)n * sizeof(int)) != hipSuccess)
+        return h->fail(MTMD_ENOMEM, "list counts of %d atoms", n);
+    if (maxp && hipMalloc((void**)&d_out, (size_t)n * maxp * sizeof(int)) != hipSuccess) {
+        (void)hipFree(d_cnt);
+        return h->fail(MTMD_ENOMEM, "lists of %d x %d entries", n, maxp);
+    }
+    mt_fast_decode_lists(h->d, d_out, d_cnt, maxp, h->stream);
+    std::vector<int32_t> cnt(n);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(cnt.data(), d_cnt, (size_t)n * sizeof(int), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess && maxp)
+        e = hipMemcpyAsync(list, d_out, (size_t)n * maxp * sizeof(int), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    (void)hipFree(d_cnt);
+    if (d_out) (void)hipFree(d_out);
+    if (e != hipSuccess) return h->fail(MTMD_EHIP, "list decode: %s", hipGetErrorString(e));
+    int mx = 0;
+    for (int v : cnt) mx = std::max(mx, v);
+    if (required) *required = mx;
+    if (count) std::memcpy(count, cnt.data(), (size_t)n * sizeof(int32_t));
+    return (maxp && mx > maxp) ? h->fail(MTMD_ENEIGH, "row of %d neighbours exceeds %d", mx, maxp) : MTMD_OK;
+}
+
 extern "C" int mtmd_synchronize(mtmd_handle* h)
 {
     if (!h) return MTMD_EINVAL;

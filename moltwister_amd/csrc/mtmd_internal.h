@@ -167,6 +167,9 @@ void mt_fast_locate(const MtDev& d, const MtParams& p, int mixed, hipStream_t s)
 void mt_fast_nve_fused(const MtDev& d, const MtParams& p, int mixed, hipStream_t s);
 // in-place read + write of the whole neighbour-list buffer (HBM reference)
 void mt_copy_lists(const MtDev& d, hipStream_t s);
+// the lists of the last build in original atom indices: out[o * maxp + j]
+// (maxp 0: counts only), cnt[o] = word count of atom o
+void mt_fast_decode_lists(const MtDev& d, int* out, int* cnt, int maxp, hipStream_t s);
 // per-block partial sums of U_cut into d.red[0 .. returned count)
 int mt_ucut(const MtDev& d, const MtParams& p, int exact, hipStream_t s);
 

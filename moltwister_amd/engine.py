@@ -152,6 +152,17 @@ class Engine:
         self._chk(self.lib.mtmd_get_stats(self.h, C.byref(r), C.byref(s), C.byref(m)))
         return dict(rebuilds=r.value, steps=s.value, capacity=m.value)
 
+    def skin_lists(self):
+        """The FP64/MIXED engine's neighbour lists of its last build (radius
+        cutoff + skin), as (count [n], list [n, max count]) in atom indices."""
+        req = C.c_int32(0)
+        cnt = np.zeros(self.n, np.int32)
+        self._chk(self.lib.mtmd_get_skin_lists(self.h, 0, _i32(cnt), None, C.byref(req)))
+        w = max(int(req.value), 1)
+        lst = np.zeros((self.n, w), np.int32)
+        self._chk(self.lib.mtmd_get_skin_lists(self.h, w, _i32(cnt), _i32(lst), C.byref(req)))
+        return cnt, lst
+
     def bench_kernel(self, which: int, reps: int) -> float:
         """Average ms per launch (HIP events on the engine's stream):
         0 pair forces, 1 list build, 2 MD step, 3 streaming copy."""

@@ -493,6 +493,7 @@ typedef struct { float x, f, e; } pt;   /* CMDFFMatrices::CPoint */
 
 struct orc_md {
     int n, ntypes;
+    int threads;       /* host threads of the per-atom loops (1: sequential) */
     v3 *r, *p;
     double* m;
     float* q;
@@ -538,6 +539,8 @@ struct orc_md {
 };
 
 int orc_natoms(const orc_md* o) { return o->n; }
+
+void orc_set_threads(orc_md* o, int threads) { o->threads = threads > 1 ? threads : 1; }
 
 /* CMt::exp / CMt::sinhXoverX (Cmd/MolTwisterCmdMolDyn/Integrators/Math.cpp:27-54) */
 double orc_cmt_exp(double x)
@@ -596,6 +599,7 @@ orc_md* orc_create(const mtmd_system* s, const mtmd_config* cfg)
     int n = s->n_atoms, T = s->n_types, k, j, i, c, r;
     orc_md* o = (orc_md*)calloc(1, sizeof(orc_md));
     o->n = n; o->ntypes = T;
+    o->threads = 1;
     o->r = (v3*)calloc(n, sizeof(v3));
     o->p = (v3*)calloc(n, sizeof(v3));
     o->m = (double*)calloc(n, sizeof(double));
@@ -983,6 +987,9 @@ int orc_build_neighbors(orc_md* o, int32_t max_per_atom, int32_t* count, int32_t
 {
     int k, mx = 0;
     int* tmp;
+    /* rows are independent: with orc_set_threads(t > 1) they are built by t
+     * threads, each row by the same operations in the same order */
+#pragma omp parallel for schedule(dynamic, 256) reduction(max : mx) if (o->threads > 1) num_threads(o->threads)
     for (k = 0; k < o->n; k++) {
         int c = neigh_row(o, k, NULL, 0);
         o->ncount[k] = c;
@@ -991,6 +998,7 @@ int orc_build_neighbors(orc_md* o, int32_t max_per_atom, int32_t* count, int32_t
     free(o->nlist);
     o->maxn = mx > 0 ? mx : 1;
     o->nlist = (int*)calloc((size_t)o->n * o->maxn, sizeof(int));
+#pragma omp parallel for schedule(dynamic, 256) if (o->threads > 1) num_threads(o->threads)
     for (k = 0; k < o->n; k++) neigh_row(o, k, o->nlist + (size_t)k * o->maxn, o->maxn);
     if (count) memcpy(count, o->ncount, sizeof(int) * (size_t)o->n);
     if (list) {
@@ -1227,6 +1235,7 @@ static void compute_forces_internal(orc_md* o)
         double hi = Lf / 2.0, lo = -Lf / 2.0;
         wbox[k] = hi - lo;
     }
+#pragma omp parallel for schedule(dynamic, 256) if (o->threads > 1) num_threads(o->threads)
     for (k = 0; k < o->n; k++) force_atom(o, k, wbox, &o->F[k], &o->U[k]);
     o->have_forces = 1;
 }

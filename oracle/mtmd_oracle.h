@@ -47,6 +47,11 @@ int orc_profile(int which, int kind, const double* params, float x0, float x1,
 orc_md* orc_create(const mtmd_system* sys, const mtmd_config* cfg);
 void    orc_destroy(orc_md* o);
 int     orc_natoms(const orc_md* o);
+/* host threads for the per-atom loops of the neighbour-list build and the
+ * force functor (default 1). Every row and every atom's force is computed by
+ * the same operations in the same order whatever the thread count, so the
+ * results are identical; only the CPU baseline's all-core figure uses t > 1. */
+void    orc_set_threads(orc_md* o, int threads);
 
 void orc_set_state(orc_md* o, const double* pos, const double* mom);
 void orc_get_state(const orc_md* o, double* pos, double* mom);

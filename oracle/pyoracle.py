@@ -67,6 +67,7 @@ def _oracle_lib():
         L.orc_create.restype = _vp
         L.orc_create.argtypes = [C.POINTER(abi.MtmdSystem), C.POINTER(abi.MtmdConfig)]
         L.orc_destroy.argtypes = [_vp]
+        L.orc_set_threads.argtypes = [_vp, C.c_int]
         L.orc_set_state.argtypes = [_vp, _dp, _dp]
         L.orc_get_state.argtypes = [_vp, _dp, _dp]
         L.orc_set_thermostat.argtypes = [_vp, _dp, _dp]
@@ -178,6 +179,10 @@ class Oracle(_Base):
         if getattr(self, "h", None):
             self.lib.orc_destroy(self.h)
             self.h = None
+
+    def set_threads(self, threads: int):
+        """host threads of the per-atom loops (the results do not depend on it)"""
+        self.lib.orc_set_threads(self.h, int(threads))
 
     def set_state(self, pos=None, mom=None):
         pos = None if pos is None else np.ascontiguousarray(pos, np.float64)

@@ -209,6 +209,43 @@ def test_ucut_brute_force_and_engines_agree():
 
 
 @pytest.mark.parametrize("engine", [abi.ENGINE_EXACT, abi.ENGINE_FP64, abi.ENGINE_MIXED])
+def test_ucut_cutoff_coulomb_water(engine):
+    """U_cut with Coulomb truncated at rc: every pair of different molecules
+    inside rc adds its tabulated U(rc) and K qi qj / rc (FP32, as the pair
+    term forms it); pairs inside a water (1-2, 1-3) are scaled by 0. Against
+    an O(N^2) numpy sum with the same float-box predicate."""
+    from moltwister_amd.engine import profile
+    print(f"case ucut water cutoff coulomb engine={engine}", flush=True)
+    s, c = make("water_300", engine=engine, coulomb=abi.COUL_CUTOFF)
+    pos = s.pos + np.random.default_rng(9).normal(0, 0.02, s.pos.shape)
+    L = np.array(s.box)
+    d = pos[:, None, :] - pos[None, :, :]
+    d -= L * np.round(d / L)
+    r2 = np.sum(d * d, axis=2)
+    rcf = np.float32(c.cutoff_radius)
+    mol = np.arange(s.n_atoms) // 3
+    inside = (r2 < float(rcf) ** 2) & (mol[:, None] != mol[None, :])
+    ti = s.types
+    T = s.n_types
+    ecut = np.zeros((T, T), np.float32)
+    for a, b, kind, prm in s.nonbonded:
+        _, _, e = profile(0, kind, prm, 0.01, float(rcf))
+        ecut[a, b] += e[-1]
+        if a != b:
+            ecut[b, a] += e[-1]
+    q = s.charge.astype(np.float32)
+    ec = (np.float32(1389.32) * q[:, None] * q[None, :]) / rcf
+    terms = (ecut[ti[:, None], ti[None, :]] + ec).astype(np.float64) * inside
+    want = 0.5 * float(np.sum(terms))
+    e = Engine(s, c)
+    e.set_state(pos=pos)
+    e.compute_forces()
+    got = e.thermo()["U_cut"]
+    e.close()
+    assert abs(got - want) <= 1e-9 * float(np.sum(np.abs(terms))), (got, want)
+
+
+@pytest.mark.parametrize("engine", [abi.ENGINE_EXACT, abi.ENGINE_FP64, abi.ENGINE_MIXED])
 def test_profile_kind_matches_parameters(engine):
     """MTMD_FF_PROFILE force fields give bit-identical forces to the same force
     fields given by parameters, in every engine."""

@@ -1,0 +1,181 @@
+"""The MIXED tile path (k_fast_force_tile on 16-bit block lists, the list
+build's FP16 distance screen) and the FP64 engine at the sizes where the tile
+path runs, held directly to the oracle (the reference algorithm: cell list,
+neighbour list at rc, force functor; FunctorGenNeighList.cpp:72-151,
+FunctorCalcForce.cpp:159-281,467-512) on the same positions.
+
+The tile path runs when the grid has at least two 2x2x2 blocks of cells per
+compute unit: 131,072 atoms at rho* = 1.2 give 17^3 cells of rc + skin 0.3
+(9^3 = 729 blocks) and 16^3 at skin 0.45 (512 blocks). At skin 0.3 every block's 4x4x4 staged cells hold at most
+2048 atoms (16-bit words); at skin 0.45 some do not, so both word formats
+meet in one system.
+
+Bars: MIXED force RMS sqrt(sum|F - Fo|^2 / sum|Fo|^2) <= 1e-5 and per atom
+|F_i - Fo_i| <= 1e-3 max(|Fo_i|, F_rms); total U within 1e-5 of sum |U_i|.
+FP64 per atom <= 1e-12 max(|Fo_i|, F_rms). Lists: every pair the oracle's
+reference list holds (inside rc) is in the engine's skin list of that atom,
+with no duplicates, no self pairs and nothing beyond rlist plus the screen's
+stated margin."""
+import numpy as np
+import pytest
+
+from moltwister_amd import abi, config
+from moltwister_amd import workloads as W
+from moltwister_amd.engine import Engine
+
+pytestmark = pytest.mark.gpu
+
+N = 131072
+FT_CAP = 2048
+
+
+def rms_rel(F, Fr):
+    return float(np.sqrt(np.sum((F - Fr) ** 2) / np.sum(Fr ** 2)))
+
+
+def per_atom_rel(F, Fr):
+    scale = np.maximum(np.linalg.norm(Fr, axis=1), np.sqrt(np.mean(np.sum(Fr ** 2, 1))))
+    return np.linalg.norm(F - Fr, axis=1) / scale
+
+
+def lj_cfg(engine, skin):
+    c = config.defaults()
+    c.engine = engine
+    c.ensemble = abi.NVE
+    c.cutoff_radius = 2.5
+    c.neigh_shell = skin
+    c.skin = skin
+    c.timestep = 0.5
+    c.temperature = 120.272
+    c.cutoff_force = 1000.0
+    c.coulomb = abi.COUL_CUTOFF
+    return c
+
+
+def staged_totals(pos, L, rlist):
+    nc = int(np.floor(L / rlist))
+    c = np.floor((pos + L / 2.0) / (L / nc)).astype(np.int64) % nc
+    cnt = np.zeros((nc, nc, nc), np.int64)
+    np.add.at(cnt, (c[:, 0], c[:, 1], c[:, 2]), 1)
+    for ax in range(3):
+        cnt = sum(np.roll(cnt, -o, axis=ax) for o in (-1, 0, 1, 2))
+    return cnt[::2, ::2, ::2].ravel()
+
+
+@pytest.fixture(scope="module")
+def relaxed(oracle_mod):
+    """The mixture after 30 descent steps on the MIXED engine (the bench's own
+    start procedure), so the pair distances are those of a liquid."""
+    s = W.binary_lj_large(N, tiles=2, seed=5)
+    e = Engine(s, lj_cfg(abi.ENGINE_MIXED, 0.3))
+    e.optimize(30, 2e-5, 0.0)
+    s.pos, _ = e.get_state()
+    e.close()
+    return s
+
+
+@pytest.fixture(scope="module")
+def oracle_ref(oracle_mod, relaxed):
+    o = oracle_mod.Oracle(relaxed, lj_cfg(abi.ENGINE_EXACT, 0.3))
+    Fo, Uo = o.compute_forces()
+    cnt, lst = o.build_neighbors()
+    return Fo, Uo, cnt, lst
+
+
+@pytest.mark.parametrize("skin", [0.3, 0.45])
+def test_tile_forces_vs_oracle(relaxed, oracle_ref, skin):
+    s = relaxed
+    tot = staged_totals(s.pos, s.box[0], 2.5 + skin)
+    share16 = float(np.mean(tot <= FT_CAP))
+    print(f"case tile vs oracle skin={skin}: {tot.size} blocks, 16-bit share {share16:.2f}", flush=True)
+    assert tot.size >= 512          # 9^3 blocks at skin 0.3, 8^3 at 0.45
+    if skin == 0.3:
+        assert share16 == 1.0
+    else:
+        assert 0.05 < share16 < 0.95
+    Fo, Uo, _, _ = oracle_ref
+    em = Engine(s, lj_cfg(abi.ENGINE_MIXED, skin))
+    Fm, Um = em.compute_forces()
+    assert rms_rel(Fm, Fo) < 1e-5, rms_rel(Fm, Fo)
+    assert per_atom_rel(Fm, Fo).max() < 1e-3, per_atom_rel(Fm, Fo).max()
+    assert abs(Um.sum(dtype=np.float64) - Uo.sum(dtype=np.float64)) <= 1e-5 * np.abs(Uo).sum(dtype=np.float64)
+    # the MD step's force-only pass gives the same forces
+    em.bench_kernel(0, 1)
+    F0, _ = em.get_forces()
+    assert np.abs(F0 - Fm).max() <= 1e-12 * np.abs(Fm).max()
+    em.close()
+
+
+@pytest.mark.parametrize("skin", [0.3, 0.45])
+def test_fp64_forces_vs_oracle_large(relaxed, oracle_ref, skin):
+    print(f"case fp64 vs oracle skin={skin}", flush=True)
+    Fo, Uo, _, _ = oracle_ref
+    ef = Engine(relaxed, lj_cfg(abi.ENGINE_FP64, skin))
+    Ff, Uf = ef.compute_forces()
+    err = per_atom_rel(Ff, Fo)
+    assert err.max() < 1e-12, err.max()
+    assert abs(Uf.sum(dtype=np.float64) - Uo.sum(dtype=np.float64)) <= 1e-6 * np.abs(Uo).sum(dtype=np.float64)
+    ef.close()
+
+
+def _pairs(cnt, lst):
+    """(row, column) of every listed pair, rows in atom order."""
+    mask = np.arange(lst.shape[1])[None, :] < cnt[:, None]
+    return np.nonzero(mask)[0].astype(np.int64), lst[mask].astype(np.int64)
+
+
+@pytest.mark.parametrize("engine,skin", [(abi.ENGINE_MIXED, 0.3), (abi.ENGINE_MIXED, 0.45),
+                                         (abi.ENGINE_FP64, 0.3)])
+def test_skin_lists_hold_every_oracle_pair(relaxed, oracle_ref, engine, skin):
+    print(f"case skin lists engine={engine} skin={skin}", flush=True)
+    s = relaxed
+    n = s.n_atoms
+    L = np.array(s.box)
+    _, _, ocnt, olst = oracle_ref
+    e = Engine(s, lj_cfg(engine, skin))
+    e.compute_forces()
+    cnt, lst = e.skin_lists()
+    e.close()
+    # a row is a set: no duplicates, never the atom itself
+    rows, cols = _pairs(cnt, lst)
+    assert np.all(cols != rows)
+    keys = rows * n + cols
+    assert np.unique(keys).size == keys.size
+    # nothing beyond rlist but the FP16 screen's margin (MIXED: 3 rlist delta +
+    # 2 delta^2 on r^2, well under 0.1 sigma^2 at these cell widths)
+    d = s.pos[cols] - s.pos[rows]
+    d -= L * np.rint(d / L)
+    r2 = np.sum(d * d, axis=1)
+    rl = 2.5 + skin
+    assert r2.max() < rl * rl + (0.1 if engine == abi.ENGINE_MIXED else 1e-4), r2.max()
+    # every reference pair inside rc is listed
+    orows, ocols = _pairs(ocnt, olst)
+    okeys = orows * n + ocols
+    missing = np.setdiff1d(okeys, keys)
+    assert missing.size == 0, (missing.size, missing[:5] // n, missing[:5] % n)
+    # and the engine's list is no shorter than the pairs inside rlist (FP64 screen)
+    inside = r2 < rl * rl
+    assert inside.sum() >= okeys.size
+
+
+def test_full_size_mixture_vs_oracle(oracle_mod):
+    """The 1,048,576-atom mixture of the bench after 30 descent steps: MIXED
+    (tile path) and FP64 forces against the oracle once at full size."""
+    print("case 1M vs oracle", flush=True)
+    s = W.binary_lj_large(1048576, seed=3, T_star=1.0)
+    em = Engine(s, lj_cfg(abi.ENGINE_MIXED, 0.3))
+    em.optimize(30, 2e-5, 0.0)
+    s.pos, _ = em.get_state()
+    Fm, Um = em.compute_forces()
+    em.close()
+    ef = Engine(s, lj_cfg(abi.ENGINE_FP64, 0.3))
+    Ff, _ = ef.compute_forces()
+    ef.close()
+    o = oracle_mod.Oracle(s, lj_cfg(abi.ENGINE_EXACT, 0.3))
+    Fo, Uo = o.compute_forces()
+    del o
+    print(f"1M: mixed rms {rms_rel(Fm, Fo):.2e}, fp64 max {per_atom_rel(Ff, Fo).max():.2e}", flush=True)
+    assert rms_rel(Fm, Fo) < 1e-5
+    assert per_atom_rel(Fm, Fo).max() < 1e-3
+    assert per_atom_rel(Ff, Fo).max() < 1e-12
+    assert abs(Um.sum(dtype=np.float64) - Uo.sum(dtype=np.float64)) <= 1e-5 * np.abs(Uo).sum(dtype=np.float64)

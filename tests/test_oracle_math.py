@@ -324,3 +324,19 @@ def test_ljbuck_tables(oracle_mod):
         xp, fp, ep = oracle_mod.profile(0, plain[0], plain[1], 0.01, 10.0)
         assert np.array_equal(x, xp) and np.array_equal(e, ep)
         assert np.all(f == 0.0) and np.any(fp != 0.0)
+
+
+def test_oracle_threads_give_identical_results(oracle_mod):
+    """The oracle's per-atom loops on several host threads (the CPU baseline's
+    all-core figure) give the single-thread results bit for bit."""
+    from systems import make
+    for name in ("binary_lj", "water_300"):
+        s, c = make(name)
+        o1, o4 = oracle_mod.Oracle(s, c), oracle_mod.Oracle(s, c)
+        o4.set_threads(4)
+        F1, U1 = o1.compute_forces()
+        F4, U4 = o4.compute_forces()
+        assert np.array_equal(F1, F4) and np.array_equal(U1, U4)
+        n1, l1 = o1.build_neighbors()
+        n4, l4 = o4.build_neighbors()
+        assert np.array_equal(n1, n4) and np.array_equal(l1, l4)

@@ -306,6 +306,18 @@ int mtmd_step(mtmd_handle* h, int32_t nsteps);
  * (SimulationBox.cpp:168-231) on the current state and last forces. */
 int mtmd_get_thermo(mtmd_handle* h, mtmd_thermo* out);
 
+/* mtmd_get_thermo for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). No
+ * counterpart in the reference. Same reductions and the same sums, bit for
+ * bit, with the block partials in a fixed device array of the library (32 KiB
+ * on each GPU, held until the process ends) instead of the handle's buffer;
+ * calls no allocating routine, device or host. Needs the forces of the
+ * current positions (after mtmd_compute_forces or mtmd_step; MTMD_EINVAL
+ * otherwise, since computing them may grow the neighbour lists) and rejects
+ * reductions of more workgroups than the array holds. Calls that overlap on
+ * one GPU share the array and race. */
+int mtmd_get_thermo_noalloc(mtmd_handle* h, mtmd_thermo* out);
+
 /* Per-atom forces and energies of the last force evaluation (capped, as the
  * integrator uses them). */
 int mtmd_get_forces(mtmd_handle* h, double* F, float* U);

@@ -141,7 +141,7 @@ EXPORTED = [
     "mtmd_abi_version", "mtmd_last_error", "mtmd_status_string", "mtmd_config_defaults",
     "mtmd_config_set", "mtmd_profile", "mtmd_create", "mtmd_destroy", "mtmd_set_state",
     "mtmd_get_state", "mtmd_set_thermostat", "mtmd_get_thermostat", "mtmd_build_cells",
-    "mtmd_build_neighbors", "mtmd_compute_forces", "mtmd_step", "mtmd_get_thermo",
+    "mtmd_build_neighbors", "mtmd_compute_forces", "mtmd_step", "mtmd_get_thermo", "mtmd_get_thermo_noalloc",
     "mtmd_get_forces", "mtmd_optimize", "mtmd_get_stats", "mtmd_get_skin_lists", "mtmd_synchronize", "mtmd_run",
     "mtmd_run_optimize", "mtmd_bench_kernel", "mtmd_append_xyz", "mtmd_append_dcd",
     "mtmd_set_barostat", "mtmd_get_barostat", "mtmd_find_topology", "mtmd_topology_last_error",
@@ -186,6 +186,7 @@ def load_library(path: str | None = None) -> C.CDLL:
         "mtmd_compute_forces": (C.c_int, [vp, _dp, _dp, C.POINTER(C.c_float)]),
         "mtmd_step": (C.c_int, [vp, C.c_int32]),
         "mtmd_get_thermo": (C.c_int, [vp, C.POINTER(MtmdThermo)]),
+        "mtmd_get_thermo_noalloc": (C.c_int, [vp, C.POINTER(MtmdThermo)]),
         "mtmd_get_forces": (C.c_int, [vp, _dp, C.POINTER(C.c_float)]),
         "mtmd_optimize": (C.c_int, [vp, C.c_int32, C.c_double, C.c_double, _ip]),
         "mtmd_get_stats": (C.c_int, [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), _ip]),

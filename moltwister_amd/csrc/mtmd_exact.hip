@@ -462,11 +462,13 @@ __global__ void __launch_bounds__(TPB) k_reduce_thermo(const double4* __restrict
     block_sum_store(c, &red[3 * blockIdx.x + 2]);
 }
 
-void mt_reduce_thermo(const MtDev& d, hipStream_t s)
+void mt_reduce_thermo_to(const MtDev& d, double* red, hipStream_t s)
 {
     hipLaunchKernelGGL(k_reduce_thermo, dim3(mt_reduce_blocks(d.n)), dim3(TPB), 0, s, d.pos, d.mom, d.frc, d.epot,
-                       d.n, d.red);
+                       d.n, red);
 }
+
+void mt_reduce_thermo(const MtDev& d, hipStream_t s) { mt_reduce_thermo_to(d, d.red, s); }
 
 // CSimulationBox::gradientDescentStep (SimulationBox.cpp:197-208): r += F*lr
 __global__ void k_gradient_step(double4* __restrict__ pos, const double4* __restrict__ frc, int n, double lr)

@@ -2011,7 +2011,7 @@ __device__ __forceinline__ void mt_block_sum_to(double u, double* out)
     if (threadIdx.x == 0) *out = sh[0];
 }
 
-__global__ void __launch_bounds__(TPB) k_ucut(MtDev d, MtParams p, int exact)
+__global__ void __launch_bounds__(TPB) k_ucut(MtDev d, MtParams p, int exact, double* __restrict__ red)
 {
     double u = 0.0;
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < d.n; k += gridDim.x * blockDim.x) {
@@ -2036,7 +2036,7 @@ __global__ void __launch_bounds__(TPB) k_ucut(MtDev d, MtParams p, int exact)
         }
         if (!exact) u += mt_ucut_special(d, p, k, rk, row, qk);
     }
-    mt_block_sum_to(u, d.red + blockIdx.x);
+    mt_block_sum_to(u, red + blockIdx.x);
 }
 
 // type pairs whose U(rc) k_ucut_tile keeps in LDS; the tile mode's 32 KB
@@ -2047,7 +2047,7 @@ __global__ void __launch_bounds__(TPB) k_ucut(MtDev d, MtParams p, int exact)
 // blocks g = blockIdx.x, blockIdx.x + gridDim.x, ...; a block on 16-bit words
 // has the FP64 positions and types of its staged cells copied to LDS, so a
 // word is two LDS reads
-__global__ void __launch_bounds__(TPB) k_ucut_tile(MtDev d, MtParams p)
+__global__ void __launch_bounds__(TPB) k_ucut_tile(MtDev d, MtParams p, double* __restrict__ red)
 {
     __shared__ double s_x[FT_CAP], s_y[FT_CAP], s_z[FT_CAP];
     __shared__ int s_ty[FT_CAP];
@@ -2115,20 +2115,29 @@ __global__ void __launch_bounds__(TPB) k_ucut_tile(MtDev d, MtParams p)
             u += mt_ucut_special(d, p, k, rk, row, qk);
         }
     }
-    mt_block_sum_to(u, d.red + blockIdx.x);
+    mt_block_sum_to(u, red + blockIdx.x);
 }
 
-int mt_ucut(const MtDev& d, const MtParams& p, int exact, hipStream_t s)
+int mt_ucut_blocks(const MtDev& d, int exact)
 {
     if (!exact && d.tile16) {
         const int nb = ((d.nc[0] + 1) / 2) * ((d.nc[1] + 1) / 2) * ((d.nc[2] + 1) / 2);
-        const int g = std::max(1, std::min(nb, mt_reduce_blocks(d.n)));
-        hipLaunchKernelGGL(k_ucut_tile, dim3(g), dim3(TPB), 0, s, d, p);
-        return g;
+        return std::max(1, std::min(nb, mt_reduce_blocks(d.n)));
     }
-    hipLaunchKernelGGL(k_ucut, dim3(mt_reduce_blocks(d.n)), dim3(TPB), 0, s, d, p, exact);
     return mt_reduce_blocks(d.n);
 }
+
+int mt_ucut_to(const MtDev& d, const MtParams& p, int exact, double* red, hipStream_t s)
+{
+    const int g = mt_ucut_blocks(d, exact);
+    if (!exact && d.tile16)
+        hipLaunchKernelGGL(k_ucut_tile, dim3(g), dim3(TPB), 0, s, d, p, red);
+    else
+        hipLaunchKernelGGL(k_ucut, dim3(g), dim3(TPB), 0, s, d, p, exact, red);
+    return g;
+}
+
+int mt_ucut(const MtDev& d, const MtParams& p, int exact, hipStream_t s) { return mt_ucut_to(d, p, exact, d.red, s); }
 
 // The lists of the last build in the caller's atom indices: row perm[k] of
 // `out` (max_per_atom entries) receives atom k's neighbours in list order,

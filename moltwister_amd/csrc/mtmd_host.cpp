@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -60,6 +61,7 @@ double urand(uint64_t& s) { return double(splitmix64(s) >> 11) * (1.0 / 90071992
 
 struct mtmd_handle {
     int n = 0, T = 0;
+    int device = 0;               // the HIP device current at create (stream, memory)
     mtmd_config cfg{};
     int engine = MTMD_ENGINE_EXACT;
     MtDev d{};
@@ -842,6 +844,7 @@ extern "C" int mtmd_create(const mtmd_system* sys, const mtmd_config* cfg, mtmd_
     }
     mtmd_handle* h = new mtmd_handle();
     h->cfg = *cfg;
+    if (hipGetDevice(&h->device) != hipSuccess) h->device = 0;
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
         g_create_error = "hipStreamCreate failed";
         delete h;
@@ -1557,6 +1560,23 @@ extern "C" int mtmd_step(mtmd_handle* h, int32_t nsteps)
     return md_steps(h, nsteps, true);
 }
 
+// calcTemp/calcPress/calcV/calcUtot/calcCurrentKineticEnergy from the sums
+// sum p.p/m, sum F.r, sum U and U_cut
+static void thermo_from_sums(const mtmd_handle* h, double sp, double sf, double su, double uc, mtmd_thermo* out)
+{
+    // calcV / getV (SimulationBox.h:44, VelVerlet.cpp:192-197)
+    const double V = h->npt ? npt_volume(h) : h->p.L[0] * h->p.L[1] * h->p.L[2];
+    out->sum_p2m = sp;
+    out->sum_fr = sf;
+    out->T = sp / (3 * (double)h->n) * MT_CONV_T;
+    out->P = (1.0 / (3.0 * V)) * (sp + sf) * MT_CONV_PRESS;
+    out->V = V;
+    out->U = su;
+    out->K = 0.5 * sp;
+    out->Etot = su + 0.5 * sp;
+    out->U_cut = uc;
+}
+
 extern "C" int mtmd_get_thermo(mtmd_handle* h, mtmd_thermo* out)
 {
     if (!h || !out) return MTMD_EINVAL;
@@ -1573,16 +1593,6 @@ extern "C" int mtmd_get_thermo(mtmd_handle* h, mtmd_thermo* out)
     h->ke_valid = false;   // d.red now holds thermo partials
     double sp = 0, sf = 0, su = 0;
     for (int b = 0; b < nb; b++) { sp += part[3 * b]; sf += part[3 * b + 1]; su += part[3 * b + 2]; }
-    // calcV / getV (SimulationBox.h:44, VelVerlet.cpp:192-197)
-    const double V = h->npt ? npt_volume(h) : h->p.L[0] * h->p.L[1] * h->p.L[2];
-    out->sum_p2m = sp;
-    out->sum_fr = sf;
-    out->T = sp / (3 * (double)h->n) * MT_CONV_T;
-    out->P = (1.0 / (3.0 * V)) * (sp + sf) * MT_CONV_PRESS;
-    out->V = V;
-    out->U = su;
-    out->K = 0.5 * sp;
-    out->Etot = su + 0.5 * sp;
     // U_cut over the lists of the last force evaluation (EXACT: rebuilt at rc
     // for those positions; FP64/MIXED: the skin list, valid until the next step)
     const int nbu = mt_ucut(d, h->p, h->fast ? 0 : 1, h->stream);
@@ -1590,7 +1600,78 @@ extern "C" int mtmd_get_thermo(mtmd_handle* h, mtmd_thermo* out)
     HIPCHK(h, hipStreamSynchronize(h->stream));
     double uc = 0;
     for (int b = 0; b < nbu; b++) uc += part[b];
-    out->U_cut = uc;
+    thermo_from_sums(h, sp, sf, su, uc, out);
+    return MTMD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// mtmd_get_thermo_noalloc: the thermo reductions with their block partials in
+// a file-scope device array instead of the handle's buffer, and no allocation
+// of any kind (device or host). Every GPU the library runs on holds its own
+// copy of `scratch` until the process ends; host code reaches it only through
+// the address hipGetSymbolAddress returns for the device (the array's name is
+// a host-side stand-in, not a device address). Calls that overlap on one GPU
+// share that array and race.
+// ---------------------------------------------------------------------------
+#define MT_NOALLOC_THERMO 3072        // 3 partials (p.p/m, F.r, U) of up to 1024 workgroups
+#define MT_NOALLOC_UCUT 1024          // U_cut partials of up to 1024 workgroups
+#define MT_NOALLOC_DEVICES 64
+static __device__ __align__(256) double scratch[MT_NOALLOC_THERMO + MT_NOALLOC_UCUT];
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[MT_NOALLOC_DEVICES] = {};
+
+// the device address of `scratch` on device dev, looked up once per device with
+// that device current
+static int scratch_address(mtmd_handle* h, double** out)
+{
+    const int dev = h->device;
+    if (dev < 0 || dev >= MT_NOALLOC_DEVICES)
+        return h->fail(MTMD_EINVAL, "device %d is beyond the %d devices the scratch table holds", dev,
+                       MT_NOALLOC_DEVICES);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        int cur = 0;
+        HIPCHK(h, hipGetDevice(&cur));
+        if (cur != dev) HIPCHK(h, hipSetDevice(dev));
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (cur != dev) (void)hipSetDevice(cur);
+        if (e != hipSuccess || !ptr)
+            return h->fail(MTMD_EHIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+        g_scratch_ptr[dev] = static_cast<double*>(ptr);
+    }
+    *out = g_scratch_ptr[dev];
+    return MTMD_OK;
+}
+
+extern "C" int mtmd_get_thermo_noalloc(mtmd_handle* h, mtmd_thermo* out)
+{
+    if (!h || !out) return MTMD_EINVAL;
+    if (!h->have_forces)
+        return h->fail(MTMD_EINVAL, "no forces of the current positions (call mtmd_compute_forces or mtmd_step "
+                                    "first: computing them may grow the neighbour lists)");
+    const int nb = mt_reduce_blocks(h->n);
+    const int nbu = mt_ucut_blocks(h->d, h->fast ? 0 : 1);
+    if (3 * nb > MT_NOALLOC_THERMO || nbu > MT_NOALLOC_UCUT)
+        return h->fail(MTMD_EINVAL, "reduction of %d + %d workgroups exceeds the scratch of %d + %d partials", nb, nbu,
+                       MT_NOALLOC_THERMO / 3, MT_NOALLOC_UCUT);
+    double* sc = nullptr;
+    int rc = scratch_address(h, &sc);
+    if (rc) return rc;
+    // every workgroup writes its own partials, so nothing needs clearing first
+    mt_reduce_thermo_to(h->d, sc, h->stream);
+    mt_ucut_to(h->d, h->p, h->fast ? 0 : 1, sc + MT_NOALLOC_THERMO, h->stream);
+    HIPCHK(h, hipGetLastError());
+    double part[MT_NOALLOC_THERMO + MT_NOALLOC_UCUT];
+    HIPCHK(h, hipMemcpyAsync(part, sc, 3 * (size_t)nb * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipMemcpyAsync(part + MT_NOALLOC_THERMO, sc + MT_NOALLOC_THERMO, (size_t)nbu * sizeof(double),
+                             hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    // the sums in the order of mtmd_get_thermo
+    double sp = 0, sf = 0, su = 0, uc = 0;
+    for (int b = 0; b < nb; b++) { sp += part[3 * b]; sf += part[3 * b + 1]; su += part[3 * b + 2]; }
+    for (int b = 0; b < nbu; b++) uc += part[MT_NOALLOC_THERMO + b];
+    thermo_from_sums(h, sp, sf, su, uc, out);
     return MTMD_OK;
 }
 

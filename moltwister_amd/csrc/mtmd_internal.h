@@ -129,6 +129,8 @@ void mt_nh_chain(const MtDev& d, int M, int nresp, double T, double dt, double g
 void mt_scale_wrap(const MtDev& d, const MtParams& p, int scale_slot, int want_ke, hipStream_t s);
 void mt_reduce_ke(const MtDev& d, hipStream_t s);
 void mt_reduce_thermo(const MtDev& d, hipStream_t s);
+// the same partials (3 a block: sum p.p/m, sum F.r, sum U) into red[0, 3 mt_reduce_blocks(n))
+void mt_reduce_thermo_to(const MtDev& d, double* red, hipStream_t s);
 void mt_gradient_step(const MtDev& d, double lr, hipStream_t s);
 // NPT particle kernels (all engines): partials of sum p.p/m in red[0, nb) and
 // sum F.r in red[nb, 2 nb), nb = mt_npt_blocks(n)
@@ -172,5 +174,8 @@ void mt_copy_lists(const MtDev& d, hipStream_t s);
 void mt_fast_decode_lists(const MtDev& d, int* out, int* cnt, int maxp, hipStream_t s);
 // per-block partial sums of U_cut into d.red[0 .. returned count)
 int mt_ucut(const MtDev& d, const MtParams& p, int exact, hipStream_t s);
+// the same into red[0 .. returned count); the count is mt_ucut_blocks (<= 1024)
+int mt_ucut_to(const MtDev& d, const MtParams& p, int exact, double* red, hipStream_t s);
+int mt_ucut_blocks(const MtDev& d, int exact);
 
 int mt_reduce_blocks(int n);

@@ -142,6 +142,13 @@ class Engine:
         self._chk(self.lib.mtmd_get_thermo(self.h, C.byref(t)))
         return {k: getattr(t, k) for k, _ in abi.MtmdThermo._fields_}
 
+    def thermo_noalloc(self) -> dict:
+        """mtmd_get_thermo_noalloc: the thermo of the last forces, computed
+        without any allocation (partials in the library's fixed device array)."""
+        t = abi.MtmdThermo()
+        self._chk(self.lib.mtmd_get_thermo_noalloc(self.h, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in abi.MtmdThermo._fields_}
+
     def optimize(self, max_steps: int, lr: float, accuracy: float) -> int:
         done = C.c_int32(0)
         self._chk(self.lib.mtmd_optimize(self.h, max_steps, lr, accuracy, C.byref(done)))

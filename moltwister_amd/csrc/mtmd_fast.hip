@@ -218,9 +218,60 @@ __global__ void __launch_bounds__(TPB) k_fast_cell_sort(MtDev d, MtParams p, int
         mt_wave_sort_cell(d.cell_list + f, cnt, lw);
 }
 
+// The exclusive scan of the counts of a rebuild in one workgroup (grids of at
+// most MT_SCAN_ONE_BLOCK cells): k_fast_cell_keys has counted into cell_next,
+// which is all zero between rebuilds; the scan writes cell_first, moves the
+// counts to cell_count, clears cell_fill for the scatter and cell_next for
+// the next rebuild. One launch in place of the count fill and the two-level
+// scan. (A one-pass variant in which the last workgroup of the key kernel
+// scans, found by a ticket, needs a device-scope fence in every workgroup;
+// with one L2 per XCD that cost 0.38 ms a build on the 1M-atom mixture.)
+#define MT_SCAN_ONE_BLOCK 65536
+__global__ void __launch_bounds__(SCAN_B) k_fast_scan_one(MtDev d, int cond, float thr)
+{
+    if (mt_skip_rebuild(cond, d.flags, thr)) return;
+    __shared__ int s_wsum[SCAN_B / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int carry = 0;
+    for (int base = 0; base < d.ncells; base += SCAN_B) {
+        const int c = base + (int)threadIdx.x;
+        const int v = c < d.ncells ? d.cell_next[c] : 0;
+        int incl = v;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int u = __shfl_up(incl, o);
+            if (lane >= o) incl += u;
+        }
+        if (lane == 63) s_wsum[w] = incl;
+        __syncthreads();
+        int wpre = 0, tot = 0;
+        for (int j = 0; j < SCAN_B / 64; j++) {
+            const int x = s_wsum[j];
+            wpre += j < w ? x : 0;
+            tot += x;
+        }
+        if (c < d.ncells) {
+            d.cell_first[c] = carry + wpre + incl - v;
+            d.cell_count[c] = v;
+            d.cell_fill[c] = 0;
+            d.cell_next[c] = 0;
+        }
+        carry += tot;
+        __syncthreads();                              // s_wsum is rewritten next tile
+    }
+}
+
 void mt_fast_cells(const MtDev& d, const MtParams& p, int cond, int with_home, hipStream_t s)
 {
     const float thr = p.half_skin2;
+    if (d.ncells <= MT_SCAN_ONE_BLOCK) {
+        hipLaunchKernelGGL(k_fast_cell_keys, dim3(nblk(d.n, TPB)), dim3(TPB), 0, s, d.pos, d.n, d.nc[0], d.nc[1],
+                           d.nc[2], p.L[0], p.L[1], p.L[2], d.cell_key, d.cell_next, cond, d.flags, thr);
+        hipLaunchKernelGGL(k_fast_scan_one, dim3(1), dim3(SCAN_B), 0, s, d, cond, thr);
+        hipLaunchKernelGGL(k_fast_scatter, dim3(nblk(d.n, TPB)), dim3(TPB), 0, s, d.cell_key, d.n, d.cell_first,
+                           d.cell_fill, d.cell_list, cond, d.flags, thr);
+        hipLaunchKernelGGL(k_fast_cell_sort, dim3(nblk(d.ncells, TPB / 64)), dim3(TPB), 0, s, d, p, cond, with_home);
+        return;
+    }
     const int nsb = nblk(d.ncells, SCAN_B);
     hipLaunchKernelGGL(k_fast_fill, dim3(nblk(d.ncells, TPB)), dim3(TPB), 0, s, d.cell_count, 0, d.ncells, cond,
                        d.flags, thr);
@@ -335,6 +386,8 @@ __device__ inline bool mt_is_partner(const MtDev& d, int k, int ia)
 // make every iteration wait for the previous chunk store.
 #define NB_CAP 2048
 #define NB_PAD_CODE 13u      // offset code of (0, 0, 0): a pad word is the atom itself
+static_assert(NB_CAP == MT_NB_STAGE_ATOMS, "staging capacity of the 32-bit list build");
+static_assert(NB_CAP / 2 == MT_NB_STAGE_ATOMS_TILE, "staging capacity of the tile-mode list build");
 
 // slot of a list-word index (the index itself unless the words are in cell order)
 __device__ __forceinline__ int mt_id_slot(const MtDev& d, int id) { return d.wsorted ? d.cell_list[id] : id; }
@@ -383,8 +436,9 @@ __device__ __forceinline__ void mt_neigh_hit(const MtDev& d, uint4* nl4, float4 
 }
 
 // the 32-bit lists of one workgroup's home atoms (the scan below the staging
-// set-up of k_fast_neigh_block), staging the 64 cells in batches that fit NB_CAP
-template <int MODE>
+// set-up of k_fast_neigh_block), staging the 64 cells in batches of whole
+// cells that fit SCAP atoms
+template <int MODE, int SCAP>
 __device__ __forceinline__ void mt_neigh_block_atoms(const MtDev& d, const MtParams& p, float4* spos, int* smol,
                                                      const int* s_first, const int* s_cnt, const int* s_off, int hxo,
                                                      int hyo, int hzo, int l, int f0, int cnt_c, int cnt_max)
@@ -408,11 +462,11 @@ __device__ __forceinline__ void mt_neigh_block_atoms(const MtDev& d, const MtPar
         uint32_t r0 = 0, r1 = 0, r2 = 0, r3 = 0;
         int cb = 0;
         while (cb < 64) {
-            // batch of whole staged cells [cb, ce) that fits NB_CAP (uniform)
+            // batch of whole staged cells [cb, ce) that fits SCAP (uniform)
             int ce = cb + 1;
-            while (ce < 64 && s_off[ce + 1] - s_off[cb] <= NB_CAP) ce++;
+            while (ce < 64 && s_off[ce + 1] - s_off[cb] <= SCAP) ce++;
             const int base = s_off[cb];
-            if (s_off[ce] - base > NB_CAP) {
+            if (s_off[ce] - base > SCAP) {
                 // a single cell larger than the staging buffer
                 if (t == 0) d.flags[5] = 1u;
                 return;
@@ -552,16 +606,16 @@ __device__ __forceinline__ void mt_neigh_run16(const MtDev& d, uint4* nl4, const
 
 template <int MODE>
 __device__ __forceinline__ void mt_neigh_block_atoms16(const MtDev& d, const MtParams& p, float4* spos, int* smol,
-                                                       const int* s_first, const int* s_cnt, const int* s_off,
-                                                       const int* s_hfirst, const int* s_hcnt)
+                                                       int* sword, const int* s_first, const int* s_cnt,
+                                                       const int* s_off, const int* s_hfirst, const int* s_hcnt)
 {
     const int t = threadIdx.x;
     const int n = d.n;
     const int cap = 2 * d.ncap;                       // 16-bit words a row holds
     uint4* __restrict__ nl4 = reinterpret_cast<uint4*>(d.nlist);
-    // the staging buffer as (FP16 x | FP16 y, FP32 z) and, for MODE 2, the word indices
+    // the staging buffer (NB_CAP / 2 float4s) as NB_CAP (FP16 x | FP16 y,
+    // FP32 z) pairs; MODE 2 keeps the word indices in sword
     uint2* sxz = reinterpret_cast<uint2*>(spos);
-    int* sword = reinterpret_cast<int*>(spos) + 2 * NB_CAP;
     // the screen's threshold: staged |x|, |y| <= 2.5 cs (cells at offsets -1..2
     // around the first cell's centre, atoms within half a cell of their
     // cell's centre at a build) round by at most h(2.6 cs) each in FP16, the
@@ -635,12 +689,18 @@ __device__ __forceinline__ void mt_neigh_block_atoms16(const MtDev& d, const MtP
     }
 }
 
-template <int MODE>
+// TILE: the launch of the MIXED tile mode (d.tile16), whose blocks mostly take
+// 16-bit words: 16 KB of staging (NB_CAP FP16/FP32 coordinate pairs), twice
+// the workgroups a CU of the full 32 KB buffer; the blocks that keep 32-bit
+// words then stage their cells in batches of NB_CAP / 2 atoms
+template <int MODE, bool TILE>
 __global__ void __launch_bounds__(256) k_fast_neigh_block(MtDev d, MtParams p, int cond)
 {
     if (mt_skip_rebuild(cond, d.flags, p.half_skin2)) return;
-    __shared__ float4 spos[NB_CAP];
+    constexpr int SCAP = TILE ? NB_CAP / 2 : NB_CAP;
+    __shared__ float4 spos[SCAP];
     __shared__ int smol_buf[MODE >= 1 ? NB_CAP : 1];
+    __shared__ int sword_buf[(TILE && MODE == 2) ? NB_CAP : 1];
     __shared__ int s_first[64], s_cnt[64], s_off[65];
     __shared__ int s_hfirst[8], s_hcnt[8];
     const int t = threadIdx.x;
@@ -689,10 +749,11 @@ __global__ void __launch_bounds__(256) k_fast_neigh_block(MtDev d, MtParams p, i
     }
     // the word format of this block's lists (uniform; k_fast_force_tile makes
     // the same choice from the same counts)
-    if (d.tile16 && s_off[64] <= NB_CAP)
-        mt_neigh_block_atoms16<MODE>(d, p, spos, smol_buf, s_first, s_cnt, s_off, s_hfirst, s_hcnt);
+    if (TILE && s_off[64] <= NB_CAP)
+        mt_neigh_block_atoms16<MODE>(d, p, spos, smol_buf, sword_buf, s_first, s_cnt, s_off, s_hfirst, s_hcnt);
     else
-        mt_neigh_block_atoms<MODE>(d, p, spos, smol_buf, s_first, s_cnt, s_off, hxo, hyo, hzo, l, f0, cnt_c, cnt_max);
+        mt_neigh_block_atoms<MODE, SCAP>(d, p, spos, smol_buf, s_first, s_cnt, s_off, hxo, hyo, hzo, l, f0, cnt_c,
+                                         cnt_max);
 }
 
 #define NBW_CAP 1024
@@ -817,12 +878,21 @@ void mt_fast_neighbors(const MtDev& d, const MtParams& p, int cond, hipStream_t 
         return;
     }
     const int nb = ((d.nc[0] + 1) / 2) * ((d.nc[1] + 1) / 2) * ((d.nc[2] + 1) / 2);
-    if (d.has_xmol)
-        hipLaunchKernelGGL(k_fast_neigh_block<2>, dim3(nb), dim3(256), 0, s, d, p, cond);
-    else if (d.has_molecules)
-        hipLaunchKernelGGL(k_fast_neigh_block<1>, dim3(nb), dim3(256), 0, s, d, p, cond);
-    else
-        hipLaunchKernelGGL(k_fast_neigh_block<0>, dim3(nb), dim3(256), 0, s, d, p, cond);
+    if (d.tile16) {
+        if (d.has_xmol)
+            hipLaunchKernelGGL((k_fast_neigh_block<2, true>), dim3(nb), dim3(256), 0, s, d, p, cond);
+        else if (d.has_molecules)
+            hipLaunchKernelGGL((k_fast_neigh_block<1, true>), dim3(nb), dim3(256), 0, s, d, p, cond);
+        else
+            hipLaunchKernelGGL((k_fast_neigh_block<0, true>), dim3(nb), dim3(256), 0, s, d, p, cond);
+    } else {
+        if (d.has_xmol)
+            hipLaunchKernelGGL((k_fast_neigh_block<2, false>), dim3(nb), dim3(256), 0, s, d, p, cond);
+        else if (d.has_molecules)
+            hipLaunchKernelGGL((k_fast_neigh_block<1, false>), dim3(nb), dim3(256), 0, s, d, p, cond);
+        else
+            hipLaunchKernelGGL((k_fast_neigh_block<0, false>), dim3(nb), dim3(256), 0, s, d, p, cond);
+    }
 }
 
 // ---------------------------------------------------------------------------

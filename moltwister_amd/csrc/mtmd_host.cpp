@@ -755,6 +755,7 @@ static int setup(mtmd_handle* h, const mtmd_system* s)
     d.cell_first = h->mem.alloc<int>(d.ncells);
     d.cell_fill = h->mem.alloc<int>(d.ncells);
     d.cell_bsum = h->mem.alloc<int>((d.ncells + 1023) / 1024 + 1);
+    d.cell_next = h->mem.alloc<int>(d.ncells);
     d.cell_list = h->mem.alloc<int>(n);
     d.ncount = h->mem.alloc<int>(n);
     d.nreq = h->mem.alloc<int>(1);
@@ -762,12 +763,13 @@ static int setup(mtmd_handle* h, const mtmd_system* s)
     d.red = h->mem.alloc<double>(d.red_blocks);
     d.flags = h->mem.alloc<unsigned int>(8);
     if (!d.frc || !d.epot || !d.cell_key || !d.cell_count || !d.cell_first || !d.cell_fill || !d.cell_list ||
-        !d.cell_bsum ||
+        !d.cell_bsum || !d.cell_next ||
         !d.ncount || !d.nreq || !d.red || !d.flags || (!h->fast && !d.cell_xyz))
         return h->fail(MTMD_ENOMEM, "device allocation failed");
     HIPCHK(h, hipMemset(d.frc, 0, n * sizeof(double4)));
     HIPCHK(h, hipMemset(d.epot, 0, n * sizeof(float)));
     HIPCHK(h, hipMemset(d.flags, 0, 8 * sizeof(unsigned int)));
+    HIPCHK(h, hipMemset(d.cell_next, 0, d.ncells * sizeof(int)));
     // neighbour capacity: the expected full count at the list radius plus margin
     {
         const double V = p.L[0] * p.L[1] * p.L[2];
@@ -1120,10 +1122,12 @@ static int grow_neighbors(mtmd_handle* h, int need)
     HIPCHK(h, hipStreamSynchronize(h->stream));
     if (hipMalloc((void**)&nl, (size_t)cap * h->n * sizeof(uint32_t)) != hipSuccess)
         return h->fail(MTMD_ENOMEM, "neighbour list of %d rows does not fit", cap);
+    // the captured step graphs hold the old buffer as a kernel argument: they
+    // go before the buffer does
+    h->drop_graphs();
     for (auto& p : h->mem.ptrs) if (p == old) { (void)hipFree(p); p = nl; }
     d.nlist = nl;
     d.ncap = cap;
-    h->drop_graphs();
     return MTMD_OK;
 }
 
@@ -1194,7 +1198,8 @@ static int fast_check_capacity(mtmd_handle* h, bool* overflowed)
     *overflowed = h->h_status[2] != 0;
     h->rebuilds = (int64_t)(unsigned)h->h_status[4];
     if (h->h_status[5])
-        return h->fail(MTMD_EBOX, "a cell holds more atoms than the neighbour-list staging buffer (2560)");
+        return h->fail(MTMD_EBOX, "a cell holds more atoms than the neighbour-list build stages at once (%d)",
+                       d.tile16 ? MT_NB_STAGE_ATOMS_TILE : MT_NB_STAGE_ATOMS);
     if (h->h_status[3]) {
         HIPCHK(h, hipMemsetAsync(d.flags + 3, 0, sizeof(unsigned int), h->stream));
         h->have_forces = false;

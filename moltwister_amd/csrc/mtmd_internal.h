@@ -37,6 +37,10 @@
 
 // neighbour word of the FP64/MIXED lists: atom index in the low 27 bits, the
 // home-cell offset code (dx+1) + 3*(dy+1) + 9*(dz+1) in the top 5 bits
+// atoms the block list build stages in LDS at once: a cell holding more than
+// this (the tile-mode figure when d.tile16 is set) cannot be listed (flags[5])
+#define MT_NB_STAGE_ATOMS 2048
+#define MT_NB_STAGE_ATOMS_TILE 1024
 #define MT_NW_IDX_BITS 27
 #define MT_NW_IDX_MASK ((1u << MT_NW_IDX_BITS) - 1u)
 
@@ -72,6 +76,7 @@ struct MtDev {
     int* cell_key; int* cell_count; int* cell_first; int* cell_fill; int* cell_list;
     int* cell_xyz;         // EXACT: ix, iy, iz per atom
     int* cell_bsum;        // FP64/MIXED: per-block sums of the cell-count scan
+    int* cell_next;        // FP64/MIXED: counts of the rebuild in progress, all zero between rebuilds
     // neighbours
     int ncap; int* ncount; uint32_t* nlist; int* nreq;
     // MIXED block lists (mt_fast_tile_mode): the lists of every 2x2x2 block of
@@ -99,7 +104,8 @@ struct MtDev {
     int red_blocks;
     double* nh;            // NH chain state: eta[M], p_eta[M], Q[M], scale, KE2
     unsigned int* flags;   // [0] max displacement^2 (float bits) of this step, [2] list overflow,
-                           // [4] rebuild count
+                           // [3] non-finite position, [4] rebuild count, [5] cell over the staging
+                           // capacity
 };
 
 struct MtParams {

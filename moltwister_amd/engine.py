@@ -6,12 +6,33 @@ MtmdError with the library's message. There is no CPU fallback.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
+import weakref
 
 import numpy as np
 
 from . import abi
 from .system import System
+
+# handles still open when the interpreter exits are closed by an atexit hook,
+# which runs before the interpreter tears its modules down and before the HIP
+# runtime's own exit handlers, instead of from __del__ during that teardown
+_open_handles: "weakref.WeakSet" = weakref.WeakSet()
+
+
+def track(obj) -> None:
+    """close obj (which has a close() method) at interpreter exit if still open"""
+    _open_handles.add(obj)
+
+
+@atexit.register
+def _close_open_handles() -> None:
+    for obj in list(_open_handles):
+        try:
+            obj.close()
+        except Exception:
+            pass
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
@@ -60,6 +81,7 @@ class Engine:
         if rc:
             raise MtmdError(rc, self.lib.mtmd_last_error(None).decode())
         self.h = h
+        track(self)
 
     def close(self):
         if getattr(self, "h", None):
@@ -240,6 +262,7 @@ class BatchEngine:
         if rc:
             raise MtmdError(rc, self.lib.mtmd_batch_last_error(None).decode())
         self.h = h
+        track(self)
 
     def _chk(self, rc):
         if rc:

@@ -11,7 +11,7 @@ import ctypes as C
 import numpy as np
 
 from . import abi
-from .engine import MtmdError
+from .engine import MtmdError, track
 from .system import System
 
 _dp = C.POINTER(C.c_double)
@@ -35,6 +35,7 @@ class ForceMatch:
         if rc:
             raise MtmdError(rc, self.lib.mtmd_fm_last_error(None).decode())
         self.h = h
+        track(self)
         self.n_params = self.lib.mtmd_fm_num_params(self.h)
 
     def _chk(self, rc):

@@ -508,11 +508,11 @@ struct MtNoSortHook {
     __device__ void operator()(int, int) const {}
 };
 #define MT_CELL_SORT_LDS 2048
-template <typename Hook = MtNoSortHook>
+template <int CAP = MT_CELL_SORT_LDS, typename Hook = MtNoSortHook>
 __device__ inline void mt_wave_sort_cell(int* __restrict__ a, int cnt, int* __restrict__ lds, Hook on = Hook())
 {
     const int lane = threadIdx.x & 63;
-    const bool in_lds = cnt > 64 && cnt <= MT_CELL_SORT_LDS;
+    const bool in_lds = cnt > 64 && cnt <= CAP;
     const int n4 = (cnt + 3) >> 2;
     if (in_lds)
         for (int i = lane; i < 4 * n4; i += 64) lds[i] = i < cnt ? a[i] : INT_MAX;
@@ -537,7 +537,7 @@ __device__ inline void mt_wave_sort_cell(int* __restrict__ a, int cnt, int* __re
             a[rank] = v;
             on(v, rank);
         }
-    } else if (cnt > MT_CELL_SORT_LDS && lane == 0) {
+    } else if (cnt > CAP && lane == 0) {
         for (int i = 1; i < cnt; i++) {
             const int v = a[i];
             int j = i - 1;

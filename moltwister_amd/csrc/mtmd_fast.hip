@@ -203,19 +203,23 @@ __device__ __forceinline__ void mt_fast_home_atom(const MtDev& d, const MtParams
 // (one wavefront per cell, mt_wave_sort_cell); with `home` set the same
 // wavefront then does the per-atom set-up of every atom of the cell, which
 // holds its sorted position already
+// cells of up to MT_FAST_SORT_LDS atoms are ranked in LDS (8 KB a workgroup, so
+// many workgroups share a CU); a larger cell is sorted by one lane
+#define MT_FAST_SORT_LDS 512
 __global__ void __launch_bounds__(TPB) k_fast_cell_sort(MtDev d, MtParams p, int cond, int home)
 {
-    __shared__ __align__(16) int lds[(TPB / 64) * MT_CELL_SORT_LDS];
+    __shared__ __align__(16) int lds[(TPB / 64) * MT_FAST_SORT_LDS];
     if (mt_skip_rebuild(cond, d.flags, p.half_skin2) || d.flags[3]) return;   // uniform over the grid
     if (home && blockIdx.x == 0 && threadIdx.x == 0) d.flags[4] += 1;        // rebuild count
     const int c = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int cnt = c < d.ncells ? d.cell_count[c] : 0;
     const int f = c < d.ncells ? d.cell_first[c] : 0;
-    int* lw = lds + (threadIdx.x >> 6) * MT_CELL_SORT_LDS;
+    int* lw = lds + (threadIdx.x >> 6) * MT_FAST_SORT_LDS;
     if (home)
-        mt_wave_sort_cell(d.cell_list + f, cnt, lw, [&](int v, int rank) { mt_fast_home_atom(d, p, v, c, f + rank); });
+        mt_wave_sort_cell<MT_FAST_SORT_LDS>(d.cell_list + f, cnt, lw,
+                                            [&](int v, int rank) { mt_fast_home_atom(d, p, v, c, f + rank); });
     else
-        mt_wave_sort_cell(d.cell_list + f, cnt, lw);
+        mt_wave_sort_cell<MT_FAST_SORT_LDS>(d.cell_list + f, cnt, lw);
 }
 
 // The exclusive scan of the counts of a rebuild in one workgroup (grids of at
@@ -227,36 +231,50 @@ __global__ void __launch_bounds__(TPB) k_fast_cell_sort(MtDev d, MtParams p, int
 // scans, found by a ticket, needs a device-scope fence in every workgroup;
 // with one L2 per XCD that cost 0.38 ms a build on the 1M-atom mixture.)
 #define MT_SCAN_ONE_BLOCK 65536
+#define MT_SCAN_ONE_TPI 4                 // tiles of SCAN_B cells an iteration
 __global__ void __launch_bounds__(SCAN_B) k_fast_scan_one(MtDev d, int cond, float thr)
 {
     if (mt_skip_rebuild(cond, d.flags, thr)) return;
-    __shared__ int s_wsum[SCAN_B / 64];
+    __shared__ int s_wsum[MT_SCAN_ONE_TPI][SCAN_B / 64];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     int carry = 0;
-    for (int base = 0; base < d.ncells; base += SCAN_B) {
-        const int c = base + (int)threadIdx.x;
-        const int v = c < d.ncells ? d.cell_next[c] : 0;
-        int incl = v;
-        for (int o = 1; o < 64; o <<= 1) {
-            const int u = __shfl_up(incl, o);
-            if (lane >= o) incl += u;
+    for (int base = 0; base < d.ncells; base += MT_SCAN_ONE_TPI * SCAN_B) {
+        // MT_SCAN_ONE_TPI coalesced tiles in flight at once
+        int v[MT_SCAN_ONE_TPI], ex[MT_SCAN_ONE_TPI];
+#pragma unroll
+        for (int t = 0; t < MT_SCAN_ONE_TPI; t++) {
+            const int c = base + t * SCAN_B + (int)threadIdx.x;
+            v[t] = c < d.ncells ? d.cell_next[c] : 0;
         }
-        if (lane == 63) s_wsum[w] = incl;
+#pragma unroll
+        for (int t = 0; t < MT_SCAN_ONE_TPI; t++) {
+            int incl = v[t];
+            for (int o = 1; o < 64; o <<= 1) {
+                const int u = __shfl_up(incl, o);
+                if (lane >= o) incl += u;
+            }
+            ex[t] = incl - v[t];
+            if (lane == 63) s_wsum[t][w] = incl;
+        }
         __syncthreads();
-        int wpre = 0, tot = 0;
-        for (int j = 0; j < SCAN_B / 64; j++) {
-            const int x = s_wsum[j];
-            wpre += j < w ? x : 0;
-            tot += x;
+#pragma unroll
+        for (int t = 0; t < MT_SCAN_ONE_TPI; t++) {
+            int wpre = 0, tot = 0;
+            for (int j = 0; j < SCAN_B / 64; j++) {
+                const int x = s_wsum[t][j];
+                wpre += j < w ? x : 0;
+                tot += x;
+            }
+            const int c = base + t * SCAN_B + (int)threadIdx.x;
+            if (c < d.ncells) {
+                d.cell_first[c] = carry + wpre + ex[t];
+                d.cell_count[c] = v[t];
+                d.cell_fill[c] = 0;
+                d.cell_next[c] = 0;
+            }
+            carry += tot;
         }
-        if (c < d.ncells) {
-            d.cell_first[c] = carry + wpre + incl - v;
-            d.cell_count[c] = v;
-            d.cell_fill[c] = 0;
-            d.cell_next[c] = 0;
-        }
-        carry += tot;
-        __syncthreads();                              // s_wsum is rewritten next tile
+        __syncthreads();                              // s_wsum is rewritten next iteration
     }
 }
 

@@ -44,19 +44,22 @@ __device__ __forceinline__ void mt_fill_shifts(float4* sh, const MtParams& p)
                             (float)(c / 9 - 1) * p.cs[2], 0.f);
 }
 
-// first statement of every pair-force kernel: one thread clears the
-// displacement maximum; no kernel of the step reads it after the list build
-#define MT_RESET_DISP(d)                                   \
-    if (blockIdx.x == 0 && threadIdx.x == 0) (d).flags[0] = 0u
+// The rebuild flag has two slots. The conditional list kernels read
+// flags[dslot]; a kernel that moves atoms (a front half) first clears that
+// slot, whose readers have all run, and sets flags[dslot ^ 1], which no
+// kernel of its launch reads, when an atom has moved more than half the skin;
+// the host then flips dslot. So no launch both clears and sets one slot.
+#define MT_WRITER_CLEAR(d, p)                              \
+    if (blockIdx.x == 0 && threadIdx.x == 0) (d).flags[(p).dslot] = 0u
 
-__device__ __forceinline__ bool mt_skip_rebuild(int cond, const unsigned int* flags, float thr)
+__device__ __forceinline__ bool mt_skip_rebuild(int cond, const unsigned int* flags, int dslot, float thr)
 {
-    return cond && !(__uint_as_float(flags[0]) > thr);
+    return cond && !(__uint_as_float(flags[dslot]) > thr);
 }
 
-__global__ void k_fast_fill(int* p, int v, int n, int cond, const unsigned int* flags, float thr)
+__global__ void k_fast_fill(int* p, int v, int n, int cond, const unsigned int* flags, int dslot, float thr)
 {
-    if (mt_skip_rebuild(cond, flags, thr)) return;
+    if (mt_skip_rebuild(cond, flags, dslot, thr)) return;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
 }
@@ -77,9 +80,9 @@ __device__ __forceinline__ void mt_key_runs(int key, int& head_lane, int& len)
 __global__ void __launch_bounds__(TPB) k_fast_cell_keys(const double4* __restrict__ pos, int n, int Nx, int Ny,
                                                         int Nz, double LX, double LY, double LZ,
                                                         int* __restrict__ key, int* __restrict__ count, int cond,
-                                                        unsigned int* flags, float thr)
+                                                        unsigned int* flags, int dslot, float thr)
 {
-    if (mt_skip_rebuild(cond, flags, thr)) return;
+    if (mt_skip_rebuild(cond, flags, dslot, thr)) return;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     int c = -1;
     if (k < n) {
@@ -103,9 +106,9 @@ __global__ void __launch_bounds__(TPB) k_fast_cell_keys(const double4* __restric
 // then each block adds the sums of the blocks before it to its own scan
 #define SCAN_B 1024
 __global__ void __launch_bounds__(SCAN_B) k_fast_scan_sums(const int* __restrict__ count, int* __restrict__ bsum,
-                                                           int ncell, int cond, const unsigned int* flags, float thr)
+                                                           int ncell, int cond, const unsigned int* flags, int dslot, float thr)
 {
-    if (mt_skip_rebuild(cond, flags, thr)) return;
+    if (mt_skip_rebuild(cond, flags, dslot, thr)) return;
     __shared__ int ws[SCAN_B / 64];
     const int c = blockIdx.x * SCAN_B + threadIdx.x;
     int v = c < ncell ? count[c] : 0;
@@ -122,9 +125,9 @@ __global__ void __launch_bounds__(SCAN_B) k_fast_scan_sums(const int* __restrict
 __global__ void __launch_bounds__(SCAN_B) k_fast_scan_apply(const int* __restrict__ count,
                                                             const int* __restrict__ bsum, int* __restrict__ first,
                                                             int* __restrict__ fill, int ncell, int cond,
-                                                            const unsigned int* flags, float thr)
+                                                            const unsigned int* flags, int dslot, float thr)
 {
-    if (mt_skip_rebuild(cond, flags, thr)) return;
+    if (mt_skip_rebuild(cond, flags, dslot, thr)) return;
     __shared__ int ws[SCAN_B / 64];
     __shared__ int s_base;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
@@ -162,9 +165,9 @@ __global__ void __launch_bounds__(SCAN_B) k_fast_scan_apply(const int* __restric
 __global__ void __launch_bounds__(TPB) k_fast_scatter(const int* __restrict__ key, int n,
                                                       const int* __restrict__ first, int* __restrict__ fill,
                                                       int* __restrict__ list, int cond, const unsigned int* flags,
-                                                      float thr)
+                                                      int dslot, float thr)
 {
-    if (mt_skip_rebuild(cond, flags, thr)) return;
+    if (mt_skip_rebuild(cond, flags, dslot, thr)) return;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63;
     const int c = k < n ? key[k] : -1;
@@ -209,7 +212,7 @@ __device__ __forceinline__ void mt_fast_home_atom(const MtDev& d, const MtParams
 __global__ void __launch_bounds__(TPB) k_fast_cell_sort(MtDev d, MtParams p, int cond, int home)
 {
     __shared__ __align__(16) int lds[(TPB / 64) * MT_FAST_SORT_LDS];
-    if (mt_skip_rebuild(cond, d.flags, p.half_skin2) || d.flags[3]) return;   // uniform over the grid
+    if (mt_skip_rebuild(cond, d.flags, p.dslot, p.half_skin2) || d.flags[3]) return;   // uniform over the grid
     if (home && blockIdx.x == 0 && threadIdx.x == 0) d.flags[4] += 1;        // rebuild count
     const int c = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int cnt = c < d.ncells ? d.cell_count[c] : 0;
@@ -232,9 +235,9 @@ __global__ void __launch_bounds__(TPB) k_fast_cell_sort(MtDev d, MtParams p, int
 // with one L2 per XCD that cost 0.38 ms a build on the 1M-atom mixture.)
 #define MT_SCAN_ONE_BLOCK 65536
 #define MT_SCAN_ONE_TPI 4                 // tiles of SCAN_B cells an iteration
-__global__ void __launch_bounds__(SCAN_B) k_fast_scan_one(MtDev d, int cond, float thr)
+__global__ void __launch_bounds__(SCAN_B) k_fast_scan_one(MtDev d, int cond, int dslot, float thr)
 {
-    if (mt_skip_rebuild(cond, d.flags, thr)) return;
+    if (mt_skip_rebuild(cond, d.flags, dslot, thr)) return;
     __shared__ int s_wsum[MT_SCAN_ONE_TPI][SCAN_B / 64];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     int carry = 0;
@@ -283,24 +286,24 @@ void mt_fast_cells(const MtDev& d, const MtParams& p, int cond, int with_home, h
     const float thr = p.half_skin2;
     if (d.ncells <= MT_SCAN_ONE_BLOCK) {
         hipLaunchKernelGGL(k_fast_cell_keys, dim3(nblk(d.n, TPB)), dim3(TPB), 0, s, d.pos, d.n, d.nc[0], d.nc[1],
-                           d.nc[2], p.L[0], p.L[1], p.L[2], d.cell_key, d.cell_next, cond, d.flags, thr);
-        hipLaunchKernelGGL(k_fast_scan_one, dim3(1), dim3(SCAN_B), 0, s, d, cond, thr);
+                           d.nc[2], p.L[0], p.L[1], p.L[2], d.cell_key, d.cell_next, cond, d.flags, p.dslot, thr);
+        hipLaunchKernelGGL(k_fast_scan_one, dim3(1), dim3(SCAN_B), 0, s, d, cond, p.dslot, thr);
         hipLaunchKernelGGL(k_fast_scatter, dim3(nblk(d.n, TPB)), dim3(TPB), 0, s, d.cell_key, d.n, d.cell_first,
-                           d.cell_fill, d.cell_list, cond, d.flags, thr);
+                           d.cell_fill, d.cell_list, cond, d.flags, p.dslot, thr);
         hipLaunchKernelGGL(k_fast_cell_sort, dim3(nblk(d.ncells, TPB / 64)), dim3(TPB), 0, s, d, p, cond, with_home);
         return;
     }
     const int nsb = nblk(d.ncells, SCAN_B);
     hipLaunchKernelGGL(k_fast_fill, dim3(nblk(d.ncells, TPB)), dim3(TPB), 0, s, d.cell_count, 0, d.ncells, cond,
-                       d.flags, thr);
+                       d.flags, p.dslot, thr);
     hipLaunchKernelGGL(k_fast_cell_keys, dim3(nblk(d.n, TPB)), dim3(TPB), 0, s, d.pos, d.n, d.nc[0], d.nc[1],
-                       d.nc[2], p.L[0], p.L[1], p.L[2], d.cell_key, d.cell_count, cond, d.flags, thr);
+                       d.nc[2], p.L[0], p.L[1], p.L[2], d.cell_key, d.cell_count, cond, d.flags, p.dslot, thr);
     hipLaunchKernelGGL(k_fast_scan_sums, dim3(nsb), dim3(SCAN_B), 0, s, d.cell_count, d.cell_bsum, d.ncells, cond,
-                       d.flags, thr);
+                       d.flags, p.dslot, thr);
     hipLaunchKernelGGL(k_fast_scan_apply, dim3(nsb), dim3(SCAN_B), 0, s, d.cell_count, d.cell_bsum, d.cell_first,
-                       d.cell_fill, d.ncells, cond, d.flags, thr);
+                       d.cell_fill, d.ncells, cond, d.flags, p.dslot, thr);
     hipLaunchKernelGGL(k_fast_scatter, dim3(nblk(d.n, TPB)), dim3(TPB), 0, s, d.cell_key, d.n, d.cell_first,
-                       d.cell_fill, d.cell_list, cond, d.flags, thr);
+                       d.cell_fill, d.cell_list, cond, d.flags, p.dslot, thr);
     hipLaunchKernelGGL(k_fast_cell_sort, dim3(nblk(d.ncells, TPB / 64)), dim3(TPB), 0, s, d, p, cond, with_home);
 }
 
@@ -390,7 +393,7 @@ __device__ inline bool mt_is_partner(const MtDev& d, int k, int ia)
 // then scans the staged cells of its own neighbourhood with broadcast reads,
 // four candidates at a time, and writes its hits as 16-byte chunks of the
 // chunked column-major layout. In the MIXED tile mode a block whose staged
-// cells fit NB_CAP gets 16-bit words (the neighbour's staged place, eight a
+// cells fit FT_CAP gets 16-bit words (the neighbour's staged place, eight a
 // chunk, mt_neigh_block_atoms16); every other block gets 32-bit words (word
 // index | offset code, four a chunk, mt_neigh_block_atoms), its cells staged
 // in batches of whole cells when they hold more than NB_CAP atoms.
@@ -405,7 +408,8 @@ __device__ inline bool mt_is_partner(const MtDev& d, int k, int ia)
 #define NB_CAP 2048
 #define NB_PAD_CODE 13u      // offset code of (0, 0, 0): a pad word is the atom itself
 static_assert(NB_CAP == MT_NB_STAGE_ATOMS, "staging capacity of the 32-bit list build");
-static_assert(NB_CAP / 2 == MT_NB_STAGE_ATOMS_TILE, "staging capacity of the tile-mode list build");
+static_assert(MT_FT_CAP <= NB_CAP && MT_FT_CAP % 8 == 0, "16-bit blocks fit the 32-bit staging");
+#define FT_CAP MT_FT_CAP
 
 // slot of a list-word index (the index itself unless the words are in cell order)
 __device__ __forceinline__ int mt_id_slot(const MtDev& d, int id) { return d.wsorted ? d.cell_list[id] : id; }
@@ -577,7 +581,7 @@ __device__ __forceinline__ float mt_half_ulp16(float v)
     return __uint_as_float((uint32_t)((int)(__float_as_uint(v) >> 23) - 11) << 23);
 }
 
-// the 16-bit lists of a block whose 64 staged cells fit NB_CAP: the cells are
+// the 16-bit lists of a block whose 64 staged cells fit FT_CAP: the cells are
 // staged once, in the frame of the centre of the block's first cell (as
 // k_fast_force_tile stages them: cell offsets times cell widths, exact in
 // FP32), so the three cells ox = -1..1 of a (oy, oz) row of a home cell's
@@ -631,7 +635,7 @@ __device__ __forceinline__ void mt_neigh_block_atoms16(const MtDev& d, const MtP
     const int n = d.n;
     const int cap = 2 * d.ncap;                       // 16-bit words a row holds
     uint4* __restrict__ nl4 = reinterpret_cast<uint4*>(d.nlist);
-    // the staging buffer (NB_CAP / 2 float4s) as NB_CAP (FP16 x | FP16 y,
+    // the staging buffer (FT_CAP / 2 float4s) as FT_CAP (FP16 x | FP16 y,
     // FP32 z) pairs; MODE 2 keeps the word indices in sword
     uint2* sxz = reinterpret_cast<uint2*>(spos);
     // the screen's threshold: staged |x|, |y| <= 2.5 cs (cells at offsets -1..2
@@ -708,14 +712,14 @@ __device__ __forceinline__ void mt_neigh_block_atoms16(const MtDev& d, const MtP
 }
 
 // TILE: the launch of the MIXED tile mode (d.tile16), whose blocks mostly take
-// 16-bit words: 16 KB of staging (NB_CAP FP16/FP32 coordinate pairs), twice
+// 16-bit words: FT_CAP FP16/FP32 coordinate pairs of staging (16 KB), twice
 // the workgroups a CU of the full 32 KB buffer; the blocks that keep 32-bit
-// words then stage their cells in batches of NB_CAP / 2 atoms
+// words then stage their cells in batches of FT_CAP / 2 atoms
 template <int MODE, bool TILE>
 __global__ void __launch_bounds__(256) k_fast_neigh_block(MtDev d, MtParams p, int cond)
 {
-    if (mt_skip_rebuild(cond, d.flags, p.half_skin2)) return;
-    constexpr int SCAP = TILE ? NB_CAP / 2 : NB_CAP;
+    if (mt_skip_rebuild(cond, d.flags, p.dslot, p.half_skin2)) return;
+    constexpr int SCAP = TILE ? MT_NB_STAGE_ATOMS_TILE : NB_CAP;
     __shared__ float4 spos[SCAP];
     __shared__ int smol_buf[MODE >= 1 ? NB_CAP : 1];
     __shared__ int sword_buf[(TILE && MODE == 2) ? NB_CAP : 1];
@@ -767,7 +771,7 @@ __global__ void __launch_bounds__(256) k_fast_neigh_block(MtDev d, MtParams p, i
     }
     // the word format of this block's lists (uniform; k_fast_force_tile makes
     // the same choice from the same counts)
-    if (TILE && s_off[64] <= NB_CAP)
+    if (TILE && s_off[64] <= FT_CAP)
         mt_neigh_block_atoms16<MODE>(d, p, spos, smol_buf, sword_buf, s_first, s_cnt, s_off, s_hfirst, s_hcnt);
     else
         mt_neigh_block_atoms<MODE, SCAP>(d, p, spos, smol_buf, s_first, s_cnt, s_off, hxo, hyo, hzo, l, f0, cnt_c,
@@ -790,7 +794,7 @@ __global__ void __launch_bounds__(256) k_fast_neigh_block(MtDev d, MtParams p, i
 template <int MODE>
 __global__ void __launch_bounds__(256) k_fast_neigh_dense(MtDev d, MtParams p, int cond)
 {
-    if (mt_skip_rebuild(cond, d.flags, p.half_skin2)) return;
+    if (mt_skip_rebuild(cond, d.flags, p.dslot, p.half_skin2)) return;
     __shared__ float4 spos[NBW_CAP];
     __shared__ int smol_buf[MODE >= 1 ? NBW_CAP : 1];
     const int t = threadIdx.x, lane = t & 63;
@@ -1089,7 +1093,6 @@ __device__ __forceinline__ void mt_lane_sum(double& x, double& y, double& z, flo
 template <bool WANT_E, bool COUL, int G>
 __global__ void __launch_bounds__(512) k_fast_force_mixed(MtDev d, MtParams p)
 {
-    MT_RESET_DISP(d);
     __shared__ float4 sh[27];
     mt_fill_shifts(sh, p);
     __syncthreads();
@@ -1117,7 +1120,6 @@ __global__ void __launch_bounds__(512) k_fast_force_mixed(MtDev d, MtParams p)
 template <bool COUL, int G>
 __global__ void __launch_bounds__(512) k_fast_force_mixed_lds(MtDev d, MtParams p)
 {
-    MT_RESET_DISP(d);
     extern __shared__ float2 stab[];
     __shared__ float4 sh[27];
     const int T = d.T;
@@ -1157,10 +1159,6 @@ __global__ void __launch_bounds__(512) k_fast_force_mixed_lds(MtDev d, MtParams 
 // workgroup whose staged cells hold more than FT_CAP atoms has 32-bit coded
 // words and gathers from global memory instead (per-offset shifts). G lanes
 // per home atom.
-#define FT_CAP 2048
-#if FT_CAP != NB_CAP
-#error the force tiles and the list build must stage the same blocks
-#endif
 
 // four pairs of a lane, without a branch on the common path: a pair outside
 // rc (or the atom itself) still reads a segment and its force is selected
@@ -1403,10 +1401,23 @@ __device__ __forceinline__ int mt_tile_home(const int* s_hoff, int v)
 
 // WANT_E: per-atom energies too, with the (a_f, b_f, a_e, b_e) coefficients
 // read from global memory; otherwise (a_f, b_f) of every type pair staged in LDS
-template <bool WANT_E, bool COUL, int G>
-__global__ void __launch_bounds__(256 * G) k_fast_force_tile(MtDev d, MtParams p)
+// the NVE integration of one atom and the rebuild flag (defined with the
+// integrator kernels below)
+__device__ __forceinline__ float mt_fast_nve_atom(const MtDev& d, const MtParams& p, int k, double4& F, double4& pm,
+                                                  double4& r, bool& capped, int mixed, float4* posf_out, bool with_w);
+__device__ __forceinline__ void mt_fast_flag_disp(const MtDev& d, const MtParams& p, float d2);
+
+// FUSE (NVE steps of a call but the last, no forces added after this pass):
+// each home atom's force is followed at once by the back half of this step
+// and the front half of the next (k_fast_nve_fused's arithmetic, the same
+// bits), the new positions written to pos2 and posf2, since the staged
+// positions of this pass must not change under other workgroups; the host
+// then swaps the buffers
+template <bool WANT_E, bool COUL, int G, bool FUSE = false>
+__global__ void __launch_bounds__(256 * G) __attribute__((amdgpu_waves_per_eu(4)))
+k_fast_force_tile(MtDev d, MtParams p)
 {
-    MT_RESET_DISP(d);
+    if constexpr (FUSE) MT_WRITER_CLEAR(d, p);
     __shared__ float4 spos[FT_CAP];
     __shared__ float sq_buf[COUL ? FT_CAP : 1];
     __shared__ float4 shd[27];                        // 32-bit blocks: shift of each offset code
@@ -1416,10 +1427,15 @@ __global__ void __launch_bounds__(256 * G) k_fast_force_tile(MtDev d, MtParams p
     const int T = d.T;
     // the segment coefficients in LDS (read from global memory instead, L1/L2
     // hits, the pass measured 1.7x slower on the 1M-atom mixture)
-    extern __shared__ float2 stab[];                  // T * T * MT_NPTS segment coefficients
+    // T * T rows of MT_NPTS segment coefficients, or 2T - 1 symmetric rows
+    // (d.tabsym: row ti + tj, so a pair's row offset is the sum of the two
+    // staged type offsets)
+    extern __shared__ float2 stab[];
+    const int trow = d.tabsym ? 1 : T;                // row offset factor of the home atom's type
     if constexpr (!WANT_E) {
-        const int ntab = T * T * MT_NPTS;
-        for (int i = t; i < ntab; i += blockDim.x) stab[i] = d.nbcoef2[i];
+        const int ntab = (d.tabsym ? 2 * T - 1 : T * T) * MT_NPTS;
+        const float2* src = d.tabsym ? d.nbcoef2s : d.nbcoef2;
+        for (int i = t; i < ntab; i += blockDim.x) stab[i] = src[i];
     }
     int x0, y0, z0;
     mt_tile_block(d, x0, y0, z0);
@@ -1458,8 +1474,9 @@ __global__ void __launch_bounds__(256 * G) k_fast_force_tile(MtDev d, MtParams p
             // the home atom's own staged entry (home cell h is staged cell qh)
             const int qh = (1 + (h & 1)) + 4 * ((1 + ((h >> 1) & 1)) + 4 * (1 + (h >> 2)));
             const float4 pk = spos[s_off[qh] + (v - s_hoff[h])];
-            // the home atom's row: its type offset (bytes) times T
-            const size_t rowb = (size_t)__float_as_int(pk.w) * T;
+            // the home atom's row: its type offset (bytes) times T, or itself
+            // in symmetric rows (the energy pass reads the full rows)
+            const size_t rowb = (size_t)__float_as_int(pk.w) * (WANT_E ? T : trow);
             if constexpr (WANT_E)
                 mt_tile_atom16<true, COUL>(d, p, k, pk, spos, sq_buf, s_first, s_off,
                                            reinterpret_cast<const float4*>(reinterpret_cast<const char*>(d.nbcoef) + rowb),
@@ -1474,8 +1491,8 @@ __global__ void __launch_bounds__(256 * G) k_fast_force_tile(MtDev d, MtParams p
                 mt_tile_atom32<true, COUL>(d, p, k, pk, shd, d.nbcoef + (size_t)__float_as_int(pk.w) * T, qk, Fx, Fy,
                                            Fz, U, gl, G);
             else
-                mt_tile_atom32<false, COUL>(d, p, k, pk, shd, stab + __float_as_int(pk.w) * T, qk, Fx, Fy, Fz, U, gl,
-                                            G);
+                mt_tile_atom32<false, COUL>(d, p, k, pk, shd, stab + __float_as_int(pk.w) * trow, qk, Fx, Fy, Fz, U,
+                                            gl, G);
         }
         mt_lane_sum<G>(Fx, Fy, Fz, U);
         if (gl == 0) {
@@ -1483,6 +1500,23 @@ __global__ void __launch_bounds__(256 * G) k_fast_force_tile(MtDev d, MtParams p
             d.frc[k] = make_double4(Fx, Fy, Fz, 0.0);
             if (WANT_E) d.epot[k] = U;
         }
+    }
+    if constexpr (FUSE) {
+        // every lane integrates the block's home atoms, their forces read back
+        // from this workgroup's own stores
+        __syncthreads();
+        float d2max = 0.0f;
+        for (int v = t; v < nh; v += (int)blockDim.x) {
+            const int h = mt_tile_home(s_hoff, v);
+            const int k = d.cell_list[s_hfirst[h] + (v - s_hoff[h])];
+            double4 F = d.frc[k], pm = d.mom[k], r = d.pos[k];
+            bool capped;
+            d2max = fmaxf(d2max, mt_fast_nve_atom(d, p, k, F, pm, r, capped, 1, d.posf2, true));
+            if (capped) d.frc[k] = F;
+            d.mom[k] = pm;
+            d.pos2[k] = r;
+        }
+        mt_fast_flag_disp(d, p, d2max);
     }
 }
 
@@ -1492,7 +1526,6 @@ __global__ void __launch_bounds__(256 * G) k_fast_force_tile(MtDev d, MtParams p
 template <bool WANT_E, bool COUL, int G>
 __global__ void __launch_bounds__(256) k_fast_force_fp64(MtDev d, MtParams p)
 {
-    MT_RESET_DISP(d);
     const int t = mt_xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const int k = t / G, g = t % G;
     if (k >= d.n) return;                       // all G lanes of an atom leave together
@@ -1865,6 +1898,7 @@ static void mt_launch_pairs(const MtDev& d, const MtParams& p, int mixed, int wa
         // every MIXED pass of these lists, energies included
         const int nb = ((d.nc[0] + 1) / 2) * ((d.nc[1] + 1) / 2) * ((d.nc[2] + 1) / 2);
         const dim3 grid(nb), tpb(256 * MT_TILE_G);
+        if (d.tabsym) lds_tab = (size_t)(2 * d.T - 1) * MT_NPTS * sizeof(float2);
         if (want_e) {
             if (coul) hipLaunchKernelGGL((k_fast_force_tile<true, true, MT_TILE_G>), grid, tpb, 0, s, d, p);
             else hipLaunchKernelGGL((k_fast_force_tile<true, false, MT_TILE_G>), grid, tpb, 0, s, d, p);
@@ -1900,6 +1934,25 @@ static void mt_launch_pairs(const MtDev& d, const MtParams& p, int mixed, int wa
 
 int mt_fast_lanes(const MtDev& d) { return mt_lanes_per_atom(d.n, d.ncap, mt_cu_count()); }
 
+int mt_fast_can_fuse(const MtDev& d, const MtParams& p, int mixed)
+{
+    const bool ap = p.coulomb_on && p.coulomb_allpairs;
+    const bool extra = d.bl.has_bonds || d.bl.has_angles || d.bl.has_dihedrals || d.sp.n_entries > 0;
+    return (mixed && d.tile16 && d.posf2 && !ap && !extra) ? 1 : 0;
+}
+
+void mt_fast_forces_fused(const MtDev& d, MtParams& p, hipStream_t s)
+{
+    const int nb = ((d.nc[0] + 1) / 2) * ((d.nc[1] + 1) / 2) * ((d.nc[2] + 1) / 2);
+    const size_t lds_tab = (size_t)(d.tabsym ? 2 * d.T - 1 : d.T * d.T) * MT_NPTS * sizeof(float2);
+    const dim3 grid(nb), tpb(256 * MT_TILE_G);
+    if (p.coulomb_on)
+        hipLaunchKernelGGL((k_fast_force_tile<false, true, MT_TILE_G, true>), grid, tpb, lds_tab, s, d, p);
+    else
+        hipLaunchKernelGGL((k_fast_force_tile<false, false, MT_TILE_G, true>), grid, tpb, lds_tab, s, d, p);
+    p.dslot ^= 1;
+}
+
 void mt_fast_forces(const MtDev& d, const MtParams& p, int mixed, int want_e, int block, hipStream_t s)
 {
     const bool coul_all = p.coulomb_on != 0;
@@ -1929,8 +1982,11 @@ void mt_fast_forces(const MtDev& d, const MtParams& p, int mixed, int want_e, in
 // (contraction off in both, so fused and separate steps agree bit for bit)
 // displacement since the last rebuild (squared, minimum image) and, MIXED,
 // the position relative to the centre of the atom's home cell
-__device__ __forceinline__ float mt_fast_locate_atom(const MtDev& d, const MtParams& p, int k, const double4& r,
-                                                     int mixed)
+// posf_out: where the MIXED local coordinates go (at the atom's place in cell
+// order); with_w: the whole float4, the type offset in .w included (a buffer
+// the rebuild did not write), else x, y, z only
+__device__ __forceinline__ float mt_fast_locate_atom_to(const MtDev& d, const MtParams& p, int k, const double4& r,
+                                                        int mixed, float4* posf_out, bool with_w)
 {
     MT_EXACT_FP
     const double4 q = d.posref[k];
@@ -1947,23 +2003,66 @@ __device__ __forceinline__ float mt_fast_locate_atom(const MtDev& d, const MtPar
         lx -= p.L[0] * rint(lx / p.L[0]);
         ly -= p.L[1] * rint(ly / p.L[1]);
         lz -= p.L[2] * rint(lz / p.L[2]);
-        // x, y, z only: .w (the type offset) stays, and the line is not read
-        float* pf = reinterpret_cast<float*>(d.posf + d.sidx[k]);
-        *reinterpret_cast<float2*>(pf) = make_float2((float)lx, (float)ly);
-        pf[2] = (float)lz;
+        if (with_w) {
+            posf_out[d.sidx[k]] = make_float4((float)lx, (float)ly, (float)lz, __int_as_float(d.type[k] * MT_NPTS));
+        } else {
+            // x, y, z only: .w (the type offset) stays, and the line is not read
+            float* pf = reinterpret_cast<float*>(posf_out + d.sidx[k]);
+            *reinterpret_cast<float2*>(pf) = make_float2((float)lx, (float)ly);
+            pf[2] = (float)lz;
+        }
     }
     return (float)(dx * dx + dy * dy + dz * dz);
 }
 
-__device__ __forceinline__ float mt_fast_front(const MtDev& d, const MtParams& p, int k, double4& pm, double4& r,
-                                               const double4& F, int mixed)
+__device__ __forceinline__ float mt_fast_locate_atom(const MtDev& d, const MtParams& p, int k, const double4& r,
+                                                     int mixed)
+{
+    return mt_fast_locate_atom_to(d, p, k, r, mixed, d.posf, false);
+}
+
+__device__ __forceinline__ float mt_fast_front_to(const MtDev& d, const MtParams& p, int k, double4& pm, double4& r,
+                                                  const double4& F, int mixed, float4* posf_out, bool with_w)
 {
     MT_EXACT_FP
     const double dt_2 = p.dt / 2.0;
     pm.x += F.x * dt_2; pm.y += F.y * dt_2; pm.z += F.z * dt_2;
     const double c = p.dt / pm.w;
     r.x += pm.x * c; r.y += pm.y * c; r.z += pm.z * c;
-    return mt_fast_locate_atom(d, p, k, r, mixed);
+    return mt_fast_locate_atom_to(d, p, k, r, mixed, posf_out, with_w);
+}
+
+__device__ __forceinline__ float mt_fast_front(const MtDev& d, const MtParams& p, int k, double4& pm, double4& r,
+                                               const double4& F, int mixed)
+{
+    return mt_fast_front_to(d, p, k, pm, r, F, mixed, d.posf, false);
+}
+
+// The back half of step t (force cap, half kick, PBC wrap; CVelVerlet +
+// pbcWrap, VelVerlet.cpp:96-122, SimulationBox.cpp:135-166) and the front
+// half of step t + 1 (half kick, drift, displacement, local coordinates) of
+// atom k for the force F of step t, which comes back capped. Returns the
+// squared displacement since the last rebuild.
+__device__ __forceinline__ float mt_fast_nve_atom(const MtDev& d, const MtParams& p, int k, double4& F, double4& pm,
+                                                  double4& r, bool& capped, int mixed, float4* posf_out, bool with_w)
+{
+    MT_EXACT_FP
+    capped = false;
+    if (p.Fcut > 0) {
+        if (fabs(F.x) > p.Fcut) { F.x = ((F.x >= 0.0) ? 1.0 : -1.0) * p.Fcut; capped = true; }
+        if (fabs(F.y) > p.Fcut) { F.y = ((F.y >= 0.0) ? 1.0 : -1.0) * p.Fcut; capped = true; }
+        if (fabs(F.z) > p.Fcut) { F.z = ((F.z >= 0.0) ? 1.0 : -1.0) * p.Fcut; capped = true; }
+    }
+    const double dt_2 = p.dt / 2.0;
+    pm.x = pm.x + F.x * dt_2; pm.y = pm.y + F.y * dt_2; pm.z = pm.z + F.z * dt_2;
+    const double X2 = p.L[0] / 2.0, Y2 = p.L[1] / 2.0, Z2 = p.L[2] / 2.0;
+    if (r.x >= X2) mt_pbc_add(r.x, X2, p.L[0]);
+    if (r.x < -X2) mt_pbc_add(r.x, -X2, p.L[0]);
+    if (r.y >= Y2) mt_pbc_add(r.y, Y2, p.L[1]);
+    if (r.y < -Y2) mt_pbc_add(r.y, -Y2, p.L[1]);
+    if (r.z >= Z2) mt_pbc_add(r.z, Z2, p.L[2]);
+    if (r.z < -Z2) mt_pbc_add(r.z, -Z2, p.L[2]);
+    return mt_fast_front_to(d, p, k, pm, r, F, mixed, posf_out, with_w);
 }
 
 // only "has any atom moved beyond skin/2" matters: one atomic per wavefront
@@ -1971,8 +2070,9 @@ __device__ __forceinline__ float mt_fast_front(const MtDev& d, const MtParams& p
 __device__ __forceinline__ void mt_fast_flag_disp(const MtDev& d, const MtParams& p, float d2)
 {
     for (int off = 32; off > 0; off >>= 1) d2 = fmaxf(d2, __shfl_xor(d2, off));
-    if ((threadIdx.x & 63) == 0 && d2 > p.half_skin2 && !(__uint_as_float(d.flags[0]) > p.half_skin2))
-        atomicMax(&d.flags[0], __float_as_uint(d2));
+    unsigned int* f = d.flags + (p.dslot ^ 1);
+    if ((threadIdx.x & 63) == 0 && d2 > p.half_skin2 && !(__uint_as_float(*f) > p.half_skin2))
+        atomicMax(f, __float_as_uint(d2));
 }
 
 // NVE steps inside one call: the back half of step t (force cap, half kick,
@@ -1981,28 +2081,14 @@ __device__ __forceinline__ void mt_fast_flag_disp(const MtDev& d, const MtParams
 // capped force is written back only where the cap changed it.
 __global__ void __launch_bounds__(TPB) k_fast_nve_fused(MtDev d, MtParams p, int mixed)
 {
-    MT_EXACT_FP
+    MT_WRITER_CLEAR(d, p);
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     float d2 = 0.0f;
     if (k < d.n) {
         double4 F = d.frc[k], pm = d.mom[k], r = d.pos[k];
-        if (p.Fcut > 0) {
-            bool capped = false;
-            if (fabs(F.x) > p.Fcut) { F.x = ((F.x >= 0.0) ? 1.0 : -1.0) * p.Fcut; capped = true; }
-            if (fabs(F.y) > p.Fcut) { F.y = ((F.y >= 0.0) ? 1.0 : -1.0) * p.Fcut; capped = true; }
-            if (fabs(F.z) > p.Fcut) { F.z = ((F.z >= 0.0) ? 1.0 : -1.0) * p.Fcut; capped = true; }
-            if (capped) d.frc[k] = F;
-        }
-        const double dt_2 = p.dt / 2.0;
-        pm.x = pm.x + F.x * dt_2; pm.y = pm.y + F.y * dt_2; pm.z = pm.z + F.z * dt_2;
-        const double X2 = p.L[0] / 2.0, Y2 = p.L[1] / 2.0, Z2 = p.L[2] / 2.0;
-        if (r.x >= X2) mt_pbc_add(r.x, X2, p.L[0]);
-        if (r.x < -X2) mt_pbc_add(r.x, -X2, p.L[0]);
-        if (r.y >= Y2) mt_pbc_add(r.y, Y2, p.L[1]);
-        if (r.y < -Y2) mt_pbc_add(r.y, -Y2, p.L[1]);
-        if (r.z >= Z2) mt_pbc_add(r.z, Z2, p.L[2]);
-        if (r.z < -Z2) mt_pbc_add(r.z, -Z2, p.L[2]);
-        d2 = mt_fast_front(d, p, k, pm, r, F, mixed);
+        bool capped;
+        d2 = mt_fast_nve_atom(d, p, k, F, pm, r, capped, mixed, d.posf, false);
+        if (capped) d.frc[k] = F;
         d.mom[k] = pm;
         d.pos[k] = r;
     }
@@ -2011,25 +2097,29 @@ __global__ void __launch_bounds__(TPB) k_fast_nve_fused(MtDev d, MtParams p, int
 
 __global__ void __launch_bounds__(TPB) k_fast_locate(MtDev d, MtParams p, int mixed)
 {
+    MT_WRITER_CLEAR(d, p);
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     float d2 = 0.0f;
     if (k < d.n) d2 = mt_fast_locate_atom(d, p, k, d.pos[k], mixed);
     mt_fast_flag_disp(d, p, d2);
 }
 
-void mt_fast_locate(const MtDev& d, const MtParams& p, int mixed, hipStream_t s)
+void mt_fast_locate(const MtDev& d, MtParams& p, int mixed, hipStream_t s)
 {
     hipLaunchKernelGGL(k_fast_locate, dim3(nblk(d.n, TPB)), dim3(TPB), 0, s, d, p, mixed);
+    p.dslot ^= 1;
 }
 
-void mt_fast_nve_fused(const MtDev& d, const MtParams& p, int mixed, hipStream_t s)
+void mt_fast_nve_fused(const MtDev& d, MtParams& p, int mixed, hipStream_t s)
 {
     hipLaunchKernelGGL(k_fast_nve_fused, dim3(nblk(d.n, TPB)), dim3(TPB), 0, s, d, p, mixed);
+    p.dslot ^= 1;
 }
 
 __global__ void __launch_bounds__(TPB) k_fast_kick_drift(MtDev d, MtParams p, int scale_slot, int mixed)
 {
     MT_EXACT_FP
+    MT_WRITER_CLEAR(d, p);
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     float d2 = 0.0f;
     if (k < d.n) {
@@ -2046,9 +2136,10 @@ __global__ void __launch_bounds__(TPB) k_fast_kick_drift(MtDev d, MtParams p, in
     mt_fast_flag_disp(d, p, d2);
 }
 
-void mt_fast_kick_drift(const MtDev& d, const MtParams& p, int scale_slot, int mixed, hipStream_t s)
+void mt_fast_kick_drift(const MtDev& d, MtParams& p, int scale_slot, int mixed, hipStream_t s)
 {
     hipLaunchKernelGGL(k_fast_kick_drift, dim3(nblk(d.n, TPB)), dim3(TPB), 0, s, d, p, scale_slot, mixed);
+    p.dslot ^= 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -2246,7 +2337,7 @@ __global__ void __launch_bounds__(TPB) k_fast_decode_lists(MtDev d, int* __restr
         mt_tile_cells(d, 2 * (g % bx), 2 * ((g / bx) % by), 2 * (g / (bx * by)), s_first, s_cnt, s_off, s_hfirst,
                       s_hoff);
         __syncthreads();
-        const bool w16 = d.tile16 && s_off[64] <= NB_CAP;
+        const bool w16 = d.tile16 && s_off[64] <= FT_CAP;
         const int nh = s_hoff[8];
         for (int v = t; v < nh; v += blockDim.x) {
             const int h = mt_tile_home(s_hoff, v);
@@ -2283,7 +2374,7 @@ void mt_fast_decode_lists(const MtDev& d, int* out, int* cnt, int maxp, hipStrea
                        maxp);
 }
 
-__global__ void k_fast_clear(unsigned int* flags) { flags[0] = 0u; }
+__global__ void k_fast_clear(unsigned int* flags) { flags[0] = 0u; flags[1] = 0u; }
 
 void mt_fast_clear_flags(const MtDev& d, hipStream_t s)
 {

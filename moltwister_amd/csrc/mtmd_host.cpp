@@ -103,12 +103,34 @@ struct mtmd_handle {
     std::vector<int> perm_host;   // valid after download_perm()
     int* h_status = nullptr;      // pinned status words
 
-    // captured plain steps (no re-sort, not the last of a call), one per
-    // buffer parity (the re-sort swaps the slot buffers, which the kernels
-    // take by value) and ensemble; dropped when the neighbour buffer moves
-    struct StepGraph { const void* key = nullptr; hipGraph_t g = nullptr; hipGraphExec_t x = nullptr; };
-    StepGraph graphs[2];
+    // captured plain steps (no re-sort, not the last of a call), four an
+    // ensemble. The kernels take the buffers and the rebuild-flag slot by
+    // value, and a step may swap them (the re-sort swaps the slot buffers, a
+    // fused NVE pass pos/pos2 and posf/posf2, a front half flips dslot; the
+    // three together reach four states): a graph is found by the state it
+    // starts from and leaves the state its captured step left; dropped when
+    // the neighbour buffer moves
+    struct StepState {
+        const void* pos = nullptr; const void* pos2 = nullptr; const void* mom = nullptr;
+        const void* posf = nullptr; const void* posf2 = nullptr; int dslot = 0;
+        bool operator==(const StepState& o) const
+        {
+            return pos == o.pos && pos2 == o.pos2 && mom == o.mom && posf == o.posf && posf2 == o.posf2 &&
+                   dslot == o.dslot;
+        }
+    };
+    struct StepGraph { bool valid = false; StepState from, to; hipGraph_t g = nullptr; hipGraphExec_t x = nullptr; };
+    static constexpr int kGraphsPerEnsemble = 4;
+    StepGraph graphs[2 * kGraphsPerEnsemble];
+    int graph_next[2] = {0, 0};
+    StepState step_state() const { return {d.pos, d.pos2, d.mom, d.posf, d.posf2, p.dslot}; }
+    void set_step_state(const StepState& st)
+    {
+        d.pos = (double4*)st.pos; d.pos2 = (double4*)st.pos2; d.mom = (double4*)st.mom;
+        d.posf = (float4*)st.posf; d.posf2 = (float4*)st.posf2; p.dslot = st.dslot;
+    }
     bool use_graphs = true;
+    bool use_fuse = true;
     void drop_graphs()
     {
         for (auto& sg : graphs) {
@@ -349,6 +371,10 @@ static int setup(mtmd_handle* h, const mtmd_system* s)
         // MTMD_GRAPHS=0 launches every step's kernels directly (same results)
         const char* g = std::getenv("MTMD_GRAPHS");
         h->use_graphs = !(g && std::strcmp(g, "0") == 0);
+        // MTMD_FUSE=0 keeps the MIXED NVE integration out of the pair pass
+        // (same results: the same per-atom arithmetic in a kernel of its own)
+        const char* f = std::getenv("MTMD_FUSE");
+        h->use_fuse = !(f && std::strcmp(f, "0") == 0);
     }
     if (n <= 0 || T <= 0 || !s->pos || !s->mass || !s->charge || !s->type)
         return h->fail(MTMD_EINVAL, "system needs atoms, types, positions, masses and charges");
@@ -716,6 +742,26 @@ static int setup(mtmd_handle* h, const mtmd_system* s)
         std::vector<float2> coef2(coef.size());
         for (size_t i = 0; i < coef.size(); i++) coef2[i] = make_float2(coef[i].x, coef[i].y);
         UP(d.nbcoef2, coef2);
+        // the tile kernel's LDS table in symmetric rows when the two types'
+        // coefficients agree bit for bit (row ti + tj holds the pair (min, max))
+        d.tabsym = 0;
+        if (T <= 2) {
+            bool sym = true;
+            if (T == 2)
+                for (int i = 0; i < MT_NPTS && sym; i++)
+                    sym = std::memcmp(&coef2[(size_t)1 * MT_NPTS + i], &coef2[(size_t)2 * MT_NPTS + i],
+                                      sizeof(float2)) == 0;
+            if (sym) {
+                std::vector<float2> cs((size_t)(2 * T - 1) * MT_NPTS);
+                for (int r = 0; r < 2 * T - 1; r++) {
+                    const int a = r < T ? 0 : r - T + 1, b = r - a;
+                    std::memcpy(&cs[(size_t)r * MT_NPTS], &coef2[((size_t)a * T + b) * MT_NPTS],
+                                MT_NPTS * sizeof(float2));
+                }
+                UP(d.nbcoef2s, cs);
+                d.tabsym = 1;
+            }
+        }
         std::vector<int> ident(n);
         std::iota(ident.begin(), ident.end(), 0);
         UP(d.perm, ident); UP(d.inv, ident);
@@ -725,6 +771,7 @@ static int setup(mtmd_handle* h, const mtmd_system* s)
         d.type2 = h->mem.alloc<int>(n); d.mol2 = h->mem.alloc<int>(n); d.mobile2 = h->mem.alloc<uint8_t>(n);
         d.posref = h->mem.alloc<double4>(n);
         d.posf = h->mem.alloc<float4>(n);
+        d.posf2 = (c.engine == MTMD_ENGINE_MIXED) ? h->mem.alloc<float4>(n) : nullptr;
         d.home = h->mem.alloc<int>(n);
         d.sidx = h->mem.alloc<int>(n);
         d.qfs = h->mem.alloc<float>(n);
@@ -743,7 +790,7 @@ static int setup(mtmd_handle* h, const mtmd_system* s)
         for (int k = 0; k < n; k++) qf[k] = (float)s->charge[k];
         HIPCHK(h, hipMemcpy(d.qf, qf.data(), n * sizeof(float), hipMemcpyHostToDevice));
         if (!d.pos2 || !d.mom2 || !d.type2 || !d.mol2 || !d.mobile2 || !d.posref || !d.posf || !d.home || !d.qf || !d.sidx || !d.qfs ||
-            !d.qf2 || !d.perm2 || !d.frc2 || !d.epot2)
+            !d.qf2 || !d.perm2 || !d.frc2 || !d.epot2 || (c.engine == MTMD_ENGINE_MIXED && !d.posf2))
             return h->fail(MTMD_ENOMEM, "device allocation failed");
     }
 #undef UP
@@ -1484,38 +1531,64 @@ static int md_steps(mtmd_handle* h, int nsteps, bool energy_last)
             mt_nh_chain(d, h->M, h->nresp, h->Tred, h->p.dt, h->g, h->w, nblocks, h->stream);
             mt_scale_wrap(d, h->p, h->nh_scale_slot, 1, h->stream);
         };
+        // NVE in the MIXED tile mode with every force inside the pair pass:
+        // the pair pass also integrates (back half of step t, front half of
+        // t + 1) into pos2/posf2, which then become pos/posf
+        const bool fuse = !nvt && h->use_fuse && mt_fast_can_fuse(d, h->p, mixed);
+        auto fused_forces = [&]() {
+            mt_fast_forces_fused(d, h->p, h->stream);
+            std::swap(d.pos, d.pos2);
+            std::swap(d.posf, d.posf2);
+        };
         // the kernels of one plain step after its front half (NVE) or whole (NVT)
         auto plain_step = [&]() -> int {
             if (nvt) nvt_front();
             int r = fast_rebuild(h, false, false);
             if (r) return r;
+            if (fuse) {
+                fused_forces();
+                return MTMD_OK;
+            }
             if ((r = fast_forces(h, false))) return r;
             if (nvt) nvt_back();
             else mt_fast_nve_fused(d, h->p, mixed, h->stream);
             return MTMD_OK;
         };
         auto replay = [&]() -> int {
-            // graph slot by ensemble; key = current slot buffer (parity)
-            mtmd_handle::StepGraph& sg = h->graphs[nvt ? 1 : 0];
-            if (sg.key != (const void*)d.pos) {
-                if (sg.x) (void)hipGraphExecDestroy(sg.x);
-                if (sg.g) (void)hipGraphDestroy(sg.g);
-                sg = mtmd_handle::StepGraph();
+            constexpr int G = mtmd_handle::kGraphsPerEnsemble;
+            const int base = nvt ? G : 0;
+            const mtmd_handle::StepState from = h->step_state();
+            mtmd_handle::StepGraph* sg = nullptr;
+            for (int i = 0; i < G; i++)
+                if (h->graphs[base + i].valid && h->graphs[base + i].from == from) sg = &h->graphs[base + i];
+            if (!sg) {
+                // capture into the slot used least recently
+                int& nx = h->graph_next[nvt ? 1 : 0];
+                sg = &h->graphs[base + nx];
+                nx = (nx + 1) % G;
+                if (sg->x) (void)hipGraphExecDestroy(sg->x);
+                if (sg->g) (void)hipGraphDestroy(sg->g);
+                *sg = mtmd_handle::StepGraph();
                 HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
                 int r = plain_step();
                 hipGraph_t g = nullptr;
                 const hipError_t e = hipStreamEndCapture(h->stream, &g);
+                // the capture ran no kernel: the step is launched below
+                const mtmd_handle::StepState to = h->step_state();
+                h->set_step_state(from);
                 if (r) { if (g) (void)hipGraphDestroy(g); return r; }
                 if (e != hipSuccess) return h->fail(MTMD_EHIP, "step capture: %s", hipGetErrorString(e));
-                sg.g = g;
-                HIPCHK(h, hipGraphInstantiate(&sg.x, g, nullptr, nullptr, 0));
-                sg.key = (const void*)d.pos;
+                sg->g = g;
+                HIPCHK(h, hipGraphInstantiate(&sg->x, g, nullptr, nullptr, 0));
+                sg->from = from;
+                sg->to = to;
+                sg->valid = true;
             }
-            HIPCHK(h, hipGraphLaunch(sg.x, h->stream));
+            HIPCHK(h, hipGraphLaunch(sg->x, h->stream));
+            h->set_step_state(sg->to);
             return MTMD_OK;
         };
-        // the pair-force kernels reset the displacement maximum for the next
-        // step's front half; this call's first front half gets a clear of its own
+        // this call's first front half finds both rebuild-flag slots clear
         mt_fast_clear_flags(d, h->stream);
         if (!nvt) mt_fast_kick_drift(d, h->p, -1, mixed, h->stream);
         for (int t = 0; t < nsteps; t++) {
@@ -1529,6 +1602,12 @@ static int md_steps(mtmd_handle* h, int nsteps, bool energy_last)
             }
             if (nvt) nvt_front();
             if ((rc = fast_rebuild(h, sort_now, sort_now))) return rc;
+            if (fuse && !last) {
+                fused_forces();
+                h->ke_valid = false;
+                h->steps++;
+                continue;
+            }
             if ((rc = fast_forces(h, last && energy_last))) return rc;
             if (nvt) {
                 nvt_back();

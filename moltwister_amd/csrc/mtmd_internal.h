@@ -39,8 +39,13 @@
 // home-cell offset code (dx+1) + 3*(dy+1) + 9*(dz+1) in the top 5 bits
 // atoms the block list build stages in LDS at once: a cell holding more than
 // this (the tile-mode figure when d.tile16 is set) cannot be listed (flags[5])
+// MT_FT_CAP: the most atoms the 4x4x4 staged cells of a 2x2x2 block may hold
+// for the block's lists to take 16-bit words (MIXED tile mode)
+#ifndef MT_FT_CAP
+#define MT_FT_CAP 2048
+#endif
 #define MT_NB_STAGE_ATOMS 2048
-#define MT_NB_STAGE_ATOMS_TILE 1024
+#define MT_NB_STAGE_ATOMS_TILE (MT_FT_CAP / 2)
 #define MT_NW_IDX_BITS 27
 #define MT_NW_IDX_MASK ((1u << MT_NW_IDX_BITS) - 1u)
 
@@ -68,6 +73,8 @@ struct MtDev {
     float4* nbtab; int* nbcnt; float4* btab; float4* atab; float4* dtab;
     float4* nbcoef;
     float2* nbcoef2;       // force-only a_f, b_f per segment (MIXED, staged in LDS)
+    float2* nbcoef2s;      // the same in symmetric rows (tabsym): row ti + tj, 2T - 1 rows
+    int tabsym;            // T <= 2 and the pair coefficients are symmetric in the two types
     float* ecut;           // [row*T + col] pair energy at the cutoff, summed over force fields
     MtBonded bl;
     MtSpecial sp;
@@ -91,6 +98,7 @@ struct MtDev {
     float* qf; float* qf2;  // float charges in slot order
     double4* posref;       // positions at the last rebuild
     float4* posf; int* home;
+    float4* posf2;         // MIXED tile mode: the local coordinates a fused force pass writes
     // posf is kept in cell order: posf[u] belongs to atom cell_list[u] (u is
     // the atom's place in the cell lists of the last rebuild, sidx[slot] = u),
     // so the atoms of a cell are one contiguous run of posf; qfs = charges in
@@ -103,7 +111,8 @@ struct MtDev {
     double* red;           // block partials
     int red_blocks;
     double* nh;            // NH chain state: eta[M], p_eta[M], Q[M], scale, KE2
-    unsigned int* flags;   // [0] max displacement^2 (float bits) of this step, [2] list overflow,
+    unsigned int* flags;   // [0], [1] max displacement^2 (float bits) above the rebuild threshold,
+                           // one slot read, the other written (MtParams::dslot), [2] list overflow,
                            // [3] non-finite position, [4] rebuild count, [5] cell over the staging
                            // capacity
 };
@@ -122,6 +131,9 @@ struct MtParams {
     float x0, inv_delta;   // non-bonded table start and 1/spacing (MIXED)
     float cs[3];           // cell size per axis (MIXED local coordinates)
     float half_skin2;      // rebuild threshold on the squared displacement
+    int dslot;             // flags[dslot]: the rebuild flag the conditional list kernels read; the
+                           // kernels that move atoms clear it and set flags[dslot ^ 1] (the host
+                           // flips dslot after launching one of them)
 };
 
 // ---- launchers (mtmd_exact.hip) ----
@@ -164,15 +176,24 @@ void mt_fast_neighbors(const MtDev& d, const MtParams& p, int conditional, hipSt
 // the host stores it in d.tile16 whenever the cell grid is set, before a build
 int mt_fast_tile_mode(const MtDev& d, const double* L, int mixed);
 void mt_fast_forces(const MtDev& d, const MtParams& p, int mixed, int want_energy, int block, hipStream_t s);
+// 1 when the NVE step may run as one fused pass (mt_fast_forces_fused): the
+// MIXED tile mode with every force of the step inside that pass
+int mt_fast_can_fuse(const MtDev& d, const MtParams& p, int mixed);
+// pair forces of the step, then the back half of the step and the front half
+// of the next: new positions in pos2 and local coordinates in posf2, which
+// the host swaps with pos and posf; flips p.dslot
+void mt_fast_forces_fused(const MtDev& d, MtParams& p, hipStream_t s);
 // lanes per atom the pair-force kernels use for this system (1, 4 or 16)
 int mt_fast_lanes(const MtDev& d);
-void mt_fast_kick_drift(const MtDev& d, const MtParams& p, int scale_slot, int mixed, hipStream_t s);
+// the kernels that move atoms (front halves) set the rebuild flag: their
+// launchers flip p.dslot
+void mt_fast_kick_drift(const MtDev& d, MtParams& p, int scale_slot, int mixed, hipStream_t s);
 void mt_fast_clear_flags(const MtDev& d, hipStream_t s);
 // NPT: displacement since the last rebuild and MIXED local coordinates of the
 // current positions (the locating half of the front-half kernel)
-void mt_fast_locate(const MtDev& d, const MtParams& p, int mixed, hipStream_t s);
+void mt_fast_locate(const MtDev& d, MtParams& p, int mixed, hipStream_t s);
 // NVE: back half of one step and front half of the next in one pass
-void mt_fast_nve_fused(const MtDev& d, const MtParams& p, int mixed, hipStream_t s);
+void mt_fast_nve_fused(const MtDev& d, MtParams& p, int mixed, hipStream_t s);
 // in-place read + write of the whole neighbour-list buffer (HBM reference)
 void mt_copy_lists(const MtDev& d, hipStream_t s);
 // the lists of the last build in original atom indices: out[o * maxp + j]

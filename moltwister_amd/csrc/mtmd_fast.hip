@@ -358,6 +358,38 @@ __global__ void k_fast_identity(int* perm, int* inv, int n)
     if (s < n) { perm[s] = s; inv[s] = s; }
 }
 
+// positions and momenta in original atom order (sp, sm), and back into the
+// current slot order: the state a multi-step call starts from, restored when
+// a list build inside the call overflowed
+__global__ void __launch_bounds__(TPB) k_fast_snapshot(MtDev d, double4* __restrict__ sp, double4* __restrict__ sm)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= d.n) return;
+    const int o = d.perm[s];
+    sp[o] = d.pos[s];
+    sm[o] = d.mom[s];
+}
+
+__global__ void __launch_bounds__(TPB) k_fast_restore(MtDev d, const double4* __restrict__ sp,
+                                                      const double4* __restrict__ sm)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= d.n) return;
+    const int o = d.perm[s];
+    d.pos[s] = sp[o];
+    d.mom[s] = sm[o];
+}
+
+void mt_fast_snapshot(const MtDev& d, double4* sp, double4* sm, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fast_snapshot, dim3(nblk(d.n, TPB)), dim3(TPB), 0, s, d, sp, sm);
+}
+
+void mt_fast_restore(const MtDev& d, const double4* sp, const double4* sm, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fast_restore, dim3(nblk(d.n, TPB)), dim3(TPB), 0, s, d, sp, sm);
+}
+
 void mt_fast_identity_perm(const MtDev& d, hipStream_t s)
 {
     hipLaunchKernelGGL(k_fast_identity, dim3(nblk(d.n, TPB)), dim3(TPB), 0, s, d.perm, d.inv, d.n);

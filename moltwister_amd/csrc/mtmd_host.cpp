@@ -131,6 +131,20 @@ struct mtmd_handle {
     }
     bool use_graphs = true;
     bool use_fuse = true;
+    bool list_slack = true;       // lists regrow with a margin over the longest row
+    // the state a multi-step call of the FP64/MIXED engines starts from (device
+    // copies in original atom order, the thermostat chain, the host-side
+    // barostat and counters), restored to re-run the call when a list build
+    // inside it overflowed
+    struct Snapshot {
+        double4* pos = nullptr;
+        double4* mom = nullptr;
+        double* nh = nullptr;
+        double eps = 0, peps = 0, L[3] = {0, 0, 0};
+        std::vector<double> etaP, petaP;
+        int64_t steps = 0;
+        int steps_since_sort = 0;
+    } snap;
     void drop_graphs()
     {
         for (auto& sg : graphs) {
@@ -375,6 +389,11 @@ static int setup(mtmd_handle* h, const mtmd_system* s)
         // (same results: the same per-atom arithmetic in a kernel of its own)
         const char* f = std::getenv("MTMD_FUSE");
         h->use_fuse = !(f && std::strcmp(f, "0") == 0);
+        // MTMD_LIST_SLACK=0 sizes the lists to the longest row of the latest
+        // build with no margin, starting from 16 words (every list growth path,
+        // the re-run of a multi-step call included, then runs early)
+        const char* ls = std::getenv("MTMD_LIST_SLACK");
+        h->list_slack = !(ls && std::strcmp(ls, "0") == 0);
     }
     if (n <= 0 || T <= 0 || !s->pos || !s->mass || !s->charge || !s->type)
         return h->fail(MTMD_EINVAL, "system needs atoms, types, positions, masses and charges");
@@ -816,6 +835,7 @@ static int setup(mtmd_handle* h, const mtmd_system* s)
     HIPCHK(h, hipMemset(d.frc, 0, n * sizeof(double4)));
     HIPCHK(h, hipMemset(d.epot, 0, n * sizeof(float)));
     HIPCHK(h, hipMemset(d.flags, 0, 8 * sizeof(unsigned int)));
+    HIPCHK(h, hipMemset(d.nreq, 0, sizeof(int)));       // the longest row a build could not hold
     HIPCHK(h, hipMemset(d.cell_next, 0, d.ncells * sizeof(int)));
     // neighbour capacity: the expected full count at the list radius plus margin
     {
@@ -825,6 +845,7 @@ static int setup(mtmd_handle* h, const mtmd_system* s)
         d.ncap = std::max(16, (int)std::ceil(nexp * 1.35) + 16);
         if (!h->fast && (d.nc[0] < 3 || d.nc[1] < 3 || d.nc[2] < 3)) d.ncap *= 2;
         d.ncap = std::min(d.ncap, n * 27 + 16);
+        if (!h->list_slack) d.ncap = 16;
         d.ncap = (d.ncap + 3) & ~3;          // whole 16-byte chunks (FP64/MIXED list layout)
         d.nlist = h->mem.alloc<uint32_t>((size_t)d.ncap * n);
         if (!d.nlist) return h->fail(MTMD_ENOMEM, "neighbour list allocation failed");
@@ -1163,7 +1184,7 @@ extern "C" int mtmd_get_thermostat(mtmd_handle* h, double* eta, double* p_eta)
 static int grow_neighbors(mtmd_handle* h, int need)
 {
     MtDev& d = h->d;
-    const int cap = (need + need / 8 + 16 + 3) & ~3;
+    const int cap = (h->list_slack ? need + need / 8 + 16 + 3 : need + 3) & ~3;
     void* old = d.nlist;
     uint32_t* nl = nullptr;
     HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -1486,7 +1507,10 @@ static int npt_steps(mtmd_handle* h, int nsteps, bool energy_last)
 // ---------------------------------------------------------------------------
 // MD loop (MDLoop.cpp:124-138): NH_T, velocity Verlet, NH_T, wrap
 // ---------------------------------------------------------------------------
-static int md_steps(mtmd_handle* h, int nsteps, bool energy_last)
+// md_steps_once: a list build of the call overflowed; the lists have grown
+static constexpr int MT_LIST_RERUN = 1000;
+
+static int md_steps_once(mtmd_handle* h, int nsteps, bool energy_last)
 {
     MtDev& d = h->d;
     const bool nvt = (h->M > 0);
@@ -1626,15 +1650,80 @@ static int md_steps(mtmd_handle* h, int nsteps, bool energy_last)
         bool ov = false;
         if ((rc = fast_check_capacity(h, &ov))) return rc;
         if (ov) {
+            // the steps after that build ran on truncated lists: grow them for
+            // md_steps to re-run the call from its starting state
             HIPCHK(h, hipMemcpy(h->h_status, d.nreq, sizeof(int), hipMemcpyDeviceToHost));
-            rc = grow_neighbors(h, h->h_status[0]);
             h->have_forces = false;
-            if (rc) return rc;
-            return h->fail(MTMD_ENEIGH, "neighbour list overflowed during these steps (capacity now %d); the "
-                                        "state is not reliable, restart from a saved state", d.ncap);
+            if ((rc = grow_neighbors(h, h->h_status[0]))) return rc;
+            HIPCHK(h, hipMemsetAsync(d.flags + 2, 0, sizeof(unsigned int), h->stream));
+            return MT_LIST_RERUN;
         }
     }
     return MTMD_OK;
+}
+
+static int snapshot_take(mtmd_handle* h)
+{
+    mtmd_handle::Snapshot& sn = h->snap;
+    MtDev& d = h->d;
+    const int nh = 3 * std::max(h->M, 1) + 2;
+    if (!sn.pos) {
+        sn.pos = h->mem.alloc<double4>(h->n);
+        sn.mom = h->mem.alloc<double4>(h->n);
+        sn.nh = h->mem.alloc<double>(nh);
+        if (!sn.pos || !sn.mom || !sn.nh) return h->fail(MTMD_ENOMEM, "state snapshot allocation failed");
+    }
+    mt_fast_snapshot(d, sn.pos, sn.mom, h->stream);
+    HIPCHK(h, hipMemcpyAsync(sn.nh, d.nh, nh * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    sn.eps = h->eps;
+    sn.peps = h->peps;
+    sn.etaP = h->etaP;
+    sn.petaP = h->petaP;
+    for (int i = 0; i < 3; i++) sn.L[i] = h->p.L[i];
+    sn.steps = h->steps;
+    sn.steps_since_sort = h->steps_since_sort;
+    return MTMD_OK;
+}
+
+static int snapshot_restore(mtmd_handle* h)
+{
+    const mtmd_handle::Snapshot& sn = h->snap;
+    MtDev& d = h->d;
+    mt_fast_restore(d, sn.pos, sn.mom, h->stream);
+    HIPCHK(h, hipMemcpyAsync(d.nh, sn.nh, (3 * std::max(h->M, 1) + 2) * sizeof(double), hipMemcpyDeviceToDevice,
+                             h->stream));
+    HIPCHK(h, hipGetLastError());
+    h->eps = sn.eps;
+    h->peps = sn.peps;
+    h->etaP = sn.etaP;
+    h->petaP = sn.petaP;
+    set_box(h, sn.L);
+    h->steps = sn.steps;
+    h->steps_since_sort = sn.steps_since_sort;
+    // forces, lists and the kinetic sums of the restored state are rebuilt
+    h->have_forces = false;
+    h->need_sort = true;
+    h->ke_valid = false;
+    return MTMD_OK;
+}
+
+// The reference grows the list and builds it again when a row overflows
+// (MDFFMatrices.cpp:163-178). Inside a multi-step call the FP64/MIXED engines
+// learn of an overflow at the call's end: they grow the lists, restore the
+// state the call started from and run it again.
+static int md_steps(mtmd_handle* h, int nsteps, bool energy_last)
+{
+    if (!h->fast) return md_steps_once(h, nsteps, energy_last);
+    int rc = snapshot_take(h);
+    if (rc) return rc;
+    for (int attempt = 0;; attempt++) {
+        rc = md_steps_once(h, nsteps, energy_last);
+        if (rc != MT_LIST_RERUN) return rc;
+        if (attempt == 3)
+            return h->fail(MTMD_ENEIGH, "neighbour list overflowed in %d runs of these steps (capacity now %d)",
+                           attempt + 1, h->d.ncap);
+        if ((rc = snapshot_restore(h))) return rc;
+    }
 }
 
 extern "C" int mtmd_step(mtmd_handle* h, int32_t nsteps)

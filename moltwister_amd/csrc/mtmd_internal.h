@@ -169,6 +169,8 @@ void mt_fast_cells(const MtDev& d, const MtParams& p, int conditional, int with_
 void mt_fast_sort_apply(const MtDev& d, hipStream_t s);   // writes the *2 buffers, perm2 and inv
 void mt_fast_unsort(const MtDev& d, hipStream_t s);       // writes the *2 buffers in original order
 void mt_fast_identity_perm(const MtDev& d, hipStream_t s);
+void mt_fast_snapshot(const MtDev& d, double4* sp, double4* sm, hipStream_t s);        // pos, mom by original index
+void mt_fast_restore(const MtDev& d, const double4* sp, const double4* sm, hipStream_t s);
 void mt_fast_neighbors(const MtDev& d, const MtParams& p, int conditional, hipStream_t s);
 // 1 when the MIXED pair forces run on 2x2x2 cell blocks staged in LDS (sparse
 // cells, the coefficient table within 32 KB, enough blocks to fill the GPU,

@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define MTMD_ABI_VERSION 3
+#define MTMD_ABI_VERSION 4
 
 /* Number of points in every force/energy profile and the maximum number of
  * non-bonded force fields per atom-type pair
@@ -384,26 +384,52 @@ int mtmd_run_optimize(const mtmd_config* cfg, const mtmd_system* sys, void* log_
 /* ---- topology search ----
  * CMolTwisterStateTools::getAllMDBondsInSystem / getAllMDAnglesInSystem /
  * getAllMDDihedralsInSystem (Cmd/Tools/MolTwisterStateTools.cpp:544-724,
- * 767-920, 993-1140) for force fields detected with the all_r_less_than
- * criterion: chains of atoms of the rule's types, each link shorter than R0,
- * in the reference's order (rule, then first, second, ... atom ascending),
- * duplicates and reversed duplicates removed keeping the first
- * (MolecularSystemTools.cpp:25-200). which: 1 bonds, 2 angles, 3 dihedrals;
- * rule r gives the types of its atoms (the first which + 1 entries) and R0.
- * box: periodic widths for the folded distance of
+ * 767-920, 993-1140): the bonds, angles or dihedrals of each force-field
+ * rule, by the rule's detection criterion (CMDFFBond / CMDFFAngle / CMDFFDih
+ * ::EDetCrit, MDFF/Bonds/MolTwisterMDFFBond.h:43):
+ *   MTMD_CRIT_ALL_R    all_r_less_than: chains of atoms of the rule's types,
+ *                      each link shorter than R0, first atom, then second, ...
+ *                      ascending (:577-603, :805-839, :1033-1077);
+ *   MTMD_CRIT_MOL_R    mol_r_less_than: the same links, but the second and
+ *                      later atoms are taken from the first atom's molecule in
+ *                      the depth-first order of CAtom::findAtomsInMolecule
+ *                      (MolTwisterAtom.cpp:578-605) (:605-636, :843-888,
+ *                      :1084-1144);
+ *   MTMD_CRIT_VISIBLE  only_visible_bonds: chains along the bond graph, each
+ *                      atom's bonds in its own order, types matched, no
+ *                      distance test (:637-666, :889-930, :1145-1198);
+ *   MTMD_CRIT_14       only_14_bonds (bonds only): the first atom and each
+ *                      atom of the second type three bonds away along the
+ *                      graph, other than the three before it (:667-716).
+ * Duplicates and reversed duplicates are removed keeping the first, and
+ * angles / dihedrals that name one atom twice are dropped
+ * (MolecularSystemTools.cpp:25-235). which: 1 bonds, 2 angles, 3 dihedrals;
+ * rule r gives the types of its atoms (the first which + 1 entries), R0 and
+ * crit; types are indices into the caller's atom-name table (-1 matches no
+ * atom). box: periodic widths for the folded distance of
  * CAtom::getDistanceTo2UsingPBC (what the force-field matrices use), or NULL
- * for the plain distance. out: [(which + 2) * max_out] entries laid out as
- * mtmd_system's bonds / angles / dihedrals (atoms, then the rule index), or
- * NULL to count only; *n_out receives the count. Returns MTMD_EINVAL when out
- * is too short. The molecule and visible-bond criteria walk the caller's own
- * bond graph and are not searched here. Runs on the first HIP device;
- * mtmd_topology_last_error() gives the message of a failed call. */
+ * for the plain distance. bond_start [n_atoms + 1] / bond_dest: the bond graph
+ * (CAtom::bonds_, in the order CAtom::attachBondTo appended them), needed when
+ * a rule uses MTMD_CRIT_MOL_R, _VISIBLE or _14, else NULL. out:
+ * [(which + 2) * max_out] entries laid out as mtmd_system's bonds / angles /
+ * dihedrals (atoms, then the rule index), or NULL to count only; *n_out
+ * receives the count. Returns MTMD_EINVAL when out is too short. Runs on the
+ * first HIP device; mtmd_topology_last_error() gives the message of a failed
+ * call. */
+#define MTMD_CRIT_ALL_R   0
+#define MTMD_CRIT_MOL_R   1
+#define MTMD_CRIT_VISIBLE 2
+#define MTMD_CRIT_14      3
+
 typedef struct mtmd_topo_rule {
     int32_t types[4];
     double  R0;
+    int32_t crit;      /* MTMD_CRIT_* */
+    int32_t reserved;  /* 0 */
 } mtmd_topo_rule;
 
 int mtmd_find_topology(int32_t which, int32_t n_atoms, const double* pos, const int32_t* type, const double* box,
+                       const int32_t* bond_start, const int32_t* bond_dest,
                        int32_t n_rules, const mtmd_topo_rule* rules, int32_t max_out, int32_t* out,
                        int32_t* n_out);
 const char* mtmd_topology_last_error(void);

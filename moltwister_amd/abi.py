@@ -10,6 +10,8 @@ from __future__ import annotations
 import ctypes as C
 import os
 
+import numpy as np
+
 MTMD_NUM_POINTS = 1000
 MTMD_MAX_FF_PER_PAIR = 5
 
@@ -126,7 +128,39 @@ class MtmdSystem(C.Structure):
 
 
 class MtmdTopoRule(C.Structure):
-    _fields_ = [("types", C.c_int32 * 4), ("R0", C.c_double)]
+    _fields_ = [("types", C.c_int32 * 4), ("R0", C.c_double), ("crit", C.c_int32), ("reserved", C.c_int32)]
+
+
+# detection criteria of a topology rule (CMDFFBond::EDetCrit)
+CRIT_ALL_R, CRIT_MOL_R, CRIT_VISIBLE, CRIT_14 = 0, 1, 2, 3
+
+
+def topo_rules(rules):
+    """An mtmd_topo_rule array from [(types tuple, R0[, crit])]; crit defaults
+    to CRIT_ALL_R."""
+    arr = (MtmdTopoRule * max(1, len(rules)))()
+    for i, rule in enumerate(rules):
+        t, r0 = rule[0], rule[1]
+        arr[i].types[:] = list(t) + [0] * (4 - len(t))
+        arr[i].R0 = r0
+        arr[i].crit = rule[2] if len(rule) > 2 else CRIT_ALL_R
+    return arr
+
+
+def bond_graph(graph, n):
+    """(bond_start [n + 1], bond_dest) int32 arrays of a bond graph given as one
+    list of bond destinations per atom, in the atom's own bond order; (None,
+    None) for no graph."""
+    if graph is None:
+        return None, None
+    if len(graph) != n:
+        raise ValueError("the bond graph needs one list of destinations per atom")
+    start = np.zeros(n + 1, np.int32)
+    start[1:] = np.cumsum([len(g) for g in graph])
+    dest = np.array([d for g in graph for d in g], np.int32)
+    if dest.size and (dest.min() < 0 or dest.max() >= n):
+        raise ValueError("bond destination out of range")
+    return start, np.ascontiguousarray(dest if dest.size else np.zeros(1, np.int32))
 
 
 class MtmdThermo(C.Structure):
@@ -197,7 +231,7 @@ def load_library(path: str | None = None) -> C.CDLL:
         "mtmd_bench_kernel": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(C.c_double)]),
         "mtmd_append_xyz": (C.c_int, [vp, C.c_char_p, C.c_int32]),
         "mtmd_set_barostat": (C.c_int, [vp, C.c_double, C.c_double, _dp, _dp]),
-        "mtmd_find_topology": (C.c_int, [C.c_int32, C.c_int32, _dp, _ip, _dp, C.c_int32, C.POINTER(MtmdTopoRule),
+        "mtmd_find_topology": (C.c_int, [C.c_int32, C.c_int32, _dp, _ip, _dp, _ip, _ip, C.c_int32, C.POINTER(MtmdTopoRule),
                                          C.c_int32, _ip, _ip]),
         "mtmd_topology_last_error": (C.c_char_p, []),
         "mtmd_get_barostat": (C.c_int, [vp, _dp, _dp, _dp, _dp, _dp]),

@@ -211,20 +211,20 @@ class Engine:
         self._chk(self.lib.mtmd_append_dcd(self.h, path.encode(), int(n_steps), int(stride)))
 
 
-def find_topology(which: int, pos, types, box, rules) -> np.ndarray:
+def find_topology(which: int, pos, types, box, rules, graph=None) -> np.ndarray:
     """mtmd_find_topology: bonds (which 1), angles (2) or dihedrals (3) of the
-    all_r_less_than rules [(types tuple, R0)], rows as mtmd_system's lists
-    (atoms, then the rule index); box None for the plain distance."""
+    rules [(types tuple, R0[, crit])], rows as mtmd_system's lists (atoms, then
+    the rule index); box None for the plain distance; graph: one list of bond
+    destinations per atom (needed by the molecule, visible-bond and 1-4
+    criteria)."""
     lib = abi.load_library()
     pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
     types = np.ascontiguousarray(types, np.int32)
-    arr = (abi.MtmdTopoRule * max(1, len(rules)))()
-    for i, (t, r0) in enumerate(rules):
-        arr[i].types[:] = list(t) + [0] * (4 - len(t))
-        arr[i].R0 = r0
+    arr = abi.topo_rules(rules)
+    bs, bd = abi.bond_graph(graph, pos.shape[0])
     b = None if box is None else np.ascontiguousarray(box, np.float64)
     n = C.c_int32(0)
-    args = (which, pos.shape[0], _f64(pos), _i32(types), _f64(b), len(rules), arr)
+    args = (which, pos.shape[0], _f64(pos), _i32(types), _f64(b), _i32(bs), _i32(bd), len(rules), arr)
     rc = lib.mtmd_find_topology(*args, 0, None, C.byref(n))
     if rc:
         raise MtmdError(rc, lib.mtmd_topology_last_error().decode())

@@ -1510,11 +1510,13 @@ int orc_optimize(orc_md* o, int max_steps, double lr, double accuracy, int out_s
 }
 
 /* ------------------------------------------------------------------ */
-/* Topology search with the all_r_less_than criterion                   */
-/* (MolTwisterStateTools.cpp:544-724,767-920,993-1140): nested loops    */
-/* over the atoms of each type in index order, then the duplicate       */
-/* removal of MolecularSystemTools.cpp:25-200 (first occurrence kept,   */
-/* an entry equals its reverse). Brute force: small systems only.       */
+/* Topology search (MolTwisterStateTools.cpp:544-724,767-920,993-1190): */
+/* for each rule, the loops of its detection criterion over the atoms   */
+/* of the rule's first type in index order (getAtomsWithID,             */
+/* MolTwisterState.cpp:598-610), then the duplicate removal of          */
+/* MolecularSystemTools.cpp:25-235 (first occurrence kept, an entry     */
+/* equals its reverse; angles and dihedrals naming an atom twice are    */
+/* dropped). Brute force: small systems only.                           */
 /* ------------------------------------------------------------------ */
 static double topo_d2(const double* pos, int a, int b, const double* w)
 {
@@ -1537,64 +1539,160 @@ static int topo_same(const int* a, const int* b, int na)
     return fwd || rev;
 }
 
-int orc_find_topology(int which, int n, const double* pos, const int32_t* type, const double* box, int n_rules,
+/* CAtom::findAtomsInMolecule (MolTwisterAtom.cpp:578-605): the atom, then
+ * recursively each bond destination in the atom's bond order, each atom
+ * once (the visited map) */
+static void topo_mol_visit(int a, const int32_t* bs, const int32_t* bd, char* seen, int* order, int* cnt)
+{
+    int i;
+    if (seen[a]) return;
+    seen[a] = 1;
+    order[(*cnt)++] = a;
+    for (i = bs[a]; i < bs[a + 1]; i++) topo_mol_visit(bd[i], bs, bd, seen, order, cnt);
+}
+
+typedef struct { int* buf; int cnt, cap, width; } topo_out;
+
+static void topo_emit(topo_out* t, int a, int b, int c, int d, int r)
+{
+    int v[5] = {a, b, c, d, r};
+    if (t->cnt == t->cap) { t->cap *= 2; t->buf = (int*)realloc(t->buf, sizeof(int) * (size_t)t->cap * t->width); }
+    memcpy(t->buf + (size_t)t->cnt * t->width, v, sizeof(int) * t->width);
+    t->cnt++;
+}
+
+int orc_find_topology(int which, int n, const double* pos, const int32_t* type, const double* box,
+                      const int32_t* bond_start, const int32_t* bond_dest, int n_rules,
                       const mtmd_topo_rule* rules, int max_out, int32_t* out)
 {
     const int na = which + 1, width = which + 2;
-    int cap = 1024, cnt = 0, r, j, k, l, m, e, f;
-    int* buf = (int*)malloc(sizeof(int) * (size_t)cap * width);
+    topo_out t;
+    int r, j, k, l, m, e, f;
     int* lists[4];
     int nlist[4];
+    char* seen = (char*)calloc((size_t)n + 1, 1);
+    int* mol = (int*)malloc(sizeof(int) * ((size_t)n + 1));
+    t.cap = 1024; t.cnt = 0; t.width = width;
+    t.buf = (int*)malloc(sizeof(int) * (size_t)t.cap * width);
     for (r = 0; r < n_rules; r++) {
         const double R02 = rules[r].R0 * rules[r].R0;
+        const int* ty = rules[r].types;
+        const int crit = rules[r].crit;
         int q;
-        for (q = 0; q < na; q++) {          /* CMolTwisterState::getAtomsWithID (MolTwisterState.cpp:584-596) */
+        for (q = 0; q < na; q++) {          /* CMolTwisterState::getAtomsWithID */
             lists[q] = (int*)malloc(sizeof(int) * (size_t)(n + 1));
             nlist[q] = 0;
-            for (j = 0; j < n; j++) if (type[j] == rules[r].types[q]) lists[q][nlist[q]++] = j;
+            for (j = 0; j < n; j++) if (type[j] == ty[q]) lists[q][nlist[q]++] = j;
         }
-#define EMIT(...) do {                                                                   \
-            int v_[5] = {__VA_ARGS__};                                                   \
-            if (cnt == cap) { cap *= 2; buf = (int*)realloc(buf, sizeof(int) * (size_t)cap * width); } \
-            memcpy(buf + (size_t)cnt * width, v_, sizeof(int) * width);                  \
-            cnt++;                                                                       \
-        } while (0)
-        for (j = 0; j < nlist[0]; j++) {
-            int a = lists[0][j];
-            for (k = 0; k < nlist[1]; k++) {
-                int b = lists[1][k];
-                if (a == b) continue;
-                if (!(topo_d2(pos, a, b, box) < R02)) continue;
-                if (which == 1) { EMIT(a, b, r, 0, 0); continue; }
-                for (l = 0; l < nlist[2]; l++) {
-                    int c = lists[2][l];
-                    if (c == b || c == a) continue;
-                    if (!(topo_d2(pos, b, c, box) < R02)) continue;
-                    if (which == 2) { EMIT(a, b, c, r, 0); continue; }
-                    for (m = 0; m < nlist[3]; m++) {
-                        int d = lists[3][m];
-                        if (d == c || d == b || d == a) continue;
-                        if (topo_d2(pos, c, d, box) < R02) EMIT(a, b, c, d, r);
+        if (crit == MTMD_CRIT_ALL_R) {      /* :577-603, :805-839, :1033-1077 */
+            for (j = 0; j < nlist[0]; j++) {
+                int a = lists[0][j];
+                for (k = 0; k < nlist[1]; k++) {
+                    int b = lists[1][k];
+                    if (a == b) continue;
+                    if (!(topo_d2(pos, a, b, box) < R02)) continue;
+                    if (which == 1) { topo_emit(&t, a, b, r, 0, 0); continue; }
+                    for (l = 0; l < nlist[2]; l++) {
+                        int c = lists[2][l];
+                        if (c == b || c == a) continue;
+                        if (!(topo_d2(pos, b, c, box) < R02)) continue;
+                        if (which == 2) { topo_emit(&t, a, b, c, r, 0); continue; }
+                        for (m = 0; m < nlist[3]; m++) {
+                            int d = lists[3][m];
+                            if (d == c || d == b || d == a) continue;
+                            if (topo_d2(pos, c, d, box) < R02) topo_emit(&t, a, b, c, d, r);
+                        }
+                    }
+                }
+            }
+        } else if (crit == MTMD_CRIT_MOL_R) {   /* :605-636, :843-888, :1084-1144 */
+            for (j = 0; j < nlist[0]; j++) {
+                int a = lists[0][j], nm = 0, i;
+                topo_mol_visit(a, bond_start, bond_dest, seen, mol, &nm);
+                for (i = 0; i < nm; i++) seen[mol[i]] = 0;
+                for (k = 0; k < nm; k++) {
+                    int b = mol[k];
+                    if (a == b) continue;
+                    if (!(type[b] == ty[1] && topo_d2(pos, a, b, box) < R02)) continue;
+                    if (which == 1) { topo_emit(&t, a, b, r, 0, 0); continue; }
+                    for (l = 0; l < nm; l++) {
+                        int c = mol[l];
+                        if (c == a || c == b) continue;
+                        if (!(type[c] == ty[2] && topo_d2(pos, b, c, box) < R02)) continue;
+                        if (which == 2) { topo_emit(&t, a, b, c, r, 0); continue; }
+                        for (m = 0; m < nm; m++) {
+                            int d = mol[m];
+                            if (d == a || d == b || d == c) continue;
+                            if (type[d] == ty[3] && topo_d2(pos, c, d, box) < R02) topo_emit(&t, a, b, c, d, r);
+                        }
+                    }
+                }
+            }
+        } else if (crit == MTMD_CRIT_VISIBLE) {  /* :637-666, :889-920, :1145-1198 */
+            for (j = 0; j < nlist[0]; j++) {
+                int a = lists[0][j];
+                for (k = bond_start[a]; k < bond_start[a + 1]; k++) {
+                    int b = bond_dest[k];
+                    if (type[b] != ty[1]) continue;
+                    if (which == 1) { topo_emit(&t, a, b, r, 0, 0); continue; }
+                    for (l = bond_start[b]; l < bond_start[b + 1]; l++) {
+                        int c = bond_dest[l];
+                        if (type[c] != ty[2]) continue;
+                        if (which == 2) { topo_emit(&t, a, b, c, r, 0); continue; }
+                        for (m = bond_start[c]; m < bond_start[c + 1]; m++) {
+                            int d = bond_dest[m];
+                            if (type[d] == ty[3]) topo_emit(&t, a, b, c, d, r);
+                        }
+                    }
+                }
+            }
+        } else if (crit == MTMD_CRIT_14 && which == 1) {   /* :667-716 */
+            for (j = 0; j < nlist[0]; j++) {
+                int a = lists[0][j];
+                for (k = bond_start[a]; k < bond_start[a + 1]; k++) {
+                    int b = bond_dest[k];
+                    for (l = bond_start[b]; l < bond_start[b + 1]; l++) {
+                        int c = bond_dest[l];
+                        for (m = bond_start[c]; m < bond_start[c + 1]; m++) {
+                            int d = bond_dest[m];
+                            if (type[d] == ty[1] && d != c && d != b && d != a) topo_emit(&t, a, d, r, 0, 0);
+                        }
                     }
                 }
             }
         }
-#undef EMIT
+        /* any other criterion: the reference prints an error and returns what it has (none here) */
         for (q = 0; q < na; q++) free(lists[q]);
     }
-    /* removeDuplicate*Indices: keep the first of equal or reversed entries */
+    /* removeDuplicate*Indices: keep the first of equal or reversed entries,
+     * then drop angles / dihedrals naming an atom twice */
     {
         int kept = 0;
-        for (e = 0; e < cnt; e++) {
+        for (e = 0; e < t.cnt; e++) {
             int dup = 0;
-            for (f = 0; f < kept && !dup; f++) dup = topo_same(buf + (size_t)f * width, buf + (size_t)e * width, na);
+            for (f = 0; f < kept && !dup; f++) dup = topo_same(t.buf + (size_t)f * width, t.buf + (size_t)e * width, na);
             if (dup) continue;
-            if (kept != e) memmove(buf + (size_t)kept * width, buf + (size_t)e * width, sizeof(int) * width);
+            if (kept != e) memmove(t.buf + (size_t)kept * width, t.buf + (size_t)e * width, sizeof(int) * width);
             kept++;
         }
-        cnt = kept;
+        t.cnt = kept;
+        if (which >= 2) {
+            kept = 0;
+            for (e = 0; e < t.cnt; e++) {
+                const int* x = t.buf + (size_t)e * width;
+                int p2, p3, rep = 0;
+                for (p2 = 0; p2 < na; p2++)
+                    for (p3 = p2 + 1; p3 < na; p3++) rep |= (x[p2] == x[p3]);
+                if (rep) continue;
+                if (kept != e) memmove(t.buf + (size_t)kept * width, x, sizeof(int) * width);
+                kept++;
+            }
+            t.cnt = kept;
+        }
     }
-    if (out) memcpy(out, buf, sizeof(int) * (size_t)(cnt < max_out ? cnt : max_out) * width);
-    free(buf);
-    return cnt;
+    if (out) memcpy(out, t.buf, sizeof(int) * (size_t)(t.cnt < max_out ? t.cnt : max_out) * width);
+    free(t.buf);
+    free(seen);
+    free(mol);
+    return t.cnt;
 }

@@ -82,9 +82,11 @@ void orc_get_forces(const orc_md* o, double* F, float* U);
 /* gradient descent (MDLoop.cpp:143-173); returns steps done */
 int  orc_optimize(orc_md* o, int max_steps, double lr, double accuracy, int out_stride);
 
-/* topology search with the all_r_less_than criterion (brute force);
- * returns the number of entries, writes at most max_out of them */
-int orc_find_topology(int which, int n, const double* pos, const int32_t* type, const double* box, int n_rules,
+/* topology search by the rules' detection criteria (brute force);
+ * bond_start / bond_dest: the bond graph (NULL when every rule is
+ * MTMD_CRIT_ALL_R); returns the number of entries, writes at most max_out */
+int orc_find_topology(int which, int n, const double* pos, const int32_t* type, const double* box,
+                      const int32_t* bond_start, const int32_t* bond_dest, int n_rules,
                       const mtmd_topo_rule* rules, int max_out, int32_t* out);
 
 /* reference scalar helpers */

@@ -74,7 +74,7 @@ def _oracle_lib():
         L.orc_get_thermostat.argtypes = [_vp, _dp, _dp]
         L.orc_get_mol.argtypes = [_vp, _ip]
         L.orc_set_barostat.argtypes = [_vp, C.c_double, C.c_double, _dp, _dp]
-        L.orc_find_topology.argtypes = [C.c_int, C.c_int, _dp, _ip, _dp, C.c_int, C.POINTER(abi.MtmdTopoRule),
+        L.orc_find_topology.argtypes = [C.c_int, C.c_int, _dp, _ip, _dp, _ip, _ip, C.c_int, C.POINTER(abi.MtmdTopoRule),
                                         C.c_int, _ip]
         L.orc_get_barostat.argtypes = [_vp, _dp, _dp, _dp, _dp, _dp]
         L.orc_build_cells.argtypes = [_vp, _ip, _ip, _ip, _ip, _ip]
@@ -140,22 +140,21 @@ def ref_dcd_append(path: str, n_steps: int, stride: int, timestep: float, xyz, b
                               float(box[0]), float(box[1]), float(box[2]), _f64(xyz))
 
 
-def find_topology(which: int, pos, types, box, rules):
-    """The oracle's topology search: rules are (types tuple, R0); box None for
-    the plain distance. Returns an (n, which + 2) int array."""
+def find_topology(which: int, pos, types, box, rules, graph=None):
+    """The oracle's topology search: rules are (types tuple, R0[, crit]); box
+    None for the plain distance; graph one list of bond destinations per atom.
+    Returns an (n, which + 2) int array."""
     pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
     types = np.ascontiguousarray(types, np.int32)
-    arr = (abi.MtmdTopoRule * max(1, len(rules)))()
-    for i, (t, r0) in enumerate(rules):
-        arr[i].types[:] = list(t) + [0] * (4 - len(t))
-        arr[i].R0 = r0
+    arr = abi.topo_rules(rules)
+    bs, bd = abi.bond_graph(graph, pos.shape[0])
     b = None if box is None else np.ascontiguousarray(box, np.float64)
     lib = _oracle_lib()
-    n = lib.orc_find_topology(which, pos.shape[0], _f64(pos), _i32(types), _f64(b) if b is not None else None,
-                              len(rules), arr, 0, None)
+    args = (which, pos.shape[0], _f64(pos), _i32(types), _f64(b) if b is not None else None,
+            _i32(bs) if bs is not None else None, _i32(bd) if bd is not None else None, len(rules), arr)
+    n = lib.orc_find_topology(*args, 0, None)
     out = np.zeros((max(n, 1), which + 2), np.int32)
-    lib.orc_find_topology(which, pos.shape[0], _f64(pos), _i32(types), _f64(b) if b is not None else None,
-                          len(rules), arr, n, _i32(out))
+    lib.orc_find_topology(*args, n, _i32(out))
     return out[:n]
 
 

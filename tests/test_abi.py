@@ -25,7 +25,7 @@ def test_library_loads_and_exports_every_declared_symbol():
     for name in syms:
         assert hasattr(lib, name), name
     assert sorted(abi.EXPORTED) == syms
-    assert lib.mtmd_abi_version() == 3
+    assert lib.mtmd_abi_version() == 4
 
 
 def test_defaults_match_reference():

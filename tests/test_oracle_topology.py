@@ -68,3 +68,106 @@ def test_first_force_field_keeps_a_shared_pair(oracle_mod):
     types = np.array([0, 0], np.int32)
     b = oracle_mod.find_topology(1, pos, types, None, [((0, 0), 1.2), ((0, 0), 1.5)])
     assert np.array_equal(b, [[0, 1, 0]])
+
+
+# ---- the bond-graph criteria (CMDFFBond::EDetCrit 1-3) ----
+ALL_R, MOL_R, VISIBLE, ONLY14 = 0, 1, 2, 3
+
+
+def symmetric(n, edges):
+    """Bond graph with each bond attached at both atoms, in the order given
+    (what `genbonds` makes: CAtom::attachBondTo on both ends)."""
+    g = [[] for _ in range(n)]
+    for a, b in edges:
+        g[a].append(b)
+        g[b].append(a)
+    return g
+
+
+def test_visible_bonds_ignore_distance(oracle_mod):
+    # butane-like chain 0-1-2-3 (types 0,1,1,0) with the atoms 10 A apart:
+    # only_visible_bonds follows the graph, all_r_less_than finds nothing
+    pos = np.array([[0.0, 0, 0], [10.0, 0, 0], [20.0, 0, 0], [30.0, 0, 0]])
+    types = np.array([0, 1, 1, 0], np.int32)
+    g = symmetric(4, [(0, 1), (1, 2), (2, 3)])
+    b = oracle_mod.find_topology(1, pos, types, None, [((0, 1), 1.2, VISIBLE), ((1, 1), 1.2, VISIBLE)], g)
+    assert np.array_equal(b, [[0, 1, 0], [3, 2, 0], [1, 2, 1]])
+    assert oracle_mod.find_topology(1, pos, types, None, [((0, 1), 1.2, ALL_R)], g).shape[0] == 0
+    a = oracle_mod.find_topology(2, pos, types, None, [((0, 1, 1), 1.2, VISIBLE)], g)
+    assert np.array_equal(a, [[0, 1, 2, 0], [3, 2, 1, 0]])
+    d = oracle_mod.find_topology(3, pos, types, None, [((0, 1, 1, 0), 1.2, VISIBLE)], g)
+    assert np.array_equal(d, [[0, 1, 2, 3, 0]])
+
+
+def test_visible_bonds_follow_the_graph_direction(oracle_mod):
+    pos = np.zeros((2, 3))
+    types = np.array([0, 1], np.int32)
+    g = [[1], []]                      # attached at atom 0 only
+    assert np.array_equal(oracle_mod.find_topology(1, pos, types, None, [((0, 1), 1.0, VISIBLE)], g), [[0, 1, 0]])
+    assert oracle_mod.find_topology(1, pos, types, None, [((1, 0), 1.0, VISIBLE)], g).shape[0] == 0
+
+
+def test_visible_angle_naming_an_atom_twice_is_dropped(oracle_mod):
+    # water O(0)-H(1), O(0)-H(2): H1 -> O -> H1 is walked, and dropped
+    pos = np.array([[0.0, 0, 0], [0.96, 0, 0], [-0.24, 0.93, 0]])
+    types = np.array([0, 1, 1], np.int32)
+    g = symmetric(3, [(0, 1), (0, 2)])
+    a = oracle_mod.find_topology(2, pos, types, None, [((1, 0, 1), 1.2, VISIBLE)], g)
+    assert np.array_equal(a, [[1, 0, 2, 0]])
+
+
+def test_14_bonds(oracle_mod):
+    # The walk a -> b -> c -> d only requires d to differ from a, b and c, so
+    # it may step back through a (c == a): a's other bonded atoms are then
+    # listed too. Each answer below is derived walk by walk.
+    pos = np.zeros((4, 3))
+    t0 = np.zeros(4, np.int32)
+    chain = symmetric(4, [(0, 1), (1, 2), (2, 3)])
+    # from 0: 0-1-2-3; from 1: 1-0-1-2, 1-2-1-0; from 2: 2-1-2-3, 2-3-2-1 (= 1-2);
+    # from 3: 3-2-1-0 (= 0-3)
+    b = oracle_mod.find_topology(1, pos, t0, None, [((0, 0), 1.0, ONLY14)], chain)
+    assert np.array_equal(b, [[0, 3, 0], [1, 2, 0], [1, 0, 0], [2, 3, 0]])
+    # star 0-{1,2,3}: from 0, 0-1-0-2, 0-1-0-3, 0-2-0-1, ...; from a leaf every
+    # walk ends on the centre, which is b
+    star = symmetric(4, [(0, 1), (0, 2), (0, 3)])
+    b = oracle_mod.find_topology(1, pos, t0, None, [((0, 0), 1.0, ONLY14)], star)
+    assert np.array_equal(b, [[0, 2, 0], [0, 3, 0], [0, 1, 0]])
+    # the atom of the second type is matched at the far end only
+    t1 = np.array([0, 0, 0, 1], np.int32)
+    assert np.array_equal(oracle_mod.find_topology(1, pos, t1, None, [((0, 1), 1.0, ONLY14)], chain),
+                          [[0, 3, 0], [2, 3, 0]])
+    # a pair of atoms has none
+    assert oracle_mod.find_topology(1, pos[:2], t0[:2], None, [((0, 0), 1.0, ONLY14)],
+                                    symmetric(2, [(0, 1)])).shape[0] == 0
+
+
+def test_mol_criterion_stays_inside_the_molecule(oracle_mod):
+    # two waters; H(1) of the first lies 1.0 A from O(3) of the second
+    pos = np.array([[0.0, 0, 0], [0.96, 0, 0], [-0.24, 0.93, 0],
+                    [1.96, 0, 0], [2.2, 0.93, 0], [2.92, 0, 0]])
+    types = np.array([0, 1, 1, 0, 1, 1], np.int32)
+    g = symmetric(6, [(0, 1), (0, 2), (3, 4), (3, 5)])
+    allr = oracle_mod.find_topology(1, pos, types, None, [((0, 1), 1.2, ALL_R)], g)
+    assert [3, 1, 0] in allr.tolist()
+    mol = oracle_mod.find_topology(1, pos, types, None, [((0, 1), 1.2, MOL_R)], g)
+    assert np.array_equal(mol, [[0, 1, 0], [0, 2, 0], [3, 4, 0], [3, 5, 0]])
+    a = oracle_mod.find_topology(2, pos, types, None, [((1, 0, 1), 1.2, MOL_R)], g)
+    assert np.array_equal(a, [[1, 0, 2, 0], [4, 3, 5, 0]])
+
+
+def test_mol_criterion_walks_depth_first(oracle_mod):
+    # atom 0 bonds 1 then 4; 1-2-3 hang below 1. Depth first from 0 visits
+    # 0, 1, 2, 3, 4 (breadth first would give 0, 1, 4, 2, 3)
+    pos = np.array([[0.0, 0, 0], [0.5, 0, 0], [0.5, 0.5, 0], [0.5, 0.5, 0.5], [-0.5, 0, 0]])
+    types = np.array([0, 1, 1, 1, 1], np.int32)
+    g = symmetric(5, [(0, 1), (0, 4), (1, 2), (2, 3)])
+    b = oracle_mod.find_topology(1, pos, types, None, [((0, 1), 2.0, MOL_R)], g)
+    assert np.array_equal(b, [[0, 1, 0], [0, 2, 0], [0, 3, 0], [0, 4, 0]])
+    # bond order in the graph decides: 0 bonds 3 first, then 1 and 2
+    pos2 = np.array([[0.0, 0, 0], [1.0, 0, 0], [0, 1.0, 0], [0, 0, 1.0]])
+    g2 = [[3, 1, 2], [0], [0], [0]]
+    t2 = np.array([0, 1, 1, 1], np.int32)
+    b2 = oracle_mod.find_topology(1, pos2, t2, None, [((0, 1), 1.2, MOL_R)], g2)
+    assert np.array_equal(b2, [[0, 3, 0], [0, 1, 0], [0, 2, 0]])
+    assert np.array_equal(oracle_mod.find_topology(1, pos2, t2, None, [((0, 1), 1.2, ALL_R)], g2),
+                          [[0, 1, 0], [0, 2, 0], [0, 3, 0]])

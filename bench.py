@@ -123,13 +123,25 @@ def host_threads() -> int:
 
 
 def cpu_model() -> str:
+    """/proc/cpuinfo's model name with its family / model / stepping (a VM may
+    report a short name), the host's logical CPUs and this process's share."""
+    info = {}
     try:
         for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                return line.split(":", 1)[1].strip()
+            if ":" in line:
+                k, v = (x.strip() for x in line.split(":", 1))
+                if k in ("model name", "cpu family", "model", "stepping", "cpu MHz") and k not in info:
+                    info[k] = v
     except OSError:
         pass
-    return "unknown"
+    name = info.get("model name", "unknown")
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count() or 1
+    return (f"{name} (family {info.get('cpu family', '?')}, model {info.get('model', '?')}, "
+            f"stepping {info.get('stepping', '?')}, {info.get('cpu MHz', '?')} MHz; "
+            f"{os.cpu_count()} logical CPUs on the host, {usable} usable by this process)")
 
 
 def timed_steps(eng, k):

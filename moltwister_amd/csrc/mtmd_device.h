@@ -547,3 +547,57 @@ __device__ inline void mt_wave_sort_cell(int* __restrict__ a, int cnt, int* __re
         for (int i = 0; i < cnt; i++) on(a[i], i);
     }
 }
+
+// The same for distinct keys key(v) (64-bit, unsigned order) instead of the
+// entries themselves: up to 64 entries ranked in registers, up to CAP / 2 in
+// this wave's share of LDS (two ints a key), a larger cell by lane 0.
+template <int CAP, typename Key, typename Hook>
+__device__ inline void mt_wave_sort_cell_keyed(int* __restrict__ a, int cnt, int* __restrict__ lds, Key key, Hook on)
+{
+    const int lane = threadIdx.x & 63;
+    const bool in_lds = cnt > 64 && cnt <= CAP / 2;
+    unsigned long long* lk = reinterpret_cast<unsigned long long*>(lds);
+    if (in_lds)
+        for (int i = lane; i < cnt; i += 64) lk[i] = key(a[i]);
+    __syncthreads();
+    if (cnt >= 1 && cnt <= 64) {
+        const int v = lane < cnt ? a[lane] : INT_MAX;
+        const unsigned long long kv = lane < cnt ? key(v) : ~0ull;
+        const unsigned lo = (unsigned)kv, hi = (unsigned)(kv >> 32);
+        int rank = 0;
+        for (int j = 0; j < cnt; j++) {
+            const unsigned long long kj = ((unsigned long long)(unsigned)__shfl((int)hi, j) << 32) |
+                                          (unsigned)__shfl((int)lo, j);
+            rank += kj < kv ? 1 : 0;
+        }
+        if (lane < cnt) {
+            a[rank] = v;
+            on(v, rank);
+        }
+    } else if (in_lds) {
+        int vs[CAP / 128], rs[CAP / 128];
+        int m = 0;
+        for (int i = lane; i < cnt; i += 64, m++) {
+            const unsigned long long kv = lk[i];
+            int rank = 0;
+            for (int j = 0; j < cnt; j++) rank += lk[j] < kv ? 1 : 0;
+            vs[m] = a[i];
+            rs[m] = rank;
+        }
+        __builtin_amdgcn_wave_barrier();          // (one wavefront: every read of a[] above precedes the writes)
+        m = 0;
+        for (int i = lane; i < cnt; i += 64, m++) {
+            a[rs[m]] = vs[m];
+            on(vs[m], rs[m]);
+        }
+    } else if (cnt > CAP / 2 && lane == 0) {
+        for (int i = 1; i < cnt; i++) {
+            const int v = a[i];
+            const unsigned long long kv = key(v);
+            int j = i - 1;
+            while (j >= 0 && key(a[j]) > kv) { a[j + 1] = a[j]; j--; }
+            a[j + 1] = v;
+        }
+        for (int i = 0; i < cnt; i++) on(a[i], i);
+    }
+}

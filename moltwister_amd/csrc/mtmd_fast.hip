@@ -77,29 +77,38 @@ __device__ __forceinline__ void mt_key_runs(int key, int& head_lane, int& len)
     len = (above ? __ffsll(above) - 1 : 64) - head_lane;
 }
 
+// The rebuild kernels are launched every step and return at once unless a
+// rebuild is due; such a launch costs GPU time in proportion to its
+// workgroups, so the per-atom and per-cell ones take at most MT_REBUILD_GRID
+// workgroups and stride over the rest
+#define MT_REBUILD_GRID 1024
+static inline int nblk_rebuild(long long n, int b) { return (int)std::min<long long>(nblk(n, b), MT_REBUILD_GRID); }
+
 __global__ void __launch_bounds__(TPB) k_fast_cell_keys(const double4* __restrict__ pos, int n, int Nx, int Ny,
                                                         int Nz, double LX, double LY, double LZ,
                                                         int* __restrict__ key, int* __restrict__ count, int cond,
                                                         unsigned int* flags, int dslot, float thr)
 {
     if (mt_skip_rebuild(cond, flags, dslot, thr)) return;
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    int c = -1;
-    if (k < n) {
-        const double4 r = pos[k];
-        if (!(isfinite(r.x) && isfinite(r.y) && isfinite(r.z))) atomicOr(&flags[3], 1u);
-        int ix = mt_d2i(floor((r.x + LX / 2.0) * (double)Nx / LX));
-        int iy = mt_d2i(floor((r.y + LY / 2.0) * (double)Ny / LY));
-        int iz = mt_d2i(floor((r.z + LZ / 2.0) * (double)Nz / LZ));
-        ix = ix < 0 ? 0 : (ix >= Nx ? Nx - 1 : ix);
-        iy = iy < 0 ? 0 : (iy >= Ny ? Ny - 1 : iy);
-        iz = iz < 0 ? 0 : (iz >= Nz ? Nz - 1 : iz);
-        c = ix + Nx * (iy + Ny * iz);
-        key[k] = c;
+    for (int k0 = blockIdx.x * blockDim.x; k0 < n; k0 += gridDim.x * blockDim.x) {   // uniform over the block
+        const int k = k0 + (int)threadIdx.x;
+        int c = -1;
+        if (k < n) {
+            const double4 r = pos[k];
+            if (!(isfinite(r.x) && isfinite(r.y) && isfinite(r.z))) atomicOr(&flags[3], 1u);
+            int ix = mt_d2i(floor((r.x + LX / 2.0) * (double)Nx / LX));
+            int iy = mt_d2i(floor((r.y + LY / 2.0) * (double)Ny / LY));
+            int iz = mt_d2i(floor((r.z + LZ / 2.0) * (double)Nz / LZ));
+            ix = ix < 0 ? 0 : (ix >= Nx ? Nx - 1 : ix);
+            iy = iy < 0 ? 0 : (iy >= Ny ? Ny - 1 : iy);
+            iz = iz < 0 ? 0 : (iz >= Nz ? Nz - 1 : iz);
+            c = ix + Nx * (iy + Ny * iz);
+            key[k] = c;
+        }
+        int hl, len;
+        mt_key_runs(c, hl, len);
+        if (c >= 0 && (int)(threadIdx.x & 63) == hl) atomicAdd(&count[c], len);
     }
-    int hl, len;
-    mt_key_runs(c, hl, len);
-    if (c >= 0 && (int)(threadIdx.x & 63) == hl) atomicAdd(&count[c], len);
 }
 
 // exclusive scan of the cell counts in blocks of SCAN_B cells: block sums,
@@ -168,15 +177,17 @@ __global__ void __launch_bounds__(TPB) k_fast_scatter(const int* __restrict__ ke
                                                       int dslot, float thr)
 {
     if (mt_skip_rebuild(cond, flags, dslot, thr)) return;
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63;
-    const int c = k < n ? key[k] : -1;
-    int hl, len;
-    mt_key_runs(c, hl, len);
-    int base = 0;
-    if (c >= 0 && lane == hl) base = atomicAdd(&fill[c], len);
-    base = __shfl(base, hl);
-    if (c >= 0) list[first[c] + base + (lane - hl)] = k;
+    for (int k0 = blockIdx.x * blockDim.x; k0 < n; k0 += gridDim.x * blockDim.x) {   // uniform over the block
+        const int k = k0 + (int)threadIdx.x;
+        const int c = k < n ? key[k] : -1;
+        int hl, len;
+        mt_key_runs(c, hl, len);
+        int base = 0;
+        if (c >= 0 && lane == hl) base = atomicAdd(&fill[c], len);
+        base = __shfl(base, hl);
+        if (c >= 0) list[first[c] + base + (lane - hl)] = k;
+    }
 }
 
 // the per-atom set-up of a rebuild for atom s, the rank-th atom of cell c in
@@ -184,13 +195,11 @@ __global__ void __launch_bounds__(TPB) k_fast_scatter(const int* __restrict__ ke
 // displacement test, its place in cell order, and the MIXED cell-local float
 // position (.w: the bits of type * MT_NPTS, the type's offset in a row of
 // segment coefficients) and charge at that place
-__device__ __forceinline__ void mt_fast_home_atom(const MtDev& d, const MtParams& p, int s, int c, int u)
+// the cell-local position of atom s in cell c (relative to the cell's centre,
+// wrapped), rounded to FP32 as the MIXED engine stages it
+__device__ __forceinline__ float3 mt_fast_local(const MtDev& d, const MtParams& p, int s, int c)
 {
     const double4 r = d.pos[s];
-    d.home[s] = c;
-    d.posref[s] = r;
-    d.sidx[s] = u;
-    d.qfs[u] = d.qf[s];
     const int ix = c % d.nc[0], iy = (c / d.nc[0]) % d.nc[1], iz = c / (d.nc[0] * d.nc[1]);
     const double cx = -p.L[0] / 2.0 + ((double)ix + 0.5) * (p.L[0] / d.nc[0]);
     const double cy = -p.L[1] / 2.0 + ((double)iy + 0.5) * (p.L[1] / d.nc[1]);
@@ -199,13 +208,27 @@ __device__ __forceinline__ void mt_fast_home_atom(const MtDev& d, const MtParams
     dx -= p.L[0] * rint(dx / p.L[0]);
     dy -= p.L[1] * rint(dy / p.L[1]);
     dz -= p.L[2] * rint(dz / p.L[2]);
-    d.posf[u] = make_float4((float)dx, (float)dy, (float)dz, __int_as_float(d.type[s] * MT_NPTS));
+    return make_float3((float)dx, (float)dy, (float)dz);
+}
+
+__device__ __forceinline__ void mt_fast_home_atom(const MtDev& d, const MtParams& p, int s, int c, int u)
+{
+    d.home[s] = c;
+    d.posref[s] = d.pos[s];
+    d.sidx[s] = u;
+    d.qfs[u] = d.qf[s];
+    const float3 l = mt_fast_local(d, p, s, c);
+    d.posf[u] = make_float4(l.x, l.y, l.z, __int_as_float(d.type[s] * MT_NPTS));
 }
 
 // ascending slot order inside each cell, so lists are reproducible run to run
 // (one wavefront per cell, mt_wave_sort_cell); with `home` set the same
 // wavefront then does the per-atom set-up of every atom of the cell, which
-// holds its sorted position already
+// holds its sorted position already. In the MIXED tile mode (d.tile16) the
+// atoms of a cell are ordered by their cell-local FP32 x instead (slot as the
+// tie-break), so the three cells of an x row of staged cells form a run whose
+// staged x never decreases inside a cell; the 16-bit list build uses that to
+// skip the ends of the row that no pair of its screen can reach.
 // cells of up to MT_FAST_SORT_LDS atoms are ranked in LDS (8 KB a workgroup, so
 // many workgroups share a CU); a larger cell is sorted by one lane
 #define MT_FAST_SORT_LDS 512
@@ -214,15 +237,44 @@ __global__ void __launch_bounds__(TPB) k_fast_cell_sort(MtDev d, MtParams p, int
     __shared__ __align__(16) int lds[(TPB / 64) * MT_FAST_SORT_LDS];
     if (mt_skip_rebuild(cond, d.flags, p.dslot, p.half_skin2) || d.flags[3]) return;   // uniform over the grid
     if (home && blockIdx.x == 0 && threadIdx.x == 0) d.flags[4] += 1;        // rebuild count
-    const int c = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int cnt = c < d.ncells ? d.cell_count[c] : 0;
-    const int f = c < d.ncells ? d.cell_first[c] : 0;
     int* lw = lds + (threadIdx.x >> 6) * MT_FAST_SORT_LDS;
-    if (home)
-        mt_wave_sort_cell<MT_FAST_SORT_LDS>(d.cell_list + f, cnt, lw,
-                                            [&](int v, int rank) { mt_fast_home_atom(d, p, v, c, f + rank); });
-    else
-        mt_wave_sort_cell<MT_FAST_SORT_LDS>(d.cell_list + f, cnt, lw);
+    const int wpb = (int)blockDim.x >> 6;             // wavefronts (cells) a block
+    for (int c0 = blockIdx.x * wpb; c0 < d.ncells; c0 += gridDim.x * wpb) {   // uniform over the block
+        const int c = c0 + ((int)threadIdx.x >> 6);
+        const int cnt = c < d.ncells ? d.cell_count[c] : 0;
+        const int f = c < d.ncells ? d.cell_first[c] : 0;
+        // the cell-local position of the wavefront's lane atom, from the key
+        // to the set-up (a cell of up to 64 atoms calls both once a lane)
+        float3 lc = make_float3(0.f, 0.f, 0.f);
+        int lc_v = -1;
+        auto set_up = [&](int v, int rank) {
+            if (v == lc_v) {
+                d.home[v] = c;
+                d.posref[v] = d.pos[v];
+                d.sidx[v] = f + rank;
+                d.qfs[f + rank] = d.qf[v];
+                d.posf[f + rank] = make_float4(lc.x, lc.y, lc.z, __int_as_float(d.type[v] * MT_NPTS));
+            } else {
+                mt_fast_home_atom(d, p, v, c, f + rank);
+            }
+        };
+        if (home && d.tile16) {
+            auto xkey = [&](int v) -> unsigned long long {
+                const float3 l = mt_fast_local(d, p, v, c);
+                lc = l;
+                lc_v = v;
+                uint32_t u = __float_as_uint(l.x);
+                u = (u & 0x80000000u) ? ~u : (u | 0x80000000u);      // float order as unsigned order
+                return ((unsigned long long)u << 32) | (uint32_t)v;
+            };
+            mt_wave_sort_cell_keyed<MT_FAST_SORT_LDS>(d.cell_list + f, cnt, lw, xkey, set_up);
+        } else if (home) {
+            mt_wave_sort_cell<MT_FAST_SORT_LDS>(d.cell_list + f, cnt, lw, set_up);
+        } else {
+            mt_wave_sort_cell<MT_FAST_SORT_LDS>(d.cell_list + f, cnt, lw);
+        }
+        __syncthreads();                              // lw is reused by the next cell
+    }
 }
 
 // The exclusive scan of the counts of a rebuild in one workgroup (grids of at
@@ -285,12 +337,14 @@ void mt_fast_cells(const MtDev& d, const MtParams& p, int cond, int with_home, h
 {
     const float thr = p.half_skin2;
     if (d.ncells <= MT_SCAN_ONE_BLOCK) {
-        hipLaunchKernelGGL(k_fast_cell_keys, dim3(nblk(d.n, TPB)), dim3(TPB), 0, s, d.pos, d.n, d.nc[0], d.nc[1],
-                           d.nc[2], p.L[0], p.L[1], p.L[2], d.cell_key, d.cell_next, cond, d.flags, p.dslot, thr);
+        hipLaunchKernelGGL(k_fast_cell_keys, dim3(nblk_rebuild(d.n, TPB)), dim3(TPB), 0, s, d.pos, d.n, d.nc[0],
+                           d.nc[1], d.nc[2], p.L[0], p.L[1], p.L[2], d.cell_key, d.cell_next, cond, d.flags, p.dslot,
+                           thr);
         hipLaunchKernelGGL(k_fast_scan_one, dim3(1), dim3(SCAN_B), 0, s, d, cond, p.dslot, thr);
-        hipLaunchKernelGGL(k_fast_scatter, dim3(nblk(d.n, TPB)), dim3(TPB), 0, s, d.cell_key, d.n, d.cell_first,
-                           d.cell_fill, d.cell_list, cond, d.flags, p.dslot, thr);
-        hipLaunchKernelGGL(k_fast_cell_sort, dim3(nblk(d.ncells, TPB / 64)), dim3(TPB), 0, s, d, p, cond, with_home);
+        hipLaunchKernelGGL(k_fast_scatter, dim3(nblk_rebuild(d.n, TPB)), dim3(TPB), 0, s, d.cell_key, d.n,
+                           d.cell_first, d.cell_fill, d.cell_list, cond, d.flags, p.dslot, thr);
+        hipLaunchKernelGGL(k_fast_cell_sort, dim3(nblk_rebuild(d.ncells, TPB / 64)), dim3(TPB), 0, s, d, p, cond,
+                           with_home);
         return;
     }
     const int nsb = nblk(d.ncells, SCAN_B);
@@ -658,6 +712,25 @@ __device__ __forceinline__ void mt_neigh_run16(const MtDev& d, uint4* nl4, const
     }
 }
 
+// first place e in [lo, hi) of a staged cell whose FP16 x difference to the
+// home atom, dx = px - x[e] rounded as the screen rounds it, satisfies
+// dx < B (LEFT: the run's ox = -1 cell) or dx <= -B (right: its ox = +1 cell);
+// hi when there is none. The cell's staged x never decreases (tile-mode cells
+// are ordered by x), so dx never increases and each predicate is false and
+// then true along the cell.
+template <bool LEFT>
+__device__ __forceinline__ int mt_run_bound16(const uint2* sxz, int lo, int hi, mt_h2 pxy, _Float16 B)
+{
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        const _Float16 dx = (pxy - __builtin_bit_cast(mt_h2, sxz[mid].x)).x;
+        const bool t = LEFT ? (dx < B) : (dx <= -B);
+        lo = t ? lo : mid + 1;
+        hi = t ? mid : hi;
+    }
+    return lo;
+}
+
 template <int MODE>
 __device__ __forceinline__ void mt_neigh_block_atoms16(const MtDev& d, const MtParams& p, float4* spos, int* smol,
                                                        int* sword, const int* s_first, const int* s_cnt,
@@ -678,6 +751,12 @@ __device__ __forceinline__ void mt_neigh_block_atoms16(const MtDev& d, const MtP
     const float csm = fmaxf(p.cs[0], fmaxf(p.cs[1], p.cs[2]));
     const float delta = 2.0f * mt_half_ulp16(2.6f * csm) + mt_half_ulp16(p.rlist);
     const float thr = p.rlist2 + 3.0f * p.rlist * delta + 2.0f * delta * delta;
+    // B: the least FP16 value with B^2 >= thr. A candidate whose FP16 x
+    // difference has |dx| >= B fails the screen whatever dy and dz are (dx^2 is
+    // exact in FP32, and the screen's sum, rounded, is at least dx^2 >= thr),
+    // so the ends of a run beyond +-B can be skipped without changing a word
+    _Float16 B = (_Float16)sqrtf(thr);
+    while ((float)B * (float)B < thr) B = __builtin_bit_cast(_Float16, (uint16_t)(__builtin_bit_cast(uint16_t, B) + 1));
     {
         // four threads a staged cell, each taking every fourth atom
         const int q = t >> 2;
@@ -723,13 +802,17 @@ __device__ __forceinline__ void mt_neigh_block_atoms16(const MtDev& d, const MtP
         for (int oz = 0; oz < 3; oz++)
             for (int oy = 0; oy < 3; oy++) {
                 const int q = hxo + 4 * ((oy + hyo) + 4 * (oz + hzo));     // the row's ox = -1 cell
+                // the run's candidates within +-B in x: the head of the left
+                // cell and the tail of the right cell are out of reach
+                const int e0 = mt_run_bound16<true>(sxz, s_off[q], s_off[q + 1], pxy, B);
+                const int e1 = mt_run_bound16<false>(sxz, s_off[q + 2], s_off[q + 3], pxy, B);
                 // only the middle run holds the atom itself
                 if (oy == 1 && oz == 1)
-                    mt_neigh_run16<MODE, true>(d, nl4, sxz, sword, smol, s_off[q], s_off[q + 3], pxy, pz, thr, lself,
-                                               k, molk, xk, n, cap, cnt, r0, r1, r2, r3);
+                    mt_neigh_run16<MODE, true>(d, nl4, sxz, sword, smol, e0, e1, pxy, pz, thr, lself, k, molk, xk, n,
+                                               cap, cnt, r0, r1, r2, r3);
                 else
-                    mt_neigh_run16<MODE, false>(d, nl4, sxz, sword, smol, s_off[q], s_off[q + 3], pxy, pz, thr, lself,
-                                                k, molk, xk, n, cap, cnt, r0, r1, r2, r3);
+                    mt_neigh_run16<MODE, false>(d, nl4, sxz, sword, smol, e0, e1, pxy, pz, thr, lself, k, molk, xk,
+                                                n, cap, cnt, r0, r1, r2, r3);
             }
         // last partial chunk, padded with the atom's own staged place
         const int slot = cnt & 7;
@@ -1440,11 +1523,11 @@ __device__ __forceinline__ float mt_fast_nve_atom(const MtDev& d, const MtParams
 __device__ __forceinline__ void mt_fast_flag_disp(const MtDev& d, const MtParams& p, float d2);
 
 // FUSE (NVE steps of a call but the last, no forces added after this pass):
-// each home atom's force is followed at once by the back half of this step
-// and the front half of the next (k_fast_nve_fused's arithmetic, the same
-// bits), the new positions written to pos2 and posf2, since the staged
-// positions of this pass must not change under other workgroups; the host
-// then swaps the buffers
+// each home atom's force is followed at once, in the lane that reduced it, by
+// the back half of this step and the front half of the next
+// (k_fast_nve_fused's arithmetic, the same bits), the new positions written
+// to pos2 and posf2, since the staged positions of this pass must not change
+// under other workgroups; the host then swaps the buffers
 template <bool WANT_E, bool COUL, int G, bool FUSE = false>
 __global__ void __launch_bounds__(256 * G) __attribute__((amdgpu_waves_per_eu(4)))
 k_fast_force_tile(MtDev d, MtParams p)
@@ -1495,6 +1578,7 @@ k_fast_force_tile(MtDev d, MtParams p)
     __syncthreads();
     const int nh = s_hoff[8];
     const int gl = t % G;
+    float d2max = 0.0f;                               // FUSE: largest squared displacement of this lane's atoms
     for (int v = t / G; v < nh; v += (int)blockDim.x / G) {
         const int h = mt_tile_home(s_hoff, v);
         const int u = s_hfirst[h] + (v - s_hoff[h]);
@@ -1529,27 +1613,22 @@ k_fast_force_tile(MtDev d, MtParams p)
         mt_lane_sum<G>(Fx, Fy, Fz, U);
         if (gl == 0) {
             if (!d.mobile[k]) { Fx = Fy = Fz = 0.0; U = 0.0f; }
-            d.frc[k] = make_double4(Fx, Fy, Fz, 0.0);
-            if (WANT_E) d.epot[k] = U;
+            if constexpr (FUSE) {
+                // the lane that holds the force integrates the atom at once;
+                // the force itself is not stored (the next pass computes the
+                // next one, and a call's last step is not fused)
+                double4 F = make_double4(Fx, Fy, Fz, 0.0), pm = d.mom[k], r = d.pos[k];
+                bool capped;
+                d2max = fmaxf(d2max, mt_fast_nve_atom(d, p, k, F, pm, r, capped, 1, d.posf2, true));
+                d.mom[k] = pm;
+                d.pos2[k] = r;
+            } else {
+                d.frc[k] = make_double4(Fx, Fy, Fz, 0.0);
+                if (WANT_E) d.epot[k] = U;
+            }
         }
     }
-    if constexpr (FUSE) {
-        // every lane integrates the block's home atoms, their forces read back
-        // from this workgroup's own stores
-        __syncthreads();
-        float d2max = 0.0f;
-        for (int v = t; v < nh; v += (int)blockDim.x) {
-            const int h = mt_tile_home(s_hoff, v);
-            const int k = d.cell_list[s_hfirst[h] + (v - s_hoff[h])];
-            double4 F = d.frc[k], pm = d.mom[k], r = d.pos[k];
-            bool capped;
-            d2max = fmaxf(d2max, mt_fast_nve_atom(d, p, k, F, pm, r, capped, 1, d.posf2, true));
-            if (capped) d.frc[k] = F;
-            d.mom[k] = pm;
-            d.pos2[k] = r;
-        }
-        mt_fast_flag_disp(d, p, d2max);
-    }
+    if constexpr (FUSE) mt_fast_flag_disp(d, p, d2max);
 }
 
 // FP64: the reference's per-pair arithmetic (FunctorCalcForce.cpp:467-535)

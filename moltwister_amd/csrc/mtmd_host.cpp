@@ -103,8 +103,9 @@ struct mtmd_handle {
     std::vector<int> perm_host;   // valid after download_perm()
     int* h_status = nullptr;      // pinned status words
 
-    // captured plain steps (no re-sort, not the last of a call), four an
-    // ensemble. The kernels take the buffers and the rebuild-flag slot by
+    // captured plain steps (no re-sort, not the last of a call), one step or a
+    // block of kBlockSteps a graph (a replay costs a few microseconds of GPU
+    // time of its own, which a block shares). The kernels take the buffers and the rebuild-flag slot by
     // value, and a step may swap them (the re-sort swaps the slot buffers, a
     // fused NVE pass pos/pos2 and posf/posf2, a front half flips dslot; the
     // three together reach four states): a graph is found by the state it
@@ -119,8 +120,16 @@ struct mtmd_handle {
                    dslot == o.dslot;
         }
     };
-    struct StepGraph { bool valid = false; StepState from, to; hipGraph_t g = nullptr; hipGraphExec_t x = nullptr; };
-    static constexpr int kGraphsPerEnsemble = 4;
+    struct StepGraph {
+        bool valid = false;
+        int nstep = 1;                // plain steps captured in the graph
+        StepState from, to;
+        hipGraph_t g = nullptr;
+        hipGraphExec_t x = nullptr;
+    };
+    // four states, each with a graph of one step and one of kBlockSteps steps
+    static constexpr int kGraphsPerEnsemble = 8;
+    static constexpr int kBlockSteps = 8;
     StepGraph graphs[2 * kGraphsPerEnsemble];
     int graph_next[2] = {0, 0};
     StepState step_state() const { return {d.pos, d.pos2, d.mom, d.posf, d.posf2, p.dslot}; }
@@ -1578,13 +1587,14 @@ static int md_steps_once(mtmd_handle* h, int nsteps, bool energy_last)
             else mt_fast_nve_fused(d, h->p, mixed, h->stream);
             return MTMD_OK;
         };
-        auto replay = [&]() -> int {
+        auto replay = [&](int nstep) -> int {
             constexpr int G = mtmd_handle::kGraphsPerEnsemble;
             const int base = nvt ? G : 0;
             const mtmd_handle::StepState from = h->step_state();
             mtmd_handle::StepGraph* sg = nullptr;
             for (int i = 0; i < G; i++)
-                if (h->graphs[base + i].valid && h->graphs[base + i].from == from) sg = &h->graphs[base + i];
+                if (h->graphs[base + i].valid && h->graphs[base + i].nstep == nstep && h->graphs[base + i].from == from)
+                    sg = &h->graphs[base + i];
             if (!sg) {
                 // capture into the slot used least recently
                 int& nx = h->graph_next[nvt ? 1 : 0];
@@ -1594,7 +1604,8 @@ static int md_steps_once(mtmd_handle* h, int nsteps, bool energy_last)
                 if (sg->g) (void)hipGraphDestroy(sg->g);
                 *sg = mtmd_handle::StepGraph();
                 HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-                int r = plain_step();
+                int r = MTMD_OK;
+                for (int i = 0; i < nstep && r == MTMD_OK; i++) r = plain_step();
                 hipGraph_t g = nullptr;
                 const hipError_t e = hipStreamEndCapture(h->stream, &g);
                 // the capture ran no kernel: the step is launched below
@@ -1606,6 +1617,7 @@ static int md_steps_once(mtmd_handle* h, int nsteps, bool energy_last)
                 HIPCHK(h, hipGraphInstantiate(&sg->x, g, nullptr, nullptr, 0));
                 sg->from = from;
                 sg->to = to;
+                sg->nstep = nstep;
                 sg->valid = true;
             }
             HIPCHK(h, hipGraphLaunch(sg->x, h->stream));
@@ -1615,11 +1627,21 @@ static int md_steps_once(mtmd_handle* h, int nsteps, bool energy_last)
         // this call's first front half finds both rebuild-flag slots clear
         mt_fast_clear_flags(d, h->stream);
         if (!nvt) mt_fast_kick_drift(d, h->p, -1, mixed, h->stream);
+        constexpr int KB = mtmd_handle::kBlockSteps;
         for (int t = 0; t < nsteps; t++) {
             const bool last = (t == nsteps - 1);
+            // a block of KB plain steps, none of them a re-sort or the last
+            if (h->use_graphs && t + KB < nsteps && h->steps_since_sort + KB < h->sort_every && (!nvt || h->ke_valid)) {
+                if ((rc = replay(KB))) return rc;
+                h->steps_since_sort += KB;
+                h->ke_valid = nvt;
+                h->steps += KB;
+                t += KB - 1;
+                continue;
+            }
             const bool sort_now = (++h->steps_since_sort >= h->sort_every);
             if (!sort_now && !last && h->use_graphs && (!nvt || h->ke_valid)) {
-                if ((rc = replay())) return rc;
+                if ((rc = replay(1))) return rc;
                 h->ke_valid = nvt;
                 h->steps++;
                 continue;

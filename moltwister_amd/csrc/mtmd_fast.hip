@@ -201,13 +201,13 @@ __device__ __forceinline__ float3 mt_fast_local(const MtDev& d, const MtParams& 
 {
     const double4 r = d.pos[s];
     const int ix = c % d.nc[0], iy = (c / d.nc[0]) % d.nc[1], iz = c / (d.nc[0] * d.nc[1]);
-    const double cx = -p.L[0] / 2.0 + ((double)ix + 0.5) * (p.L[0] / d.nc[0]);
-    const double cy = -p.L[1] / 2.0 + ((double)iy + 0.5) * (p.L[1] / d.nc[1]);
-    const double cz = -p.L[2] / 2.0 + ((double)iz + 0.5) * (p.L[2] / d.nc[2]);
+    const double cx = -p.L[0] / 2.0 + ((double)ix + 0.5) * p.cw[0];
+    const double cy = -p.L[1] / 2.0 + ((double)iy + 0.5) * p.cw[1];
+    const double cz = -p.L[2] / 2.0 + ((double)iz + 0.5) * p.cw[2];
     double dx = r.x - cx, dy = r.y - cy, dz = r.z - cz;
-    dx -= p.L[0] * rint(dx / p.L[0]);
-    dy -= p.L[1] * rint(dy / p.L[1]);
-    dz -= p.L[2] * rint(dz / p.L[2]);
+    dx -= p.L[0] * rint(dx * p.invL[0]);
+    dy -= p.L[1] * rint(dy * p.invL[1]);
+    dz -= p.L[2] * rint(dz * p.invL[2]);
     return make_float3((float)dx, (float)dy, (float)dz);
 }
 
@@ -712,6 +712,75 @@ __device__ __forceinline__ void mt_neigh_run16(const MtDev& d, uint4* nl4, const
     }
 }
 
+// The words of a 16-bit list row pass through a ring of 16 words (two
+// chunks) per lane in LDS: a hit is one 16-bit store and a count, and a chunk
+// that has filled is copied to the list once per group of four candidates (at
+// most four words arrive in a group, so at most one chunk completes and the
+// ring never holds more than 8 + 3 + 4 words). Lanes' rings are 9 dwords
+// apart, so the 16-bit stores of a wavefront fall in distinct banks.
+#define MT_RING_DW 9
+struct MtRing16 {
+    uint32_t* ring;    // this lane's ring
+    uint4* out;        // chunk 0 of the atom's row (column k of the chunked layout)
+    size_t stride;     // chunks between a row's chunks (n)
+    int capc;          // chunks a row holds
+    int cnt;           // words written
+    int fl;            // chunks copied out
+};
+
+__device__ __forceinline__ void mt_ring_put(MtRing16& g, int w)
+{
+    reinterpret_cast<uint16_t*>(g.ring)[g.cnt & 15] = (uint16_t)w;
+    g.cnt++;
+}
+
+__device__ __forceinline__ void mt_ring_flush(MtRing16& g)
+{
+    if ((g.cnt >> 3) > g.fl) {
+        const uint32_t* c = g.ring + (g.fl & 1) * 4;
+        const uint4 v = make_uint4(c[0], c[1], c[2], c[3]);
+        if (g.fl < g.capc) g.out[(size_t)g.fl * g.stride] = v;
+        g.fl++;
+    }
+}
+
+// the hits of staged places [e, e1) of a run, in order, into the ring (the
+// screen of mt_neigh_run16)
+template <int MODE, bool SELF>
+__device__ __forceinline__ void mt_neigh_run16r(const MtDev& d, const uint2* sxz, const int* sword, const int* smol,
+                                                int e, int e1, mt_h2 pxy, float pz, float thr, int lself, int k,
+                                                int molk, bool xk, MtRing16& g)
+{
+    for (; e + 3 < e1; e += 4) {
+        uint2 c[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) c[i] = sxz[e + i];
+        bool hv[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const mt_h2 dxy = pxy - __builtin_bit_cast(mt_h2, c[i].x);
+            const float dz = pz - __uint_as_float(c[i].y);
+            hv[i] = __builtin_amdgcn_fdot2(dxy, dxy, dz * dz, false) < thr && (!SELF || e + i != lself);
+            if constexpr (MODE >= 1) hv[i] = hv[i] && smol[e + i] != molk;
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+            if (hv[i] && (MODE != 2 || !xk || !mt_is_partner(d, k, mt_id_slot(d, sword[e + i])))) mt_ring_put(g, e + i);
+        mt_ring_flush(g);
+    }
+    for (; e < e1; e++) {
+        const uint2 c0 = sxz[e];
+        const mt_h2 dxy = pxy - __builtin_bit_cast(mt_h2, c0.x);
+        const float dz = pz - __uint_as_float(c0.y);
+        if (__builtin_amdgcn_fdot2(dxy, dxy, dz * dz, false) < thr && (!SELF || e != lself) &&
+            (MODE < 1 || smol[e] != molk) &&
+            (MODE != 2 || !xk || !mt_is_partner(d, k, mt_id_slot(d, sword[e])))) {
+            mt_ring_put(g, e);
+            mt_ring_flush(g);
+        }
+    }
+}
+
 // first place e in [lo, hi) of a staged cell whose FP16 x difference to the
 // home atom, dx = px - x[e] rounded as the screen rounds it, satisfies
 // dx < B (LEFT: the run's ox = -1 cell) or dx <= -B (right: its ox = +1 cell);
@@ -734,7 +803,8 @@ __device__ __forceinline__ int mt_run_bound16(const uint2* sxz, int lo, int hi, 
 template <int MODE>
 __device__ __forceinline__ void mt_neigh_block_atoms16(const MtDev& d, const MtParams& p, float4* spos, int* smol,
                                                        int* sword, const int* s_first, const int* s_cnt,
-                                                       const int* s_off, const int* s_hfirst, const int* s_hcnt)
+                                                       const int* s_off, const int* s_hfirst, const int* s_hcnt,
+                                                       uint32_t* sring)
 {
     const int t = threadIdx.x;
     const int n = d.n;
@@ -797,8 +867,7 @@ __device__ __forceinline__ void mt_neigh_block_atoms16(const MtDev& d, const MtP
         const float pz = __uint_as_float(hv.y);
         const int molk = MODE >= 1 ? d.mol[k] : -1;
         const bool xk = MODE == 2 && d.xmol[d.perm[k]] != 0;
-        int cnt = 0;
-        uint32_t r0 = 0, r1 = 0, r2 = 0, r3 = 0;
+        MtRing16 g{sring + (size_t)t * MT_RING_DW, nl4 + k, (size_t)n, cap >> 3, 0, 0};
         for (int oz = 0; oz < 3; oz++)
             for (int oy = 0; oy < 3; oy++) {
                 const int q = hxo + 4 * ((oy + hyo) + 4 * (oz + hzo));     // the row's ox = -1 cell
@@ -808,17 +877,16 @@ __device__ __forceinline__ void mt_neigh_block_atoms16(const MtDev& d, const MtP
                 const int e1 = mt_run_bound16<false>(sxz, s_off[q + 2], s_off[q + 3], pxy, B);
                 // only the middle run holds the atom itself
                 if (oy == 1 && oz == 1)
-                    mt_neigh_run16<MODE, true>(d, nl4, sxz, sword, smol, e0, e1, pxy, pz, thr, lself, k, molk, xk, n,
-                                               cap, cnt, r0, r1, r2, r3);
+                    mt_neigh_run16r<MODE, true>(d, sxz, sword, smol, e0, e1, pxy, pz, thr, lself, k, molk, xk, g);
                 else
-                    mt_neigh_run16<MODE, false>(d, nl4, sxz, sword, smol, e0, e1, pxy, pz, thr, lself, k, molk, xk,
-                                                n, cap, cnt, r0, r1, r2, r3);
+                    mt_neigh_run16r<MODE, false>(d, sxz, sword, smol, e0, e1, pxy, pz, thr, lself, k, molk, xk, g);
             }
         // last partial chunk, padded with the atom's own staged place
+        const int cnt = g.cnt;
         const int slot = cnt & 7;
         if (slot && cnt < cap) {
-            for (int i = slot; i < 8; i++) mt_neigh_push<true>((uint32_t)lself, r0, r1, r2, r3);
-            nl4[(size_t)(cnt >> 3) * n + k] = make_uint4(r0, r1, r2, r3);
+            for (int i = slot; i < 8; i++) mt_ring_put(g, lself);
+            mt_ring_flush(g);
         }
         d.ncount[k] = cnt;
         // the capacity request is in 32-bit words
@@ -840,6 +908,7 @@ __global__ void __launch_bounds__(256) k_fast_neigh_block(MtDev d, MtParams p, i
     __shared__ int sword_buf[(TILE && MODE == 2) ? NB_CAP : 1];
     __shared__ int s_first[64], s_cnt[64], s_off[65];
     __shared__ int s_hfirst[8], s_hcnt[8];
+    __shared__ uint32_t sring[TILE ? 256 * MT_RING_DW : 1];   // 16-bit rows: a ring of list words a lane
     const int t = threadIdx.x;
     const int Nx = d.nc[0], Ny = d.nc[1], Nz = d.nc[2];
     const int bx = (Nx + 1) >> 1, by = (Ny + 1) >> 1;
@@ -887,7 +956,7 @@ __global__ void __launch_bounds__(256) k_fast_neigh_block(MtDev d, MtParams p, i
     // the word format of this block's lists (uniform; k_fast_force_tile makes
     // the same choice from the same counts)
     if (TILE && s_off[64] <= FT_CAP)
-        mt_neigh_block_atoms16<MODE>(d, p, spos, smol_buf, sword_buf, s_first, s_cnt, s_off, s_hfirst, s_hcnt);
+        mt_neigh_block_atoms16<MODE>(d, p, spos, smol_buf, sword_buf, s_first, s_cnt, s_off, s_hfirst, s_hcnt, sring);
     else
         mt_neigh_block_atoms<MODE, SCAP>(d, p, spos, smol_buf, s_first, s_cnt, s_off, hxo, hyo, hzo, l, f0, cnt_c,
                                          cnt_max);
@@ -2102,18 +2171,24 @@ __device__ __forceinline__ float mt_fast_locate_atom_to(const MtDev& d, const Mt
     MT_EXACT_FP
     const double4 q = d.posref[k];
     double dx = r.x - q.x, dy = r.y - q.y, dz = r.z - q.z;
-    dx -= p.L[0] * rint(dx / p.L[0]);
-    dy -= p.L[1] * rint(dy / p.L[1]);
-    dz -= p.L[2] * rint(dz / p.L[2]);
+    // nearest image by a multiplication with 1/L: the count differs from
+    // rint(dx / L) only where dx / L lies within rounding of a half-integer,
+    // and a displacement is far below L / 2 or (across the box) near L
+    dx -= p.L[0] * rint(dx * p.invL[0]);
+    dy -= p.L[1] * rint(dy * p.invL[1]);
+    dz -= p.L[2] * rint(dz * p.invL[2]);
     if (mixed) {
         const int c0 = d.home[k];
         const int ix = c0 % d.nc[0], iy = (c0 / d.nc[0]) % d.nc[1], iz = c0 / (d.nc[0] * d.nc[1]);
-        double lx = r.x - (-p.L[0] / 2.0 + ((double)ix + 0.5) * (p.L[0] / d.nc[0]));
-        double ly = r.y - (-p.L[1] / 2.0 + ((double)iy + 0.5) * (p.L[1] / d.nc[1]));
-        double lz = r.z - (-p.L[2] / 2.0 + ((double)iz + 0.5) * (p.L[2] / d.nc[2]));
-        lx -= p.L[0] * rint(lx / p.L[0]);
-        ly -= p.L[1] * rint(ly / p.L[1]);
-        lz -= p.L[2] * rint(lz / p.L[2]);
+        // the cell centre with cw = L / nc computed once (the same bits)
+        double lx = r.x - (-p.L[0] / 2.0 + ((double)ix + 0.5) * p.cw[0]);
+        double ly = r.y - (-p.L[1] / 2.0 + ((double)iy + 0.5) * p.cw[1]);
+        double lz = r.z - (-p.L[2] / 2.0 + ((double)iz + 0.5) * p.cw[2]);
+        // a local coordinate is within half a cell of zero, or near +-L once
+        // the atom's wrap has moved it across the box
+        lx -= p.L[0] * rint(lx * p.invL[0]);
+        ly -= p.L[1] * rint(ly * p.invL[1]);
+        lz -= p.L[2] * rint(lz * p.invL[2]);
         if (with_w) {
             posf_out[d.sidx[k]] = make_float4((float)lx, (float)ly, (float)lz, __int_as_float(d.type[k] * MT_NPTS));
         } else {

@@ -691,6 +691,8 @@ static int setup(mtmd_handle* h, const mtmd_system* s)
                                           "(box %.4g, cell %.4g)", p.L[i], (double)p.rlist);
             if (p.L[i] < 2.0 * p.rlist) return h->fail(MTMD_EBOX, "box below twice the list radius");
             p.cs[i] = (float)(p.L[i] / d.nc[i]);
+            p.cw[i] = p.L[i] / d.nc[i];
+            p.invL[i] = 1.0 / p.L[i];
         }
         h->half_skin2 = (c.skin / 2.0) * (c.skin / 2.0);
         p.half_skin2 = (float)h->half_skin2;
@@ -1050,7 +1052,11 @@ static void set_box(mtmd_handle* h, const double L[3])
         const double Lf = (double)p.Lf[i];
         p.wbox[i] = (Lf / 2.0) - (-Lf / 2.0);
         p.wnl[i] = (double)(p.lowf[i] + p.Lf[i]) - (double)p.lowf[i];
-        if (h->fast) p.cs[i] = (float)(L[i] / h->d.nc[i]);
+        if (h->fast) {
+            p.cs[i] = (float)(L[i] / h->d.nc[i]);
+            p.cw[i] = L[i] / h->d.nc[i];
+            p.invL[i] = 1.0 / L[i];
+        }
     }
 }
 

@@ -130,6 +130,8 @@ struct MtParams {
     double dt, Fcut;
     float x0, inv_delta;   // non-bonded table start and 1/spacing (MIXED)
     float cs[3];           // cell size per axis (MIXED local coordinates)
+    double cw[3];          // the same in double, L / nc (cell centres)
+    double invL[3];        // 1 / L (nearest-image counts of displacements and local coordinates)
     float half_skin2;      // rebuild threshold on the squared displacement
     int dslot;             // flags[dslot]: the rebuild flag the conditional list kernels read; the
                            // kernels that move atoms clear it and set flags[dslot ^ 1] (the host

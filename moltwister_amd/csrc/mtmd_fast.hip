@@ -1344,6 +1344,29 @@ __global__ void __launch_bounds__(512) k_fast_force_mixed_lds(MtDev d, MtParams 
 // words and gathers from global memory instead (per-offset shifts). G lanes
 // per home atom.
 
+// The tile passes without charges (LEAN) need no inside test: a pair at or
+// beyond rc, or closer than the table's first point, indexes segment
+// MT_NPTS - 1 of its row, whose force and energy coefficients are zero, so its
+// terms vanish by themselves. Only the band around rc, where FP32 and the FP64
+// reference predicate may disagree, is corrected afterwards: the band path
+// takes back these same terms when the FP64 test puts the pair outside. Both
+// places compute them with these helpers, the same operations in the same
+// order, and the force-only and energy passes give the same forces.
+__device__ __forceinline__ float mt_lean_r2(float dx, float dy, float dz)
+{
+    return fmaf(dz, dz, fmaf(dx, dx, dy * dy));
+}
+
+// 1/r (r^2 clamped away from zero: the pad word is the atom itself) and the
+// segment of the pair
+__device__ __forceinline__ float mt_lean_rinv(float r2, float x0, float invd, int& seg)
+{
+    const float rinv = __builtin_amdgcn_rsqf(fmaxf(r2, 1.0e-6f));
+    const int idx = (int)(fmaf(r2, rinv, -x0) * invd);
+    seg = (int)min((unsigned)idx, (unsigned)(MT_NPTS - 1));
+    return rinv;
+}
+
 // four pairs of a lane, without a branch on the common path: a pair outside
 // rc (or the atom itself) still reads a segment and its force is selected
 // away, and a pair outside the table reads segment MT_NPTS - 1 of its row,
@@ -1355,6 +1378,7 @@ __device__ __forceinline__ uint32_t mt_tile_pairs4(const float4 (&a)[4], const f
                                                    const float4& pk, const CoefT* row, float lo, float hi, float x0,
                                                    float invd, float qk, float& cx, float& cy, float& cz, float& U)
 {
+    constexpr bool LEAN = !COUL;
     float dx[4], dy[4], dz[4], rinv[4], r[4];
     bool in[4];
     int ix[4];
@@ -1364,14 +1388,23 @@ __device__ __forceinline__ uint32_t mt_tile_pairs4(const float4 (&a)[4], const f
         dx[i] = SHIFT ? pk.x - a[i].x - o[i].x : pk.x - a[i].x;
         dy[i] = SHIFT ? pk.y - a[i].y - o[i].y : pk.y - a[i].y;
         dz[i] = SHIFT ? pk.z - a[i].z - o[i].z : pk.z - a[i].z;
-        const float r2 = dx[i] * dx[i] + dy[i] * dy[i] + dz[i] * dz[i];
-        const bool lt = r2 < lo;
-        in[i] = lt && r2 >= 1.0e-6f;
-        band |= (!lt && r2 < hi) ? 1u << i : 0u;
-        rinv[i] = __builtin_amdgcn_rsqf(fmaxf(r2, 1.0e-6f));
-        r[i] = r2 * rinv[i];
-        const int idx = (int)((r[i] - x0) * invd);
-        const int seg = (int)min((unsigned)idx, (unsigned)(MT_NPTS - 1));
+        int seg;
+        if constexpr (LEAN) {
+            const float r2 = mt_lean_r2(dx[i], dy[i], dz[i]);
+            band |= (r2 >= lo && r2 < hi) ? 1u << i : 0u;
+            rinv[i] = mt_lean_rinv(r2, x0, invd, seg);
+            in[i] = true;
+            r[i] = WANT_E ? r2 * rinv[i] : 0.0f;
+        } else {
+            const float r2 = dx[i] * dx[i] + dy[i] * dy[i] + dz[i] * dz[i];
+            const bool lt = r2 < lo;
+            in[i] = lt && r2 >= 1.0e-6f;
+            band |= (!lt && r2 < hi) ? 1u << i : 0u;
+            rinv[i] = __builtin_amdgcn_rsqf(fmaxf(r2, 1.0e-6f));
+            r[i] = r2 * rinv[i];
+            const int idx = (int)((r[i] - x0) * invd);
+            seg = (int)min((unsigned)idx, (unsigned)(MT_NPTS - 1));
+        }
         // .w: the neighbour's type times MT_NPTS, its offset in the row; in
         // elements for global neighbours (SHIFT), in bytes for staged ones
         ix[i] = __float_as_int(a[i].w) + (SHIFT ? seg : seg * (int)sizeof(CoefT));
@@ -1382,6 +1415,14 @@ __device__ __forceinline__ uint32_t mt_tile_pairs4(const float4 (&a)[4], const f
         c[i] = SHIFT ? row[ix[i]] : *reinterpret_cast<const CoefT*>(reinterpret_cast<const char*>(row) + ix[i]);
 #pragma unroll
     for (int i = 0; i < 4; i++) {
+        if constexpr (LEAN) {
+            const float fr = fmaf(c[i].y, rinv[i], c[i].x);
+            if constexpr (WANT_E) U += 0.5f * fmaf(c[i].z, r[i], c[i].w);
+            cx = fmaf(dx[i], fr, cx);
+            cy = fmaf(dy[i], fr, cy);
+            cz = fmaf(dz[i], fr, cz);
+            continue;
+        }
         float fr = c[i].x + c[i].y * rinv[i];
         float K = 0.0f;
         if constexpr (COUL) {
@@ -1416,7 +1457,25 @@ __device__ __forceinline__ void mt_tile_band16(int l, const float4& a, float qa,
     }
     const int u = s_first[lo] + (l - s_off[lo]);
     const double4 rk4 = d.pos[k], ra4 = d.pos[d.cell_list[u]];
-    if (!(mt_dist2_pbc({rk4.x, rk4.y, rk4.z}, {ra4.x, ra4.y, ra4.z}, p.wnl) < (double)p.rc2f)) return;
+    const bool inside = mt_dist2_pbc({rk4.x, rk4.y, rk4.z}, {ra4.x, ra4.y, ra4.z}, p.wnl) < (double)p.rc2f;
+    if constexpr (!COUL) {
+        // the pair loop added this pair's table terms: they stand when FP64
+        // puts the pair inside, and are taken back otherwise
+        if (inside) return;
+        const float dx = pk.x - a.x, dy = pk.y - a.y, dz = pk.z - a.z;
+        int seg;
+        const float r2 = mt_lean_r2(dx, dy, dz);
+        const float rinv = mt_lean_rinv(r2, x0, invd, seg);
+        const CoefT c = *reinterpret_cast<const CoefT*>(reinterpret_cast<const char*>(row) + __float_as_int(a.w) +
+                                                        seg * (int)sizeof(CoefT));
+        const float fr = fmaf(c.y, rinv, c.x);
+        if constexpr (WANT_E) U -= 0.5f * fmaf(c.z, r2 * rinv, c.w);
+        cx = fmaf(-dx, fr, cx);
+        cy = fmaf(-dy, fr, cy);
+        cz = fmaf(-dz, fr, cz);
+        return;
+    }
+    if (!inside) return;
     const float dx = pk.x - a.x, dy = pk.y - a.y, dz = pk.z - a.z;
     const float r2 = dx * dx + dy * dy + dz * dz;
     // the staged type offset is in bytes, mt_mixed_force's in elements
@@ -1474,6 +1533,31 @@ __device__ __forceinline__ void mt_tile_atom16(const MtDev& d, const MtParams& p
     }
 }
 
+// a band pair of a 32-bit block list in a pass without charges: the pair
+// loop's terms (mt_tile_pairs4 with SHIFT) are taken back when the FP64
+// reference predicate puts the pair outside rc
+template <bool WANT_E, typename CoefT>
+__device__ __forceinline__ void mt_tile_band32_lean(uint32_t w, const float4& pk, const float4* shd, const CoefT* row,
+                                                    float x0, float invd, float& cx, float& cy, float& cz, float& U,
+                                                    const MtDev& d, const MtParams& p, int k)
+{
+    const int u = (int)(w & MT_NW_IDX_MASK);          // place in cell order
+    const double4 rk4 = d.pos[k], ra4 = d.pos[d.cell_list[u]];
+    if (mt_dist2_pbc({rk4.x, rk4.y, rk4.z}, {ra4.x, ra4.y, ra4.z}, p.wnl) < (double)p.rc2f) return;
+    const float4 a = d.posf[u];
+    const float4 o = shd[__builtin_amdgcn_ubfe(w, MT_NW_IDX_BITS, 5)];
+    const float dx = pk.x - a.x - o.x, dy = pk.y - a.y - o.y, dz = pk.z - a.z - o.z;
+    int seg;
+    const float r2 = mt_lean_r2(dx, dy, dz);
+    const float rinv = mt_lean_rinv(r2, x0, invd, seg);
+    const CoefT c = row[__float_as_int(a.w) + seg];
+    const float fr = fmaf(c.y, rinv, c.x);
+    if constexpr (WANT_E) U -= 0.5f * fmaf(c.z, r2 * rinv, c.w);
+    cx = fmaf(-dx, fr, cx);
+    cy = fmaf(-dy, fr, cy);
+    cz = fmaf(-dz, fr, cz);
+}
+
 // the same for a block over FT_CAP atoms: 32-bit coded words, positions and
 // charges gathered from global memory, the offset's shift from shd
 template <bool WANT_E, bool COUL, typename CoefT>
@@ -1506,8 +1590,12 @@ __device__ __forceinline__ void mt_tile_atom32(const MtDev& d, const MtParams& p
         if (band) {
 #pragma unroll
             for (int i = 0; i < 4; i++)
-                if (band & (1u << i))
-                    mt_mixed_band<WANT_E, COUL>(wv[i], pk, shd, row, x0, invd, qk, cx, cy, cz, U, d, p, k);
+                if (band & (1u << i)) {
+                    if constexpr (!COUL)
+                        mt_tile_band32_lean<WANT_E>(wv[i], pk, shd, row, x0, invd, cx, cy, cz, U, d, p, k);
+                    else
+                        mt_mixed_band<WANT_E, COUL>(wv[i], pk, shd, row, x0, invd, qk, cx, cy, cz, U, d, p, k);
+                }
         }
         Fx += (double)cx;
         Fy += (double)cy;

@@ -159,6 +159,43 @@ def test_thermo_definitions(oracle_mod, lj):
     assert th["V"] == pytest.approx(V, rel=1e-15)
 
 
+def test_thermo_closed_forms(oracle_mod):
+    """T, K, P, U and V from states whose values are known without the
+    reduction formulas: four free atoms with masses 1..4 and momenta chosen so
+    sum p.p/m = 8, and an isolated LJ dimer at rest, whose virial sum F.r is
+    d f(d) with f(d) = 24 (2/d^13 - 1/d^7) and whose energy is 4 (d^-12 - d^-6).
+    A wrong degree count (2N), a dropped 1/m, a virial of the wrong sign or
+    without its 1/3, or a missing unit factor each fails one of them."""
+    from moltwister_amd.system import System
+    L = 20.0
+    V = L ** 3
+    # free atoms: farther apart than the cutoff, so no forces and U = 0
+    pos = [[-6.0, -6.0, -6.0], [6.0, -6.0, -6.0], [-6.0, 6.0, 6.0], [6.0, 6.0, 0.0]]
+    mom = [[1.0, 0.0, 0.0], [0.0, 2.0, 0.0], [0.0, 0.0, 3.0], [2.0, 2.0, 0.0]]     # p.p/m = 1, 2, 3, 2
+    s = System(pos=pos, mass=[1.0, 2.0, 3.0, 4.0], charge=[0] * 4, types=[0] * 4, box=(L, L, L),
+               nonbonded=[(0, 0, abi.NB_LJ, (1.0, 1.0, 1.0))], mom=mom)
+    o = oracle_mod.Oracle(s, cfg(2.5, 0.3, ensemble=abi.NVE))
+    o.compute_forces()
+    th = o.thermo()
+    assert th["T"] == pytest.approx(8.0 / 12.0 * 120.272, rel=1e-14)
+    assert th["K"] == pytest.approx(4.0, rel=1e-14)
+    assert th["P"] == pytest.approx(8.0 / (3.0 * V) * 16388.2461, rel=1e-14)
+    assert th["U"] == 0.0
+    assert th["V"] == pytest.approx(V, rel=1e-15)
+    # dimer at rest at separation d: T = K = 0, P = d f(d) / (3V), U = LJ(d)
+    for d in (1.05, 1.3, 2.0):
+        s = System(pos=[[-d / 2, 0.0, 0.0], [d / 2, 0.0, 0.0]], mass=[1.0, 1.0], charge=[0, 0], types=[0, 0],
+                   box=(L, L, L), nonbonded=[(0, 0, abi.NB_LJ, (1.0, 1.0, 1.0))], mom=[[0.0] * 3] * 2)
+        o = oracle_mod.Oracle(s, cfg(2.5, 0.3, ensemble=abi.NVE))
+        o.compute_forces()
+        th = o.thermo()
+        f = 24.0 * (2.0 / d ** 13 - 1.0 / d ** 7)
+        assert th["T"] == 0.0 and th["K"] == 0.0
+        # the tabulated force is the analytic one to ~1e-4 of 24 eps/sigma
+        assert th["P"] == pytest.approx(d * f / (3.0 * V) * 16388.2461, abs=d * 24 * 2e-4 / (3.0 * V) * 16388.2461)
+        assert th["U"] == pytest.approx(4.0 * (d ** -12 - d ** -6), abs=5e-4)   # tabulation error (1.2e-4 at 1.05)
+
+
 def test_gradient_descent_step(oracle_mod, lj):
     """runOptimization: r += lr F with uncapped forces (MDLoop.cpp:143-173)."""
     s, c = lj
@@ -340,3 +377,51 @@ def test_oracle_threads_give_identical_results(oracle_mod):
         n1, l1 = o1.build_neighbors()
         n4, l4 = o4.build_neighbors()
         assert np.array_equal(n1, n4) and np.array_equal(l1, l4)
+
+
+def test_exclusion_scaling_known_answer(oracle_mod):
+    """Which pairs the per-atom bonded lists mark as 1-2, 1-3, 1-4 and same
+    molecule (FunctorCalcForce.cpp:95-157, priority 1-2 > 1-3 > 1-4 > 1N),
+    from the forces alone: a five-atom chain 0-1-2-3-4 (four bonds, three
+    angles, two dihedrals, all of zero strength) and a sixth atom in its own
+    molecule, LJ between every pair, the four scale factors all different.
+    The force on each atom is sum_j s_ij f_LJ(r_ij) r_ij / r_ij with s = 0.1
+    (1-2), 0.25 (1-3), 0.5 (1-4), 0.75 (same molecule otherwise: 0-4) and 1
+    (other molecule), computed here analytically."""
+    from moltwister_amd.system import System
+    pos = np.array([[0.0, 0.0, 0.0], [1.1, 0.0, 0.0], [1.1, 1.1, 0.0], [0.0, 1.1, 0.0], [0.0, 1.1, 1.1],
+                    [-0.3, -0.2, 1.4]])
+    bonds = [[0, 1, 0], [1, 2, 0], [2, 3, 0], [3, 4, 0]]
+    angles = [[0, 1, 2, 0], [1, 2, 3, 0], [2, 3, 4, 0]]
+    dih = [[0, 1, 2, 3, 0], [1, 2, 3, 4, 0]]
+    s = System(pos=pos, mass=[1.0] * 6, charge=[0.0] * 6, types=[0] * 6, box=(20, 20, 20),
+               nonbonded=[(0, 0, abi.NB_LJ, (1.0, 1.0, 1.0))],
+               bond_ff=[(abi.BOND_HARM, (0.0, 1.1))], angle_ff=[(abi.ANGLE_HARM, (0.0, 90.0))],
+               dihedral_ff=[(abi.DIH_HARM, (0.0, 1, 3))], bonds=bonds, angles=angles, dihedrals=dih)
+    c = cfg(2.5, 0.3)
+    c.scale12, c.scale13, c.scale14, c.scale1N = 0.1, 0.25, 0.5, 0.75
+    o = oracle_mod.Oracle(s, c)
+    F, U = o.compute_forces()
+    sc = np.ones((6, 6))
+    for a, b in ((0, 4),):
+        sc[a, b] = sc[b, a] = 0.75
+    for a, b in ((0, 3), (1, 4)):
+        sc[a, b] = sc[b, a] = 0.5
+    for a, b in ((0, 2), (1, 3), (2, 4)):
+        sc[a, b] = sc[b, a] = 0.25
+    for a, b, _ in bonds:
+        sc[a, b] = sc[b, a] = 0.1
+    want = np.zeros((6, 3))
+    u_want = 0.0
+    for i in range(6):
+        for j in range(6):
+            if i == j:
+                continue
+            d = pos[i] - pos[j]
+            r = np.linalg.norm(d)
+            want[i] += sc[i, j] * 24.0 * (2.0 / r ** 13 - 1.0 / r ** 7) * d / r
+            if i > j:
+                u_want += sc[i, j] * 4.0 * (r ** -12 - r ** -6)
+    # the tabulated LJ is the analytic one to ~2e-4 of 24 eps/sigma at r >= 1.1
+    assert np.abs(F - want).max() < 24 * 5e-4, np.abs(F - want).max()
+    assert U.sum(dtype=np.float64) == pytest.approx(u_want, abs=1e-3)

@@ -28,7 +28,8 @@ METRIC = "atom-steps/s (1M LJ, 30k water), mixed vs FP64 on same GPU at stated d
 HBM_PEAK = 8.0e12          # MI355X HBM3E, bytes/s
 EQ_ROUNDS, EQ_STEPS = 10, 200      # equilibration of the 1M-atom mixture (NVE rounds rescaled to T* = 1)
 DRIFT_STEPS, DRIFT_EVERY = 10000, 1000
-WATER_EQ_ROUNDS, WATER_EQ_STEPS, WATER_DRIFT_STEPS = 5, 200, 1000
+WATER_EQ_ROUNDS, WATER_EQ_STEPS, WATER_DRIFT_STEPS = 40, 200, 1000
+WATER_NVT_EQ_STEPS = 2000
 
 
 def log(*a):
@@ -326,6 +327,23 @@ def main() -> int:
                 eth.set_state(mom=mw)
             eth.close()
             sysW.pos, sysW.mom = pw, mw
+            # then the timed run's own Nose-Hoover NVT at 298 K until the
+            # temperature has settled; its chain state goes on into the timed runs
+            ceq = water_config(abi.ENGINE_MIXED)
+            ceq.coulomb = coul
+            eeq = Engine(sysW, ceq)
+            t_eq = []
+            for _ in range(WATER_NVT_EQ_STEPS // 200):
+                eeq.step(200)
+                t_eq.append(eeq.thermo()["T"])
+            sysW.pos, sysW.mom = eeq.get_state()
+            nh_w = eeq.get_thermostat()
+            eeq.close()
+            # the reference chain resets p_eta[0] > 15 to 10 at every call
+            # (NHChain.cpp:61-65), which caps its friction at about 15 / Q0 with
+            # Q0 = 3 N T tau^2: the chain's first momentum shows when it runs there
+            water[f"thermostat_{mode}"] = {"p_eta0_after_equilibration": float(nh_w[1][0]),
+                                           "Q0": 3 * sysW.n_atoms * (298.0 / 120.272) * (20.0 / 100.0) ** 2}
             F_w = {}
             for eng_id, ename in ((abi.ENGINE_MIXED, "mixed"), (abi.ENGINE_FP64, "fp64")):
                 key = f"{ename}_{mode}"
@@ -334,13 +352,24 @@ def main() -> int:
                 cwt.block_size = args.block
                 try:
                     ew = Engine(sysW, cwt)
+                    ew.set_thermostat(*nh_w)
                     F_w[ename] = ew.compute_forces()[0]
                     T0 = ew.thermo()["T"]
                     ew.step(20)
                     kw = 200 if eng_id == abi.ENGINE_MIXED else 100
                     dtw = timed_steps(ew, kw)
+                    T1 = ew.thermo()["T"]
+                    # the temperature over the 200 NVT steps that follow (untimed)
+                    t_after = []
+                    for _ in range(10):
+                        ew.step(20)
+                        t_after.append(ew.thermo()["T"])
+                    t_mean = float(np.mean(t_after))
                     water[key] = {"atom_steps_per_s": sysW.n_atoms * kw / dtw, "ms_per_step": dtw / kw * 1e3,
-                                  "nvt_steps": kw, "T_start_K": T0, "T_end_K": ew.thermo()["T"]}
+                                  "nvt_steps": kw, "T_start_K": T0, "T_end_K": T1,
+                                  "T_mean_next_200_K": t_mean, "T_within_10K_of_298": bool(abs(t_mean - 298.0) <= 10.0
+                                                                                          and abs(T1 - 298.0) <= 10.0),
+                                  "nvt_equilibration_T_K": [round(x, 1) for x in t_eq]}
                     ew.close()
                     # NVE drift from the same thermalised state
                     cwe = water_config(eng_id)
@@ -360,7 +389,8 @@ def main() -> int:
                 water[f"force_rms_rel_mixed_vs_fp64_{mode}"] = rms_rel(F_w["mixed"], F_w["fp64"])
         water["workload"] = ("10,000 flexible 3-site waters (30,000 atoms), 1 g/cm^3, L = 66.88 A; 300 descent "
                              f"steps, then per Coulomb mode {WATER_EQ_ROUNDS} x {WATER_EQ_STEPS} NVE steps each "
-                             "rescaled to 298 K; timed Nose-Hoover NVT at 298 K (tau 20 fs, M 4, n 4), dt 0.5 fs; "
+                             f"rescaled to 298 K and {WATER_NVT_EQ_STEPS} Nose-Hoover NVT steps at 298 K; timed "
+                             "Nose-Hoover NVT at 298 K (tau 20 fs, M 4, n 4), dt 0.5 fs, from that state and chain; "
                              f"NVE drift over {WATER_DRIFT_STEPS} steps from the same state")
         log(f"bench: water {water}")
 

@@ -471,11 +471,94 @@ __device__ inline double mt_nh_propagate_fixed(double* eta_g, double* peta_g, co
     return S;
 }
 
+// The FP64/MIXED engines run the same chain to trace agreement rather than bit
+// for bit: the constants of a call are formed once (1/Q, Dt/8, Dt/4, Dt/2 and
+// g T for g / beta) and each scale factor is computed once and applied twice.
+// CMt::exp keeps its linear branch for x > 1; its small arguments (|x| < 1/32,
+// the chain's usual range, |x| = Dt |p_eta| / (8 Q)) take the Taylor series
+// of exp to x^8, whose truncation is below x^9 / 9! < 1e-19.
+__device__ inline double mt_cmt_exp_fast(double x)
+{
+    if (x > 1.0) return 2.71828 * x;
+    if (fabs(x) < 0.03125) {
+        double t = 1.0 / 40320.0;
+        t = fma(t, x, 1.0 / 5040.0);
+        t = fma(t, x, 1.0 / 720.0);
+        t = fma(t, x, 1.0 / 120.0);
+        t = fma(t, x, 1.0 / 24.0);
+        t = fma(t, x, 1.0 / 6.0);
+        t = fma(t, x, 0.5);
+        t = fma(t, x, 1.0);
+        return fma(t, x, 1.0);
+    }
+    return exp(x);
+}
+
+template <int M>
+__device__ inline double mt_nh_propagate_fast(double* eta_g, double* peta_g, const double* Q_g, int nresp, double T,
+                                              double dt, double g, const double* w, double KE2)
+{
+    double eta[M], peta[M], iQ[M];
+#pragma unroll
+    for (int j = 0; j < M; j++) {
+        eta[j] = eta_g[j];
+        peta[j] = peta_g[j];
+        iQ[j] = 1.0 / Q_g[j];
+    }
+    double S = 1.0;
+    const double gT = g * T;
+    if (peta[0] > 15.0) peta[0] = 10.0;
+    for (int mu = 0; mu < 3; mu++) {
+        const double Dt = w[mu] * dt / (double)nresp;
+        const double q8 = Dt * 0.125, q4 = Dt * 0.25, q2 = Dt * 0.5;
+        for (int i = 0; i < nresp; i++) {
+#define MT_GF(j) ((j) == 0 ? (KE2 - gT) : (peta[(j) - 1] * peta[(j) - 1] * iQ[(j) - 1] - T))
+            peta[M - 1] += q4 * MT_GF(M - 1);
+#pragma unroll
+            for (int j = 0; j < M - 1; j++) {
+                const double c = mt_cmt_exp_fast(-q8 * (peta[j + 1] * iQ[j + 1]));
+                peta[j] = (peta[j] * c + q4 * MT_GF(j)) * c;
+            }
+#pragma unroll
+            for (int j = 0; j < M; j++) eta[j] += q2 * (peta[j] * iQ[j]);
+            const double c0 = mt_cmt_exp_fast(-(q2 * (peta[0] * iQ[0])));
+            S *= c0;
+            KE2 *= c0 * c0;
+#pragma unroll
+            for (int j = M - 2; j >= 0; j--) {
+                const double c = mt_cmt_exp_fast(-q8 * (peta[j + 1] * iQ[j + 1]));
+                peta[j] = (peta[j] * c + q4 * MT_GF(j)) * c;
+            }
+            peta[M - 1] += q4 * MT_GF(M - 1);
+#undef MT_GF
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < M; j++) {
+        eta_g[j] = eta[j];
+        peta_g[j] = peta[j];
+    }
+    return S;
+}
+
 // the chain propagation of a thermostat step: in registers for chains of up
 // to 8 links, else on the state in memory
 __device__ inline double mt_nh_step(double* eta, double* peta, const double* Q, int M, int nresp, double T,
-                                    double dt, double g, const double* w, double KE2)
+                                    double dt, double g, const double* w, double KE2, bool fast = false)
 {
+    if (fast) {
+        switch (M) {
+        case 1: return mt_nh_propagate_fast<1>(eta, peta, Q, nresp, T, dt, g, w, KE2);
+        case 2: return mt_nh_propagate_fast<2>(eta, peta, Q, nresp, T, dt, g, w, KE2);
+        case 3: return mt_nh_propagate_fast<3>(eta, peta, Q, nresp, T, dt, g, w, KE2);
+        case 4: return mt_nh_propagate_fast<4>(eta, peta, Q, nresp, T, dt, g, w, KE2);
+        case 5: return mt_nh_propagate_fast<5>(eta, peta, Q, nresp, T, dt, g, w, KE2);
+        case 6: return mt_nh_propagate_fast<6>(eta, peta, Q, nresp, T, dt, g, w, KE2);
+        case 7: return mt_nh_propagate_fast<7>(eta, peta, Q, nresp, T, dt, g, w, KE2);
+        case 8: return mt_nh_propagate_fast<8>(eta, peta, Q, nresp, T, dt, g, w, KE2);
+        default: break;                           // longer chains: the reference order below
+        }
+    }
     switch (M) {
     case 1: return mt_nh_propagate_fixed<1>(eta, peta, Q, nresp, T, dt, g, w, KE2);
     case 2: return mt_nh_propagate_fixed<2>(eta, peta, Q, nresp, T, dt, g, w, KE2);

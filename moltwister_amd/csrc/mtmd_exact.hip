@@ -371,7 +371,7 @@ void mt_cap_kick(const MtDev& d, const MtParams& p, int want_ke, hipStream_t s)
 // mt_nh_propagate in mtmd_device.h; one chain, nh layout: eta[M], p_eta[M],
 // Q[M], scale. Sums the block partials of p.p/m first.
 __global__ void k_nh_chain(double* nh, const double* red, int nblocks, int M, int nresp, double T, double dt,
-                           double g, double w0, double w1, double w2)
+                           double g, double w0, double w1, double w2, int fast)
 {
     __shared__ double sh[64];
     double s = 0.0;
@@ -382,14 +382,14 @@ __global__ void k_nh_chain(double* nh, const double* red, int nblocks, int M, in
     double KE2 = 0.0;
     for (int t = 0; t < 64; t++) KE2 += sh[t];
     const double w[3] = {w0, w1, w2};
-    nh[3 * M] = mt_nh_step(nh, nh + M, nh + 2 * M, M, nresp, T, dt, g, w, KE2);
+    nh[3 * M] = mt_nh_step(nh, nh + M, nh + 2 * M, M, nresp, T, dt, g, w, KE2, fast != 0);
 }
 
 void mt_nh_chain(const MtDev& d, int M, int nresp, double T, double dt, double g, const double* w, int nblocks,
-                 hipStream_t s)
+                 hipStream_t s, int fast)
 {
     hipLaunchKernelGGL(k_nh_chain, dim3(1), dim3(64), 0, s, d.nh, d.red, nblocks, M, nresp, T, dt, g, w[0], w[1],
-                       w[2]);
+                       w[2], fast);
 }
 
 // p *= S; pbcWrap (SimulationBox.cpp:135-166); optional partials of p.p/m

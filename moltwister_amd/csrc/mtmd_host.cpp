@@ -1461,7 +1461,7 @@ static int npt_steps(mtmd_handle* h, int nsteps, bool energy_last)
         const bool last = (t == nsteps - 1);
         npt_nhp(h);
         if (!h->ke_valid) mt_reduce_ke(d, h->stream);
-        mt_nh_chain(d, h->M, h->nresp, h->Tred, p.dt, h->g, h->w, nb, h->stream);
+        mt_nh_chain(d, h->M, h->nresp, h->Tred, p.dt, h->g, h->w, nb, h->stream, h->fast ? 1 : 0);
         mt_npt_scale_sums(d, h->nh_scale_slot, h->stream);
         double sp, sf;
         if ((rc = sums(sp, sf))) return rc;
@@ -1509,7 +1509,7 @@ static int npt_steps(mtmd_handle* h, int nsteps, bool energy_last)
             }
             if (h->h_status[2]) break;                // list overflow: reported below
         }
-        mt_nh_chain(d, h->M, h->nresp, h->Tred, p.dt, h->g, h->w, nb, h->stream);
+        mt_nh_chain(d, h->M, h->nresp, h->Tred, p.dt, h->g, h->w, nb, h->stream, h->fast ? 1 : 0);
         npt_nhp(h);
         mt_scale_wrap(d, p, h->nh_scale_slot, 1, h->stream);
         h->ke_valid = true;
@@ -1546,12 +1546,12 @@ static int md_steps_once(mtmd_handle* h, int nsteps, bool energy_last)
         for (int t = 0; t < nsteps; t++) {
             if (nvt) {
                 if (!h->ke_valid) mt_reduce_ke(d, h->stream);
-                mt_nh_chain(d, h->M, h->nresp, h->Tred, h->p.dt, h->g, h->w, nblocks, h->stream);
+                mt_nh_chain(d, h->M, h->nresp, h->Tred, h->p.dt, h->g, h->w, nblocks, h->stream, h->fast ? 1 : 0);
             }
             mt_kick_drift(d, h->p, nvt ? h->nh_scale_slot : -1, 0, h->stream);
             if ((rc = exact_forces(h))) return rc;
             mt_cap_kick(d, h->p, nvt ? 1 : 0, h->stream);
-            if (nvt) mt_nh_chain(d, h->M, h->nresp, h->Tred, h->p.dt, h->g, h->w, nblocks, h->stream);
+            if (nvt) mt_nh_chain(d, h->M, h->nresp, h->Tred, h->p.dt, h->g, h->w, nblocks, h->stream, h->fast ? 1 : 0);
             mt_scale_wrap(d, h->p, nvt ? h->nh_scale_slot : -1, nvt ? 1 : 0, h->stream);
             h->ke_valid = nvt;
             h->steps++;
@@ -1562,12 +1562,12 @@ static int md_steps_once(mtmd_handle* h, int nsteps, bool energy_last)
         // the last of the call) replay a captured graph of their kernels.
         auto nvt_front = [&]() {
             if (!h->ke_valid) mt_reduce_ke(d, h->stream);
-            mt_nh_chain(d, h->M, h->nresp, h->Tred, h->p.dt, h->g, h->w, nblocks, h->stream);
+            mt_nh_chain(d, h->M, h->nresp, h->Tred, h->p.dt, h->g, h->w, nblocks, h->stream, h->fast ? 1 : 0);
             mt_fast_kick_drift(d, h->p, h->nh_scale_slot, mixed, h->stream);
         };
         auto nvt_back = [&]() {
             mt_cap_kick(d, h->p, 1, h->stream);
-            mt_nh_chain(d, h->M, h->nresp, h->Tred, h->p.dt, h->g, h->w, nblocks, h->stream);
+            mt_nh_chain(d, h->M, h->nresp, h->Tred, h->p.dt, h->g, h->w, nblocks, h->stream, h->fast ? 1 : 0);
             mt_scale_wrap(d, h->p, h->nh_scale_slot, 1, h->stream);
         };
         // NVE in the MIXED tile mode with every force inside the pair pass:

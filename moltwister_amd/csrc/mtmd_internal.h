@@ -144,8 +144,10 @@ void mt_exact_neighbors(const MtDev& d, const MtParams& p, hipStream_t s);
 void mt_exact_forces(const MtDev& d, const MtParams& p, hipStream_t s);
 void mt_kick_drift(const MtDev& d, const MtParams& p, int scale_slot, int track_disp, hipStream_t s);
 void mt_cap_kick(const MtDev& d, const MtParams& p, int want_ke, hipStream_t s);
+// fast: the FP64/MIXED form of the chain (mt_nh_propagate_fast), else the
+// reference's operation order
 void mt_nh_chain(const MtDev& d, int M, int nresp, double T, double dt, double g, const double* w,
-                 int nblocks, hipStream_t s);
+                 int nblocks, hipStream_t s, int fast = 0);
 void mt_scale_wrap(const MtDev& d, const MtParams& p, int scale_slot, int want_ke, hipStream_t s);
 void mt_reduce_ke(const MtDev& d, hipStream_t s);
 void mt_reduce_thermo(const MtDev& d, hipStream_t s);

@@ -451,11 +451,16 @@ typedef struct mtmd_batch mtmd_batch;
 int  mtmd_batch_create(int n_systems, const mtmd_system* systems, const mtmd_config* cfgs, mtmd_batch** out);
 void mtmd_batch_destroy(mtmd_batch* b);
 const char* mtmd_batch_last_error(const mtmd_batch* b);
-/* nsteps of every replica: one captured hipGraph of the whole batch's step,
- * replayed; the last step runs eagerly and also computes energies */
+/* nsteps of every replica, energies computed in the last. When every replica
+ * holds at most 4096 atoms and 1024 cells and there are at most two atom
+ * types, one launch of a persistent kernel runs all nsteps: a workgroup per
+ * replica, its FP32 positions and the pair table resident in LDS, list
+ * rebuilds, forces, integration and chain inside the workgroup. Otherwise (or
+ * with MTMD_BATCH_KERNELS set in the environment at create time) a captured
+ * hipGraph of the whole batch's step is replayed, the last step eagerly. */
 int mtmd_batch_step(mtmd_batch* b, int32_t nsteps);
-/* the same steps with every kernel launched replica by replica (the cost of
- * running n separate systems); same results */
+/* the same steps launched replica by replica, one launch a replica a step
+ * (the cost of running n separate systems); same results, bit for bit */
 int mtmd_batch_step_separate(mtmd_batch* b, int32_t nsteps);
 int mtmd_batch_synchronize(mtmd_batch* b);
 /* thermo of every replica, out[n_systems] (T, P, V, U, K, Etot, sums; U_cut 0) */
@@ -463,10 +468,14 @@ int mtmd_batch_get_thermo(mtmd_batch* b, mtmd_thermo* out);
 /* state, forces (capped, as integrated) and per-atom energies of replica i */
 int mtmd_batch_get_state(mtmd_batch* b, int32_t i, double* pos, double* mom);
 int mtmd_batch_get_forces(mtmd_batch* b, int32_t i, double* F, float* U);
-/* block-size autotuner: times the pair-force kernel (64..512) and the list
- * build (64..256) over the whole batch, `reps` launches each, keeps the
- * fastest; returns the chosen kernels' average times */
+/* autotuner over the whole batch, `reps` launches a candidate; returns the
+ * chosen shape's force-pass and list-build times. Persistent kernel: the
+ * workgroup size (256, 512, 1024) times the lanes per atom (1, 2, 4), scored
+ * as a force pass plus a fifth of a list build. Graph path: the pair-force
+ * kernel's block (64..512) and the list build's (64..256). */
 int mtmd_batch_autotune(mtmd_batch* b, int reps, double* force_ms, double* list_ms);
+/* force_block / list_block: the persistent kernel's workgroup size and lanes
+ * per atom, or the graph path's force and list kernel blocks */
 int mtmd_batch_info(const mtmd_batch* b, int32_t* n_systems, int32_t* n_atoms, int32_t* force_block,
                     int32_t* list_block);
 

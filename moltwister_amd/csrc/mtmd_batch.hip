@@ -29,6 +29,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <cstdlib>
 #include <memory>
 #include <string>
 #include <vector>
@@ -427,6 +428,517 @@ __global__ void k_mb_fill_u32(unsigned int* a, unsigned int v, int n)
 }
 
 // ---------------------------------------------------------------------------
+// The persistent replica kernel: one workgroup per replica advances it through
+// whole steps, the replica's FP32 positions (in cell order) and the pair table
+// resident in LDS. A step of replica r (MDLoop.cpp:124-138): the chain and
+// scale at its start, half kick and drift (FP64, global state in atom order
+// through the replica's slot -> atom map), the displacement test over the
+// replica, when due the cell sort in LDS and the 16-bit skin lists (words are
+// LDS places), the pair forces from LDS with the back half kick, wrap and
+// force cap in the lane that reduced the force, then the chain and scale at
+// the step's end. Nothing is shared between replicas, so the workgroups never
+// wait for each other; one launch runs every step of a call.
+// ---------------------------------------------------------------------------
+#define MB_RUN_MAXN 4096      // atoms a replica may hold (64 KB of staged float4 positions)
+#define MB_RUN_MAXCELLS 1024  // cells of a replica's grid
+#define MB_RUN_TPB 1024       // largest workgroup
+
+struct MbRun {
+    const int* order;         // replica of workgroup b: large replicas first, then small ones, so the two
+                              // workgroups a CU holds together carry a large and a small replica
+    int* perm;                // [n] slot -> atom of each replica segment (slots in cell order)
+    int* perm2;               // [n] the new order of a rebuild
+    uint32_t* nl16;           // 16-bit rows: chunk q (8 words) of slot a0 + s at uint4 q * n + a0 + s
+    int* ncount16;            // [n] words in each slot's row
+    int ncap16;               // words a row holds (a multiple of 8)
+    int maxn, maxcells;       // staging capacities of the launch (largest replica)
+};
+
+__device__ __forceinline__ double mb_block_sum(double v, double* sh)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) sh[w] = v;
+    __syncthreads();
+    double t = 0.0;
+    for (int i = 0; i < nw; i++) t += sh[i];      // fixed order: the same bits in every launch
+    return t;
+}
+
+__device__ __forceinline__ float mb_block_max(float v, float* sh)
+{
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+    const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) sh[w] = v;
+    __syncthreads();
+    float t = 0.0f;
+    for (int i = 0; i < nw; i++) t = fmaxf(t, sh[i]);
+    return t;
+}
+
+// a position folded into the replica's box (k_mb_cell_keys' formula); the
+// staged positions and the cells come from the same bits
+__device__ __forceinline__ double3 mb_fold(const MbRep& rp, const double4& x)
+{
+    return make_double3(x.x - rp.L[0] * floor(x.x / rp.L[0] + 0.5), x.y - rp.L[1] * floor(x.y / rp.L[1] + 0.5),
+                        x.z - rp.L[2] * floor(x.z / rp.L[2] + 0.5));
+}
+
+__device__ __forceinline__ int mb_cell_of(const MbRep& rp, const double3& f)
+{
+    const double us[3] = {f.x, f.y, f.z};
+    int i3[3];
+    for (int a = 0; a < 3; a++) {
+        const int ic = (int)floor((us[a] + rp.L[a] / 2.0) * (double)rp.nc[a] / rp.L[a]);
+        i3[a] = ic < 0 ? 0 : (ic >= rp.nc[a] ? rp.nc[a] - 1 : ic);
+    }
+    return i3[0] + rp.nc[0] * (i3[1] + rp.nc[1] * i3[2]);
+}
+
+// the candidates of places [e0, e1) for the slot at place s (position px, py,
+// pz with the run's periodic shift applied), MB_GROUP at a time: the distances
+// first, as a mask of hits, then the hits in place order into the slot's
+// 16-bit row (chunks of eight words at row4[q * ntot]). MI: minimum image per
+// pair, for replicas with fewer than three cells on an axis. The last group's
+// loads may pass e1 by up to MB_GROUP - 1 places (the staging is padded by
+// 16); those are masked off.
+#ifndef MB_GROUP
+#define MB_GROUP 8
+#endif
+template <bool MI>
+__device__ __forceinline__ void mb_scan_run(const float4* __restrict__ spos, int e0, int e1, int s, float px,
+                                            float py, float pz, const MbRep& rp, float rl2, int cap,
+                                            uint4* __restrict__ row4, int ntot, int& cnt, uint32_t& w0,
+                                            uint32_t& w1, uint32_t& w2, uint32_t& w3)
+{
+    for (int e = e0; e < e1; e += MB_GROUP) {
+        uint32_t m = 0;
+#pragma unroll
+        for (int i = 0; i < MB_GROUP; i++) {
+            const float4 pa = spos[e + i];
+            float dx = px - pa.x, dy = py - pa.y, dz = pz - pa.z;
+            if (MI) {
+                dx = mb_min_image(dx, rp.Lf[0], rp.iLf[0]);
+                dy = mb_min_image(dy, rp.Lf[1], rp.iLf[1]);
+                dz = mb_min_image(dz, rp.Lf[2], rp.iLf[2]);
+            }
+            m |= (fmaf(dz, dz, fmaf(dy, dy, dx * dx)) < rl2 ? 1u : 0u) << i;
+        }
+        const int ne = e1 - e;
+        if (ne < MB_GROUP) m &= (1u << ne) - 1u;
+        const unsigned so = (unsigned)(s - e);
+        if (so < (unsigned)MB_GROUP) m &= ~(1u << so);
+        while (m) {
+            const int i = __builtin_ctz(m);
+            m &= m - 1u;
+            mt_neigh_push<true>((uint32_t)(e + i), w0, w1, w2, w3);
+            if ((cnt & 7) == 7 && cnt < cap) row4[(size_t)(cnt >> 3) * ntot] = make_uint4(w0, w1, w2, w3);
+            cnt++;
+        }
+    }
+}
+
+// ascending sort of one cell's entries a[0, cnt) (places of the previous
+// order) by their staged x, ties by atom index (perm[place]), by one
+// wavefront: two entries a lane ranked in registers (cnt <= 128), lane 0 by
+// insertion beyond that
+__device__ inline void mb_sort_cell_x(int* __restrict__ a, int cnt, const float4* __restrict__ spos,
+                                      const int* __restrict__ perm)
+{
+    const int lane = threadIdx.x & 63;
+    auto key = [&](int q) -> unsigned long long {
+        const unsigned b = __float_as_uint(spos[q].x);
+        const unsigned o = (b & 0x80000000u) ? ~b : (b | 0x80000000u);   // float order as unsigned order
+        return ((unsigned long long)o << 32) | (unsigned)perm[q];
+    };
+    if (cnt <= 128) {
+        const int v0 = lane < cnt ? a[lane] : 0, v1 = lane + 64 < cnt ? a[lane + 64] : 0;
+        const unsigned long long k0 = lane < cnt ? key(v0) : ~0ull, k1 = lane + 64 < cnt ? key(v1) : ~0ull;
+        int q0 = 0, q1 = 0;
+        const int n0 = min(cnt, 64);
+        for (int j = 0; j < n0; j++) {
+            const unsigned long long kj = ((unsigned long long)(unsigned)__shfl((int)(k0 >> 32), j) << 32) |
+                                          (unsigned)__shfl((int)k0, j);
+            q0 += kj < k0 ? 1 : 0;
+            q1 += kj < k1 ? 1 : 0;
+        }
+        for (int j = 0; j < cnt - 64; j++) {
+            const unsigned long long kj = ((unsigned long long)(unsigned)__shfl((int)(k1 >> 32), j) << 32) |
+                                          (unsigned)__shfl((int)k1, j);
+            q0 += kj < k0 ? 1 : 0;
+            q1 += kj < k1 ? 1 : 0;
+        }
+        __builtin_amdgcn_wave_barrier();              // (one wavefront: every read of a[] above precedes the writes)
+        if (lane < cnt) a[q0] = v0;
+        if (lane + 64 < cnt) a[q1] = v1;
+    } else if (lane == 0) {
+        for (int i = 1; i < cnt; i++) {
+            const int v = a[i];
+            const unsigned long long kv = key(v);
+            int j = i - 1;
+            while (j >= 0 && key(a[j]) > kv) { a[j + 1] = a[j]; j--; }
+            a[j + 1] = v;
+        }
+    }
+}
+
+// first place of [e0, e1) whose staged x is at least t (e1 if none); the
+// places are in ascending x
+__device__ __forceinline__ int mb_lower_x(const float4* __restrict__ spos, int e0, int e1, float t)
+{
+    int len = e1 - e0;
+    while (len > 0) {
+        const int h = len >> 1;
+        if (spos[e0 + h].x < t) { e0 += h + 1; len -= h + 1; } else len = h;
+    }
+    return e0;
+}
+
+// a replica's chain (fast arithmetic; chains of 1..8, which the batch takes)
+__device__ __forceinline__ double mb_nh_fast(double* nh, int M, const MbPar& p, const MbRep& rp, double KE2)
+{
+    double *eta = nh, *peta = nh + M;
+    const double* Q = nh + 2 * M;
+    switch (M) {
+    case 1: return mt_nh_propagate_fast<1>(eta, peta, Q, p.nresp, rp.T, p.dt, rp.g, p.w, KE2);
+    case 2: return mt_nh_propagate_fast<2>(eta, peta, Q, p.nresp, rp.T, p.dt, rp.g, p.w, KE2);
+    case 3: return mt_nh_propagate_fast<3>(eta, peta, Q, p.nresp, rp.T, p.dt, rp.g, p.w, KE2);
+    case 4: return mt_nh_propagate_fast<4>(eta, peta, Q, p.nresp, rp.T, p.dt, rp.g, p.w, KE2);
+    case 5: return mt_nh_propagate_fast<5>(eta, peta, Q, p.nresp, rp.T, p.dt, rp.g, p.w, KE2);
+    case 6: return mt_nh_propagate_fast<6>(eta, peta, Q, p.nresp, rp.T, p.dt, rp.g, p.w, KE2);
+    case 7: return mt_nh_propagate_fast<7>(eta, peta, Q, p.nresp, rp.T, p.dt, rp.g, p.w, KE2);
+    case 8: return mt_nh_propagate_fast<8>(eta, peta, Q, p.nresp, rp.T, p.dt, rp.g, p.w, KE2);
+    default: return 1.0;
+    }
+}
+
+// mode 0: prime (list build, forces with energies, no motion); 1: nsteps MD
+// steps, energies in the last when want_e_last; 2: forces only, 3: list build
+// only (timing)
+template <bool NVT, int G>
+__global__ void __launch_bounds__(MB_RUN_TPB) k_mb_run(MbDev d, MbPar p, MbRun u, int nsteps, int mode,
+                                                       int want_e_last)
+{
+    extern __shared__ float4 smem4[];
+    __shared__ double shd[MB_RUN_TPB / 64];
+    __shared__ float shf[MB_RUN_TPB / 64];
+    __shared__ double s_scale[2];                     // [0] end scale of the last step, [1] start scale
+    const int T = d.T;
+    float4* spos = smem4;                             // [maxn + 16] staged positions; .w the type's row offset in bytes
+    float2* stab = reinterpret_cast<float2*>(spos + u.maxn + 16);   // [T T MT_NPTS] (a_f, b_f)
+    int* cst = reinterpret_cast<int*>(stab + (size_t)T * T * MT_NPTS);   // [maxcells + 1] cell starts
+    int* cfill = cst + u.maxcells + 1;                // [maxcells]
+    const int tid = threadIdx.x, nt = blockDim.x, wave = tid >> 6, nwav = nt >> 6;
+    const int r = u.order[blockIdx.x];
+    const MbRep rp = d.rep[r];
+    const int n = rp.n, a0 = rp.atom0, ntot = d.n;
+    const int M = d.M, stride = 3 * M + 2;
+    double* nh = d.nh + (size_t)r * stride;
+    for (int i = tid; i < T * T * MT_NPTS; i += nt) stab[i] = d.nbcoef2[i];
+    const float Lx = rp.Lf[0], Ly = rp.Lf[1], Lz = rp.Lf[2];
+    const float iLx = rp.iLf[0], iLy = rp.iLf[1], iLz = rp.iLf[2];
+    const double dt_2 = p.dt / 2.0;
+    const float x0 = p.x0, invd = p.inv_delta;
+    uint4* __restrict__ nl4 = reinterpret_cast<uint4*>(u.nl16);
+
+    auto stage = [&]() {
+        for (int s = tid; s < n; s += nt) {
+            const int k = u.perm[a0 + s];
+            const double3 x = mb_fold(rp, d.pos[k]);
+            const int ty = __float_as_int(d.posf[k].w);
+            spos[s] = make_float4((float)x.x, (float)x.y, (float)x.z, __int_as_float(ty * MT_NPTS * (int)sizeof(float2)));
+        }
+    };
+
+    // cell sort of the replica (counting sort in LDS, then ascending atom
+    // order inside each cell) and its 16-bit skin lists
+    auto rebuild = [&]() {
+        const int ncell = rp.ncell;
+        for (int c = tid; c < ncell; c += nt) { cst[c] = 0; cfill[c] = 0; }
+        __syncthreads();
+        for (int s = tid; s < n; s += nt) {
+            const double3 f = mb_fold(rp, d.pos[u.perm[a0 + s]]);
+            const int key = mb_cell_of(rp, f);
+            spos[s].x = (float)f.x;                   // the sort's key (restaged below)
+            spos[s].w = __int_as_float(key);
+            atomicAdd(&cst[key], 1);
+        }
+        __syncthreads();
+        if (wave == 0) {
+            // exclusive scan of the counts by one wavefront: a run of cells a lane
+            const int lane = tid & 63, per = (ncell + 63) >> 6, c0 = lane * per;
+            int sum = 0;
+            for (int c = c0; c < min(c0 + per, ncell); c++) sum += cst[c];
+            int incl = sum;
+            for (int o = 1; o < 64; o <<= 1) {
+                const int v = __shfl_up(incl, o);
+                if (lane >= o) incl += v;
+            }
+            int run = incl - sum;
+            for (int c = c0; c < min(c0 + per, ncell); c++) {
+                const int v = cst[c];
+                cst[c] = run;
+                run += v;
+            }
+            if (lane == 63) cst[ncell] = incl;
+        }
+        __syncthreads();
+        for (int s = tid; s < n; s += nt) {
+            const int key = __float_as_int(spos[s].w);
+            u.perm2[a0 + cst[key] + atomicAdd(&cfill[key], 1)] = s;
+        }
+        __syncthreads();
+        // each cell in ascending x (ties by atom index), a wavefront per cell
+        for (int c = wave; c < ncell; c += nwav)
+            mb_sort_cell_x(u.perm2 + a0 + cst[c], cst[c + 1] - cst[c], spos, u.perm + a0);
+        __syncthreads();
+        for (int s = tid; s < n; s += nt) u.perm2[a0 + s] = u.perm[a0 + u.perm2[a0 + s]];   // places -> atoms
+        __syncthreads();
+        for (int s = tid; s < n; s += nt) {
+            const int k = u.perm2[a0 + s];
+            u.perm[a0 + s] = k;
+            d.posref[k] = d.pos[k];
+        }
+        __syncthreads();
+        stage();
+        __syncthreads();
+        // lists: every distinct cell within one cell of the slot's own (per
+        // axis three offsets with three or more cells, else every cell once),
+        // in place order within each row of cells; with three or more cells
+        // on every axis only the rows and x windows that can hold a pair
+        // within the list radius are tested (the same lists)
+        const int cap = u.ncap16;
+        for (int s = tid; s < n; s += nt) {
+            int lo = 0, hi = ncell;                   // the cell holding place s
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (cst[mid] <= s) lo = mid; else hi = mid;
+            }
+            const int hx = lo % rp.nc[0], hy = (lo / rp.nc[0]) % rp.nc[1], hz = lo / (rp.nc[0] * rp.nc[1]);
+            int olo[3], ohi[3];
+            for (int a = 0; a < 3; a++) {
+                if (rp.nc[a] >= 3) { olo[a] = -1; ohi[a] = 1; }
+                else { olo[a] = 0; ohi[a] = rp.nc[a] - 1; }
+            }
+            const float4 pk = spos[s];
+            int cnt = 0;
+            uint32_t r0 = 0, r1 = 0, r2 = 0, r3 = 0;
+            const int nc0 = rp.nc[0], nc1 = rp.nc[1], nc2 = rp.nc[2];
+            const bool mi = nc0 < 3 || nc1 < 3 || nc2 < 3;
+            uint4* row4 = nl4 + a0 + s;
+            const float rl2 = p.rlist2, wy = Ly / (float)nc1, wz = Lz / (float)nc2;
+            for (int oz = olo[2]; oz <= ohi[2]; oz++) {
+                // neighbour cells across the box's faces: the slot's position
+                // moved by a box length instead of a minimum image per pair
+                int cz = nc2 >= 3 ? hz + oz : oz;
+                float pz = pk.z;
+                if (cz < 0) { cz += nc2; pz += Lz; } else if (cz >= nc2) { cz -= nc2; pz -= Lz; }
+                // distance from the slot to the cells' z range, less a margin
+                // for the rounding of the edges (so never more than the truth)
+                const float zlo = fmaf((float)cz, wz, -0.5f * Lz);
+                const float dz = fmaxf(fmaxf(zlo - pz, pz - (zlo + wz)) - 1e-3f, 0.f);
+                for (int oy = olo[1]; oy <= ohi[1]; oy++) {
+                    int cy = nc1 >= 3 ? hy + oy : oy;
+                    float py = pk.y;
+                    if (cy < 0) { cy += nc1; py += Ly; } else if (cy >= nc1) { cy -= nc1; py -= Ly; }
+                    const float ylo = fmaf((float)cy, wy, -0.5f * Ly);
+                    const float dy = fmaxf(fmaxf(ylo - py, py - (ylo + wy)) - 1e-3f, 0.f);
+                    const float r2yz = fmaf(dy, dy, dz * dz);
+                    // no atom of the row within the list radius
+                    if (!mi && r2yz >= rl2) continue;
+                    // half-width of the x window the row's atoms can fall in
+                    const float W = sqrtf(fmaxf(rl2 - r2yz, 0.f)) + 1e-3f;
+                    // the row's cells within one of hx are contiguous places:
+                    // one run, or two where the row wraps
+                    const int row = nc0 * (cy + nc1 * cz);
+                    int ea0, ea1, eb0 = 0, eb1 = 0;
+                    float pxb = pk.x;
+                    if (nc0 < 3) {
+                        ea0 = cst[row]; ea1 = cst[row + nc0];
+                    } else if (hx == 0) {
+                        ea0 = cst[row]; ea1 = cst[row + 2];
+                        eb0 = cst[row + nc0 - 1]; eb1 = cst[row + nc0]; pxb = pk.x + Lx;
+                    } else if (hx == nc0 - 1) {
+                        ea0 = cst[row + hx - 1]; ea1 = cst[row + nc0];
+                        eb0 = cst[row]; eb1 = cst[row + 1]; pxb = pk.x - Lx;
+                    } else {
+                        ea0 = cst[row + hx - 1]; ea1 = cst[row + hx + 2];
+                    }
+                    for (int part = 0; part < 2; part++) {
+                        int e0 = part ? eb0 : ea0, e1 = part ? eb1 : ea1;
+                        const float px = part ? pxb : pk.x;
+                        if (mi) {
+                            mb_scan_run<true>(spos, e0, e1, s, px, py, pz, rp, rl2, cap, row4, ntot, cnt, r0, r1, r2, r3);
+                        } else {
+                            // the run is in ascending x: only its x window
+                            e0 = mb_lower_x(spos, e0, e1, px - W);
+                            e1 = mb_lower_x(spos, e0, e1, px + W);
+                            mb_scan_run<false>(spos, e0, e1, s, px, py, pz, rp, rl2, cap, row4, ntot, cnt, r0, r1, r2, r3);
+                        }
+                    }
+                }
+            }
+            const int slot = cnt & 7;
+            if (slot && cnt < cap) {
+                for (int i = slot; i < 8; i++) mt_neigh_push<true>((uint32_t)s, r0, r1, r2, r3);   // pad: the atom itself
+                nl4[(size_t)(cnt >> 3) * ntot + a0 + s] = make_uint4(r0, r1, r2, r3);
+            }
+            u.ncount16[a0 + s] = cnt;
+            if (cnt > cap) { atomicOr(&d.flags[0], 1u); atomicMax(d.nreq, cnt); }
+        }
+        __syncthreads();
+    };
+
+    // pair forces of every slot (G lanes a slot): FP32 minimum image on the
+    // staged positions, the table's segment (its last one, zero, beyond rc or
+    // below the first point), eight pairs of a chunk summed in FP32 and the
+    // chunk sums in FP64. Then, in the lane that holds the force: the cap, the
+    // half kick and the wrap (integrate), or the force alone (mode 0 and 2).
+    // Returns this thread's part of sum p.p/m after the kick.
+    auto forces = [&](bool want_e, bool integrate) -> double {
+        const int gl = tid % G;
+        double ke = 0.0;
+        for (int v = tid / G; v < n; v += nt / G) {   // the G lanes of a slot share v
+            const float4 pk = spos[v];
+            const int tk = __float_as_int(pk.w);      // home row: its type's offset times T
+            const char* row = reinterpret_cast<const char*>(stab) + (size_t)tk * T;
+            const int nch = (min(u.ncount16[a0 + v], u.ncap16) + 7) >> 3;
+            double Fx = 0.0, Fy = 0.0, Fz = 0.0;
+            float U = 0.0f;
+            for (int q = gl; q < nch; q += G) {
+                const uint4 w = nl4[(size_t)q * ntot + a0 + v];
+                const int li[8] = {(int)(w.x & 0xffffu), (int)(w.x >> 16), (int)(w.y & 0xffffu), (int)(w.y >> 16),
+                                   (int)(w.z & 0xffffu), (int)(w.z >> 16), (int)(w.w & 0xffffu), (int)(w.w >> 16)};
+                float cx = 0.f, cy = 0.f, cz = 0.f;
+#pragma unroll
+                for (int i = 0; i < 8; i++) {
+                    const float4 a = spos[li[i]];
+                    const float dx = mb_min_image(pk.x - a.x, Lx, iLx);
+                    const float dy = mb_min_image(pk.y - a.y, Ly, iLy);
+                    const float dz = mb_min_image(pk.z - a.z, Lz, iLz);
+                    const float r2 = mt_lean_r2(dx, dy, dz);
+                    int seg;
+                    const float rinv = mt_lean_rinv(r2, x0, invd, seg);
+                    const int off = __float_as_int(a.w) + seg * (int)sizeof(float2);
+                    const float2 c = *reinterpret_cast<const float2*>(row + off);
+                    const float fr = fmaf(c.y, rinv, c.x);
+                    if (want_e) {
+                        const float4 c4 = d.nbcoef[((size_t)tk * T + __float_as_int(a.w)) / sizeof(float2) + seg];
+                        U += 0.5f * fmaf(c4.z, r2 * rinv, c4.w);
+                    }
+                    cx = fmaf(dx, fr, cx);
+                    cy = fmaf(dy, fr, cy);
+                    cz = fmaf(dz, fr, cz);
+                }
+                Fx += (double)cx;
+                Fy += (double)cy;
+                Fz += (double)cz;
+            }
+            mt_lane_sum<G>(Fx, Fy, Fz, U);
+            if (gl == 0) {
+                const int k = u.perm[a0 + v];
+                double4 F = make_double4(Fx, Fy, Fz, 0.0);
+                if (p.Fcut > 0) {                     // cutForces (VelVerlet.cpp:81,96-122)
+                    if (fabs(F.x) > p.Fcut) F.x = ((F.x >= 0.0) ? 1.0 : -1.0) * p.Fcut;
+                    if (fabs(F.y) > p.Fcut) F.y = ((F.y >= 0.0) ? 1.0 : -1.0) * p.Fcut;
+                    if (fabs(F.z) > p.Fcut) F.z = ((F.z >= 0.0) ? 1.0 : -1.0) * p.Fcut;
+                }
+                d.frc[k] = F;
+                if (want_e) d.epot[k] = U;
+                if (integrate) {
+                    double4 pm = d.mom[k], x = d.pos[k];
+                    pm.x += F.x * dt_2; pm.y += F.y * dt_2; pm.z += F.z * dt_2;
+                    double* xs[3] = {&x.x, &x.y, &x.z};
+                    for (int a = 0; a < 3; a++) {     // pbcWrap into the replica's box
+                        const double h = rp.L[a] / 2.0;
+                        if (*xs[a] >= h || *xs[a] < -h) *xs[a] -= rp.L[a] * floor(*xs[a] / rp.L[a] + 0.5);
+                    }
+                    d.mom[k] = pm;
+                    d.pos[k] = x;
+                    ke += ((pm.x * pm.x) + (pm.y * pm.y) + (pm.z * pm.z)) / pm.w;
+                }
+            }
+        }
+        return ke;
+    };
+
+    stage();
+    __syncthreads();
+    if (mode != 1) {
+        if (mode == 0 || mode == 3) rebuild();
+        if (mode != 3) forces(mode == 0, false);
+        return;
+    }
+    for (int t = 0; t < nsteps; t++) {
+        const bool last = (t == nsteps - 1);
+        // the step's start: chain on the kinetic energy after the previous
+        // step's end scale (kept analytically, KE2 S^2), one thread
+        if (NVT) {
+            if (tid == 0) {
+                s_scale[0] = nh[3 * M];               // end scale of the previous step, not yet applied
+                const double Sf = mb_nh_fast(nh, M, p, rp, nh[3 * M + 1]);
+                nh[3 * M + 1] *= Sf * Sf;
+                nh[3 * M] = 1.0;
+                s_scale[1] = Sf;
+            }
+            __syncthreads();
+        }
+        float d2 = 0.0f;
+        for (int s = tid; s < n; s += nt) {
+            const int k = u.perm[a0 + s];
+            double4 pm = d.mom[k], x = d.pos[k];
+            const double4 F = d.frc[k];
+            if (NVT) {
+                const double Sb = s_scale[0], Sf = s_scale[1];
+                pm.x *= Sb; pm.y *= Sb; pm.z *= Sb;
+                pm.x *= Sf; pm.y *= Sf; pm.z *= Sf;
+            }
+            pm.x += F.x * dt_2; pm.y += F.y * dt_2; pm.z += F.z * dt_2;
+            const double c = p.dt / pm.w;
+            x.x += pm.x * c; x.y += pm.y * c; x.z += pm.z * c;
+            d.mom[k] = pm;
+            d.pos[k] = x;
+            spos[s].x = (float)x.x; spos[s].y = (float)x.y; spos[s].z = (float)x.z;
+            const double4 q = d.posref[k];
+            double dx = x.x - q.x, dy = x.y - q.y, dz = x.z - q.z;
+            dx -= rp.L[0] * rint(dx / rp.L[0]);
+            dy -= rp.L[1] * rint(dy / rp.L[1]);
+            dz -= rp.L[2] * rint(dz / rp.L[2]);
+            const float dd = (float)(dx * dx + dy * dy + dz * dz);
+            if (isnan(dd)) atomicOr(&d.flags[1], 1u);  // a non-finite position (NaN or inf)
+            d2 = fmaxf(d2, dd);
+        }
+        if (mb_block_max(d2, shf) > p.half_skin2) rebuild();
+        __syncthreads();
+        const double ke = forces(last && want_e_last, true);
+        if (NVT) {
+            // the step's end: chain on sum p.p/m after the kick; its scale is
+            // applied at the next step's start, or below after the last step
+            const double KE2 = mb_block_sum(ke, shd);
+            if (tid == 0) {
+                const double Sb = mb_nh_fast(nh, M, p, rp, KE2);
+                nh[3 * M] = Sb;
+                nh[3 * M + 1] = KE2 * Sb * Sb;
+            }
+        }
+        __syncthreads();
+    }
+    if (NVT) {
+        const double Sb = nh[3 * M];
+        for (int s = tid; s < n; s += nt) {
+            const int k = u.perm[a0 + s];
+            double4 pm = d.mom[k];
+            pm.x *= Sb; pm.y *= Sb; pm.z *= Sb;
+            d.mom[k] = pm;
+        }
+        __syncthreads();
+        if (tid == 0) nh[3 * M] = 1.0;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
 namespace {
@@ -458,6 +970,13 @@ struct mtmd_batch {
     std::vector<MbRep> rep;
     int nrep = 0, n = 0;
     int force_block = 256, list_block = 128;
+    // the persistent replica kernel (k_mb_run), used when every replica fits
+    // its staging: workgroup size and lanes per atom (autotuned)
+    bool run = false;
+    MbRun u{};
+    int run_block = 512, run_g = 2;
+    int* order_batch = nullptr;   // workgroup -> replica of a whole-batch launch
+    int* order_ident = nullptr;   // replica i at i (one-replica launches)
     bool have_forces = false, ke_valid = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
@@ -534,6 +1053,53 @@ static void mb_step(mtmd_batch* b, int r0, int r1, bool want_e, hipStream_t s)
     if (b->p.nvt) hipLaunchKernelGGL(k_mb_reduce, dim3(nr), dim3(MB_TPB), 0, s, d, r0, 0);
 }
 
+// dynamic LDS of a k_mb_run launch with `block` threads
+static size_t mb_run_lds(const mtmd_batch* b, int block)
+{
+    size_t bytes = (size_t)(b->u.maxn + 16) * sizeof(float4) + (size_t)b->d.T * b->d.T * MT_NPTS * sizeof(float2) +
+                   (size_t)(2 * b->u.maxcells + 1) * sizeof(int);
+    return (bytes + 15) & ~(size_t)15;
+}
+
+template <bool NVT, int G>
+static hipError_t mb_run_one(mtmd_batch* b, const MbRun& u, int grid, int nsteps, int mode, int want_e,
+                             hipStream_t s)
+{
+    const size_t lds = mb_run_lds(b, b->run_block);
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_mb_run<NVT, G>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_mb_run<NVT, G>), dim3(grid), dim3(b->run_block), lds, s, b->d, b->p, u, nsteps, mode,
+                       want_e);
+    return hipGetLastError();
+}
+
+// k_mb_run over the replicas order[0, grid)
+static hipError_t mb_run_launch(mtmd_batch* b, const int* order, int grid, int nsteps, int mode, int want_e,
+                                hipStream_t s)
+{
+    MbRun u = b->u;
+    u.order = order;
+    if (b->p.nvt) {
+        if (b->run_g == 1) return mb_run_one<true, 1>(b, u, grid, nsteps, mode, want_e, s);
+        if (b->run_g == 2) return mb_run_one<true, 2>(b, u, grid, nsteps, mode, want_e, s);
+        return mb_run_one<true, 4>(b, u, grid, nsteps, mode, want_e, s);
+    }
+    if (b->run_g == 1) return mb_run_one<false, 1>(b, u, grid, nsteps, mode, want_e, s);
+    if (b->run_g == 2) return mb_run_one<false, 2>(b, u, grid, nsteps, mode, want_e, s);
+    return mb_run_one<false, 4>(b, u, grid, nsteps, mode, want_e, s);
+}
+
+// the chain's kinetic energy slot from the reduced sums, no scale pending
+__global__ void k_mb_ke_slot(MbDev d)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= d.nrep || d.M == 0) return;
+    double* nh = d.nh + (size_t)r * (3 * d.M + 2);
+    nh[3 * d.M] = 1.0;
+    nh[3 * d.M + 1] = d.red[3 * r];
+}
+
 static int mb_check(mtmd_batch* b)
 {
     unsigned int fl[2];
@@ -543,7 +1109,8 @@ static int mb_check(mtmd_batch* b)
     if (fl[0]) {
         int need = 0;
         MB_HIP(b, hipMemcpy(&need, b->d.nreq, sizeof(int), hipMemcpyDeviceToHost));
-        return b->fail(MTMD_ENEIGH, "neighbour list overflow (capacity %d, needed %d)", b->d.ncap, need);
+        return b->fail(MTMD_ENEIGH, "neighbour list overflow (capacity %d, needed %d)",
+                       b->run ? b->u.ncap16 : b->d.ncap, need);
     }
     return MTMD_OK;
 }
@@ -552,6 +1119,16 @@ static int mb_check(mtmd_batch* b)
 static int mb_prime(mtmd_batch* b)
 {
     MbDev& d = b->d;
+    if (b->run) {
+        MB_HIP(b, mb_run_launch(b, b->order_batch, b->nrep, 0, 0, 1, b->stream));
+        hipLaunchKernelGGL(k_mb_reduce, dim3(b->nrep), dim3(MB_TPB), 0, b->stream, d, 0, 0);
+        hipLaunchKernelGGL(k_mb_ke_slot, dim3(mb_blk(b->nrep, 64)), dim3(64), 0, b->stream, d);
+        MB_HIP(b, hipGetLastError());
+        int rc = mb_check(b);
+        if (rc) return rc;
+        b->have_forces = true;
+        return MTMD_OK;
+    }
     hipLaunchKernelGGL(k_mb_fill_u32, dim3(mb_blk(b->nrep, 64)), dim3(64), 0, b->stream, d.rflag, 1u, b->nrep);
     mb_rebuild(b, 0, b->nrep, b->stream);
     mb_forces(b, 0, b->nrep, true, b->stream);
@@ -761,6 +1338,40 @@ extern "C" int mtmd_batch_create(int n_systems, const mtmd_system* systems, cons
               hipMemset(d.frc, 0, natoms * sizeof(double4)) == hipSuccess &&
               hipMemset(d.epot, 0, natoms * sizeof(float)) == hipSuccess;
     if (!ok) return bad(MTMD_EHIP, "upload failed");
+    // the persistent replica kernel when every replica fits its staging
+    {
+        int maxn = 0, maxcells = 0;
+        for (const MbRep& rp : b->rep) { maxn = std::max(maxn, rp.n); maxcells = std::max(maxcells, rp.ncell); }
+        b->u.maxn = maxn;
+        b->u.maxcells = maxcells;
+        b->run = maxn <= MB_RUN_MAXN && maxcells <= MB_RUN_MAXCELLS && T <= 2 &&
+                 mb_run_lds(b, MB_RUN_TPB) <= 160 * 1024 && std::getenv("MTMD_BATCH_KERNELS") == nullptr;
+    }
+    if (b->run) {
+        MbRun& u = b->u;
+        u.ncap16 = ((int)std::ceil(nexp_max * 1.35) + 16 + 7) & ~7;
+        u.perm = b->mem.alloc<int>(natoms);
+        u.perm2 = b->mem.alloc<int>(natoms);
+        u.ncount16 = b->mem.alloc<int>(natoms);
+        u.nl16 = b->mem.alloc<uint32_t>((size_t)(u.ncap16 / 2) * natoms);
+        b->order_batch = b->mem.alloc<int>(n_systems);
+        b->order_ident = b->mem.alloc<int>(n_systems);
+        if (!u.perm || !u.perm2 || !u.ncount16 || !u.nl16 || !b->order_batch || !b->order_ident)
+            return bad(MTMD_ENOMEM, "device allocation failed");
+        std::vector<int> ident(natoms);
+        for (int i = 0; i < natoms; i++) ident[i] = i;
+        // whole-batch order: the larger half of the replicas by size, largest
+        // first, then the smaller half, smallest first; the two workgroups a CU
+        // runs together then carry about the average size between them
+        std::vector<int> bysize(n_systems), ord(n_systems), id(n_systems);
+        for (int i = 0; i < n_systems; i++) bysize[i] = id[i] = i;
+        std::stable_sort(bysize.begin(), bysize.end(), [&](int x, int y) { return b->rep[x].n > b->rep[y].n; });
+        const int half = (n_systems + 1) / 2;
+        for (int i = 0; i < n_systems; i++) ord[i] = i < half ? bysize[i] : bysize[n_systems - 1 - (i - half)];
+        ok = up(u.perm, ident.data(), natoms * sizeof(int)) && up(b->order_batch, ord.data(), n_systems * sizeof(int)) &&
+             up(b->order_ident, id.data(), n_systems * sizeof(int));
+        if (!ok) return bad(MTMD_EHIP, "upload failed");
+    }
     int rc0 = mb_prime(b);
     if (rc0) { g_batch_err = b->err; return rc0; }
     *out = guard.release();
@@ -775,6 +1386,12 @@ extern "C" int mtmd_batch_step(mtmd_batch* b, int32_t nsteps)
 {
     if (!b || nsteps < 0) return MTMD_EINVAL;
     if (nsteps == 0) return MTMD_OK;
+    if (b->run) {
+        // every step of every replica in one launch, energies in the last
+        MB_HIP(b, mb_run_launch(b, b->order_batch, b->nrep, nsteps, 1, 1, b->stream));
+        b->steps += nsteps;
+        return mb_check(b);
+    }
     if (!b->graph_exec) {
         MB_HIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
         mb_step(b, 0, b->nrep, false, b->stream);
@@ -795,6 +1412,14 @@ extern "C" int mtmd_batch_step(mtmd_batch* b, int32_t nsteps)
 extern "C" int mtmd_batch_step_separate(mtmd_batch* b, int32_t nsteps)
 {
     if (!b || nsteps < 0) return MTMD_EINVAL;
+    if (b->run) {
+        // one launch a replica a step, as n separate engines would run
+        for (int t = 0; t < nsteps; t++)
+            for (int r = 0; r < b->nrep; r++)
+                MB_HIP(b, mb_run_launch(b, b->order_ident + r, 1, 1, 1, t == nsteps - 1, b->stream));
+        b->steps += nsteps;
+        return mb_check(b);
+    }
     for (int t = 0; t < nsteps; t++)
         for (int r = 0; r < b->nrep; r++) mb_step(b, r, r + 1, t == nsteps - 1, b->stream);
     MB_HIP(b, hipGetLastError());
@@ -815,8 +1440,8 @@ extern "C" int mtmd_batch_info(const mtmd_batch* b, int32_t* n_systems, int32_t*
     if (!b) return MTMD_EINVAL;
     if (n_systems) *n_systems = b->nrep;
     if (n_atoms) *n_atoms = b->n;
-    if (force_block) *force_block = b->force_block;
-    if (list_block) *list_block = b->list_block;
+    if (force_block) *force_block = b->run ? b->run_block : b->force_block;
+    if (list_block) *list_block = b->run ? b->run_g : b->list_block;
     return MTMD_OK;
 }
 
@@ -885,6 +1510,49 @@ extern "C" int mtmd_batch_autotune(mtmd_batch* b, int reps, double* force_ms, do
     hipEvent_t e0, e1;
     MB_HIP(b, hipEventCreate(&e0));
     MB_HIP(b, hipEventCreate(&e1));
+    if (b->run) {
+        // the persistent kernel: workgroup size x lanes per atom, each timed on
+        // a force pass (mode 2) and a list build (mode 3), scored as one force
+        // pass plus a fifth of a build (lists are rebuilt every few steps);
+        // state and lists are unchanged by either
+        auto timed = [&](int mode, double& ms_out) -> int {
+            MB_HIP(b, mb_run_launch(b, b->order_batch, b->nrep, 0, mode, 0, b->stream));
+            MB_HIP(b, hipEventRecord(e0, b->stream));
+            for (int i = 0; i < reps; i++) MB_HIP(b, mb_run_launch(b, b->order_batch, b->nrep, 0, mode, 0, b->stream));
+            MB_HIP(b, hipEventRecord(e1, b->stream));
+            MB_HIP(b, hipEventSynchronize(e1));
+            float ms = 0.f;
+            MB_HIP(b, hipEventElapsedTime(&ms, e0, e1));
+            ms_out = ms / reps;
+            return MTMD_OK;
+        };
+        const int bcand[3] = {256, 512, 1024}, gcand[3] = {1, 2, 4};
+        double best = 1e30, bf = 0.0, bl = 0.0;
+        int bb = b->run_block, bg = b->run_g;
+        for (int bs : bcand)
+            for (int g : gcand) {
+                if (mb_run_lds(b, bs) > 160 * 1024) continue;
+                b->run_block = bs;
+                b->run_g = g;
+                double tf = 0.0, tb = 0.0;
+                int rc = timed(2, tf);
+                if (!rc) rc = timed(3, tb);
+                if (rc) return rc;
+                const double score = tf + tb / 5.0;
+                if (std::getenv("MTMD_BATCH_TUNE_LOG"))
+                    std::fprintf(stderr, "batch tune: block %d lanes %d force %.4f ms list %.4f ms\n", bs, g, tf, tb);
+                if (score < best) { best = score; bb = bs; bg = g; bf = tf; bl = tb; }
+            }
+        b->run_block = bb;
+        b->run_g = bg;
+        if (force_ms) *force_ms = bf;
+        if (list_ms) *list_ms = bl;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        // the timed passes ran without energies: refresh them for the thermo
+        MB_HIP(b, mb_run_launch(b, b->order_batch, b->nrep, 0, 0, 1, b->stream));
+        return mb_check(b);
+    }
     auto time_it = [&](auto&& launch, double& ms_out) -> int {
         launch();   // warm
         MB_HIP(b, hipEventRecord(e0, b->stream));

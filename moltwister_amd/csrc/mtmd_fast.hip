@@ -500,26 +500,6 @@ static_assert(MT_FT_CAP <= NB_CAP && MT_FT_CAP % 8 == 0, "16-bit blocks fit the 
 // slot of a list-word index (the index itself unless the words are in cell order)
 __device__ __forceinline__ int mt_id_slot(const MtDev& d, int id) { return d.wsorted ? d.cell_list[id] : id; }
 
-// the chunk being filled is a window of the last words written, r0..r3 (the
-// oldest lowest): a word enters at the top and the window moves down one
-// word, so after a chunk's last word the window is the chunk itself
-template <bool W16>
-__device__ __forceinline__ void mt_neigh_push(uint32_t w, uint32_t& r0, uint32_t& r1, uint32_t& r2, uint32_t& r3)
-{
-    if constexpr (W16) {
-        // eight 16-bit words: ((hi:lo) >> 16) moves each register down one word
-        r0 = __builtin_amdgcn_alignbit(r1, r0, 16);
-        r1 = __builtin_amdgcn_alignbit(r2, r1, 16);
-        r2 = __builtin_amdgcn_alignbit(r3, r2, 16);
-        r3 = __builtin_amdgcn_alignbit(w, r3, 16);
-    } else {
-        r0 = r1;
-        r1 = r2;
-        r2 = r3;
-        r3 = w;
-    }
-}
-
 // a candidate inside rlist that is not the atom itself. k: the home atom's
 // slot (its list column); e: the candidate's place in the staged cells.
 // 32-bit words: the candidate's word index | offset code, four to a chunk.
@@ -1258,19 +1238,6 @@ __device__ __forceinline__ void mt_mixed_atom(const MtDev& d, const MtParams& p,
     }
 }
 
-// sum over the G consecutive lanes that share an atom (butterfly, fixed
-// order, so every run gives the same bits); every lane ends with the total
-template <int G>
-__device__ __forceinline__ void mt_lane_sum(double& x, double& y, double& z, float& u)
-{
-#pragma unroll
-    for (int m = 1; m < G; m <<= 1) {
-        x += __shfl_xor(x, m);
-        y += __shfl_xor(y, m);
-        z += __shfl_xor(z, m);
-        u += __shfl_xor(u, m);
-    }
-}
 
 // MIXED with energies: coefficients (a_f, b_f, a_e, b_e) read from global
 // memory; G lanes per atom
@@ -1343,29 +1310,6 @@ __global__ void __launch_bounds__(512) k_fast_force_mixed_lds(MtDev d, MtParams 
 // workgroup whose staged cells hold more than FT_CAP atoms has 32-bit coded
 // words and gathers from global memory instead (per-offset shifts). G lanes
 // per home atom.
-
-// The tile passes without charges (LEAN) need no inside test: a pair at or
-// beyond rc, or closer than the table's first point, indexes segment
-// MT_NPTS - 1 of its row, whose force and energy coefficients are zero, so its
-// terms vanish by themselves. Only the band around rc, where FP32 and the FP64
-// reference predicate may disagree, is corrected afterwards: the band path
-// takes back these same terms when the FP64 test puts the pair outside. Both
-// places compute them with these helpers, the same operations in the same
-// order, and the force-only and energy passes give the same forces.
-__device__ __forceinline__ float mt_lean_r2(float dx, float dy, float dz)
-{
-    return fmaf(dz, dz, fmaf(dx, dx, dy * dy));
-}
-
-// 1/r (r^2 clamped away from zero: the pad word is the atom itself) and the
-// segment of the pair
-__device__ __forceinline__ float mt_lean_rinv(float r2, float x0, float invd, int& seg)
-{
-    const float rinv = __builtin_amdgcn_rsqf(fmaxf(r2, 1.0e-6f));
-    const int idx = (int)(fmaf(r2, rinv, -x0) * invd);
-    seg = (int)min((unsigned)idx, (unsigned)(MT_NPTS - 1));
-    return rinv;
-}
 
 // four pairs of a lane, without a branch on the common path: a pair outside
 // rc (or the atom itself) still reads a segment and its force is selected

@@ -470,6 +470,7 @@ int mtmd_batch_get_state(mtmd_batch* b, int32_t i, double* pos, double* mom);
 int mtmd_batch_get_forces(mtmd_batch* b, int32_t i, double* F, float* U);
 /* autotuner over the whole batch, `reps` launches a candidate; returns the
  * chosen shape's force-pass and list-build times. Persistent kernel: the
+ * cell shape (one or two cells per list radius along y and z) times the
  * workgroup size (256, 512, 1024) times the lanes per atom (1, 2, 4), scored
  * as a force pass plus a fifth of a list build. Graph path: the pair-force
  * kernel's block (64..512) and the list build's (64..256). */
@@ -478,6 +479,11 @@ int mtmd_batch_autotune(mtmd_batch* b, int reps, double* force_ms, double* list_
  * per atom, or the graph path's force and list kernel blocks */
 int mtmd_batch_info(const mtmd_batch* b, int32_t* n_systems, int32_t* n_atoms, int32_t* force_block,
                     int32_t* list_block);
+/* the shape in use: persistent kernel (1) or graph path (0), workgroup size,
+ * lanes per atom, and cells per list radius along y and z (1, or 2 where a
+ * replica's box holds five or more such cells on both axes and three cells
+ * along x) */
+int mtmd_batch_tuning(const mtmd_batch* b, int32_t* persistent, int32_t* block, int32_t* lanes, int32_t* cell_split);
 
 /* ---- force matching ----
  * No counterpart in the reference (it has no gradients). Loss

@@ -181,7 +181,7 @@ EXPORTED = [
     "mtmd_set_barostat", "mtmd_get_barostat", "mtmd_find_topology", "mtmd_topology_last_error",
     "mtmd_batch_create", "mtmd_batch_destroy", "mtmd_batch_last_error", "mtmd_batch_step",
     "mtmd_batch_step_separate", "mtmd_batch_synchronize", "mtmd_batch_get_thermo", "mtmd_batch_get_state",
-    "mtmd_batch_get_forces", "mtmd_batch_autotune", "mtmd_batch_info",
+    "mtmd_batch_get_forces", "mtmd_batch_autotune", "mtmd_batch_info", "mtmd_batch_tuning",
     "mtmd_fm_create", "mtmd_fm_destroy", "mtmd_fm_last_error", "mtmd_fm_num_params", "mtmd_fm_set_target",
     "mtmd_fm_target_from_params", "mtmd_fm_forces", "mtmd_fm_loss_grad",
 ]
@@ -248,6 +248,7 @@ def load_library(path: str | None = None) -> C.CDLL:
         "mtmd_batch_get_forces": (C.c_int, [vp, C.c_int32, _dp, C.POINTER(C.c_float)]),
         "mtmd_batch_autotune": (C.c_int, [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
         "mtmd_batch_info": (C.c_int, [vp, _ip, _ip, _ip, _ip]),
+        "mtmd_batch_tuning": (C.c_int, [vp, _ip, _ip, _ip, _ip]),
         "mtmd_fm_create": (C.c_int, [C.POINTER(MtmdSystem), C.c_double, C.c_int, _dp, _dp, C.POINTER(C.c_void_p)]),
         "mtmd_fm_destroy": (None, [vp]),
         "mtmd_fm_last_error": (C.c_char_p, [vp]),

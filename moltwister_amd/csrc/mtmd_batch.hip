@@ -46,6 +46,8 @@ struct MbRep {
     float Lf[3], iLf[3];    // FP32 widths and their inverses (minimum image)
     int nc[3];              // cell grid
     int cell0, ncell;       // first global cell, number of cells
+    int sk;                 // persistent kernel: cells in y and z per list radius (1 or 2), the
+                            // stencil's half-width on those axes
     int atom0, n;           // atom range
     double T;               // target temperature (reduced units)
     double g;               // 3 n degrees of freedom
@@ -499,19 +501,17 @@ __device__ __forceinline__ int mb_cell_of(const MbRep& rp, const double3& f)
 
 // the candidates of places [e0, e1) for the slot at place s (position px, py,
 // pz with the run's periodic shift applied), MB_GROUP at a time: the distances
-// first, as a mask of hits, then the hits in place order into the slot's
-// 16-bit row (chunks of eight words at row4[q * ntot]). MI: minimum image per
-// pair, for replicas with fewer than three cells on an axis. The last group's
-// loads may pass e1 by up to MB_GROUP - 1 places (the staging is padded by
-// 16); those are masked off.
+// first, as a mask of hits, then the hits in place order into the slot's ring
+// of 16-bit list words, a completed chunk copied out after each group. MI:
+// minimum image per pair, for replicas with fewer than three cells on an axis.
+// The last group's loads may pass e1 by up to MB_GROUP - 1 places (the staging
+// is padded by 16); those are masked off.
 #ifndef MB_GROUP
 #define MB_GROUP 8
 #endif
 template <bool MI>
 __device__ __forceinline__ void mb_scan_run(const float4* __restrict__ spos, int e0, int e1, int s, float px,
-                                            float py, float pz, const MbRep& rp, float rl2, int cap,
-                                            uint4* __restrict__ row4, int ntot, int& cnt, uint32_t& w0,
-                                            uint32_t& w1, uint32_t& w2, uint32_t& w3)
+                                            float py, float pz, const MbRep& rp, float rl2, MtRing16& g)
 {
     for (int e = e0; e < e1; e += MB_GROUP) {
         uint32_t m = 0;
@@ -531,12 +531,10 @@ __device__ __forceinline__ void mb_scan_run(const float4* __restrict__ spos, int
         const unsigned so = (unsigned)(s - e);
         if (so < (unsigned)MB_GROUP) m &= ~(1u << so);
         while (m) {
-            const int i = __builtin_ctz(m);
+            mt_ring_put(g, e + __builtin_ctz(m));
             m &= m - 1u;
-            mt_neigh_push<true>((uint32_t)(e + i), w0, w1, w2, w3);
-            if ((cnt & 7) == 7 && cnt < cap) row4[(size_t)(cnt >> 3) * ntot] = make_uint4(w0, w1, w2, w3);
-            cnt++;
         }
+        mt_ring_flush(g);
     }
 }
 
@@ -630,6 +628,7 @@ __global__ void __launch_bounds__(MB_RUN_TPB) k_mb_run(MbDev d, MbPar p, MbRun u
     float2* stab = reinterpret_cast<float2*>(spos + u.maxn + 16);   // [T T MT_NPTS] (a_f, b_f)
     int* cst = reinterpret_cast<int*>(stab + (size_t)T * T * MT_NPTS);   // [maxcells + 1] cell starts
     int* cfill = cst + u.maxcells + 1;                // [maxcells]
+    uint32_t* sring = reinterpret_cast<uint32_t*>(cfill + u.maxcells);   // [block * MT_RING_DW] list-word rings
     const int tid = threadIdx.x, nt = blockDim.x, wave = tid >> 6, nwav = nt >> 6;
     const int r = u.order[blockIdx.x];
     const MbRep rp = d.rep[r];
@@ -719,15 +718,14 @@ __global__ void __launch_bounds__(MB_RUN_TPB) k_mb_run(MbDev d, MbPar p, MbRun u
             const int hx = lo % rp.nc[0], hy = (lo / rp.nc[0]) % rp.nc[1], hz = lo / (rp.nc[0] * rp.nc[1]);
             int olo[3], ohi[3];
             for (int a = 0; a < 3; a++) {
-                if (rp.nc[a] >= 3) { olo[a] = -1; ohi[a] = 1; }
+                const int h = a == 0 ? 1 : rp.sk;    // (two per list radius only with 5 or more cells)
+                if (rp.nc[a] >= 3) { olo[a] = -h; ohi[a] = h; }
                 else { olo[a] = 0; ohi[a] = rp.nc[a] - 1; }
             }
             const float4 pk = spos[s];
-            int cnt = 0;
-            uint32_t r0 = 0, r1 = 0, r2 = 0, r3 = 0;
+            MtRing16 g{sring + (size_t)tid * MT_RING_DW, nl4 + a0 + s, (size_t)ntot, cap >> 3, 0, 0};
             const int nc0 = rp.nc[0], nc1 = rp.nc[1], nc2 = rp.nc[2];
             const bool mi = nc0 < 3 || nc1 < 3 || nc2 < 3;
-            uint4* row4 = nl4 + a0 + s;
             const float rl2 = p.rlist2, wy = Ly / (float)nc1, wz = Lz / (float)nc2;
             for (int oz = olo[2]; oz <= ohi[2]; oz++) {
                 // neighbour cells across the box's faces: the slot's position
@@ -770,20 +768,20 @@ __global__ void __launch_bounds__(MB_RUN_TPB) k_mb_run(MbDev d, MbPar p, MbRun u
                         int e0 = part ? eb0 : ea0, e1 = part ? eb1 : ea1;
                         const float px = part ? pxb : pk.x;
                         if (mi) {
-                            mb_scan_run<true>(spos, e0, e1, s, px, py, pz, rp, rl2, cap, row4, ntot, cnt, r0, r1, r2, r3);
+                            mb_scan_run<true>(spos, e0, e1, s, px, py, pz, rp, rl2, g);
                         } else {
                             // the run is in ascending x: only its x window
                             e0 = mb_lower_x(spos, e0, e1, px - W);
                             e1 = mb_lower_x(spos, e0, e1, px + W);
-                            mb_scan_run<false>(spos, e0, e1, s, px, py, pz, rp, rl2, cap, row4, ntot, cnt, r0, r1, r2, r3);
+                            mb_scan_run<false>(spos, e0, e1, s, px, py, pz, rp, rl2, g);
                         }
                     }
                 }
             }
-            const int slot = cnt & 7;
-            if (slot && cnt < cap) {
-                for (int i = slot; i < 8; i++) mt_neigh_push<true>((uint32_t)s, r0, r1, r2, r3);   // pad: the atom itself
-                nl4[(size_t)(cnt >> 3) * ntot + a0 + s] = make_uint4(r0, r1, r2, r3);
+            const int cnt = g.cnt;
+            if (cnt & 7) {
+                for (int i = cnt & 7; i < 8; i++) mt_ring_put(g, s);   // pad: the atom itself (r = 0, no force)
+                mt_ring_flush(g);
             }
             u.ncount16[a0 + s] = cnt;
             if (cnt > cap) { atomicOr(&d.flags[0], 1u); atomicMax(d.nreq, cnt); }
@@ -974,7 +972,8 @@ struct mtmd_batch {
     // its staging: workgroup size and lanes per atom (autotuned)
     bool run = false;
     MbRun u{};
-    int run_block = 512, run_g = 2;
+    int run_block = 512, run_g = 2, run_split = 1;
+    float rlist = 0.f;
     int* order_batch = nullptr;   // workgroup -> replica of a whole-batch launch
     int* order_ident = nullptr;   // replica i at i (one-replica launches)
     bool have_forces = false, ke_valid = false;
@@ -1057,7 +1056,7 @@ static void mb_step(mtmd_batch* b, int r0, int r1, bool want_e, hipStream_t s)
 static size_t mb_run_lds(const mtmd_batch* b, int block)
 {
     size_t bytes = (size_t)(b->u.maxn + 16) * sizeof(float4) + (size_t)b->d.T * b->d.T * MT_NPTS * sizeof(float2) +
-                   (size_t)(2 * b->u.maxcells + 1) * sizeof(int);
+                   (size_t)(2 * b->u.maxcells + 1) * sizeof(int) + (size_t)block * MT_RING_DW * sizeof(uint32_t);
     return (bytes + 15) & ~(size_t)15;
 }
 
@@ -1088,6 +1087,32 @@ static hipError_t mb_run_launch(mtmd_batch* b, const int* order, int grid, int n
     if (b->run_g == 1) return mb_run_one<false, 1>(b, u, grid, nsteps, mode, want_e, s);
     if (b->run_g == 2) return mb_run_one<false, 2>(b, u, grid, nsteps, mode, want_e, s);
     return mb_run_one<false, 4>(b, u, grid, nsteps, mode, want_e, s);
+}
+
+// the persistent kernel's cell grids: x cells at least a list radius wide,
+// y and z cells `split` per list radius where each of those axes then holds
+// five or more cells and x three or more (else one per radius); uploads the
+// replicas and sets the staging's cell capacity. False when a grid would
+// exceed MB_RUN_MAXCELLS cells.
+static bool mb_set_split(mtmd_batch* b, int split, float rlist)
+{
+    int maxcells = 0;
+    std::vector<MbRep> rep = b->rep;
+    for (MbRep& rp : rep) {
+        for (int a = 0; a < 3; a++) rp.nc[a] = std::max(1, (int)std::floor(rp.L[a] / (double)rlist));
+        rp.sk = 1;
+        if (split == 2 && rp.nc[0] >= 3) {
+            const int ny = (int)std::floor(2.0 * rp.L[1] / (double)rlist), nz = (int)std::floor(2.0 * rp.L[2] / (double)rlist);
+            if (ny >= 5 && nz >= 5) { rp.nc[1] = ny; rp.nc[2] = nz; rp.sk = 2; }
+        }
+        rp.ncell = rp.nc[0] * rp.nc[1] * rp.nc[2];
+        if (rp.ncell > MB_RUN_MAXCELLS) return false;
+        maxcells = std::max(maxcells, rp.ncell);
+    }
+    if (hipMemcpy(b->d.rep, rep.data(), rep.size() * sizeof(MbRep), hipMemcpyHostToDevice) != hipSuccess) return false;
+    b->u.maxcells = maxcells;
+    b->run_split = split;
+    return true;
 }
 
 // the chain's kinetic energy slot from the reduced sums, no scale pending
@@ -1174,6 +1199,7 @@ extern "C" int mtmd_batch_create(int n_systems, const mtmd_system* systems, cons
     b->nrep = n_systems;
     const float rc = (float)c0.cutoff_radius;
     const float rlist = (float)(c0.cutoff_radius + c0.skin);
+    b->rlist = rlist;
     b->p.dt = c0.timestep / MT_CONV_t;
     b->p.Fcut = c0.cutoff_force;
     b->p.rc2 = rc * rc;
@@ -1254,6 +1280,7 @@ extern "C" int mtmd_batch_create(int n_systems, const mtmd_system* systems, cons
         }
         rp.cell0 = ncells;
         rp.ncell = rp.nc[0] * rp.nc[1] * rp.nc[2];
+        rp.sk = 1;
         rp.atom0 = natoms;
         rp.n = s.n_atoms;
         rp.T = cfgs[r].temperature / MT_CONV_T;
@@ -1371,6 +1398,9 @@ extern "C" int mtmd_batch_create(int n_systems, const mtmd_system* systems, cons
         ok = up(u.perm, ident.data(), natoms * sizeof(int)) && up(b->order_batch, ord.data(), n_systems * sizeof(int)) &&
              up(b->order_ident, id.data(), n_systems * sizeof(int));
         if (!ok) return bad(MTMD_EHIP, "upload failed");
+        // a cell shape chosen before any autotune (MTMD_BATCH_CELL_SPLIT=2)
+        const char* sp = std::getenv("MTMD_BATCH_CELL_SPLIT");
+        if (sp && std::atoi(sp) == 2 && !mb_set_split(b, 2, rlist)) return bad(MTMD_EHIP, "cell grid upload failed");
     }
     int rc0 = mb_prime(b);
     if (rc0) { g_batch_err = b->err; return rc0; }
@@ -1442,6 +1472,17 @@ extern "C" int mtmd_batch_info(const mtmd_batch* b, int32_t* n_systems, int32_t*
     if (n_atoms) *n_atoms = b->n;
     if (force_block) *force_block = b->run ? b->run_block : b->force_block;
     if (list_block) *list_block = b->run ? b->run_g : b->list_block;
+    return MTMD_OK;
+}
+
+extern "C" int mtmd_batch_tuning(const mtmd_batch* b, int32_t* persistent, int32_t* block, int32_t* lanes,
+                                 int32_t* cell_split)
+{
+    if (!b) return MTMD_EINVAL;
+    if (persistent) *persistent = b->run ? 1 : 0;
+    if (block) *block = b->run ? b->run_block : b->force_block;
+    if (lanes) *lanes = b->run ? b->run_g : 1;
+    if (cell_split) *cell_split = b->run ? b->run_split : 1;
     return MTMD_OK;
 }
 
@@ -1528,21 +1569,28 @@ extern "C" int mtmd_batch_autotune(mtmd_batch* b, int reps, double* force_ms, do
         };
         const int bcand[3] = {256, 512, 1024}, gcand[3] = {1, 2, 4};
         double best = 1e30, bf = 0.0, bl = 0.0;
-        int bb = b->run_block, bg = b->run_g;
-        for (int bs : bcand)
-            for (int g : gcand) {
-                if (mb_run_lds(b, bs) > 160 * 1024) continue;
-                b->run_block = bs;
-                b->run_g = g;
-                double tf = 0.0, tb = 0.0;
-                int rc = timed(2, tf);
-                if (!rc) rc = timed(3, tb);
-                if (rc) return rc;
-                const double score = tf + tb / 5.0;
-                if (std::getenv("MTMD_BATCH_TUNE_LOG"))
-                    std::fprintf(stderr, "batch tune: block %d lanes %d force %.4f ms list %.4f ms\n", bs, g, tf, tb);
-                if (score < best) { best = score; bb = bs; bg = g; bf = tf; bl = tb; }
-            }
+        int bb = b->run_block, bg = b->run_g, bsp = b->run_split;
+        for (int split = 1; split <= 2; split++) {
+            // cell shape: the grid changes the lists' order, so one build first
+            if (!mb_set_split(b, split, b->rlist)) continue;
+            MB_HIP(b, mb_run_launch(b, b->order_batch, b->nrep, 0, 3, 0, b->stream));
+            for (int bs : bcand)
+                for (int g : gcand) {
+                    if (mb_run_lds(b, bs) > 160 * 1024) continue;
+                    b->run_block = bs;
+                    b->run_g = g;
+                    double tf = 0.0, tb = 0.0;
+                    int rc = timed(2, tf);
+                    if (!rc) rc = timed(3, tb);
+                    if (rc) return rc;
+                    const double score = tf + tb / 5.0;
+                    if (std::getenv("MTMD_BATCH_TUNE_LOG"))
+                        std::fprintf(stderr, "batch tune: cells/radius %d block %d lanes %d force %.4f ms list %.4f ms\n",
+                                     split, bs, g, tf, tb);
+                    if (score < best) { best = score; bb = bs; bg = g; bsp = split; bf = tf; bl = tb; }
+                }
+        }
+        if (!mb_set_split(b, bsp, b->rlist)) return b->fail(MTMD_EHIP, "cell grid upload failed");
         b->run_block = bb;
         b->run_g = bg;
         if (force_ms) *force_ms = bf;

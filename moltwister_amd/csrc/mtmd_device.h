@@ -572,6 +572,38 @@ __device__ inline double mt_nh_step(double* eta, double* peta, const double* Q, 
     }
 }
 
+// The words of a 16-bit list row pass through a ring of 16 words (two
+// chunks) per lane in LDS: a hit is one 16-bit store and a count, and a chunk
+// that has filled is copied to the list once per group of candidates. A group
+// brings at most eight words, so at most one chunk completes in it and the
+// ring never holds more than 7 + 8 words. Lanes' rings are 9 dwords apart, so
+// the 16-bit stores of a wavefront fall in distinct banks.
+#define MT_RING_DW 9
+struct MtRing16 {
+    uint32_t* ring;    // this lane's ring
+    uint4* out;        // chunk 0 of the atom's row (column k of the chunked layout)
+    size_t stride;     // chunks between a row's chunks (n)
+    int capc;          // chunks a row holds
+    int cnt;           // words written
+    int fl;            // chunks copied out
+};
+
+__device__ __forceinline__ void mt_ring_put(MtRing16& g, int w)
+{
+    reinterpret_cast<uint16_t*>(g.ring)[g.cnt & 15] = (uint16_t)w;
+    g.cnt++;
+}
+
+__device__ __forceinline__ void mt_ring_flush(MtRing16& g)
+{
+    if ((g.cnt >> 3) > g.fl) {
+        const uint32_t* c = g.ring + (g.fl & 1) * 4;
+        const uint4 v = make_uint4(c[0], c[1], c[2], c[3]);
+        if (g.fl < g.capc) g.out[(size_t)g.fl * g.stride] = v;
+        g.fl++;
+    }
+}
+
 // the chunk being filled is a window of the last words written, r0..r3 (the
 // oldest lowest): a word enters at the top and the window moves down one
 // word, so after a chunk's last word the window is the chunk itself

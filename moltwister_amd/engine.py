@@ -305,7 +305,12 @@ class BatchEngine:
     def autotune(self, reps: int = 3) -> dict:
         f, l = C.c_double(0), C.c_double(0)
         self._chk(self.lib.mtmd_batch_autotune(self.h, reps, C.byref(f), C.byref(l)))
-        return dict(force_ms=f.value, list_ms=l.value, **self.info())
+        return dict(force_ms=f.value, list_ms=l.value, **self.info(), **self.tuning())
+
+    def tuning(self) -> dict:
+        v = [C.c_int32(0) for _ in range(4)]
+        self._chk(self.lib.mtmd_batch_tuning(self.h, *[C.byref(x) for x in v]))
+        return dict(persistent=bool(v[0].value), block=v[1].value, lanes=v[2].value, cell_split=v[3].value)
 
     def info(self) -> dict:
         v = [C.c_int32(0) for _ in range(4)]

@@ -144,8 +144,9 @@ def test_batch_autotune_and_nve_energy():
     warm.close()
     b = BatchEngine(systems, [lj_cfg(t, abi.NVE) for t in temps])
     info = b.autotune(2)
-    # every replica fits the persistent kernel: workgroup size x lanes per atom
-    assert info["force_block"] in (256, 512, 1024) and info["list_block"] in (1, 2, 4), info
+    # every replica fits the persistent kernel: cell shape x workgroup size x lanes per atom
+    assert info["persistent"] and info["block"] in (256, 512, 1024) and info["lanes"] in (1, 2, 4), info
+    assert info["cell_split"] in (1, 2), info
     e0 = [t["Etot"] for t in b.thermo()]
     b.step(200)
     e1 = [t["Etot"] for t in b.thermo()]
@@ -177,6 +178,32 @@ def test_batch_kernel_path_agrees(monkeypatch):
         assert abs(x["T"] - y["T"]) < 2e-3 * y["T"] and abs(x["Etot"] - y["Etot"]) < 2e-3 * abs(y["Etot"]), (x, y)
     a.close()
     k.close()
+
+
+def test_batch_two_cells_per_radius(monkeypatch):
+    """The persistent kernel with y and z cells of half a list radius (a
+    five-row stencil on those axes) for the replicas large enough: forces
+    against the brute-force table sum, and the same NVE run as with whole
+    cells up to FP32 summation order."""
+    print("case batch cell split 2", flush=True)
+    systems, temps = small_batch()
+    cfgs = [lj_cfg(t, abi.NVE) for t in temps]
+    a = BatchEngine(systems, cfgs)
+    monkeypatch.setenv("MTMD_BATCH_CELL_SPLIT", "2")
+    b = BatchEngine(systems, cfgs)
+    monkeypatch.delenv("MTMD_BATCH_CELL_SPLIT")
+    assert b.tuning()["cell_split"] == 2 and a.tuning()["cell_split"] == 1
+    for i, s in enumerate(systems):
+        F, _ = b.get_forces(i)
+        Fa, _ = a.get_forces(i)
+        assert rms_rel(F, table_forces(s)) < 1e-5, i
+        assert rms_rel(F, Fa) < 1e-6, i
+    a.step(100)
+    b.step(100)
+    for x, y in zip(a.thermo(), b.thermo()):
+        assert abs(x["Etot"] - y["Etot"]) < 1e-4 * abs(x["Etot"]) and abs(x["T"] - y["T"]) < 1e-3 * x["T"], (x, y)
+    a.close()
+    b.close()
 
 
 def test_batch_rejects_charges():

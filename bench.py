@@ -175,6 +175,55 @@ def kernels_only(args, warm, sysA) -> int:
     return 0
 
 
+def batch_leg(K: int) -> dict:
+    """Configuration 4: 512 LJ replicas of uneven size advanced together
+    (persistent kernel, autotuned), timed over max(100, K / 4) steps after 200
+    warm-up steps, beside one launch per replica per step."""
+    from moltwister_amd import abi, workloads as W
+    from moltwister_amd.engine import BatchEngine
+    t0 = time.perf_counter()
+    systems, temps = W.replicas(512, seed=7)
+    cfgs = []
+    for t in temps:
+        cb = lj_config(abi.ENGINE_MIXED, abi.NVT)
+        cb.temperature = t * 120.272
+        cfgs.append(cb)
+    be = BatchEngine(systems, cfgs)
+    tune = be.autotune(3)
+    t_setup = time.perf_counter() - t0
+    be.step(200)                                  # warm-up
+    Kb = max(100, K // 4)
+    be.synchronize()
+    t1 = time.perf_counter()
+    be.step(Kb)
+    be.synchronize()
+    dt_b = time.perf_counter() - t1
+    Ks = 4
+    t1 = time.perf_counter()
+    be.step_separate(Ks)
+    be.synchronize()
+    dt_s = time.perf_counter() - t1
+    th = be.thermo()
+    natoms = sum(x.n_atoms for x in systems)
+    batch = {"workload": "512 LJ replicas, N uniform in [256, 4096], rho* in [0.6, 1.0], T* in [0.6, 1.5], "
+                         "NH NVT, rc 2.5, skin 0.3, dt 0.005 tau",
+             "kernel": "persistent: one launch for all steps of a call, a workgroup per replica"
+                       if tune.get("persistent") else "captured step graph of separate kernels",
+             "atoms": natoms, "steps": Kb,
+             "replica_steps_per_s": 512 * Kb / dt_b, "ms_per_step": dt_b / Kb * 1e3,
+             "atom_steps_per_s": natoms * Kb / dt_b,
+             "separate_launches": {"steps": Ks, "replica_steps_per_s": 512 * Ks / dt_s,
+                                   "ms_per_step": dt_s / Ks * 1e3,
+                                   "what": "one launch a replica a step (the cost of 512 separate systems)"},
+             "speedup_vs_separate": (512 * Kb / dt_b) / (512 * Ks / dt_s),
+             "autotune": tune,
+             "T_err_rel_mean": float(np.mean([abs(x["T"] / 120.272 - t) / t for x, t in zip(th, temps)])),
+             "setup_s": t_setup}
+    be.close()
+    log(f"bench: batch {batch}")
+    return batch
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -186,6 +235,8 @@ def main() -> int:
     ap.add_argument("--skip-fm", action="store_true")
     ap.add_argument("--fm-frames", type=int, default=2048)
     ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--batch-only", action="store_true",
+                    help="only the 512-replica batch leg (for profiler runs); prints its JSON")
     ap.add_argument("--kernels-only", action="store_true",
                     help="short warm-up, then only the kernel timings (force pass, list build, copy) of the "
                          "mixed and FP64 engines; for profiler runs")
@@ -199,6 +250,9 @@ def main() -> int:
     from moltwister_amd.engine import Engine
 
     K, Wm = args.steps, args.warmup
+    if args.batch_only:
+        print(json.dumps(batch_leg(K)))
+        return 0
     t_gen = time.perf_counter()
     sysA = W.binary_lj_large(1048576, seed=3, T_star=1.0)
     n = sysA.n_atoms
@@ -412,48 +466,8 @@ def main() -> int:
     cfg1["workload"] = "FCC LJ, 4000 atoms, rho*=0.8442, T*=0.72, rc 2.5, NVE, dt 0.005 tau, 1000 timed steps"
     log(f"bench: configuration 1 {cfg1}")
 
-    batch = {}
-    if not args.skip_batch:
-        from moltwister_amd.engine import BatchEngine
-        t0 = time.perf_counter()
-        systems, temps = W.replicas(512, seed=7)
-        cfgs = []
-        for t in temps:
-            cb = lj_config(abi.ENGINE_MIXED, abi.NVT)
-            cb.temperature = t * 120.272
-            cfgs.append(cb)
-        be = BatchEngine(systems, cfgs)
-        tune = be.autotune(3)
-        be.step(200)                                  # warm-up
-        Kb = max(100, K // 4)
-        be.synchronize()
-        t1 = time.perf_counter()
-        be.step(Kb)
-        be.synchronize()
-        dt_b = time.perf_counter() - t1
-        Ks = 4
-        t1 = time.perf_counter()
-        be.step_separate(Ks)
-        be.synchronize()
-        dt_s = time.perf_counter() - t1
-        th = be.thermo()
-        natoms = sum(x.n_atoms for x in systems)
-        batch = {"workload": "512 LJ replicas, N uniform in [256, 4096], rho* in [0.6, 1.0], T* in [0.6, 1.5], "
-                             "NH NVT, rc 2.5, skin 0.3, dt 0.005 tau",
-                 "kernel": "persistent: one launch for all steps of a call, a workgroup per replica"
-                           if tune.get("persistent") else "captured step graph of separate kernels",
-                 "atoms": natoms, "steps": Kb,
-                 "replica_steps_per_s": 512 * Kb / dt_b, "ms_per_step": dt_b / Kb * 1e3,
-                 "atom_steps_per_s": natoms * Kb / dt_b,
-                 "separate_launches": {"steps": Ks, "replica_steps_per_s": 512 * Ks / dt_s,
-                                       "ms_per_step": dt_s / Ks * 1e3,
-                                       "what": "one launch a replica a step (the cost of 512 separate systems)"},
-                 "speedup_vs_separate": (512 * Kb / dt_b) / (512 * Ks / dt_s),
-                 "autotune": tune,
-                 "T_err_rel_mean": float(np.mean([abs(x["T"] / 120.272 - t) / t for x, t in zip(th, temps)])),
-                 "setup_s": t1 - t0}
-        be.close()
-        log(f"bench: batch {batch}")
+    batch = {} if args.skip_batch else batch_leg(K)
+
 
     fmres = {}
     if not args.skip_fm:

@@ -267,6 +267,22 @@ int mtmd_get_state(mtmd_handle* h, double* pos, double* mom);
 int mtmd_set_thermostat(mtmd_handle* h, const double* eta, const double* p_eta);
 int mtmd_get_thermostat(mtmd_handle* h, double* eta, double* p_eta);
 
+/* The reference's initial random state, drawn as CSimulationBox::initSystem
+ * draws it (SimulationBox.cpp:53-75) with glibc's rand() after srand(seed):
+ * thermostat chain positions (NHChain.cpp:126-132), barostat chain positions,
+ * momenta (SimulationBox.cpp:98-133: a rand() - 1 per component, a zero
+ * replaced by 1, immobile atoms zeroed, scaled to the temperature with unit
+ * masses, as the reference's particles still have the default mass of 1 at
+ * that point), thermostat chain momenta (NHChain.cpp:134-143), barostat chain
+ * momenta, barostat momentum (VelVerlet.cpp:176-183). A first `moldyn run` in a
+ * fresh MolTwister process on glibc draws with seed 1. Host only (no device).
+ * Outputs, any may be NULL: mom[3 n], eta_T and p_eta_T [temperature_nh_len],
+ * eta_P and p_eta_P [pressure_nh_len], p_eps. Pass them to mtmd_create (as
+ * sys->mom), mtmd_set_thermostat and mtmd_set_barostat to start from the
+ * reference's state. */
+int mtmd_reference_draws(const mtmd_system* s, const mtmd_config* c, uint32_t seed, double* mom,
+                         double* eta_T, double* p_eta_T, double* eta_P, double* p_eta_P, double* p_eps);
+
 /* NPT barostat of CVelVerlet (VelVerlet.h: eps_, p_eps_) and its chain NH_P_
  * (NHChain.h:59-61, arrays of pressure_nh_len). The reference draws p_eps and
  * the chain at random (VelVerlet.cpp:176-183, NHChain.cpp:126-143); a handle

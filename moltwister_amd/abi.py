@@ -182,6 +182,7 @@ EXPORTED = [
     "mtmd_batch_create", "mtmd_batch_destroy", "mtmd_batch_last_error", "mtmd_batch_step",
     "mtmd_batch_step_separate", "mtmd_batch_synchronize", "mtmd_batch_get_thermo", "mtmd_batch_get_state",
     "mtmd_batch_get_forces", "mtmd_batch_autotune", "mtmd_batch_info", "mtmd_batch_tuning",
+    "mtmd_reference_draws",
     "mtmd_fm_create", "mtmd_fm_destroy", "mtmd_fm_last_error", "mtmd_fm_num_params", "mtmd_fm_set_target",
     "mtmd_fm_target_from_params", "mtmd_fm_forces", "mtmd_fm_loss_grad",
 ]
@@ -249,6 +250,8 @@ def load_library(path: str | None = None) -> C.CDLL:
         "mtmd_batch_autotune": (C.c_int, [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
         "mtmd_batch_info": (C.c_int, [vp, _ip, _ip, _ip, _ip]),
         "mtmd_batch_tuning": (C.c_int, [vp, _ip, _ip, _ip, _ip]),
+        "mtmd_reference_draws": (C.c_int, [C.POINTER(MtmdSystem), C.POINTER(MtmdConfig), C.c_uint32, _dp, _dp, _dp,
+                                           _dp, _dp, _dp]),
         "mtmd_fm_create": (C.c_int, [C.POINTER(MtmdSystem), C.c_double, C.c_int, _dp, _dp, C.POINTER(C.c_void_p)]),
         "mtmd_fm_destroy": (None, [vp]),
         "mtmd_fm_last_error": (C.c_char_p, [vp]),

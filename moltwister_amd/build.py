@@ -6,6 +6,7 @@ Translation units:
   mtmd_fast.hip    FP64/MIXED engine kernels (contraction allowed; the exact
                    device helpers disable it locally)
   mtmd_host.cpp    C ABI, set-up and step driver (hipcc, host code)
+  mtmd_draws.cpp   the reference's initial random draws (hipcc, host code)
   mtmd_topo.hip    topology search (bonds, angles, dihedrals by distance
                    criteria), -ffp-contract=off
   mtmd_tables.cpp  force/energy profiles, g++ -ffp-contract=off so the float
@@ -57,6 +58,8 @@ def build(verbose: bool = False) -> str:
          os.path.join(SRC, "mtmd_topo.hip")],
         [HIPCC, *HIP_FLAGS, "-x", "hip", "-c", "-o", os.path.join(OUT, "mtmd_host.o"),
          os.path.join(SRC, "mtmd_host.cpp")],
+        [HIPCC, *HIP_FLAGS, "-ffp-contract=off", "-x", "hip", "-c", "-o", os.path.join(OUT, "mtmd_draws.o"),
+         os.path.join(SRC, "mtmd_draws.cpp")],
         [shutil.which("g++") or "g++", "-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-c",
          "-o", os.path.join(OUT, "mtmd_tables.o"), os.path.join(SRC, "mtmd_tables.cpp")],
         [shutil.which("g++") or "g++", "-O2", "-fPIC", "-std=c++17", "-Wall", "-c",
@@ -67,7 +70,7 @@ def build(verbose: bool = False) -> str:
             if verbose and (r.stdout or r.stderr):
                 sys.stderr.write(r.stdout + r.stderr)
     objs = [os.path.join(OUT, f) for f in ("mtmd_exact.o", "mtmd_fast.o", "mtmd_batch.o", "mtmd_fm.o", "mtmd_topo.o",
-                                           "mtmd_host.o", "mtmd_tables.o", "mtmd_output.o")]
+                                           "mtmd_host.o", "mtmd_draws.o", "mtmd_tables.o", "mtmd_output.o")]
     # workload generator helpers (host only)
     _run([shutil.which("gcc") or "gcc", "-O2", "-fPIC", "-shared", "-o", GENLIB, os.path.join(SRC, "mtmd_gen.c"), "-lm"])
     tmp = LIB + ".tmp"

@@ -69,6 +69,25 @@ def profile(which: int, kind: int, params, x0: float, x1: float, n: int = 1000):
     return x, f, e
 
 
+def reference_draws(system: System, cfg: abi.MtmdConfig, seed: int = 1) -> dict:
+    """The reference's initial random state (CSimulationBox::initSystem,
+    SimulationBox.cpp:53-75, with glibc's rand() after srand(seed)) through
+    mtmd_reference_draws: momenta [n, 3], the thermostat chain (eta_T, p_eta_T),
+    the barostat chain (eta_P, p_eta_P) and p_eps. Seed 1 is the state a first
+    `moldyn run` in a fresh MolTwister process starts from."""
+    lib = abi.load_library()
+    n, mt, mp = system.n_atoms, cfg.temperature_nh_len, cfg.pressure_nh_len
+    mom = np.zeros((n, 3))
+    et, pt, ep, pp = np.zeros(max(mt, 1)), np.zeros(max(mt, 1)), np.zeros(max(mp, 1)), np.zeros(max(mp, 1))
+    pe = C.c_double(0.0)
+    csys = system.to_c()
+    rc = lib.mtmd_reference_draws(C.byref(csys), C.byref(cfg), int(seed) & 0xFFFFFFFF, _f64(mom), _f64(et),
+                                  _f64(pt), _f64(ep), _f64(pp), C.byref(pe))
+    if rc:
+        raise MtmdError(rc, "mtmd_reference_draws")
+    return dict(mom=mom, eta_T=et[:mt], p_eta_T=pt[:mt], eta_P=ep[:mp], p_eta_P=pp[:mp], p_eps=pe.value)
+
+
 class Engine:
     def __init__(self, system: System, cfg: abi.MtmdConfig):
         self.lib = abi.load_library()

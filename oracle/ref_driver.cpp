@@ -11,7 +11,8 @@
 //   - cells:     CFunctorGenCellList::operator() + assembleCellList
 //   - neighbours:CFunctorGenNeighList::operator() with the grow-and-retry loop
 //   - forces:    CFunctorCalcForce::operator()
-//   - NH chain:  CNHChain::propagator, CMt::exp/sinhXoverX
+//   - NH chain:  CNHChain::propagator, CMt::exp/sinhXoverX, and its initial
+//                draws (setRandNHPos, setRandNHMom)
 //   - DCD files: CDCDFile::createDCDFileIfNotExists + appendToDCDFile, called
 //                as CMDLoop::appendToDCDFile calls them (MDLoop.cpp:248-258)
 //
@@ -509,6 +510,25 @@ EXPORT void ref_get_thermostat(const ref_md* o, double* eta, double* p_eta)
     for (int j = 0; j < o->nh.M_; j++) {
         if (eta) eta[j] = o->nh.eta_[j];
         if (p_eta) p_eta[j] = o->nh.p_eta_[j];
+    }
+}
+
+// CNHChain's own initial draws with the C library's rand() in its current
+// state: prepareArrays(n, dim) (NHChain.cpp:115-124), then setRandNHPos
+// (which = 0, NHChain.cpp:126-132) or setRandNHMom (which = 1, :134-143).
+// A caller interleaves these with its own rand() calls to replay
+// CSimulationBox::initSystem's sequence (SimulationBox.cpp:66-71).
+EXPORT void ref_nh_draw(int which, int M, int n, int dim, double T, double tau, double* eta, double* p_eta)
+{
+    CNHChain c;
+    c.M_ = M;
+    c.T_ = T;
+    c.tau_ = tau;
+    c.prepareArrays(n, dim);
+    if (which == 0) c.setRandNHPos(); else c.setRandNHMom();
+    for (int j = 0; j < M; j++) {
+        if (eta) eta[j] = c.eta_[j];
+        if (p_eta) p_eta[j] = c.p_eta_[j];
     }
 }
 

@@ -104,6 +104,26 @@ def test_one_step_state(P, name, ensemble):
         assert _rel(tg[k], to[k]) < 1e-10, (k, tg[k], to[k])
 
 
+def test_trace_from_reference_draws(P):
+    """A run started as a first `moldyn run` starts: momenta and thermostat
+    chain from the reference's own draws (mtmd_reference_draws, seed 1),
+    EXACT engine against the oracle over 50 NVT steps, pointwise within 1e-8."""
+    print("case trace from reference draws water_300", flush=True)
+    from moltwister_amd.engine import reference_draws
+    s, c = make("water_300", ensemble=abi.NVT)
+    d = reference_draws(s, c, 1)
+    s.mom = d["mom"]
+    o, e = P.Oracle(s, c), Engine(s, c)
+    o.set_thermostat(d["eta_T"], d["p_eta_T"])
+    e.set_thermostat(d["eta_T"], d["p_eta_T"])
+    for blk in range(5):
+        o.step(10)
+        e.step(10)
+        to, tg = o.thermo(), e.thermo()
+        for k in ("T", "P", "U", "K", "Etot"):
+            assert _rel(tg[k], to[k]) < 1e-8, (blk, k, tg[k], to[k])
+
+
 @pytest.mark.parametrize("name,ensemble", [("fcc_4000", abi.NVE), ("water_300", abi.NVT)])
 def test_trace_100_steps(P, name, ensemble):
     """Thermo trace over 100 steps, pointwise within 1e-8 relative."""

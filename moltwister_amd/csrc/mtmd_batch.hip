@@ -453,6 +453,7 @@ struct MbRun {
     uint32_t* nl16;           // 16-bit rows: chunk q (8 words) of slot a0 + s at uint4 q * n + a0 + s
     int* ncount16;            // [n] words in each slot's row
     int ncap16;               // words a row holds (a multiple of 8)
+    int nlimit;               // longest row accepted without a regrow (ncap16, or less with no slack)
     int maxn, maxcells;       // staging capacities of the launch (largest replica)
 };
 
@@ -784,7 +785,7 @@ __global__ void __launch_bounds__(MB_RUN_TPB) k_mb_run(MbDev d, MbPar p, MbRun u
                 mt_ring_flush(g);
             }
             u.ncount16[a0 + s] = cnt;
-            if (cnt > cap) { atomicOr(&d.flags[0], 1u); atomicMax(d.nreq, cnt); }
+            if (cnt > u.nlimit) { atomicOr(&d.flags[0], 1u); atomicMax(d.nreq, cnt); }
         }
         __syncthreads();
     };
@@ -952,6 +953,11 @@ struct DevMem {
         ptrs.push_back(p);
         return (T*)p;
     }
+    void release(void* p)
+    {
+        for (size_t i = 0; i < ptrs.size(); i++)
+            if (ptrs[i] == p) { (void)hipFree(p); ptrs.erase(ptrs.begin() + (long)i); return; }
+    }
     ~DevMem() { for (void* p : ptrs) (void)hipFree(p); }
 };
 
@@ -976,6 +982,10 @@ struct mtmd_batch {
     float rlist = 0.f;
     int* order_batch = nullptr;   // workgroup -> replica of a whole-batch launch
     int* order_ident = nullptr;   // replica i at i (one-replica launches)
+    // the state a call starts from, to run it again after the lists grow
+    double4 *snap_pos = nullptr, *snap_mom = nullptr, *snap_frc = nullptr;
+    double* snap_nh = nullptr;
+    bool list_slack = true;       // MTMD_BATCH_LIST_SLACK=0: rows sized to the longest, no margin
     bool have_forces = false, ke_valid = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
@@ -1140,17 +1150,114 @@ static int mb_check(mtmd_batch* b)
     return MTMD_OK;
 }
 
+// list overflow of the last launches: the longest row requested (0: none);
+// non-finite positions are an error
+static int mb_overflow(mtmd_batch* b, int& need)
+{
+    unsigned int fl[2];
+    need = 0;
+    MB_HIP(b, hipMemcpyAsync(fl, b->d.flags, sizeof fl, hipMemcpyDeviceToHost, b->stream));
+    MB_HIP(b, hipStreamSynchronize(b->stream));
+    if (fl[1]) return b->fail(MTMD_ENAN, "non-finite atom position");
+    if (fl[0]) {
+        MB_HIP(b, hipMemcpy(&need, b->d.nreq, sizeof(int), hipMemcpyDeviceToHost));
+        need = std::max(need, b->u.nlimit + 1);
+    }
+    return MTMD_OK;
+}
+
+// the persistent kernel's rows grown to hold `need` words (with a fifth more
+// and 16 words of margin unless MTMD_BATCH_LIST_SLACK=0, which accepts rows
+// up to `need` only, so that any longer row grows them again), the overflow
+// flag and request cleared (CMDFFMatrices grows its list and builds it again,
+// MDFFMatrices.cpp:163-178)
+static int mb_grow16(mtmd_batch* b, int need)
+{
+    int cap = b->list_slack ? (int)std::ceil(need * 1.2) + 16 : need;
+    cap = (cap + 7) & ~7;
+    if (cap > 65536) return b->fail(MTMD_ENEIGH, "neighbour list rows above 65536 words requested (%d)", need);
+    if (cap > b->u.ncap16) {
+        b->mem.release(b->u.nl16);
+        b->u.nl16 = b->mem.alloc<uint32_t>((size_t)(cap / 2) * b->n);
+        if (!b->u.nl16) return b->fail(MTMD_ENOMEM, "list growth to %d words a row failed", cap);
+        b->u.ncap16 = cap;
+    }
+    b->u.nlimit = b->list_slack ? b->u.ncap16 : need;
+    MB_HIP(b, hipMemsetAsync(b->d.flags, 0, sizeof(unsigned int), b->stream));
+    MB_HIP(b, hipMemsetAsync(b->d.nreq, 0, sizeof(int), b->stream));
+    return MTMD_OK;
+}
+
+static int mb_snapshot(mtmd_batch* b, bool restore)
+{
+    const size_t a = (size_t)b->n * sizeof(double4), h = (size_t)b->nrep * (3 * b->d.M + 2) * sizeof(double);
+    const hipMemcpyKind k = hipMemcpyDeviceToDevice;
+    if (restore) {
+        MB_HIP(b, hipMemcpyAsync(b->d.pos, b->snap_pos, a, k, b->stream));
+        MB_HIP(b, hipMemcpyAsync(b->d.mom, b->snap_mom, a, k, b->stream));
+        MB_HIP(b, hipMemcpyAsync(b->d.frc, b->snap_frc, a, k, b->stream));
+        MB_HIP(b, hipMemcpyAsync(b->d.nh, b->snap_nh, h, k, b->stream));
+    } else {
+        MB_HIP(b, hipMemcpyAsync(b->snap_pos, b->d.pos, a, k, b->stream));
+        MB_HIP(b, hipMemcpyAsync(b->snap_mom, b->d.mom, a, k, b->stream));
+        MB_HIP(b, hipMemcpyAsync(b->snap_frc, b->d.frc, a, k, b->stream));
+        MB_HIP(b, hipMemcpyAsync(b->snap_nh, b->d.nh, h, k, b->stream));
+    }
+    return MTMD_OK;
+}
+
+// launches of a call on the persistent kernel (run(): whole batch or replica
+// by replica), with list growth: after an overflow the rows grow, the state
+// the call started from is restored, lists are built from it, and the call
+// runs again
+template <class Run>
+static int mb_run_call(mtmd_batch* b, Run run)
+{
+    int rc = mb_snapshot(b, false);
+    if (rc) return rc;
+    for (int attempt = 0; attempt < 16; attempt++) {
+        MB_HIP(b, run());
+        int need = 0;
+        if ((rc = mb_overflow(b, need))) return rc;
+        if (!need) return MTMD_OK;
+        // grow until a build from the restored state fits, then run again
+        for (;;) {
+            if ((rc = mb_grow16(b, need)) || (rc = mb_snapshot(b, true))) return rc;
+            MB_HIP(b, mb_run_launch(b, b->order_batch, b->nrep, 0, 3, 0, b->stream));
+            if ((rc = mb_overflow(b, need))) return rc;
+            if (!need) break;
+        }
+        if (std::getenv("MTMD_BATCH_LOG"))
+            std::fprintf(stderr, "batch: lists grown to %d words a row, call run again\n", b->u.ncap16);
+    }
+    return b->fail(MTMD_ENEIGH, "neighbour lists kept overflowing (capacity %d)", b->u.ncap16);
+}
+
+// lists built and forces and energies computed on the current state (mode 0),
+// the rows grown until the lists fit
+static int mb_build_fit(mtmd_batch* b)
+{
+    for (int attempt = 0; attempt <= 16; attempt++) {
+        MB_HIP(b, mb_run_launch(b, b->order_batch, b->nrep, 0, 0, 1, b->stream));
+        int need = 0, rc = mb_overflow(b, need);
+        if (rc) return rc;
+        if (!need) return MTMD_OK;
+        if ((rc = mb_grow16(b, need))) return rc;
+    }
+    return b->fail(MTMD_ENEIGH, "neighbour lists kept overflowing (capacity %d)", b->u.ncap16);
+}
+
 // initial lists, forces and kinetic energies
 static int mb_prime(mtmd_batch* b)
 {
     MbDev& d = b->d;
     if (b->run) {
-        MB_HIP(b, mb_run_launch(b, b->order_batch, b->nrep, 0, 0, 1, b->stream));
+        int rc = mb_build_fit(b);
+        if (rc) return rc;
         hipLaunchKernelGGL(k_mb_reduce, dim3(b->nrep), dim3(MB_TPB), 0, b->stream, d, 0, 0);
         hipLaunchKernelGGL(k_mb_ke_slot, dim3(mb_blk(b->nrep, 64)), dim3(64), 0, b->stream, d);
         MB_HIP(b, hipGetLastError());
-        int rc = mb_check(b);
-        if (rc) return rc;
+        if ((rc = mb_check(b))) return rc;
         b->have_forces = true;
         return MTMD_OK;
     }
@@ -1376,7 +1483,16 @@ extern "C" int mtmd_batch_create(int n_systems, const mtmd_system* systems, cons
     }
     if (b->run) {
         MbRun& u = b->u;
-        u.ncap16 = ((int)std::ceil(nexp_max * 1.35) + 16 + 7) & ~7;
+        const char* ls = std::getenv("MTMD_BATCH_LIST_SLACK");
+        b->list_slack = !(ls && std::strcmp(ls, "0") == 0);
+        u.ncap16 = b->list_slack ? (((int)std::ceil(nexp_max * 1.35) + 16 + 7) & ~7) : 16;
+        u.nlimit = u.ncap16;
+        b->snap_pos = b->mem.alloc<double4>(natoms);
+        b->snap_mom = b->mem.alloc<double4>(natoms);
+        b->snap_frc = b->mem.alloc<double4>(natoms);
+        b->snap_nh = b->mem.alloc<double>((size_t)n_systems * stride);
+        if (!b->snap_pos || !b->snap_mom || !b->snap_frc || !b->snap_nh)
+            return bad(MTMD_ENOMEM, "device allocation failed");
         u.perm = b->mem.alloc<int>(natoms);
         u.perm2 = b->mem.alloc<int>(natoms);
         u.ncount16 = b->mem.alloc<int>(natoms);
@@ -1418,9 +1534,9 @@ extern "C" int mtmd_batch_step(mtmd_batch* b, int32_t nsteps)
     if (nsteps == 0) return MTMD_OK;
     if (b->run) {
         // every step of every replica in one launch, energies in the last
-        MB_HIP(b, mb_run_launch(b, b->order_batch, b->nrep, nsteps, 1, 1, b->stream));
-        b->steps += nsteps;
-        return mb_check(b);
+        const int rc = mb_run_call(b, [&]() { return mb_run_launch(b, b->order_batch, b->nrep, nsteps, 1, 1, b->stream); });
+        if (!rc) b->steps += nsteps;
+        return rc;
     }
     if (!b->graph_exec) {
         MB_HIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
@@ -1444,11 +1560,16 @@ extern "C" int mtmd_batch_step_separate(mtmd_batch* b, int32_t nsteps)
     if (!b || nsteps < 0) return MTMD_EINVAL;
     if (b->run) {
         // one launch a replica a step, as n separate engines would run
-        for (int t = 0; t < nsteps; t++)
-            for (int r = 0; r < b->nrep; r++)
-                MB_HIP(b, mb_run_launch(b, b->order_ident + r, 1, 1, 1, t == nsteps - 1, b->stream));
-        b->steps += nsteps;
-        return mb_check(b);
+        const int rc = mb_run_call(b, [&]() -> hipError_t {
+            for (int t = 0; t < nsteps; t++)
+                for (int r = 0; r < b->nrep; r++) {
+                    const hipError_t e = mb_run_launch(b, b->order_ident + r, 1, 1, 1, t == nsteps - 1, b->stream);
+                    if (e != hipSuccess) return e;
+                }
+            return hipSuccess;
+        });
+        if (!rc) b->steps += nsteps;
+        return rc;
     }
     for (int t = 0; t < nsteps; t++)
         for (int r = 0; r < b->nrep; r++) mb_step(b, r, r + 1, t == nsteps - 1, b->stream);
@@ -1598,8 +1719,7 @@ extern "C" int mtmd_batch_autotune(mtmd_batch* b, int reps, double* force_ms, do
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         // the timed passes ran without energies: refresh them for the thermo
-        MB_HIP(b, mb_run_launch(b, b->order_batch, b->nrep, 0, 0, 1, b->stream));
-        return mb_check(b);
+        return mb_build_fit(b);
     }
     auto time_it = [&](auto&& launch, double& ms_out) -> int {
         launch();   // warm

@@ -206,6 +206,45 @@ def test_batch_two_cells_per_radius(monkeypatch):
     b.close()
 
 
+def test_batch_list_regrow(monkeypatch, capfd):
+    """List growth inside a call (MDFFMatrices.cpp:163-178 grows and builds
+    again): with MTMD_BATCH_LIST_SLACK=0 the rows start at 16 words and any
+    row longer than the longest one so far grows them, so builds inside a call
+    overflow as the rows fluctuate.
+    The first build grows until it fits and then equals a batch with room to
+    spare, bit for bit; calls that overflow restore their starting state and
+    run again, ending on the same trajectory up to the summation order of
+    lists built at other steps."""
+    print("case batch list regrow", flush=True)
+    systems, temps = small_batch()
+    cfgs = [lj_cfg(t) for t in temps]
+    warm = BatchEngine(systems, cfgs)     # from a liquid: the lattice start has the longest rows
+    warm.step(400)
+    for i, s in enumerate(systems):
+        s.pos, s.mom = warm.get_state(i)
+    warm.close()
+    a = BatchEngine(systems, cfgs)
+    monkeypatch.setenv("MTMD_BATCH_LIST_SLACK", "0")
+    monkeypatch.setenv("MTMD_BATCH_LOG", "1")
+    b = BatchEngine(systems, cfgs)
+    for i in range(len(systems)):
+        assert np.array_equal(a.get_forces(i)[0], b.get_forces(i)[0]), i
+    grown = ""
+    for _ in range(40):                   # until a call has overflowed (rows fluctuate as atoms move)
+        a.step(50)
+        b.step(50)
+        grown = capfd.readouterr().err
+        if "lists grown" in grown:
+            break
+    assert "lists grown" in grown
+    b.step_separate(5)
+    a.step_separate(5)
+    for x, y in zip(a.thermo(), b.thermo()):
+        assert abs(x["T"] - y["T"]) < 1e-4 * x["T"] and abs(x["Etot"] - y["Etot"]) < 1e-4 * abs(x["Etot"]), (x, y)
+    a.close()
+    b.close()
+
+
 def test_batch_rejects_charges():
     systems, temps = small_batch()
     systems[1].charge[:] = 0.1

@@ -27,21 +27,36 @@ def lj_cfg(T_star, ensemble=abi.NVT):
     return c
 
 
-def table_forces(s, rc=2.5):
-    x, f, _ = profile(0, abi.NB_LJ, (1.0, 1.0, 1.0), 0.01, float(np.float32(rc)))
-    x, f = x.astype(np.float64), f.astype(np.float64)
+def table_forces(s, rc=2.5, chunk=512, energy=False):
+    """Brute-force sum of the tabulated pair force (and with energy=True also
+    the total tabulated pair energy) over minimum images, per type pair (the
+    system's non-bonded entries), rows in chunks"""
+    tabs = {}
+    for ta, tb, kind, prm in s.nonbonded:
+        x, f, e = profile(0, kind, prm, 0.01, float(np.float32(rc)))
+        tabs[(ta, tb)] = tabs[(tb, ta)] = (x.astype(np.float64), f.astype(np.float64), e.astype(np.float64))
     L = np.array(s.box)
-    d = s.pos[:, None, :] - s.pos[None, :, :]
-    d -= L * np.round(d / L)
-    r = np.sqrt(np.sum(d * d, axis=2))
-    np.fill_diagonal(r, 1e9)
-    delta = (x[-1] - x[0]) / 999.0
-    idx = np.floor((r - x[0]) / delta).astype(np.int64)
-    ok = (r < np.float32(rc)) & (idx >= 0) & (idx < 999)
-    i = np.clip(idx, 0, 998)
-    fr = f[i] + (f[i + 1] - f[i]) * (r - x[i]) / (x[i + 1] - x[i])
-    fr = np.where(ok, fr, 0.0) / r
-    return np.sum(d * fr[:, :, None], axis=1)
+    F = np.zeros((s.n_atoms, 3))
+    U = 0.0
+    for i0 in range(0, s.n_atoms, chunk):
+        d = s.pos[i0:i0 + chunk, None, :] - s.pos[None, :, :]
+        d -= L * np.round(d / L)
+        r = np.sqrt(np.sum(d * d, axis=2))
+        rows = np.arange(i0, min(i0 + chunk, s.n_atoms))
+        r[rows - i0, rows] = 1e9
+        fr = np.zeros_like(r)
+        for (ta, tb), (x, f, e) in tabs.items():
+            m = (s.types[rows][:, None] == ta) & (s.types[None, :] == tb)
+            delta = (x[-1] - x[0]) / 999.0
+            idx = np.floor((r - x[0]) / delta).astype(np.int64)
+            ok = m & (r < np.float32(rc)) & (idx >= 0) & (idx < 999)
+            i = np.clip(idx, 0, 998)
+            w = (r - x[i]) / (x[i + 1] - x[i])
+            fr += np.where(ok, f[i] + (f[i + 1] - f[i]) * w, 0.0)
+            if energy:
+                U += 0.5 * float(np.sum(np.where(ok, e[i] + (e[i + 1] - e[i]) * w, 0.0)))
+        F[rows] = np.sum(d * (fr / r)[:, :, None], axis=1)
+    return (F, U) if energy else F
 
 
 def rms_rel(F, Fr):
@@ -242,6 +257,47 @@ def test_batch_list_regrow(monkeypatch, capfd):
     for x, y in zip(a.thermo(), b.thermo()):
         assert abs(x["T"] - y["T"]) < 1e-4 * x["T"] and abs(x["Etot"] - y["Etot"]) < 1e-4 * abs(x["Etot"]), (x, y)
     a.close()
+    b.close()
+
+
+def binary_replica(n, rho, seed):
+    """a two-type LJ replica (80:20, eps/sigma AA 1/1, AB 1.5/0.8, BB 0.5/0.88),
+    type 0 always A, so that every replica shares the force-field list"""
+    s = W.replica_lj(n, rho, 1.0, seed)
+    rng = np.random.default_rng(seed + 1)
+    s.types = (rng.random(n) < 0.2).astype(np.int32)
+    s.type_names = ["A", "B"]
+    s.nonbonded = [(0, 0, abi.NB_LJ, (1.0, 1.0, 1.0)), (0, 1, abi.NB_LJ, (1.5, 0.8, 1.0)),
+                   (1, 1, abi.NB_LJ, (0.5, 0.88, 1.0))]
+    return s
+
+
+def test_batch_two_types_and_largest_replica():
+    """The persistent kernel at its limits: two atom types (a table of four
+    type-pair rows in LDS), a replica of 4096 atoms (the staging's capacity),
+    and a box under three cells per axis (minimum image per pair), against the
+    brute-force table sum per type pair; then 200 NVE steps keep each
+    replica's energy."""
+    print("case batch two types, 4096 atoms", flush=True)
+    systems = [binary_replica(4096, 1.0, 31), binary_replica(300, 0.9, 32), binary_replica(2000, 0.7, 33)]
+    warm = BatchEngine(systems, [lj_cfg(1.0) for _ in systems])
+    assert warm.tuning()["persistent"]
+    for i, s in enumerate(systems):
+        F, Ua = warm.get_forces(i)
+        Fr, Ur = table_forces(s, energy=True)
+        assert rms_rel(F, Fr) < 1e-5, (i, rms_rel(F, Fr))
+        assert abs(float(np.sum(Ua)) - Ur) < 1e-5 * abs(Ur), (i, float(np.sum(Ua)), Ur)
+    # from a liquid (the lattice start melts, and pairs crossing the unshifted
+    # cutoff move Etot): 200 NVE steps keep each replica's energy
+    warm.step(400)
+    for i, s in enumerate(systems):
+        s.pos, s.mom = warm.get_state(i)
+    warm.close()
+    b = BatchEngine(systems, [lj_cfg(1.0, abi.NVE) for _ in systems])
+    e0 = [t["Etot"] for t in b.thermo()]
+    b.step(200)
+    for x, t in zip(e0, b.thermo()):
+        assert abs(t["Etot"] - x) < 5e-3 * abs(x), (x, t["Etot"])
     b.close()
 
 

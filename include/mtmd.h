@@ -469,9 +469,12 @@ void mtmd_batch_destroy(mtmd_batch* b);
 const char* mtmd_batch_last_error(const mtmd_batch* b);
 /* nsteps of every replica, energies computed in the last. When every replica
  * holds at most 4096 atoms and 1024 cells and there are at most two atom
- * types, one launch of a persistent kernel runs all nsteps: a workgroup per
- * replica, its FP32 positions and the pair table resident in LDS, list
- * rebuilds, forces, integration and chain inside the workgroup. Otherwise (or
+ * types, one launch of a persistent kernel runs up to 256 steps: a workgroup
+ * per replica, its FP32 positions and the pair table resident in LDS, list
+ * rebuilds, forces, integration and chain inside the workgroup; a list that
+ * overflows is grown and the launch's steps run again from their starting
+ * state, as the reference grows its list and builds it again
+ * (MDFFMatrices.cpp:163-178). Otherwise (or
  * with MTMD_BATCH_KERNELS set in the environment at create time) a captured
  * hipGraph of the whole batch's step is replayed, the last step eagerly. */
 int mtmd_batch_step(mtmd_batch* b, int32_t nsteps);
@@ -484,12 +487,12 @@ int mtmd_batch_get_thermo(mtmd_batch* b, mtmd_thermo* out);
 /* state, forces (capped, as integrated) and per-atom energies of replica i */
 int mtmd_batch_get_state(mtmd_batch* b, int32_t i, double* pos, double* mom);
 int mtmd_batch_get_forces(mtmd_batch* b, int32_t i, double* F, float* U);
-/* autotuner over the whole batch, `reps` launches a candidate; returns the
- * chosen shape's force-pass and list-build times. Persistent kernel: the
- * cell shape (one or two cells per list radius along y and z) times the
- * workgroup size (256, 512, 1024) times the lanes per atom (1, 2, 4), scored
- * as a force pass plus a fifth of a list build. Graph path: the pair-force
- * kernel's block (64..512) and the list build's (64..256). */
+/* autotuner over the whole batch; returns the chosen shape's force-pass and
+ * list-build times. Persistent kernel: the cell shape (one or two cells per
+ * list radius along y and z) times the workgroup size (256, 512, 1024) times
+ * the lanes per atom (1, 2, 4), each timed on 8 reps MD steps from the same
+ * state, which is restored afterwards. Graph path: the pair-force kernel's
+ * block (64..512) and the list build's (64..256), `reps` launches each. */
 int mtmd_batch_autotune(mtmd_batch* b, int reps, double* force_ms, double* list_ms);
 /* force_block / list_block: the persistent kernel's workgroup size and lanes
  * per atom, or the graph path's force and list kernel blocks */

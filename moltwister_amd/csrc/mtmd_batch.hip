@@ -444,6 +444,7 @@ __global__ void k_mb_fill_u32(unsigned int* a, unsigned int v, int n)
 #define MB_RUN_MAXN 4096      // atoms a replica may hold (64 KB of staged float4 positions)
 #define MB_RUN_MAXCELLS 1024  // cells of a replica's grid
 #define MB_RUN_TPB 1024       // largest workgroup
+#define MB_RUN_CHUNK 256      // steps a launch of a call runs at most (the unit a list growth runs again)
 
 struct MbRun {
     const int* order;         // replica of workgroup b: large replicas first, then small ones, so the two
@@ -1526,17 +1527,24 @@ extern "C" int mtmd_batch_create(int n_systems, const mtmd_system* systems, cons
 
 extern "C" void mtmd_batch_destroy(mtmd_batch* b) { delete b; }
 
-// nsteps of every replica; steps before the last are replays of one captured
-// step graph, the last step runs eagerly with energies
+// nsteps of every replica: the persistent kernel, MB_RUN_CHUNK steps a
+// launch; or (graph path) replays of one captured step graph, the last step
+// eagerly with energies
 extern "C" int mtmd_batch_step(mtmd_batch* b, int32_t nsteps)
 {
     if (!b || nsteps < 0) return MTMD_EINVAL;
     if (nsteps == 0) return MTMD_OK;
     if (b->run) {
-        // every step of every replica in one launch, energies in the last
-        const int rc = mb_run_call(b, [&]() { return mb_run_launch(b, b->order_batch, b->nrep, nsteps, 1, 1, b->stream); });
-        if (!rc) b->steps += nsteps;
-        return rc;
+        // every step of every replica in one launch a chunk of up to
+        // MB_RUN_CHUNK steps (a list growth runs one chunk again), energies
+        // in the last step of the call
+        for (int t = 0; t < nsteps; t += MB_RUN_CHUNK) {
+            const int k = std::min(MB_RUN_CHUNK, nsteps - t), last = t + k == nsteps;
+            const int rc = mb_run_call(b, [&]() { return mb_run_launch(b, b->order_batch, b->nrep, k, 1, last, b->stream); });
+            if (rc) return rc;
+            b->steps += k;
+        }
+        return MTMD_OK;
     }
     if (!b->graph_exec) {
         MB_HIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
@@ -1673,11 +1681,37 @@ extern "C" int mtmd_batch_autotune(mtmd_batch* b, int reps, double* force_ms, do
     MB_HIP(b, hipEventCreate(&e0));
     MB_HIP(b, hipEventCreate(&e1));
     if (b->run) {
-        // the persistent kernel: workgroup size x lanes per atom, each timed on
-        // a force pass (mode 2) and a list build (mode 3), scored as one force
-        // pass plus a fifth of a build (lists are rebuilt every few steps);
-        // state and lists are unchanged by either
-        auto timed = [&](int mode, double& ms_out) -> int {
+        // the persistent kernel: every candidate runs the same steps from the
+        // same state (kept on the device, restored and its lists built again
+        // before each), timed with HIP events; the state is restored at the
+        // end. The chosen shape's force pass (mode 2) and list build (mode 3)
+        // are timed for the report.
+        int rc = mb_snapshot(b, false);
+        if (rc) return rc;
+        const int ns = std::max(8, 8 * reps);
+        auto steps_ms = [&](double& ms_out) -> int {
+            for (int attempt = 0; attempt < 16; attempt++) {
+                int r2 = mb_snapshot(b, true);
+                if (r2) return r2;
+                MB_HIP(b, mb_run_launch(b, b->order_batch, b->nrep, 0, 3, 0, b->stream));
+                MB_HIP(b, hipEventRecord(e0, b->stream));
+                MB_HIP(b, mb_run_launch(b, b->order_batch, b->nrep, ns, 1, 0, b->stream));
+                MB_HIP(b, hipEventRecord(e1, b->stream));
+                MB_HIP(b, hipEventSynchronize(e1));
+                int need = 0;
+                if ((r2 = mb_overflow(b, need))) return r2;
+                if (need) {                       // lists grown: measure again
+                    if ((r2 = mb_grow16(b, need))) return r2;
+                    continue;
+                }
+                float ms = 0.f;
+                MB_HIP(b, hipEventElapsedTime(&ms, e0, e1));
+                ms_out = ms / ns;
+                return MTMD_OK;
+            }
+            return b->fail(MTMD_ENEIGH, "neighbour lists kept overflowing (capacity %d)", b->u.ncap16);
+        };
+        auto pass_ms = [&](int mode, double& ms_out) -> int {
             MB_HIP(b, mb_run_launch(b, b->order_batch, b->nrep, 0, mode, 0, b->stream));
             MB_HIP(b, hipEventRecord(e0, b->stream));
             for (int i = 0; i < reps; i++) MB_HIP(b, mb_run_launch(b, b->order_batch, b->nrep, 0, mode, 0, b->stream));
@@ -1689,36 +1723,35 @@ extern "C" int mtmd_batch_autotune(mtmd_batch* b, int reps, double* force_ms, do
             return MTMD_OK;
         };
         const int bcand[3] = {256, 512, 1024}, gcand[3] = {1, 2, 4};
-        double best = 1e30, bf = 0.0, bl = 0.0;
+        double best = 1e30;
         int bb = b->run_block, bg = b->run_g, bsp = b->run_split;
         for (int split = 1; split <= 2; split++) {
-            // cell shape: the grid changes the lists' order, so one build first
             if (!mb_set_split(b, split, b->rlist)) continue;
-            MB_HIP(b, mb_run_launch(b, b->order_batch, b->nrep, 0, 3, 0, b->stream));
             for (int bs : bcand)
                 for (int g : gcand) {
                     if (mb_run_lds(b, bs) > 160 * 1024) continue;
                     b->run_block = bs;
                     b->run_g = g;
-                    double tf = 0.0, tb = 0.0;
-                    int rc = timed(2, tf);
-                    if (!rc) rc = timed(3, tb);
-                    if (rc) return rc;
-                    const double score = tf + tb / 5.0;
+                    double t = 0.0;
+                    if ((rc = steps_ms(t))) return rc;
                     if (std::getenv("MTMD_BATCH_TUNE_LOG"))
-                        std::fprintf(stderr, "batch tune: cells/radius %d block %d lanes %d force %.4f ms list %.4f ms\n",
-                                     split, bs, g, tf, tb);
-                    if (score < best) { best = score; bb = bs; bg = g; bsp = split; bf = tf; bl = tb; }
+                        std::fprintf(stderr, "batch tune: cells/radius %d block %d lanes %d step %.4f ms\n", split, bs,
+                                     g, t);
+                    if (t < best) { best = t; bb = bs; bg = g; bsp = split; }
                 }
         }
         if (!mb_set_split(b, bsp, b->rlist)) return b->fail(MTMD_EHIP, "cell grid upload failed");
         b->run_block = bb;
         b->run_g = bg;
-        if (force_ms) *force_ms = bf;
-        if (list_ms) *list_ms = bl;
+        double tf = 0.0, tb = 0.0;
+        if ((rc = mb_snapshot(b, true))) return rc;
+        if ((rc = pass_ms(3, tb)) || (rc = pass_ms(2, tf))) return rc;
+        if (force_ms) *force_ms = tf;
+        if (list_ms) *list_ms = tb;
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
-        // the timed passes ran without energies: refresh them for the thermo
+        // the starting state with its lists, forces and energies
+        if ((rc = mb_snapshot(b, true))) return rc;
         return mb_build_fit(b);
     }
     auto time_it = [&](auto&& launch, double& ms_out) -> int {

@@ -316,6 +316,14 @@ def main() -> int:
     # FP64 engine on the same state and the force error of MIXED against it
     pos_m, mom_m = em.get_state()
     F_mixed, U_mixed = em.compute_forces()
+    # the same tiled force pass in FP64 arithmetic (the same lists, staging and
+    # table; FP64 positions, square root and sums): what precision alone costs
+    try:
+        t_tile64 = em.bench_kernel(4, 20) * 1e-3
+        F_tile64, _ = em.get_forces()
+    except Exception as ex:                          # tile mode off, or 32-bit blocks
+        log(f"bench: no FP64 tiled pass ({ex})")
+        t_tile64, F_tile64 = None, None
     em.close()
     c64 = lj_config(abi.ENGINE_FP64, abi.NVE)
     c64.block_size = args.block
@@ -628,7 +636,14 @@ def main() -> int:
         },
         "cpu_baseline": cpu,
         "fp64": {"value": value64, "ms_per_step": dt_64 / K64 * 1e3, "steps": K64,
-                 "force_kernel_ms": t_force64 * 1e3, "mixed_speedup": value / value64},
+                 "force_kernel_ms": t_force64 * 1e3, "mixed_speedup": value / value64,
+                 "tiled_fp64_force_pass": None if t_tile64 is None else {
+                     "kernel_ms": t_tile64 * 1e3, "vs_mixed_force_pass": t_tile64 / t_force,
+                     "vs_fp64_engine_force_pass": t_tile64 / t_force64,
+                     "force_rms_rel_vs_fp64_engine": rms_rel(F_tile64, F64),
+                     "what": "k_fast_force_tile's force-only pass with FP64 arithmetic (block-frame FP64 positions, "
+                             "FP64 rsqrt and sums) on the same 16-bit lists and table: the MIXED/FP64 force-pass "
+                             "ratio at equal design"}},
         "precision": {"force_rms_rel_vs_fp64": force_rms,
                       "vs_oracle": oracle_check,
                       "nve_drift_steps": DRIFT_STEPS,

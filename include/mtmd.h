@@ -369,7 +369,11 @@ int mtmd_synchronize(mtmd_handle* h);
  * over `reps` back-to-back launches. which: 0 pair-force pass, 1 full
  * neighbour-list build, 2 one MD step, 3 in-place read + write of the whole
  * neighbour-list buffer, 4 * capacity bytes per atom each way (FP64/MIXED
- * engines; calibrates the HBM byte counters). */
+ * engines; calibrates the HBM byte counters), 4 the MIXED engine's tiled
+ * force-only pass with FP64 arithmetic (block-frame FP64 positions, FP64
+ * square root and sums, the same lists and table; prices precision alone
+ * beside which = 0; its forces replace the handle's until the next force
+ * call; MIXED tile mode only, no charges). */
 int mtmd_bench_kernel(mtmd_handle* h, int which, int reps, double* avg_ms);
 
 /* CMDLoop::appendToXYZFile (MDLoop/MDLoop.cpp:229-246): appends one frame of

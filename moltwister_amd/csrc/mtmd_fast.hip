@@ -1700,6 +1700,111 @@ k_fast_force_tile(MtDev d, MtParams p)
     if constexpr (FUSE) mt_fast_flag_disp(d, p, d2max);
 }
 
+// FP64 arithmetic on the tiled design, for measurement (mtmd_bench_kernel 4):
+// the force-only pass of k_fast_force_tile without charges, with everything
+// its pair math touches in FP64. Positions are staged in the block frame from
+// the FP64 positions (the cell-local coordinate of mt_fast_local, kept in
+// FP64, plus the staged cell's offset in FP64 cell widths), r comes from an
+// FP64 reciprocal square root, the segment coefficients are the same table's
+// widened to FP64 (in LDS), and forces are summed in FP64. It reads the same
+// 16-bit block lists, so beside k_fast_force_tile it prices the precision
+// alone. Blocks whose staged cells exceed FT_CAP atoms (32-bit lists) are
+// left out and counted in flags[6].
+#define FT64_TPB 256
+__global__ void __launch_bounds__(FT64_TPB) k_fast_force_tile64(MtDev d, MtParams p)
+{
+    __shared__ double4 spos[FT_CAP];                  // block-frame x, y, z; .w the atom's type
+    __shared__ int s_first[64], s_cnt[64], s_off[65];
+    __shared__ int s_hfirst[8], s_hoff[9];
+    extern __shared__ double2 stab64[];              // the coefficient rows (symmetric or T x T) in FP64
+    const int t = threadIdx.x;
+    const int T = d.T;
+    {
+        const int ntab = (d.tabsym ? 2 * T - 1 : T * T) * MT_NPTS;
+        const float2* src = d.tabsym ? d.nbcoef2s : d.nbcoef2;
+        for (int i = t; i < ntab; i += blockDim.x) stab64[i] = make_double2((double)src[i].x, (double)src[i].y);
+    }
+    int x0, y0, z0;
+    mt_tile_block(d, x0, y0, z0);
+    mt_tile_cells(d, x0, y0, z0, s_first, s_cnt, s_off, s_hfirst, s_hoff);
+    __syncthreads();
+    if (s_off[64] > FT_CAP) {                         // uniform: a block of 32-bit lists
+        if (t == 0) atomicAdd(&d.flags[6], 1u);
+        return;
+    }
+    const int Nx = d.nc[0], Ny = d.nc[1], Nz = d.nc[2];
+    const int per = (int)blockDim.x >> 6, q = t / per;
+    {
+        const int ox = (q & 3) - 1, oy = ((q >> 2) & 3) - 1, oz = (q >> 4) - 1;
+        int cx = x0 + ox, cy = y0 + oy, cz = z0 + oz;
+        cx = cx < 0 ? cx + Nx : (cx >= Nx ? cx - Nx : cx);
+        cy = cy < 0 ? cy + Ny : (cy >= Ny ? cy - Ny : cy);
+        cz = cz < 0 ? cz + Nz : (cz >= Nz ? cz - Nz : cz);
+        const double ccx = -p.L[0] / 2.0 + ((double)cx + 0.5) * p.cw[0];
+        const double ccy = -p.L[1] / 2.0 + ((double)cy + 0.5) * p.cw[1];
+        const double ccz = -p.L[2] / 2.0 + ((double)cz + 0.5) * p.cw[2];
+        const double sx = (double)ox * p.cw[0], sy = (double)oy * p.cw[1], sz = (double)oz * p.cw[2];
+        for (int m = t % per; m < s_cnt[q]; m += per) {
+            const int u = s_first[q] + m, k = d.cell_list[u];
+            const double4 r = d.pos[k];
+            double dx = r.x - ccx, dy = r.y - ccy, dz = r.z - ccz;
+            dx -= p.L[0] * rint(dx * p.invL[0]);
+            dy -= p.L[1] * rint(dy * p.invL[1]);
+            dz -= p.L[2] * rint(dz * p.invL[2]);
+            spos[s_off[q] + m] = make_double4(dx + sx, dy + sy, dz + sz, (double)d.type[k]);
+        }
+    }
+    __syncthreads();
+    const int nh = s_hoff[8];
+    const double x0d = (double)p.x0, invd = (double)p.inv_delta;
+    const mt_u32x4* __restrict__ nl = reinterpret_cast<const mt_u32x4*>(d.nlist);
+    for (int v = t; v < nh; v += (int)blockDim.x) {
+        const int h = mt_tile_home(s_hoff, v);
+        const int u = s_hfirst[h] + (v - s_hoff[h]);
+        const int k = d.cell_list[u];
+        const int qh = (1 + (h & 1)) + 4 * ((1 + ((h >> 1) & 1)) + 4 * (1 + (h >> 2)));
+        const double4 pk = spos[s_off[qh] + (v - s_hoff[h])];
+        const int tk = (int)pk.w, rowbase = d.tabsym ? tk : tk * T;
+        const int nch = (min(d.ncount[k], 2 * d.ncap) + 7) >> 3;
+        double Fx = 0.0, Fy = 0.0, Fz = 0.0;
+        for (int c = 0; c < nch; c++) {
+            const mt_u32x4 w = __builtin_nontemporal_load(nl + (size_t)c * d.n + k);
+            const int li[8] = {(int)(w.x & 0xffffu), (int)(w.x >> 16), (int)(w.y & 0xffffu), (int)(w.y >> 16),
+                               (int)(w.z & 0xffffu), (int)(w.z >> 16), (int)(w.w & 0xffffu), (int)(w.w >> 16)};
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+                const double4 a = spos[li[i]];
+                const double dx = pk.x - a.x, dy = pk.y - a.y, dz = pk.z - a.z;
+                const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+                // the atom itself (row padding): rinv 0, below the table, no force
+                const double rinv = r2 > 0.0 ? rsqrt(r2) : 0.0;
+                const int idx = (int)((r2 * rinv - x0d) * invd);
+                const int seg = (int)min((unsigned)idx, (unsigned)(MT_NPTS - 1));   // beyond rc or below x0: zero row end
+                const double2 cf = stab64[(rowbase + (int)a.w) * MT_NPTS + seg];
+                const double fr = fma(cf.y, rinv, cf.x);
+                Fx = fma(dx, fr, Fx);
+                Fy = fma(dy, fr, Fy);
+                Fz = fma(dz, fr, Fz);
+            }
+        }
+        if (!d.mobile[k]) Fx = Fy = Fz = 0.0;
+        d.frc[k] = make_double4(Fx, Fy, Fz, 0.0);
+    }
+}
+
+int mt_fast_force_tile64(const MtDev& d, const MtParams& p, hipStream_t s)
+{
+    if (!d.tile16 || d.T < 1) return -1;
+    const int nb = ((d.nc[0] + 1) / 2) * ((d.nc[1] + 1) / 2) * ((d.nc[2] + 1) / 2);
+    const size_t lds = (size_t)(d.tabsym ? 2 * d.T - 1 : d.T * d.T) * MT_NPTS * sizeof(double2);
+    if (lds + FT_CAP * sizeof(double4) + 2048 > 160 * 1024) return -1;
+    if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fast_force_tile64),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
+        return -1;
+    hipLaunchKernelGGL(k_fast_force_tile64, dim3(nb), dim3(FT64_TPB), lds, s, d, p);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 // FP64: the reference's per-pair arithmetic (FunctorCalcForce.cpp:467-535)
 // on the skin list, restricted to pairs inside rc by the reference predicate.
 // G lanes per atom: lane g takes the list's 4-word chunks g, g + G, ...

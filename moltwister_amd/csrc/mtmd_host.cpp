@@ -2002,8 +2002,10 @@ extern "C" int mtmd_synchronize(mtmd_handle* h)
 
 extern "C" int mtmd_bench_kernel(mtmd_handle* h, int which, int reps, double* avg_ms)
 {
-    if (!h || reps < 1 || !avg_ms || which < 0 || which > 3) return MTMD_EINVAL;
+    if (!h || reps < 1 || !avg_ms || which < 0 || which > 4) return MTMD_EINVAL;
     if (which == 3 && !h->fast) return h->fail(MTMD_EINVAL, "the copy reference needs the fp64/mixed engine");
+    if (which == 4 && !(h->fast && h->engine == MTMD_ENGINE_MIXED && h->d.tile16 && !h->p.coulomb_on))
+        return h->fail(MTMD_EINVAL, "the FP64 tiled pass needs the mixed engine in tile mode, without charges");
     int rc;
     if (!h->have_forces || (h->fast && h->need_sort)) {
         if ((rc = compute_forces_now(h))) return rc;
@@ -2021,6 +2023,9 @@ extern "C" int mtmd_bench_kernel(mtmd_handle* h, int which, int reps, double* av
             else { mt_exact_cells(h->d, h->p, h->stream); mt_exact_neighbors(h->d, h->p, h->stream); rc = MTMD_OK; }
         } else if (which == 2) {
             rc = md_steps(h, 1, false);
+        } else if (which == 4) {
+            rc = mt_fast_force_tile64(h->d, h->p, h->stream) ? h->fail(MTMD_EHIP, "FP64 tiled pass launch failed")
+                                                             : MTMD_OK;
         } else {
             mt_copy_lists(h->d, h->stream);
             rc = MTMD_OK;
@@ -2034,6 +2039,14 @@ extern "C" int mtmd_bench_kernel(mtmd_handle* h, int which, int reps, double* av
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (rc) return rc;
+    if (which == 4) {
+        unsigned int skipped = 0;
+        HIPCHK(h, hipMemcpy(&skipped, h->d.flags + 6, sizeof skipped, hipMemcpyDeviceToHost));
+        if (skipped) {
+            HIPCHK(h, hipMemset(h->d.flags + 6, 0, sizeof skipped));
+            return h->fail(MTMD_EINVAL, "the FP64 tiled pass left out %u blocks of 32-bit lists", skipped / reps);
+        }
+    }
     *avg_ms = (double)ms / reps;
     return MTMD_OK;
 }

@@ -181,6 +181,9 @@ void mt_fast_neighbors(const MtDev& d, const MtParams& p, int conditional, hipSt
 // cell widths of box L within the FP16 range of the list build's screen);
 // the host stores it in d.tile16 whenever the cell grid is set, before a build
 int mt_fast_tile_mode(const MtDev& d, const double* L, int mixed);
+// the tiled force-only pass in FP64 arithmetic (measurement); -1 when the
+// tile mode is off or its LDS does not fit
+int mt_fast_force_tile64(const MtDev& d, const MtParams& p, hipStream_t s);
 void mt_fast_forces(const MtDev& d, const MtParams& p, int mixed, int want_energy, int block, hipStream_t s);
 // 1 when the NVE step may run as one fused pass (mt_fast_forces_fused): the
 // MIXED tile mode with every force of the step inside that pass

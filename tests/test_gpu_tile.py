@@ -77,6 +77,19 @@ def test_tile_forces_energies_ucut(mixture, skin):
     em.bench_kernel(0, 1)
     F0, _ = em.get_forces()
     assert np.abs(F0 - Fm).max() <= 1e-12 * np.abs(Fm).max(), np.abs(F0 - Fm).max()
+    # the measurement pass with FP64 arithmetic on the same lists: the FP64
+    # engine's forces up to the FP32 segment coefficients a_f + b_f / r that
+    # both tiled passes use (about 1e-6 against the reference's interpolation),
+    # below MIXED's error; 32-bit blocks refused
+    if share16 == 1.0:
+        em.bench_kernel(4, 1)
+        F64t, _ = em.get_forces()
+        assert rms_rel(F64t, Ff) < 2e-6 and rms_rel(F64t, Ff) < rms_rel(Fm, Ff), (rms_rel(F64t, Ff), rms_rel(Fm, Ff))
+        em.bench_kernel(0, 1)
+    else:
+        with pytest.raises(Exception, match="32-bit"):
+            em.bench_kernel(4, 1)
+        em.bench_kernel(0, 1)
     # U_cut: the same pairs inside rc by the same FP64 predicate
     tm, tf = em.thermo(), ef.thermo()
     assert tm["U_cut"] == pytest.approx(tf["U_cut"], rel=1e-12)

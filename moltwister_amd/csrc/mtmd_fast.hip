@@ -1706,23 +1706,26 @@ k_fast_force_tile(MtDev d, MtParams p)
 // the FP64 positions (the cell-local coordinate of mt_fast_local, kept in
 // FP64, plus the staged cell's offset in FP64 cell widths), r comes from an
 // FP64 reciprocal square root, the segment coefficients are the same table's
-// widened to FP64 (in LDS), and forces are summed in FP64. It reads the same
-// 16-bit block lists, so beside k_fast_force_tile it prices the precision
-// alone. Blocks whose staged cells exceed FT_CAP atoms (32-bit lists) are
-// left out and counted in flags[6].
-#define FT64_TPB 256
+// (FP32 in LDS, widened at use), and forces are summed in FP64. Two lanes an
+// atom take alternate chunks of the same 16-bit block lists, as the FP32 pass
+// does, so beside k_fast_force_tile it prices the precision alone. Blocks
+// whose staged cells exceed FT_CAP atoms (32-bit lists) are left out and
+// counted in flags[6].
+#define FT64_TPB 512
+#define FT64_G 2
 __global__ void __launch_bounds__(FT64_TPB) k_fast_force_tile64(MtDev d, MtParams p)
 {
-    __shared__ double4 spos[FT_CAP];                  // block-frame x, y, z; .w the atom's type
+    __shared__ double sx[FT_CAP], sy[FT_CAP], sz[FT_CAP];   // block-frame positions
+    __shared__ unsigned char sty[FT_CAP];                   // types
     __shared__ int s_first[64], s_cnt[64], s_off[65];
     __shared__ int s_hfirst[8], s_hoff[9];
-    extern __shared__ double2 stab64[];              // the coefficient rows (symmetric or T x T) in FP64
+    extern __shared__ float2 stab[];                        // the coefficient rows (symmetric or T x T)
     const int t = threadIdx.x;
     const int T = d.T;
     {
         const int ntab = (d.tabsym ? 2 * T - 1 : T * T) * MT_NPTS;
         const float2* src = d.tabsym ? d.nbcoef2s : d.nbcoef2;
-        for (int i = t; i < ntab; i += blockDim.x) stab64[i] = make_double2((double)src[i].x, (double)src[i].y);
+        for (int i = t; i < ntab; i += blockDim.x) stab[i] = src[i];
     }
     int x0, y0, z0;
     mt_tile_block(d, x0, y0, z0);
@@ -1743,50 +1746,58 @@ __global__ void __launch_bounds__(FT64_TPB) k_fast_force_tile64(MtDev d, MtParam
         const double ccx = -p.L[0] / 2.0 + ((double)cx + 0.5) * p.cw[0];
         const double ccy = -p.L[1] / 2.0 + ((double)cy + 0.5) * p.cw[1];
         const double ccz = -p.L[2] / 2.0 + ((double)cz + 0.5) * p.cw[2];
-        const double sx = (double)ox * p.cw[0], sy = (double)oy * p.cw[1], sz = (double)oz * p.cw[2];
+        const double ox_ = (double)ox * p.cw[0], oy_ = (double)oy * p.cw[1], oz_ = (double)oz * p.cw[2];
         for (int m = t % per; m < s_cnt[q]; m += per) {
-            const int u = s_first[q] + m, k = d.cell_list[u];
+            const int u = s_first[q] + m, k = d.cell_list[u], i = s_off[q] + m;
             const double4 r = d.pos[k];
             double dx = r.x - ccx, dy = r.y - ccy, dz = r.z - ccz;
             dx -= p.L[0] * rint(dx * p.invL[0]);
             dy -= p.L[1] * rint(dy * p.invL[1]);
             dz -= p.L[2] * rint(dz * p.invL[2]);
-            spos[s_off[q] + m] = make_double4(dx + sx, dy + sy, dz + sz, (double)d.type[k]);
+            sx[i] = dx + ox_;
+            sy[i] = dy + oy_;
+            sz[i] = dz + oz_;
+            sty[i] = (unsigned char)d.type[k];
         }
     }
     __syncthreads();
     const int nh = s_hoff[8];
     const double x0d = (double)p.x0, invd = (double)p.inv_delta;
     const mt_u32x4* __restrict__ nl = reinterpret_cast<const mt_u32x4*>(d.nlist);
-    for (int v = t; v < nh; v += (int)blockDim.x) {
+    const int gl = t % FT64_G;
+    for (int v = t / FT64_G; v < nh; v += (int)blockDim.x / FT64_G) {
         const int h = mt_tile_home(s_hoff, v);
         const int u = s_hfirst[h] + (v - s_hoff[h]);
         const int k = d.cell_list[u];
         const int qh = (1 + (h & 1)) + 4 * ((1 + ((h >> 1) & 1)) + 4 * (1 + (h >> 2)));
-        const double4 pk = spos[s_off[qh] + (v - s_hoff[h])];
-        const int tk = (int)pk.w, rowbase = d.tabsym ? tk : tk * T;
+        const int pl = s_off[qh] + (v - s_hoff[h]);
+        const double px = sx[pl], py = sy[pl], pz = sz[pl];
+        const int tk = sty[pl], rowbase = (d.tabsym ? tk : tk * T) * MT_NPTS;
         const int nch = (min(d.ncount[k], 2 * d.ncap) + 7) >> 3;
         double Fx = 0.0, Fy = 0.0, Fz = 0.0;
-        for (int c = 0; c < nch; c++) {
+        float U = 0.0f;
+        for (int c = gl; c < nch; c += FT64_G) {
             const mt_u32x4 w = __builtin_nontemporal_load(nl + (size_t)c * d.n + k);
             const int li[8] = {(int)(w.x & 0xffffu), (int)(w.x >> 16), (int)(w.y & 0xffffu), (int)(w.y >> 16),
                                (int)(w.z & 0xffffu), (int)(w.z >> 16), (int)(w.w & 0xffffu), (int)(w.w >> 16)};
 #pragma unroll
             for (int i = 0; i < 8; i++) {
-                const double4 a = spos[li[i]];
-                const double dx = pk.x - a.x, dy = pk.y - a.y, dz = pk.z - a.z;
+                const int l = li[i];
+                const double dx = px - sx[l], dy = py - sy[l], dz = pz - sz[l];
                 const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
                 // the atom itself (row padding): rinv 0, below the table, no force
                 const double rinv = r2 > 0.0 ? rsqrt(r2) : 0.0;
                 const int idx = (int)((r2 * rinv - x0d) * invd);
                 const int seg = (int)min((unsigned)idx, (unsigned)(MT_NPTS - 1));   // beyond rc or below x0: zero row end
-                const double2 cf = stab64[(rowbase + (int)a.w) * MT_NPTS + seg];
-                const double fr = fma(cf.y, rinv, cf.x);
+                const float2 cf = stab[rowbase + (int)sty[l] * MT_NPTS + seg];
+                const double fr = fma((double)cf.y, rinv, (double)cf.x);
                 Fx = fma(dx, fr, Fx);
                 Fy = fma(dy, fr, Fy);
                 Fz = fma(dz, fr, Fz);
             }
         }
+        mt_lane_sum<FT64_G>(Fx, Fy, Fz, U);
+        if (gl != 0) continue;
         if (!d.mobile[k]) Fx = Fy = Fz = 0.0;
         d.frc[k] = make_double4(Fx, Fy, Fz, 0.0);
     }
@@ -1794,10 +1805,10 @@ __global__ void __launch_bounds__(FT64_TPB) k_fast_force_tile64(MtDev d, MtParam
 
 int mt_fast_force_tile64(const MtDev& d, const MtParams& p, hipStream_t s)
 {
-    if (!d.tile16 || d.T < 1) return -1;
+    if (!d.tile16 || d.T < 1 || d.T > 255) return -1;
     const int nb = ((d.nc[0] + 1) / 2) * ((d.nc[1] + 1) / 2) * ((d.nc[2] + 1) / 2);
-    const size_t lds = (size_t)(d.tabsym ? 2 * d.T - 1 : d.T * d.T) * MT_NPTS * sizeof(double2);
-    if (lds + FT_CAP * sizeof(double4) + 2048 > 160 * 1024) return -1;
+    const size_t lds = (size_t)(d.tabsym ? 2 * d.T - 1 : d.T * d.T) * MT_NPTS * sizeof(float2);
+    if (lds + FT_CAP * (3 * sizeof(double) + 1) + 2048 > 160 * 1024) return -1;
     if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fast_force_tile64),
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
         return -1;

@@ -242,8 +242,8 @@ def main() -> int:
                          "mixed and FP64 engines; for profiler runs")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described in DESIGN.md "
-                                   "and not implemented", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this engine runs on one GPU; sharding a system over several GPUs is a "
+                                   "design only, not implemented", "n_gpus": args.gpus}))
         return 2
 
     from moltwister_amd import abi, workloads as W

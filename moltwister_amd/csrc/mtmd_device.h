@@ -476,23 +476,27 @@ __device__ inline double mt_nh_propagate_fixed(double* eta_g, double* peta_g, co
 // g T for g / beta) and each scale factor is computed once and applied twice.
 // CMt::exp keeps its linear branch for x > 1; its small arguments (|x| < 1/32,
 // the chain's usual range, |x| = Dt |p_eta| / (8 Q)) take the Taylor series
-// of exp to x^8, whose truncation is below x^9 / 9! < 1e-19.
-__device__ inline double mt_cmt_exp_fast(double x)
+// of exp to x^8, whose truncation is below x^9 / 9! < 1e-19, evaluated by
+// Estrin's scheme (about half the dependent steps of Horner's rule: the chain
+// runs on one lane, where latency is its cost).
+__device__ inline double mt_exp_taylor8(double x)
 {
-    if (x > 1.0) return 2.71828 * x;
-    if (fabs(x) < 0.03125) {
-        double t = 1.0 / 40320.0;
-        t = fma(t, x, 1.0 / 5040.0);
-        t = fma(t, x, 1.0 / 720.0);
-        t = fma(t, x, 1.0 / 120.0);
-        t = fma(t, x, 1.0 / 24.0);
-        t = fma(t, x, 1.0 / 6.0);
-        t = fma(t, x, 0.5);
-        t = fma(t, x, 1.0);
-        return fma(t, x, 1.0);
-    }
-    return exp(x);
+    const double x2 = x * x, x4 = x2 * x2;
+    const double p01 = 1.0 + x, p23 = fma(x, 1.0 / 6.0, 0.5);
+    const double p45 = fma(x, 1.0 / 120.0, 1.0 / 24.0), p67 = fma(x, 1.0 / 5040.0, 1.0 / 720.0);
+    const double q0 = fma(x2, p23, p01), q1 = fma(x2, p67, p45);
+    return fma(x4, fma(x4, 1.0 / 40320.0, q1), q0);
 }
+
+// CMt::exp from its series value t = mt_exp_taylor8(x), which holds for
+// |x| < 1/32; outside, the linear branch (x > 1) or exp
+__device__ inline double mt_cmt_exp_from(double x, double t)
+{
+    if (fabs(x) < 0.03125) return t;
+    return x > 1.0 ? 2.71828 * x : exp(x);
+}
+
+__device__ inline double mt_cmt_exp_fast(double x) { return mt_cmt_exp_from(x, mt_exp_taylor8(x)); }
 
 template <int M>
 __device__ inline double mt_nh_propagate_fast(double* eta_g, double* peta_g, const double* Q_g, int nresp, double T,
@@ -514,11 +518,18 @@ __device__ inline double mt_nh_propagate_fast(double* eta_g, double* peta_g, con
         for (int i = 0; i < nresp; i++) {
 #define MT_GF(j) ((j) == 0 ? (KE2 - gT) : (peta[(j) - 1] * peta[(j) - 1] * iQ[(j) - 1] - T))
             peta[M - 1] += q4 * MT_GF(M - 1);
+            // the forward sweep's factors depend on links not yet updated in
+            // it (and the last link, updated just above): all at once
+            double c[M > 1 ? M - 1 : 1];
 #pragma unroll
             for (int j = 0; j < M - 1; j++) {
-                const double c = mt_cmt_exp_fast(-q8 * (peta[j + 1] * iQ[j + 1]));
-                peta[j] = (peta[j] * c + q4 * MT_GF(j)) * c;
+                const double x = -q8 * (peta[j + 1] * iQ[j + 1]);
+                c[j] = mt_exp_taylor8(x);
             }
+#pragma unroll
+            for (int j = 0; j < M - 1; j++) c[j] = mt_cmt_exp_from(-q8 * (peta[j + 1] * iQ[j + 1]), c[j]);
+#pragma unroll
+            for (int j = 0; j < M - 1; j++) peta[j] = (peta[j] * c[j] + q4 * MT_GF(j)) * c[j];
 #pragma unroll
             for (int j = 0; j < M; j++) eta[j] += q2 * (peta[j] * iQ[j]);
             const double c0 = mt_cmt_exp_fast(-(q2 * (peta[0] * iQ[0])));
@@ -526,8 +537,8 @@ __device__ inline double mt_nh_propagate_fast(double* eta_g, double* peta_g, con
             KE2 *= c0 * c0;
 #pragma unroll
             for (int j = M - 2; j >= 0; j--) {
-                const double c = mt_cmt_exp_fast(-q8 * (peta[j + 1] * iQ[j + 1]));
-                peta[j] = (peta[j] * c + q4 * MT_GF(j)) * c;
+                const double cj = mt_cmt_exp_fast(-q8 * (peta[j + 1] * iQ[j + 1]));
+                peta[j] = (peta[j] * cj + q4 * MT_GF(j)) * cj;
             }
             peta[M - 1] += q4 * MT_GF(M - 1);
 #undef MT_GF

@@ -374,14 +374,27 @@ __global__ void k_nh_chain(double* nh, const double* red, int nblocks, int M, in
                            double g, double w0, double w1, double w2, int fast)
 {
     __shared__ double sh[64];
+    __shared__ double snh[3 * 8];
+    const double w[3] = {w0, w1, w2};
     double s = 0.0;
+    if (fast && M <= 8) {
+        // FP64/MIXED (trace agreement): the chain state staged in LDS while
+        // the partials load, the partials summed by a butterfly of shuffles
+        for (int i = threadIdx.x; i < 3 * M; i += 64) snh[i] = nh[i];
+        for (int b = threadIdx.x; b < nblocks; b += 64) s += red[b];
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+        __syncthreads();
+        if (threadIdx.x != 0) return;
+        nh[3 * M] = mt_nh_step(snh, snh + M, snh + 2 * M, M, nresp, T, dt, g, w, s, true);
+        for (int i = 0; i < 2 * M; i++) nh[i] = snh[i];
+        return;
+    }
     for (int b = threadIdx.x; b < nblocks; b += 64) s += red[b];
     sh[threadIdx.x] = s;
     __syncthreads();
     if (threadIdx.x != 0) return;
     double KE2 = 0.0;
     for (int t = 0; t < 64; t++) KE2 += sh[t];
-    const double w[3] = {w0, w1, w2};
     nh[3 * M] = mt_nh_step(nh, nh + M, nh + 2 * M, M, nresp, T, dt, g, w, KE2, fast != 0);
 }
 

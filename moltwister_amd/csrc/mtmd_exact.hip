@@ -370,23 +370,35 @@ void mt_cap_kick(const MtDev& d, const MtParams& p, int want_ke, hipStream_t s)
 // Nose-Hoover chain propagator (NHChain.cpp:54-113) with CFctT (MDLoop.cpp:29-60),
 // mt_nh_propagate in mtmd_device.h; one chain, nh layout: eta[M], p_eta[M],
 // Q[M], scale. Sums the block partials of p.p/m first.
+// fast (FP64/MIXED, trace agreement): the chain state staged in LDS while the
+// partials load, the partials summed by a butterfly of shuffles; with
+// from_slot the kinetic energy is nh[3M + 1] instead of the partials. The
+// kinetic energy after this scale, KE S^2, goes to nh[3M + 1] for the next
+// propagation.
 __global__ void k_nh_chain(double* nh, const double* red, int nblocks, int M, int nresp, double T, double dt,
-                           double g, double w0, double w1, double w2, int fast)
+                           double g, double w0, double w1, double w2, int fast, int from_slot)
 {
     __shared__ double sh[64];
     __shared__ double snh[3 * 8];
     const double w[3] = {w0, w1, w2};
     double s = 0.0;
-    if (fast && M <= 8) {
-        // FP64/MIXED (trace agreement): the chain state staged in LDS while
-        // the partials load, the partials summed by a butterfly of shuffles
-        for (int i = threadIdx.x; i < 3 * M; i += 64) snh[i] = nh[i];
-        for (int b = threadIdx.x; b < nblocks; b += 64) s += red[b];
-        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (fast) {
+        const bool staged = M <= 8;
+        if (staged)
+            for (int i = threadIdx.x; i < 3 * M; i += 64) snh[i] = nh[i];
+        if (!from_slot) {
+            for (int b = threadIdx.x; b < nblocks; b += 64) s += red[b];
+            for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+        }
         __syncthreads();
         if (threadIdx.x != 0) return;
-        nh[3 * M] = mt_nh_step(snh, snh + M, snh + 2 * M, M, nresp, T, dt, g, w, s, true);
-        for (int i = 0; i < 2 * M; i++) nh[i] = snh[i];
+        if (from_slot) s = nh[3 * M + 1];
+        double* c = staged ? snh : nh;
+        const double S = mt_nh_step(c, c + M, c + 2 * M, M, nresp, T, dt, g, w, s, true);
+        if (staged)
+            for (int i = 0; i < 2 * M; i++) nh[i] = snh[i];
+        nh[3 * M] = S;
+        nh[3 * M + 1] = s * S * S;
         return;
     }
     for (int b = threadIdx.x; b < nblocks; b += 64) s += red[b];
@@ -398,11 +410,49 @@ __global__ void k_nh_chain(double* nh, const double* red, int nblocks, int M, in
     nh[3 * M] = mt_nh_step(nh, nh + M, nh + 2 * M, M, nresp, T, dt, g, w, KE2, fast != 0);
 }
 
+// FP64/MIXED, between two steps of one call: the chain at the end of step t
+// on the summed p.p/m after the kick (its scale Sb into nh[3M]) and, at once,
+// the chain at the start of step t + 1 on that sum times Sb^2, the kinetic
+// energy after Sb is applied (its scale Sf into nh[3M + 1]); the next front
+// half applies Sb and Sf. The second propagation's kinetic energy is formed
+// analytically rather than summed again, as k_nh_chain's from_slot takes it
+// when a call ends and the next begins, so the steps of one call and the same
+// steps in separate calls give the same bits.
+__global__ void k_nh_chain2(double* nh, const double* red, int nblocks, int M, int nresp, double T, double dt,
+                            double g, double w0, double w1, double w2)
+{
+    __shared__ double snh[3 * 8];
+    const double w[3] = {w0, w1, w2};
+    const bool staged = M <= 8;
+    if (staged)
+        for (int i = threadIdx.x; i < 3 * M; i += 64) snh[i] = nh[i];
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nblocks; b += 64) s += red[b];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    double* c = staged ? snh : nh;
+    const double Sb = mt_nh_step(c, c + M, c + 2 * M, M, nresp, T, dt, g, w, s, true);
+    const double ke = s * Sb * Sb;
+    const double Sf = mt_nh_step(c, c + M, c + 2 * M, M, nresp, T, dt, g, w, ke, true);
+    if (staged)
+        for (int i = 0; i < 2 * M; i++) nh[i] = snh[i];
+    nh[3 * M] = Sb;
+    nh[3 * M + 1] = Sf;
+}
+
+void mt_nh_chain2(const MtDev& d, int M, int nresp, double T, double dt, double g, const double* w, int nblocks,
+                  hipStream_t s)
+{
+    hipLaunchKernelGGL(k_nh_chain2, dim3(1), dim3(64), 0, s, d.nh, d.red, nblocks, M, nresp, T, dt, g, w[0], w[1],
+                       w[2]);
+}
+
 void mt_nh_chain(const MtDev& d, int M, int nresp, double T, double dt, double g, const double* w, int nblocks,
-                 hipStream_t s, int fast)
+                 hipStream_t s, int fast, int from_slot)
 {
     hipLaunchKernelGGL(k_nh_chain, dim3(1), dim3(64), 0, s, d.nh, d.red, nblocks, M, nresp, T, dt, g, w[0], w[1],
-                       w[2], fast);
+                       w[2], fast, from_slot);
 }
 
 // p *= S; pbcWrap (SimulationBox.cpp:135-166); optional partials of p.p/m

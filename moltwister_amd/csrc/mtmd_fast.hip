@@ -2444,6 +2444,42 @@ void mt_fast_kick_drift(const MtDev& d, MtParams& p, int scale_slot, int mixed, 
     p.dslot ^= 1;
 }
 
+// NVT between two steps of one call: the end of step t (momenta scaled by
+// nh[sb], positions wrapped as k_scale_wrap wraps them, SimulationBox.cpp:135-166)
+// and the front half of step t + 1 (momenta scaled by nh[sf], kick, drift),
+// the same operations in the same order as k_scale_wrap then k_fast_kick_drift
+__global__ void __launch_bounds__(TPB) k_fast_scale_kick_drift(MtDev d, MtParams p, int sb, int sf, int mixed)
+{
+    MT_EXACT_FP
+    MT_WRITER_CLEAR(d, p);
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    float d2 = 0.0f;
+    if (k < d.n) {
+        double4 pm = d.mom[k], r = d.pos[k];
+        const double4 F = d.frc[k];
+        const double X2 = p.L[0] / 2.0, Y2 = p.L[1] / 2.0, Z2 = p.L[2] / 2.0;
+        if (r.x >= X2) mt_pbc_add(r.x, X2, p.L[0]);
+        if (r.x < -X2) mt_pbc_add(r.x, -X2, p.L[0]);
+        if (r.y >= Y2) mt_pbc_add(r.y, Y2, p.L[1]);
+        if (r.y < -Y2) mt_pbc_add(r.y, -Y2, p.L[1]);
+        if (r.z >= Z2) mt_pbc_add(r.z, Z2, p.L[2]);
+        if (r.z < -Z2) mt_pbc_add(r.z, -Z2, p.L[2]);
+        const double Sb = d.nh[sb], Sf = d.nh[sf];
+        pm.x *= Sb; pm.y *= Sb; pm.z *= Sb;
+        pm.x *= Sf; pm.y *= Sf; pm.z *= Sf;
+        d2 = mt_fast_front(d, p, k, pm, r, F, mixed);
+        d.mom[k] = pm;
+        d.pos[k] = r;
+    }
+    mt_fast_flag_disp(d, p, d2);
+}
+
+void mt_fast_scale_kick_drift(const MtDev& d, MtParams& p, int sb, int sf, int mixed, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fast_scale_kick_drift, dim3(nblk(d.n, TPB)), dim3(TPB), 0, s, d, p, sb, sf, mixed);
+    p.dslot ^= 1;
+}
+
 // ---------------------------------------------------------------------------
 // U_cut = sum over the pairs inside rc of U(rc) of the pair's force fields,
 // scaled like the pair. The reference's pair potentials are truncated without

@@ -92,6 +92,8 @@ struct mtmd_handle {
 
     bool have_forces = false;     // forces of the current positions exist
     bool ke_valid = false;        // d.red holds per-block sums of p.p/m of the current momenta
+    bool ke_slot = false;         // FP64/MIXED NVT: nh[3M + 1] holds p.p/m of the current momenta (formed by
+                                  // the last chain propagation), taken by the next call's first chain
     int64_t rebuilds = 0, steps = 0;
 
     // FP64/MIXED engines
@@ -153,6 +155,7 @@ struct mtmd_handle {
         std::vector<double> etaP, petaP;
         int64_t steps = 0;
         int steps_since_sort = 0;
+        bool ke_slot = false;
     } snap;
     void drop_graphs()
     {
@@ -1013,6 +1016,7 @@ extern "C" int mtmd_set_state(mtmd_handle* h, const double* pos, const double* m
         for (int k = 0; k < n; k++) { tmp[k].x = mom[3 * k]; tmp[k].y = mom[3 * k + 1]; tmp[k].z = mom[3 * k + 2]; }
         HIPCHK(h, hipMemcpy(h->d.mom, tmp.data(), n * sizeof(double4), hipMemcpyHostToDevice));
         h->ke_valid = false;
+        h->ke_slot = false;
     }
     return MTMD_OK;
 }
@@ -1541,6 +1545,7 @@ static int md_steps_once(mtmd_handle* h, int nsteps, bool energy_last)
     }
     const int mixed = h->engine == MTMD_ENGINE_MIXED;
     if (h->npt) {
+        h->ke_slot = false;                             // (the barostat scales the momenta too)
         if ((rc = npt_steps(h, nsteps, energy_last))) return rc;
     } else if (!h->fast) {
         for (int t = 0; t < nsteps; t++) {
@@ -1561,14 +1566,29 @@ static int md_steps_once(mtmd_handle* h, int nsteps, bool energy_last)
         // half of step t + 1 run as one kernel. Plain steps (no re-sort, not
         // the last of the call) replay a captured graph of their kernels.
         auto nvt_front = [&]() {
-            if (!h->ke_valid) mt_reduce_ke(d, h->stream);
-            mt_nh_chain(d, h->M, h->nresp, h->Tred, h->p.dt, h->g, h->w, nblocks, h->stream, h->fast ? 1 : 0);
+            // the kinetic energy the last call's final chain formed, or summed
+            // anew when the momenta were set since
+            if (h->ke_slot) {
+                mt_nh_chain(d, h->M, h->nresp, h->Tred, h->p.dt, h->g, h->w, nblocks, h->stream, 1, 1);
+            } else {
+                if (!h->ke_valid) mt_reduce_ke(d, h->stream);
+                mt_nh_chain(d, h->M, h->nresp, h->Tred, h->p.dt, h->g, h->w, nblocks, h->stream, 1, 0);
+            }
             mt_fast_kick_drift(d, h->p, h->nh_scale_slot, mixed, h->stream);
         };
         auto nvt_back = [&]() {
             mt_cap_kick(d, h->p, 1, h->stream);
             mt_nh_chain(d, h->M, h->nresp, h->Tred, h->p.dt, h->g, h->w, nblocks, h->stream, h->fast ? 1 : 0);
             mt_scale_wrap(d, h->p, h->nh_scale_slot, 1, h->stream);
+        };
+        // NVT between two steps of the call: the back half of step t and the
+        // front half of t + 1, both chain propagations in one launch and the
+        // scale, wrap, kick and drift in one pass (a call's first front half
+        // and last back half run as above)
+        auto nvt_mid = [&]() {
+            mt_cap_kick(d, h->p, 1, h->stream);
+            mt_nh_chain2(d, h->M, h->nresp, h->Tred, h->p.dt, h->g, h->w, nblocks, h->stream);
+            mt_fast_scale_kick_drift(d, h->p, h->nh_scale_slot, h->nh_scale_slot + 1, mixed, h->stream);
         };
         // NVE in the MIXED tile mode with every force inside the pair pass:
         // the pair pass also integrates (back half of step t, front half of
@@ -1579,9 +1599,9 @@ static int md_steps_once(mtmd_handle* h, int nsteps, bool energy_last)
             std::swap(d.pos, d.pos2);
             std::swap(d.posf, d.posf2);
         };
-        // the kernels of one plain step after its front half (NVE) or whole (NVT)
+        // the kernels of one plain step after its front half, through the
+        // front half of the next
         auto plain_step = [&]() -> int {
-            if (nvt) nvt_front();
             int r = fast_rebuild(h, false, false);
             if (r) return r;
             if (fuse) {
@@ -1589,7 +1609,7 @@ static int md_steps_once(mtmd_handle* h, int nsteps, bool energy_last)
                 return MTMD_OK;
             }
             if ((r = fast_forces(h, false))) return r;
-            if (nvt) nvt_back();
+            if (nvt) nvt_mid();
             else mt_fast_nve_fused(d, h->p, mixed, h->stream);
             return MTMD_OK;
         };
@@ -1632,12 +1652,13 @@ static int md_steps_once(mtmd_handle* h, int nsteps, bool energy_last)
         };
         // this call's first front half finds both rebuild-flag slots clear
         mt_fast_clear_flags(d, h->stream);
-        if (!nvt) mt_fast_kick_drift(d, h->p, -1, mixed, h->stream);
+        if (nvt) nvt_front();
+        else mt_fast_kick_drift(d, h->p, -1, mixed, h->stream);
         constexpr int KB = mtmd_handle::kBlockSteps;
         for (int t = 0; t < nsteps; t++) {
             const bool last = (t == nsteps - 1);
             // a block of KB plain steps, none of them a re-sort or the last
-            if (h->use_graphs && t + KB < nsteps && h->steps_since_sort + KB < h->sort_every && (!nvt || h->ke_valid)) {
+            if (h->use_graphs && t + KB < nsteps && h->steps_since_sort + KB < h->sort_every) {
                 if ((rc = replay(KB))) return rc;
                 h->steps_since_sort += KB;
                 h->ke_valid = nvt;
@@ -1646,13 +1667,12 @@ static int md_steps_once(mtmd_handle* h, int nsteps, bool energy_last)
                 continue;
             }
             const bool sort_now = (++h->steps_since_sort >= h->sort_every);
-            if (!sort_now && !last && h->use_graphs && (!nvt || h->ke_valid)) {
+            if (!sort_now && !last && h->use_graphs) {
                 if ((rc = replay(1))) return rc;
                 h->ke_valid = nvt;
                 h->steps++;
                 continue;
             }
-            if (nvt) nvt_front();
             if ((rc = fast_rebuild(h, sort_now, sort_now))) return rc;
             if (fuse && !last) {
                 fused_forces();
@@ -1662,7 +1682,8 @@ static int md_steps_once(mtmd_handle* h, int nsteps, bool energy_last)
             }
             if ((rc = fast_forces(h, last && energy_last))) return rc;
             if (nvt) {
-                nvt_back();
+                if (last) nvt_back();
+                else nvt_mid();
             } else if (last) {
                 mt_cap_kick(d, h->p, 0, h->stream);
                 mt_scale_wrap(d, h->p, -1, 0, h->stream);
@@ -1672,6 +1693,8 @@ static int md_steps_once(mtmd_handle* h, int nsteps, bool energy_last)
             h->ke_valid = nvt;
             h->steps++;
         }
+        // the last chain of the call left p.p/m after its scale in nh[3M + 1]
+        h->ke_slot = nvt;
     }
     HIPCHK(h, hipGetLastError());
     if (h->fast) {
@@ -1710,6 +1733,7 @@ static int snapshot_take(mtmd_handle* h)
     for (int i = 0; i < 3; i++) sn.L[i] = h->p.L[i];
     sn.steps = h->steps;
     sn.steps_since_sort = h->steps_since_sort;
+    sn.ke_slot = h->ke_slot;
     return MTMD_OK;
 }
 
@@ -1728,6 +1752,7 @@ static int snapshot_restore(mtmd_handle* h)
     set_box(h, sn.L);
     h->steps = sn.steps;
     h->steps_since_sort = sn.steps_since_sort;
+    h->ke_slot = sn.ke_slot;                          // (nh[3M + 1] is restored with the chain)
     // forces, lists and the kinetic sums of the restored state are rebuilt
     h->have_forces = false;
     h->need_sort = true;

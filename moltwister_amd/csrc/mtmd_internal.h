@@ -147,7 +147,11 @@ void mt_cap_kick(const MtDev& d, const MtParams& p, int want_ke, hipStream_t s);
 // fast: the FP64/MIXED form of the chain (mt_nh_propagate_fast), else the
 // reference's operation order
 void mt_nh_chain(const MtDev& d, int M, int nresp, double T, double dt, double g, const double* w,
-                 int nblocks, hipStream_t s, int fast = 0);
+                 int nblocks, hipStream_t s, int fast = 0, int from_slot = 0);
+// FP64/MIXED NVT between two steps of a call: both chain propagations (end of
+// step t into nh[3M], start of t + 1 into nh[3M + 1]) in one launch
+void mt_nh_chain2(const MtDev& d, int M, int nresp, double T, double dt, double g, const double* w, int nblocks,
+                  hipStream_t s);
 void mt_scale_wrap(const MtDev& d, const MtParams& p, int scale_slot, int want_ke, hipStream_t s);
 void mt_reduce_ke(const MtDev& d, hipStream_t s);
 void mt_reduce_thermo(const MtDev& d, hipStream_t s);
@@ -197,6 +201,8 @@ int mt_fast_lanes(const MtDev& d);
 // the kernels that move atoms (front halves) set the rebuild flag: their
 // launchers flip p.dslot
 void mt_fast_kick_drift(const MtDev& d, MtParams& p, int scale_slot, int mixed, hipStream_t s);
+// the end of step t (scale nh[sb], wrap) and the front half of t + 1 (scale nh[sf], kick, drift)
+void mt_fast_scale_kick_drift(const MtDev& d, MtParams& p, int sb, int sf, int mixed, hipStream_t s);
 void mt_fast_clear_flags(const MtDev& d, hipStream_t s);
 // NPT: displacement since the last rebuild and MIXED local coordinates of the
 // current positions (the locating half of the front-half kernel)

@@ -1889,6 +1889,15 @@ __device__ __forceinline__ MtApGeom mt_ap_geom(const MtParams& p)
     return g;
 }
 
+// a position within 0.75 w - edge of the box centre on every axis: two such
+// positions are less than 3w/2 - edge apart, with the rounding of their FP32
+// difference to spare
+__device__ __forceinline__ bool mt_ap_inner(float x, float y, float z, const MtApGeom& g)
+{
+    return fabsf(x) <= 1.5f * g.hw[0] - g.edge && fabsf(y) <= 1.5f * g.hw[1] - g.edge &&
+           fabsf(z) <= 1.5f * g.hw[2] - g.edge;
+}
+
 // One pair term of the MIXED all-pairs Coulomb sum, added to (fx, fy, fz, u):
 // F_k += K q_k q_i d / |d|^3, u += K q_k q_i / |d|, d = r_k - image(r_i), in
 // FP32 from FP32 positions, the image taken by rounding d / w. Within `edge` of
@@ -1898,6 +1907,10 @@ __device__ __forceinline__ MtApGeom mt_ap_geom(const MtParams& p)
 // rounding never picks the other image of a pair than the FP64 engine does.
 // Contraction is off and the FMAs are explicit: the special-pair pass
 // recomputes this term for subtraction and must get the same bits.
+// FAR = false leaves out the test for |d| near 3w/2, for pairs whose positions
+// are both within 0.75 w - edge of the box centre (mt_ap_inner), where it
+// cannot hold: the same bits as with it.
+template <bool FAR = true>
 __device__ __forceinline__ void mt_ap_term(float xk, float yk, float zk, const mt_d3& rk, float qks, float4 q, int i,
                                            const double4* pos, const MtParams& p, const MtApGeom& g, float& fx,
                                            float& fy, float& fz, float& u)
@@ -1905,8 +1918,9 @@ __device__ __forceinline__ void mt_ap_term(float xk, float yk, float zk, const m
 #pragma clang fp contract(off)
     float dx = xk - q.x, dy = yk - q.y, dz = zk - q.z;
     const float ax = fabsf(dx), ay = fabsf(dy), az = fabsf(dz);
-    const bool nearb = fabsf(ax - g.hw[0]) < g.edge || fabsf(ay - g.hw[1]) < g.edge || fabsf(az - g.hw[2]) < g.edge ||
-                       ax > 3.0f * g.hw[0] - g.edge || ay > 3.0f * g.hw[1] - g.edge || az > 3.0f * g.hw[2] - g.edge;
+    bool nearb = fabsf(ax - g.hw[0]) < g.edge || fabsf(ay - g.hw[1]) < g.edge || fabsf(az - g.hw[2]) < g.edge;
+    if constexpr (FAR)
+        nearb = nearb || ax > 3.0f * g.hw[0] - g.edge || ay > 3.0f * g.hw[1] - g.edge || az > 3.0f * g.hw[2] - g.edge;
     if (nearb) {
         const double4 r4 = pos[i];
         const mt_d3 ri = mt_move_same_side(rk, {r4.x, r4.y, r4.z}, p.wbox);
@@ -2029,23 +2043,37 @@ __global__ void __launch_bounds__(AP_TPB) k_ap_coulomb_mixed(MtDev d, MtParams p
         qks = MT_COULOMB_K * d.qf[k];
     }
     double Fx = 0.0, Fy = 0.0, Fz = 0.0, U = 0.0;
+    // the wavefront's own atoms all inside 0.75 w (uniform over the wavefront)
+    const bool own_in = __all(!act || mt_ap_inner(xk, yk, zk, g));
     for (int base = i0; base < i1; base += AP_TPB) {
         __syncthreads();
         const int i = base + t;
+        bool in = true;
         if (i < i1) {
             const double4 q = d.pos[i];
             tq[t] = make_float4((float)q.x, (float)q.y, (float)q.z, d.qf[i]);
+            in = mt_ap_inner(tq[t].x, tq[t].y, tq[t].z, g);
         } else {
             tq[t] = make_float4(xk, yk, zk, 0.f);       // zero charge: no contribution
         }
-        __syncthreads();
+        // the tile's atoms all inside 0.75 w (uniform over the workgroup)
+        const bool tile_in = __syncthreads_and(in);
         if (qks == 0.0f) continue;
         float fx = 0.f, fy = 0.f, fz = 0.f, u = 0.f;
         const int lim = min(AP_TPB, i1 - base);
+        if (tile_in && own_in) {
+            // no pair can be near 3w/2: the shorter test, the same bits
 #pragma unroll 4
-        for (int e = 0; e < AP_TPB; e++) {
-            const int ie = e < lim ? base + e : k;       // padded entries: a zero charge at the atom itself
-            mt_ap_term(xk, yk, zk, rk, qks, tq[e], ie, d.pos, p, g, fx, fy, fz, u);
+            for (int e = 0; e < AP_TPB; e++) {
+                const int ie = e < lim ? base + e : k;   // padded entries: a zero charge at the atom itself
+                mt_ap_term<false>(xk, yk, zk, rk, qks, tq[e], ie, d.pos, p, g, fx, fy, fz, u);
+            }
+        } else {
+#pragma unroll 4
+            for (int e = 0; e < AP_TPB; e++) {
+                const int ie = e < lim ? base + e : k;
+                mt_ap_term<true>(xk, yk, zk, rk, qks, tq[e], ie, d.pos, p, g, fx, fy, fz, u);
+            }
         }
         Fx += (double)fx; Fy += (double)fy; Fz += (double)fz;
         U += 0.5 * (double)u;

@@ -643,16 +643,21 @@ __device__ __forceinline__ void mt_neigh_push(uint32_t w, uint32_t& r0, uint32_t
 // takes back these same terms when the FP64 test puts the pair outside. Both
 // places compute them with these helpers, the same operations in the same
 // order, and the force-only and energy passes give the same forces.
+//
+// r^2 carries 1e-10 from its first product on, so it is never zero (the pad
+// word of a list is the atom itself): 1/r stays finite, r = 1e-5 at the
+// least, below the table's first point 0.01, and such a pair indexes the zero
+// segment as the reference skips r < 1e-3. From r = 0.1 on, the 1e-10 is
+// below half an FP32 ulp of r^2 and changes it only where it tips a rounding.
 __device__ __forceinline__ float mt_lean_r2(float dx, float dy, float dz)
 {
-    return fmaf(dz, dz, fmaf(dx, dx, dy * dy));
+    return fmaf(dz, dz, fmaf(dx, dx, fmaf(dy, dy, 1.0e-10f)));
 }
 
-// 1/r (r^2 clamped away from zero: the pad word is the atom itself) and the
-// segment of the pair
+// 1/r of a mt_lean_r2 value and the segment of the pair
 __device__ __forceinline__ float mt_lean_rinv(float r2, float x0, float invd, int& seg)
 {
-    const float rinv = __builtin_amdgcn_rsqf(fmaxf(r2, 1.0e-6f));
+    const float rinv = __builtin_amdgcn_rsqf(r2);
     const int idx = (int)(fmaf(r2, rinv, -x0) * invd);
     seg = (int)min((unsigned)idx, (unsigned)(MT_NPTS - 1));
     return rinv;

@@ -171,10 +171,14 @@ __global__ void __launch_bounds__(SCAN_B) k_fast_scan_apply(const int* __restric
     }
 }
 
+// places the atoms of each cell at cell_first. With `down` set, fill holds the
+// cell counts and each run of a cell takes the places below the count it
+// finds, so the counts end at zero (the state k_fast_cell_keys counts from);
+// otherwise fill starts at zero and counts up.
 __global__ void __launch_bounds__(TPB) k_fast_scatter(const int* __restrict__ key, int n,
                                                       const int* __restrict__ first, int* __restrict__ fill,
-                                                      int* __restrict__ list, int cond, const unsigned int* flags,
-                                                      int dslot, float thr)
+                                                      int* __restrict__ list, int down, int cond,
+                                                      const unsigned int* flags, int dslot, float thr)
 {
     if (mt_skip_rebuild(cond, flags, dslot, thr)) return;
     const int lane = threadIdx.x & 63;
@@ -184,7 +188,7 @@ __global__ void __launch_bounds__(TPB) k_fast_scatter(const int* __restrict__ ke
         int hl, len;
         mt_key_runs(c, hl, len);
         int base = 0;
-        if (c >= 0 && lane == hl) base = atomicAdd(&fill[c], len);
+        if (c >= 0 && lane == hl) base = down ? atomicSub(&fill[c], len) - len : atomicAdd(&fill[c], len);
         base = __shfl(base, hl);
         if (c >= 0) list[first[c] + base + (lane - hl)] = k;
     }
@@ -221,115 +225,275 @@ __device__ __forceinline__ void mt_fast_home_atom(const MtDev& d, const MtParams
     d.posf[u] = make_float4(l.x, l.y, l.z, __int_as_float(d.type[s] * MT_NPTS));
 }
 
-// ascending slot order inside each cell, so lists are reproducible run to run
-// (one wavefront per cell, mt_wave_sort_cell); with `home` set the same
-// wavefront then does the per-atom set-up of every atom of the cell, which
-// holds its sorted position already. In the MIXED tile mode (d.tile16) the
-// atoms of a cell are ordered by their cell-local FP32 x instead (slot as the
+// ascending slot order inside each cell, so lists are reproducible run to run;
+// with `home` set the per-atom set-up of every atom at its sorted place
+// follows (mt_fast_home_atom). In the MIXED tile mode (d.tile16) the atoms of
+// a cell are ordered by their cell-local FP32 x instead (slot as the
 // tie-break), so the three cells of an x row of staged cells form a run whose
 // staged x never decreases inside a cell; the 16-bit list build uses that to
 // skip the ends of the row that no pair of its screen can reach.
-// cells of up to MT_FAST_SORT_LDS atoms are ranked in LDS (8 KB a workgroup, so
-// many workgroups share a CU); a larger cell is sorted by one lane
-#define MT_FAST_SORT_LDS 512
+//
+// Each wavefront takes a contiguous range of cells, whose atoms are
+// contiguous in cell_list, in chunks of whole cells of at most MT_SORT_CHUNK
+// atoms. A chunk's counts, list entries and keys are each fetched with every
+// load of the kind in flight at once (one wait on memory per kind, not one a
+// cell), the keys go to this wavefront's share of LDS, and an atom's place in
+// its cell is the number of smaller keys there. The places are distinct
+// because the keys are (slots are distinct). A cell of more than
+// MT_SORT_CHUNK atoms is sorted by lane 0 (insertion). No workgroup barrier:
+// the wavefronts of a workgroup work independently.
+#define MT_SORT_CHUNK 512
+#define MT_SORT_PER (MT_SORT_CHUNK / 64)
+
+// the sort key of slot v in cell c, and its cell-local position when xk
+__device__ __forceinline__ unsigned long long mt_sort_key(const MtDev& d, const MtParams& p, bool xk, int v, int c,
+                                                          float3& l)
+{
+    if (!xk) return (unsigned long long)(unsigned)v;
+    l = mt_fast_local(d, p, v, c);
+    uint32_t u = __float_as_uint(l.x);
+    u = (u & 0x80000000u) ? ~u : (u | 0x80000000u);          // float order as unsigned order
+    return ((unsigned long long)u << 32) | (uint32_t)v;
+}
+
+__device__ __forceinline__ void mt_sort_set_up(const MtDev& d, const MtParams& p, bool xk, int v, int c, int u,
+                                               const float3& l)
+{
+    if (!xk) {
+        mt_fast_home_atom(d, p, v, c, u);
+        return;
+    }
+    d.home[v] = c;
+    d.posref[v] = d.pos[v];
+    d.sidx[v] = u;
+    d.qfs[u] = d.qf[v];
+    d.posf[u] = make_float4(l.x, l.y, l.z, __int_as_float(d.type[v] * MT_NPTS));
+}
+
 __global__ void __launch_bounds__(TPB) k_fast_cell_sort(MtDev d, MtParams p, int cond, int home)
 {
-    __shared__ __align__(16) int lds[(TPB / 64) * MT_FAST_SORT_LDS];
+    __shared__ unsigned long long s_key[TPB / 64][MT_SORT_CHUNK];
+    __shared__ unsigned char s_cell[TPB / 64][MT_SORT_CHUNK];   // chunk place -> cell of the chunk (0..63)
+    __shared__ int s_off[TPB / 64][65];                           // cell of the chunk -> its first chunk place
     if (mt_skip_rebuild(cond, d.flags, p.dslot, p.half_skin2) || d.flags[3]) return;   // uniform over the grid
     if (home && blockIdx.x == 0 && threadIdx.x == 0) d.flags[4] += 1;        // rebuild count
-    int* lw = lds + (threadIdx.x >> 6) * MT_FAST_SORT_LDS;
-    const int wpb = (int)blockDim.x >> 6;             // wavefronts (cells) a block
-    for (int c0 = blockIdx.x * wpb; c0 < d.ncells; c0 += gridDim.x * wpb) {   // uniform over the block
-        const int c = c0 + ((int)threadIdx.x >> 6);
-        const int cnt = c < d.ncells ? d.cell_count[c] : 0;
-        const int f = c < d.ncells ? d.cell_first[c] : 0;
-        // the cell-local position of the wavefront's lane atom, from the key
-        // to the set-up (a cell of up to 64 atoms calls both once a lane)
-        float3 lc = make_float3(0.f, 0.f, 0.f);
-        int lc_v = -1;
-        auto set_up = [&](int v, int rank) {
-            if (v == lc_v) {
-                d.home[v] = c;
-                d.posref[v] = d.pos[v];
-                d.sidx[v] = f + rank;
-                d.qfs[f + rank] = d.qf[v];
-                d.posf[f + rank] = make_float4(lc.x, lc.y, lc.z, __int_as_float(d.type[v] * MT_NPTS));
-            } else {
-                mt_fast_home_atom(d, p, v, c, f + rank);
-            }
-        };
-        if (home && d.tile16) {
-            auto xkey = [&](int v) -> unsigned long long {
-                const float3 l = mt_fast_local(d, p, v, c);
-                lc = l;
-                lc_v = v;
-                uint32_t u = __float_as_uint(l.x);
-                u = (u & 0x80000000u) ? ~u : (u | 0x80000000u);      // float order as unsigned order
-                return ((unsigned long long)u << 32) | (uint32_t)v;
-            };
-            mt_wave_sort_cell_keyed<MT_FAST_SORT_LDS>(d.cell_list + f, cnt, lw, xkey, set_up);
-        } else if (home) {
-            mt_wave_sort_cell<MT_FAST_SORT_LDS>(d.cell_list + f, cnt, lw, set_up);
-        } else {
-            mt_wave_sort_cell<MT_FAST_SORT_LDS>(d.cell_list + f, cnt, lw);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const long long nwave = (long long)gridDim.x * (blockDim.x >> 6);
+    const long long gw = (long long)blockIdx.x * (blockDim.x >> 6) + wv;
+    const int cb = (int)(d.ncells * gw / nwave), ce = (int)(d.ncells * (gw + 1) / nwave);
+    const bool xk = home && d.tile16;
+    unsigned long long* sk = s_key[wv];
+    unsigned char* sc = s_cell[wv];
+    int* so = s_off[wv];
+    for (int cur = cb; cur < ce;) {                   // uniform over the wavefront
+        const int cj = cur + lane;
+        const int cnt = cj < ce ? d.cell_count[cj] : 0;
+        const int fst = cj < ce ? d.cell_first[cj] : 0;
+        int incl = cnt;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int u = __shfl_up(incl, o);
+            if (lane >= o) incl += u;
         }
-        __syncthreads();                              // lw is reused by the next cell
+        // the cells of this chunk: a prefix of the lanes (incl never decreases)
+        const int ncj = __popcll(__ballot(cj < ce && incl <= MT_SORT_CHUNK));
+        const int f0 = __shfl(fst, 0);
+        if (ncj == 0) {
+            // the first cell alone holds more than a chunk
+            const int n0 = __shfl(cnt, 0);
+            if (lane == 0) {
+                int* a = d.cell_list + f0;
+                float3 l;
+                for (int i = 1; i < n0; i++) {
+                    const int v = a[i];
+                    const unsigned long long kv = mt_sort_key(d, p, xk, v, cur, l);
+                    int j = i - 1;
+                    while (j >= 0 && mt_sort_key(d, p, xk, a[j], cur, l) > kv) { a[j + 1] = a[j]; j--; }
+                    a[j + 1] = v;
+                }
+                if (home)
+                    for (int i = 0; i < n0; i++) {
+                        const int v = a[i];
+                        mt_sort_key(d, p, xk, v, cur, l);
+                        mt_sort_set_up(d, p, xk, v, cur, f0 + i, l);
+                    }
+            }
+            cur += 1;
+            continue;
+        }
+        const int A = __shfl(incl, ncj - 1);           // atoms of the chunk
+        const int excl = incl - cnt;
+        // place -> cell: zeros, each non-empty cell's index at its first
+        // place, then a running maximum over the places
+        for (int m = 0; m < MT_SORT_PER; m++)
+            if (lane + 64 * m < A) sc[lane + 64 * m] = 0;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (lane < ncj) {
+            so[lane] = excl;
+            if (cnt > 0) sc[excl] = (unsigned char)lane;
+        }
+        if (lane == 0) so[ncj] = A;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        int run[MT_SORT_PER], lmax = 0;
+#pragma unroll
+        for (int m = 0; m < MT_SORT_PER; m++) {
+            const int i = MT_SORT_PER * lane + m;
+            lmax = max(lmax, i < A ? (int)sc[i] : 0);
+            run[m] = lmax;
+        }
+        int pmax = lmax;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int u = __shfl_up(pmax, o);
+            if (lane >= o) pmax = max(pmax, u);
+        }
+        pmax = __shfl_up(pmax, 1);
+        if (lane == 0) pmax = 0;
+#pragma unroll
+        for (int m = 0; m < MT_SORT_PER; m++) {
+            const int i = MT_SORT_PER * lane + m;
+            if (i < A) sc[i] = (unsigned char)max(pmax, run[m]);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // the chunk's slots and keys (place lane + 64 m), every load in flight
+        int vs[MT_SORT_PER], js[MT_SORT_PER];
+        unsigned long long ks[MT_SORT_PER];
+        float3 ls[MT_SORT_PER];
+#pragma unroll
+        for (int m = 0; m < MT_SORT_PER; m++) {
+            const int i = lane + 64 * m;
+            vs[m] = i < A ? d.cell_list[f0 + i] : 0;
+            js[m] = i < A ? (int)sc[i] : 0;
+        }
+#pragma unroll
+        for (int m = 0; m < MT_SORT_PER; m++) {
+            const int i = lane + 64 * m;
+            ls[m] = make_float3(0.f, 0.f, 0.f);
+            if (i < A) {
+                ks[m] = mt_sort_key(d, p, xk, vs[m], cur + js[m], ls[m]);
+                sk[i] = ks[m];
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // place in the cell: the number of smaller keys of the cell; every
+        // read of cell_list above has completed (its values are used), so
+        // the writes below cannot overtake it
+#pragma unroll
+        for (int m = 0; m < MT_SORT_PER; m++) {
+            const int i = lane + 64 * m;
+            if (i < A) {
+                const int o0 = so[js[m]], o1 = so[js[m] + 1];
+                int rank = 0;
+                for (int t = o0; t < o1; t++) rank += sk[t] < ks[m] ? 1 : 0;
+                const int u = f0 + o0 + rank;
+                d.cell_list[u] = vs[m];
+                if (home) mt_sort_set_up(d, p, xk, vs[m], cur + js[m], u, ls[m]);
+            }
+        }
+        // this chunk's LDS is rewritten by the next one
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        cur += ncj;
     }
 }
 
 // The exclusive scan of the counts of a rebuild in one workgroup (grids of at
 // most MT_SCAN_ONE_BLOCK cells): k_fast_cell_keys has counted into cell_next,
-// which is all zero between rebuilds; the scan writes cell_first, moves the
-// counts to cell_count, clears cell_fill for the scatter and cell_next for
-// the next rebuild. One launch in place of the count fill and the two-level
-// scan. (A one-pass variant in which the last workgroup of the key kernel
-// scans, found by a ticket, needs a device-scope fence in every workgroup;
-// with one L2 per XCD that cost 0.38 ms a build on the 1M-atom mixture.)
+// which is all zero between rebuilds (k_fast_scatter counts each cell back
+// down to zero as it places the cell's atoms). The scan writes cell_first and
+// copies the counts to cell_count. Every tile of 4 SCAN_B cells (four
+// consecutive cells a thread, one 16-byte load) is loaded at once, so the
+// workgroup waits on memory once; each tile's thread sums are scanned across
+// the wavefront, and the wavefront totals of all tiles by wavefront 0, one
+// barrier before and one after. (A one-pass variant in which the last
+// workgroup of the key kernel scans, found by a ticket, needs a device-scope
+// fence in every workgroup; with one L2 per XCD that cost 0.38 ms a build on
+// the 1M-atom mixture.)
 #define MT_SCAN_ONE_BLOCK 65536
-#define MT_SCAN_ONE_TPI 4                 // tiles of SCAN_B cells an iteration
+#define MT_SCAN_ONE_TILES (MT_SCAN_ONE_BLOCK / (4 * SCAN_B))
 __global__ void __launch_bounds__(SCAN_B) k_fast_scan_one(MtDev d, int cond, int dslot, float thr)
 {
     if (mt_skip_rebuild(cond, d.flags, dslot, thr)) return;
-    __shared__ int s_wsum[MT_SCAN_ONE_TPI][SCAN_B / 64];
+    constexpr int NW = SCAN_B / 64;
+    __shared__ int s_w[MT_SCAN_ONE_TILES * NW];       // wavefront totals, then their exclusive prefixes
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    int carry = 0;
-    for (int base = 0; base < d.ncells; base += MT_SCAN_ONE_TPI * SCAN_B) {
-        // MT_SCAN_ONE_TPI coalesced tiles in flight at once
-        int v[MT_SCAN_ONE_TPI], ex[MT_SCAN_ONE_TPI];
+    const int nc = d.ncells;
+    const int ntile = (nc + 4 * SCAN_B - 1) / (4 * SCAN_B);
+    int4 v[MT_SCAN_ONE_TILES];
+    int ex[MT_SCAN_ONE_TILES];
 #pragma unroll
-        for (int t = 0; t < MT_SCAN_ONE_TPI; t++) {
-            const int c = base + t * SCAN_B + (int)threadIdx.x;
-            v[t] = c < d.ncells ? d.cell_next[c] : 0;
+    for (int t = 0; t < MT_SCAN_ONE_TILES; t++) {
+        const int c = 4 * (t * SCAN_B + (int)threadIdx.x);
+        v[t] = make_int4(0, 0, 0, 0);
+        if (c + 3 < nc) {
+            v[t] = *reinterpret_cast<const int4*>(d.cell_next + c);   // c is a multiple of 4: 16-byte aligned
+        } else if (c < nc) {
+            v[t].x = d.cell_next[c];
+            if (c + 1 < nc) v[t].y = d.cell_next[c + 1];
+            if (c + 2 < nc) v[t].z = d.cell_next[c + 2];
         }
+    }
 #pragma unroll
-        for (int t = 0; t < MT_SCAN_ONE_TPI; t++) {
-            int incl = v[t];
-            for (int o = 1; o < 64; o <<= 1) {
-                const int u = __shfl_up(incl, o);
-                if (lane >= o) incl += u;
-            }
-            ex[t] = incl - v[t];
-            if (lane == 63) s_wsum[t][w] = incl;
-        }
-        __syncthreads();
+    for (int t = 0; t < MT_SCAN_ONE_TILES; t++) {
+        // (tiles past the last hold zeros: scanned, not stored)
+        const int sum = v[t].x + v[t].y + v[t].z + v[t].w;
+        int incl = sum;
 #pragma unroll
-        for (int t = 0; t < MT_SCAN_ONE_TPI; t++) {
-            int wpre = 0, tot = 0;
-            for (int j = 0; j < SCAN_B / 64; j++) {
-                const int x = s_wsum[t][j];
-                wpre += j < w ? x : 0;
-                tot += x;
-            }
-            const int c = base + t * SCAN_B + (int)threadIdx.x;
-            if (c < d.ncells) {
-                d.cell_first[c] = carry + wpre + ex[t];
-                d.cell_count[c] = v[t];
-                d.cell_fill[c] = 0;
-                d.cell_next[c] = 0;
-            }
-            carry += tot;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int u = __shfl_up(incl, o);
+            if (lane >= o) incl += u;
         }
-        __syncthreads();                              // s_wsum is rewritten next iteration
+        ex[t] = incl - sum;
+        if (lane == 63 && t < ntile) s_w[t * NW + w] = incl;
+    }
+    __syncthreads();
+    if (w == 0) {
+        // exclusive scan of the ntile * NW totals in tile, then wavefront order:
+        // lane j takes entries 4j .. 4j + 3
+        const int m = ntile * NW;
+        int e[4], tot = 0;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int j = 4 * lane + i;
+            e[i] = j < m ? s_w[j] : 0;
+            tot += e[i];
+        }
+        int incl = tot;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int u = __shfl_up(incl, o);
+            if (lane >= o) incl += u;
+        }
+        int run = incl - tot;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int j = 4 * lane + i;
+            if (j < m) s_w[j] = run;
+            run += e[i];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < MT_SCAN_ONE_TILES; t++) {
+        const int c = 4 * (t * SCAN_B + (int)threadIdx.x);
+        if (c >= nc) continue;
+        const int b = s_w[t * NW + w] + ex[t];
+        const int4 f = make_int4(b, b + v[t].x, b + v[t].x + v[t].y, b + v[t].x + v[t].y + v[t].z);
+        if (c + 3 < nc) {
+            *reinterpret_cast<int4*>(d.cell_first + c) = f;
+            *reinterpret_cast<int4*>(d.cell_count + c) = v[t];
+        } else if (c < nc) {
+            d.cell_first[c] = f.x;
+            d.cell_count[c] = v[t].x;
+            if (c + 1 < nc) { d.cell_first[c + 1] = f.y; d.cell_count[c + 1] = v[t].y; }
+            if (c + 2 < nc) { d.cell_first[c + 2] = f.z; d.cell_count[c + 2] = v[t].z; }
+        }
     }
 }
 
@@ -342,7 +506,7 @@ void mt_fast_cells(const MtDev& d, const MtParams& p, int cond, int with_home, h
                            thr);
         hipLaunchKernelGGL(k_fast_scan_one, dim3(1), dim3(SCAN_B), 0, s, d, cond, p.dslot, thr);
         hipLaunchKernelGGL(k_fast_scatter, dim3(nblk_rebuild(d.n, TPB)), dim3(TPB), 0, s, d.cell_key, d.n,
-                           d.cell_first, d.cell_fill, d.cell_list, cond, d.flags, p.dslot, thr);
+                           d.cell_first, d.cell_next, d.cell_list, 1, cond, d.flags, p.dslot, thr);
         hipLaunchKernelGGL(k_fast_cell_sort, dim3(nblk_rebuild(d.ncells, TPB / 64)), dim3(TPB), 0, s, d, p, cond,
                            with_home);
         return;
@@ -357,7 +521,7 @@ void mt_fast_cells(const MtDev& d, const MtParams& p, int cond, int with_home, h
     hipLaunchKernelGGL(k_fast_scan_apply, dim3(nsb), dim3(SCAN_B), 0, s, d.cell_count, d.cell_bsum, d.cell_first,
                        d.cell_fill, d.ncells, cond, d.flags, p.dslot, thr);
     hipLaunchKernelGGL(k_fast_scatter, dim3(nblk(d.n, TPB)), dim3(TPB), 0, s, d.cell_key, d.n, d.cell_first,
-                       d.cell_fill, d.cell_list, cond, d.flags, p.dslot, thr);
+                       d.cell_fill, d.cell_list, 0, cond, d.flags, p.dslot, thr);
     hipLaunchKernelGGL(k_fast_cell_sort, dim3(nblk(d.ncells, TPB / 64)), dim3(TPB), 0, s, d, p, cond, with_home);
 }
 

@@ -389,8 +389,12 @@ __global__ void __launch_bounds__(TPB) k_fast_cell_sort(MtDev d, MtParams p, int
             const int i = lane + 64 * m;
             if (i < A) {
                 const int o0 = so[js[m]], o1 = so[js[m] + 1];
-                int rank = 0;
-                for (int t = o0; t < o1; t++) rank += sk[t] < ks[m] ? 1 : 0;
+                int rank = 0, t = o0;
+                for (; t + 4 <= o1; t += 4) {         // four reads in flight
+                    const unsigned long long k0 = sk[t], k1 = sk[t + 1], k2 = sk[t + 2], k3 = sk[t + 3];
+                    rank += (k0 < ks[m] ? 1 : 0) + (k1 < ks[m] ? 1 : 0) + (k2 < ks[m] ? 1 : 0) + (k3 < ks[m] ? 1 : 0);
+                }
+                for (; t < o1; t++) rank += sk[t] < ks[m] ? 1 : 0;
                 const int u = f0 + o0 + rank;
                 d.cell_list[u] = vs[m];
                 if (home) mt_sort_set_up(d, p, xk, vs[m], cur + js[m], u, ls[m]);

@@ -1755,8 +1755,12 @@ __device__ __forceinline__ int mt_tile_home(const int* s_hoff, int v)
 // read from global memory; otherwise (a_f, b_f) of every type pair staged in LDS
 // the NVE integration of one atom and the rebuild flag (defined with the
 // integrator kernels below)
-__device__ __forceinline__ float mt_fast_nve_atom(const MtDev& d, const MtParams& p, int k, double4& F, double4& pm,
-                                                  double4& r, bool& capped, int mixed, float4* posf_out, bool with_w);
+__device__ __forceinline__ void mt_fast_back_half(const MtParams& p, double4& F, double4& pm, double4& r,
+                                                  bool& capped);
+__device__ __forceinline__ void mt_fast_kick_drift_atom(const MtParams& p, double4& pm, double4& r, const double4& F);
+__device__ __forceinline__ float mt_fast_locate_at(const MtDev& d, const MtParams& p, const double4& q,
+                                                   const double4& r, int mixed, int c0, int u, int tw,
+                                                   float4* posf_out, bool with_w);
 __device__ __forceinline__ void mt_fast_flag_disp(const MtDev& d, const MtParams& p, float d2);
 
 // FUSE (NVE steps of a call but the last, no forces added after this pass):
@@ -1823,10 +1827,12 @@ k_fast_force_tile(MtDev d, MtParams p)
         const float qk = COUL ? d.qf[k] : 0.0f;
         double Fx = 0.0, Fy = 0.0, Fz = 0.0;
         float U = 0.0f;
+        int tw16;                                     // .w of the home atom's entry
         if (w16) {
             // the home atom's own staged entry (home cell h is staged cell qh)
             const int qh = (1 + (h & 1)) + 4 * ((1 + ((h >> 1) & 1)) + 4 * (1 + (h >> 2)));
             const float4 pk = spos[s_off[qh] + (v - s_hoff[h])];
+            tw16 = __float_as_int(pk.w);
             // the home atom's row: its type offset (bytes) times T, or itself
             // in symmetric rows (the energy pass reads the full rows)
             const size_t rowb = (size_t)__float_as_int(pk.w) * (WANT_E ? T : trow);
@@ -1840,6 +1846,7 @@ k_fast_force_tile(MtDev d, MtParams p)
                                             qk, Fx, Fy, Fz, U, gl, G);
         } else {
             const float4 pk = d.posf[u];
+            tw16 = __float_as_int(pk.w);
             if constexpr (WANT_E)
                 mt_tile_atom32<true, COUL>(d, p, k, pk, shd, d.nbcoef + (size_t)__float_as_int(pk.w) * T, qk, Fx, Fy,
                                            Fz, U, gl, G);
@@ -1853,10 +1860,18 @@ k_fast_force_tile(MtDev d, MtParams p)
             if constexpr (FUSE) {
                 // the lane that holds the force integrates the atom at once;
                 // the force itself is not stored (the next pass computes the
-                // next one, and a call's last step is not fused)
+                // next one, and a call's last step is not fused). The atom's
+                // home cell, place and type come from the block, not from
+                // home, sidx and type (the same values), so the state loads
+                // below are all the tail waits for.
+                const double4 q = d.posref[k];
                 double4 F = make_double4(Fx, Fy, Fz, 0.0), pm = d.mom[k], r = d.pos[k];
+                const int hc = (x0 + (h & 1)) + d.nc[0] * ((y0 + ((h >> 1) & 1)) + d.nc[1] * (z0 + (h >> 2)));
+                const int tw = w16 ? (int)(tw16 / (WANT_E ? (int)sizeof(float4) : (int)sizeof(float2))) : tw16;
                 bool capped;
-                d2max = fmaxf(d2max, mt_fast_nve_atom(d, p, k, F, pm, r, capped, 1, d.posf2, true));
+                mt_fast_back_half(p, F, pm, r, capped);
+                mt_fast_kick_drift_atom(p, pm, r, F);
+                d2max = fmaxf(d2max, mt_fast_locate_at(d, p, q, r, 1, hc, u, tw, d.posf2, true));
                 d.mom[k] = pm;
                 d.pos2[k] = r;
             } else {
@@ -2477,11 +2492,14 @@ void mt_fast_forces(const MtDev& d, const MtParams& p, int mixed, int want_e, in
 // posf_out: where the MIXED local coordinates go (at the atom's place in cell
 // order); with_w: the whole float4, the type offset in .w included (a buffer
 // the rebuild did not write), else x, y, z only
-__device__ __forceinline__ float mt_fast_locate_atom_to(const MtDev& d, const MtParams& p, int k, const double4& r,
-                                                        int mixed, float4* posf_out, bool with_w)
+// q: the atom's reference position of the last rebuild; MIXED: c0 its home
+// cell, u its place in cell order and tw (with_w) the bits of type * MT_NPTS,
+// as the rebuild stored them in home, sidx and type
+__device__ __forceinline__ float mt_fast_locate_at(const MtDev& d, const MtParams& p, const double4& q,
+                                                   const double4& r, int mixed, int c0, int u, int tw,
+                                                   float4* posf_out, bool with_w)
 {
     MT_EXACT_FP
-    const double4 q = d.posref[k];
     double dx = r.x - q.x, dy = r.y - q.y, dz = r.z - q.z;
     // nearest image by a multiplication with 1/L: the count differs from
     // rint(dx / L) only where dx / L lies within rounding of a half-integer,
@@ -2490,7 +2508,6 @@ __device__ __forceinline__ float mt_fast_locate_atom_to(const MtDev& d, const Mt
     dy -= p.L[1] * rint(dy * p.invL[1]);
     dz -= p.L[2] * rint(dz * p.invL[2]);
     if (mixed) {
-        const int c0 = d.home[k];
         const int ix = c0 % d.nc[0], iy = (c0 / d.nc[0]) % d.nc[1], iz = c0 / (d.nc[0] * d.nc[1]);
         // the cell centre with cw = L / nc computed once (the same bits)
         double lx = r.x - (-p.L[0] / 2.0 + ((double)ix + 0.5) * p.cw[0]);
@@ -2502,15 +2519,23 @@ __device__ __forceinline__ float mt_fast_locate_atom_to(const MtDev& d, const Mt
         ly -= p.L[1] * rint(ly * p.invL[1]);
         lz -= p.L[2] * rint(lz * p.invL[2]);
         if (with_w) {
-            posf_out[d.sidx[k]] = make_float4((float)lx, (float)ly, (float)lz, __int_as_float(d.type[k] * MT_NPTS));
+            posf_out[u] = make_float4((float)lx, (float)ly, (float)lz, __int_as_float(tw));
         } else {
             // x, y, z only: .w (the type offset) stays, and the line is not read
-            float* pf = reinterpret_cast<float*>(posf_out + d.sidx[k]);
+            float* pf = reinterpret_cast<float*>(posf_out + u);
             *reinterpret_cast<float2*>(pf) = make_float2((float)lx, (float)ly);
             pf[2] = (float)lz;
         }
     }
     return (float)(dx * dx + dy * dy + dz * dz);
+}
+
+// the same for atom k, its rebuild data read back
+__device__ __forceinline__ float mt_fast_locate_atom_to(const MtDev& d, const MtParams& p, int k, const double4& r,
+                                                        int mixed, float4* posf_out, bool with_w)
+{
+    return mt_fast_locate_at(d, p, d.posref[k], r, mixed, mixed ? d.home[k] : 0, mixed ? d.sidx[k] : 0,
+                             (mixed && with_w) ? d.type[k] * MT_NPTS : 0, posf_out, with_w);
 }
 
 __device__ __forceinline__ float mt_fast_locate_atom(const MtDev& d, const MtParams& p, int k, const double4& r,
@@ -2519,14 +2544,20 @@ __device__ __forceinline__ float mt_fast_locate_atom(const MtDev& d, const MtPar
     return mt_fast_locate_atom_to(d, p, k, r, mixed, d.posf, false);
 }
 
-__device__ __forceinline__ float mt_fast_front_to(const MtDev& d, const MtParams& p, int k, double4& pm, double4& r,
-                                                  const double4& F, int mixed, float4* posf_out, bool with_w)
+// half kick and drift
+__device__ __forceinline__ void mt_fast_kick_drift_atom(const MtParams& p, double4& pm, double4& r, const double4& F)
 {
     MT_EXACT_FP
     const double dt_2 = p.dt / 2.0;
     pm.x += F.x * dt_2; pm.y += F.y * dt_2; pm.z += F.z * dt_2;
     const double c = p.dt / pm.w;
     r.x += pm.x * c; r.y += pm.y * c; r.z += pm.z * c;
+}
+
+__device__ __forceinline__ float mt_fast_front_to(const MtDev& d, const MtParams& p, int k, double4& pm, double4& r,
+                                                  const double4& F, int mixed, float4* posf_out, bool with_w)
+{
+    mt_fast_kick_drift_atom(p, pm, r, F);
     return mt_fast_locate_atom_to(d, p, k, r, mixed, posf_out, with_w);
 }
 
@@ -2537,12 +2568,10 @@ __device__ __forceinline__ float mt_fast_front(const MtDev& d, const MtParams& p
 }
 
 // The back half of step t (force cap, half kick, PBC wrap; CVelVerlet +
-// pbcWrap, VelVerlet.cpp:96-122, SimulationBox.cpp:135-166) and the front
-// half of step t + 1 (half kick, drift, displacement, local coordinates) of
-// atom k for the force F of step t, which comes back capped. Returns the
-// squared displacement since the last rebuild.
-__device__ __forceinline__ float mt_fast_nve_atom(const MtDev& d, const MtParams& p, int k, double4& F, double4& pm,
-                                                  double4& r, bool& capped, int mixed, float4* posf_out, bool with_w)
+// pbcWrap, VelVerlet.cpp:96-122, SimulationBox.cpp:135-166) for the force F
+// of step t, which comes back capped
+__device__ __forceinline__ void mt_fast_back_half(const MtParams& p, double4& F, double4& pm, double4& r,
+                                                  bool& capped)
 {
     MT_EXACT_FP
     capped = false;
@@ -2560,6 +2589,15 @@ __device__ __forceinline__ float mt_fast_nve_atom(const MtDev& d, const MtParams
     if (r.y < -Y2) mt_pbc_add(r.y, -Y2, p.L[1]);
     if (r.z >= Z2) mt_pbc_add(r.z, Z2, p.L[2]);
     if (r.z < -Z2) mt_pbc_add(r.z, -Z2, p.L[2]);
+}
+
+// the back half of step t and the front half of step t + 1 (half kick,
+// drift, displacement, local coordinates) of atom k. Returns the squared
+// displacement since the last rebuild.
+__device__ __forceinline__ float mt_fast_nve_atom(const MtDev& d, const MtParams& p, int k, double4& F, double4& pm,
+                                                  double4& r, bool& capped, int mixed, float4* posf_out, bool with_w)
+{
+    mt_fast_back_half(p, F, pm, r, capped);
     return mt_fast_front_to(d, p, k, pm, r, F, mixed, posf_out, with_w);
 }
 

@@ -179,3 +179,35 @@ def test_full_size_mixture_vs_oracle(oracle_mod):
     assert per_atom_rel(Fm, Fo).max() < 1e-3
     assert per_atom_rel(Ff, Fo).max() < 1e-12
     assert abs(Um.sum(dtype=np.float64) - Uo.sum(dtype=np.float64)) <= 1e-5 * np.abs(Uo).sum(dtype=np.float64)
+
+
+BUCK = (1000.0, 0.1, 1.0)
+
+
+def test_buckingham_variant_vs_oracle(oracle_mod, relaxed):
+    """The mixture's Buckingham variant (A = 1000, rho = 0.1, C = 1 for every
+    type pair; MolTwisterMDFFNonBonded_Buck.cpp:116-143) at the same positions:
+    the tile path reads its tables through the same kernels. MIXED and FP64
+    against the oracle at the LJ bars."""
+    print("case buckingham tile vs oracle", flush=True)
+    s = relaxed
+    saved = s.nonbonded
+    s.nonbonded = [(0, 0, abi.NB_BUCK, BUCK), (0, 1, abi.NB_BUCK, BUCK), (1, 1, abi.NB_BUCK, BUCK)]
+    try:
+        o = oracle_mod.Oracle(s, lj_cfg(abi.ENGINE_EXACT, 0.3))
+        Fo, Uo = o.compute_forces()
+        del o
+        em = Engine(s, lj_cfg(abi.ENGINE_MIXED, 0.3))
+        Fm, Um = em.compute_forces()
+        em.bench_kernel(4, 1)                       # refused unless the tile path runs
+        em.close()
+        ef = Engine(s, lj_cfg(abi.ENGINE_FP64, 0.3))
+        Ff, _ = ef.compute_forces()
+        ef.close()
+    finally:
+        s.nonbonded = saved
+    print(f"buckingham: mixed rms {rms_rel(Fm, Fo):.2e}, fp64 max {per_atom_rel(Ff, Fo).max():.2e}", flush=True)
+    assert rms_rel(Fm, Fo) < 1e-5, rms_rel(Fm, Fo)
+    assert per_atom_rel(Fm, Fo).max() < 1e-3, per_atom_rel(Fm, Fo).max()
+    assert per_atom_rel(Ff, Fo).max() < 1e-12, per_atom_rel(Ff, Fo).max()
+    assert abs(Um.sum(dtype=np.float64) - Uo.sum(dtype=np.float64)) <= 1e-5 * np.abs(Uo).sum(dtype=np.float64)

@@ -318,3 +318,40 @@ def test_graph_replayed_nvt_steps_equal_single_steps(engine):
     ea, pea = a.get_thermostat()
     eb, peb = b.get_thermostat()
     assert np.array_equal(ea, eb) and np.array_equal(pea, peb)
+
+
+@pytest.mark.parametrize("rc,per_cell", [(6.0, (64, 512)), (9.0, (512, 2048))])
+@pytest.mark.parametrize("engine", [abi.ENGINE_FP64, abi.ENGINE_MIXED])
+def test_dense_lj_cells_sorted_either_way(oracle_mod, engine, rc, per_cell):
+    """An LJ fluid of 19,652 atoms with a long cutoff: at rc = 6 the cells hold
+    a few hundred atoms (the cell sort ranks them in chunks of whole cells in
+    LDS), at rc = 9 more than 512 (a cell larger than a chunk, sorted by one
+    lane). Forces against the oracle, and two engines built from the same
+    state give the same bits (the cells' order is deterministic)."""
+    from moltwister_amd import workloads as W
+    from systems import cfg
+    print(f"case dense lj cells engine={engine} rc={rc}", flush=True)
+    s = W.fcc_lj(17)
+    L = s.box[0]
+    nc = int(np.floor(L / (rc + 0.3)))
+    assert nc >= 3 and per_cell[0] < s.n_atoms / nc ** 3 <= per_cell[1], (nc, s.n_atoms / nc ** 3)
+    c = cfg(rc, 0.3, engine=engine, coulomb=abi.COUL_CUTOFF, skin=0.3)
+    co = cfg(rc, 0.3, coulomb=abi.COUL_CUTOFF)
+    pos = s.pos + np.random.default_rng(8).normal(0, 0.05, s.pos.shape)
+    s.pos = pos
+    o = oracle_mod.Oracle(s, co)
+    Fo, Uo = o.compute_forces()
+    del o
+    e1 = Engine(s, c)
+    Fe, Ue = e1.compute_forces()
+    e1.close()
+    e2 = Engine(s, c)
+    Fe2, Ue2 = e2.compute_forces()
+    e2.close()
+    assert np.array_equal(Fe, Fe2) and np.array_equal(Ue, Ue2)
+    if engine == abi.ENGINE_FP64:
+        scale = np.maximum(np.linalg.norm(Fo, axis=1), np.sqrt(np.mean(np.sum(Fo ** 2, 1))))
+        assert (np.linalg.norm(Fe - Fo, axis=1) / scale).max() < 1e-12
+    else:
+        assert rms_rel(Fe, Fo) < 1e-5, rms_rel(Fe, Fo)
+    assert abs(Ue.astype(np.float64).sum() - Uo.astype(np.float64).sum()) <= 1e-5 * np.abs(Uo).sum()

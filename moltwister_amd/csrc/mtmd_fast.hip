@@ -2326,13 +2326,16 @@ __global__ void k_ap_sum(MtDev d, int nsplit, int want_e)
 // atoms' list chunks, keeping at least ~4 chunks per lane. The choice depends
 // only on n, the list capacity and the CU count, so it is fixed for a run of
 // steps (the per-atom sums, and so the bits, depend on it).
-static int mt_lanes_per_atom(int n, int ncap, int ncu)
+static int mt_lanes_per_atom(int n, int ncap, int ncu, int mixed)
 {
     const long long fill = (long long)ncu * 2048;
     const int chunks = (ncap + 3) / 4;
     if ((long long)n >= fill || chunks < 16) return 1;
     if ((long long)n * 4 >= fill || chunks < 64) return 4;
-    return 16;
+    // MIXED: (the 30k-atom water box at a 10 A cutoff, 159 chunks an atom:
+    // its force pass measured 0.061 ms at 8 lanes, 0.064 at 16, 0.073 at 32;
+    // the FP64 pass was slower at 8)
+    return (mixed && chunks < 256) ? 8 : 16;
 }
 
 static int mt_cu_count()
@@ -2350,7 +2353,7 @@ static int mt_cu_count()
 }
 
 #ifndef MT_FORCE_ITERS
-#define MT_FORCE_ITERS 2
+#define MT_FORCE_ITERS 1
 #endif
 template <bool COUL, int G>
 static void mt_launch_mixed_lds(const MtDev& d, const MtParams& p, size_t lds_tab, hipStream_t s)
@@ -2439,7 +2442,7 @@ static void mt_launch_pairs(const MtDev& d, const MtParams& p, int mixed, int wa
     }
 }
 
-int mt_fast_lanes(const MtDev& d) { return mt_lanes_per_atom(d.n, d.ncap, mt_cu_count()); }
+int mt_fast_lanes(const MtDev& d, int mixed) { return mt_lanes_per_atom(d.n, d.ncap, mt_cu_count(), mixed); }
 
 int mt_fast_can_fuse(const MtDev& d, const MtParams& p, int mixed)
 {
@@ -2466,8 +2469,9 @@ void mt_fast_forces(const MtDev& d, const MtParams& p, int mixed, int want_e, in
     const bool ap = coul_all && p.coulomb_allpairs;
     const bool coul = coul_all && !ap;      // Coulomb inside the list kernels
     const size_t lds_tab = (size_t)d.T * d.T * MT_NPTS * sizeof(float2);
-    const int G = mt_fast_lanes(d);
+    const int G = mt_fast_lanes(d, mixed);
     if (G == 16) mt_launch_pairs<16>(d, p, mixed, want_e, coul, block, lds_tab, s);
+    else if (G == 8) mt_launch_pairs<8>(d, p, mixed, want_e, coul, block, lds_tab, s);
     else if (G == 4) mt_launch_pairs<4>(d, p, mixed, want_e, coul, block, lds_tab, s);
     else mt_launch_pairs<1>(d, p, mixed, want_e, coul, block, lds_tab, s);
     if (ap) {

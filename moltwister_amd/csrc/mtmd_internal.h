@@ -196,8 +196,9 @@ int mt_fast_can_fuse(const MtDev& d, const MtParams& p, int mixed);
 // of the next: new positions in pos2 and local coordinates in posf2, which
 // the host swaps with pos and posf; flips p.dslot
 void mt_fast_forces_fused(const MtDev& d, MtParams& p, hipStream_t s);
-// lanes per atom the pair-force kernels use for this system (1, 4 or 16)
-int mt_fast_lanes(const MtDev& d);
+// lanes per atom the pair-force kernels use for this system (1, 4, 8 or 16;
+// 8 for MIXED only)
+int mt_fast_lanes(const MtDev& d, int mixed);
 // the kernels that move atoms (front halves) set the rebuild flag: their
 // launchers flip p.dslot
 void mt_fast_kick_drift(const MtDev& d, MtParams& p, int scale_slot, int mixed, hipStream_t s);

@@ -663,6 +663,21 @@ __device__ __forceinline__ float mt_lean_rinv(float r2, float x0, float invd, in
     return rinv;
 }
 
+// the sum of v over a workgroup of NT threads (a tree in LDS, the same order
+// every run); thread 0 stores it
+template <int NT>
+__device__ inline void mt_block_sum_store(double v, double* out)
+{
+    __shared__ double sh[NT];
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = NT / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = sh[0];
+}
+
 // sum over the G consecutive lanes that share an atom (butterfly, fixed
 // order, so every run gives the same bits); every lane ends with the total
 template <int G>

@@ -290,17 +290,7 @@ void mt_exact_forces(const MtDev& d, const MtParams& p, hipStream_t s)
 // cutForces (:96-122), pbcWrap (SimulationBox.cpp:135-166)
 // ---------------------------------------------------------------------------
 
-__device__ inline void block_sum_store(double v, double* out)
-{
-    __shared__ double sh[TPB];
-    sh[threadIdx.x] = v;
-    __syncthreads();
-    for (int off = TPB / 2; off > 0; off >>= 1) {
-        if (threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) *out = sh[0];
-}
+__device__ inline void block_sum_store(double v, double* out) { mt_block_sum_store<TPB>(v, out); }
 
 // p *= S (thermostat), p += F dt/2, r += p dt/m; optional max |r - r_ref|^2
 __global__ void __launch_bounds__(TPB) k_kick_drift(double4* __restrict__ pos, double4* __restrict__ mom,

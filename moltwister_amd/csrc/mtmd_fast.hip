@@ -2125,20 +2125,19 @@ __device__ __forceinline__ void mt_ap_term(float xk, float yk, float zk, const m
     u += kr;
 }
 
-// bonded terms and special (same-molecule / scaled) pairs, added on top of the
-// pair forces with the exact FP64 helpers; one thread per atom with work
-__global__ void __launch_bounds__(TPB) k_fast_bonded_special(MtDev d, MtParams p, int want_e, int coul,
-                                                             int mixed)
+// bonded terms and special (same-molecule / scaled) pairs of atom k with the
+// exact FP64 helpers, added on top of its pair force into Fk (with its
+// energy, want_e); false, and Fk untouched, when the atom has none
+__device__ __forceinline__ bool mt_fast_bonded_atom(const MtDev& d, const MtParams& p, int k, int want_e, int coul,
+                                                    int mixed, double4& Fk)
 {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= d.n) return;
-    if (!d.mobile[k]) return;
+    if (!d.mobile[k]) return false;
     const int o = d.perm[k];
     const MtBonded& b = d.bl;
     const bool has_b = (b.has_bonds && b.b_num[o]) || (b.has_angles && b.a_num[o]) ||
                        (b.has_dihedrals && b.d_num[o]);
     const bool has_s = d.sp.n_entries > 0 && d.sp.num[o] > 0;
-    if (!has_b && !has_s) return;
+    if (!has_b && !has_s) return false;
     mt_d3 F = {0.0, 0.0, 0.0};
     float U = 0.0f;
     if (has_b) mt_exact_bonded(d, p, k, F, U, o, d.inv);
@@ -2190,9 +2189,45 @@ __global__ void __launch_bounds__(TPB) k_fast_bonded_special(MtDev d, MtParams p
             }
         }
     }
-    double4 f0 = d.frc[k];
-    d.frc[k] = make_double4(f0.x + F.x, f0.y + F.y, f0.z + F.z, 0.0);
+    const double4 f0 = d.frc[k];
+    Fk = make_double4(f0.x + F.x, f0.y + F.y, f0.z + F.z, 0.0);
     if (want_e) d.epot[k] += U;
+    return true;
+}
+
+// one thread an atom. KICK (NVT steps): then the back half's force cap and
+// half kick of every atom and the block partials of p.p/m, as k_cap_kick
+// computes them (the same operations on the same values, the same blocks), in
+// place of that kernel
+template <bool KICK>
+__global__ void __launch_bounds__(TPB) k_fast_bonded_special(MtDev d, MtParams p, int want_e, int coul,
+                                                             int mixed)
+{
+    MT_EXACT_FP
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    double ke = 0.0;
+    if (k < d.n) {
+        double4 F;
+        const bool had = mt_fast_bonded_atom(d, p, k, want_e, coul, mixed, F);
+        if constexpr (!KICK) {
+            if (had) d.frc[k] = F;
+        } else {
+            if (!had) F = d.frc[k];
+            const double Fcut = p.Fcut;
+            if (Fcut > 0) {
+                if (fabs(F.x) > Fcut) F.x = ((F.x >= 0.0) ? 1.0 : -1.0) * Fcut;
+                if (fabs(F.y) > Fcut) F.y = ((F.y >= 0.0) ? 1.0 : -1.0) * Fcut;
+                if (fabs(F.z) > Fcut) F.z = ((F.z >= 0.0) ? 1.0 : -1.0) * Fcut;
+            }
+            if (had || Fcut > 0) d.frc[k] = F;
+            double4 pm = d.mom[k];
+            const double dt_2 = p.dt / 2.0;
+            pm.x = pm.x + F.x * dt_2; pm.y = pm.y + F.y * dt_2; pm.z = pm.z + F.z * dt_2;
+            d.mom[k] = pm;
+            ke = ((pm.x * pm.x) + (pm.y * pm.y) + (pm.z * pm.z)) / pm.w;
+        }
+    }
+    if constexpr (KICK) mt_block_sum_store<TPB>(ke, &d.red[blockIdx.x]);
 }
 
 // ---------------------------------------------------------------------------
@@ -2463,7 +2498,7 @@ void mt_fast_forces_fused(const MtDev& d, MtParams& p, hipStream_t s)
     p.dslot ^= 1;
 }
 
-void mt_fast_forces(const MtDev& d, const MtParams& p, int mixed, int want_e, int block, hipStream_t s)
+int mt_fast_forces(const MtDev& d, const MtParams& p, int mixed, int want_e, int block, hipStream_t s, int kick)
 {
     const bool coul_all = p.coulomb_on != 0;
     const bool ap = coul_all && p.coulomb_allpairs;
@@ -2480,9 +2515,16 @@ void mt_fast_forces(const MtDev& d, const MtParams& p, int mixed, int want_e, in
         else hipLaunchKernelGGL(k_ap_coulomb_fp64, g, dim3(AP_TPB), 0, s, d, p, want_e);
         hipLaunchKernelGGL(k_ap_sum, dim3(nblk(d.n, TPB)), dim3(TPB), 0, s, d, d.ap_nsplit, want_e);
     }
-    if (d.bl.has_bonds || d.bl.has_angles || d.bl.has_dihedrals || d.sp.n_entries > 0)
-        hipLaunchKernelGGL(k_fast_bonded_special, dim3(nblk(d.n, TPB)), dim3(TPB), 0, s, d, p, want_e,
-                           coul_all ? 1 : 0, mixed);
+    if (d.bl.has_bonds || d.bl.has_angles || d.bl.has_dihedrals || d.sp.n_entries > 0) {
+        if (kick)
+            hipLaunchKernelGGL(k_fast_bonded_special<true>, dim3(nblk(d.n, TPB)), dim3(TPB), 0, s, d, p, want_e,
+                               coul_all ? 1 : 0, mixed);
+        else
+            hipLaunchKernelGGL(k_fast_bonded_special<false>, dim3(nblk(d.n, TPB)), dim3(TPB), 0, s, d, p, want_e,
+                               coul_all ? 1 : 0, mixed);
+        return kick ? 1 : 0;
+    }
+    return 0;
 }
 
 // ---------------------------------------------------------------------------

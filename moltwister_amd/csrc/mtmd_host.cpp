@@ -1295,9 +1295,13 @@ static int fast_check_capacity(mtmd_handle* h, bool* overflowed)
     return MTMD_OK;
 }
 
-static int fast_forces(mtmd_handle* h, bool want_energy)
+// kicked (NVT steps): set when the forces' bonded pass also did the back
+// half's cap and kick with the kinetic-energy partials (see mt_fast_forces)
+static int fast_forces(mtmd_handle* h, bool want_energy, bool* kicked = nullptr)
 {
-    mt_fast_forces(h->d, h->p, h->engine == MTMD_ENGINE_MIXED, want_energy ? 1 : 0, h->cfg.block_size, h->stream);
+    const int k = mt_fast_forces(h->d, h->p, h->engine == MTMD_ENGINE_MIXED, want_energy ? 1 : 0,
+                                 h->cfg.block_size, h->stream, kicked ? 1 : 0);
+    if (kicked) *kicked = k != 0;
     HIPCHK(h, hipGetLastError());
     return MTMD_OK;
 }
@@ -1576,8 +1580,8 @@ static int md_steps_once(mtmd_handle* h, int nsteps, bool energy_last)
             }
             mt_fast_kick_drift(d, h->p, h->nh_scale_slot, mixed, h->stream);
         };
-        auto nvt_back = [&]() {
-            mt_cap_kick(d, h->p, 1, h->stream);
+        auto nvt_back = [&](bool kicked) {
+            if (!kicked) mt_cap_kick(d, h->p, 1, h->stream);
             mt_nh_chain(d, h->M, h->nresp, h->Tred, h->p.dt, h->g, h->w, nblocks, h->stream, h->fast ? 1 : 0);
             mt_scale_wrap(d, h->p, h->nh_scale_slot, 1, h->stream);
         };
@@ -1585,8 +1589,8 @@ static int md_steps_once(mtmd_handle* h, int nsteps, bool energy_last)
         // front half of t + 1, both chain propagations in one launch and the
         // scale, wrap, kick and drift in one pass (a call's first front half
         // and last back half run as above)
-        auto nvt_mid = [&]() {
-            mt_cap_kick(d, h->p, 1, h->stream);
+        auto nvt_mid = [&](bool kicked) {
+            if (!kicked) mt_cap_kick(d, h->p, 1, h->stream);
             mt_nh_chain2(d, h->M, h->nresp, h->Tred, h->p.dt, h->g, h->w, nblocks, h->stream);
             mt_fast_scale_kick_drift(d, h->p, h->nh_scale_slot, h->nh_scale_slot + 1, mixed, h->stream);
         };
@@ -1608,8 +1612,9 @@ static int md_steps_once(mtmd_handle* h, int nsteps, bool energy_last)
                 fused_forces();
                 return MTMD_OK;
             }
-            if ((r = fast_forces(h, false))) return r;
-            if (nvt) nvt_mid();
+            bool kicked = false;
+            if ((r = fast_forces(h, false, nvt ? &kicked : nullptr))) return r;
+            if (nvt) nvt_mid(kicked);
             else mt_fast_nve_fused(d, h->p, mixed, h->stream);
             return MTMD_OK;
         };
@@ -1680,10 +1685,11 @@ static int md_steps_once(mtmd_handle* h, int nsteps, bool energy_last)
                 h->steps++;
                 continue;
             }
-            if ((rc = fast_forces(h, last && energy_last))) return rc;
+            bool kicked = false;
+            if ((rc = fast_forces(h, last && energy_last, nvt ? &kicked : nullptr))) return rc;
             if (nvt) {
-                if (last) nvt_back();
-                else nvt_mid();
+                if (last) nvt_back(kicked);
+                else nvt_mid(kicked);
             } else if (last) {
                 mt_cap_kick(d, h->p, 0, h->stream);
                 mt_scale_wrap(d, h->p, -1, 0, h->stream);

@@ -188,7 +188,12 @@ int mt_fast_tile_mode(const MtDev& d, const double* L, int mixed);
 // the tiled force-only pass in FP64 arithmetic (measurement); -1 when the
 // tile mode is off or its LDS does not fit
 int mt_fast_force_tile64(const MtDev& d, const MtParams& p, hipStream_t s);
-void mt_fast_forces(const MtDev& d, const MtParams& p, int mixed, int want_energy, int block, hipStream_t s);
+// pair, all-pairs Coulomb and bonded forces. kick (NVT): the pass that adds
+// the bonded terms also applies the force cap and half kick and leaves the
+// block partials of p.p/m (mt_cap_kick's work); returns 1 when it did, 0
+// when the system has no bonded or special terms (then call mt_cap_kick)
+int mt_fast_forces(const MtDev& d, const MtParams& p, int mixed, int want_energy, int block, hipStream_t s,
+                   int kick = 0);
 // 1 when the NVE step may run as one fused pass (mt_fast_forces_fused): the
 // MIXED tile mode with every force of the step inside that pass
 int mt_fast_can_fuse(const MtDev& d, const MtParams& p, int mixed);

@@ -723,8 +723,12 @@ __device__ __forceinline__ void mt_neigh_block_atoms(const MtDev& d, const MtPar
             while (ce < 64 && s_off[ce + 1] - s_off[cb] <= SCAP) ce++;
             const int base = s_off[cb];
             if (s_off[ce] - base > SCAP) {
-                // a single cell larger than the staging buffer
+                // a single cell larger than the staging buffer: this block's
+                // lists are not built, and its atoms' counts are zeroed, so no
+                // pass reads rows of an earlier build before the host sees
+                // the flag (the condition is uniform over the workgroup)
                 if (t == 0) d.flags[5] = 1u;
+                for (int i = l; i < cnt_c; i += 32) d.ncount[d.cell_list[f0 + i]] = 0;
                 return;
             }
             if (cb != staged_b || ce != staged_e) {
@@ -1186,7 +1190,7 @@ __global__ void __launch_bounds__(256) k_fast_neigh_dense(MtDev d, MtParams p, i
 
 void mt_fast_neighbors(const MtDev& d, const MtParams& p, int cond, hipStream_t s)
 {
-    if (d.n > 40 * d.ncells) {
+    if (d.force_dense || d.n > 40 * d.ncells) {
         // dense cells: workgroups of NBD_ATOMS home atoms, about 1.5x the mean
         // cell's batches per cell (a fuller cell loops)
         const int mean = (d.n + d.ncells - 1) / d.ncells;
@@ -2426,7 +2430,7 @@ static void mt_launch_fp64(const MtDev& d, const MtParams& p, int bs, hipStream_
 int mt_fast_tile_mode(const MtDev& d, const double* L, int mixed)
 {
     const size_t lds_tab = (size_t)d.T * d.T * MT_NPTS * sizeof(float2);
-    if (!mixed || !d.wsorted || lds_tab > 32768 || d.n > 40 * d.ncells) return 0;
+    if (!mixed || !d.wsorted || d.force_dense || lds_tab > 32768 || d.n > 40 * d.ncells) return 0;
     // the 16-bit list build screens staged coordinates (up to 2.5 cell widths)
     // in FP16: keep them far inside its range
     for (int i = 0; i < 3; i++)

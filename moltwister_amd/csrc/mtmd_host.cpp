@@ -1203,6 +1203,7 @@ extern "C" int mtmd_get_thermostat(mtmd_handle* h, double* eta, double* p_eta)
 static int grow_neighbors(mtmd_handle* h, int need)
 {
     MtDev& d = h->d;
+    need = std::max(need, d.ncap);                    // (grows, never shrinks)
     const int cap = (h->list_slack ? need + need / 8 + 16 + 3 : need + 3) & ~3;
     void* old = d.nlist;
     uint32_t* nl = nullptr;
@@ -1284,9 +1285,19 @@ static int fast_check_capacity(mtmd_handle* h, bool* overflowed)
     HIPCHK(h, hipStreamSynchronize(h->stream));
     *overflowed = h->h_status[2] != 0;
     h->rebuilds = (int64_t)(unsigned)h->h_status[4];
-    if (h->h_status[5])
-        return h->fail(MTMD_EBOX, "a cell holds more atoms than the neighbour-list build stages at once (%d)",
-                       d.tile16 ? MT_NB_STAGE_ATOMS_TILE : MT_NB_STAGE_ATOMS);
+    if (h->h_status[5]) {
+        if (d.force_dense)
+            return h->fail(MTMD_EBOX, "a cell holds more atoms than the neighbour-list build stages at once (%d)",
+                           d.tile16 ? MT_NB_STAGE_ATOMS_TILE : MT_NB_STAGE_ATOMS);
+        // a cell too large for the block build: the lists come from the
+        // dense-cell build from now on (32-bit words, tile mode off); the
+        // caller builds them again, as after an overflow
+        d.force_dense = 1;
+        d.tile16 = 0;
+        h->drop_graphs();
+        HIPCHK(h, hipMemsetAsync(d.flags + 5, 0, sizeof(unsigned int), h->stream));
+        *overflowed = true;
+    }
     if (h->h_status[3]) {
         HIPCHK(h, hipMemsetAsync(d.flags + 3, 0, sizeof(unsigned int), h->stream));
         h->have_forces = false;
@@ -1312,16 +1323,17 @@ static int fast_full_forces(mtmd_handle* h)
     MtDev& d = h->d;
     int rc = fast_rebuild(h, true, true);
     if (rc) return rc;
-    bool ov = false;
-    if ((rc = fast_check_capacity(h, &ov))) return rc;
-    if (ov) {
-        // count pass told us the largest row (nreq); regrow and rebuild
+    for (int attempt = 0;; attempt++) {
+        bool ov = false;
+        if ((rc = fast_check_capacity(h, &ov))) return rc;
+        if (!ov) break;
+        if (attempt == 2) return h->fail(MTMD_ENEIGH, "neighbour list overflow after regrowth");
+        // the build told us the largest row (nreq), or moved to the
+        // dense-cell build: regrow and build again
         HIPCHK(h, hipMemcpy(h->h_status, d.nreq, sizeof(int), hipMemcpyDeviceToHost));
         if ((rc = grow_neighbors(h, h->h_status[0]))) return rc;
         HIPCHK(h, hipMemsetAsync(d.flags + 2, 0, sizeof(unsigned int), h->stream));
         if ((rc = fast_rebuild(h, true, false))) return rc;
-        if ((rc = fast_check_capacity(h, &ov))) return rc;
-        if (ov) return h->fail(MTMD_ENEIGH, "neighbour list overflow after regrowth");
     }
     return fast_forces(h, true);
 }

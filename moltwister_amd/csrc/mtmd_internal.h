@@ -91,6 +91,11 @@ struct MtDev {
     // words, the neighbour's place in those staged cells (8 words a chunk,
     // 2 ncap words a row); the other blocks keep the 32-bit coded words
     int tile16;
+    // set once a block list build met a cell larger than its staging (one cell
+    // over 2048 atoms in an otherwise sparse system): from then on the lists
+    // come from the dense-cell build, which takes cells of any size, and the
+    // tile mode is off
+    int force_dense;
     // sorting (FP64/MIXED): perm[s] = original index of sorted slot s, inv[o] = slot
     int* perm; int* inv;
     int* perm2;

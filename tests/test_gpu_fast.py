@@ -355,3 +355,46 @@ def test_dense_lj_cells_sorted_either_way(oracle_mod, engine, rc, per_cell):
     else:
         assert rms_rel(Fe, Fo) < 1e-5, rms_rel(Fe, Fo)
     assert abs(Ue.astype(np.float64).sum() - Uo.astype(np.float64).sum()) <= 1e-5 * np.abs(Uo).sum()
+
+
+@pytest.mark.parametrize("engine", [abi.ENGINE_FP64, abi.ENGINE_MIXED])
+def test_crowded_cell_falls_back_to_dense_build(oracle_mod, engine):
+    """A liquid-density FCC cube of 2916 atoms inside one cell of a 75 sigma
+    box (rc 14.5, 5^3 cells): the mean cell is sparse, so the block list build
+    runs first and meets a cell larger than its staging; the engine moves to
+    the dense-cell build (any cell size) and goes on. Forces against the
+    oracle, whose cells are sized by their counts as the reference's are
+    (FunctorGenCellList.cpp:95-137), and steps that keep the energy."""
+    from moltwister_amd.system import System
+    from systems import cfg
+    print(f"case crowded cell engine={engine}", flush=True)
+    a = (4.0 / 1.0) ** (1.0 / 3.0)
+    basis = np.array([[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5], [0, 0.5, 0.5]])
+    g = np.stack(np.meshgrid(*[np.arange(9)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    pos = ((g[:, None, :] + basis[None]) * a).reshape(-1, 3)
+    L = 75.0
+    pos = pos - L / 2.0 + 0.4 + np.random.default_rng(9).normal(0, 0.02, pos.shape)
+    n = pos.shape[0]
+    s = System(pos=pos, mom=np.zeros_like(pos), mass=np.ones(n), charge=np.zeros(n), types=np.zeros(n, np.int32),
+               box=(L, L, L), nonbonded=[(0, 0, abi.NB_LJ, (1.0, 1.0, 1.0))])
+    rc = 14.5
+    assert int(np.floor(L / (rc + 0.3))) == 5 and n <= 40 * 125
+    c = cfg(rc, 0.3, engine=engine, coulomb=abi.COUL_CUTOFF, skin=0.3, T=120.272)
+    co = cfg(rc, 0.3, coulomb=abi.COUL_CUTOFF, T=120.272)
+    o = oracle_mod.Oracle(s, co)
+    Fo, Uo = o.compute_forces()
+    del o
+    e = Engine(s, c)
+    Fe, Ue = e.compute_forces()
+    if engine == abi.ENGINE_FP64:
+        scale = np.maximum(np.linalg.norm(Fo, axis=1), np.sqrt(np.mean(np.sum(Fo ** 2, 1))))
+        assert (np.linalg.norm(Fe - Fo, axis=1) / scale).max() < 1e-12
+    else:
+        assert rms_rel(Fe, Fo) < 1e-5, rms_rel(Fe, Fo)
+    assert abs(Ue.astype(np.float64).sum() - Uo.astype(np.float64).sum()) <= 1e-5 * np.abs(Uo).sum()
+    th0 = e.thermo()
+    e.step(50)
+    th1 = e.thermo()
+    assert np.isfinite(th1["Etot"])
+    assert abs(th1["Etot"] - th0["Etot"]) <= 1e-3 * abs(th0["Etot"]), (th0["Etot"], th1["Etot"])
+    e.close()
